@@ -1,0 +1,243 @@
+"""ctypes binding of ``libbse_mi355x.so`` (C ABI in include/bse_mi355x.h).
+
+There is no CPU fallback: if the library is missing or no GPU is visible,
+construction raises.  Arrays cross the boundary as host numpy buffers (or raw
+device pointers where an entry point takes ``*_on_device``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libbse_mi355x.so"
+
+PREC_FP64, PREC_FP32, PREC_BF16X2 = 0, 1, 2
+ORDER_REFERENCE, ORDER_KMAJOR = 0, 1
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x2": PREC_BF16X2}
+
+# Symbols every build must export (mirrors include/bse_mi355x.h).
+ABI_SYMBOLS = (
+    "bse_create", "bse_destroy", "bse_last_error", "bse_abi_version", "bse_set_problem",
+    "bse_problem_sizes", "bse_tb_states", "bse_build", "bse_get_matrix", "bse_get_dipoles",
+    "bse_eigh", "bse_get_eigenvalues", "bse_get_eigenvectors", "bse_oscillator",
+    "bse_spectrum", "bse_eigh_dense", "bse_stage_times",
+)
+
+
+class BSEError(RuntimeError):
+    pass
+
+
+class TBModelC(C.Structure):
+    _fields_ = [("norb", C.c_int32), ("nr", C.c_int32), ("r_cart", C.POINTER(C.c_double)),
+                ("h_r", C.POINTER(C.c_double)), ("tau", C.POINTER(C.c_double))]
+
+
+class SystemC(C.Structure):
+    _fields_ = [("bravais", C.c_double * 9), ("n1", C.c_int32), ("n2", C.c_int32),
+                ("shift", C.c_double * 2), ("n_occ", C.c_int32), ("nv", C.c_int32),
+                ("nc", C.c_int32), ("scissor", C.c_double), ("q", C.c_double * 3),
+                ("r0", C.c_double), ("eps_env", C.c_double), ("g_cut", C.c_double),
+                ("f_s", C.c_double), ("exchange_scale", C.c_double),
+                ("direct_scale", C.c_double), ("dipole_dir", C.c_double * 3)]
+
+
+_lib = None
+
+
+def load_library(path: pathlib.Path | str | None = None):
+    """Load the shared library once; raises if it is absent (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = pathlib.Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise BSEError(f"{p} is missing: build it with build.py (hipcc, gfx950)")
+    lib = C.CDLL(str(p))
+    vp, i32, i64, dp = C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_double)
+    sig = {
+        "bse_create": (i32, [C.POINTER(vp), i32]),
+        "bse_destroy": (i32, [vp]),
+        "bse_last_error": (C.c_char_p, [vp]),
+        "bse_abi_version": (i32, []),
+        "bse_set_problem": (i32, [vp, C.POINTER(TBModelC), C.POINTER(SystemC)]),
+        "bse_problem_sizes": (i32, [vp, C.POINTER(C.c_int64)]),
+        "bse_tb_states": (i32, [vp, dp, dp]),
+        "bse_build": (i32, [vp, i32]),
+        "bse_get_matrix": (i32, [vp, vp, i64, i32, i32]),
+        "bse_get_dipoles": (i32, [vp, dp]),
+        "bse_eigh": (i32, [vp, i32, i32]),
+        "bse_get_eigenvalues": (i32, [vp, dp]),
+        "bse_get_eigenvectors": (i32, [vp, dp, i64, i32, i32]),
+        "bse_oscillator": (i32, [vp, dp]),
+        "bse_spectrum": (i32, [vp, dp, i32, C.c_double, dp, dp]),
+        "bse_eigh_dense": (i32, [vp, i32, dp, i64, i32, i32, i32, dp, dp, i64]),
+        "bse_stage_times": (i32, [vp, dp, i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    for name in ("bse_grad_energies", "bse_grad_spectrum", "bse_batch_sweep"):
+        if hasattr(lib, name):
+            getattr(lib, name).restype = i32
+    _lib = lib
+    return lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+STAGES = ("tb", "tables", "assembly", "tridiagonalisation", "tridiagonal_values",
+          "tridiagonal_vectors", "back_transform", "refinement", "spectrum")
+
+
+class Context:
+    """One device context (stream + device buffers) for one host thread."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.bse_create(C.byref(h), int(device))
+        if rc != 0:
+            raise BSEError(f"bse_create failed ({rc}): no usable GPU")
+        self.h = h
+        self.system = None
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bse_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.bse_last_error(self.h)
+            raise BSEError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # ------------------------------------------------------------------
+    def set_system(self, s):
+        """Upload a host-side system description (BSESystem)."""
+        m = s.model
+        r_cart = np.ascontiguousarray(m.r_cart, dtype=np.float64)
+        h_r = np.ascontiguousarray(m.h_r, dtype=np.complex128)
+        tau = np.ascontiguousarray(m.tau, dtype=np.float64)
+        mc = TBModelC(m.norb, m.nr, _dptr(r_cart), h_r.ctypes.data_as(C.POINTER(C.c_double)),
+                      _dptr(tau))
+        sc = SystemC()
+        sc.bravais[:] = list(np.asarray(m.lattice.bravais, dtype=np.float64).T.ravel())
+        sc.n1, sc.n2 = s.grid.n1, s.grid.n2
+        sc.shift[:] = list(s.grid.shift)
+        sc.n_occ, sc.nv, sc.nc = s.n_occ, s.nv, s.nc
+        sc.scissor = s.scissor
+        sc.q[:] = list(s.q_opt)
+        sc.r0, sc.eps_env, sc.g_cut = s.r0, s.eps_env, s.g_cut
+        sc.f_s = s.spin_factor
+        sc.exchange_scale, sc.direct_scale = s.exchange_scale, s.direct_scale
+        sc.dipole_dir[:] = [1.0, 0.0, 0.0]
+        self._keep = [r_cart, h_r, tau]
+        self._check(self.lib.bse_set_problem(self.h, C.byref(mc), C.byref(sc)), "bse_set_problem")
+        self.system = s
+        return self
+
+    def sizes(self):
+        out = (C.c_int64 * 4)()
+        self._check(self.lib.bse_problem_sizes(self.h, out), "bse_problem_sizes")
+        return dict(nk=out[0], n=out[1], norb=out[2], ng=out[3])
+
+    def tb_states(self):
+        s = self.system
+        ns = s.nv + s.nc
+        eps = np.zeros((s.nk, ns))
+        coef = np.zeros((s.nk, ns, s.norb), dtype=np.complex128)
+        self._check(self.lib.bse_tb_states(self.h, _dptr(eps), coef.ctypes.data_as(C.POINTER(C.c_double))),
+                    "bse_tb_states")
+        return eps, coef
+
+    def build(self, precision="fp64"):
+        p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self._check(self.lib.bse_build(self.h, p), "bse_build")
+        self.precision = p
+        return self
+
+    def matrix(self, order=ORDER_REFERENCE):
+        n = self.sizes()["n"]
+        dt = np.complex128 if self.precision == PREC_FP64 else np.complex64
+        out = np.zeros((n, n), dtype=dt, order="F")
+        self._check(self.lib.bse_get_matrix(self.h, out.ctypes.data, n, 0, order), "bse_get_matrix")
+        return out
+
+    def dipoles(self):
+        n = self.sizes()["n"]
+        out = np.zeros(n, dtype=np.complex128)
+        self._check(self.lib.bse_get_dipoles(self.h, out.ctypes.data_as(C.POINTER(C.c_double))),
+                    "bse_get_dipoles")
+        return out
+
+    def eigh(self, n_vectors=-1, refine=False):
+        self._check(self.lib.bse_eigh(self.h, int(n_vectors), int(bool(refine))), "bse_eigh")
+        self.n_vectors = n_vectors
+        return self
+
+    def eigenvalues(self):
+        n = self.sizes()["n"] if self.system is not None else self._n_dense
+        out = np.zeros(n)
+        self._check(self.lib.bse_get_eigenvalues(self.h, _dptr(out)), "bse_get_eigenvalues")
+        return out
+
+    def eigenvectors(self, nvec):
+        n = self.sizes()["n"]
+        out = np.zeros((n, nvec), dtype=np.complex128, order="F")
+        self._check(self.lib.bse_get_eigenvectors(self.h, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  n, int(nvec), 0), "bse_get_eigenvectors")
+        return out
+
+    def oscillator(self):
+        n = self.sizes()["n"]
+        out = np.zeros(n, dtype=np.complex128)
+        self._check(self.lib.bse_oscillator(self.h, out.ctypes.data_as(C.POINTER(C.c_double))),
+                    "bse_oscillator")
+        return out
+
+    def spectrum(self, omegas, eta):
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        re = np.zeros_like(om)
+        im = np.zeros_like(om)
+        self._check(self.lib.bse_spectrum(self.h, _dptr(om), om.size, float(eta), _dptr(re), _dptr(im)),
+                    "bse_spectrum")
+        return re + 1j * im
+
+    def eigh_dense(self, a, precision="fp64", n_vectors=-1):
+        a = np.asfortranarray(a, dtype=np.complex128)
+        n = a.shape[0]
+        p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        ev = np.zeros(n)
+        nv = n if n_vectors < 0 else int(n_vectors)
+        z = np.zeros((n, max(nv, 1)), dtype=np.complex128, order="F")
+        self._check(self.lib.bse_eigh_dense(self.h, n, a.ctypes.data_as(C.POINTER(C.c_double)), n, 0, p,
+                                            int(n_vectors), _dptr(ev),
+                                            z.ctypes.data_as(C.POINTER(C.c_double)), n),
+                    "bse_eigh_dense")
+        self.system = None
+        self._n_dense = n
+        return ev, z[:, :nv]
+
+    def stage_times(self):
+        out = np.zeros(len(STAGES))
+        self._check(self.lib.bse_stage_times(self.h, _dptr(out), len(STAGES)), "bse_stage_times")
+        return dict(zip(STAGES, out.tolist()))

@@ -1,0 +1,257 @@
+// Dense complex FP64 building blocks for K7 (back-transform), K8 (refinement)
+// and K9 (oscillator strengths): a tiled GEMM with optional split-K, and the
+// compact-WY triangular factor of a panel of Householder reflectors
+// (Schreiber & Van Loan; forward, columnwise), which turns the application of
+// nb reflectors into three GEMMs: Z <- Z - V (T (V^H Z)).
+#include "bse_common.hpp"
+#include <algorithm>
+
+// op codes
+#define OP_N 0
+#define OP_C 1
+
+// C[M x N] (+)= alpha * op(A) * op(B); column-major.  Tile 64 x 64, K step 16,
+// 256 threads, 4 x 4 outputs per thread strided by 16 (conflict-free LDS).
+// gridDim.z > 1: split-K, partial results to P + z * M * N (ld M), no beta.
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, double2 alpha,
+                                              const double2* __restrict__ A, int64_t lda,
+                                              const double2* __restrict__ B, int64_t ldb,
+                                              double2 beta, double2* __restrict__ C, int64_t ldc,
+                                              double2* __restrict__ P, int kchunk)
+{
+    __shared__ double2 sA[16][64 + 1];
+    __shared__ double2 sB[16][64 + 1];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(K, kbeg + kchunk);
+    double2 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = cmk(0, 0);
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
+        // load op(A) tile: 64 (m) x 16 (k)
+        for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+            int mm, kk;
+            double2 val = cmk(0, 0);
+            if (OPA == OP_N) {
+                mm = e & 63; kk = e >> 6;
+                const int gm = m0 + mm, gk = k0 + kk;
+                if (gm < M && gk < kend) val = A[gm + (int64_t)gk * lda];
+            } else {
+                kk = e & 15; mm = e >> 4;
+                const int gm = m0 + mm, gk = k0 + kk;
+                if (gm < M && gk < kend) val = cconj(A[gk + (int64_t)gm * lda]);
+            }
+            sA[kk][mm] = val;
+        }
+        // load op(B) tile: 16 (k) x 64 (n)
+        for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+            int kk, nn;
+            double2 val = cmk(0, 0);
+            if (OPB == OP_N) {
+                kk = e & 15; nn = e >> 4;
+                const int gk = k0 + kk, gn = n0 + nn;
+                if (gk < kend && gn < N) val = B[gk + (int64_t)gn * ldb];
+            } else {
+                nn = e & 63; kk = e >> 6;
+                const int gk = k0 + kk, gn = n0 + nn;
+                if (gk < kend && gn < N) val = cconj(B[gn + (int64_t)gk * ldb]);
+            }
+            sB[kk][nn] = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            double2 av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = sA[kk][tx + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = sB[kk][ty + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) cfma(acc[a][b], av[a], bv[b]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int gm = m0 + tx + 16 * a, gn = n0 + ty + 16 * b;
+            if (gm >= M || gn >= N) continue;
+            if (gridDim.z > 1) {
+                P[(int64_t)blockIdx.z * M * N + gm + (int64_t)gn * M] = acc[a][b];
+            } else {
+                double2 r = cmul(alpha, acc[a][b]);
+                if (beta.x != 0.0 || beta.y != 0.0) r = cadd(r, cmul(beta, C[gm + (int64_t)gn * ldc]));
+                C[gm + (int64_t)gn * ldc] = r;
+            }
+        }
+}
+
+__global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int nsplit, double2 alpha,
+                                                      const double2* __restrict__ P, double2 beta,
+                                                      double2* __restrict__ C, int64_t ldc)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)M * N) return;
+    const int gm = (int)(idx % M), gn = (int)(idx / M);
+    double2 s = cmk(0, 0);
+    for (int z = 0; z < nsplit; ++z) s = cadd(s, P[(int64_t)z * M * N + idx]);
+    double2 r = cmul(alpha, s);
+    if (beta.x != 0.0 || beta.y != 0.0) r = cadd(r, cmul(beta, C[gm + (int64_t)gn * ldc]));
+    C[gm + (int64_t)gn * ldc] = r;
+}
+
+// Host wrapper.  ws: split-K workspace (>= nsplit*M*N double2); ws_elems its size.
+extern "C" hipError_t bse_zgemm(hipStream_t st, int opa, int opb, int M, int N, int K,
+                                double ar, double ai, const double2* A, int64_t lda,
+                                const double2* B, int64_t ldb, double br, double bi, double2* C,
+                                int64_t ldc, double2* ws, int64_t ws_elems)
+{
+    if (M <= 0 || N <= 0) return hipSuccess;
+    const double2 alpha = cmk(ar, ai), beta = cmk(br, bi);
+    const int64_t tiles = ceil_div64(M, 64) * ceil_div64(N, 64);
+    int nsplit = 1;
+    if (K > 0 && tiles < 256 && ws != nullptr) {
+        nsplit = (int)std::min<int64_t>(ceil_div64(K, 256), ceil_div64(512, tiles));
+        while (nsplit > 1 && (int64_t)nsplit * M * N > ws_elems) --nsplit;
+    }
+    int kchunk = (int)(ceil_div64(ceil_div64(K > 0 ? K : 1, nsplit), 16) * 16);
+    nsplit = (int)ceil_div64(K > 0 ? K : 1, kchunk);
+    dim3 grid((unsigned)ceil_div64(M, 64), (unsigned)ceil_div64(N, 64), (unsigned)nsplit);
+#define ZG(OA, OB)                                                                         \
+    hipLaunchKernelGGL((k_zgemm<OA, OB>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B, \
+                       ldb, beta, C, ldc, ws, kchunk)
+    if (opa == OP_N && opb == OP_N) ZG(OP_N, OP_N);
+    else if (opa == OP_C && opb == OP_N) ZG(OP_C, OP_N);
+    else if (opa == OP_N && opb == OP_C) ZG(OP_N, OP_C);
+    else ZG(OP_C, OP_C);
+#undef ZG
+    if (nsplit > 1) {
+        const int64_t tot = (int64_t)M * N;
+        hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)ceil_div64(tot, 256)), dim3(256), 0, st, M,
+                           N, nsplit, alpha, (const double2*)ws, beta, C, ldc);
+    }
+    return hipGetLastError();
+}
+
+// Compact-WY factor: T (nb x nb, upper, ld ldt) from tau (nb) and the Gram
+// matrix G = V^H V (nb x nb, ld nb):  T(i,i) = tau_i,
+// T(0:i, i) = -tau_i * T(0:i, 0:i) G(0:i, i).
+__global__ void k_larft(int nb, const double2* __restrict__ tau, const double2* __restrict__ G,
+                        double2* __restrict__ T, int ldt)
+{
+    __shared__ double2 col[64];
+    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) T[(e % nb) + (e / nb) * ldt] = cmk(0, 0);
+    __syncthreads();
+    for (int i = 0; i < nb; ++i) {
+        const double2 ti = tau[i];
+        const int r = threadIdx.x;
+        if (r < i) {
+            double2 s = cmk(0, 0);
+            for (int l = r; l < i; ++l) cfma(s, T[r + l * ldt], G[l + i * nb]);
+            col[r] = cscale(cmul(ti, s), -1.0);
+        }
+        __syncthreads();
+        if (r < i) T[r + i * ldt] = col[r];
+        if (r == 0) T[i + i * ldt] = ti;
+        __syncthreads();
+    }
+}
+
+extern "C" hipError_t bse_launch_larft(hipStream_t st, int nb, const double2* tau, const double2* G,
+                                       double2* T, int ldt)
+{
+    hipLaunchKernelGGL(k_larft, dim3(1), dim3(64), 0, st, nb, tau, G, T, ldt);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// K9: TDA dielectric function, eps(w) = 1 - pref sum_m |f_m|^2/(w - E_m + i eta)
+// (bse.version13.cpp:3621-3634).  One thread per frequency, oscillator table
+// streamed through LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dielectric(int nw, const double* __restrict__ omega,
+                                                   double eta, int N, const double* __restrict__ E,
+                                                   const double2* __restrict__ f, double pref,
+                                                   double* __restrict__ eps_re,
+                                                   double* __restrict__ eps_im)
+{
+    __shared__ double sE[256], sF[256];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const double w = (s < nw) ? omega[s] : 0.0;
+    double re = 0.0, im = 0.0;
+    for (int m0 = 0; m0 < N; m0 += 256) {
+        const int m = m0 + threadIdx.x;
+        sE[threadIdx.x] = (m < N) ? E[m] : 0.0;
+        sF[threadIdx.x] = (m < N) ? cabs2(f[m]) : 0.0;
+        __syncthreads();
+        const int cnt = min(256, N - m0);
+        for (int j = 0; j < cnt; ++j) {
+            const double dw = w - sE[j];
+            const double den = dw * dw + eta * eta;
+            const double g = sF[j] / den;
+            re += g * dw;      // Re 1/(dw + i eta) = dw/den
+            im -= g * eta;     // Im 1/(dw + i eta) = -eta/den
+        }
+        __syncthreads();
+    }
+    if (s < nw) {
+        eps_re[s] = 1.0 - pref * re;
+        eps_im[s] = -pref * im;
+    }
+}
+
+extern "C" hipError_t bse_launch_dielectric(hipStream_t st, int nw, const double* omega, double eta,
+                                            int N, const double* E, const double2* f, double pref,
+                                            double* eps_re, double* eps_im)
+{
+    hipLaunchKernelGGL(k_dielectric, dim3((unsigned)ceil_div64(nw, 256)), dim3(256), 0, st, nw,
+                       omega, eta, N, E, f, pref, eps_re, eps_im);
+    return hipGetLastError();
+}
+
+// Permute rows of a complex N x ncol matrix from internal k-major order to the
+// reference order: dst[ref(r'), j] = src[r', j], ref(r') = (r' % B) * nk + r' / B.
+__global__ __launch_bounds__(256) void k_rows_to_reference(int64_t N, int B, int nk, int ncol,
+                                                          const double2* __restrict__ src,
+                                                          int64_t lds, double2* __restrict__ dst,
+                                                          int64_t ldd)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (r >= N || j >= ncol) return;
+    const int64_t rr = (r % B) * nk + r / B;
+    dst[rr + (int64_t)j * ldd] = src[r + (int64_t)j * lds];
+}
+
+extern "C" hipError_t bse_launch_rows_to_reference(hipStream_t st, int64_t N, int B, int nk,
+                                                   int ncol, const double2* src, int64_t lds,
+                                                   double2* dst, int64_t ldd)
+{
+    if (ncol <= 0) return hipSuccess;
+    dim3 grid((unsigned)ceil_div64(N, 256), (unsigned)ncol);
+    hipLaunchKernelGGL(k_rows_to_reference, grid, dim3(256), 0, st, N, B, nk, ncol, src, lds, dst,
+                       ldd);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_convert_to_fp32(int64_t n, const double2* __restrict__ src,
+                                                        float2* __restrict__ dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = make_float2((float)src[i].x, (float)src[i].y);
+}
+
+extern "C" hipError_t bse_launch_convert_to_fp32(hipStream_t st, int64_t n, const double2* src,
+                                                 float2* dst)
+{
+    hipLaunchKernelGGL(k_convert_to_fp32, dim3((unsigned)ceil_div64(n, 256)), dim3(256), 0, st, n,
+                       src, dst);
+    return hipGetLastError();
+}

@@ -1,0 +1,742 @@
+// C ABI of the MI355X excitonic hot path (include/bse_mi355x.h).
+//
+// Host-side orchestration of the kernels; every heavy step runs on the
+// context's HIP stream.  The only host work is O(N) bookkeeping of the
+// tridiagonal stage (split points, sort of eigenvalues, cluster and batch
+// boundaries), which needs the tridiagonal (2N doubles) and the eigenvalues
+// (N doubles) on the host.
+#include "bse_common.hpp"
+#include "../../include/bse_mi355x.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <string>
+#include <vector>
+
+// ---- kernel launchers implemented in the other translation units ----------
+extern "C" {
+hipError_t bse_launch_tb_states(hipStream_t, int, int, int, const double*, const double2*,
+                                const double*, const double*, int, int, int, double,
+                                const double2*, double*, double2*, double*, double2*);
+hipError_t bse_launch_w_table(hipStream_t, int, int, int, int, const double*, const double*,
+                              const double*, double, double, double, double, int, double2*);
+hipError_t bse_launch_v_exchange(hipStream_t, int, int, const double*, const double*, double,
+                                 double2*);
+hipError_t bse_launch_transitions(hipStream_t, int, int, int, int, const double*, const double2*,
+                                  const double*, const double2*, const double*, const double*,
+                                  const double2*, double2*, double2*, double2*, double*);
+size_t bse_assemble_lds_bytes(int, int, int, int);
+int bse_assemble_pick_tk(int, int, int);
+hipError_t bse_launch_assemble(hipStream_t, int, int, int, int, int, int, int, int, int64_t,
+                               const double2*, const double2*, const double2*, const double2*,
+                               const double2*, const double2*, const double2*, const double*,
+                               double, double, void*, int);
+hipError_t bse_launch_expand(hipStream_t, int, int64_t, int, int, const void*, int64_t, void*,
+                             int64_t, int);
+void bse_hetrd_workspace(int, int, int64_t*, int64_t*, int64_t*, int64_t*);
+hipError_t bse_hetrd(hipStream_t, int, int, void*, int64_t, int, double*, double*, double2*,
+                     double2*, const int64_t*, double2*, double2*, double2*, double2*, double2*,
+                     double*, double*, unsigned int*);
+hipError_t bse_launch_bisect(hipStream_t, int, const int4*, const double2*, const double*,
+                             const double*, double, double, double*);
+hipError_t bse_launch_inverse_iter(hipStream_t, int, int, int, const double*, const int*,
+                                   const int*, const double*, const double*, double, double*, int,
+                                   double*);
+hipError_t bse_launch_cluster_mgs(hipStream_t, int, int, const int2*, int, const int*, const int*,
+                                  double*);
+hipError_t bse_launch_commit(hipStream_t, int, int, int, int, int, const int*, const int*,
+                             const double*, double2*, int64_t, int);
+hipError_t bse_launch_contract(hipStream_t, int, int, int, const int*, const int*, const double*,
+                               const double2*, double2*);
+hipError_t bse_zgemm(hipStream_t, int, int, int, int, int, double, double, const double2*, int64_t,
+                     const double2*, int64_t, double, double, double2*, int64_t, double2*, int64_t);
+hipError_t bse_launch_larft(hipStream_t, int, const double2*, const double2*, double2*, int);
+hipError_t bse_launch_dielectric(hipStream_t, int, const double*, double, int, const double*,
+                                 const double2*, double, double*, double*);
+hipError_t bse_launch_rows_to_reference(hipStream_t, int64_t, int, int, int, const double2*,
+                                        int64_t, double2*, int64_t);
+hipError_t bse_launch_convert_to_fp32(hipStream_t, int64_t, const double2*, float2*);
+}
+
+static const double kC = (1.602176634 * 1000.0) / 8.8541878176;   // e^2/eps0, eV*A
+static const int kPanel = 32;                                      // hetrd panel width
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+enum Stage { ST_TB, ST_TABLES, ST_ASM, ST_TRD, ST_TEVAL, ST_TVEC, ST_BT, ST_REF, ST_SPEC, ST_N };
+
+}  // namespace
+
+struct bse_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+    std::vector<DevBuf> bufs;   // indexed by slot id
+    // problem
+    bool have_problem = false;
+    bse_system sys{};
+    int norb = 0, nr = 0, nk = 0, ng = 0, B = 0, ns = 0;
+    int64_t N = 0;
+    double recip[9] = {0}, area = 0, volume = 0, wbar = 0, qhat[3] = {1, 0, 0};
+    std::vector<double> kpts_h, gvec_h;
+    // state
+    bool have_tb = false, have_tables = false, built = false, have_eig = false, have_f = false;
+    int precision = BSE_PREC_FP64;
+    int64_t ldH = 0;
+    int64_t n_eig = 0;          // size of the last eigenproblem
+    int nvec = 0;               // eigenvectors formed
+    bool eig_on_bse = false;    // last eig was of the assembled BSE matrix
+    int TK = 1;
+    hipEvent_t ev[ST_N + 1] = {};
+    bool ev_used[ST_N] = {};
+    double stage_ms[16] = {0};
+};
+
+namespace {
+
+enum Slot {
+    S_RCART, S_HR, S_TAU, S_KPTS, S_GVEC, S_Q, S_EPS, S_COEF, S_WTAB, S_VX, S_P, S_PV, S_RHO, S_DE,
+    S_H, S_D, S_E, S_TAUH, S_VSTORE, S_WP, S_ACOL, S_YR, S_YC, S_T12, S_PART, S_SCAL, S_COUNTER,
+    S_TALL, S_GRAM, S_E2, S_ITEMS, S_BOUNDS, S_LAMI, S_LAM, S_BSV, S_NBV, S_CLUST, S_IISCR, S_IIX,
+    S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_NSLOTS
+};
+
+int fail(bse_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HCK(expr)                                                                       \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return fail(ctx, BSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+void* buf(bse_ctx* ctx, int slot, size_t bytes, int* status) {
+    DevBuf& b = ctx->bufs[slot];
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return b.p;
+    if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.cap = 0; }
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+        b.p = nullptr;
+        *status = fail(ctx, BSE_ERR_ALLOC, "hipMalloc failed for " + std::to_string(bytes) + " bytes");
+        return nullptr;
+    }
+    b.cap = bytes;
+    return b.p;
+}
+
+template <typename T>
+T* bufT(bse_ctx* ctx, int slot, size_t count, int* status) {
+    return (T*)buf(ctx, slot, count * sizeof(T), status);
+}
+
+#define BUF(T, slot, count)                                  \
+    ({                                                        \
+        int _s = BSE_OK;                                      \
+        T* _p = bufT<T>(ctx, slot, (size_t)(count), &_s);     \
+        if (!_p) return _s;                                   \
+        _p;                                                   \
+    })
+
+void tick(bse_ctx* ctx, int stage) { (void)hipEventRecord(ctx->ev[stage], ctx->st); }
+
+void stage_begin(bse_ctx* ctx, int stage) {
+    (void)hipEventRecord(ctx->ev[stage], ctx->st);
+    ctx->ev_used[stage] = true;
+}
+void stage_end(bse_ctx* ctx, int stage) {
+    (void)hipEventRecord(ctx->ev[ST_N], ctx->st);
+    (void)hipEventSynchronize(ctx->ev[ST_N]);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev[stage], ctx->ev[ST_N]);
+    ctx->stage_ms[stage] = ms;
+}
+
+bool finite_all(const double* p, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" int bse_abi_version(void) { return BSE_ABI_VERSION; }
+
+extern "C" const char* bse_last_error(const bse_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" int bse_create(bse_ctx** out, int device) {
+    if (!out) return BSE_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BSE_ERR_HIP;
+    if (device < 0 || device >= ndev) return BSE_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return BSE_ERR_HIP;
+    bse_ctx* ctx = new bse_ctx();
+    ctx->device = device;
+    ctx->bufs.resize(S_NSLOTS);
+    if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return BSE_ERR_HIP;
+    }
+    for (int i = 0; i <= ST_N; ++i) (void)hipEventCreate(&ctx->ev[i]);
+    *out = ctx;
+    return BSE_OK;
+}
+
+extern "C" int bse_destroy(bse_ctx* ctx) {
+    if (!ctx) return BSE_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->st);
+    for (auto& b : ctx->bufs)
+        if (b.p) (void)hipFree(b.p);
+    for (int i = 0; i <= ST_N; ++i)
+        if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipStreamDestroy(ctx->st);
+    delete ctx;
+    return BSE_OK;
+}
+
+extern "C" int bse_stage_times(const bse_ctx* ctx, double* ms, int n) {
+    if (!ctx || !ms || n < 0 || n > 16) return BSE_ERR_ARG;
+    for (int i = 0; i < n; ++i) ms[i] = ctx->stage_ms[i];
+    return BSE_OK;
+}
+
+extern "C" int bse_problem_sizes(const bse_ctx* ctx, int64_t out[4]) {
+    if (!ctx || !out) return BSE_ERR_ARG;
+    out[0] = ctx->nk; out[1] = ctx->N; out[2] = ctx->norb; out[3] = ctx->ng;
+    return BSE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Problem upload: geometry (Crystal_Lattice 85-124, K_points 183-242,
+// G_points 319-385) is derived on the host and copied to the device.
+// ---------------------------------------------------------------------------
+extern "C" int bse_set_problem(bse_ctx* ctx, const bse_tb_model* m, const bse_system* s) {
+    if (!ctx) return BSE_ERR_ARG;
+    if (!m || !s || !m->r_cart || !m->h_r || !m->tau) return fail(ctx, BSE_ERR_ARG, "null argument");
+    if (m->norb < 1 || m->norb > BSE_MAXORB) return fail(ctx, BSE_ERR_UNSUPPORTED, "norb must be in [1, 16]");
+    if (m->nr < 1) return fail(ctx, BSE_ERR_ARG, "nr must be >= 1");
+    if (s->n1 < 1 || s->n2 < 1) return fail(ctx, BSE_ERR_ARG, "k grid must be at least 1 x 1");
+    if (s->nv < 1 || s->nc < 1 || s->n_occ < s->nv || s->n_occ + s->nc > m->norb)
+        return fail(ctx, BSE_ERR_ARG, "band window outside the Hamiltonian (nv <= n_occ, n_occ + nc <= norb)");
+    if (!(s->eps_env > 0.0) || !(s->r0 >= 0.0) || !(s->g_cut >= 0.0))
+        return fail(ctx, BSE_ERR_ARG, "screening parameters must satisfy eps_env > 0, r0 >= 0, g_cut >= 0");
+    const size_t nh = (size_t)m->nr * m->norb * m->norb * 2;
+    if (!finite_all(m->h_r, nh) || !finite_all(m->r_cart, 3 * (size_t)m->nr) ||
+        !finite_all(m->tau, 3 * (size_t)m->norb) || !finite_all(s->bravais, 9))
+        return fail(ctx, BSE_ERR_NUMERIC, "non-finite model input");
+    (void)hipSetDevice(ctx->device);
+    ctx->sys = *s;
+    ctx->norb = m->norb;
+    ctx->nr = m->nr;
+    ctx->nk = s->n1 * s->n2;
+    ctx->B = s->nv * s->nc;
+    ctx->ns = s->nv + s->nc;
+    ctx->N = (int64_t)ctx->nk * ctx->B;
+    // reciprocal vectors b_i = 2 pi (a_j x a_k) / Omega
+    const double* a = s->bravais;
+    auto cross = [](const double* x, const double* y, double* z) {
+        z[0] = x[1] * y[2] - x[2] * y[1];
+        z[1] = x[2] * y[0] - x[0] * y[2];
+        z[2] = x[0] * y[1] - x[1] * y[0];
+    };
+    double c0[3], c1[3], c2[3];
+    cross(a + 3, a + 6, c0);
+    cross(a + 6, a + 0, c1);
+    cross(a + 0, a + 3, c2);
+    const double det = a[0] * c0[0] + a[1] * c0[1] + a[2] * c0[2];
+    if (det == 0.0) return fail(ctx, BSE_ERR_ARG, "singular Bravais lattice");
+    ctx->volume = fabs(det);
+    const double f = 2.0 * M_PI / ctx->volume;
+    for (int i = 0; i < 3; ++i) {
+        ctx->recip[i] = f * c0[i];
+        ctx->recip[3 + i] = f * c1[i];
+        ctx->recip[6 + i] = f * c2[i];
+    }
+    double axb[3];
+    cross(a, a + 3, axb);
+    ctx->area = sqrt(axb[0] * axb[0] + axb[1] * axb[1] + axb[2] * axb[2]);
+    // k grid, k index = i * n2 + j
+    ctx->kpts_h.assign(3 * (size_t)ctx->nk, 0.0);
+    for (int i = 0; i < s->n1; ++i)
+        for (int j = 0; j < s->n2; ++j) {
+            const double f1 = (i + s->shift[0]) / s->n1, f2 = (j + s->shift[1]) / s->n2;
+            double* kp = &ctx->kpts_h[3 * ((size_t)i * s->n2 + j)];
+            for (int x = 0; x < 3; ++x) kp[x] = f1 * ctx->recip[x] + f2 * ctx->recip[3 + x];
+        }
+    // 2D parallelepiped G-list, G = 0 at index Ng/2
+    const double nb1 = sqrt(ctx->recip[0] * ctx->recip[0] + ctx->recip[1] * ctx->recip[1] + ctx->recip[2] * ctx->recip[2]);
+    const double nb2 = sqrt(ctx->recip[3] * ctx->recip[3] + ctx->recip[4] * ctx->recip[4] + ctx->recip[5] * ctx->recip[5]);
+    const int g1 = (int)(s->g_cut / nb1), g2 = (int)(s->g_cut / nb2);
+    ctx->gvec_h.clear();
+    for (int i = -g1; i <= g1; ++i)
+        for (int j = -g2; j <= g2; ++j)
+            for (int x = 0; x < 3; ++x) ctx->gvec_h.push_back(i * ctx->recip[x] + j * ctx->recip[3 + x]);
+    ctx->ng = (int)(ctx->gvec_h.size() / 3);
+    // q = 0 mini-BZ average of the Keldysh W over a disk of area (2pi)^2/(A Nk)
+    const double da = 4.0 * M_PI * M_PI / (ctx->area * ctx->nk);
+    const double rad = sqrt(da / M_PI);
+    ctx->wbar = (s->r0 > 0.0) ? (M_PI * kC / s->eps_env) * log1p(s->r0 * rad) / (s->r0 * da)
+                              : (M_PI * kC / s->eps_env) * rad / da;
+    ctx->TK = bse_assemble_pick_tk(s->nv, s->nc, m->norb);
+
+    double* d_rcart = BUF(double, S_RCART, 3 * (size_t)m->nr);
+    double2* d_hr = BUF(double2, S_HR, (size_t)m->nr * m->norb * m->norb);
+    double* d_tau = BUF(double, S_TAU, 3 * (size_t)m->norb);
+    double* d_k = BUF(double, S_KPTS, 3 * (size_t)ctx->nk);
+    double* d_g = BUF(double, S_GVEC, 3 * (size_t)ctx->ng);
+    double* d_q = BUF(double, S_Q, 3);
+    HCK(hipMemcpyAsync(d_rcart, m->r_cart, 3 * sizeof(double) * m->nr, hipMemcpyHostToDevice, ctx->st));
+    HCK(hipMemcpyAsync(d_hr, m->h_r, nh * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    HCK(hipMemcpyAsync(d_tau, m->tau, 3 * sizeof(double) * m->norb, hipMemcpyHostToDevice, ctx->st));
+    HCK(hipMemcpyAsync(d_k, ctx->kpts_h.data(), 3 * sizeof(double) * ctx->nk, hipMemcpyHostToDevice, ctx->st));
+    HCK(hipMemcpyAsync(d_g, ctx->gvec_h.data(), 3 * sizeof(double) * ctx->ng, hipMemcpyHostToDevice, ctx->st));
+    HCK(hipMemcpyAsync(d_q, s->q, 3 * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    double qh[3] = {s->dipole_dir[0], s->dipole_dir[1], s->dipole_dir[2]};
+    const double qn = sqrt(qh[0] * qh[0] + qh[1] * qh[1] + qh[2] * qh[2]);
+    if (!(qn > 0.0)) { qh[0] = 1.0; qh[1] = qh[2] = 0.0; }
+    else for (double& x : qh) x /= qn;
+    for (int x = 0; x < 3; ++x) ctx->qhat[x] = qh[x];
+    HCK(hipStreamSynchronize(ctx->st));
+    ctx->have_problem = true;
+    ctx->have_tb = ctx->have_tables = ctx->built = ctx->have_eig = ctx->have_f = false;
+    return BSE_OK;
+}
+
+// ---------------------------------------------------------------------------
+static int run_tb(bse_ctx* ctx) {
+    const bse_system& s = ctx->sys;
+    double* eps = BUF(double, S_EPS, (size_t)ctx->nk * ctx->ns);
+    double2* coef = BUF(double2, S_COEF, (size_t)ctx->nk * ctx->ns * ctx->norb);
+    stage_begin(ctx, ST_TB);
+    HCK(bse_launch_tb_states(ctx->st, ctx->nk, ctx->norb, ctx->nr,
+                             (const double*)ctx->bufs[S_RCART].p, (const double2*)ctx->bufs[S_HR].p,
+                             (const double*)ctx->bufs[S_KPTS].p, s.q, s.n_occ, s.nv, s.nc,
+                             s.scissor, nullptr, eps, coef, nullptr, nullptr));
+    stage_end(ctx, ST_TB);
+    ctx->have_tb = true;
+    return BSE_OK;
+}
+
+extern "C" int bse_tb_states(bse_ctx* ctx, double* eps, double* coef) {
+    if (!ctx) return BSE_ERR_ARG;
+    if (!ctx->have_problem) return fail(ctx, BSE_ERR_STATE, "no problem loaded");
+    (void)hipSetDevice(ctx->device);
+    int rc = run_tb(ctx);
+    if (rc) return rc;
+    if (eps)
+        HCK(hipMemcpyAsync(eps, ctx->bufs[S_EPS].p, sizeof(double) * ctx->nk * ctx->ns,
+                           hipMemcpyDeviceToHost, ctx->st));
+    if (coef)
+        HCK(hipMemcpyAsync(coef, ctx->bufs[S_COEF].p, sizeof(double2) * ctx->nk * ctx->ns * ctx->norb,
+                           hipMemcpyDeviceToHost, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    return BSE_OK;
+}
+
+static int run_tables(bse_ctx* ctx) {
+    const bse_system& s = ctx->sys;
+    const int no = ctx->norb;
+    const int64_t nw = (int64_t)(2 * s.n1 - 1) * (2 * s.n2 - 1) * no * no;
+    double2* wtab = BUF(double2, S_WTAB, nw);
+    double2* vx = BUF(double2, S_VX, (size_t)no * no);
+    double2* P = BUF(double2, S_P, (size_t)ctx->N * no);
+    double2* PV = BUF(double2, S_PV, (size_t)ctx->N * no);
+    double2* rho = BUF(double2, S_RHO, (size_t)ctx->N);
+    double* dE = BUF(double, S_DE, (size_t)ctx->N);
+    const double* gvec = (const double*)ctx->bufs[S_GVEC].p;
+    const double* tau = (const double*)ctx->bufs[S_TAU].p;
+    stage_begin(ctx, ST_TABLES);
+    double bvec[6] = {ctx->recip[0], ctx->recip[1], ctx->recip[2], ctx->recip[3], ctx->recip[4], ctx->recip[5]};
+    HCK(bse_launch_w_table(ctx->st, s.n1, s.n2, no, ctx->ng, gvec, bvec, tau, s.r0, s.eps_env,
+                           0.5 * kC, ctx->wbar, 0, wtab));
+    HCK(bse_launch_v_exchange(ctx->st, no, ctx->ng, gvec, tau, 0.5 * kC, vx));
+    HCK(bse_launch_transitions(ctx->st, ctx->nk, s.nv, s.nc, no, (const double*)ctx->bufs[S_EPS].p,
+                               (const double2*)ctx->bufs[S_COEF].p, nullptr, nullptr, tau, ctx->qhat,
+                               vx, P, PV, rho, dE));
+    stage_end(ctx, ST_TABLES);
+    ctx->have_tables = true;
+    return BSE_OK;
+}
+
+extern "C" int bse_build(bse_ctx* ctx, int precision) {
+    if (!ctx) return BSE_ERR_ARG;
+    if (!ctx->have_problem) return fail(ctx, BSE_ERR_STATE, "no problem loaded");
+    if (precision != BSE_PREC_FP64 && precision != BSE_PREC_FP32 && precision != BSE_PREC_BF16X2)
+        return fail(ctx, BSE_ERR_ARG, "unknown precision");
+    (void)hipSetDevice(ctx->device);
+    int rc;
+    if ((rc = run_tb(ctx))) return rc;
+    if ((rc = run_tables(ctx))) return rc;
+    const bse_system& s = ctx->sys;
+    const int fp32 = precision != BSE_PREC_FP64;
+    const int64_t N = ctx->N;
+    ctx->ldH = N;
+    void* H = buf(ctx, S_H, (size_t)N * N * (fp32 ? sizeof(float2) : sizeof(double2)), &rc);
+    if (!H) return rc;
+    const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
+    const double beta = s.f_s * s.exchange_scale / (ctx->area * ctx->nk);
+    const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
+    stage_begin(ctx, ST_ASM);
+    HCK(bse_launch_assemble(ctx->st, fp32, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb, ctx->TK,
+                            ctx->ldH, coef, coef, coef, coef, (const double2*)ctx->bufs[S_WTAB].p,
+                            (const double2*)ctx->bufs[S_PV].p, (const double2*)ctx->bufs[S_P].p,
+                            (const double*)ctx->bufs[S_DE].p, alpha, beta, H, 1));
+    stage_end(ctx, ST_ASM);
+    ctx->precision = precision;
+    ctx->built = true;
+    ctx->have_eig = ctx->have_f = false;
+    return BSE_OK;
+}
+
+extern "C" int bse_get_matrix(bse_ctx* ctx, void* dst, int64_t ld, int dst_on_device, int order) {
+    if (!ctx || !dst) return BSE_ERR_ARG;
+    if (!ctx->built) return fail(ctx, BSE_ERR_STATE, "matrix not built");
+    if (ctx->have_eig) return fail(ctx, BSE_ERR_STATE, "matrix storage was overwritten by bse_eigh; build again");
+    const int64_t N = ctx->N;
+    if (ld < N) return fail(ctx, BSE_ERR_ARG, "ld < N");
+    (void)hipSetDevice(ctx->device);
+    const int fp32 = ctx->precision != BSE_PREC_FP64;
+    const size_t es = fp32 ? sizeof(float2) : sizeof(double2);
+    void* out = dst;
+    int rc = BSE_OK;
+    if (!dst_on_device) {
+        out = buf(ctx, S_Z, (size_t)ld * N * es, &rc);
+        if (!out) return rc;
+    }
+    HCK(bse_launch_expand(ctx->st, fp32, N, ctx->B, ctx->nk, ctx->bufs[S_H].p, ctx->ldH, out, ld,
+                          order == BSE_ORDER_REFERENCE));
+    if (!dst_on_device) HCK(hipMemcpyAsync(dst, out, (size_t)ld * N * es, hipMemcpyDeviceToHost, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    ctx->have_eig = false;
+    return BSE_OK;
+}
+
+extern "C" int bse_get_dipoles(bse_ctx* ctx, double* rho) {
+    if (!ctx || !rho) return BSE_ERR_ARG;
+    if (!ctx->have_tables) return fail(ctx, BSE_ERR_STATE, "dipoles not built");
+    (void)hipSetDevice(ctx->device);
+    double2* tmp = BUF(double2, S_W1, ctx->N);
+    HCK(bse_launch_rows_to_reference(ctx->st, ctx->N, ctx->B, ctx->nk, 1,
+                                     (const double2*)ctx->bufs[S_RHO].p, ctx->N, tmp, ctx->N));
+    HCK(hipMemcpyAsync(rho, tmp, sizeof(double2) * ctx->N, hipMemcpyDeviceToHost, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    return BSE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Eigen pipeline on a device matrix H (n x n, lower triangle, ld) of element
+// type FP64 (fp32 = 0) or FP32 complex.  rho (may be null): contraction vector
+// for the oscillator strengths.
+// ---------------------------------------------------------------------------
+static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32,
+                        const double2* rho, int n_vectors)
+{
+    const int n = (int)n64;
+    const int nb = kPanel;
+    hipStream_t st = ctx->st;
+    // ---------------- K5: tridiagonalisation
+    int64_t vs_el, yr_el, yc_el, part_d;
+    bse_hetrd_workspace(n, nb, &vs_el, &yr_el, &yc_el, &part_d);
+    std::vector<int64_t> voff;
+    {
+        int64_t off = 0;
+        for (int j0 = 0; j0 < n - 1; j0 += nb) { voff.push_back(off); off += (int64_t)(n - j0) * nb; }
+    }
+    const int npan = (int)voff.size();
+    double* d = BUF(double, S_D, n);
+    double* e = BUF(double, S_E, std::max(n - 1, 1));
+    double2* tauh = BUF(double2, S_TAUH, std::max(n - 1, 1));
+    double2* Vst = BUF(double2, S_VSTORE, vs_el);
+    double2* Wp = BUF(double2, S_WP, (size_t)n * nb);
+    double2* acol = BUF(double2, S_ACOL, n);
+    double2* Yr = BUF(double2, S_YR, yr_el);
+    double2* Yc = BUF(double2, S_YC, yc_el);
+    double2* t12 = BUF(double2, S_T12, 128);
+    double* part = BUF(double, S_PART, part_d);
+    double* scal = BUF(double, S_SCAL, 8);
+    unsigned int* counter = BUF(unsigned int, S_COUNTER, 16);
+    HCK(hipMemsetAsync(counter, 0, 16 * sizeof(unsigned int), st));
+    stage_begin(ctx, ST_TRD);
+    HCK(bse_hetrd(st, fp32, n, H, ld, nb, d, e, tauh, Vst, voff.data(), Wp, acol, Yr, Yc, t12, part,
+                  scal, counter));
+    // compact-WY factors of every panel
+    double2* Tall = BUF(double2, S_TALL, (size_t)std::max(npan, 1) * nb * nb);
+    double2* Gram = BUF(double2, S_GRAM, (size_t)nb * nb);
+    const int64_t skws = (int64_t)64 * 1024 * 1024 / sizeof(double2) * 8;   // 512 MiB split-K workspace cap
+    const int64_t sk_el = std::min<int64_t>(skws, std::max<int64_t>((int64_t)64 * nb * std::max(n_vectors, 64), (int64_t)nb * nb * 512));
+    double2* skw = BUF(double2, S_SPLITK, sk_el);
+    for (int p = 0; p < npan; ++p) {
+        const int j0 = p * nb, m = n - j0, nbp = std::min(nb, n - 1 - j0);
+        const double2* Vp = Vst + voff[p];
+        HCK(bse_zgemm(st, 1, 0, nbp, nbp, m, 1, 0, Vp, m, Vp, m, 0, 0, Gram, nbp, skw, sk_el));
+        HCK(bse_launch_larft(st, nbp, tauh + j0, Gram, Tall + (size_t)p * nb * nb, nb));
+    }
+    stage_end(ctx, ST_TRD);
+
+    // ---------------- K6: tridiagonal eigenvalues (host bookkeeping + bisection)
+    stage_begin(ctx, ST_TEVAL);
+    std::vector<double> hd(n), he(std::max(n - 1, 0));
+    HCK(hipMemcpyAsync(hd.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (n > 1) HCK(hipMemcpyAsync(he.data(), e, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, st));
+    HCK(hipStreamSynchronize(st));
+    if (!finite_all(hd.data(), hd.size()) || !finite_all(he.data(), he.size()))
+        return fail(ctx, BSE_ERR_NUMERIC, "non-finite tridiagonal (input matrix not finite?)");
+    double tnorm = 0.0, emax2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double r = fabs(hd[i]);
+        if (i > 0) r += fabs(he[i - 1]);
+        if (i < n - 1) r += fabs(he[i]);
+        tnorm = std::max(tnorm, r);
+        if (i < n - 1) emax2 = std::max(emax2, he[i] * he[i]);
+    }
+    if (tnorm == 0.0) tnorm = 1.0;
+    const double pivmin = DBL_MIN * std::max(1.0, emax2);
+    std::vector<double> he2(std::max(n - 1, 1), 0.0);
+    std::vector<int> bstart;
+    bstart.push_back(0);
+    for (int i = 0; i < n - 1; ++i) {
+        const double ee = he[i] * he[i];
+        if (fabs(hd[i] * hd[i + 1]) * DBL_EPSILON * DBL_EPSILON + DBL_MIN >= ee) {
+            he[i] = 0.0;
+            bstart.push_back(i + 1);
+        } else {
+            he2[i] = ee;
+        }
+    }
+    const int nblk = (int)bstart.size();
+    std::vector<int4> items(n);
+    std::vector<double2> bounds(nblk);
+    for (int b = 0; b < nblk; ++b) {
+        const int bs = bstart[b], be = (b + 1 < nblk) ? bstart[b + 1] : n;
+        double gl = DBL_MAX, gu = -DBL_MAX;
+        for (int i = bs; i < be; ++i) {
+            double r = 0.0;
+            if (i > bs) r += fabs(he[i - 1]);
+            if (i < be - 1) r += fabs(he[i]);
+            gl = std::min(gl, hd[i] - r);
+            gu = std::max(gu, hd[i] + r);
+        }
+        const double fud = 2.1 * DBL_EPSILON * tnorm * (be - bs) + 4.2 * pivmin;
+        bounds[b] = make_double2(gl - fud, gu + fud);
+        for (int i = bs; i < be; ++i) items[i] = make_int4(bs, be - bs, i - bs, b);
+    }
+    double* d_e2 = BUF(double, S_E2, std::max(n - 1, 1));
+    int4* d_items = BUF(int4, S_ITEMS, n);
+    double2* d_bounds = BUF(double2, S_BOUNDS, nblk);
+    double* d_lami = BUF(double, S_LAMI, n);
+    HCK(hipMemcpyAsync(d_e2, he2.data(), sizeof(double) * he2.size(), hipMemcpyHostToDevice, st));
+    HCK(hipMemcpyAsync(e, he.data(), sizeof(double) * std::max(n - 1, 0), hipMemcpyHostToDevice, st));
+    HCK(hipMemcpyAsync(d_items, items.data(), sizeof(int4) * n, hipMemcpyHostToDevice, st));
+    HCK(hipMemcpyAsync(d_bounds, bounds.data(), sizeof(double2) * nblk, hipMemcpyHostToDevice, st));
+    HCK(bse_launch_bisect(st, n, d_items, d_bounds, d, d_e2, pivmin, tnorm, d_lami));
+    std::vector<double> lami(n);
+    HCK(hipMemcpyAsync(lami.data(), d_lami, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HCK(hipStreamSynchronize(st));
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return lami[a] < lami[b]; });
+    std::vector<double> lam(n);
+    std::vector<int> bsv(n), nbv(n);
+    for (int i = 0; i < n; ++i) {
+        lam[i] = lami[ord[i]];
+        bsv[i] = items[ord[i]].x;
+        nbv[i] = items[ord[i]].y;
+    }
+    double* d_lam = BUF(double, S_LAM, n);
+    int* d_bsv = BUF(int, S_BSV, n);
+    int* d_nbv = BUF(int, S_NBV, n);
+    HCK(hipMemcpyAsync(d_lam, lam.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HCK(hipMemcpyAsync(d_bsv, bsv.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HCK(hipMemcpyAsync(d_nbv, nbv.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    stage_end(ctx, ST_TEVAL);
+
+    // ---------------- y = Q^H rho for the oscillator contraction
+    double2* y = nullptr;
+    if (rho) {
+        y = BUF(double2, S_Y, n);
+        double2* w1 = BUF(double2, S_W1, (size_t)nb * 64);
+        double2* w2 = BUF(double2, S_W2, (size_t)nb * 64);
+        HCK(hipMemcpyAsync(y, rho, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+        for (int p = 0; p < npan; ++p) {
+            const int j0 = p * nb, m = n - j0, nbp = std::min(nb, n - 1 - j0);
+            const double2* Vp = Vst + voff[p];
+            const double2* Tp = Tall + (size_t)p * nb * nb;
+            HCK(bse_zgemm(st, 1, 0, nbp, 1, m, 1, 0, Vp, m, y + j0, n, 0, 0, w1, nbp, skw, sk_el));
+            HCK(bse_zgemm(st, 1, 0, nbp, 1, nbp, 1, 0, Tp, nb, w1, nbp, 0, 0, w2, nbp, nullptr, 0));
+            HCK(bse_zgemm(st, 0, 0, m, 1, nbp, -1, 0, Vp, m, w2, nbp, 1, 0, y + j0, n, nullptr, 0));
+        }
+    }
+
+    // ---------------- K6: tridiagonal eigenvectors by batches
+    stage_begin(ctx, ST_TVEC);
+    const int nstore = (n_vectors < 0) ? n : std::min(n_vectors, n);
+    const double tolc = 1e-7 * tnorm;
+    int bsz = std::min(n, 4096);
+    const int64_t iscr = (int64_t)5 * n * bsz;
+    double* scr = BUF(double, S_IISCR, iscr);
+    double* xb = BUF(double, S_IIX, (size_t)n * bsz);
+    double2* Z = nullptr;
+    if (nstore > 0) Z = BUF(double2, S_Z, (size_t)n * nstore);
+    double2* fvec = rho ? BUF(double2, S_F, n) : nullptr;
+    int2* d_cl = BUF(int2, S_CLUST, bsz);
+    int m0 = 0;
+    while (m0 < n) {
+        int m1 = std::min(n, m0 + bsz);
+        // do not split a cluster across batches
+        while (m1 < n && m1 > m0 + 1 && bsv[m1] == bsv[m1 - 1] && lam[m1] - lam[m1 - 1] <= tolc) --m1;
+        const int cnt = m1 - m0;
+        std::vector<int2> cls;
+        int i = m0;
+        while (i < m1) {
+            int j = i + 1;
+            while (j < m1 && bsv[j] == bsv[i] && lam[j] - lam[j - 1] <= tolc) ++j;
+            if (j - i > 1) cls.push_back(make_int2(i - m0, j - i));
+            i = j;
+        }
+        HCK(bse_launch_inverse_iter(st, bsz, cnt, m0, d_lam, d_bsv, d_nbv, d, e, tnorm, scr, n, xb));
+        if (!cls.empty()) {
+            HCK(hipMemcpyAsync(d_cl, cls.data(), sizeof(int2) * cls.size(), hipMemcpyHostToDevice, st));
+            HCK(bse_launch_cluster_mgs(st, bsz, (int)cls.size(), d_cl, m0, d_bsv, d_nbv, xb));
+        }
+        if (rho) HCK(bse_launch_contract(st, bsz, cnt, m0, d_bsv, d_nbv, xb, y, fvec));
+        const int ncommit = std::max(0, std::min(m1, nstore) - m0);
+        if (ncommit > 0) HCK(bse_launch_commit(st, n, bsz, cnt, ncommit, m0, d_bsv, d_nbv, xb, Z, n, m0));
+        HCK(hipStreamSynchronize(st));   // keeps d_cl and the host vectors alive
+        m0 = m1;
+    }
+    stage_end(ctx, ST_TVEC);
+
+    // ---------------- K7: back-transform Z <- Q Z
+    stage_begin(ctx, ST_BT);
+    if (nstore > 0) {
+        double2* w1 = BUF(double2, S_W1, (size_t)nb * nstore);
+        double2* w2 = BUF(double2, S_W2, (size_t)nb * nstore);
+        for (int p = npan - 1; p >= 0; --p) {
+            const int j0 = p * nb, m = n - j0, nbp = std::min(nb, n - 1 - j0);
+            const double2* Vp = Vst + voff[p];
+            const double2* Tp = Tall + (size_t)p * nb * nb;
+            HCK(bse_zgemm(st, 1, 0, nbp, nstore, m, 1, 0, Vp, m, Z + j0, n, 0, 0, w1, nbp, skw, sk_el));
+            HCK(bse_zgemm(st, 0, 0, nbp, nstore, nbp, 1, 0, Tp, nb, w1, nbp, 0, 0, w2, nbp, nullptr, 0));
+            HCK(bse_zgemm(st, 0, 0, m, nstore, nbp, -1, 0, Vp, m, w2, nbp, 1, 0, Z + j0, n, nullptr, 0));
+        }
+    }
+    stage_end(ctx, ST_BT);
+    double* ev = BUF(double, S_EVAL, n);   // eigenvalues live in the S_LAM slot; keep a stable copy
+    HCK(hipMemcpyAsync(ev, d_lam, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    HCK(hipStreamSynchronize(st));
+    ctx->n_eig = n;
+    ctx->nvec = nstore;
+    ctx->have_eig = true;
+    ctx->have_f = (rho != nullptr);
+    return BSE_OK;
+}
+
+extern "C" int bse_eigh(bse_ctx* ctx, int n_vectors, int refine) {
+    if (!ctx) return BSE_ERR_ARG;
+    if (!ctx->built) return fail(ctx, BSE_ERR_STATE, "matrix not built");
+    if (ctx->have_eig) return fail(ctx, BSE_ERR_STATE, "matrix already reduced; build again");
+    if (refine) return fail(ctx, BSE_ERR_UNSUPPORTED, "refinement not available in this build");
+    (void)hipSetDevice(ctx->device);
+    const int fp32 = ctx->precision != BSE_PREC_FP64;
+    ctx->eig_on_bse = true;
+    return eig_pipeline(ctx, ctx->N, ctx->bufs[S_H].p, ctx->ldH, fp32,
+                        (const double2*)ctx->bufs[S_RHO].p, n_vectors);
+}
+
+extern "C" int bse_get_eigenvalues(bse_ctx* ctx, double* evals) {
+    if (!ctx || !evals) return BSE_ERR_ARG;
+    if (!ctx->have_eig) return fail(ctx, BSE_ERR_STATE, "no eigendecomposition");
+    HCK(hipMemcpy(evals, ctx->bufs[S_EVAL].p, sizeof(double) * ctx->n_eig, hipMemcpyDeviceToHost));
+    return BSE_OK;
+}
+
+extern "C" int bse_get_eigenvectors(bse_ctx* ctx, double* z, int64_t ld, int nvec, int dst_on_device) {
+    if (!ctx || !z) return BSE_ERR_ARG;
+    if (!ctx->have_eig) return fail(ctx, BSE_ERR_STATE, "no eigendecomposition");
+    if (nvec < 0 || nvec > ctx->nvec) return fail(ctx, BSE_ERR_ARG, "nvec exceeds the formed eigenvectors");
+    if (ld < ctx->n_eig) return fail(ctx, BSE_ERR_ARG, "ld < N");
+    if (nvec == 0) return BSE_OK;
+    (void)hipSetDevice(ctx->device);
+    const int64_t n = ctx->n_eig;
+    const double2* Z = (const double2*)ctx->bufs[S_Z].p;
+    const int B = ctx->eig_on_bse ? ctx->B : 1;
+    const int nk = ctx->eig_on_bse ? ctx->nk : (int)n;
+    double2* dst = (double2*)z;
+    if (!dst_on_device) dst = BUF(double2, S_W2, (size_t)ld * nvec);
+    HCK(bse_launch_rows_to_reference(ctx->st, n, B, nk, nvec, Z, n, dst, ld));
+    if (!dst_on_device)
+        HCK(hipMemcpyAsync(z, dst, sizeof(double2) * ld * nvec, hipMemcpyDeviceToHost, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    return BSE_OK;
+}
+
+extern "C" int bse_oscillator(bse_ctx* ctx, double* f) {
+    if (!ctx || !f) return BSE_ERR_ARG;
+    if (!ctx->have_f) return fail(ctx, BSE_ERR_STATE, "no oscillator strengths (run bse_eigh on a built matrix)");
+    HCK(hipMemcpy(f, ctx->bufs[S_F].p, sizeof(double2) * ctx->n_eig, hipMemcpyDeviceToHost));
+    return BSE_OK;
+}
+
+extern "C" int bse_spectrum(bse_ctx* ctx, const double* omega, int n_omega, double eta,
+                            double* eps_re, double* eps_im) {
+    if (!ctx || !omega || !eps_re || !eps_im || n_omega <= 0) return BSE_ERR_ARG;
+    if (!ctx->have_f) return fail(ctx, BSE_ERR_STATE, "no oscillator strengths");
+    (void)hipSetDevice(ctx->device);
+    double* d_om = BUF(double, S_EPSOUT, 3 * (size_t)n_omega);
+    HCK(hipMemcpyAsync(d_om, omega, sizeof(double) * n_omega, hipMemcpyHostToDevice, ctx->st));
+    const double pref = kC / (ctx->volume * ctx->nk);
+    stage_begin(ctx, ST_SPEC);
+    HCK(bse_launch_dielectric(ctx->st, n_omega, d_om, eta, (int)ctx->n_eig,
+                              (const double*)ctx->bufs[S_EVAL].p, (const double2*)ctx->bufs[S_F].p,
+                              pref, d_om + n_omega, d_om + 2 * n_omega));
+    stage_end(ctx, ST_SPEC);
+    HCK(hipMemcpyAsync(eps_re, d_om + n_omega, sizeof(double) * n_omega, hipMemcpyDeviceToHost, ctx->st));
+    HCK(hipMemcpyAsync(eps_im, d_om + 2 * n_omega, sizeof(double) * n_omega, hipMemcpyDeviceToHost, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    return BSE_OK;
+}
+
+extern "C" int bse_eigh_dense(bse_ctx* ctx, int n, const double* a, int64_t lda, int a_on_device,
+                              int precision, int n_vectors, double* evals, double* z, int64_t ldz) {
+    if (!ctx || !a || n <= 0 || lda < n) return BSE_ERR_ARG;
+    if (precision != BSE_PREC_FP64 && precision != BSE_PREC_FP32)
+        return fail(ctx, BSE_ERR_ARG, "dense eig precision must be FP64 or FP32 storage");
+    (void)hipSetDevice(ctx->device);
+    const int fp32 = precision == BSE_PREC_FP32;
+    int rc = BSE_OK;
+    // stage the FP64 input; FP32 storage is converted on the device
+    double2* stage = fp32 ? BUF(double2, S_STAGE, (size_t)n * n) : BUF(double2, S_H, (size_t)n * n);
+    HCK(hipMemcpy2DAsync(stage, sizeof(double2) * n, a, sizeof(double2) * lda, sizeof(double2) * n, n,
+                         a_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->st));
+    void* Hm = stage;
+    if (fp32) {
+        float2* H32 = BUF(float2, S_H, (size_t)n * n);
+        HCK(bse_launch_convert_to_fp32(ctx->st, (int64_t)n * n, stage, H32));
+        Hm = H32;
+    }
+    ctx->built = false;
+    ctx->have_tables = false;
+    ctx->eig_on_bse = false;
+    rc = eig_pipeline(ctx, n, Hm, n, fp32, nullptr, n_vectors);
+    if (rc) return rc;
+    if (evals) HCK(hipMemcpy(evals, ctx->bufs[S_EVAL].p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (z && ctx->nvec > 0) {
+        if (ldz < n) return fail(ctx, BSE_ERR_ARG, "ldz < n");
+        HCK(hipMemcpy2D(z, sizeof(double2) * ldz, ctx->bufs[S_Z].p, sizeof(double2) * n,
+                        sizeof(double2) * n, ctx->nvec, hipMemcpyDeviceToHost));
+    }
+    return BSE_OK;
+}

@@ -1,0 +1,116 @@
+// Shared device/host helpers for the gfx950 excitonic kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cmath>
+
+// Largest number of orbitals (spin components included) per cell.
+#define BSE_MAXORB 16
+
+// ---------------------------------------------------------------------------
+// Complex arithmetic on HIP vector types (double2 = FP64 complex, float2 =
+// FP32 complex); .x = real part, .y = imaginary part.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline double2 cmk(double r, double i) { return make_double2(r, i); }
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__host__ __device__ inline double2 cmulc(double2 a, double2 b) {
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ inline double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+// acc += a * b
+__host__ __device__ inline void cfma(double2& acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+}
+// acc += conj(a) * b
+__host__ __device__ inline void cfmac(double2& acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, fma(a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(-a.y, b.x, acc.y));
+}
+__host__ __device__ inline double2 cdiv(double2 a, double2 b) {
+    double s = fabs(b.x) + fabs(b.y);
+    double br = b.x / s, bi = b.y / s;
+    double ar = a.x / s, ai = a.y / s;
+    double d = br * br + bi * bi;
+    return make_double2((ar * br + ai * bi) / d, (ai * br - ar * bi) / d);
+}
+
+// Storage conversions between the matrix element types and FP64 complex.
+template <typename T> struct cplx;
+template <> struct cplx<double2> {
+    __device__ static inline double2 load(const double2* p) { return *p; }
+    __device__ static inline void store(double2* p, double2 v) { *p = v; }
+};
+template <> struct cplx<float2> {
+    __device__ static inline double2 load(const float2* p) {
+        float2 v = *p;
+        return make_double2((double)v.x, (double)v.y);
+    }
+    __device__ static inline void store(float2* p, double2 v) {
+        *p = make_float2((float)v.x, (float)v.y);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Wave-level reductions (64-lane wavefronts on CDNA4).
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block reduction of a double over blockDim.x threads (multiple of 64, <= 1024).
+// Every thread receives the total.  Deterministic order.
+__device__ inline double block_sum(double v, double* sh /* >= 16 doubles */) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    const int nw = blockDim.x >> 6;
+    for (int i = 0; i < nw; ++i) t += sh[i];
+    return t;
+}
+
+__device__ inline double2 block_sum2(double2 v, double* sh /* >= 32 doubles */) {
+    v.x = wave_sum(v.x);
+    v.y = wave_sum(v.y);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) { sh[2 * w] = v.x; sh[2 * w + 1] = v.y; }
+    __syncthreads();
+    double2 t = make_double2(0.0, 0.0);
+    const int nw = blockDim.x >> 6;
+    for (int i = 0; i < nw; ++i) { t.x += sh[2 * i]; t.y += sh[2 * i + 1]; }
+    return t;
+}
+
+// "Last block done" ticket: returns true in exactly one block (the last to
+// arrive) after every block has published its partial results.  The counter
+// is a 32-bit device word that the last block resets to zero.
+__device__ inline bool last_block_ticket(unsigned int* counter, unsigned int nblocks) {
+    __shared__ bool is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int prev = atomicAdd(counter, 1u);
+        is_last = (prev == nblocks - 1);
+    }
+    __syncthreads();
+    if (is_last) {
+        __threadfence();
+        if (threadIdx.x == 0) atomicExch(counter, 0u);
+    }
+    return is_last;
+}
+
+__host__ __device__ inline int64_t ceil_div64(int64_t a, int64_t b) { return (a + b - 1) / b; }

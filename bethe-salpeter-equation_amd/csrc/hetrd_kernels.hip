@@ -1,0 +1,438 @@
+// K5: blocked Householder reduction of a Hermitian matrix to real symmetric
+// tridiagonal form, Q^H A Q = T (lower storage).
+//
+// This is the dense half of the eigensolver that Armadillo's eig_sym hands to
+// LAPACK zheevd (bse.version13.cpp:3330).  Algorithm: the panel form of the
+// lower-triangular reduction (Golub & Van Loan, 5th ed., sec. 8.3; the
+// LAPACK hetrd/latrd structure).  For each panel of nb columns, column by
+// column:
+//   (a) update the column with the panel's earlier reflectors:
+//       a -= V conj(W(c,:))^T + W conj(V(c,:))^T
+//   (b) reflector: beta = -sign(Re alpha) ||x||, tau = (beta - alpha)/beta,
+//       v = [1; x / (alpha - beta)]
+//   (c) y = A22 v          (Hermitian matrix-vector product, lower triangle)
+//   (d) y -= V (W^H v) + W (V^H v)
+//   (e) y *= tau;  w = y - (tau/2)(y^H v) v
+// and after the panel the trailing update A22 -= V W^H + W V^H.
+//
+// The matrix may be stored in FP64 or FP32 complex; the column, reflectors
+// and the V/W panels are always FP64.  Reflectors go to a separate packed
+// FP64 store so the back-transform is exact FP64 even for FP32 storage.
+//
+// All reductions are deterministic: per-block partials reduced in a fixed
+// order by the last block to finish ("ticket" counter).
+#include "bse_common.hpp"
+
+struct HetrdCol {
+    int n;            // matrix size
+    int64_t lda;
+    int j0;           // panel start column
+    int i;            // column inside the panel
+    int m;            // n - j0 (panel rows)
+    double2* Vp;      // panel reflectors, m x nb, ld m
+    double2* Wp;      // panel W, m x nb, ld m
+    double2* acol;    // updated column, length m - i (row c .. n-1)
+    double* d;
+    double* e;
+    double2* tau;
+    double* scal;     // [0,1] scale (1/(alpha-beta)), [2,3] tau, [4,5] alpha2
+    double* part;     // reduction partials
+    unsigned int* counter;
+};
+
+// (a) + (b): column update, norm, reflector scalars.
+template <typename T>
+__global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict__ A)
+{
+    __shared__ double sh[32];
+    const int c = h.j0 + h.i;
+    const int L = h.m - h.i;                 // rows c .. n-1
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double nrm = 0.0;
+    if (t < L) {
+        const int lrow = h.i + t;
+        double2 a = cplx<T>::load(A + (c + t) + (int64_t)c * h.lda);
+        for (int l = 0; l < h.i; ++l) {
+            const double2 vr = h.Vp[lrow + (int64_t)l * h.m];
+            const double2 wr = h.Wp[lrow + (int64_t)l * h.m];
+            const double2 wc = h.Wp[h.i + (int64_t)l * h.m];
+            const double2 vc = h.Vp[h.i + (int64_t)l * h.m];
+            a = csub(a, cadd(cmul(vr, cconj(wc)), cmul(wr, cconj(vc))));
+        }
+        if (t == 0) a.y = 0.0;
+        h.acol[t] = a;
+        if (t >= 2) nrm = cabs2(a);
+    }
+    nrm = block_sum(nrm, sh);
+    if (threadIdx.x == 0) h.part[blockIdx.x] = nrm;
+    if (!last_block_ticket(h.counter, gridDim.x)) return;
+    if (threadIdx.x == 0) {
+        double xn2 = 0.0;
+        for (unsigned b = 0; b < gridDim.x; ++b) xn2 += h.part[b];
+        const double2 a0 = h.acol[0];
+        h.d[c] = a0.x;
+        const double2 alpha = h.acol[1];
+        if (xn2 == 0.0 && alpha.y == 0.0) {
+            h.e[c] = alpha.x;
+            h.tau[c] = cmk(0, 0);
+            h.scal[0] = 0.0; h.scal[1] = 0.0; h.scal[2] = 0.0; h.scal[3] = 0.0;
+        } else {
+            const double nr = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+            const double beta = (alpha.x >= 0.0) ? -nr : nr;
+            const double2 tau = cmk((beta - alpha.x) / beta, -alpha.y / beta);
+            const double2 sc = cdiv(cmk(1, 0), cmk(alpha.x - beta, alpha.y));
+            h.e[c] = beta;
+            h.tau[c] = tau;
+            h.scal[0] = sc.x; h.scal[1] = sc.y; h.scal[2] = tau.x; h.scal[3] = tau.y;
+        }
+    }
+}
+
+// Reflector vector into Vp(:, i), plus partials of t1 = W^H v and t2 = V^H v
+// over rows i+1 .. m-1 for the panel's earlier columns l < i.
+__global__ __launch_bounds__(256) void k_makev(HetrdCol h, double2* __restrict__ t12)
+{
+    __shared__ double sh[32];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;   // local panel row
+    const double2 sc = cmk(h.scal[0], h.scal[1]);
+    double2 v = cmk(0, 0);
+    if (r < h.m) {
+        if (r == h.i + 1) v = cmk(1, 0);
+        else if (r > h.i + 1) v = cmul(h.acol[r - h.i], sc);
+        h.Vp[r + (int64_t)h.i * h.m] = v;
+    }
+    // partials
+    for (int l = 0; l < h.i; ++l) {
+        double2 p1 = cmk(0, 0), p2 = cmk(0, 0);
+        if (r < h.m && r > h.i) {
+            cfmac(p1, h.Wp[r + (int64_t)l * h.m], v);
+            cfmac(p2, h.Vp[r + (int64_t)l * h.m], v);
+        }
+        p1 = block_sum2(p1, sh);
+        p2 = block_sum2(p2, sh);
+        if (threadIdx.x == 0) {
+            double2* pp = (double2*)h.part;
+            pp[(blockIdx.x * 2 + 0) * 64 + l] = p1;
+            pp[(blockIdx.x * 2 + 1) * 64 + l] = p2;
+        }
+    }
+    if (!last_block_ticket(h.counter, gridDim.x)) return;
+    const int l = threadIdx.x;
+    if (l < h.i) {
+        const double2* pp = (const double2*)h.part;
+        double2 s1 = cmk(0, 0), s2 = cmk(0, 0);
+        for (unsigned b = 0; b < gridDim.x; ++b) {
+            s1 = cadd(s1, pp[(b * 2 + 0) * 64 + l]);
+            s2 = cadd(s2, pp[(b * 2 + 1) * 64 + l]);
+        }
+        t12[l] = s1;
+        t12[64 + l] = s2;
+    }
+}
+
+// (c): y = A22 v for the trailing Hermitian block (lower triangle stored).
+// Workgroup (q, bj) handles block column bj (64 columns) and block rows
+// bi in [bj + q*CH, bj + (q+1)*CH).  Row results of each 64x64 block go to
+// Yr[bi][bj]; the transposed (conjugate) contribution for column block bj is
+// accumulated across the chunk and written to Yc[bj][q].
+#define HEMV_CH 8
+template <typename T>
+__global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict__ A, int64_t lda,
+                                                   const double2* __restrict__ v,
+                                                   double2* __restrict__ Yr,
+                                                   double2* __restrict__ Yc, int nbk, int nq)
+{
+    const int q = blockIdx.x, bj = blockIdx.y;
+    const int bi0 = bj + q * HEMV_CH;
+    if (bi0 >= nbk) return;
+    const int bi1 = min(nbk, bi0 + HEMV_CH);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ double2 s_v[64];
+    __shared__ double2 s_part[4][64];
+    const int col0 = bj * 64;
+    if (threadIdx.x < 64) {
+        const int cidx = col0 + threadIdx.x;
+        s_v[threadIdx.x] = (cidx < nt) ? v[cidx] : cmk(0, 0);
+    }
+    __syncthreads();
+    double2 cacc[16];
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) cacc[jj] = cmk(0, 0);
+    const int cw = col0 + w * 16;             // first column of this wave
+    for (int bi = bi0; bi < bi1; ++bi) {
+        const int row = bi * 64 + lane;
+        const bool rvalid = row < nt;
+        const double2 vr = rvalid ? v[row] : cmk(0, 0);
+        double2 racc = cmk(0, 0);
+        const T* Ap = A + row + (int64_t)cw * lda;
+        if (bi > bj && (bi + 1) * 64 <= nt) {
+            double2 a[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) a[jj] = cplx<T>::load(Ap + (int64_t)jj * lda);
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                cfma(racc, a[jj], s_v[w * 16 + jj]);
+                cfmac(cacc[jj], a[jj], vr);
+            }
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const int col = cw + jj;
+                if (!rvalid || col >= nt || row < col) continue;
+                double2 a = cplx<T>::load(Ap + (int64_t)jj * lda);
+                if (row == col) {
+                    cfma(racc, cmk(a.x, 0.0), s_v[w * 16 + jj]);
+                } else {
+                    cfma(racc, a, s_v[w * 16 + jj]);
+                    cfmac(cacc[jj], a, vr);
+                }
+            }
+        }
+        s_part[w][lane] = racc;
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            double2 s = cadd(cadd(s_part[0][lane], s_part[1][lane]),
+                             cadd(s_part[2][lane], s_part[3][lane]));
+            Yr[((int64_t)bi * nbk + bj) * 64 + lane] = s;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) {
+        const double x = wave_sum(cacc[jj].x), y = wave_sum(cacc[jj].y);
+        if (lane == 0) Yc[((int64_t)bj * nq + q) * 64 + w * 16 + jj] = cmk(x, y);
+    }
+}
+
+// Reduce the hemv partials, apply (d) and (e) scaling, partial dot y^H v.
+// Writes the unfinished W column (y) and alpha2 = -tau/2 (y^H v).
+__global__ __launch_bounds__(256) void k_hemv_reduce(HetrdCol h, int nt, const double2* __restrict__ Yr,
+                                                     const double2* __restrict__ Yc, int nbk, int nq,
+                                                     const double2* __restrict__ t12)
+{
+    __shared__ double sh[32];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;   // trailing row
+    double2 dotp = cmk(0, 0);
+    const double2 tau = cmk(h.scal[2], h.scal[3]);
+    if (r < nt) {
+        const int b = r / 64, lane = r % 64;
+        double2 y = cmk(0, 0);
+        for (int bj = 0; bj <= b; ++bj) y = cadd(y, Yr[((int64_t)b * nbk + bj) * 64 + lane]);
+        const int nqb = (int)ceil_div64(nbk - b, HEMV_CH);
+        for (int q = 0; q < nqb; ++q) y = cadd(y, Yc[((int64_t)b * nq + q) * 64 + lane]);
+        const int lr = h.i + 1 + r;                         // panel-local row
+        for (int l = 0; l < h.i; ++l) {
+            const double2 vv = h.Vp[lr + (int64_t)l * h.m];
+            const double2 ww = h.Wp[lr + (int64_t)l * h.m];
+            y = csub(y, cadd(cmul(vv, t12[l]), cmul(ww, t12[64 + l])));
+        }
+        y = cmul(y, tau);
+        h.Wp[lr + (int64_t)h.i * h.m] = y;
+        const double2 v = h.Vp[lr + (int64_t)h.i * h.m];
+        cfmac(dotp, y, v);
+    }
+    dotp = block_sum2(dotp, sh);
+    double2* pp = (double2*)h.part;
+    if (threadIdx.x == 0) pp[blockIdx.x] = dotp;
+    if (!last_block_ticket(h.counter, gridDim.x)) return;
+    if (threadIdx.x == 0) {
+        double2 s = cmk(0, 0);
+        for (unsigned b = 0; b < gridDim.x; ++b) s = cadd(s, pp[b]);
+        const double2 a2 = cscale(cmul(tau, s), -0.5);
+        h.scal[4] = a2.x;
+        h.scal[5] = a2.y;
+    }
+}
+
+// W(:, i) = y + alpha2 v, zero above the reflector.
+__global__ __launch_bounds__(256) void k_wfinal(HetrdCol h)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= h.m) return;
+    double2* wcol = h.Wp + (int64_t)h.i * h.m;
+    if (r <= h.i) { wcol[r] = cmk(0, 0); return; }
+    const double2 a2 = cmk(h.scal[4], h.scal[5]);
+    const double2 v = h.Vp[r + (int64_t)h.i * h.m];
+    wcol[r] = cadd(wcol[r], cmul(a2, v));
+}
+
+// Trailing update A22 -= V W^H + W V^H on lower 64x64 tiles, K = nb.
+// V, W rows start at panel row nb (ld m).
+template <typename T>
+__global__ __launch_bounds__(256) void k_her2k_lower(int n2, T* __restrict__ A, int64_t lda,
+                                                    const double2* __restrict__ V,
+                                                    const double2* __restrict__ W, int64_t ldv,
+                                                    int nb)
+{
+    int ti, tj;
+    {
+        const int64_t idx = blockIdx.x;
+        int64_t r = (int64_t)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+        while (r * (r + 1) / 2 > idx) --r;
+        while ((r + 1) * (r + 2) / 2 <= idx) ++r;
+        ti = (int)r;
+        tj = (int)(idx - r * (r + 1) / 2);
+    }
+    __shared__ double2 sVr[8][64], sWr[8][64], sVc[8][64], sWc[8][64];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;   // 16 x 16 threads
+    const int r0 = ti * 64, c0 = tj * 64;
+    double2 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = cmk(0, 0);
+    for (int l0 = 0; l0 < nb; l0 += 8) {
+        for (int e = threadIdx.x; e < 8 * 64; e += 256) {
+            const int rr = e & 63, ll = e >> 6;
+            const int l = l0 + ll;
+            const bool lok = l < nb;
+            const int gr = r0 + rr, gc = c0 + rr;
+            sVr[ll][rr] = (lok && gr < n2) ? V[gr + (int64_t)l * ldv] : cmk(0, 0);
+            sWr[ll][rr] = (lok && gr < n2) ? W[gr + (int64_t)l * ldv] : cmk(0, 0);
+            sVc[ll][rr] = (lok && gc < n2) ? V[gc + (int64_t)l * ldv] : cmk(0, 0);
+            sWc[ll][rr] = (lok && gc < n2) ? W[gc + (int64_t)l * ldv] : cmk(0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ll = 0; ll < 8; ++ll) {
+            double2 vr[4], wr[4], vc[4], wc[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                vr[a] = sVr[ll][tx + 16 * a];
+                wr[a] = sWr[ll][tx + 16 * a];
+                vc[a] = sVc[ll][ty + 16 * a];
+                wc[a] = sWc[ll][ty + 16 * a];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    cfma(acc[a][b], vr[a], cconj(wc[b]));
+                    cfma(acc[a][b], wr[a], cconj(vc[b]));
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int gr = r0 + tx + 16 * a, gc = c0 + ty + 16 * b;
+            if (gr < n2 && gc < n2 && gr >= gc) {
+                T* p = A + gr + (int64_t)gc * lda;
+                double2 x = cplx<T>::load(p);
+                x = csub(x, acc[a][b]);
+                if (gr == gc) x.y = 0.0;
+                cplx<T>::store(p, x);
+            }
+        }
+}
+
+__global__ void k_last_diag(int n, const void* A, int64_t lda, int fp32, double* d)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (fp32) d[n - 1] = (double)((const float2*)A)[(n - 1) + (int64_t)(n - 1) * lda].x;
+        else d[n - 1] = ((const double2*)A)[(n - 1) + (int64_t)(n - 1) * lda].x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host driver (runs on the context's stream; no host synchronisation).
+// ---------------------------------------------------------------------------
+struct HetrdWork {
+    double2* Vstore;     // packed panel reflectors
+    int64_t* voff;       // host array: offset of panel p in Vstore
+    double2* Wp;         // n x nb
+    double2* acol;       // n
+    double2* Yr;         // nbk^2 x 64
+    double2* Yc;         // nbk * nq * 64
+    double2* t12;        // 128
+    double* part;        // >= 2 * 64 * nblocks doubles
+    double* scal;        // 8
+    unsigned int* counter;
+};
+
+template <typename T>
+static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, double* d, double* e,
+                            double2* tau, const HetrdWork& w)
+{
+    hipError_t err;
+    if (n == 1) {
+        hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, (const void*)A, lda,
+                           (int)(sizeof(T) == sizeof(float2)), d);
+        return hipGetLastError();
+    }
+    int p = 0;
+    for (int j0 = 0; j0 < n - 1; j0 += nb, ++p) {
+        const int nbp = min(nb, n - 1 - j0);
+        const int m = n - j0;
+        HetrdCol h;
+        h.n = n; h.lda = lda; h.j0 = j0; h.m = m;
+        h.Vp = w.Vstore + w.voff[p];
+        h.Wp = w.Wp;
+        h.acol = w.acol; h.d = d; h.e = e; h.tau = tau; h.scal = w.scal; h.part = w.part;
+        h.counter = w.counter;
+        for (int i = 0; i < nbp; ++i) {
+            h.i = i;
+            const int L = m - i;
+            hipLaunchKernelGGL(k_colupd<T>, dim3((unsigned)ceil_div64(L, 256)), dim3(256), 0, st, h,
+                               (const T*)A);
+            hipLaunchKernelGGL(k_makev, dim3((unsigned)ceil_div64(m, 256)), dim3(256), 0, st, h,
+                               w.t12);
+            const int c = j0 + i;
+            const int nt = n - c - 1;
+            const int nbk = (int)ceil_div64(nt, 64);
+            const int nq = (int)ceil_div64(nbk, HEMV_CH);
+            const T* A22 = A + (c + 1) + (int64_t)(c + 1) * lda;
+            const double2* vtr = h.Vp + (i + 1) + (int64_t)i * m;
+            hipLaunchKernelGGL(k_hemv_lower<T>, dim3(nq, nbk), dim3(256), 0, st, nt, A22, lda, vtr,
+                               w.Yr, w.Yc, nbk, nq);
+            hipLaunchKernelGGL(k_hemv_reduce, dim3((unsigned)ceil_div64(nt, 256)), dim3(256), 0, st,
+                               h, nt, (const double2*)w.Yr, (const double2*)w.Yc, nbk, nq,
+                               (const double2*)w.t12);
+            hipLaunchKernelGGL(k_wfinal, dim3((unsigned)ceil_div64(m, 256)), dim3(256), 0, st, h);
+            if ((err = hipGetLastError()) != hipSuccess) return err;
+        }
+        // zero the unused reflector columns of a partial last panel
+        const int n2 = m - nbp;
+        if (n2 > 0) {
+            const int64_t nt2 = ceil_div64(n2, 64);
+            hipLaunchKernelGGL(k_her2k_lower<T>, dim3((unsigned)(nt2 * (nt2 + 1) / 2)), dim3(256), 0,
+                               st, n2, A + (j0 + nbp) + (int64_t)(j0 + nbp) * lda, lda,
+                               (const double2*)(h.Vp + nbp), (const double2*)(w.Wp + nbp),
+                               (int64_t)m, nbp);
+        }
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+    }
+    hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, (const void*)A, lda,
+                       (int)(sizeof(T) == sizeof(float2)), d);
+    return hipGetLastError();
+}
+
+// Workspace sizes (bytes) for a given n, nb.
+extern "C" void bse_hetrd_workspace(int n, int nb, int64_t* vstore_elems, int64_t* yr_elems,
+                                    int64_t* yc_elems, int64_t* part_doubles)
+{
+    int64_t vs = 0;
+    for (int j0 = 0; j0 < n - 1; j0 += nb) vs += (int64_t)(n - j0) * nb;
+    *vstore_elems = vs > 0 ? vs : 1;
+    const int64_t nbk = ceil_div64(n, 64);
+    const int64_t nq = ceil_div64(nbk, HEMV_CH);
+    *yr_elems = nbk * nbk * 64;
+    *yc_elems = nbk * nq * 64;
+    const int64_t nblk = ceil_div64(n, 256) + 1;
+    *part_doubles = 2 * 2 * 64 * nblk + 64;
+}
+
+extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int64_t lda, int nb,
+                                double* d, double* e, double2* tau, double2* Vstore,
+                                const int64_t* voff, double2* Wp, double2* acol, double2* Yr,
+                                double2* Yc, double2* t12, double* part, double* scal,
+                                unsigned int* counter)
+{
+    HetrdWork w;
+    w.Vstore = Vstore; w.voff = (int64_t*)voff; w.Wp = Wp; w.acol = acol; w.Yr = Yr; w.Yc = Yc;
+    w.t12 = t12; w.part = part; w.scal = scal; w.counter = counter;
+    if (fp32) return hetrd_run<float2>(st, n, (float2*)A, lda, nb, d, e, tau, w);
+    return hetrd_run<double2>(st, n, (double2*)A, lda, nb, d, e, tau, w);
+}

@@ -1,0 +1,157 @@
+// K3: interaction tables and per-transition factors.
+//
+// Replaces Excitonic_Hamiltonian::pull_coulomb_potentials
+// (bse.version13.cpp:2547-2728) and the pair-density construction of
+// Dipole_Elements::pull_values (1922-2158) with point-orbital M
+// (function_building_exponential_factor, 1459-1501).  The reference tabulates
+// W_GG'(k - k') for all Nk^2 k pairs; on a Monkhorst-Pack grid k - k' takes
+// only (2 n1 - 1)(2 n2 - 1) values, so the orbital-resolved table
+//     W_ij(Q) = sum_G W(|Q + G|) e^{i (Q + G).(tau_i - tau_j)}
+// is built once per distinct Q (the single Q + G = 0 term takes the mini-BZ
+// average).  The exchange needs V_ij = sum_{G != 0} v(|G|) e^{i G.(tau_i -
+// tau_j)} (G0 excluded as at 2811-2816).
+#include "bse_common.hpp"
+
+// Rytova-Keldysh W(Q) = (C/2) / (eps Q (1 + r0 Q)) and its parameter derivatives.
+__device__ inline double keldysh_w(double qn, double r0, double eps, double c_half, int mode) {
+    const double den = eps * qn * (1.0 + r0 * qn);
+    if (mode == 0) return c_half / den;
+    if (mode == 1) return -c_half / (eps * (1.0 + r0 * qn) * (1.0 + r0 * qn));  // d/dr0
+    return -c_half / (eps * den);                                             // d/deps
+}
+
+// Table entry (d1, d2, i, j); d = index - (n - 1).
+__global__ __launch_bounds__(256) void k_w_table(
+    int n1, int n2, int norb, int ng, const double* __restrict__ gvec /* 3 x ng */,
+    double b1x, double b1y, double b1z, double b2x, double b2y, double b2z,
+    const double* __restrict__ tau /* norb x 3 */, double r0, double eps, double c_half,
+    double wbar, int mode, double2* __restrict__ wtab)
+{
+    const int64_t total = (int64_t)(2 * n1 - 1) * (2 * n2 - 1) * norb * norb;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int j = (int)(t % norb);
+    const int i = (int)((t / norb) % norb);
+    const int64_t dd = t / ((int64_t)norb * norb);
+    const int d2i = (int)(dd % (2 * n2 - 1));
+    const int d1i = (int)(dd / (2 * n2 - 1));
+    const double f1 = (double)(d1i - (n1 - 1)) / n1, f2 = (double)(d2i - (n2 - 1)) / n2;
+    const double qx = f1 * b1x + f2 * b2x, qy = f1 * b1y + f2 * b2y, qz = f1 * b1z + f2 * b2z;
+    const double tx = tau[3 * i] - tau[3 * j], ty = tau[3 * i + 1] - tau[3 * j + 1],
+                 tz = tau[3 * i + 2] - tau[3 * j + 2];
+    double2 acc = cmk(0, 0);
+    for (int g = 0; g < ng; ++g) {
+        const double gx = qx + gvec[3 * g], gy = qy + gvec[3 * g + 1], gz = qz + gvec[3 * g + 2];
+        const double qn = sqrt(gx * gx + gy * gy + gz * gz);
+        const double w = (qn < 1e-12) ? wbar : keldysh_w(qn, r0, eps, c_half, mode);
+        double s, c;
+        sincos(gx * tx + gy * ty + gz * tz, &s, &c);
+        acc.x += w * c;
+        acc.y += w * s;
+    }
+    wtab[t] = acc;
+}
+
+// V_ij (exchange), bare 2D v(G) = (C/2)/|G|, G0 excluded.
+__global__ void k_v_exchange(int norb, int ng, const double* __restrict__ gvec,
+                             const double* __restrict__ tau, double c_half,
+                             double2* __restrict__ vx)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= norb * norb) return;
+    const int i = t / norb, j = t % norb;
+    const double tx = tau[3 * i] - tau[3 * j], ty = tau[3 * i + 1] - tau[3 * j + 1],
+                 tz = tau[3 * i + 2] - tau[3 * j + 2];
+    double2 acc = cmk(0, 0);
+    for (int g = 0; g < ng; ++g) {
+        if (g == ng / 2) continue;
+        const double gx = gvec[3 * g], gy = gvec[3 * g + 1], gz = gvec[3 * g + 2];
+        const double gn = sqrt(gx * gx + gy * gy + gz * gz);
+        const double v = c_half / gn;
+        double s, c;
+        sincos(gx * tx + gy * ty + gz * tz, &s, &c);
+        acc.x += v * c;
+        acc.y += v * s;
+    }
+    vx[t] = acc;
+}
+
+// Per-transition factors in the k-major internal order r' = k*B + c*nv + v:
+//   P[r', i]  = conj(c_{ck,i}) c_{v,k-q,i}            (exchange factor)
+//   PV[r', j] = sum_i P[r', i] V_ij
+//   rho[r']   = i sum_i conj(c_{ck,i}) (tau_i . qhat) c_{v,k-q,i}
+//   dE[r']    = eps_c(k) - eps_v(k - q)
+// With dcoef != nullptr the derivative factors (product rule) are formed
+// instead: dP = conj(dc_c) c_v + conj(c_c) dc_v, d rho likewise, d dE.
+__global__ __launch_bounds__(256) void k_transitions(
+    int nk, int nv, int nc, int norb, const double* __restrict__ eps,
+    const double2* __restrict__ coef, const double* __restrict__ deps,
+    const double2* __restrict__ dcoef, const double* __restrict__ tau, double qhx,
+    double qhy, double qhz, const double2* __restrict__ vx, double2* __restrict__ P,
+    double2* __restrict__ PV, double2* __restrict__ rho, double* __restrict__ dE)
+{
+    const int B = nc * nv, ns = nv + nc;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (int64_t)nk * B) return;
+    const int k = (int)(r / B), b = (int)(r % B);
+    const int c = b / nv, v = b % nv;
+    const double2* cc = coef + ((size_t)k * ns + nv + c) * norb;
+    const double2* cv = coef + ((size_t)k * ns + v) * norb;
+    double2 p[BSE_MAXORB];
+    double2 rh = cmk(0, 0);
+    for (int i = 0; i < norb; ++i) {
+        double2 pi;
+        if (dcoef == nullptr) {
+            pi = cmulc(cc[i], cv[i]);
+        } else {
+            const double2* dcc = dcoef + ((size_t)k * ns + nv + c) * norb;
+            const double2* dcv = dcoef + ((size_t)k * ns + v) * norb;
+            pi = cadd(cmulc(dcc[i], cv[i]), cmulc(cc[i], dcv[i]));
+        }
+        p[i] = pi;
+        P[r * norb + i] = pi;
+        const double x = tau[3 * i] * qhx + tau[3 * i + 1] * qhy + tau[3 * i + 2] * qhz;
+        rh = cadd(rh, cscale(pi, x));
+    }
+    rho[r] = cmk(-rh.y, rh.x);   // i * rh
+    for (int j = 0; j < norb; ++j) {
+        double2 acc = cmk(0, 0);
+        for (int i = 0; i < norb; ++i) cfma(acc, p[i], vx[i * norb + j]);
+        PV[r * norb + j] = acc;
+    }
+    const double* e = (dcoef == nullptr) ? eps : deps;
+    dE[r] = e[(size_t)k * ns + nv + c] - e[(size_t)k * ns + v];
+}
+
+extern "C" hipError_t bse_launch_w_table(hipStream_t st, int n1, int n2, int norb, int ng,
+                                         const double* gvec, const double* b, const double* tau,
+                                         double r0, double eps, double c_half, double wbar,
+                                         int mode, double2* wtab)
+{
+    const int64_t total = (int64_t)(2 * n1 - 1) * (2 * n2 - 1) * norb * norb;
+    hipLaunchKernelGGL(k_w_table, dim3((unsigned)ceil_div64(total, 256)), dim3(256), 0, st, n1,
+                       n2, norb, ng, gvec, b[0], b[1], b[2], b[3], b[4], b[5], tau, r0, eps,
+                       c_half, wbar, mode, wtab);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_launch_v_exchange(hipStream_t st, int norb, int ng, const double* gvec,
+                                            const double* tau, double c_half, double2* vx)
+{
+    hipLaunchKernelGGL(k_v_exchange, dim3(1), dim3(256), 0, st, norb, ng, gvec, tau, c_half, vx);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_launch_transitions(hipStream_t st, int nk, int nv, int nc, int norb,
+                                             const double* eps, const double2* coef,
+                                             const double* deps, const double2* dcoef,
+                                             const double* tau, const double* qhat,
+                                             const double2* vx, double2* P, double2* PV,
+                                             double2* rho, double* dE)
+{
+    const int64_t n = (int64_t)nk * nv * nc;
+    hipLaunchKernelGGL(k_transitions, dim3((unsigned)ceil_div64(n, 256)), dim3(256), 0, st, nk,
+                       nv, nc, norb, eps, coef, deps, dcoef, tau, qhat[0], qhat[1], qhat[2], vx,
+                       P, PV, rho, dE);
+    return hipGetLastError();
+}

@@ -1,0 +1,320 @@
+// K6: eigenpairs of the real symmetric tridiagonal T (diagonal d, off-diagonal
+// e) produced by K5.
+//
+// Eigenvalues: Sturm-count bisection (Golub & Van Loan, sec. 8.4.1) run as an
+// 8-way multisection -- 8 lanes of a wavefront evaluate the count at 8
+// interior points of the current interval, so each round narrows it 9-fold.
+// The matrix is first split at negligible off-diagonals
+// (e_i^2 <= eps^2 |d_i d_{i+1}| + safmin) and each unreduced block is solved
+// on its own, so exactly degenerate eigenvalues of the BSE matrix (which show
+// up as splits) get orthogonal eigenvectors by construction.
+//
+// Eigenvectors: inverse iteration (Wilkinson; G&VL sec. 8.2.2) with the
+// tridiagonal LU factorisation with partial pivoting, three solves from a
+// deterministic pseudo-random start, one thread per eigenvalue; the
+// factorisation and iterate are interleaved across threads so that every
+// step is a coalesced load.  Eigenvalues closer than a cluster tolerance are
+// re-orthogonalised by modified Gram-Schmidt (twice) in a second kernel.
+#include "bse_common.hpp"
+#include <cfloat>
+
+// Count of eigenvalues of block [bs, bs+nbk) strictly below x.
+__device__ inline int sturm_count(const double* __restrict__ d, const double* __restrict__ e2,
+                                  int bs, int nbk, double x, double pivmin)
+{
+    double q = d[bs] - x;
+    if (fabs(q) <= pivmin) q = -pivmin;
+    int cnt = q < 0.0;
+    for (int i = 1; i < nbk; ++i) {
+        q = (d[bs + i] - x) - e2[bs + i - 1] / q;
+        if (fabs(q) <= pivmin) q = -pivmin;
+        cnt += q < 0.0;
+    }
+    return cnt;
+}
+
+// items: int4 (block start, block size, local index, block id); bounds per
+// block id: (lo, hi).  One 8-lane group per item.
+__global__ __launch_bounds__(256) void k_bisect(int nitems, const int4* __restrict__ items,
+                                               const double2* __restrict__ bounds,
+                                               const double* __restrict__ d,
+                                               const double* __restrict__ e2, double pivmin,
+                                               double tnorm, double* __restrict__ lam)
+{
+    const int gid = (blockIdx.x * blockDim.x + threadIdx.x);
+    const int item = gid >> 3, g = gid & 7;
+    const int lane = threadIdx.x & 63;
+    const int base = lane & ~7;
+    const bool active = item < nitems;
+    int4 it = make_int4(0, 1, 0, 0);
+    double lo = 0.0, hi = 0.0;
+    if (active) {
+        it = items[item];
+        const double2 b = bounds[it.w];
+        lo = b.x;
+        hi = b.y;
+    }
+    if (active && it.y == 1) {
+        if (g == 0) lam[item] = d[it.x];
+    }
+    const bool work = active && it.y > 1;
+    const double eps = DBL_EPSILON;
+    for (int round = 0; round < 40; ++round) {
+        const double tol = 2.0 * eps * fmax(fabs(lo), fabs(hi)) + eps * tnorm;
+        const bool conv = !work || (hi - lo) <= tol;
+        // group-uniform; the whole wave keeps looping until every group converged
+        const unsigned long long notdone = __ballot(!conv);
+        if (notdone == 0ull) break;
+        const double x = lo + (double)(g + 1) * (hi - lo) / 9.0;
+        int cnt = 0;
+        if (!conv) cnt = sturm_count(d, e2, it.x, it.y, x, pivmin);
+        const unsigned long long below = __ballot(!conv && cnt <= it.z);
+        const int nlo = __popcll((below >> base) & 0xffull);
+        const double xlo = __shfl(x, base + max(nlo - 1, 0), 64);
+        const double xhi = __shfl(x, base + min(nlo, 7), 64);
+        if (!conv) {
+            if (nlo > 0) lo = xlo;
+            if (nlo < 8) hi = xhi;
+        }
+    }
+    if (work && g == 0) lam[item] = 0.5 * (lo + hi);
+}
+
+__device__ inline double hash_uniform(uint64_t a, uint64_t b) {
+    uint64_t z = a * 0x9E3779B97F4A7C15ull + b * 0xD1B54A32D192ED03ull + 0x94D049BB133111EBull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// Inverse iteration.  Thread t of the batch owns sorted eigenvalue m0 + t in
+// block (bs, nb).  Scratch arrays are indexed [global row * bsz + t].
+__global__ __launch_bounds__(128) void k_inverse_iter(
+    int bsz, int cnt, int m0, const double* __restrict__ lam, const int* __restrict__ bsv,
+    const int* __restrict__ nbv, const double* __restrict__ d, const double* __restrict__ e,
+    double tnorm, double* __restrict__ u0, double* __restrict__ u1, double* __restrict__ u2,
+    double* __restrict__ mul, double* __restrict__ pf, double* __restrict__ x)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const double lm = lam[m0 + t];
+    const int bs = bsv[m0 + t], nb = nbv[m0 + t];
+    const int64_t S = bsz;
+    const double tiny = DBL_EPSILON * tnorm + DBL_MIN;
+    if (nb == 1) {
+        x[(int64_t)bs * S + t] = 1.0;
+        return;
+    }
+    // LU with partial pivoting of T_block - lam I.
+    double cur_d = d[bs] - lm;
+    double cur_s = e[bs];
+    for (int k = 0; k < nb - 1; ++k) {
+        const int64_t gi = bs + k;
+        const double sub = e[gi];
+        const double nd = d[gi + 1] - lm;
+        const double ns = (k + 1 < nb - 1) ? e[gi + 1] : 0.0;
+        double a0, a1, a2, m, fl;
+        if (fabs(cur_d) >= fabs(sub)) {
+            fl = 0.0;
+            m = (cur_d != 0.0) ? sub / cur_d : 0.0;
+            a0 = cur_d; a1 = cur_s; a2 = 0.0;
+            cur_d = nd - m * cur_s;
+            cur_s = ns;
+        } else {
+            fl = 1.0;
+            m = cur_d / sub;
+            a0 = sub; a1 = nd; a2 = ns;
+            cur_d = cur_s - m * nd;
+            cur_s = -m * ns;
+        }
+        if (fabs(a0) < tiny) a0 = (a0 < 0.0) ? -tiny : tiny;
+        u0[gi * S + t] = a0;
+        u1[gi * S + t] = a1;
+        u2[gi * S + t] = a2;
+        mul[gi * S + t] = m;
+        pf[gi * S + t] = fl;
+    }
+    {
+        double a0 = cur_d;
+        if (fabs(a0) < tiny) a0 = (a0 < 0.0) ? -tiny : tiny;
+        u0[(int64_t)(bs + nb - 1) * S + t] = a0;
+    }
+    // Random start.
+    for (int k = 0; k < nb; ++k) x[(int64_t)(bs + k) * S + t] = hash_uniform(m0 + t, k);
+    for (int it = 0; it < 3; ++it) {
+        // forward: L^-1 with the recorded row swaps
+        double cur = x[(int64_t)bs * S + t];
+        for (int k = 0; k < nb - 1; ++k) {
+            const int64_t gi = bs + k;
+            double nxt = x[(gi + 1) * S + t];
+            if (pf[gi * S + t] != 0.0) { const double tmp = cur; cur = nxt; nxt = tmp; }
+            nxt -= mul[gi * S + t] * cur;
+            x[gi * S + t] = cur;
+            cur = nxt;
+        }
+        // back substitution with U (diag u0, super u1, second super u2)
+        const int64_t gl = bs + nb - 1;
+        double x1 = cur / u0[gl * S + t];
+        x[gl * S + t] = x1;
+        double x2 = 0.0, nrm = x1 * x1;
+        for (int k = nb - 2; k >= 0; --k) {
+            const int64_t gi = bs + k;
+            const double xk = (x[gi * S + t] - u1[gi * S + t] * x1 - u2[gi * S + t] * x2) / u0[gi * S + t];
+            x[gi * S + t] = xk;
+            nrm += xk * xk;
+            x2 = x1;
+            x1 = xk;
+        }
+        // normalise; final pass also fixes the sign (largest component positive)
+        const double inv = 1.0 / sqrt(nrm);
+        double best = -1.0, sgn = 1.0;
+        for (int k = 0; k < nb; ++k) {
+            const int64_t gi = bs + k;
+            const double v = x[gi * S + t] * inv;
+            x[gi * S + t] = v;
+            if (fabs(v) > best) { best = fabs(v); sgn = (v < 0.0) ? -1.0 : 1.0; }
+        }
+        if (it == 2 && sgn < 0.0)
+            for (int k = 0; k < nb; ++k) x[(int64_t)(bs + k) * S + t] *= -1.0;
+    }
+}
+
+// Modified Gram-Schmidt (two passes) inside clusters of close eigenvalues.
+// clusters: int2 (first batch index, size); one workgroup per cluster.  All
+// members of a cluster share the block (bs, nb).
+__global__ __launch_bounds__(256) void k_cluster_mgs(int bsz, const int2* __restrict__ clusters,
+                                                     int m0, const int* __restrict__ bsv,
+                                                     const int* __restrict__ nbv,
+                                                     double* __restrict__ x)
+{
+    __shared__ double sh[32];
+    const int2 cl = clusters[blockIdx.x];
+    const int t0 = cl.x, csz = cl.y;
+    const int bs = bsv[m0 + t0], nb = nbv[m0 + t0];
+    const int64_t S = bsz;
+    for (int j = 0; j < csz; ++j) {
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int i = 0; i < j; ++i) {
+                double dot = 0.0;
+                for (int k = threadIdx.x; k < nb; k += blockDim.x)
+                    dot += x[(int64_t)(bs + k) * S + t0 + i] * x[(int64_t)(bs + k) * S + t0 + j];
+                dot = block_sum(dot, sh);
+                for (int k = threadIdx.x; k < nb; k += blockDim.x)
+                    x[(int64_t)(bs + k) * S + t0 + j] -= dot * x[(int64_t)(bs + k) * S + t0 + i];
+                __syncthreads();
+            }
+        }
+        double nrm = 0.0;
+        for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+            const double v = x[(int64_t)(bs + k) * S + t0 + j];
+            nrm += v * v;
+        }
+        nrm = block_sum(nrm, sh);
+        const double inv = 1.0 / sqrt(nrm);
+        for (int k = threadIdx.x; k < nb; k += blockDim.x) x[(int64_t)(bs + k) * S + t0 + j] *= inv;
+        __syncthreads();
+    }
+}
+
+// Commit the batch: Z[:, col0 + t] = x_t (complex, imaginary part zero) for
+// t < nstore, via a 64x64 LDS transpose; rows outside the vector's block are
+// zero.
+__global__ __launch_bounds__(256) void k_commit(int n, int bsz, int cnt, int nstore, int m0,
+                                                const int* __restrict__ bsv,
+                                                const int* __restrict__ nbv,
+                                                const double* __restrict__ x,
+                                                double2* __restrict__ Z, int64_t ldz, int col0)
+{
+    __shared__ double tile[64][65];
+    const int r0 = blockIdx.x * 64, t0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int rr = ty; rr < 64; rr += 4) {
+        const int row = r0 + rr, t = t0 + tx;
+        double v = 0.0;
+        if (row < n && t < cnt) {
+            const int bs = bsv[m0 + t], nb = nbv[m0 + t];
+            if (row >= bs && row < bs + nb) v = x[(int64_t)row * bsz + t];
+        }
+        tile[rr][tx] = v;
+    }
+    __syncthreads();
+    for (int tt = ty; tt < 64; tt += 4) {
+        const int t = t0 + tt, row = r0 + tx;
+        if (t < nstore && row < n) Z[row + (int64_t)(col0 + t) * ldz] = cmk(tile[tx][tt], 0.0);
+    }
+}
+
+// Contract the batch with a complex vector: f[m0 + t] = sum_rows x_t[row] conj(y[row]).
+// With y = Q^H rho this is f_m = sum_r conj(rho_r) (Q x_m)_r, the oscillator
+// strength of the back-transformed eigenvector, without forming it.
+__global__ __launch_bounds__(256) void k_contract(int bsz, int cnt, int m0,
+                                                  const int* __restrict__ bsv,
+                                                  const int* __restrict__ nbv,
+                                                  const double* __restrict__ x,
+                                                  const double2* __restrict__ yc,
+                                                  double2* __restrict__ f)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int bs = bsv[m0 + t], nb = nbv[m0 + t];
+    double2 acc = cmk(0, 0);
+    for (int k = 0; k < nb; ++k) {
+        const double v = x[(int64_t)(bs + k) * bsz + t];
+        const double2 y = yc[bs + k];
+        acc.x += v * y.x;
+        acc.y -= v * y.y;
+    }
+    f[m0 + t] = acc;
+}
+
+extern "C" hipError_t bse_launch_bisect(hipStream_t st, int nitems, const int4* items,
+                                        const double2* bounds, const double* d, const double* e2,
+                                        double pivmin, double tnorm, double* lam)
+{
+    const int64_t threads = (int64_t)nitems * 8;
+    hipLaunchKernelGGL(k_bisect, dim3((unsigned)ceil_div64(threads, 256)), dim3(256), 0, st,
+                       nitems, items, bounds, d, e2, pivmin, tnorm, lam);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_launch_inverse_iter(hipStream_t st, int bsz, int cnt, int m0,
+                                              const double* lam, const int* bsv, const int* nbv,
+                                              const double* d, const double* e, double tnorm,
+                                              double* scratch /* 5 * n * bsz */, int n,
+                                              double* x)
+{
+    const int64_t A = (int64_t)n * bsz;
+    hipLaunchKernelGGL(k_inverse_iter, dim3((unsigned)ceil_div64(cnt, 128)), dim3(128), 0, st, bsz,
+                       cnt, m0, lam, bsv, nbv, d, e, tnorm, scratch, scratch + A, scratch + 2 * A,
+                       scratch + 3 * A, scratch + 4 * A, x);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_launch_cluster_mgs(hipStream_t st, int bsz, int ncl, const int2* clusters,
+                                             int m0, const int* bsv, const int* nbv, double* x)
+{
+    if (ncl <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cluster_mgs, dim3(ncl), dim3(256), 0, st, bsz, clusters, m0, bsv, nbv, x);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_launch_commit(hipStream_t st, int n, int bsz, int cnt, int nstore, int m0,
+                                        const int* bsv, const int* nbv, const double* x,
+                                        double2* Z, int64_t ldz, int col0)
+{
+    if (nstore <= 0) return hipSuccess;
+    dim3 grid((unsigned)ceil_div64(n, 64), (unsigned)ceil_div64(nstore, 64));
+    hipLaunchKernelGGL(k_commit, grid, dim3(256), 0, st, n, bsz, cnt, nstore, m0, bsv, nbv, x, Z,
+                       ldz, col0);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_launch_contract(hipStream_t st, int bsz, int cnt, int m0, const int* bsv,
+                                          const int* nbv, const double* x, const double2* yc,
+                                          double2* f)
+{
+    hipLaunchKernelGGL(k_contract, dim3((unsigned)ceil_div64(cnt, 256)), dim3(256), 0, st, bsz, cnt,
+                       m0, bsv, nbv, x, yc, f);
+    return hipGetLastError();
+}

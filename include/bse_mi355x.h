@@ -1,0 +1,158 @@
+/*
+ * bse_mi355x.h -- C ABI of the MI355X excitonic hot path.
+ *
+ * Drop-in boundary for the build-and-eig calls of the tight-binding
+ * Bethe-Salpeter solver (reference: bse.version13.cpp).  Every entry point
+ * names the reference member function it replaces.  The reference seams are
+ * C++ member functions that take and return Armadillo objects by value and
+ * have no error returns; here every call returns an int status (0 = ok,
+ * negative = error class) and buffers are caller-owned plain pointers.
+ *
+ * Conventions
+ *   - complex numbers are interleaved (re, im) doubles or floats, layout
+ *     compatible with std::complex / arma::cx_double;
+ *   - matrices are column-major with an explicit leading dimension;
+ *   - BSE basis index r = c*nv*Nk + v*Nk + k (k fastest), as the reference's
+ *     energies_diff / rho layout (bse.version13.cpp:2004, 2022) and row index
+ *     of the direct term (2876);
+ *   - valence slot v = 0 is the highest valence band (812-813), conduction
+ *     slot c = 0 the lowest conduction band (815-816); valence states are
+ *     taken at k - q (1966, 2836);
+ *   - eigenvalues ascending, eigenvectors of unit 2-norm (3334-3345);
+ *   - textbook (Rohlfing-Louie) orientation of the direct term; the
+ *     reference stores its complex conjugate (Armadillo .t() at 2878), which
+ *     has the same eigenvalues and conjugated eigenvectors;
+ *   - energies in eV, lengths in Angstrom, k and G in 1/Angstrom;
+ *   - one context per host thread; calls on a context are serialised on the
+ *     context's HIP stream.
+ */
+#ifndef BSE_MI355X_H
+#define BSE_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSE_ABI_VERSION 1
+
+/* status codes */
+#define BSE_OK               0
+#define BSE_ERR_ARG         -1   /* bad argument, size or null pointer        */
+#define BSE_ERR_HIP         -2   /* HIP runtime failure                       */
+#define BSE_ERR_ALLOC       -3   /* device or host allocation failed          */
+#define BSE_ERR_STATE       -4   /* call made out of order                    */
+#define BSE_ERR_NUMERIC     -5   /* non-finite input or failed convergence    */
+#define BSE_ERR_UNSUPPORTED -6   /* shape or option not supported             */
+
+/* precision of the assembled matrix and of the eigensolver path */
+#define BSE_PREC_FP64   0  /* FP64 assembly, FP64 storage, FP64 reduction     */
+#define BSE_PREC_FP32   1  /* FP32 MFMA assembly, FP32 storage; FP64 reflectors */
+#define BSE_PREC_BF16X2 2  /* BF16 hi+lo split MFMA products, FP32 accumulation
+                              and storage; FP64 reflectors                     */
+
+/* matrix order for copies */
+#define BSE_ORDER_REFERENCE 0  /* r = c*nv*Nk + v*Nk + k                       */
+#define BSE_ORDER_KMAJOR    1  /* internal r' = k*(nc*nv) + c*nv + v           */
+
+typedef struct bse_ctx bse_ctx;
+
+/* Tight-binding model: H_lm(k) = sum_n H_lm(R_n) exp(i k.R_n), the reference's
+ * Hamiltonian_TB::FFT (bse.version13.cpp:647-705) with ndegen weights already
+ * applied (divided, the Wannier90 meaning). */
+typedef struct {
+    int32_t norb;          /* orbitals per cell incl. spin components (<= 16) */
+    int32_t nr;            /* number of lattice vectors                       */
+    const double* r_cart;  /* nr x 3 row-major Cartesian R, Angstrom          */
+    const double* h_r;     /* complex nr x norb x norb, C order [n][l][m]     */
+    const double* tau;     /* norb x 3 row-major orbital centres, Angstrom    */
+} bse_tb_model;
+
+/* One excitonic problem (q = 0 TDA). */
+typedef struct {
+    double bravais[9];     /* column-major 3x3, column i = a_i (93-96)        */
+    int32_t n1, n2;        /* k grid, k = ((i+s1)/n1) b1 + ((j+s2)/n2) b2,
+                              k index = i*n2 + j                              */
+    double shift[2];
+    int32_t n_occ;         /* occupied bands; conduction starts at n_occ       */
+    int32_t nv, nc;        /* valence / conduction bands kept                 */
+    double scissor;        /* eV added to conduction energies (816)           */
+    double q[3];           /* optical momentum; valence states at k - q       */
+    double r0, eps_env;    /* Rytova-Keldysh screening                        */
+    double g_cut;          /* 2D G-list cutoff (319-385, parallelepiped)      */
+    double f_s;            /* exchange spin factor, 2 - spinorial (2801)      */
+    double exchange_scale; /* multiplies the exchange term                    */
+    double direct_scale;   /* multiplies the screened direct term            */
+    double dipole_dir[3];  /* direction qhat of the optical dipole (3562)     */
+} bse_system;
+
+/* ---- context ------------------------------------------------------------- */
+int  bse_create(bse_ctx** ctx, int device);
+int  bse_destroy(bse_ctx* ctx);
+const char* bse_last_error(const bse_ctx* ctx);
+int  bse_abi_version(void);
+
+/* Upload a problem (model + parameters); computes nothing.  Replaces the
+ * object construction of main (3686-3843): Crystal_Lattice, K_points,
+ * G_points, Hamiltonian_TB, Coulomb_Potential. */
+int  bse_set_problem(bse_ctx* ctx, const bse_tb_model* model, const bse_system* sys);
+
+/* Sizes of the loaded problem: out[0]=Nk, out[1]=N, out[2]=norb, out[3]=Ng. */
+int  bse_problem_sizes(const bse_ctx* ctx, int64_t out[4]);
+
+/* K1+K2: per-k H(k) and FP64 Hermitian eig for all grid k (conduction) and
+ * k - q (valence).  Replaces Hamiltonian_TB::pull_ks_states_subset
+ * (bse.version13.cpp:778-821) called for every k.
+ *   eps  : Nk x (nv+nc) doubles, [k][slot], slots = valence(top-down) then
+ *          conduction(bottom-up, + scissor); may be NULL
+ *   coef : complex Nk x (nv+nc) x norb, [k][slot][orb]; may be NULL        */
+int  bse_tb_states(bse_ctx* ctx, double* eps, double* coef);
+
+/* K3+K4: assemble the TDA matrix on the device in the given precision.
+ * Replaces Excitonic_Hamiltonian::extract_hbse_and_rcv (3041-3047), i.e.
+ * pull_coulomb_potentials (2547) + pull_resonant_part_and_rcv (2767). */
+int  bse_build(bse_ctx* ctx, int precision);
+
+/* Copy the assembled matrix out.  dst: complex N x N (ld >= N), doubles for
+ * BSE_PREC_FP64 and floats otherwise; both triangles are written.           */
+int  bse_get_matrix(bse_ctx* ctx, void* dst, int64_t ld, int dst_on_device, int order);
+
+/* rho_r(G0) of the optical dipole (the rcv half of extract_hbse_and_rcv),
+ * complex N, reference order. */
+int  bse_get_dipoles(bse_ctx* ctx, double* rho);
+
+/* K5-K8: eigendecomposition of the assembled matrix.  Replaces
+ * Excitonic_Hamiltonian::common_diagonalization (3120-3368; eig_sym -> zheevd
+ * at 3330).  All N eigenvalues are computed.
+ *   n_vectors : eigenvectors formed for the lowest n_vectors (-1 = all)
+ *   refine    : 1 = FP64 Rayleigh-Ritz refinement of the formed vectors
+ *               against the FP64 matrix recomputed on the fly           */
+int  bse_eigh(bse_ctx* ctx, int n_vectors, int refine);
+int  bse_get_eigenvalues(bse_ctx* ctx, double* evals);
+/* complex N x nvec (ld >= N), reference order, lowest nvec columns. */
+int  bse_get_eigenvectors(bse_ctx* ctx, double* z, int64_t ld, int nvec, int dst_on_device);
+
+/* K9: oscillator strengths f_m = sum_r conj(rho_r) A_rm for all m (3486-3502)
+ * and the TDA dielectric function (3621-3634):
+ *   eps(w) = 1 - C/(Omega Nk) sum_m |f_m|^2 / (w - E_m + i eta).           */
+int  bse_oscillator(bse_ctx* ctx, double* f);
+int  bse_spectrum(bse_ctx* ctx, const double* omega, int n_omega, double eta,
+                  double* eps_re, double* eps_im);
+
+/* Dense Hermitian eig of any matrix (drop-in for arma::eig_sym, zheevd):
+ * a: complex n x n (lower triangle read), doubles, host or device.
+ * evals: n ascending; z: complex n x n_vectors (may be NULL).             */
+int  bse_eigh_dense(bse_ctx* ctx, int n, const double* a, int64_t lda, int a_on_device,
+                    int precision, int n_vectors, double* evals, double* z, int64_t ldz);
+
+/* Stage timings of the last build/eigh (milliseconds, HIP events):
+ * [0] tb, [1] tables, [2] assembly, [3] tridiagonalisation, [4] tridiagonal
+ * eigenvalues, [5] tridiagonal vectors, [6] back-transform, [7] refinement,
+ * [8] oscillator+spectrum.  n <= 16. */
+int  bse_stage_times(const bse_ctx* ctx, double* ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BSE_MI355X_H */

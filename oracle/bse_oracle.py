@@ -1,0 +1,510 @@
+"""CPU FP64 oracle for the excitonic hot path -- TEST INFRASTRUCTURE ONLY.
+
+This module is the checker: it is imported by ``tests/``, by the smoke check
+and by the CPU-baseline leg of the benchmark, never by the product path.  It
+restates, in numpy complex128, the reference's algorithm (bse.version13.cpp)
+with point-orbital pair densities:
+
+  geometry ........ Crystal_Lattice 85-124, K_points 183-242, G_points 319-385
+  TB states ....... FFT 647-705, pull_ks_states 706-777, _subset 778-821
+                    (numpy.linalg.eigh = LAPACK zheevd stands in for
+                    Armadillo eig_sym; ascending, unit columns)
+  point-orbital M . function_building_exponential_factor 1459-1501
+  W / v tables .... pull_coulomb_potentials 2547-2728 (model W, q=0 average)
+  assembly ........ pull_resonant_part_and_rcv 2767-2928 (textbook orientation,
+                    Hermitian exchange; see Q1/Q2/Q23 of the defect register)
+  eig ............. common_diagonalization 3318-3345 (zheevd, sort, normalise)
+  oscillator ...... pull_excitonic_oscillator_force 3486-3502
+  spectrum ........ pull_macroscopic_bse_dielectric_function 3525-3657
+
+Index convention of the BSE basis: r = c*nv*Nk + v*Nk + k (k fastest), as in
+the energies_diff / rho layout (2004, 2022) and the row index of 2876.
+
+Pins (see tests/test_oracle_*.py): closed-form honeycomb bands, IPA limit,
+Hermiticity, exchange invariance under a constant V shift, gauge invariance of
+|f_m|^2, trace identity, sum rule, brute-force agreement with the reference's
+loop structure (``bse_bruteforce``), mpmath-free small-N exact cases, finite
+differences for the Hellmann-Feynman gradients.  No golden vectors exist in the
+reference (it ships no tests or data), so nothing else can pin it.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+import numpy as np
+
+C_EV_A = (1.602176634 * 1000.0) / 8.8541878176   # e^2/eps0 in eV*A (line 32)
+MINVAL = 1.0e-5                                   # line 28
+
+
+# ----------------------------------------------------------------------------
+# Geometry (restated; lines 85-124, 183-242, 319-385)
+# ----------------------------------------------------------------------------
+
+def reciprocal(bravais):
+    a = np.asarray(bravais, dtype=np.float64)
+    vol = abs(np.linalg.det(a))
+    b = np.stack([np.cross(a[:, 1], a[:, 2]), np.cross(a[:, 2], a[:, 0]),
+                  np.cross(a[:, 0], a[:, 1])], axis=1)
+    return 2.0 * np.pi / vol * b
+
+
+def kgrid(bravais, n1, n2, shift=(0.0, 0.0)):
+    """Cartesian k (3 x nk), k index = i*n2 + j."""
+    b = reciprocal(bravais)
+    i, j = np.meshgrid(np.arange(n1), np.arange(n2), indexing="ij")
+    c = np.stack([(i.ravel() + shift[0]) / n1, (j.ravel() + shift[1]) / n2,
+                  np.zeros(n1 * n2)])
+    return b @ c
+
+
+def glist2d(bravais, g_cut):
+    """Parallelepiped 2D G-list, G = 0 at index Ng//2 (intended form of 352-379)."""
+    b = reciprocal(bravais)
+    n = [int(g_cut / np.linalg.norm(b[:, d])) for d in range(2)]
+    out = []
+    for i in range(-n[0], n[0] + 1):
+        for j in range(-n[1], n[1] + 1):
+            out.append(i * b[:, 0] + j * b[:, 1])
+    return np.array(out).T
+
+
+# ----------------------------------------------------------------------------
+# Inputs in plain arrays
+# ----------------------------------------------------------------------------
+
+@dataclass
+class Problem:
+    bravais: np.ndarray      # 3x3 columns a_i
+    r_cart: np.ndarray       # (nr, 3)
+    h_r: np.ndarray          # (nr, norb, norb)
+    tau: np.ndarray          # (norb, 3)
+    n1: int
+    n2: int
+    n_occ: int
+    nv: int
+    nc: int
+    r0: float
+    eps_env: float
+    g_cut: float
+    scissor: float = 0.0
+    f_s: float = 2.0
+    exchange_scale: float = 1.0
+    direct_scale: float = 1.0
+    q: tuple = (MINVAL, 0.0, 0.0)
+    eta: float = 0.05
+    omegas: np.ndarray = None
+
+    @property
+    def nk(self):
+        return self.n1 * self.n2
+
+    @property
+    def norb(self):
+        return self.h_r.shape[1]
+
+    @property
+    def n_bse(self):
+        return self.nk * self.nv * self.nc
+
+    @property
+    def area(self):
+        a = np.asarray(self.bravais)
+        return float(np.linalg.norm(np.cross(a[:, 0], a[:, 1])))
+
+    @property
+    def volume(self):
+        return float(abs(np.linalg.det(np.asarray(self.bravais))))
+
+
+def problem_from_system(s) -> Problem:
+    """Read plain arrays off a host-side system description (attributes only)."""
+    m = s.model
+    return Problem(bravais=np.array(m.lattice.bravais), r_cart=np.array(m.r_cart),
+                   h_r=np.array(m.h_r), tau=np.array(m.tau), n1=s.grid.n1, n2=s.grid.n2,
+                   n_occ=s.n_occ, nv=s.nv, nc=s.nc, r0=s.r0, eps_env=s.eps_env,
+                   g_cut=s.g_cut, scissor=s.scissor, f_s=s.spin_factor,
+                   exchange_scale=s.exchange_scale, direct_scale=s.direct_scale,
+                   q=tuple(s.q_opt), eta=s.eta, omegas=s.omegas())
+
+
+# ----------------------------------------------------------------------------
+# Single-particle states (FFT 647-705; pull_ks_states 706-777; _subset 778-821)
+# ----------------------------------------------------------------------------
+
+def fix_gauge(vecs):
+    """Make the largest-modulus component of each column real and positive.
+
+    The reference keeps LAPACK's arbitrary phases; every quantity compared is
+    gauge invariant, this only makes outputs deterministic.  vecs: (..., n, m).
+    """
+    idx = np.argmax(np.abs(vecs), axis=-2)
+    piv = np.take_along_axis(vecs, idx[..., None, :], axis=-2)
+    ph = piv / np.abs(piv)
+    return vecs / ph
+
+
+def hk(p: Problem, kcart):
+    ph = np.exp(1j * (p.r_cart @ kcart))             # (nr, nk)
+    return np.einsum("rk,rlm->klm", ph, p.h_r)
+
+
+def tb_states(p: Problem, kcart):
+    """Ascending eigenpairs of H(k) (zheevd), unit columns, fixed gauge."""
+    e, v = np.linalg.eigh(hk(p, kcart))
+    return e, fix_gauge(v)
+
+
+def band_window(p: Problem, kcart=None):
+    """Conduction at k and valence at k - q (1964-1966, 2836).
+
+    Returns eps_c (nk, nc) incl. scissor, C (nk, nc, norb), eps_v (nk, nv),
+    V (nk, nv, norb); v = 0 is the highest valence band (812-813), c = 0 the
+    lowest conduction band (815-816).
+    """
+    if kcart is None:
+        kcart = kgrid(p.bravais, p.n1, p.n2)
+    q = np.asarray(p.q, dtype=np.float64)[:, None]
+    ek, vk = tb_states(p, kcart)
+    eq, vq = tb_states(p, kcart - q)
+    cidx = p.n_occ + np.arange(p.nc)
+    vidx = p.n_occ - 1 - np.arange(p.nv)
+    eps_c = ek[:, cidx] + p.scissor
+    C = np.transpose(vk[:, :, cidx], (0, 2, 1))
+    eps_v = eq[:, vidx]
+    V = np.transpose(vq[:, :, vidx], (0, 2, 1))
+    return eps_c, C, eps_v, V
+
+
+# ----------------------------------------------------------------------------
+# Potentials and tables
+# ----------------------------------------------------------------------------
+
+def keldysh(qn, r0, eps_env):
+    """Rytova-Keldysh W(Q) = (C/2)/(eps Q (1 + r0 Q)), eV*A^2."""
+    return 0.5 * C_EV_A / (eps_env * qn * (1.0 + r0 * qn))
+
+
+def keldysh_q0_average(area, nk, r0, eps_env):
+    """Mean of W over the disk pi R^2 = (2 pi)^2/(A Nk) around Q = 0."""
+    da = (2.0 * np.pi) ** 2 / (area * nk)
+    rad = np.sqrt(da / np.pi)
+    if r0 == 0.0:
+        return (np.pi * C_EV_A / eps_env) * rad / da
+    return (np.pi * C_EV_A / eps_env) * np.log1p(r0 * rad) / (r0 * da)
+
+
+def bare2d(gn):
+    return 0.5 * C_EV_A / gn
+
+
+def w_table(p: Problem, r0=None, eps_env=None):
+    """W_ij(Q) on every grid difference Q = (d1/n1) b1 + (d2/n2) b2.
+
+    Returns (2n1-1, 2n2-1, norb, norb) indexed [d1 + n1-1, d2 + n2-1].
+    W_ij(Q) = sum_G W(|Q+G|) e^{i(Q+G).(tau_i - tau_j)}; the single term with
+    Q + G = 0 takes the mini-BZ average.
+    """
+    r0 = p.r0 if r0 is None else r0
+    eps_env = p.eps_env if eps_env is None else eps_env
+    b = reciprocal(p.bravais)
+    g = glist2d(p.bravais, p.g_cut)
+    d1 = np.arange(-(p.n1 - 1), p.n1)
+    d2 = np.arange(-(p.n2 - 1), p.n2)
+    dt = p.tau[:, None, :] - p.tau[None, :, :]          # (norb, norb, 3)
+    wbar = keldysh_q0_average(p.area, p.nk, r0, eps_env)
+    out = np.zeros((d1.size, d2.size, p.norb, p.norb), dtype=np.complex128)
+    for a, x in enumerate(d1):
+        for c, y in enumerate(d2):
+            qv = (x / p.n1) * b[:, 0] + (y / p.n2) * b[:, 1]
+            qg = qv[:, None] + g                        # (3, ng)
+            qn = np.linalg.norm(qg, axis=0)
+            zero = qn < 1e-12
+            wq = np.where(zero, wbar, keldysh(np.where(zero, 1.0, qn), r0, eps_env))
+            ph = np.exp(1j * np.einsum("xg,ijx->gij", qg, dt))
+            out[a, c] = np.einsum("g,gij->ij", wq, ph)
+    return out
+
+
+def v_exchange(p: Problem):
+    """V_ij = sum_{G != 0} v(|G|) e^{iG.(tau_i - tau_j)} with bare 2D v."""
+    g = glist2d(p.bravais, p.g_cut)
+    g = np.delete(g, g.shape[1] // 2, axis=1)             # exclude G0 (2811-2816)
+    gn = np.linalg.norm(g, axis=0)
+    dt = p.tau[:, None, :] - p.tau[None, :, :]
+    ph = np.exp(1j * np.einsum("xg,ijx->gij", g, dt))
+    return np.einsum("g,gij->ij", bare2d(gn), ph)
+
+
+# ----------------------------------------------------------------------------
+# Assembly (2767-2928), reference index order r = c*nv*Nk + v*Nk + k
+# ----------------------------------------------------------------------------
+
+def _kidx(p: Problem):
+    i, j = np.meshgrid(np.arange(p.n1), np.arange(p.n2), indexing="ij")
+    return i.ravel(), j.ravel()
+
+
+def direct_term(p: Problem, Cl, Cr, Vl, Vr, wt):
+    """D[(c,v,k),(c',v',k')] = -(1/(A Nk)) sum_ij X_{cc',i} W_ij(k-k') Y_{vv',j}.
+
+    X_{cc',i} = conj(Cl[k,c,i]) Cr[k',c',i];  Y_{vv',j} = Vl[k,v,j] conj(Vr[k',v',j]).
+    The four coefficient arrays are separate arguments so that derivatives can
+    substitute one factor at a time.  Returns (nc,nv,nk,nc,nv,nk).
+    """
+    ki, kj = _kidx(p)
+    nk, nc, nv = p.nk, p.nc, p.nv
+    alpha = -p.direct_scale / (p.area * nk)
+    out = np.empty((nc, nv, nk, nc, nv, nk), dtype=np.complex128)
+    for k in range(nk):
+        wk = wt[ki[k] - ki + p.n1 - 1, kj[k] - kj + p.n2 - 1]       # (nk', i, j)
+        X = np.einsum("ci,kdi->kcdi", Cl[k].conj(), Cr)              # (k', c, c', i)
+        A = np.einsum("kcdi,kij->kcdj", X, wk)                       # (k', c, c', j)
+        Y = np.einsum("vj,kwj->kvwj", Vl[k], Vr.conj())              # (k', v, v', j)
+        blk = np.einsum("kcdj,kvwj->cvdwk", A, Y)                    # (c, v, c', v', k')
+        out[:, :, k] = alpha * blk
+    return out
+
+
+def exchange_factors(p: Problem, C, V):
+    """P[(c,v,k), i] = conj(C[k,c,i]) V[k,v,i] (N x norb)."""
+    P = np.einsum("kci,kvi->cvki", C.conj(), V)
+    return P.reshape(p.n_bse, p.norb)
+
+
+def assemble(p: Problem, bands=None, wt=None, vx=None):
+    """Full TDA matrix H = diag(eps_c(k) - eps_v(k-q)) + D + X (reference order)."""
+    eps_c, C, eps_v, V = band_window(p) if bands is None else bands
+    wt = w_table(p) if wt is None else wt
+    vx = v_exchange(p) if vx is None else vx
+    N = p.n_bse
+    H = direct_term(p, C, C, V, V, wt).reshape(N, N)
+    P = exchange_factors(p, C, V)
+    H += (p.exchange_scale * p.f_s / (p.area * p.nk)) * (P @ vx @ P.conj().T)
+    de = (eps_c[:, :, None] - eps_v[:, None, :])                     # (k, c, v)
+    H[np.diag_indices(N)] += np.transpose(de, (1, 2, 0)).ravel()
+    return H
+
+
+def transition_energies(p: Problem, bands=None):
+    eps_c, C, eps_v, V = band_window(p) if bands is None else bands
+    return np.transpose(eps_c[:, :, None] - eps_v[:, None, :], (1, 2, 0)).ravel()
+
+
+def dipoles(p: Problem, bands=None, direction=(1.0, 0.0, 0.0)):
+    """rho_r(G0) = i sum_i conj(c_{ck,i}) (tau_i . qhat) c_{v,k-q,i}.
+
+    Point-orbital small-q limit of the reference's dipole route (1535-1570,
+    1614-1633): M_ii = i tau_i . qhat, independent of G.
+    """
+    eps_c, C, eps_v, V = band_window(p) if bands is None else bands
+    qh = np.asarray(direction, dtype=np.float64)
+    qh = qh / np.linalg.norm(qh)
+    x = p.tau @ qh
+    rho = 1j * np.einsum("kci,i,kvi->cvk", C.conj(), x, V)
+    return rho.ravel()
+
+
+# ----------------------------------------------------------------------------
+# Eig, oscillator strengths, spectrum (3318-3345, 3486-3502, 3621-3634)
+# ----------------------------------------------------------------------------
+
+def eigh(H):
+    """Ascending eigenvalues, unit eigenvectors (LAPACK zheevd via numpy)."""
+    e, z = np.linalg.eigh(H)
+    return e, z
+
+
+def oscillator(Z, rho0):
+    """f_m = sum_r conj(rho_r) Z_rm (3490-3493)."""
+    return Z.T @ rho0.conj()
+
+
+def dielectric(E, f, omegas, eta, prefactor):
+    """eps_xx(w) = 1 - pref * sum_m |f_m|^2 / (w - E_m + i eta) (3621-3634)."""
+    w = np.asarray(omegas)[:, None]
+    s = (np.abs(f) ** 2)[None, :] / (w - E[None, :] + 1j * eta)
+    return 1.0 - prefactor * s.sum(axis=1)
+
+
+def solve(p: Problem, omegas=None):
+    """Whole path; returns dict with H, E, Z, f, eps."""
+    bands = band_window(p)
+    H = assemble(p, bands)
+    E, Z = eigh(H)
+    rho0 = dipoles(p, bands)
+    f = oscillator(Z, rho0)
+    om = p.omegas if omegas is None else omegas
+    eps = dielectric(E, f, om, p.eta, C_EV_A / (p.volume * p.nk)) if om is not None else None
+    return dict(H=H, E=E, Z=Z, f=f, rho0=rho0, eps=eps, bands=bands)
+
+
+# ----------------------------------------------------------------------------
+# Hellmann-Feynman reference gradients (restated analytically)
+# ----------------------------------------------------------------------------
+
+def _tb_derivative(p: Problem, kcart, dh_of_k):
+    """First-order band derivatives: d eps_n = c_n^H dH c_n,
+    dc_n = sum_{m != n} c_m (c_m^H dH c_n)/(eps_n - eps_m) (parallel transport)."""
+    e, v = tb_states(p, kcart)
+    dh = dh_of_k(kcart)                                   # (nk, norb, norb)
+    m = np.einsum("kin,kij,kjm->knm", v.conj(), dh, v)    # c_n^H dH c_m
+    de = np.real(np.einsum("knn->kn", m))
+    gap = e[:, None, :] - e[:, :, None]                   # [k, m, n] = e_n - e_m
+    with np.errstate(divide="ignore", invalid="ignore"):
+        coef = np.where(np.abs(gap) > 0, m / gap, 0.0)    # coefficient of c_m in dc_n
+    idx = np.arange(p.norb)
+    coef[:, idx, idx] = 0.0
+    dv = np.einsum("kim,kmn->kin", v, coef)
+    return e, v, de, dv
+
+
+def _window(p, e, v, which):
+    if which == "c":
+        idx = p.n_occ + np.arange(p.nc)
+    else:
+        idx = p.n_occ - 1 - np.arange(p.nv)
+    return e[:, idx], np.transpose(v[:, :, idx], (0, 2, 1))
+
+
+def dH_dtheta(p: Problem, name: str):
+    """Exact derivative of the assembled matrix (reference order) w.r.t. one
+    parameter of the gradient set (hopping_scale, scissor, r0, eps_env,
+    exchange_field).  Uses the product rule on the four coefficient factors."""
+    N = p.n_bse
+    if name == "scissor":
+        return np.eye(N, dtype=np.complex128)
+    bands = band_window(p)
+    eps_c, C, eps_v, V = bands
+    if name in ("r0", "eps_env"):
+        h = 1e-6 * (p.r0 if name == "r0" else p.eps_env)
+        kw_p = {name: getattr(p, name) + h}
+        kw_m = {name: getattr(p, name) - h}
+        # W depends analytically on r0/eps; difference of tables at +-h is
+        # exact to O(h^2) and only enters the (linear) direct term.
+        dwt = (w_table(p, **kw_p) - w_table(p, **kw_m)) / (2 * h)
+        return direct_term(p, C, C, V, V, dwt).reshape(N, N)
+    kc = kgrid(p.bravais, p.n1, p.n2)
+    q = np.asarray(p.q, dtype=np.float64)[:, None]
+    zero = np.all(np.abs(p.r_cart) < 1e-12, axis=1)
+    if name == "hopping_scale":
+        def dh_of_k(kk):
+            ph = np.exp(1j * (p.r_cart[~zero] @ kk))
+            return np.einsum("rk,rlm->klm", ph, p.h_r[~zero])
+    elif name == "exchange_field":
+        sz = np.where(np.arange(p.norb) % 2 == 0, 1.0, -1.0)
+
+        def dh_of_k(kk):
+            return np.broadcast_to(np.diag(sz).astype(complex), (kk.shape[1], p.norb, p.norb))
+    else:
+        raise ValueError(name)
+    ek, vk, dek, dvk = _tb_derivative(p, kc, dh_of_k)
+    eq, vq, deq, dvq = _tb_derivative(p, kc - q, dh_of_k)
+    dec, dC = _window(p, dek, dvk, "c")
+    dev, dV = _window(p, deq, dvq, "v")
+    wt = w_table(p)
+    vx = v_exchange(p)
+    dD = (direct_term(p, dC, C, V, V, wt) + direct_term(p, C, dC, V, V, wt)
+          + direct_term(p, C, C, dV, V, wt) + direct_term(p, C, C, V, dV, wt)).reshape(N, N)
+    P = exchange_factors(p, C, V)
+    dP = exchange_factors(p, dC, V) + exchange_factors(p, C, dV)
+    bx = p.exchange_scale * p.f_s / (p.area * p.nk)
+    dX = bx * (dP @ vx @ P.conj().T + P @ vx @ dP.conj().T)
+    dde = np.transpose(dec[:, :, None] - dev[:, None, :], (1, 2, 0)).ravel()
+    out = dD + dX
+    out[np.diag_indices(N)] += dde
+    return out
+
+
+def hf_energy_gradients(p: Problem, n_exc=8, params=("hopping_scale", "scissor", "r0",
+                                                     "eps_env", "exchange_field")):
+    """dE_m/dtheta = A_m^H (dH/dtheta) A_m for the lowest n_exc excitons."""
+    sol = solve(p, omegas=None)
+    Z = sol["Z"][:, :n_exc]
+    g = np.zeros((n_exc, len(params)))
+    for t, name in enumerate(params):
+        dH = dH_dtheta(p, name)
+        g[:, t] = np.real(np.einsum("rm,rs,sm->m", Z.conj(), dH, Z))
+    return g, sol["E"][:n_exc]
+
+
+def with_param(p: Problem, name: str, value: float) -> Problem:
+    """Copy of the problem with one gradient parameter set (for finite differences)."""
+    from dataclasses import replace
+    if name in ("r0", "eps_env", "scissor"):
+        return replace(p, **{name: value})
+    zero = np.all(np.abs(p.r_cart) < 1e-12, axis=1)
+    h = p.h_r.copy()
+    if name == "hopping_scale":
+        base = p.h_r.copy()
+        h[~zero] = base[~zero] * value
+        return replace(p, h_r=h)
+    if name == "exchange_field":
+        sz = np.where(np.arange(p.norb) % 2 == 0, 1.0, -1.0)
+        i0 = int(np.nonzero(zero)[0][0])
+        h[i0] = h[i0] + np.diag(value * sz)
+        return replace(p, h_r=h)
+    raise ValueError(name)
+
+
+def spectrum_l2(p: Problem, target, omegas=None):
+    """L = sum_w (Im eps(w) - target(w))^2."""
+    sol = solve(p, omegas)
+    return float(np.sum((np.imag(sol["eps"]) - target) ** 2)), sol
+
+
+def spectrum_l2_gradient(p: Problem, target, params=("hopping_scale", "scissor", "r0",
+                                                     "eps_env", "exchange_field")):
+    """dL/dtheta via the resolvent: Im eps = -pref Im(rho^H G(z) rho),
+    G = (z - H)^-1 = A diag(1/(z - E)) A^H.  Dense restatement for small N."""
+    om = p.omegas
+    L, sol = spectrum_l2(p, target, om)
+    E, Z, rho = sol["E"], sol["Z"], sol["rho0"]
+    pref = C_EV_A / (p.volume * p.nk)
+    r = np.imag(sol["eps"]) - target                        # (nw,)
+    z = om + 1j * p.eta
+    g = 1.0 / (z[:, None] - E[None, :])                     # (nw, N)
+    f = Z.T @ rho.conj()                                    # f_m = rho^H A_m
+    # K_mn = f_m conj(f_n) sum_w r_w g_m g_n
+    K = (f[:, None] * f.conj()[None, :]) * np.einsum("w,wm,wn->mn", r, g, g)
+    out = np.zeros(len(params))
+    for t, name in enumerate(params):
+        dH = dH_dtheta(p, name)
+        M = Z.conj().T @ dH @ Z
+        term = np.sum(K * M)
+        val = term
+        if name in ("hopping_scale", "exchange_field"):
+            drho = _drho(p, name)
+            # d/dtheta of rho^H G rho from rho: drho^H G rho + rho^H G drho
+            Gr = Z @ (g.T * f.conj()[:, None])              # (N, nw): G(z_w) rho
+            rG = (f[:, None] * g.T)                         # (N_m, nw) coefficients
+            t1 = np.einsum("rw,r->w", Gr, drho.conj())      # drho^H G rho
+            t2 = np.einsum("mw,rm,r->w", rG, Z.conj(), drho)  # rho^H G drho
+            val = term + np.sum(r * (t1 + t2))
+        out[t] = -2.0 * pref * np.imag(val)
+    return out, L
+
+
+def _drho(p: Problem, name):
+    """Derivative of the dipole vector from the TB coefficient derivatives."""
+    kc = kgrid(p.bravais, p.n1, p.n2)
+    q = np.asarray(p.q, dtype=np.float64)[:, None]
+    zero = np.all(np.abs(p.r_cart) < 1e-12, axis=1)
+    if name == "hopping_scale":
+        def dh_of_k(kk):
+            ph = np.exp(1j * (p.r_cart[~zero] @ kk))
+            return np.einsum("rk,rlm->klm", ph, p.h_r[~zero])
+    else:
+        sz = np.where(np.arange(p.norb) % 2 == 0, 1.0, -1.0)
+
+        def dh_of_k(kk):
+            return np.broadcast_to(np.diag(sz).astype(complex), (kk.shape[1], p.norb, p.norb))
+    ek, vk, dek, dvk = _tb_derivative(p, kc, dh_of_k)
+    eq, vq, deq, dvq = _tb_derivative(p, kc - q, dh_of_k)
+    _, C = _window(p, ek, vk, "c")
+    _, V = _window(p, eq, vq, "v")
+    _, dC = _window(p, dek, dvk, "c")
+    _, dV = _window(p, deq, dvq, "v")
+    x = p.tau[:, 0]
+    d = 1j * (np.einsum("kci,i,kvi->cvk", dC.conj(), x, V)
+              + np.einsum("kci,i,kvi->cvk", C.conj(), x, dV))
+    return d.ravel()
