@@ -22,7 +22,7 @@ ABI_SYMBOLS = (
     "bse_create", "bse_destroy", "bse_last_error", "bse_abi_version", "bse_set_problem",
     "bse_problem_sizes", "bse_tb_states", "bse_build", "bse_get_matrix", "bse_get_dipoles",
     "bse_eigh", "bse_get_eigenvalues", "bse_get_eigenvectors", "bse_oscillator",
-    "bse_spectrum", "bse_eigh_dense", "bse_stage_times",
+    "bse_spectrum", "bse_eigh_dense", "bse_stage_times", "bse_profile",
 )
 
 
@@ -75,6 +75,7 @@ def load_library(path: pathlib.Path | str | None = None):
         "bse_spectrum": (i32, [vp, dp, i32, C.c_double, dp, dp]),
         "bse_eigh_dense": (i32, [vp, i32, dp, i64, i32, i32, i32, dp, dp, i64]),
         "bse_stage_times": (i32, [vp, dp, i32]),
+        "bse_profile": (i32, [vp, i32, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -236,6 +237,16 @@ class Context:
         self.system = None
         self._n_dense = n
         return ev, z[:, :nv]
+
+    def profile(self, enable=None):
+        """Hemv kernel profile of the last eig: (ms, algorithmic bytes, launches)."""
+        out = np.zeros(3)
+        en = 1 if enable else 0
+        if enable is None:
+            en = 1 if getattr(self, "_prof", False) else 0
+        self._prof = bool(en)
+        self._check(self.lib.bse_profile(self.h, en, _dptr(out)), "bse_profile")
+        return dict(hemv_ms=out[0], hemv_bytes=out[1], hemv_launches=int(out[2]))
 
     def stage_times(self):
         out = np.zeros(len(STAGES))

@@ -37,7 +37,7 @@ hipError_t bse_launch_expand(hipStream_t, int, int64_t, int, int, const void*, i
 void bse_hetrd_workspace(int, int, int64_t*, int64_t*, int64_t*, int64_t*);
 hipError_t bse_hetrd(hipStream_t, int, int, void*, int64_t, int, double*, double*, double2*,
                      double2*, const int64_t*, double2*, double2*, double2*, double2*, double2*,
-                     double*, double*, unsigned int*);
+                     double*, double*, unsigned int*, hipEvent_t*);
 hipError_t bse_launch_bisect(hipStream_t, int, const int4*, const double2*, const double*,
                              const double*, double, double, double*);
 hipError_t bse_launch_inverse_iter(hipStream_t, int, int, int, const double*, const int*,
@@ -96,6 +96,11 @@ struct bse_ctx {
     hipEvent_t ev[ST_N + 1] = {};
     bool ev_used[ST_N] = {};
     double stage_ms[16] = {0};
+    // optional per-launch timing of the tridiagonalisation's hemv kernel
+    bool prof = false;
+    std::vector<hipEvent_t> prof_ev;
+    double hemv_ms = 0.0, hemv_bytes = 0.0;
+    int64_t hemv_launches = 0;
 };
 
 namespace {
@@ -202,6 +207,7 @@ extern "C" int bse_destroy(bse_ctx* ctx) {
         if (b.p) (void)hipFree(b.p);
     for (int i = 0; i <= ST_N; ++i)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+    for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->st);
     delete ctx;
     return BSE_OK;
@@ -210,6 +216,20 @@ extern "C" int bse_destroy(bse_ctx* ctx) {
 extern "C" int bse_stage_times(const bse_ctx* ctx, double* ms, int n) {
     if (!ctx || !ms || n < 0 || n > 16) return BSE_ERR_ARG;
     for (int i = 0; i < n; ++i) ms[i] = ctx->stage_ms[i];
+    return BSE_OK;
+}
+
+// Kernel-level profile of the last tridiagonalisation: out[0] = total hemv
+// time (ms, HIP events around every launch), out[1] = algorithmic bytes read
+// (lower triangle of each trailing block), out[2] = number of launches.
+extern "C" int bse_profile(bse_ctx* ctx, int enable, double* out) {
+    if (!ctx) return BSE_ERR_ARG;
+    ctx->prof = enable != 0;
+    if (out) {
+        out[0] = ctx->hemv_ms;
+        out[1] = ctx->hemv_bytes;
+        out[2] = (double)ctx->hemv_launches;
+    }
     return BSE_OK;
 }
 
@@ -470,8 +490,32 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32
     unsigned int* counter = BUF(unsigned int, S_COUNTER, 16);
     HCK(hipMemsetAsync(counter, 0, 16 * sizeof(unsigned int), st));
     stage_begin(ctx, ST_TRD);
+    hipEvent_t* pev = nullptr;
+    if (ctx->prof && n > 1) {
+        while ((int64_t)ctx->prof_ev.size() < 2 * (int64_t)(n - 1)) {
+            hipEvent_t ev;
+            HCK(hipEventCreate(&ev));
+            ctx->prof_ev.push_back(ev);
+        }
+        pev = ctx->prof_ev.data();
+    }
     HCK(bse_hetrd(st, fp32, n, H, ld, nb, d, e, tauh, Vst, voff.data(), Wp, acol, Yr, Yc, t12, part,
-                  scal, counter));
+                  scal, counter, pev));
+    if (pev) {
+        HCK(hipStreamSynchronize(st));
+        double ms = 0.0, bytes = 0.0;
+        const double es = fp32 ? 8.0 : 16.0;
+        for (int c = 0; c < n - 1; ++c) {
+            float t = 0.f;
+            HCK(hipEventElapsedTime(&t, pev[2 * c], pev[2 * c + 1]));
+            ms += t;
+            const double nt = (double)(n - c - 1);
+            bytes += es * nt * (nt + 1.0) * 0.5;   // lower triangle of the trailing block
+        }
+        ctx->hemv_ms = ms;
+        ctx->hemv_bytes = bytes;
+        ctx->hemv_launches = n - 1;
+    }
     // compact-WY factors of every panel
     double2* Tall = BUF(double2, S_TALL, (size_t)std::max(npan, 1) * nb * nb);
     double2* Gram = BUF(double2, S_GRAM, (size_t)nb * nb);

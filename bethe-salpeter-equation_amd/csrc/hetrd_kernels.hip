@@ -340,6 +340,7 @@ __global__ void k_last_diag(int n, const void* A, int64_t lda, int fp32, double*
 // Host driver (runs on the context's stream; no host synchronisation).
 // ---------------------------------------------------------------------------
 struct HetrdWork {
+    hipEvent_t* prof_ev; // optional: 2 events per column around the hemv launch
     double2* Vstore;     // packed panel reflectors
     int64_t* voff;       // host array: offset of panel p in Vstore
     double2* Wp;         // n x nb
@@ -385,8 +386,10 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
             const int nq = (int)ceil_div64(nbk, HEMV_CH);
             const T* A22 = A + (c + 1) + (int64_t)(c + 1) * lda;
             const double2* vtr = h.Vp + (i + 1) + (int64_t)i * m;
+            if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c], st);
             hipLaunchKernelGGL(k_hemv_lower<T>, dim3(nq, nbk), dim3(256), 0, st, nt, A22, lda, vtr,
                                w.Yr, w.Yc, nbk, nq);
+            if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c + 1], st);
             hipLaunchKernelGGL(k_hemv_reduce, dim3((unsigned)ceil_div64(nt, 256)), dim3(256), 0, st,
                                h, nt, (const double2*)w.Yr, (const double2*)w.Yc, nbk, nq,
                                (const double2*)w.t12);
@@ -428,9 +431,10 @@ extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int64_
                                 double* d, double* e, double2* tau, double2* Vstore,
                                 const int64_t* voff, double2* Wp, double2* acol, double2* Yr,
                                 double2* Yc, double2* t12, double* part, double* scal,
-                                unsigned int* counter)
+                                unsigned int* counter, hipEvent_t* prof_ev)
 {
     HetrdWork w;
+    w.prof_ev = prof_ev;
     w.Vstore = Vstore; w.voff = (int64_t*)voff; w.Wp = Wp; w.acol = acol; w.Yr = Yr; w.Yc = Yc;
     w.t12 = t12; w.part = part; w.scal = scal; w.counter = counter;
     if (fp32) return hetrd_run<float2>(st, n, (float2*)A, lda, nb, d, e, tau, w);

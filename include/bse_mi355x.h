@@ -152,6 +152,13 @@ int  bse_eigh_dense(bse_ctx* ctx, int n, const double* a, int64_t lda, int a_on_
  * [8] oscillator+spectrum.  n <= 16. */
 int  bse_stage_times(const bse_ctx* ctx, double* ms, int n);
 
+/* Kernel-level profile of the tridiagonalisation's Hermitian matrix-vector
+ * kernel (HIP events around every launch on the context stream).  enable
+ * switches recording on for later calls; out (may be NULL) receives
+ * [0] total kernel time in ms, [1] algorithmic bytes (lower triangle of every
+ * trailing block), [2] launches, for the last bse_eigh. */
+int  bse_profile(bse_ctx* ctx, int enable, double* out);
+
 #ifdef __cplusplus
 }
 #endif
