@@ -7,11 +7,13 @@ device pointers where an entry point takes ``*_on_device``).
 from __future__ import annotations
 
 import ctypes as C
+import os
 import pathlib
 
 import numpy as np
 
-LIB_PATH = pathlib.Path(__file__).resolve().parent / "libbse_mi355x.so"
+LIB_PATH = pathlib.Path(os.environ.get(
+    "BSE_MI355X_LIB", pathlib.Path(__file__).resolve().parent / "libbse_mi355x.so"))
 
 PREC_FP64, PREC_FP32, PREC_BF16X2 = 0, 1, 2
 ORDER_REFERENCE, ORDER_KMAJOR = 0, 1

@@ -17,14 +17,15 @@ import sys
 
 PKG = pathlib.Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
-BUILD = PKG / "_build"
-LIB = PKG / "libbse_mi355x.so"
+BUILD = pathlib.Path(os.environ.get("BSE_BUILD_DIR", PKG / "_build"))
+LIB = pathlib.Path(os.environ.get("BSE_MI355X_LIB", PKG / "libbse_mi355x.so"))
 ARCH = os.environ.get("BSE_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-Wno-unused-result", "-munsafe-fp-atomics"]
+          "-Wno-unused-result", "-munsafe-fp-atomics",
+          *os.environ.get("BSE_HIPCC_FLAGS", "").split()]
 
 
 def _compile(src: pathlib.Path) -> pathlib.Path:

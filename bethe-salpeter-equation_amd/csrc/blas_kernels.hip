@@ -141,8 +141,8 @@ extern "C" hipError_t bse_zgemm(hipStream_t st, int opa, int opb, int M, int N, 
 }
 
 // Compact-WY factor: T (nb x nb, upper, ld ldt) from tau (nb) and the Gram
-// matrix G = V^H V (nb x nb, ld nb):  T(i,i) = tau_i,
-// T(0:i, i) = -tau_i * T(0:i, 0:i) G(0:i, i).
+// matrix G = V^H V (nb x nb, ld 64; only G(l, i), l < i, is read):
+// T(i,i) = tau_i, T(0:i, i) = -tau_i * T(0:i, 0:i) G(0:i, i).
 __global__ void k_larft(int nb, const double2* __restrict__ tau, const double2* __restrict__ G,
                         double2* __restrict__ T, int ldt)
 {
@@ -154,7 +154,7 @@ __global__ void k_larft(int nb, const double2* __restrict__ tau, const double2* 
         const int r = threadIdx.x;
         if (r < i) {
             double2 s = cmk(0, 0);
-            for (int l = r; l < i; ++l) cfma(s, T[r + l * ldt], G[l + i * nb]);
+            for (int l = r; l < i; ++l) cfma(s, T[r + l * ldt], G[l + i * 64]);
             col[r] = cscale(cmul(ti, s), -1.0);
         }
         __syncthreads();

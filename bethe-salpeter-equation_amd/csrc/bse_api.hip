@@ -37,7 +37,7 @@ hipError_t bse_launch_expand(hipStream_t, int, int64_t, int, int, const void*, i
 void bse_hetrd_workspace(int, int, int64_t*, int64_t*, int64_t*, int64_t*);
 hipError_t bse_hetrd(hipStream_t, int, int, void*, int64_t, int, double*, double*, double2*,
                      double2*, const int64_t*, double2*, double2*, double2*, double2*, double2*,
-                     double*, double*, unsigned int*, hipEvent_t*);
+                     double2*, double2*, double2*, double*, double*, unsigned int*, hipEvent_t*);
 hipError_t bse_launch_bisect(hipStream_t, int, const int4*, const double2*, const double*,
                              const double*, double, double, double*);
 hipError_t bse_launch_inverse_iter(hipStream_t, int, int, int, const double*, const int*,
@@ -109,7 +109,7 @@ enum Slot {
     S_RCART, S_HR, S_TAU, S_KPTS, S_GVEC, S_Q, S_EPS, S_COEF, S_WTAB, S_VX, S_P, S_PV, S_RHO, S_DE,
     S_H, S_D, S_E, S_TAUH, S_VSTORE, S_WP, S_ACOL, S_YR, S_YC, S_T12, S_PART, S_SCAL, S_COUNTER,
     S_TALL, S_GRAM, S_E2, S_ITEMS, S_BOUNDS, S_LAMI, S_LAM, S_BSV, S_NBV, S_CLUST, S_IISCR, S_IIX,
-    S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_NSLOTS
+    S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_ALPHA2, S_NSLOTS
 };
 
 int fail(bse_ctx* c, int code, const std::string& msg) {
@@ -499,8 +499,11 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32
         }
         pev = ctx->prof_ev.data();
     }
-    HCK(bse_hetrd(st, fp32, n, H, ld, nb, d, e, tauh, Vst, voff.data(), Wp, acol, Yr, Yc, t12, part,
-                  scal, counter, pev));
+    double2* Tall = BUF(double2, S_TALL, (size_t)std::max(npan, 1) * nb * nb);
+    double2* Gram = BUF(double2, S_GRAM, (size_t)64 * 64);
+    double2* alpha2 = BUF(double2, S_ALPHA2, 64);
+    HCK(bse_hetrd(st, fp32, n, H, ld, nb, d, e, tauh, Vst, voff.data(), Tall, Wp, alpha2, Gram, acol,
+                  Yr, Yc, t12, part, scal, counter, pev));
     if (pev) {
         HCK(hipStreamSynchronize(st));
         double ms = 0.0, bytes = 0.0;
@@ -516,18 +519,9 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32
         ctx->hemv_bytes = bytes;
         ctx->hemv_launches = n - 1;
     }
-    // compact-WY factors of every panel
-    double2* Tall = BUF(double2, S_TALL, (size_t)std::max(npan, 1) * nb * nb);
-    double2* Gram = BUF(double2, S_GRAM, (size_t)nb * nb);
     const int64_t skws = (int64_t)64 * 1024 * 1024 / sizeof(double2) * 8;   // 512 MiB split-K workspace cap
     const int64_t sk_el = std::min<int64_t>(skws, std::max<int64_t>((int64_t)64 * nb * std::max(n_vectors, 64), (int64_t)nb * nb * 512));
     double2* skw = BUF(double2, S_SPLITK, sk_el);
-    for (int p = 0; p < npan; ++p) {
-        const int j0 = p * nb, m = n - j0, nbp = std::min(nb, n - 1 - j0);
-        const double2* Vp = Vst + voff[p];
-        HCK(bse_zgemm(st, 1, 0, nbp, nbp, m, 1, 0, Vp, m, Vp, m, 0, 0, Gram, nbp, skw, sk_el));
-        HCK(bse_launch_larft(st, nbp, tauh + j0, Gram, Tall + (size_t)p * nb * nb, nb));
-    }
     stage_end(ctx, ST_TRD);
 
     // ---------------- K6: tridiagonal eigenvalues (host bookkeeping + bisection)
@@ -628,7 +622,9 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32
     stage_begin(ctx, ST_TVEC);
     const int nstore = (n_vectors < 0) ? n : std::min(n_vectors, n);
     const double tolc = 1e-7 * tnorm;
-    int bsz = std::min(n, 4096);
+    // one thread per eigenvalue; as many per batch as 24 GB of scratch allows
+    const int64_t cap = std::max<int64_t>(4096, (int64_t)(24e9 / (48.0 * n)));
+    int bsz = (int)std::min<int64_t>(n, cap);
     const int64_t iscr = (int64_t)5 * n * bsz;
     double* scr = BUF(double, S_IISCR, iscr);
     double* xb = BUF(double, S_IIX, (size_t)n * bsz);

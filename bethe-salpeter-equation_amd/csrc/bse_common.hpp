@@ -59,6 +59,18 @@ template <> struct cplx<float2> {
     }
 };
 
+// Streaming (non-temporal) loads of complex elements.
+typedef float bse_f2v __attribute__((ext_vector_type(2)));
+typedef double bse_d2v __attribute__((ext_vector_type(2)));
+__device__ inline float2 nt_load(const float2* p) {
+    const bse_f2v v = __builtin_nontemporal_load(reinterpret_cast<const bse_f2v*>(p));
+    return make_float2(v.x, v.y);
+}
+__device__ inline double2 nt_load(const double2* p) {
+    const bse_d2v v = __builtin_nontemporal_load(reinterpret_cast<const bse_d2v*>(p));
+    return make_double2(v.x, v.y);
+}
+
 // ---------------------------------------------------------------------------
 // Wave-level reductions (64-lane wavefronts on CDNA4).
 // ---------------------------------------------------------------------------

@@ -30,15 +30,22 @@ struct HetrdCol {
     int i;            // column inside the panel
     int m;            // n - j0 (panel rows)
     double2* Vp;      // panel reflectors, m x nb, ld m
-    double2* Wp;      // panel W, m x nb, ld m
+    double2* Wp;      // panel W without its alpha2 v term, m x nb, ld m
+    double2* alpha2;  // per panel column: W(:, l) = Wp(:, l) + alpha2[l] Vp(:, l)
+    double2* G;       // nb x nb: G(l, i) = V(:, l)^H v_i  (for the compact-WY factor)
     double2* acol;    // updated column, length m - i (row c .. n-1)
     double* d;
     double* e;
     double2* tau;
-    double* scal;     // [0,1] scale (1/(alpha-beta)), [2,3] tau, [4,5] alpha2
+    double* scal;     // [0,1] scale (1/(alpha-beta)), [2,3] tau
     double* part;     // reduction partials
     unsigned int* counter;
 };
+
+// W(r, l) of the panel, with the alpha2 v term applied on the fly.
+__device__ inline double2 w_at(const HetrdCol& h, int64_t r, int l) {
+    return cadd(h.Wp[r + (int64_t)l * h.m], cmul(h.alpha2[l], h.Vp[r + (int64_t)l * h.m]));
+}
 
 // (a) + (b): column update, norm, reflector scalars.
 template <typename T>
@@ -54,8 +61,8 @@ __global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict_
         double2 a = cplx<T>::load(A + (c + t) + (int64_t)c * h.lda);
         for (int l = 0; l < h.i; ++l) {
             const double2 vr = h.Vp[lrow + (int64_t)l * h.m];
-            const double2 wr = h.Wp[lrow + (int64_t)l * h.m];
-            const double2 wc = h.Wp[h.i + (int64_t)l * h.m];
+            const double2 wr = w_at(h, lrow, l);
+            const double2 wc = w_at(h, h.i, l);
             const double2 vc = h.Vp[h.i + (int64_t)l * h.m];
             a = csub(a, cadd(cmul(vr, cconj(wc)), cmul(wr, cconj(vc))));
         }
@@ -88,45 +95,54 @@ __global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict_
     }
 }
 
-// Reflector vector into Vp(:, i), plus partials of t1 = W^H v and t2 = V^H v
-// over rows i+1 .. m-1 for the panel's earlier columns l < i.
+// Reflector vector into Vp(:, i), plus t1 = W^H v and t2 = V^H v over the
+// panel's earlier columns l < i (rows i+1 .. m-1).  Grid (row blocks, l):
+// block (rb, l) covers MAKEV_ROWS rows for one column l, so the dot products
+// of all l run concurrently; the last block reduces the partials in a fixed
+// order.  t2 also fills column i of the Gram matrix used for the compact-WY
+// factor.  The l = 0 blocks write the reflector.
+#define MAKEV_RPT 4
+#define MAKEV_ROWS (256 * MAKEV_RPT)
 __global__ __launch_bounds__(256) void k_makev(HetrdCol h, double2* __restrict__ t12)
 {
     __shared__ double sh[32];
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;   // local panel row
+    const int rb = blockIdx.x, l = blockIdx.y, nrb = gridDim.x;
     const double2 sc = cmk(h.scal[0], h.scal[1]);
-    double2 v = cmk(0, 0);
-    if (r < h.m) {
-        if (r == h.i + 1) v = cmk(1, 0);
-        else if (r > h.i + 1) v = cmul(h.acol[r - h.i], sc);
-        h.Vp[r + (int64_t)h.i * h.m] = v;
-    }
-    // partials
-    for (int l = 0; l < h.i; ++l) {
-        double2 p1 = cmk(0, 0), p2 = cmk(0, 0);
-        if (r < h.m && r > h.i) {
-            cfmac(p1, h.Wp[r + (int64_t)l * h.m], v);
-            cfmac(p2, h.Vp[r + (int64_t)l * h.m], v);
+    double2 p1 = cmk(0, 0), p2 = cmk(0, 0);
+#pragma unroll
+    for (int j = 0; j < MAKEV_RPT; ++j) {
+        const int r = rb * MAKEV_ROWS + threadIdx.x + 256 * j;
+        if (r >= h.m) continue;
+        double2 vv = cmk(0, 0);
+        if (r == h.i + 1) vv = cmk(1, 0);
+        else if (r > h.i + 1) vv = cmul(h.acol[r - h.i], sc);
+        if (l == 0) h.Vp[r + (int64_t)h.i * h.m] = vv;
+        if (l < h.i && r > h.i) {
+            cfmac(p1, w_at(h, r, l), vv);
+            cfmac(p2, h.Vp[r + (int64_t)l * h.m], vv);
         }
+    }
+    double2* pp = (double2*)h.part;
+    if (h.i > 0) {
         p1 = block_sum2(p1, sh);
         p2 = block_sum2(p2, sh);
         if (threadIdx.x == 0) {
-            double2* pp = (double2*)h.part;
-            pp[(blockIdx.x * 2 + 0) * 64 + l] = p1;
-            pp[(blockIdx.x * 2 + 1) * 64 + l] = p2;
+            pp[((int64_t)l * nrb + rb) * 2 + 0] = p1;
+            pp[((int64_t)l * nrb + rb) * 2 + 1] = p2;
         }
     }
-    if (!last_block_ticket(h.counter, gridDim.x)) return;
-    const int l = threadIdx.x;
-    if (l < h.i) {
-        const double2* pp = (const double2*)h.part;
-        double2 s1 = cmk(0, 0), s2 = cmk(0, 0);
-        for (unsigned b = 0; b < gridDim.x; ++b) {
-            s1 = cadd(s1, pp[(b * 2 + 0) * 64 + l]);
-            s2 = cadd(s2, pp[(b * 2 + 1) * 64 + l]);
+    if (!last_block_ticket(h.counter, gridDim.x * gridDim.y)) return;
+    const int t = threadIdx.x;
+    if (t < 2 * h.i) {
+        const int ll = t >> 1, which = t & 1;
+        double2 s = cmk(0, 0);
+        for (int b = 0; b < nrb; ++b) s = cadd(s, pp[((int64_t)ll * nrb + b) * 2 + which]);
+        if (which) {
+            t12[64 + ll] = s;
+            h.G[ll + (int64_t)h.i * 64] = s;
+        } else {
+            t12[ll] = s;
         }
-        t12[l] = s1;
-        t12[64 + l] = s2;
     }
 }
 
@@ -134,8 +150,14 @@ __global__ __launch_bounds__(256) void k_makev(HetrdCol h, double2* __restrict__
 // Workgroup (q, bj) handles block column bj (64 columns) and block rows
 // bi in [bj + q*CH, bj + (q+1)*CH).  Row results of each 64x64 block go to
 // Yr[bi][bj]; the transposed (conjugate) contribution for column block bj is
-// accumulated across the chunk and written to Yc[bj][q].
+// accumulated across the chunk and written to Yc[bj][q].  One wave owns 16
+// columns; each load instruction reads 64 consecutive rows of one column.
+#ifndef HEMV_CH
 #define HEMV_CH 8
+#endif
+#ifndef HEMV_NT
+#define HEMV_NT 0
+#endif
 template <typename T>
 __global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict__ A, int64_t lda,
                                                    const double2* __restrict__ v,
@@ -148,7 +170,7 @@ __global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict_
     const int bi1 = min(nbk, bi0 + HEMV_CH);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     __shared__ double2 s_v[64];
-    __shared__ double2 s_part[4][64];
+    __shared__ double2 s_part[4][HEMV_CH][64];
     const int col0 = bj * 64;
     if (threadIdx.x < 64) {
         const int cidx = col0 + threadIdx.x;
@@ -166,13 +188,17 @@ __global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict_
         double2 racc = cmk(0, 0);
         const T* Ap = A + row + (int64_t)cw * lda;
         if (bi > bj && (bi + 1) * 64 <= nt) {
-            double2 a[16];
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) a[jj] = cplx<T>::load(Ap + (int64_t)jj * lda);
+            T a[16];
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) {
-                cfma(racc, a[jj], s_v[w * 16 + jj]);
-                cfmac(cacc[jj], a[jj], vr);
+                if (HEMV_NT) a[jj] = nt_load(Ap + (int64_t)jj * lda);
+                else a[jj] = Ap[(int64_t)jj * lda];
+            }
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
+                cfma(racc, x, s_v[w * 16 + jj]);
+                cfmac(cacc[jj], x, vr);
             }
         } else {
 #pragma unroll
@@ -188,14 +214,14 @@ __global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict_
                 }
             }
         }
-        s_part[w][lane] = racc;
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            double2 s = cadd(cadd(s_part[0][lane], s_part[1][lane]),
-                             cadd(s_part[2][lane], s_part[3][lane]));
-            Yr[((int64_t)bi * nbk + bj) * 64 + lane] = s;
-        }
-        __syncthreads();
+        s_part[w][bi - bi0][lane] = racc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < (bi1 - bi0) * 64; e += 256) {
+        const int bb = e >> 6, ln = e & 63;
+        const double2 s = cadd(cadd(s_part[0][bb][ln], s_part[1][bb][ln]),
+                               cadd(s_part[2][bb][ln], s_part[3][bb][ln]));
+        Yr[((int64_t)(bi0 + bb) * nbk + bj) * 64 + ln] = s;
     }
 #pragma unroll
     for (int jj = 0; jj < 16; ++jj) {
@@ -204,32 +230,50 @@ __global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict_
     }
 }
 
-// Reduce the hemv partials, apply (d) and (e) scaling, partial dot y^H v.
-// Writes the unfinished W column (y) and alpha2 = -tau/2 (y^H v).
-__global__ __launch_bounds__(256) void k_hemv_reduce(HetrdCol h, int nt, const double2* __restrict__ Yr,
-                                                     const double2* __restrict__ Yc, int nbk, int nq,
-                                                     const double2* __restrict__ t12)
+// Reduce the hemv partials of one block row (64 rows) per workgroup, apply the
+// panel corrections (d) and the tau scaling (e), partial dot y^H v.  Sixteen
+// lane groups split the partial sums so that many loads are in flight.
+// Stores y in Wp(:, i); the last block computes alpha2[i] = -tau/2 (y^H v).
+#define RED_G 16
+__global__ __launch_bounds__(1024) void k_hemv_reduce(HetrdCol h, int nt, const double2* __restrict__ Yr,
+                                                      const double2* __restrict__ Yc, int nbk, int nq,
+                                                      const double2* __restrict__ t12)
 {
+    __shared__ double2 sacc[RED_G][64];
     __shared__ double sh[32];
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;   // trailing row
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int r = b * 64 + lane;
+    const bool valid = r < nt;
+    const int lr = h.i + 1 + r;                         // panel-local row
+    double2 acc0 = cmk(0, 0), acc1 = cmk(0, 0);
+    if (valid) {
+        const double2* yr = Yr + (int64_t)b * nbk * 64 + lane;
+        int bj = g;
+        for (; bj + RED_G <= b; bj += 2 * RED_G) {
+            acc0 = cadd(acc0, yr[(int64_t)bj * 64]);
+            acc1 = cadd(acc1, yr[(int64_t)(bj + RED_G) * 64]);
+        }
+        if (bj <= b) acc0 = cadd(acc0, yr[(int64_t)bj * 64]);
+        const int nqb = (int)ceil_div64(nbk - b, HEMV_CH);
+        for (int q = g; q < nqb; q += RED_G) acc1 = cadd(acc1, Yc[((int64_t)b * nq + q) * 64 + lane]);
+        for (int l = g; l < h.i; l += RED_G) {
+            const double2 vv = h.Vp[lr + (int64_t)l * h.m];
+            const double2 ww = w_at(h, lr, l);
+            acc0 = csub(acc0, cadd(cmul(vv, t12[l]), cmul(ww, t12[64 + l])));
+        }
+    }
+    sacc[g][lane] = cadd(acc0, acc1);
+    __syncthreads();
     double2 dotp = cmk(0, 0);
     const double2 tau = cmk(h.scal[2], h.scal[3]);
-    if (r < nt) {
-        const int b = r / 64, lane = r % 64;
+    if (g == 0 && valid) {
         double2 y = cmk(0, 0);
-        for (int bj = 0; bj <= b; ++bj) y = cadd(y, Yr[((int64_t)b * nbk + bj) * 64 + lane]);
-        const int nqb = (int)ceil_div64(nbk - b, HEMV_CH);
-        for (int q = 0; q < nqb; ++q) y = cadd(y, Yc[((int64_t)b * nq + q) * 64 + lane]);
-        const int lr = h.i + 1 + r;                         // panel-local row
-        for (int l = 0; l < h.i; ++l) {
-            const double2 vv = h.Vp[lr + (int64_t)l * h.m];
-            const double2 ww = h.Wp[lr + (int64_t)l * h.m];
-            y = csub(y, cadd(cmul(vv, t12[l]), cmul(ww, t12[64 + l])));
-        }
+#pragma unroll
+        for (int k = 0; k < RED_G; ++k) y = cadd(y, sacc[k][lane]);
         y = cmul(y, tau);
         h.Wp[lr + (int64_t)h.i * h.m] = y;
-        const double2 v = h.Vp[lr + (int64_t)h.i * h.m];
-        cfmac(dotp, y, v);
+        cfmac(dotp, y, h.Vp[lr + (int64_t)h.i * h.m]);
     }
     dotp = block_sum2(dotp, sh);
     double2* pp = (double2*)h.part;
@@ -237,23 +281,18 @@ __global__ __launch_bounds__(256) void k_hemv_reduce(HetrdCol h, int nt, const d
     if (!last_block_ticket(h.counter, gridDim.x)) return;
     if (threadIdx.x == 0) {
         double2 s = cmk(0, 0);
-        for (unsigned b = 0; b < gridDim.x; ++b) s = cadd(s, pp[b]);
-        const double2 a2 = cscale(cmul(tau, s), -0.5);
-        h.scal[4] = a2.x;
-        h.scal[5] = a2.y;
+        for (unsigned bb = 0; bb < gridDim.x; ++bb) s = cadd(s, pp[bb]);
+        h.alpha2[h.i] = cscale(cmul(tau, s), -0.5);
     }
 }
 
-// W(:, i) = y + alpha2 v, zero above the reflector.
-__global__ __launch_bounds__(256) void k_wfinal(HetrdCol h)
+// Zero W(0:i, i) and alpha2 (rows above the reflector are never read, but the
+// whole column is kept defined).
+__global__ void k_wclear(HetrdCol h)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= h.m) return;
-    double2* wcol = h.Wp + (int64_t)h.i * h.m;
-    if (r <= h.i) { wcol[r] = cmk(0, 0); return; }
-    const double2 a2 = cmk(h.scal[4], h.scal[5]);
-    const double2 v = h.Vp[r + (int64_t)h.i * h.m];
-    wcol[r] = cadd(wcol[r], cmul(a2, v));
+    if (r <= h.i && r < h.m) h.Wp[r + (int64_t)h.i * h.m] = cmk(0, 0);
+    if (r == 0) h.alpha2[h.i] = cmk(0, 0);
 }
 
 // Trailing update A22 -= V W^H + W V^H on lower 64x64 tiles, K = nb.
@@ -262,7 +301,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_her2k_lower(int n2, T* __restrict__ A, int64_t lda,
                                                     const double2* __restrict__ V,
                                                     const double2* __restrict__ W, int64_t ldv,
-                                                    int nb)
+                                                    int nb, const double2* __restrict__ alpha2)
 {
     int ti, tj;
     {
@@ -287,10 +326,13 @@ __global__ __launch_bounds__(256) void k_her2k_lower(int n2, T* __restrict__ A, 
             const int l = l0 + ll;
             const bool lok = l < nb;
             const int gr = r0 + rr, gc = c0 + rr;
-            sVr[ll][rr] = (lok && gr < n2) ? V[gr + (int64_t)l * ldv] : cmk(0, 0);
-            sWr[ll][rr] = (lok && gr < n2) ? W[gr + (int64_t)l * ldv] : cmk(0, 0);
-            sVc[ll][rr] = (lok && gc < n2) ? V[gc + (int64_t)l * ldv] : cmk(0, 0);
-            sWc[ll][rr] = (lok && gc < n2) ? W[gc + (int64_t)l * ldv] : cmk(0, 0);
+            const double2 a2 = lok ? alpha2[l] : cmk(0, 0);
+            const double2 vr = (lok && gr < n2) ? V[gr + (int64_t)l * ldv] : cmk(0, 0);
+            const double2 vc = (lok && gc < n2) ? V[gc + (int64_t)l * ldv] : cmk(0, 0);
+            sVr[ll][rr] = vr;
+            sWr[ll][rr] = (lok && gr < n2) ? cadd(W[gr + (int64_t)l * ldv], cmul(a2, vr)) : cmk(0, 0);
+            sVc[ll][rr] = vc;
+            sWc[ll][rr] = (lok && gc < n2) ? cadd(W[gc + (int64_t)l * ldv], cmul(a2, vc)) : cmk(0, 0);
         }
         __syncthreads();
 #pragma unroll
@@ -339,16 +381,22 @@ __global__ void k_last_diag(int n, const void* A, int64_t lda, int fp32, double*
 // ---------------------------------------------------------------------------
 // Host driver (runs on the context's stream; no host synchronisation).
 // ---------------------------------------------------------------------------
+extern "C" hipError_t bse_launch_larft(hipStream_t st, int nb, const double2* tau, const double2* G,
+                                       double2* T, int ldt);
+
 struct HetrdWork {
     hipEvent_t* prof_ev; // optional: 2 events per column around the hemv launch
     double2* Vstore;     // packed panel reflectors
     int64_t* voff;       // host array: offset of panel p in Vstore
+    double2* Tall;       // compact-WY factors, nb x nb per panel
     double2* Wp;         // n x nb
+    double2* alpha2;     // nb
+    double2* G;          // 64 x 64
     double2* acol;       // n
     double2* Yr;         // nbk^2 x 64
     double2* Yc;         // nbk * nq * 64
     double2* t12;        // 128
-    double* part;        // >= 2 * 64 * nblocks doubles
+    double* part;        // reduction partials
     double* scal;        // 8
     unsigned int* counter;
 };
@@ -371,17 +419,20 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
         h.n = n; h.lda = lda; h.j0 = j0; h.m = m;
         h.Vp = w.Vstore + w.voff[p];
         h.Wp = w.Wp;
+        h.alpha2 = w.alpha2;
+        h.G = w.G;
         h.acol = w.acol; h.d = d; h.e = e; h.tau = tau; h.scal = w.scal; h.part = w.part;
         h.counter = w.counter;
         for (int i = 0; i < nbp; ++i) {
             h.i = i;
             const int L = m - i;
-            hipLaunchKernelGGL(k_colupd<T>, dim3((unsigned)ceil_div64(L, 256)), dim3(256), 0, st, h,
-                               (const T*)A);
-            hipLaunchKernelGGL(k_makev, dim3((unsigned)ceil_div64(m, 256)), dim3(256), 0, st, h,
-                               w.t12);
             const int c = j0 + i;
             const int nt = n - c - 1;
+            hipLaunchKernelGGL(k_wclear, dim3((unsigned)ceil_div64(i + 1, 64)), dim3(64), 0, st, h);
+            hipLaunchKernelGGL(k_colupd<T>, dim3((unsigned)ceil_div64(L, 256)), dim3(256), 0, st, h,
+                               (const T*)A);
+            hipLaunchKernelGGL(k_makev, dim3((unsigned)ceil_div64(m, MAKEV_ROWS), (unsigned)max(i, 1)),
+                               dim3(256), 0, st, h, w.t12);
             const int nbk = (int)ceil_div64(nt, 64);
             const int nq = (int)ceil_div64(nbk, HEMV_CH);
             const T* A22 = A + (c + 1) + (int64_t)(c + 1) * lda;
@@ -390,20 +441,22 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
             hipLaunchKernelGGL(k_hemv_lower<T>, dim3(nq, nbk), dim3(256), 0, st, nt, A22, lda, vtr,
                                w.Yr, w.Yc, nbk, nq);
             if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c + 1], st);
-            hipLaunchKernelGGL(k_hemv_reduce, dim3((unsigned)ceil_div64(nt, 256)), dim3(256), 0, st,
-                               h, nt, (const double2*)w.Yr, (const double2*)w.Yc, nbk, nq,
+            hipLaunchKernelGGL(k_hemv_reduce, dim3((unsigned)nbk), dim3(1024), 0, st, h, nt,
+                               (const double2*)w.Yr, (const double2*)w.Yc, nbk, nq,
                                (const double2*)w.t12);
-            hipLaunchKernelGGL(k_wfinal, dim3((unsigned)ceil_div64(m, 256)), dim3(256), 0, st, h);
             if ((err = hipGetLastError()) != hipSuccess) return err;
         }
-        // zero the unused reflector columns of a partial last panel
+        // compact-WY factor of this panel from G(l, i) = V(:, l)^H v_i
+        if ((err = bse_launch_larft(st, nbp, tau + j0, w.G, w.Tall + (size_t)p * nb * nb, nb)) !=
+            hipSuccess)
+            return err;
         const int n2 = m - nbp;
         if (n2 > 0) {
             const int64_t nt2 = ceil_div64(n2, 64);
             hipLaunchKernelGGL(k_her2k_lower<T>, dim3((unsigned)(nt2 * (nt2 + 1) / 2)), dim3(256), 0,
                                st, n2, A + (j0 + nbp) + (int64_t)(j0 + nbp) * lda, lda,
                                (const double2*)(h.Vp + nbp), (const double2*)(w.Wp + nbp),
-                               (int64_t)m, nbp);
+                               (int64_t)m, nbp, (const double2*)w.alpha2);
         }
         if ((err = hipGetLastError()) != hipSuccess) return err;
     }
@@ -412,7 +465,7 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
     return hipGetLastError();
 }
 
-// Workspace sizes (bytes) for a given n, nb.
+// Workspace sizes for a given n, nb.
 extern "C" void bse_hetrd_workspace(int n, int nb, int64_t* vstore_elems, int64_t* yr_elems,
                                     int64_t* yc_elems, int64_t* part_doubles)
 {
@@ -423,20 +476,24 @@ extern "C" void bse_hetrd_workspace(int n, int nb, int64_t* vstore_elems, int64_
     const int64_t nq = ceil_div64(nbk, HEMV_CH);
     *yr_elems = nbk * nbk * 64;
     *yc_elems = nbk * nq * 64;
-    const int64_t nblk = ceil_div64(n, 256) + 1;
+    // makev: 2 complex per (l, row block) for l < 64; reduce: 1 complex per block row
+    const int64_t nblk = ceil_div64(n, 64) + 1;
     *part_doubles = 2 * 2 * 64 * nblk + 64;
 }
 
+// nb <= 64.  Tall receives one nb x nb compact-WY factor per panel.
 extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int64_t lda, int nb,
                                 double* d, double* e, double2* tau, double2* Vstore,
-                                const int64_t* voff, double2* Wp, double2* acol, double2* Yr,
-                                double2* Yc, double2* t12, double* part, double* scal,
-                                unsigned int* counter, hipEvent_t* prof_ev)
+                                const int64_t* voff, double2* Tall, double2* Wp, double2* alpha2,
+                                double2* G, double2* acol, double2* Yr, double2* Yc, double2* t12,
+                                double* part, double* scal, unsigned int* counter,
+                                hipEvent_t* prof_ev)
 {
     HetrdWork w;
     w.prof_ev = prof_ev;
-    w.Vstore = Vstore; w.voff = (int64_t*)voff; w.Wp = Wp; w.acol = acol; w.Yr = Yr; w.Yc = Yc;
-    w.t12 = t12; w.part = part; w.scal = scal; w.counter = counter;
+    w.Vstore = Vstore; w.voff = (int64_t*)voff; w.Tall = Tall; w.Wp = Wp; w.alpha2 = alpha2;
+    w.G = G; w.acol = acol; w.Yr = Yr; w.Yc = Yc; w.t12 = t12; w.part = part; w.scal = scal;
+    w.counter = counter;
     if (fp32) return hetrd_run<float2>(st, n, (float2*)A, lda, nb, d, e, tau, w);
     return hetrd_run<double2>(st, n, (double2*)A, lda, nb, d, e, tau, w);
 }

@@ -90,7 +90,7 @@ __device__ inline double hash_uniform(uint64_t a, uint64_t b) {
 
 // Inverse iteration.  Thread t of the batch owns sorted eigenvalue m0 + t in
 // block (bs, nb).  Scratch arrays are indexed [global row * bsz + t].
-__global__ __launch_bounds__(128) void k_inverse_iter(
+__global__ __launch_bounds__(64) void k_inverse_iter(
     int bsz, int cnt, int m0, const double* __restrict__ lam, const int* __restrict__ bsv,
     const int* __restrict__ nbv, const double* __restrict__ d, const double* __restrict__ e,
     double tnorm, double* __restrict__ u0, double* __restrict__ u1, double* __restrict__ u2,
@@ -140,44 +140,39 @@ __global__ __launch_bounds__(128) void k_inverse_iter(
         if (fabs(a0) < tiny) a0 = (a0 < 0.0) ? -tiny : tiny;
         u0[(int64_t)(bs + nb - 1) * S + t] = a0;
     }
-    // Random start.
+    // Random start; three solves.  The normalisation of each iterate is folded
+    // into the next forward pass (scale s), and the last back substitution
+    // records the largest component so the final pass also fixes the sign.
     for (int k = 0; k < nb; ++k) x[(int64_t)(bs + k) * S + t] = hash_uniform(m0 + t, k);
+    double s = 1.0, big = 0.0;
     for (int it = 0; it < 3; ++it) {
-        // forward: L^-1 with the recorded row swaps
-        double cur = x[(int64_t)bs * S + t];
+        double cur = x[(int64_t)bs * S + t] * s;
         for (int k = 0; k < nb - 1; ++k) {
             const int64_t gi = bs + k;
-            double nxt = x[(gi + 1) * S + t];
+            double nxt = x[(gi + 1) * S + t] * s;
             if (pf[gi * S + t] != 0.0) { const double tmp = cur; cur = nxt; nxt = tmp; }
             nxt -= mul[gi * S + t] * cur;
             x[gi * S + t] = cur;
             cur = nxt;
         }
-        // back substitution with U (diag u0, super u1, second super u2)
         const int64_t gl = bs + nb - 1;
         double x1 = cur / u0[gl * S + t];
         x[gl * S + t] = x1;
         double x2 = 0.0, nrm = x1 * x1;
+        big = x1;
         for (int k = nb - 2; k >= 0; --k) {
             const int64_t gi = bs + k;
             const double xk = (x[gi * S + t] - u1[gi * S + t] * x1 - u2[gi * S + t] * x2) / u0[gi * S + t];
             x[gi * S + t] = xk;
             nrm += xk * xk;
+            if (fabs(xk) > fabs(big)) big = xk;
             x2 = x1;
             x1 = xk;
         }
-        // normalise; final pass also fixes the sign (largest component positive)
-        const double inv = 1.0 / sqrt(nrm);
-        double best = -1.0, sgn = 1.0;
-        for (int k = 0; k < nb; ++k) {
-            const int64_t gi = bs + k;
-            const double v = x[gi * S + t] * inv;
-            x[gi * S + t] = v;
-            if (fabs(v) > best) { best = fabs(v); sgn = (v < 0.0) ? -1.0 : 1.0; }
-        }
-        if (it == 2 && sgn < 0.0)
-            for (int k = 0; k < nb; ++k) x[(int64_t)(bs + k) * S + t] *= -1.0;
+        s = 1.0 / sqrt(nrm);
     }
+    const double f = (big < 0.0) ? -s : s;
+    for (int k = 0; k < nb; ++k) x[(int64_t)(bs + k) * S + t] *= f;
 }
 
 // Modified Gram-Schmidt (two passes) inside clusters of close eigenvalues.
@@ -285,7 +280,7 @@ extern "C" hipError_t bse_launch_inverse_iter(hipStream_t st, int bsz, int cnt, 
                                               double* x)
 {
     const int64_t A = (int64_t)n * bsz;
-    hipLaunchKernelGGL(k_inverse_iter, dim3((unsigned)ceil_div64(cnt, 128)), dim3(128), 0, st, bsz,
+    hipLaunchKernelGGL(k_inverse_iter, dim3((unsigned)ceil_div64(cnt, 64)), dim3(64), 0, st, bsz,
                        cnt, m0, lam, bsv, nbv, d, e, tnorm, scratch, scratch + A, scratch + 2 * A,
                        scratch + 3 * A, scratch + 4 * A, x);
     return hipGetLastError();
