@@ -6,9 +6,11 @@ r0 = 15 A.  A step is one full pass of the hot path over that system with the
 model inputs already resident on the device: per-k TB eig, W/V tables,
 assembly of the N x N TDA matrix, and its eigendecomposition.
 
-  mixed path (the timed metric): FP32-storage assembly, Householder
-  reduction with FP64 reflectors, all eigenvalues, the lowest 16 eigenvectors
-  back-transformed in FP64, oscillator strengths of all excitons;
+  mixed path (the timed metric): compensated BF16 (hi+lo) MFMA assembly with
+  FP32 accumulation and storage, Householder reduction with FP64 reflectors,
+  all eigenvalues, the lowest 16 eigenvectors back-transformed in FP64 and
+  refined by FP64 Rayleigh-Ritz against the matrix recomputed on the fly,
+  oscillator strengths of all excitons;
   FP64 path (reported beside it): FP64 assembly and FP64 reduction with all
   N eigenvectors formed.
 
@@ -85,11 +87,11 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=36, help="k-grid n (N = 16 n^2)")
-    ap.add_argument("--mixed", default="fp32", choices=["fp32", "bf16x2"])
-    ap.add_argument("--refine", action="store_true")
+    ap.add_argument("--mixed", default="bf16x2", choices=["fp32", "bf16x2"])
+    ap.add_argument("--no-refine", dest="refine", action="store_false")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample-grid", type=int, default=16)
+    ap.add_argument("--cpu-sample-grid", type=int, default=13)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",

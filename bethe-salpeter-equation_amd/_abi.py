@@ -24,7 +24,7 @@ ABI_SYMBOLS = (
     "bse_create", "bse_destroy", "bse_last_error", "bse_abi_version", "bse_set_problem",
     "bse_problem_sizes", "bse_tb_states", "bse_build", "bse_get_matrix", "bse_get_dipoles",
     "bse_eigh", "bse_get_eigenvalues", "bse_get_eigenvectors", "bse_oscillator",
-    "bse_spectrum", "bse_eigh_dense", "bse_stage_times", "bse_profile",
+    "bse_spectrum", "bse_eigh_dense", "bse_stage_times", "bse_profile", "bse_apply",
 )
 
 
@@ -78,6 +78,7 @@ def load_library(path: pathlib.Path | str | None = None):
         "bse_eigh_dense": (i32, [vp, i32, dp, i64, i32, i32, i32, dp, dp, i64]),
         "bse_stage_times": (i32, [vp, dp, i32]),
         "bse_profile": (i32, [vp, i32, dp]),
+        "bse_apply": (i32, [vp, i32, dp, i64, dp, i64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -191,6 +192,17 @@ class Context:
         self._check(self.lib.bse_get_dipoles(self.h, out.ctypes.data_as(C.POINTER(C.c_double))),
                     "bse_get_dipoles")
         return out
+
+    def apply(self, x):
+        """Matrix-free FP64 product H x (reference order); x: (N,) or (N, k)."""
+        x = np.asarray(x, dtype=np.complex128)
+        vec = x.ndim == 1
+        xf = np.asfortranarray(x.reshape(x.shape[0], -1))
+        n, k = xf.shape
+        y = np.zeros((n, k), dtype=np.complex128, order="F")
+        self._check(self.lib.bse_apply(self.h, k, xf.ctypes.data_as(C.POINTER(C.c_double)), n,
+                                       y.ctypes.data_as(C.POINTER(C.c_double)), n), "bse_apply")
+        return y[:, 0] if vec else y
 
     def eigh(self, n_vectors=-1, refine=False):
         self._check(self.lib.bse_eigh(self.h, int(n_vectors), int(bool(refine))), "bse_eigh")

@@ -1,0 +1,275 @@
+// K4 on matrix cores: the direct term of the TDA matrix as two chained complex
+// contractions per (k, k') pair, realified onto 16x16 MFMA tiles.
+//
+//   step 1:  A_p[(c,c'), j] = sum_i X_p[(c,c'), i] W_ij(k - k')
+//            X_p[(c,c'), i] = conj(C[k,c,i]) C[k',c',i]
+//   step 2:  D_p[(c,c'), (v,v')] = sum_j A_p[(c,c'), j] Y_p[(v,v'), j]
+//            Y_p[(v,v'), j] = V[k,v,j] conj(V[k',v',j])
+//   H[(c,v,k), (c',v',k')] = alpha D_p + exchange + diagonal     (2796-2897)
+//
+// Realification (K = 16 real for norb <= 8):
+//   [Ar | Ai] = [Xr | Xi] [[Wr, Wi], [-Wi, Wr]]
+//   Dr = [Ar | Ai] [Yr | -Yi]^T,   Di = [Ar | Ai] [Yi | Yr]^T
+// so one pair costs 3 real 16x16x16 GEMMs: 12 v_mfma_*_16x16x4 (FP64 or FP32)
+// or, in the compensated BF16 mode, 6 v_mfma_f32_16x16x32_bf16 where every
+// FP32 operand is split a = a_hi + a_lo (bf16) and the three significant
+// products a_hi b_hi + a_hi b_lo + a_lo b_hi are laid end to end along K
+// (3 x 16 = 48, padded to 64), accumulated in FP32.
+//
+// Shapes: nc*nc <= 16, nv*nv <= 16, norb <= 8 (padded with zeros).  Each
+// wave processes whole pairs; the step-1 result is transposed through a
+// per-wave LDS tile into the A-operand layout of step 2.  The exchange
+// (rank-norb, sum_j PV[r,j] conj(P[s,j])) and the diagonal are added in the
+// epilogue with VALU FP64 arithmetic.
+#include "bse_common.hpp"
+
+#define PREC_F64 0
+#define PREC_F32 1
+#define PREC_BF16 2
+
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mf_b8 __attribute__((ext_vector_type(8)));
+
+template <int P> struct mf_t;
+template <> struct mf_t<PREC_F64> {
+    typedef double real; typedef mf_d4 acc; typedef double2 out;
+    __device__ static inline int row(int lane, int r) { return mfma16_row_f64(lane, r); }
+};
+template <> struct mf_t<PREC_F32> {
+    typedef float real; typedef mf_f4 acc; typedef float2 out;
+    __device__ static inline int row(int lane, int r) { return mfma16_row_f32(lane, r); }
+};
+template <> struct mf_t<PREC_BF16> {
+    typedef float real; typedef mf_f4 acc; typedef float2 out;
+    __device__ static inline int row(int lane, int r) { return mfma16_row_f32(lane, r); }
+};
+
+// One real 16x16x16 product  acc += A(16x16) B(16x16).  a[ks], b[ks] hold the
+// lane's operands for K step ks (A[l%16][4ks + l/16], B[4ks + l/16][l%16]).
+__device__ inline mf_d4 mm16(const double* a, const double* b, mf_d4 acc) {
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], b[ks], acc, 0, 0, 0);
+    return acc;
+}
+__device__ inline mf_f4 mm16(const float* a, const float* b, mf_f4 acc) {
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], b[ks], acc, 0, 0, 0);
+    return acc;
+}
+
+// Compensated BF16 product.  The operands are the lane's 16 K values in the
+// f32 layout (a[ks] = A[l%16][4ks + l/16]); they are regathered through LDS
+// into the bf16 layout (8 consecutive K per lane).
+__device__ inline mf_f4 mm16_bf16x2(const float* a, const float* b, mf_f4 acc, float* sbuf)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, q = lane >> 4;
+    // sbuf: A[16][17] then B[16][17] (row = i for A, col = j for B)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+        sbuf[i * 17 + 4 * ks + q] = a[ks];
+        sbuf[272 + i * 17 + 4 * ks + q] = b[ks];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // K' = [hi*hi (0..15) | hi*lo (16..31) | lo*hi (32..47) | 0 (48..63)]
+    mf_b8 A0, B0, A1, B1;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int k0 = 8 * q + t;          // first instruction: K' in [0, 32)
+        const int kk = k0 & 15;
+        const float av = sbuf[i * 17 + kk];
+        const float bv = sbuf[272 + i * 17 + kk];
+        const __bf16 ah = (__bf16)av, bh = (__bf16)bv;
+        const __bf16 bl = (__bf16)(bv - (float)bh);
+        A0[t] = ah;
+        B0[t] = (k0 < 16) ? bh : bl;
+        const int k1 = 32 + 8 * q + t;     // second instruction: K' in [32, 64)
+        const __bf16 al = (__bf16)(av - (float)ah);
+        A1[t] = (k1 < 48) ? al : (__bf16)0.0f;
+        B1[t] = (k1 < 48) ? bh : (__bf16)0.0f;
+    }
+    __builtin_amdgcn_wave_barrier();
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, B0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, B1, acc, 0, 0, 0);
+    return acc;
+}
+
+struct MfArgs {
+    int nk, n1, n2, nv, nc, norb, TK;
+    int64_t N, ld;
+    const double2* coef;
+    const double2* wtab;
+    const double2* PV;
+    const double2* P;
+    const double* dE;
+    double alpha, beta;
+};
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_assemble_mfma(MfArgs a, typename mf_t<PREC>::out* __restrict__ H)
+{
+    typedef typename mf_t<PREC>::real R;
+    typedef typename mf_t<PREC>::acc ACC;
+    typedef typename mf_t<PREC>::out OUT;
+    extern __shared__ double2 smem[];
+    const int B = a.nc * a.nv, ns = a.nv + a.nc, no = a.norb, TK = a.TK;
+    const int TR = TK * B;
+    int ti, tj;
+    {
+        const int64_t idx = blockIdx.x;
+        int64_t r = (int64_t)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+        while (r * (r + 1) / 2 > idx) --r;
+        while ((r + 1) * (r + 2) / 2 <= idx) ++r;
+        ti = (int)r;
+        tj = (int)(idx - r * (r + 1) / 2);
+    }
+    const int k0 = ti * TK, kp0 = tj * TK;
+    double2* sCr = smem;                       // [TK][ns][8]  row k-points, all slots
+    double2* sCc = sCr + TK * ns * 8;          // [TK][ns][8]  column k-points
+    double2* sPV = sCc + TK * ns * 8;          // [TR][8]
+    double2* sP = sPV + TR * 8;                // [TR][8]
+    R* sT = (R*)(sP + TR * 8);                 // per-wave 16x17 (x2 for bf16) transposition
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int e = tid; e < TK * ns * 8; e += 256) {
+        const int i = e & 7, sl = (e >> 3) % ns, kk = e / (8 * ns);
+        const int k = k0 + kk, kp = kp0 + kk;
+        sCr[e] = (k < a.nk && i < no) ? a.coef[((size_t)k * ns + sl) * no + i] : cmk(0, 0);
+        sCc[e] = (kp < a.nk && i < no) ? a.coef[((size_t)kp * ns + sl) * no + i] : cmk(0, 0);
+    }
+    for (int e = tid; e < TR * 8; e += 256) {
+        const int i = e & 7, rr = e >> 3;
+        const int64_t row = (int64_t)k0 * B + rr, col = (int64_t)kp0 * B + rr;
+        sPV[e] = (row < a.N && i < no) ? a.PV[row * no + i] : cmk(0, 0);
+        sP[e] = (col < a.N && i < no) ? a.P[col * no + i] : cmk(0, 0);
+    }
+    __syncthreads();
+    R* tw = sT + wv * (PREC == PREC_BF16 ? 544 : 272);
+    const int li = lane & 15, lq = lane >> 4;
+    const int w2 = 2 * a.n2 - 1;
+    for (int p = wv; p < TK * TK; p += 4) {
+        const int kk = p / TK, kq = p % TK;
+        const int k = k0 + kk, kp = kp0 + kq;
+        if (k >= a.nk || kp >= a.nk) continue;
+        const double2* wq = a.wtab + ((size_t)((k / a.n2 - kp / a.n2 + a.n1 - 1) * w2 +
+                                               (k % a.n2 - kp % a.n2 + a.n2 - 1))) * no * no;
+        // ---- step 1 operands: A1[rho][kappa] (rho = c*nc + c'), B1[kappa][j']
+        R a1[4], b1[4];
+        const int rho = li, c = rho / a.nc, cp = rho % a.nc;
+        const bool rok = rho < a.nc * a.nc;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const int kap = 4 * ks + lq;
+            const int i = kap & 7;
+            double2 x = cmk(0, 0);
+            if (rok && i < no)
+                x = cmulc(sCr[(kk * ns + a.nv + c) * 8 + i], sCc[(kq * ns + a.nv + cp) * 8 + i]);
+            a1[ks] = (R)(kap < 8 ? x.x : x.y);
+            const int jp = li;                        // output column of [Ar | Ai]
+            const int jj = jp & 7, ii = kap & 7;
+            double wv2 = 0.0;
+            if (ii < no && jj < no) {
+                const double2 w = wq[ii * no + jj];
+                if (kap < 8) wv2 = (jp < 8) ? w.x : w.y;
+                else wv2 = (jp < 8) ? -w.y : w.x;
+            }
+            b1[ks] = (R)wv2;
+        }
+        ACC d1 = ACC{0, 0, 0, 0};
+        if constexpr (PREC == PREC_BF16) d1 = mm16_bf16x2((const float*)a1, (const float*)b1, d1, (float*)tw);
+        else d1 = mm16(a1, b1, d1);
+        // ---- transpose D1 (lane: row mf_t::row(lane, reg), col li) into the A2
+        // layout (row li, col 4ks + lq)
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int reg = 0; reg < 4; ++reg) tw[mf_t<PREC>::row(lane, reg) * 17 + li] = d1[reg];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        R a2[4], b2r[4], b2i[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) a2[ks] = tw[li * 17 + 4 * ks + lq];
+        __builtin_amdgcn_wave_barrier();
+        // ---- step 2 operands: Y[sigma][j], sigma = v*nv + v'
+        const int sg = li, v = sg / a.nv, vp = sg % a.nv;
+        const bool sok = sg < a.nv * a.nv;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const int kap = 4 * ks + lq;
+            const int j = kap & 7;
+            double2 y = cmk(0, 0);
+            if (sok && j < no)
+                y = cmul(sCr[(kk * ns + v) * 8 + j], cconj(sCc[(kq * ns + vp) * 8 + j]));
+            b2r[ks] = (R)(kap < 8 ? y.x : -y.y);
+            b2i[ks] = (R)(kap < 8 ? y.y : y.x);
+        }
+        ACC dr = ACC{0, 0, 0, 0}, di = ACC{0, 0, 0, 0};
+        if constexpr (PREC == PREC_BF16) {
+            dr = mm16_bf16x2((const float*)a2, (const float*)b2r, dr, (float*)tw);
+            di = mm16_bf16x2((const float*)a2, (const float*)b2i, di, (float*)tw);
+        } else {
+            dr = mm16(a2, b2r, dr);
+            di = mm16(a2, b2i, di);
+        }
+        // ---- epilogue: lane holds D[rho = mf_t::row(lane, reg)][sigma = li]
+#pragma unroll
+        for (int reg = 0; reg < 4; ++reg) {
+            const int rh = mf_t<PREC>::row(lane, reg);
+            if (rh >= a.nc * a.nc || !sok) continue;
+            const int cc = rh / a.nc, ccp = rh % a.nc;
+            const int rl = kk * B + cc * a.nv + v, cl = kq * B + ccp * a.nv + vp;
+            const int64_t grow = (int64_t)k0 * B + rl, gcol = (int64_t)kp0 * B + cl;
+            double2 h = cmk(a.alpha * (double)dr[reg], a.alpha * (double)di[reg]);
+            double2 x = cmk(0, 0);
+            for (int j = 0; j < no; ++j) cfma(x, sPV[rl * 8 + j], cconj(sP[cl * 8 + j]));
+            h = cadd(h, cscale(x, a.beta));
+            if (grow == gcol) { h.x += a.dE[grow]; h.y = 0.0; }
+            cplx<OUT>::store(H + grow + gcol * a.ld, h);
+        }
+    }
+}
+
+extern "C" int bse_assemble_mfma_supported(int nv, int nc, int norb) {
+    return nc * nc <= 16 && nv * nv <= 16 && norb <= 8 && nv * nc >= 4;
+}
+
+extern "C" int bse_assemble_mfma_tk(int nv, int nc) {
+    const int B = nv * nc;
+    int tk = 1;
+    while (tk * 2 * B <= 64) tk *= 2;
+    return tk;
+}
+
+static size_t mfma_lds_bytes(int TK, int nv, int nc, int prec) {
+    const int ns = nv + nc, B = nv * nc;
+    const size_t c2 = (size_t)2 * TK * ns * 8 + (size_t)2 * TK * B * 8;
+    const size_t rb = (prec == PREC_F64 ? 8 : 4) * (size_t)4 * (prec == PREC_BF16 ? 544 : 272);
+    return c2 * sizeof(double2) + rb;
+}
+
+extern "C" hipError_t bse_launch_assemble_mfma(hipStream_t st, int prec, int nk, int n1, int n2,
+                                               int nv, int nc, int norb, int64_t ld,
+                                               const double2* coef, const double2* wtab,
+                                               const double2* PV, const double2* P,
+                                               const double* dE, double alpha, double beta,
+                                               void* H)
+{
+    if (!bse_assemble_mfma_supported(nv, nc, norb)) return hipErrorInvalidValue;
+    MfArgs a;
+    a.nk = nk; a.n1 = n1; a.n2 = n2; a.nv = nv; a.nc = nc; a.norb = norb;
+    a.TK = bse_assemble_mfma_tk(nv, nc);
+    a.N = (int64_t)nk * nv * nc; a.ld = ld;
+    a.coef = coef; a.wtab = wtab; a.PV = PV; a.P = P; a.dE = dE; a.alpha = alpha; a.beta = beta;
+    const int64_t nt = ceil_div64(nk, a.TK);
+    const dim3 grid((unsigned)(nt * (nt + 1) / 2));
+    const size_t lds = mfma_lds_bytes(a.TK, nv, nc, prec);
+    if (prec == PREC_F64)
+        hipLaunchKernelGGL(k_assemble_mfma<PREC_F64>, grid, dim3(256), lds, st, a, (double2*)H);
+    else if (prec == PREC_F32)
+        hipLaunchKernelGGL(k_assemble_mfma<PREC_F32>, grid, dim3(256), lds, st, a, (float2*)H);
+    else
+        hipLaunchKernelGGL(k_assemble_mfma<PREC_BF16>, grid, dim3(256), lds, st, a, (float2*)H);
+    return hipGetLastError();
+}

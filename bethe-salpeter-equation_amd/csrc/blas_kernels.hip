@@ -255,3 +255,27 @@ extern "C" hipError_t bse_launch_convert_to_fp32(hipStream_t st, int64_t n, cons
                        src, dst);
     return hipGetLastError();
 }
+
+// Inverse of k_rows_to_reference: dst[r', j] = src[ref(r'), j].
+__global__ __launch_bounds__(256) void k_rows_from_reference(int64_t N, int B, int nk, int ncol,
+                                                            const double2* __restrict__ src,
+                                                            int64_t lds, double2* __restrict__ dst,
+                                                            int64_t ldd)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (r >= N || j >= ncol) return;
+    const int64_t rr = (r % B) * nk + r / B;
+    dst[r + (int64_t)j * ldd] = src[rr + (int64_t)j * lds];
+}
+
+extern "C" hipError_t bse_launch_rows_from_reference(hipStream_t st, int64_t N, int B, int nk,
+                                                     int ncol, const double2* src, int64_t lds,
+                                                     double2* dst, int64_t ldd)
+{
+    if (ncol <= 0) return hipSuccess;
+    dim3 grid((unsigned)ceil_div64(N, 256), (unsigned)ncol);
+    hipLaunchKernelGGL(k_rows_from_reference, grid, dim3(256), 0, st, N, B, nk, ncol, src, lds, dst,
+                       ldd);
+    return hipGetLastError();
+}

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -57,6 +58,21 @@ hipError_t bse_launch_dielectric(hipStream_t, int, const double*, double, int, c
 hipError_t bse_launch_rows_to_reference(hipStream_t, int64_t, int, int, int, const double2*,
                                         int64_t, double2*, int64_t);
 hipError_t bse_launch_convert_to_fp32(hipStream_t, int64_t, const double2*, float2*);
+hipError_t bse_launch_rows_from_reference(hipStream_t, int64_t, int, int, int, const double2*,
+                                          int64_t, double2*, int64_t);
+int bse_apply_pick_tk(int, int, int);
+int bse_assemble_mfma_supported(int, int, int);
+hipError_t bse_launch_assemble_mfma(hipStream_t, int, int, int, int, int, int, int, int64_t,
+                                    const double2*, const double2*, const double2*, const double2*,
+                                    const double*, double, double, void*);
+hipError_t bse_launch_apply(hipStream_t, int, int, int, int, int, int, int, const double2*,
+                            const double2*, const double2*, const double2*, const double2*,
+                            const double2*, const double2*, const double*, double, double, int,
+                            const double2*, int64_t, double2*, int64_t);
+hipError_t bse_launch_small_rr(hipStream_t, int, int, const double2*, const double2*, double,
+                               double*, double2*, int*);
+hipError_t bse_launch_residual(hipStream_t, int64_t, int, const double2*, const double2*, int64_t,
+                               double2*, double*);
 }
 
 static const double kC = (1.602176634 * 1000.0) / 8.8541878176;   // e^2/eps0, eV*A
@@ -100,6 +116,8 @@ struct bse_ctx {
     bool prof = false;
     std::vector<hipEvent_t> prof_ev;
     double hemv_ms = 0.0, hemv_bytes = 0.0;
+    bool force_valu_assembly = false;   // BSE_ASSEMBLY=valu selects the VALU tiles
+    int asm_kernel = 0;                 // 0 VALU, 1 FP64 MFMA, 2 FP32 MFMA, 3 BF16x2 MFMA
     int64_t hemv_launches = 0;
 };
 
@@ -109,7 +127,8 @@ enum Slot {
     S_RCART, S_HR, S_TAU, S_KPTS, S_GVEC, S_Q, S_EPS, S_COEF, S_WTAB, S_VX, S_P, S_PV, S_RHO, S_DE,
     S_H, S_D, S_E, S_TAUH, S_VSTORE, S_WP, S_ACOL, S_YR, S_YC, S_T12, S_PART, S_SCAL, S_COUNTER,
     S_TALL, S_GRAM, S_E2, S_ITEMS, S_BOUNDS, S_LAMI, S_LAM, S_BSV, S_NBV, S_CLUST, S_IISCR, S_IIX,
-    S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_ALPHA2, S_NSLOTS
+    S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_ALPHA2,
+    S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL, S_NSLOTS
 };
 
 int fail(bse_ctx* c, int code, const std::string& msg) {
@@ -195,6 +214,8 @@ extern "C" int bse_create(bse_ctx** out, int device) {
         return BSE_ERR_HIP;
     }
     for (int i = 0; i <= ST_N; ++i) (void)hipEventCreate(&ctx->ev[i]);
+    const char* am = getenv("BSE_ASSEMBLY");
+    ctx->force_valu_assembly = am && strcmp(am, "valu") == 0;
     *out = ctx;
     return BSE_OK;
 }
@@ -410,10 +431,22 @@ extern "C" int bse_build(bse_ctx* ctx, int precision) {
     const double beta = s.f_s * s.exchange_scale / (ctx->area * ctx->nk);
     const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
     stage_begin(ctx, ST_ASM);
-    HCK(bse_launch_assemble(ctx->st, fp32, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb, ctx->TK,
-                            ctx->ldH, coef, coef, coef, coef, (const double2*)ctx->bufs[S_WTAB].p,
-                            (const double2*)ctx->bufs[S_PV].p, (const double2*)ctx->bufs[S_P].p,
-                            (const double*)ctx->bufs[S_DE].p, alpha, beta, H, 1));
+    if (!ctx->force_valu_assembly && bse_assemble_mfma_supported(s.nv, s.nc, ctx->norb)) {
+        // matrix cores: FP64 MFMA (FP64 storage), FP32 MFMA or compensated BF16 MFMA (FP32 storage)
+        HCK(bse_launch_assemble_mfma(ctx->st, precision, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb,
+                                     ctx->ldH, coef, (const double2*)ctx->bufs[S_WTAB].p,
+                                     (const double2*)ctx->bufs[S_PV].p,
+                                     (const double2*)ctx->bufs[S_P].p,
+                                     (const double*)ctx->bufs[S_DE].p, alpha, beta, H));
+        ctx->asm_kernel = 1 + precision;
+    } else {
+        // small band windows (e.g. 1v+1c): FP64 VALU tiles, FP64 or FP32 storage
+        HCK(bse_launch_assemble(ctx->st, fp32, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb, ctx->TK,
+                                ctx->ldH, coef, coef, coef, coef, (const double2*)ctx->bufs[S_WTAB].p,
+                                (const double2*)ctx->bufs[S_PV].p, (const double2*)ctx->bufs[S_P].p,
+                                (const double*)ctx->bufs[S_DE].p, alpha, beta, H, 1));
+        ctx->asm_kernel = 0;
+    }
     stage_end(ctx, ST_ASM);
     ctx->precision = precision;
     ctx->built = true;
@@ -684,16 +717,103 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32
     return BSE_OK;
 }
 
+// ---------------------------------------------------------------------------
+// K8: FP64 Rayleigh-Ritz refinement of the lowest k eigenpairs against the
+// FP64 matrix recomputed on the fly (no stored FP64 matrix).  Basis
+// S = [X, R] with R the normalised residuals H X - X diag(x^H H x); Ritz
+// pairs of S by the SVQB-stabilised small eigenproblem; iters rounds.
+// ---------------------------------------------------------------------------
+static int apply_H(bse_ctx* ctx, int k, const double2* X, int64_t ldx, double2* Y, int64_t ldy) {
+    const bse_system& s = ctx->sys;
+    const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
+    const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
+    const double beta = s.f_s * s.exchange_scale / (ctx->area * ctx->nk);
+    HCK(hipMemsetAsync(Y, 0, sizeof(double2) * ldy * k, ctx->st));
+    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb,
+                         bse_apply_pick_tk(s.nv, s.nc, ctx->norb), coef, coef, coef, coef,
+                         (const double2*)ctx->bufs[S_WTAB].p, (const double2*)ctx->bufs[S_PV].p,
+                         (const double2*)ctx->bufs[S_P].p, (const double*)ctx->bufs[S_DE].p, alpha,
+                         beta, k, X, ldx, Y, ldy));
+    return BSE_OK;
+}
+
+static int refine_lowest(bse_ctx* ctx, int k, int iters) {
+    const int64_t N = ctx->N;
+    const int m = 2 * k;
+    hipStream_t st = ctx->st;
+    double2* Z = (double2*)ctx->bufs[S_Z].p;               // N x nvec, internal order
+    double2* S = BUF(double2, S_RS, (size_t)N * m);        // [X | R]
+    double2* HS = BUF(double2, S_RHS, (size_t)N * m);      // [HX | HR]
+    double2* T = BUF(double2, S_RX, (size_t)N * 2 * k);    // new X, new HX
+    double2* Gm = BUF(double2, S_RG, (size_t)m * m);
+    double2* Hm = BUF(double2, S_RHG, (size_t)m * m);
+    double2* Cm = BUF(double2, S_RC, (size_t)m * k);
+    double* small = BUF(double, S_RSMALL, 4 * k + 8);      // evals k, rnorm k, info
+    double2* skw = (double2*)ctx->bufs[S_SPLITK].p;
+    const int64_t sk_el = (int64_t)(ctx->bufs[S_SPLITK].cap / sizeof(double2));
+    HCK(hipMemcpyAsync(S, Z, sizeof(double2) * N * k, hipMemcpyDeviceToDevice, st));
+    int rc = apply_H(ctx, k, S, N, HS, N);
+    if (rc) return rc;
+    for (int it = 0; it < iters; ++it) {
+        HCK(bse_launch_residual(st, N, k, S, HS, N, S + N * k, small + k));
+        rc = apply_H(ctx, k, S + N * k, N, HS + N * k, N);
+        if (rc) return rc;
+        HCK(bse_zgemm(st, 1, 0, m, m, (int)N, 1, 0, S, N, S, N, 0, 0, Gm, m, skw, sk_el));
+        HCK(bse_zgemm(st, 1, 0, m, m, (int)N, 1, 0, S, N, HS, N, 0, 0, Hm, m, skw, sk_el));
+        HCK(bse_launch_small_rr(st, m, k, Gm, Hm, 1e-10, small, Cm, (int*)(small + 2 * k)));
+        HCK(bse_zgemm(st, 0, 0, (int)N, k, m, 1, 0, S, N, Cm, m, 0, 0, T, N, nullptr, 0));
+        HCK(bse_zgemm(st, 0, 0, (int)N, k, m, 1, 0, HS, N, Cm, m, 0, 0, T + N * k, N, nullptr, 0));
+        HCK(hipMemcpyAsync(S, T, sizeof(double2) * N * k, hipMemcpyDeviceToDevice, st));
+        HCK(hipMemcpyAsync(HS, T + N * k, sizeof(double2) * N * k, hipMemcpyDeviceToDevice, st));
+    }
+    HCK(hipMemcpyAsync(Z, S, sizeof(double2) * N * k, hipMemcpyDeviceToDevice, st));
+    HCK(hipMemcpyAsync(ctx->bufs[S_EVAL].p, small, sizeof(double) * k, hipMemcpyDeviceToDevice, st));
+    // oscillator strengths of the refined vectors: f^T = rho^H X
+    HCK(bse_zgemm(st, 1, 0, 1, k, (int)N, 1, 0, (const double2*)ctx->bufs[S_RHO].p, N, S, N, 0, 0,
+                  (double2*)ctx->bufs[S_F].p, 1, skw, sk_el));
+    HCK(hipStreamSynchronize(st));
+    return BSE_OK;
+}
+
+// Matrix-free FP64 product Y = H X with the loaded problem (reference order).
+extern "C" int bse_apply(bse_ctx* ctx, int k, const double* x, int64_t ldx, double* y, int64_t ldy) {
+    if (!ctx || !x || !y || k <= 0) return BSE_ERR_ARG;
+    if (!ctx->have_tables) return fail(ctx, BSE_ERR_STATE, "build the problem first");
+    const int64_t N = ctx->N;
+    if (ldx < N || ldy < N) return fail(ctx, BSE_ERR_ARG, "leading dimension < N");
+    (void)hipSetDevice(ctx->device);
+    double2* xin = BUF(double2, S_RX, (size_t)N * 2 * k);
+    double2* tmp = xin + N * k;
+    double2* yout = BUF(double2, S_RHS, (size_t)N * k);
+    HCK(hipMemcpy2DAsync(tmp, sizeof(double2) * N, x, sizeof(double2) * ldx, sizeof(double2) * N, k,
+                         hipMemcpyHostToDevice, ctx->st));
+    // reference order -> internal order: xin[r'] = x[ref(r')]
+    HCK(bse_launch_rows_from_reference(ctx->st, N, ctx->B, ctx->nk, k, tmp, N, xin, N));
+    int rc = apply_H(ctx, k, xin, N, yout, N);
+    if (rc) return rc;
+    HCK(bse_launch_rows_to_reference(ctx->st, N, ctx->B, ctx->nk, k, yout, N, tmp, N));
+    HCK(hipMemcpy2DAsync(y, sizeof(double2) * ldy, tmp, sizeof(double2) * N, sizeof(double2) * N, k,
+                         hipMemcpyDeviceToHost, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    return BSE_OK;
+}
+
 extern "C" int bse_eigh(bse_ctx* ctx, int n_vectors, int refine) {
     if (!ctx) return BSE_ERR_ARG;
     if (!ctx->built) return fail(ctx, BSE_ERR_STATE, "matrix not built");
     if (ctx->have_eig) return fail(ctx, BSE_ERR_STATE, "matrix already reduced; build again");
-    if (refine) return fail(ctx, BSE_ERR_UNSUPPORTED, "refinement not available in this build");
+    if (refine && (n_vectors <= 0 || n_vectors > 16))
+        return fail(ctx, BSE_ERR_UNSUPPORTED, "refinement needs 1 <= n_vectors <= 16");
     (void)hipSetDevice(ctx->device);
     const int fp32 = ctx->precision != BSE_PREC_FP64;
     ctx->eig_on_bse = true;
-    return eig_pipeline(ctx, ctx->N, ctx->bufs[S_H].p, ctx->ldH, fp32,
-                        (const double2*)ctx->bufs[S_RHO].p, n_vectors);
+    int rc = eig_pipeline(ctx, ctx->N, ctx->bufs[S_H].p, ctx->ldH, fp32,
+                          (const double2*)ctx->bufs[S_RHO].p, n_vectors);
+    if (rc || !refine) return rc;
+    stage_begin(ctx, ST_REF);
+    rc = refine_lowest(ctx, n_vectors, 2);
+    stage_end(ctx, ST_REF);
+    return rc;
 }
 
 extern "C" int bse_get_eigenvalues(bse_ctx* ctx, double* evals) {

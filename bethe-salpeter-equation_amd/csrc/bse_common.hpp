@@ -109,20 +109,31 @@ __device__ inline double2 block_sum2(double2 v, double* sh /* >= 32 doubles */) 
 // "Last block done" ticket: returns true in exactly one block (the last to
 // arrive) after every block has published its partial results.  The counter
 // is a 32-bit device word that the last block resets to zero.
+// Every thread releases its stores at agent scope; only the last block
+// acquires, so the other blocks never invalidate their XCD's L2.
 __device__ inline bool last_block_ticket(unsigned int* counter, unsigned int nblocks) {
     __shared__ bool is_last;
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned int prev = atomicAdd(counter, 1u);
+        const unsigned int prev =
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         is_last = (prev == nblocks - 1);
     }
     __syncthreads();
     if (is_last) {
-        __threadfence();
-        if (threadIdx.x == 0) atomicExch(counter, 0u);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (threadIdx.x == 0)
+            __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     return is_last;
 }
+
+// Output-row layout of the 16x16x4 MFMA tiles (measured on gfx950): lane l,
+// accumulator register r holds D[row][l % 16] with
+//   f32 (v_mfma_f32_16x16x4, and the bf16 16x16x32 variants): row = 4 (l/16) + r
+//   f64 (v_mfma_f64_16x16x4):                                  row = (l/16) + 4 r
+__device__ inline int mfma16_row_f32(int lane, int r) { return 4 * (lane >> 4) + r; }
+__device__ inline int mfma16_row_f64(int lane, int r) { return (lane >> 4) + 4 * r; }
 
 __host__ __device__ inline int64_t ceil_div64(int64_t a, int64_t b) { return (a + b - 1) / b; }

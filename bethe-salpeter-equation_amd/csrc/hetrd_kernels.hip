@@ -22,6 +22,7 @@
 // All reductions are deterministic: per-block partials reduced in a fixed
 // order by the last block to finish ("ticket" counter).
 #include "bse_common.hpp"
+#include <cstdlib>
 
 struct HetrdCol {
     int n;            // matrix size
@@ -370,6 +371,143 @@ __global__ __launch_bounds__(256) void k_her2k_lower(int n2, T* __restrict__ A, 
         }
 }
 
+// MFMA form of the trailing update, A22 -= V W^H + W V^H, lower 64x64 tiles.
+// With P = [V W] and Q = [W V] (complex, K = 2 nb columns) and the real
+// concatenations Pcat = [Re P | Im P], Pimc = [Im P | -Re P], Qcat = [Re Q | Im Q]:
+//   Re(P Q^H) = Pcat Qcat^T,   Im(P Q^H) = Pimc Qcat^T.
+// The kernel computes the transposed products Qcat Pcat^T so that the 16 lanes
+// holding one MFMA output column map to 16 consecutive rows of the
+// column-major matrix (coalesced read-modify-write).  FP64 storage uses
+// v_mfma_f64_16x16x4 with FP64 operands; FP32 storage uses v_mfma_f32_16x16x4
+// with the FP64 panels rounded to FP32 when staged into LDS.
+typedef double her2k_d4 __attribute__((ext_vector_type(4)));
+typedef float her2k_f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct her2k_traits;
+template <> struct her2k_traits<double2> {
+    typedef double real;
+    typedef her2k_d4 acc;
+    __device__ static inline acc mfma(real a, real b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int lane, int r) { return mfma16_row_f64(lane, r); }
+};
+template <> struct her2k_traits<float2> {
+    typedef float real;
+    typedef her2k_f4 acc;
+    __device__ static inline acc mfma(real a, real b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int lane, int r) { return mfma16_row_f32(lane, r); }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_her2k_mfma(int n2, T* __restrict__ A, int64_t lda,
+                                                   const double2* __restrict__ V,
+                                                   const double2* __restrict__ W, int64_t ldv,
+                                                   int nb, const double2* __restrict__ alpha2)
+{
+    typedef typename her2k_traits<T>::real R;
+    typedef typename her2k_traits<T>::acc ACC;
+    int ti, tj;
+    {
+        const int64_t idx = blockIdx.x;
+        int64_t r = (int64_t)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+        while (r * (r + 1) / 2 > idx) --r;
+        while ((r + 1) * (r + 2) / 2 <= idx) ++r;
+        ti = (int)r;
+        tj = (int)(idx - r * (r + 1) / 2);
+    }
+    // chunk of 16 complex K columns: real and imaginary parts, [kcol][row]
+    __shared__ R sPr[16][64], sPi[16][64], sQr[16][64], sQi[16][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r0 = ti * 64, c0 = tj * 64;
+    const int rowbase = (wv >> 1) * 32, colbase = (wv & 1) * 32;
+    ACC accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accR[a][b] = ACC{0, 0, 0, 0};
+            accI[a][b] = ACC{0, 0, 0, 0};
+        }
+    const int K2 = 2 * nb;                    // complex K columns of [V W]
+    for (int l0 = 0; l0 < K2; l0 += 16) {
+        for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+            const int rr = e & 63, kc = e >> 6;
+            const int kk = l0 + kc;           // column of P = [V W], Q = [W V]
+            double2 pv = cmk(0, 0), qv = cmk(0, 0);
+            if (kk < K2) {
+                const bool first = kk < nb;
+                const int l = first ? kk : kk - nb;
+                const double2 a2 = alpha2[l];
+                const int gr = r0 + rr, gc = c0 + rr;
+                if (gr < n2) {
+                    const double2 v = V[gr + (int64_t)l * ldv];
+                    const double2 w = cadd(W[gr + (int64_t)l * ldv], cmul(a2, v));
+                    pv = first ? v : w;
+                }
+                if (gc < n2) {
+                    const double2 v = V[gc + (int64_t)l * ldv];
+                    const double2 w = cadd(W[gc + (int64_t)l * ldv], cmul(a2, v));
+                    qv = first ? w : v;
+                }
+            }
+            sPr[kc][rr] = (R)pv.x;
+            sPi[kc][rr] = (R)pv.y;
+            sQr[kc][rr] = (R)qv.x;
+            sQi[kc][rr] = (R)qv.y;
+        }
+        __syncthreads();
+        // real K index kappa in [0, 32): < 16 real parts, >= 16 imaginary parts
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+            const int kap = ks * 4 + (lane >> 4);
+            const int kc = kap & 15;
+            const bool re = kap < 16;
+            R bP[2], bPi[2], aQ[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int row = rowbase + a * 16 + (lane & 15);
+                bP[a] = re ? sPr[kc][row] : sPi[kc][row];
+                bPi[a] = re ? sPi[kc][row] : -sPr[kc][row];
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int col = colbase + b * 16 + (lane & 15);
+                aQ[b] = re ? sQr[kc][col] : sQi[kc][col];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accR[a][b] = her2k_traits<T>::mfma(aQ[b], bP[a], accR[a][b]);
+                    accI[a][b] = her2k_traits<T>::mfma(aQ[b], bPi[a], accI[a][b]);
+                }
+        }
+        __syncthreads();
+    }
+    // lane holds D'[i][j = lane % 16] with i = her2k_traits<T>::row(lane, reg):
+    // column i, row j of the subtile
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int reg = 0; reg < 4; ++reg) {
+                const int gr = r0 + rowbase + a * 16 + (lane & 15);
+                const int gc = c0 + colbase + b * 16 + her2k_traits<T>::row(lane, reg);
+                if (gr < n2 && gc < n2 && gr >= gc) {
+                    T* p = A + gr + (int64_t)gc * lda;
+                    double2 x = cplx<T>::load(p);
+                    x.x -= (double)accR[a][b][reg];
+                    x.y -= (double)accI[a][b][reg];
+                    if (gr == gc) x.y = 0.0;
+                    cplx<T>::store(p, x);
+                }
+            }
+}
+
 __global__ void k_last_diag(int n, const void* A, int64_t lda, int fp32, double* d)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -385,6 +523,7 @@ extern "C" hipError_t bse_launch_larft(hipStream_t st, int nb, const double2* ta
                                        double2* T, int ldt);
 
 struct HetrdWork {
+    int her2k_valu;      // 1: VALU trailing update (BSE_HER2K=valu), 0: MFMA
     hipEvent_t* prof_ev; // optional: 2 events per column around the hemv launch
     double2* Vstore;     // packed panel reflectors
     int64_t* voff;       // host array: offset of panel p in Vstore
@@ -453,10 +592,16 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
         const int n2 = m - nbp;
         if (n2 > 0) {
             const int64_t nt2 = ceil_div64(n2, 64);
-            hipLaunchKernelGGL(k_her2k_lower<T>, dim3((unsigned)(nt2 * (nt2 + 1) / 2)), dim3(256), 0,
-                               st, n2, A + (j0 + nbp) + (int64_t)(j0 + nbp) * lda, lda,
-                               (const double2*)(h.Vp + nbp), (const double2*)(w.Wp + nbp),
-                               (int64_t)m, nbp, (const double2*)w.alpha2);
+            if (w.her2k_valu)
+                hipLaunchKernelGGL(k_her2k_lower<T>, dim3((unsigned)(nt2 * (nt2 + 1) / 2)), dim3(256),
+                                   0, st, n2, A + (j0 + nbp) + (int64_t)(j0 + nbp) * lda, lda,
+                                   (const double2*)(h.Vp + nbp), (const double2*)(w.Wp + nbp),
+                                   (int64_t)m, nbp, (const double2*)w.alpha2);
+            else
+                hipLaunchKernelGGL(k_her2k_mfma<T>, dim3((unsigned)(nt2 * (nt2 + 1) / 2)), dim3(256),
+                                   0, st, n2, A + (j0 + nbp) + (int64_t)(j0 + nbp) * lda, lda,
+                                   (const double2*)(h.Vp + nbp), (const double2*)(w.Wp + nbp),
+                                   (int64_t)m, nbp, (const double2*)w.alpha2);
         }
         if ((err = hipGetLastError()) != hipSuccess) return err;
     }
@@ -490,6 +635,8 @@ extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int64_
                                 hipEvent_t* prof_ev)
 {
     HetrdWork w;
+    const char* hv = getenv("BSE_HER2K");
+    w.her2k_valu = (hv != nullptr && hv[0] == 'v') ? 1 : 0;
     w.prof_ev = prof_ev;
     w.Vstore = Vstore; w.voff = (int64_t*)voff; w.Tall = Tall; w.Wp = Wp; w.alpha2 = alpha2;
     w.G = G; w.acol = acol; w.Yr = Yr; w.Yc = Yc; w.t12 = t12; w.part = part; w.scal = scal;

@@ -133,6 +133,12 @@ int  bse_get_eigenvalues(bse_ctx* ctx, double* evals);
 /* complex N x nvec (ld >= N), reference order, lowest nvec columns. */
 int  bse_get_eigenvectors(bse_ctx* ctx, double* z, int64_t ld, int nvec, int dst_on_device);
 
+/* Matrix-free FP64 product Y = H X with the assembled system's matrix, every
+ * element recomputed from the interaction tables (the kernel used by the
+ * refinement and the gradients).  x, y: complex N x k host buffers,
+ * reference order.  Requires bse_build. */
+int  bse_apply(bse_ctx* ctx, int k, const double* x, int64_t ldx, double* y, int64_t ldy);
+
 /* K9: oscillator strengths f_m = sum_r conj(rho_r) A_rm for all m (3486-3502)
  * and the TDA dielectric function (3621-3634):
  *   eps(w) = 1 - C/(Omega Nk) sum_m |f_m|^2 / (w - E_m + i eta).           */
