@@ -24,7 +24,7 @@ ABI_SYMBOLS = (
     "bse_create", "bse_destroy", "bse_last_error", "bse_abi_version", "bse_set_problem",
     "bse_problem_sizes", "bse_tb_states", "bse_build", "bse_get_matrix", "bse_get_dipoles",
     "bse_eigh", "bse_get_eigenvalues", "bse_get_eigenvectors", "bse_oscillator",
-    "bse_spectrum", "bse_eigh_dense", "bse_stage_times", "bse_profile", "bse_apply",
+    "bse_spectrum", "bse_eigh_dense", "bse_stage_times", "bse_profile", "bse_apply", "bse_grad_energies",
 )
 
 
@@ -79,6 +79,7 @@ def load_library(path: pathlib.Path | str | None = None):
         "bse_stage_times": (i32, [vp, dp, i32]),
         "bse_profile": (i32, [vp, i32, dp]),
         "bse_apply": (i32, [vp, i32, dp, i64, dp, i64]),
+        "bse_grad_energies": (i32, [vp, i32, C.POINTER(C.c_int), dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -203,6 +204,17 @@ class Context:
         self._check(self.lib.bse_apply(self.h, k, xf.ctypes.data_as(C.POINTER(C.c_double)), n,
                                        y.ctypes.data_as(C.POINTER(C.c_double)), n), "bse_apply")
         return y[:, 0] if vec else y
+
+    def grad_energies(self, n_exc, which=None):
+        """(n_exc, 5) dE_m/dtheta for theta = (hopping scale, scissor, r0, eps_env,
+        exchange field); requires eigh(n_vectors >= n_exc)."""
+        out = np.zeros((5, n_exc))
+        w = None
+        if which is not None:
+            w = (C.c_int * 5)(*[int(bool(x)) for x in which])
+        self._check(self.lib.bse_grad_energies(self.h, int(n_exc), w, _dptr(out)),
+                    "bse_grad_energies")
+        return out.T.copy()
 
     def eigh(self, n_vectors=-1, refine=False):
         self._check(self.lib.bse_eigh(self.h, int(n_vectors), int(bool(refine))), "bse_eigh")

@@ -73,6 +73,8 @@ hipError_t bse_launch_small_rr(hipStream_t, int, int, const double2*, const doub
                                double*, double2*, int*);
 hipError_t bse_launch_residual(hipStream_t, int64_t, int, const double2*, const double2*, int64_t,
                                double2*, double*);
+hipError_t bse_launch_coldot(hipStream_t, int64_t, int, const double2*, const double2*, int64_t,
+                             double*);
 }
 
 static const double kC = (1.602176634 * 1000.0) / 8.8541878176;   // e^2/eps0, eV*A
@@ -100,7 +102,7 @@ struct bse_ctx {
     int norb = 0, nr = 0, nk = 0, ng = 0, B = 0, ns = 0;
     int64_t N = 0;
     double recip[9] = {0}, area = 0, volume = 0, wbar = 0, qhat[3] = {1, 0, 0};
-    std::vector<double> kpts_h, gvec_h;
+    std::vector<double> kpts_h, gvec_h, hr_h, rcart_h;
     // state
     bool have_tb = false, have_tables = false, built = false, have_eig = false, have_f = false;
     int precision = BSE_PREC_FP64;
@@ -128,7 +130,8 @@ enum Slot {
     S_H, S_D, S_E, S_TAUH, S_VSTORE, S_WP, S_ACOL, S_YR, S_YC, S_T12, S_PART, S_SCAL, S_COUNTER,
     S_TALL, S_GRAM, S_E2, S_ITEMS, S_BOUNDS, S_LAMI, S_LAM, S_BSV, S_NBV, S_CLUST, S_IISCR, S_IIX,
     S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_ALPHA2,
-    S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL, S_NSLOTS
+    S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL,
+    S_GDHR, S_GDEPS, S_GDCOEF, S_GDP, S_GDPV, S_GDRHO, S_GDDE, S_GDW, S_GY, S_GOUT, S_NSLOTS
 };
 
 int fail(bse_ctx* c, int code, const std::string& msg) {
@@ -339,6 +342,8 @@ extern "C" int bse_set_problem(bse_ctx* ctx, const bse_tb_model* m, const bse_sy
     double* d_k = BUF(double, S_KPTS, 3 * (size_t)ctx->nk);
     double* d_g = BUF(double, S_GVEC, 3 * (size_t)ctx->ng);
     double* d_q = BUF(double, S_Q, 3);
+    ctx->hr_h.assign(m->h_r, m->h_r + nh);
+    ctx->rcart_h.assign(m->r_cart, m->r_cart + 3 * (size_t)m->nr);
     HCK(hipMemcpyAsync(d_rcart, m->r_cart, 3 * sizeof(double) * m->nr, hipMemcpyHostToDevice, ctx->st));
     HCK(hipMemcpyAsync(d_hr, m->h_r, nh * sizeof(double), hipMemcpyHostToDevice, ctx->st));
     HCK(hipMemcpyAsync(d_tau, m->tau, 3 * sizeof(double) * m->norb, hipMemcpyHostToDevice, ctx->st));
@@ -795,6 +800,124 @@ extern "C" int bse_apply(bse_ctx* ctx, int k, const double* x, int64_t ldx, doub
     HCK(hipMemcpy2DAsync(y, sizeof(double2) * ldy, tmp, sizeof(double2) * N, sizeof(double2) * N, k,
                          hipMemcpyDeviceToHost, ctx->st));
     HCK(hipStreamSynchronize(ctx->st));
+    return BSE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// K10: Hellmann-Feynman gradients of the lowest exciton energies,
+// dE_m/dtheta = A_m^H (dH/dtheta) A_m, with dH applied matrix-free.
+// Parameters (column order): 0 hopping scale (all R != 0 blocks), 1 scissor,
+// 2 r0, 3 eps_env, 4 exchange field h (h * diag(+1,-1,+1,-1,...) on H(R=0)).
+// The TB-derived ones use first-order perturbation theory for the band
+// energies and coefficients (parallel-transport gauge); the gauge part of dc
+// drops out of A^H dH A.
+// ---------------------------------------------------------------------------
+static int grad_tb_apply(bse_ctx* ctx, int which, int k, const double2* A, double2* Y) {
+    const bse_system& s = ctx->sys;
+    const int no = ctx->norb, nr = ctx->nr;
+    std::vector<double> dh((size_t)nr * no * no * 2, 0.0);
+    for (int r = 0; r < nr; ++r) {
+        const double* R = &ctx->rcart_h[3 * r];
+        const bool zero = fabs(R[0]) < 1e-12 && fabs(R[1]) < 1e-12 && fabs(R[2]) < 1e-12;
+        for (int l = 0; l < no; ++l)
+            for (int mm = 0; mm < no; ++mm) {
+                const size_t e = ((size_t)r * no + l) * no + mm;
+                if (which == 0 && !zero) {
+                    dh[2 * e] = ctx->hr_h[2 * e];
+                    dh[2 * e + 1] = ctx->hr_h[2 * e + 1];
+                } else if (which == 4 && zero && l == mm) {
+                    dh[2 * e] = (l % 2 == 0) ? 1.0 : -1.0;
+                }
+            }
+    }
+    double2* d_dh = BUF(double2, S_GDHR, (size_t)nr * no * no);
+    double* deps = BUF(double, S_GDEPS, (size_t)ctx->nk * ctx->ns);
+    double2* dcoef = BUF(double2, S_GDCOEF, (size_t)ctx->nk * ctx->ns * no);
+    double2* dP = BUF(double2, S_GDP, (size_t)ctx->N * no);
+    double2* dPV = BUF(double2, S_GDPV, (size_t)ctx->N * no);
+    double2* drho = BUF(double2, S_GDRHO, (size_t)ctx->N);
+    double* ddE = BUF(double, S_GDDE, (size_t)ctx->N);
+    HCK(hipMemcpyAsync(d_dh, dh.data(), sizeof(double) * dh.size(), hipMemcpyHostToDevice, ctx->st));
+    // TB derivatives (energies and coefficients of the same gauge-fixed states)
+    double* eps2 = BUF(double, S_GDW, (size_t)ctx->nk * ctx->ns + (size_t)ctx->nk * ctx->ns * no * 2);
+    double2* coef2 = (double2*)(eps2 + (size_t)ctx->nk * ctx->ns);
+    HCK(bse_launch_tb_states(ctx->st, ctx->nk, no, nr, (const double*)ctx->bufs[S_RCART].p,
+                             (const double2*)ctx->bufs[S_HR].p, (const double*)ctx->bufs[S_KPTS].p,
+                             s.q, s.n_occ, s.nv, s.nc, s.scissor, d_dh, eps2, coef2, deps, dcoef));
+    HCK(bse_launch_transitions(ctx->st, ctx->nk, s.nv, s.nc, no, (const double*)ctx->bufs[S_EPS].p,
+                               (const double2*)ctx->bufs[S_COEF].p, deps, dcoef,
+                               (const double*)ctx->bufs[S_TAU].p, ctx->qhat,
+                               (const double2*)ctx->bufs[S_VX].p, dP, dPV, drho, ddE));
+    const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
+    const double2* wt = (const double2*)ctx->bufs[S_WTAB].p;
+    const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
+    const double beta = s.f_s * s.exchange_scale / (ctx->area * ctx->nk);
+    const int tk = bse_apply_pick_tk(s.nv, s.nc, no);
+    HCK(hipMemsetAsync(Y, 0, sizeof(double2) * ctx->N * k, ctx->st));
+    const double2* fac[4][4] = {{dcoef, coef, coef, coef}, {coef, dcoef, coef, coef},
+                                {coef, coef, dcoef, coef}, {coef, coef, coef, dcoef}};
+    for (int t = 0; t < 4; ++t)
+        HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no, tk, fac[t][0], fac[t][1],
+                             fac[t][2], fac[t][3], wt, nullptr, nullptr, nullptr, alpha, 0.0, k, A,
+                             ctx->N, Y, ctx->N));
+    // exchange: beta (dP V P^H + P V dP^H); diagonal d(eps_c - eps_v)
+    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no, tk, coef, coef, coef, coef, wt,
+                         dPV, (const double2*)ctx->bufs[S_P].p, ddE, 0.0, beta, k, A, ctx->N, Y, ctx->N));
+    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no, tk, coef, coef, coef, coef, wt,
+                         (const double2*)ctx->bufs[S_PV].p, dP, nullptr, 0.0, beta, k, A, ctx->N, Y,
+                         ctx->N));
+    return BSE_OK;
+}
+
+static int grad_w_apply(bse_ctx* ctx, int which, int k, const double2* A, double2* Y) {
+    const bse_system& s = ctx->sys;
+    const int no = ctx->norb;
+    const int64_t nw = (int64_t)(2 * s.n1 - 1) * (2 * s.n2 - 1) * no * no;
+    double2* dW = BUF(double2, S_GDW, nw);
+    // derivative of the mini-BZ average
+    const double da = 4.0 * M_PI * M_PI / (ctx->area * ctx->nk);
+    const double rad = sqrt(da / M_PI);
+    double dwbar;
+    if (which == 2) {
+        const double r0 = s.r0;
+        dwbar = (r0 > 0.0) ? (M_PI * kC / (s.eps_env * da)) * (rad / ((1.0 + r0 * rad) * r0) - log1p(r0 * rad) / (r0 * r0))
+                           : -(M_PI * kC / (s.eps_env * da)) * 0.5 * rad * rad;
+    } else {
+        dwbar = -ctx->wbar / s.eps_env;
+    }
+    double bvec[6] = {ctx->recip[0], ctx->recip[1], ctx->recip[2], ctx->recip[3], ctx->recip[4], ctx->recip[5]};
+    HCK(bse_launch_w_table(ctx->st, s.n1, s.n2, no, ctx->ng, (const double*)ctx->bufs[S_GVEC].p, bvec,
+                           (const double*)ctx->bufs[S_TAU].p, s.r0, s.eps_env, 0.5 * kC, dwbar,
+                           which == 2 ? 1 : 2, dW));
+    const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
+    const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
+    HCK(hipMemsetAsync(Y, 0, sizeof(double2) * ctx->N * k, ctx->st));
+    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no,
+                         bse_apply_pick_tk(s.nv, s.nc, no), coef, coef, coef, coef, dW, nullptr,
+                         nullptr, nullptr, alpha, 0.0, k, A, ctx->N, Y, ctx->N));
+    return BSE_OK;
+}
+
+extern "C" int bse_grad_energies(bse_ctx* ctx, int n_exc, const int* which, double* dE) {
+    if (!ctx || !dE || n_exc <= 0) return BSE_ERR_ARG;
+    if (!ctx->have_eig || !ctx->eig_on_bse) return fail(ctx, BSE_ERR_STATE, "run bse_eigh on a built system first");
+    if (n_exc > ctx->nvec) return fail(ctx, BSE_ERR_STATE, "bse_eigh formed fewer eigenvectors than n_exc");
+    (void)hipSetDevice(ctx->device);
+    const int64_t N = ctx->N;
+    const double2* A = (const double2*)ctx->bufs[S_Z].p;
+    double2* Y = BUF(double2, S_RHS, (size_t)N * n_exc);
+    double* out = BUF(double, S_GOUT, n_exc);
+    std::vector<double> col(n_exc);
+    for (int t = 0; t < 5; ++t) {
+        double* dst = dE + (size_t)t * n_exc;
+        if (which && !which[t]) { for (int m = 0; m < n_exc; ++m) dst[m] = 0.0; continue; }
+        if (t == 1) { for (int m = 0; m < n_exc; ++m) dst[m] = 1.0; continue; }   // dH = I
+        int rc = (t == 0 || t == 4) ? grad_tb_apply(ctx, t, n_exc, A, Y) : grad_w_apply(ctx, t, n_exc, A, Y);
+        if (rc) return rc;
+        HCK(bse_launch_coldot(ctx->st, N, n_exc, A, Y, N, out));
+        HCK(hipMemcpyAsync(dst, out, sizeof(double) * n_exc, hipMemcpyDeviceToHost, ctx->st));
+        HCK(hipStreamSynchronize(ctx->st));
+    }
     return BSE_OK;
 }
 

@@ -259,3 +259,26 @@ extern "C" hipError_t bse_launch_residual(hipStream_t st, int64_t N, int k, cons
     hipLaunchKernelGGL(k_residual, dim3(k), dim3(256), 0, st, N, X, Y, ld, R, rnorm);
     return hipGetLastError();
 }
+
+// out[j] = Re(x_j^H y_j) for k columns (one workgroup per column).
+__global__ __launch_bounds__(256) void k_coldot(int64_t N, const double2* __restrict__ X,
+                                               const double2* __restrict__ Y, int64_t ld,
+                                               double* __restrict__ out)
+{
+    __shared__ double sh[32];
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+        const double2 a = X[i + (int64_t)j * ld], b = Y[i + (int64_t)j * ld];
+        s += a.x * b.x + a.y * b.y;
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[j] = s;
+}
+
+extern "C" hipError_t bse_launch_coldot(hipStream_t st, int64_t N, int k, const double2* X,
+                                        const double2* Y, int64_t ld, double* out)
+{
+    hipLaunchKernelGGL(k_coldot, dim3(k), dim3(256), 0, st, N, X, Y, ld, out);
+    return hipGetLastError();
+}

@@ -146,6 +146,15 @@ int  bse_oscillator(bse_ctx* ctx, double* f);
 int  bse_spectrum(bse_ctx* ctx, const double* omega, int n_omega, double eta,
                   double* eps_re, double* eps_im);
 
+/* K10: Hellmann-Feynman gradients of the lowest n_exc exciton energies,
+ * dE_m/dtheta = A_m^H (dH/dtheta) A_m, with dH recomputed on the fly (no
+ * stored derivative matrix).  theta columns: 0 relative scale lambda of the
+ * loaded model's R != 0 blocks (H(R) -> lambda H(R), at lambda = 1), 1 scissor, 2 Keldysh r0, 3 eps_env, 4 exchange field h
+ * (h * diag(+1,-1,+1,...) added to H(R=0)).  which: 5 flags (NULL = all).
+ * dE: n_exc x 5, column-major.  Requires bse_eigh with n_vectors >= n_exc.
+ * No counterpart in the reference (new capability). */
+int  bse_grad_energies(bse_ctx* ctx, int n_exc, const int* which, double* dE);
+
 /* Dense Hermitian eig of any matrix (drop-in for arma::eig_sym, zheevd):
  * a: complex n x n (lower triangle read), doubles, host or device.
  * evals: n ascending; z: complex n x n_vectors (may be NULL).             */

@@ -224,3 +224,43 @@ def test_refinement_reaches_fp64(ctx):
     Z = ctx.eigenvectors(16)
     assert np.abs(H @ Z - Z * E[None, :16]).max() < 1e-9 * scale
     assert np.abs(Z.conj().T @ Z - np.eye(16)).max() < 1e-10
+
+
+PARAMS = ("hopping_scale", "scissor", "r0", "eps_env", "exchange_field")
+
+
+def test_hf_gradients_match_oracle(ctx):
+    s = bse_amd.random_multiorbital(4)
+    p = O.problem_from_system(s)
+    g_ref, E_ref = O.hf_energy_gradients(p, n_exc=8, params=PARAMS)
+    ctx.set_system(s).build("fp64").eigh(8)
+    g = ctx.grad_energies(8)
+    assert np.allclose(ctx.eigenvalues()[:8], E_ref, atol=1e-10)
+    assert np.allclose(g, g_ref, rtol=1e-8, atol=1e-10), (g, g_ref)
+
+
+def test_hf_gradients_match_gpu_finite_differences(ctx):
+    from bse_amd import autograd as ag
+    base = bse_amd.random_multiorbital(5)
+    th0 = ag.default_theta(base)
+    E0, g = ag.energies_and_gradients(ctx, base, th0, 8)
+    h = 1e-4
+    for t in range(5):
+        tp, tm = th0.copy(), th0.copy()
+        tp[t] += h
+        tm[t] -= h
+        Ep, _ = ag.energies_and_gradients(ctx, base, tp, 8)
+        Em, _ = ag.energies_and_gradients(ctx, base, tm, 8)
+        fd = (Ep - Em) / (2 * h)
+        assert np.allclose(g[:, t], fd, rtol=1e-4, atol=1e-7), (PARAMS[t], g[:, t], fd)
+
+
+def test_torch_autograd_function(ctx):
+    import torch
+    from bse_amd import autograd as ag
+    base = bse_amd.random_multiorbital(4)
+    th = torch.tensor(ag.default_theta(base), dtype=torch.float64, requires_grad=True)
+    E = ag.exciton_energies(th, base, n_exc=8, ctx=ctx)
+    E.sum().backward()
+    _, g = ag.energies_and_gradients(ctx, base, ag.default_theta(base), 8)
+    assert np.allclose(th.grad.numpy(), g.sum(axis=0), rtol=1e-12, atol=1e-14)
