@@ -25,6 +25,7 @@ ABI_SYMBOLS = (
     "bse_problem_sizes", "bse_tb_states", "bse_build", "bse_get_matrix", "bse_get_dipoles",
     "bse_eigh", "bse_get_eigenvalues", "bse_get_eigenvectors", "bse_oscillator",
     "bse_spectrum", "bse_eigh_dense", "bse_stage_times", "bse_profile", "bse_apply", "bse_grad_energies",
+    "bse_batch_prepare", "bse_batch_run", "bse_batch_results", "bse_batch_info",
 )
 
 
@@ -44,6 +45,33 @@ class SystemC(C.Structure):
                 ("r0", C.c_double), ("eps_env", C.c_double), ("g_cut", C.c_double),
                 ("f_s", C.c_double), ("exchange_scale", C.c_double),
                 ("direct_scale", C.c_double), ("dipole_dir", C.c_double * 3)]
+
+
+class BatchOptionsC(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("n_lowest", C.c_int32), ("refine", C.c_int32),
+                ("n_omega", C.c_int32), ("omega", C.POINTER(C.c_double)), ("eta", C.c_double)]
+
+
+def _system_to_c(s):
+    """(TBModelC, SystemC, arrays to keep alive) for a host BSESystem."""
+    m = s.model
+    r_cart = np.ascontiguousarray(m.r_cart, dtype=np.float64)
+    h_r = np.ascontiguousarray(m.h_r, dtype=np.complex128)
+    tau = np.ascontiguousarray(m.tau, dtype=np.float64)
+    mc = TBModelC(m.norb, m.nr, _dptr(r_cart), h_r.ctypes.data_as(C.POINTER(C.c_double)),
+                  _dptr(tau))
+    sc = SystemC()
+    sc.bravais[:] = list(np.asarray(m.lattice.bravais, dtype=np.float64).T.ravel())
+    sc.n1, sc.n2 = s.grid.n1, s.grid.n2
+    sc.shift[:] = list(s.grid.shift)
+    sc.n_occ, sc.nv, sc.nc = s.n_occ, s.nv, s.nc
+    sc.scissor = s.scissor
+    sc.q[:] = list(s.q_opt)
+    sc.r0, sc.eps_env, sc.g_cut = s.r0, s.eps_env, s.g_cut
+    sc.f_s = s.spin_factor
+    sc.exchange_scale, sc.direct_scale = s.exchange_scale, s.direct_scale
+    sc.dipole_dir[:] = [1.0, 0.0, 0.0]
+    return mc, sc, [r_cart, h_r, tau]
 
 
 _lib = None
@@ -80,14 +108,16 @@ def load_library(path: pathlib.Path | str | None = None):
         "bse_profile": (i32, [vp, i32, dp]),
         "bse_apply": (i32, [vp, i32, dp, i64, dp, i64]),
         "bse_grad_energies": (i32, [vp, i32, C.POINTER(C.c_int), dp]),
+        "bse_batch_prepare": (i32, [vp, i32, C.POINTER(TBModelC), C.POINTER(SystemC),
+                                    C.POINTER(BatchOptionsC)]),
+        "bse_batch_run": (i32, [vp, dp, dp]),
+        "bse_batch_results": (i32, [vp, dp, dp, dp]),
+        "bse_batch_info": (i32, [vp, C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    for name in ("bse_grad_energies", "bse_grad_spectrum", "bse_batch_sweep"):
-        if hasattr(lib, name):
-            getattr(lib, name).restype = i32
     _lib = lib
     return lib
 
@@ -138,24 +168,8 @@ class Context:
     # ------------------------------------------------------------------
     def set_system(self, s):
         """Upload a host-side system description (BSESystem)."""
-        m = s.model
-        r_cart = np.ascontiguousarray(m.r_cart, dtype=np.float64)
-        h_r = np.ascontiguousarray(m.h_r, dtype=np.complex128)
-        tau = np.ascontiguousarray(m.tau, dtype=np.float64)
-        mc = TBModelC(m.norb, m.nr, _dptr(r_cart), h_r.ctypes.data_as(C.POINTER(C.c_double)),
-                      _dptr(tau))
-        sc = SystemC()
-        sc.bravais[:] = list(np.asarray(m.lattice.bravais, dtype=np.float64).T.ravel())
-        sc.n1, sc.n2 = s.grid.n1, s.grid.n2
-        sc.shift[:] = list(s.grid.shift)
-        sc.n_occ, sc.nv, sc.nc = s.n_occ, s.nv, s.nc
-        sc.scissor = s.scissor
-        sc.q[:] = list(s.q_opt)
-        sc.r0, sc.eps_env, sc.g_cut = s.r0, s.eps_env, s.g_cut
-        sc.f_s = s.spin_factor
-        sc.exchange_scale, sc.direct_scale = s.exchange_scale, s.direct_scale
-        sc.dipole_dir[:] = [1.0, 0.0, 0.0]
-        self._keep = [r_cart, h_r, tau]
+        mc, sc, keep = _system_to_c(s)
+        self._keep = keep
         self._check(self.lib.bse_set_problem(self.h, C.byref(mc), C.byref(sc)), "bse_set_problem")
         self.system = s
         return self
@@ -278,3 +292,52 @@ class Context:
         out = np.zeros(len(STAGES))
         self._check(self.lib.bse_stage_times(self.h, _dptr(out), len(STAGES)), "bse_stage_times")
         return dict(zip(STAGES, out.tolist()))
+
+    # ---- batched sweep ---------------------------------------------------
+    def batch_prepare(self, systems, precision="fp32", n_lowest=16, refine=True, omegas=None,
+                      eta=0.05):
+        """Upload many systems and record the sweep graph (bse_batch_prepare)."""
+        n = len(systems)
+        models = (TBModelC * n)()
+        syss = (SystemC * n)()
+        keep = []
+        for i, s in enumerate(systems):
+            mc, sc, k = _system_to_c(s)
+            models[i] = mc
+            syss[i] = sc
+            keep.extend(k)
+        om = np.ascontiguousarray(omegas if omegas is not None else np.zeros(0), dtype=np.float64)
+        opt = BatchOptionsC(PRECISIONS[precision] if isinstance(precision, str) else int(precision),
+                            int(n_lowest), 1 if refine else 0, int(om.size),
+                            _dptr(om) if om.size else None, float(eta))
+        self._check(self.lib.bse_batch_prepare(self.h, n, models, syss, C.byref(opt)),
+                    "bse_batch_prepare")
+        self._batch = dict(n=n, n_lowest=int(n_lowest), n_omega=int(om.size))
+        return self
+
+    def batch_run(self):
+        """One graph launch: (total ms, per-system completion ms in input order)."""
+        b = self._batch
+        tot = C.c_double(0.0)
+        per = np.zeros(b["n"])
+        self._check(self.lib.bse_batch_run(self.h, C.byref(tot), _dptr(per)), "bse_batch_run")
+        return tot.value, per
+
+    def batch_results(self):
+        """(evals [n, n_lowest], eps [n, n_omega] complex or None)."""
+        b = self._batch
+        ev = np.zeros((b["n"], b["n_lowest"]))
+        if b["n_omega"] > 0:
+            re = np.zeros((b["n"], b["n_omega"]))
+            im = np.zeros((b["n"], b["n_omega"]))
+            self._check(self.lib.bse_batch_results(self.h, _dptr(ev), _dptr(re), _dptr(im)),
+                        "bse_batch_results")
+            return ev, re + 1j * im
+        self._check(self.lib.bse_batch_results(self.h, _dptr(ev), None, None), "bse_batch_results")
+        return ev, None
+
+    def batch_info(self):
+        out = (C.c_int64 * 6)()
+        self._check(self.lib.bse_batch_info(self.h, out), "bse_batch_info")
+        return dict(graph_nodes=out[0], arena_bytes=out[1], max_n=out[2], systems=out[3],
+                    hemv_ch_top=out[4], her2k_valu=out[5])

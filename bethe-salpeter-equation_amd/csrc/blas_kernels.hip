@@ -147,21 +147,7 @@ __global__ void k_larft(int nb, const double2* __restrict__ tau, const double2* 
                         double2* __restrict__ T, int ldt)
 {
     __shared__ double2 col[64];
-    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) T[(e % nb) + (e / nb) * ldt] = cmk(0, 0);
-    __syncthreads();
-    for (int i = 0; i < nb; ++i) {
-        const double2 ti = tau[i];
-        const int r = threadIdx.x;
-        if (r < i) {
-            double2 s = cmk(0, 0);
-            for (int l = r; l < i; ++l) cfma(s, T[r + l * ldt], G[l + i * 64]);
-            col[r] = cscale(cmul(ti, s), -1.0);
-        }
-        __syncthreads();
-        if (r < i) T[r + i * ldt] = col[r];
-        if (r == 0) T[i + i * ldt] = ti;
-        __syncthreads();
-    }
+    larft_body(nb, tau, G, T, ldt, col);
 }
 
 extern "C" hipError_t bse_launch_larft(hipStream_t st, int nb, const double2* tau, const double2* G,
@@ -277,5 +263,129 @@ extern "C" hipError_t bse_launch_rows_from_reference(hipStream_t st, int64_t N, 
     dim3 grid((unsigned)ceil_div64(N, 256), (unsigned)ncol);
     hipLaunchKernelGGL(k_rows_from_reference, grid, dim3(256), 0, st, N, B, nk, ncol, src, lds, dst,
                        ldd);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Application of the reduction's orthogonal factor to a few vectors without
+// forming it: Y <- Q^H Y (adjoint = 1; panels in increasing order, T^H) or
+// Y <- Q Y (adjoint = 0; decreasing order, T), with Q = prod_p (I - V_p T_p V_p^H)
+// from the packed FP64 reflector store (panel p at nb (p n - nb p (p-1)/2),
+// ld n - p nb) and the per-panel compact-WY factors (nb x nb, ld nb).
+// One workgroup per vector walks all panels; for each it forms
+// w1 = V^H y (wave sums), w2 = T^(H) w1 and y -= V w2.  For the small
+// matrices of the batched sweep, where one vector is a few MB of reflector
+// reads, this replaces three GEMM launches per panel.  nb <= 32.
+// ---------------------------------------------------------------------------
+#define APQ_NB 32
+__global__ __launch_bounds__(256) void k_apply_q(int n, int nb, const double2* __restrict__ Vstore,
+                                                 const double2* __restrict__ Tall,
+                                                 double2* __restrict__ Y, int64_t ldy, int adjoint)
+{
+    __shared__ double2 s_red[4][APQ_NB];
+    __shared__ double2 s_w1[APQ_NB], s_w2[APQ_NB];
+    double2* y = Y + (int64_t)blockIdx.x * ldy;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int npan = (int)ceil_div64(n - 1, nb);
+    for (int pp = 0; pp < npan; ++pp) {
+        const int p = adjoint ? pp : npan - 1 - pp;
+        const int j0 = p * nb, m = n - j0, nbp = min(nb, n - 1 - j0);
+        const double2* V = Vstore + (int64_t)nb * ((int64_t)p * n - (int64_t)nb * p * (p - 1) / 2);
+        const double2* T = Tall + (size_t)p * nb * nb;
+        double2 acc[APQ_NB];
+#pragma unroll
+        for (int l = 0; l < APQ_NB; ++l) acc[l] = cmk(0, 0);
+        for (int r = threadIdx.x; r < m; r += 256) {
+            const double2 yr = y[j0 + r];
+#pragma unroll
+            for (int l = 0; l < APQ_NB; ++l)
+                if (l < nbp) cfmac(acc[l], V[r + (int64_t)l * m], yr);
+        }
+#pragma unroll
+        for (int l = 0; l < APQ_NB; ++l) {
+            const double x = wave_sum(acc[l].x), z = wave_sum(acc[l].y);
+            if (lane == 0) s_red[w][l] = cmk(x, z);
+        }
+        __syncthreads();
+        if (threadIdx.x < APQ_NB) {
+            const int l = threadIdx.x;
+            s_w1[l] = cadd(cadd(s_red[0][l], s_red[1][l]), cadd(s_red[2][l], s_red[3][l]));
+        }
+        __syncthreads();
+        if (threadIdx.x < nbp) {
+            const int l = threadIdx.x;
+            double2 s = cmk(0, 0);
+            if (adjoint) {
+                for (int k = 0; k <= l; ++k) cfmac(s, T[k + l * nb], s_w1[k]);
+            } else {
+                for (int k = l; k < nbp; ++k) cfma(s, T[l + k * nb], s_w1[k]);
+            }
+            s_w2[l] = s;
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < m; r += 256) {
+            double2 s = cmk(0, 0);
+            for (int l = 0; l < nbp; ++l) cfma(s, V[r + (int64_t)l * m], s_w2[l]);
+            y[j0 + r] = csub(y[j0 + r], s);
+        }
+        __syncthreads();
+    }
+}
+
+extern "C" hipError_t bse_launch_apply_q(hipStream_t st, int n, int nb, const double2* Vstore,
+                                         const double2* Tall, int nvec, double2* Y, int64_t ldy,
+                                         int adjoint)
+{
+    if (nb > APQ_NB || nvec <= 0) return hipErrorInvalidValue;
+    if (n < 2) return hipSuccess;
+    hipLaunchKernelGGL(k_apply_q, dim3((unsigned)nvec), dim3(256), 0, st, n, nb, Vstore, Tall, Y,
+                       ldy, adjoint);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Copies, fills and time stamps as kernels, so that sequences recorded into a
+// graph contain kernel nodes only.
+// ---------------------------------------------------------------------------
+__global__ void k_copy_words(int64_t n, const uint64_t* __restrict__ src, uint64_t* __restrict__ dst)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void k_fill_words(int64_t n, uint64_t v, uint64_t* __restrict__ dst)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = v;
+}
+
+// dst[0] = device wall clock (constant-rate counter).
+__global__ void k_stamp(unsigned long long* dst)
+{
+    if (threadIdx.x == 0) dst[0] = wall_clock64();
+}
+
+// n8: number of 8-byte words.
+extern "C" hipError_t bse_launch_copy_words(hipStream_t st, int64_t n8, const void* src, void* dst)
+{
+    if (n8 <= 0) return hipSuccess;
+    const unsigned g = (unsigned)std::min<int64_t>(ceil_div64(n8, 256), 2048);
+    hipLaunchKernelGGL(k_copy_words, dim3(g), dim3(256), 0, st, n8, (const uint64_t*)src, (uint64_t*)dst);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_launch_zero_words(hipStream_t st, int64_t n8, void* dst)
+{
+    if (n8 <= 0) return hipSuccess;
+    const unsigned g = (unsigned)std::min<int64_t>(ceil_div64(n8, 256), 2048);
+    hipLaunchKernelGGL(k_fill_words, dim3(g), dim3(256), 0, st, n8, (uint64_t)0, (uint64_t*)dst);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_launch_stamp(hipStream_t st, unsigned long long* dst)
+{
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, st, dst);
     return hipGetLastError();
 }

@@ -5,195 +5,7 @@
 // tridiagonal stage (split points, sort of eigenvalues, cluster and batch
 // boundaries), which needs the tridiagonal (2N doubles) and the eigenvalues
 // (N doubles) on the host.
-#include "bse_common.hpp"
-#include "../../include/bse_mi355x.h"
-
-#include <algorithm>
-#include <cfloat>
-#include <cstdlib>
-#include <cstring>
-#include <string>
-#include <vector>
-
-// ---- kernel launchers implemented in the other translation units ----------
-extern "C" {
-hipError_t bse_launch_tb_states(hipStream_t, int, int, int, const double*, const double2*,
-                                const double*, const double*, int, int, int, double,
-                                const double2*, double*, double2*, double*, double2*);
-hipError_t bse_launch_w_table(hipStream_t, int, int, int, int, const double*, const double*,
-                              const double*, double, double, double, double, int, double2*);
-hipError_t bse_launch_v_exchange(hipStream_t, int, int, const double*, const double*, double,
-                                 double2*);
-hipError_t bse_launch_transitions(hipStream_t, int, int, int, int, const double*, const double2*,
-                                  const double*, const double2*, const double*, const double*,
-                                  const double2*, double2*, double2*, double2*, double*);
-size_t bse_assemble_lds_bytes(int, int, int, int);
-int bse_assemble_pick_tk(int, int, int);
-hipError_t bse_launch_assemble(hipStream_t, int, int, int, int, int, int, int, int, int64_t,
-                               const double2*, const double2*, const double2*, const double2*,
-                               const double2*, const double2*, const double2*, const double*,
-                               double, double, void*, int);
-hipError_t bse_launch_expand(hipStream_t, int, int64_t, int, int, const void*, int64_t, void*,
-                             int64_t, int);
-void bse_hetrd_workspace(int, int, int64_t*, int64_t*, int64_t*, int64_t*);
-hipError_t bse_hetrd(hipStream_t, int, int, void*, int64_t, int, double*, double*, double2*,
-                     double2*, const int64_t*, double2*, double2*, double2*, double2*, double2*,
-                     double2*, double2*, double2*, double*, double*, unsigned int*, hipEvent_t*);
-hipError_t bse_launch_bisect(hipStream_t, int, const int4*, const double2*, const double*,
-                             const double*, double, double, double*);
-hipError_t bse_launch_inverse_iter(hipStream_t, int, int, int, const double*, const int*,
-                                   const int*, const double*, const double*, double, double*, int,
-                                   double*);
-hipError_t bse_launch_cluster_mgs(hipStream_t, int, int, const int2*, int, const int*, const int*,
-                                  double*);
-hipError_t bse_launch_commit(hipStream_t, int, int, int, int, int, const int*, const int*,
-                             const double*, double2*, int64_t, int);
-hipError_t bse_launch_contract(hipStream_t, int, int, int, const int*, const int*, const double*,
-                               const double2*, double2*);
-hipError_t bse_zgemm(hipStream_t, int, int, int, int, int, double, double, const double2*, int64_t,
-                     const double2*, int64_t, double, double, double2*, int64_t, double2*, int64_t);
-hipError_t bse_launch_larft(hipStream_t, int, const double2*, const double2*, double2*, int);
-hipError_t bse_launch_dielectric(hipStream_t, int, const double*, double, int, const double*,
-                                 const double2*, double, double*, double*);
-hipError_t bse_launch_rows_to_reference(hipStream_t, int64_t, int, int, int, const double2*,
-                                        int64_t, double2*, int64_t);
-hipError_t bse_launch_convert_to_fp32(hipStream_t, int64_t, const double2*, float2*);
-hipError_t bse_launch_rows_from_reference(hipStream_t, int64_t, int, int, int, const double2*,
-                                          int64_t, double2*, int64_t);
-int bse_apply_pick_tk(int, int, int);
-int bse_assemble_mfma_supported(int, int, int);
-hipError_t bse_launch_assemble_mfma(hipStream_t, int, int, int, int, int, int, int, int64_t,
-                                    const double2*, const double2*, const double2*, const double2*,
-                                    const double*, double, double, void*);
-hipError_t bse_launch_apply(hipStream_t, int, int, int, int, int, int, int, const double2*,
-                            const double2*, const double2*, const double2*, const double2*,
-                            const double2*, const double2*, const double*, double, double, int,
-                            const double2*, int64_t, double2*, int64_t);
-hipError_t bse_launch_small_rr(hipStream_t, int, int, const double2*, const double2*, double,
-                               double*, double2*, int*);
-hipError_t bse_launch_residual(hipStream_t, int64_t, int, const double2*, const double2*, int64_t,
-                               double2*, double*);
-hipError_t bse_launch_coldot(hipStream_t, int64_t, int, const double2*, const double2*, int64_t,
-                             double*);
-}
-
-static const double kC = (1.602176634 * 1000.0) / 8.8541878176;   // e^2/eps0, eV*A
-static const int kPanel = 32;                                      // hetrd panel width
-
-namespace {
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-};
-
-enum Stage { ST_TB, ST_TABLES, ST_ASM, ST_TRD, ST_TEVAL, ST_TVEC, ST_BT, ST_REF, ST_SPEC, ST_N };
-
-}  // namespace
-
-struct bse_ctx {
-    int device = 0;
-    hipStream_t st = nullptr;
-    std::string err;
-    std::vector<DevBuf> bufs;   // indexed by slot id
-    // problem
-    bool have_problem = false;
-    bse_system sys{};
-    int norb = 0, nr = 0, nk = 0, ng = 0, B = 0, ns = 0;
-    int64_t N = 0;
-    double recip[9] = {0}, area = 0, volume = 0, wbar = 0, qhat[3] = {1, 0, 0};
-    std::vector<double> kpts_h, gvec_h, hr_h, rcart_h;
-    // state
-    bool have_tb = false, have_tables = false, built = false, have_eig = false, have_f = false;
-    int precision = BSE_PREC_FP64;
-    int64_t ldH = 0;
-    int64_t n_eig = 0;          // size of the last eigenproblem
-    int nvec = 0;               // eigenvectors formed
-    bool eig_on_bse = false;    // last eig was of the assembled BSE matrix
-    int TK = 1;
-    hipEvent_t ev[ST_N + 1] = {};
-    bool ev_used[ST_N] = {};
-    double stage_ms[16] = {0};
-    // optional per-launch timing of the tridiagonalisation's hemv kernel
-    bool prof = false;
-    std::vector<hipEvent_t> prof_ev;
-    double hemv_ms = 0.0, hemv_bytes = 0.0;
-    bool force_valu_assembly = false;   // BSE_ASSEMBLY=valu selects the VALU tiles
-    int asm_kernel = 0;                 // 0 VALU, 1 FP64 MFMA, 2 FP32 MFMA, 3 BF16x2 MFMA
-    int64_t hemv_launches = 0;
-};
-
-namespace {
-
-enum Slot {
-    S_RCART, S_HR, S_TAU, S_KPTS, S_GVEC, S_Q, S_EPS, S_COEF, S_WTAB, S_VX, S_P, S_PV, S_RHO, S_DE,
-    S_H, S_D, S_E, S_TAUH, S_VSTORE, S_WP, S_ACOL, S_YR, S_YC, S_T12, S_PART, S_SCAL, S_COUNTER,
-    S_TALL, S_GRAM, S_E2, S_ITEMS, S_BOUNDS, S_LAMI, S_LAM, S_BSV, S_NBV, S_CLUST, S_IISCR, S_IIX,
-    S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_ALPHA2,
-    S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL,
-    S_GDHR, S_GDEPS, S_GDCOEF, S_GDP, S_GDPV, S_GDRHO, S_GDDE, S_GDW, S_GY, S_GOUT, S_NSLOTS
-};
-
-int fail(bse_ctx* c, int code, const std::string& msg) {
-    if (c) c->err = msg;
-    return code;
-}
-
-#define HCK(expr)                                                                       \
-    do {                                                                                \
-        hipError_t _e = (expr);                                                         \
-        if (_e != hipSuccess)                                                           \
-            return fail(ctx, BSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
-    } while (0)
-
-void* buf(bse_ctx* ctx, int slot, size_t bytes, int* status) {
-    DevBuf& b = ctx->bufs[slot];
-    if (bytes == 0) bytes = 16;
-    if (b.cap >= bytes) return b.p;
-    if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.cap = 0; }
-    if (hipMalloc(&b.p, bytes) != hipSuccess) {
-        b.p = nullptr;
-        *status = fail(ctx, BSE_ERR_ALLOC, "hipMalloc failed for " + std::to_string(bytes) + " bytes");
-        return nullptr;
-    }
-    b.cap = bytes;
-    return b.p;
-}
-
-template <typename T>
-T* bufT(bse_ctx* ctx, int slot, size_t count, int* status) {
-    return (T*)buf(ctx, slot, count * sizeof(T), status);
-}
-
-#define BUF(T, slot, count)                                  \
-    ({                                                        \
-        int _s = BSE_OK;                                      \
-        T* _p = bufT<T>(ctx, slot, (size_t)(count), &_s);     \
-        if (!_p) return _s;                                   \
-        _p;                                                   \
-    })
-
-void tick(bse_ctx* ctx, int stage) { (void)hipEventRecord(ctx->ev[stage], ctx->st); }
-
-void stage_begin(bse_ctx* ctx, int stage) {
-    (void)hipEventRecord(ctx->ev[stage], ctx->st);
-    ctx->ev_used[stage] = true;
-}
-void stage_end(bse_ctx* ctx, int stage) {
-    (void)hipEventRecord(ctx->ev[ST_N], ctx->st);
-    (void)hipEventSynchronize(ctx->ev[ST_N]);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, ctx->ev[stage], ctx->ev[ST_N]);
-    ctx->stage_ms[stage] = ms;
-}
-
-bool finite_all(const double* p, size_t n) {
-    for (size_t i = 0; i < n; ++i)
-        if (!std::isfinite(p[i])) return false;
-    return true;
-}
-
-}  // namespace
+#include "bse_internal.hpp"
 
 // ===========================================================================
 extern "C" int bse_abi_version(void) { return BSE_ABI_VERSION; }
@@ -217,6 +29,9 @@ extern "C" int bse_create(bse_ctx** out, int device) {
         return BSE_ERR_HIP;
     }
     for (int i = 0; i <= ST_N; ++i) (void)hipEventCreate(&ctx->ev[i]);
+    for (int i = 0; i < 2; ++i) (void)hipEventCreate(&ctx->tev[i]);
+    const char* at = getenv("BSE_AUTOTUNE");
+    ctx->autotune = !(at && at[0] == '0');
     const char* am = getenv("BSE_ASSEMBLY");
     ctx->force_valu_assembly = am && strcmp(am, "valu") == 0;
     *out = ctx;
@@ -227,11 +42,14 @@ extern "C" int bse_destroy(bse_ctx* ctx) {
     if (!ctx) return BSE_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->st);
+    batch_destroy(ctx);
     for (auto& b : ctx->bufs)
         if (b.p) (void)hipFree(b.p);
     for (int i = 0; i <= ST_N; ++i)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; ++i)
+        if (ctx->tev[i]) (void)hipEventDestroy(ctx->tev[i]);
     (void)hipStreamDestroy(ctx->st);
     delete ctx;
     return BSE_OK;
@@ -269,72 +87,27 @@ extern "C" int bse_problem_sizes(const bse_ctx* ctx, int64_t out[4]) {
 // ---------------------------------------------------------------------------
 extern "C" int bse_set_problem(bse_ctx* ctx, const bse_tb_model* m, const bse_system* s) {
     if (!ctx) return BSE_ERR_ARG;
-    if (!m || !s || !m->r_cart || !m->h_r || !m->tau) return fail(ctx, BSE_ERR_ARG, "null argument");
-    if (m->norb < 1 || m->norb > BSE_MAXORB) return fail(ctx, BSE_ERR_UNSUPPORTED, "norb must be in [1, 16]");
-    if (m->nr < 1) return fail(ctx, BSE_ERR_ARG, "nr must be >= 1");
-    if (s->n1 < 1 || s->n2 < 1) return fail(ctx, BSE_ERR_ARG, "k grid must be at least 1 x 1");
-    if (s->nv < 1 || s->nc < 1 || s->n_occ < s->nv || s->n_occ + s->nc > m->norb)
-        return fail(ctx, BSE_ERR_ARG, "band window outside the Hamiltonian (nv <= n_occ, n_occ + nc <= norb)");
-    if (!(s->eps_env > 0.0) || !(s->r0 >= 0.0) || !(s->g_cut >= 0.0))
-        return fail(ctx, BSE_ERR_ARG, "screening parameters must satisfy eps_env > 0, r0 >= 0, g_cut >= 0");
-    const size_t nh = (size_t)m->nr * m->norb * m->norb * 2;
-    if (!finite_all(m->h_r, nh) || !finite_all(m->r_cart, 3 * (size_t)m->nr) ||
-        !finite_all(m->tau, 3 * (size_t)m->norb) || !finite_all(s->bravais, 9))
-        return fail(ctx, BSE_ERR_NUMERIC, "non-finite model input");
+    Geom g;
+    std::string msg;
+    const int vrc = make_geom(m, s, g, msg);
+    if (vrc) return fail(ctx, vrc, msg);
     (void)hipSetDevice(ctx->device);
-    ctx->sys = *s;
-    ctx->norb = m->norb;
-    ctx->nr = m->nr;
-    ctx->nk = s->n1 * s->n2;
-    ctx->B = s->nv * s->nc;
-    ctx->ns = s->nv + s->nc;
-    ctx->N = (int64_t)ctx->nk * ctx->B;
-    // reciprocal vectors b_i = 2 pi (a_j x a_k) / Omega
-    const double* a = s->bravais;
-    auto cross = [](const double* x, const double* y, double* z) {
-        z[0] = x[1] * y[2] - x[2] * y[1];
-        z[1] = x[2] * y[0] - x[0] * y[2];
-        z[2] = x[0] * y[1] - x[1] * y[0];
-    };
-    double c0[3], c1[3], c2[3];
-    cross(a + 3, a + 6, c0);
-    cross(a + 6, a + 0, c1);
-    cross(a + 0, a + 3, c2);
-    const double det = a[0] * c0[0] + a[1] * c0[1] + a[2] * c0[2];
-    if (det == 0.0) return fail(ctx, BSE_ERR_ARG, "singular Bravais lattice");
-    ctx->volume = fabs(det);
-    const double f = 2.0 * M_PI / ctx->volume;
-    for (int i = 0; i < 3; ++i) {
-        ctx->recip[i] = f * c0[i];
-        ctx->recip[3 + i] = f * c1[i];
-        ctx->recip[6 + i] = f * c2[i];
-    }
-    double axb[3];
-    cross(a, a + 3, axb);
-    ctx->area = sqrt(axb[0] * axb[0] + axb[1] * axb[1] + axb[2] * axb[2]);
-    // k grid, k index = i * n2 + j
-    ctx->kpts_h.assign(3 * (size_t)ctx->nk, 0.0);
-    for (int i = 0; i < s->n1; ++i)
-        for (int j = 0; j < s->n2; ++j) {
-            const double f1 = (i + s->shift[0]) / s->n1, f2 = (j + s->shift[1]) / s->n2;
-            double* kp = &ctx->kpts_h[3 * ((size_t)i * s->n2 + j)];
-            for (int x = 0; x < 3; ++x) kp[x] = f1 * ctx->recip[x] + f2 * ctx->recip[3 + x];
-        }
-    // 2D parallelepiped G-list, G = 0 at index Ng/2
-    const double nb1 = sqrt(ctx->recip[0] * ctx->recip[0] + ctx->recip[1] * ctx->recip[1] + ctx->recip[2] * ctx->recip[2]);
-    const double nb2 = sqrt(ctx->recip[3] * ctx->recip[3] + ctx->recip[4] * ctx->recip[4] + ctx->recip[5] * ctx->recip[5]);
-    const int g1 = (int)(s->g_cut / nb1), g2 = (int)(s->g_cut / nb2);
-    ctx->gvec_h.clear();
-    for (int i = -g1; i <= g1; ++i)
-        for (int j = -g2; j <= g2; ++j)
-            for (int x = 0; x < 3; ++x) ctx->gvec_h.push_back(i * ctx->recip[x] + j * ctx->recip[3 + x]);
-    ctx->ng = (int)(ctx->gvec_h.size() / 3);
-    // q = 0 mini-BZ average of the Keldysh W over a disk of area (2pi)^2/(A Nk)
-    const double da = 4.0 * M_PI * M_PI / (ctx->area * ctx->nk);
-    const double rad = sqrt(da / M_PI);
-    ctx->wbar = (s->r0 > 0.0) ? (M_PI * kC / s->eps_env) * log1p(s->r0 * rad) / (s->r0 * da)
-                              : (M_PI * kC / s->eps_env) * rad / da;
+    ctx->sys = g.sys;
+    ctx->norb = g.norb;
+    ctx->nr = g.nr;
+    ctx->nk = g.nk;
+    ctx->B = g.B;
+    ctx->ns = g.ns;
+    ctx->N = g.N;
+    for (int i = 0; i < 9; ++i) ctx->recip[i] = g.recip[i];
+    ctx->volume = g.volume;
+    ctx->area = g.area;
+    ctx->kpts_h = g.kpts;
+    ctx->gvec_h = g.gvec;
+    ctx->ng = g.ng;
+    ctx->wbar = g.wbar;
     ctx->TK = bse_assemble_pick_tk(s->nv, s->nc, m->norb);
+    const size_t nh = (size_t)m->nr * m->norb * m->norb * 2;
 
     double* d_rcart = BUF(double, S_RCART, 3 * (size_t)m->nr);
     double2* d_hr = BUF(double2, S_HR, (size_t)m->nr * m->norb * m->norb);
@@ -350,11 +123,7 @@ extern "C" int bse_set_problem(bse_ctx* ctx, const bse_tb_model* m, const bse_sy
     HCK(hipMemcpyAsync(d_k, ctx->kpts_h.data(), 3 * sizeof(double) * ctx->nk, hipMemcpyHostToDevice, ctx->st));
     HCK(hipMemcpyAsync(d_g, ctx->gvec_h.data(), 3 * sizeof(double) * ctx->ng, hipMemcpyHostToDevice, ctx->st));
     HCK(hipMemcpyAsync(d_q, s->q, 3 * sizeof(double), hipMemcpyHostToDevice, ctx->st));
-    double qh[3] = {s->dipole_dir[0], s->dipole_dir[1], s->dipole_dir[2]};
-    const double qn = sqrt(qh[0] * qh[0] + qh[1] * qh[1] + qh[2] * qh[2]);
-    if (!(qn > 0.0)) { qh[0] = 1.0; qh[1] = qh[2] = 0.0; }
-    else for (double& x : qh) x /= qn;
-    for (int x = 0; x < 3; ++x) ctx->qhat[x] = qh[x];
+    for (int x = 0; x < 3; ++x) ctx->qhat[x] = g.qhat[x];
     HCK(hipStreamSynchronize(ctx->st));
     ctx->have_problem = true;
     ctx->have_tb = ctx->have_tables = ctx->built = ctx->have_eig = ctx->have_f = false;
@@ -499,6 +268,46 @@ extern "C" int bse_get_dipoles(bse_ctx* ctx, double* rho) {
 // type FP64 (fp32 = 0) or FP32 complex.  rho (may be null): contraction vector
 // for the oscillator strengths.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Autotuner for the tridiagonalisation's Hermitian matrix-vector kernel: for
+// every floor(log2(nt)) bucket up to the current size that is not tuned yet,
+// time chunk lengths 4, 8 and 16 on a trailing block of the matrix about to
+// be reduced (the kernel only reads it) and keep the fastest.  The table is
+// per context and storage precision; BSE_AUTOTUNE=0 keeps the default 8.
+// ---------------------------------------------------------------------------
+int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, int64_t ld, double2* v,
+                  double2* Yr, double2* Yc) {
+    if (!ctx->autotune || n < 128) return BSE_OK;
+    signed char* tab = ctx->hemv_ch[fp32];
+    const size_t es = fp32 ? sizeof(float2) : sizeof(double2);
+    hipStream_t st = ctx->st;
+    bool zeroed = false;
+    for (int b = 6; b < BSE_HEMV_BUCKETS && (1 << b) <= n - 1; ++b) {
+        if (tab[b] != 0) continue;
+        if (!zeroed) {
+            HCK(hipMemsetAsync(v, 0, sizeof(double2) * n, st));
+            zeroed = true;
+        }
+        const int nt = std::min(3 << (b - 1), n - 1);
+        const void* A22 = (const char*)H + es * ((size_t)(n - nt) + (size_t)(n - nt) * ld);
+        const int reps = nt > 4096 ? 3 : 10;
+        float best = 0.f;
+        int best_ch = 8;
+        for (int ch : {8, 4, 16}) {
+            HCK(bse_launch_hemv(st, fp32, ch, nt, A22, ld, v, Yr, Yc));
+            HCK(hipEventRecord(ctx->tev[0], st));
+            for (int r = 0; r < reps; ++r) HCK(bse_launch_hemv(st, fp32, ch, nt, A22, ld, v, Yr, Yc));
+            HCK(hipEventRecord(ctx->tev[1], st));
+            HCK(hipEventSynchronize(ctx->tev[1]));
+            float ms = 0.f;
+            HCK(hipEventElapsedTime(&ms, ctx->tev[0], ctx->tev[1]));
+            if (best == 0.f || ms < 0.97f * best) { best = ms; best_ch = ch; }
+        }
+        tab[b] = (signed char)best_ch;
+    }
+    return BSE_OK;
+}
+
 static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32,
                         const double2* rho, int n_vectors)
 {
@@ -540,8 +349,12 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32
     double2* Tall = BUF(double2, S_TALL, (size_t)std::max(npan, 1) * nb * nb);
     double2* Gram = BUF(double2, S_GRAM, (size_t)64 * 64);
     double2* alpha2 = BUF(double2, S_ALPHA2, 64);
+    {
+        const int rc = autotune_hemv(ctx, fp32, n, H, ld, acol, Yr, Yc);
+        if (rc) return rc;
+    }
     HCK(bse_hetrd(st, fp32, n, H, ld, nb, d, e, tauh, Vst, voff.data(), Tall, Wp, alpha2, Gram, acol,
-                  Yr, Yc, t12, part, scal, counter, pev));
+                  Yr, Yc, t12, part, scal, counter, pev, ctx->hemv_ch[fp32]));
     if (pev) {
         HCK(hipStreamSynchronize(st));
         double ms = 0.0, bytes = 0.0;

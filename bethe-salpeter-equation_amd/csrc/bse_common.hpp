@@ -8,6 +8,9 @@
 // Largest number of orbitals (spin components included) per cell.
 #define BSE_MAXORB 16
 
+// Number of floor(log2(size)) buckets of the per-device autotune tables.
+#define BSE_HEMV_BUCKETS 24
+
 // ---------------------------------------------------------------------------
 // Complex arithmetic on HIP vector types (double2 = FP64 complex, float2 =
 // FP32 complex); .x = real part, .y = imaginary part.
@@ -137,3 +140,53 @@ __device__ inline int mfma16_row_f32(int lane, int r) { return 4 * (lane >> 4) +
 __device__ inline int mfma16_row_f64(int lane, int r) { return (lane >> 4) + 4 * r; }
 
 __host__ __device__ inline int64_t ceil_div64(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Compact-WY factor of one Householder panel (LAPACK larft, forward,
+// columnwise): Q = H_0 H_1 ... H_{nb-1} = I - V T V^H with T upper
+// triangular, built column by column as
+//   T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i),  T(i, i) = tau_i,
+// from the Gram column G(l, i) = V(:, l)^H v_i (ld 64).  One block of >= nb
+// threads (nb <= 64); col is a 64-entry shared scratch.
+__device__ inline void larft_body(int nb, const double2* __restrict__ tau,
+                                  const double2* __restrict__ G, double2* __restrict__ T, int ldt,
+                                  double2* col)
+{
+    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) T[(e % nb) + (e / nb) * ldt] = cmk(0, 0);
+    __syncthreads();
+    for (int i = 0; i < nb; ++i) {
+        const double2 ti = tau[i];
+        const int r = threadIdx.x;
+        if (r < i) {
+            double2 s = cmk(0, 0);
+            for (int l = r; l < i; ++l) cfma(s, T[r + l * ldt], G[l + i * 64]);
+            col[r] = cscale(cmul(ti, s), -1.0);
+        }
+        __syncthreads();
+        if (r < i) T[r + i * ldt] = col[r];
+        if (r == 0) T[i + i * ldt] = ti;
+        __syncthreads();
+    }
+}
+
+// Per-matrix descriptor of the batched Householder reduction (storage and
+// workspace of one matrix of an uneven batch; filled on the host).
+struct HetrdSys {
+    int n;
+    int64_t lda;
+    void* A;
+    double2* Vstore;
+    double2* Tall;
+    double2* Wp;
+    double2* alpha2;
+    double2* G;
+    double2* acol;
+    double* d;
+    double* e;
+    double2* tau;
+    double* scal;
+    double* part;
+    unsigned int* counter;
+    double2* Yr;
+    double2* Yc;
+    double2* t12;
+};

@@ -1,0 +1,326 @@
+// Internal state of a bse_ctx and the host helpers shared by the C ABI's
+// translation units (single-system path and batched sweep).
+#pragma once
+#include "bse_common.hpp"
+#include "../../include/bse_mi355x.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+// ---- kernel launchers implemented in the other translation units ----------
+extern "C" {
+hipError_t bse_launch_tb_states(hipStream_t, int, int, int, const double*, const double2*,
+                                const double*, const double*, int, int, int, double,
+                                const double2*, double*, double2*, double*, double2*);
+hipError_t bse_launch_w_table(hipStream_t, int, int, int, int, const double*, const double*,
+                              const double*, double, double, double, double, int, double2*);
+hipError_t bse_launch_v_exchange(hipStream_t, int, int, const double*, const double*, double,
+                                 double2*);
+hipError_t bse_launch_transitions(hipStream_t, int, int, int, int, const double*, const double2*,
+                                  const double*, const double2*, const double*, const double*,
+                                  const double2*, double2*, double2*, double2*, double*);
+size_t bse_assemble_lds_bytes(int, int, int, int);
+int bse_assemble_pick_tk(int, int, int);
+hipError_t bse_launch_assemble(hipStream_t, int, int, int, int, int, int, int, int, int64_t,
+                               const double2*, const double2*, const double2*, const double2*,
+                               const double2*, const double2*, const double2*, const double*,
+                               double, double, void*, int);
+hipError_t bse_launch_expand(hipStream_t, int, int64_t, int, int, const void*, int64_t, void*,
+                             int64_t, int);
+void bse_hetrd_workspace(int, int, int64_t*, int64_t*, int64_t*, int64_t*);
+hipError_t bse_hetrd(hipStream_t, int, int, void*, int64_t, int, double*, double*, double2*,
+                     double2*, const int64_t*, double2*, double2*, double2*, double2*, double2*,
+                     double2*, double2*, double2*, double*, double*, unsigned int*, hipEvent_t*,
+                     const signed char*);
+hipError_t bse_launch_hemv(hipStream_t, int, int, int, const void*, int64_t, const double2*,
+                           double2*, double2*);
+hipError_t bse_launch_last_diag(hipStream_t, int, int, const void*, int64_t, double*);
+hipError_t bse_hetrd_batch_column(hipStream_t, int, int, int, int, int, const void*, int);
+hipError_t bse_hetrd_batch_panel(hipStream_t, int, int, int, int, const void*, int64_t, int);
+hipError_t bse_launch_her2k(hipStream_t, int, int, int, void*, int64_t, const double2*,
+                            const double2*, int64_t, int, const double2*);
+hipError_t bse_launch_tri_prepare(hipStream_t, int, const double*, double*, double*, int4*,
+                                  double2*, double*);
+hipError_t bse_launch_tri_sort(hipStream_t, int, const double*, const int4*, const double*,
+                               double*, int*, int*, int2*, int*);
+hipError_t bse_launch_tri_vectors_dev(hipStream_t, int, const int4*, const double2*, const double*,
+                                      const double*, const double*, const double*, double*,
+                                      const double*, const int*, const int*, const int2*,
+                                      const int*, double*, double*, int);
+hipError_t bse_launch_apply_q(hipStream_t, int, int, const double2*, const double2*, int, double2*,
+                              int64_t, int);
+hipError_t bse_launch_copy_words(hipStream_t, int64_t, const void*, void*);
+hipError_t bse_launch_zero_words(hipStream_t, int64_t, void*);
+hipError_t bse_launch_stamp(hipStream_t, unsigned long long*);
+hipError_t bse_launch_bisect(hipStream_t, int, const int4*, const double2*, const double*,
+                             const double*, double, double, double*);
+hipError_t bse_launch_inverse_iter(hipStream_t, int, int, int, const double*, const int*,
+                                   const int*, const double*, const double*, double, double*, int,
+                                   double*);
+hipError_t bse_launch_cluster_mgs(hipStream_t, int, int, const int2*, int, const int*, const int*,
+                                  double*);
+hipError_t bse_launch_commit(hipStream_t, int, int, int, int, int, const int*, const int*,
+                             const double*, double2*, int64_t, int);
+hipError_t bse_launch_contract(hipStream_t, int, int, int, const int*, const int*, const double*,
+                               const double2*, double2*);
+hipError_t bse_zgemm(hipStream_t, int, int, int, int, int, double, double, const double2*, int64_t,
+                     const double2*, int64_t, double, double, double2*, int64_t, double2*, int64_t);
+hipError_t bse_launch_larft(hipStream_t, int, const double2*, const double2*, double2*, int);
+hipError_t bse_launch_dielectric(hipStream_t, int, const double*, double, int, const double*,
+                                 const double2*, double, double*, double*);
+hipError_t bse_launch_rows_to_reference(hipStream_t, int64_t, int, int, int, const double2*,
+                                        int64_t, double2*, int64_t);
+hipError_t bse_launch_convert_to_fp32(hipStream_t, int64_t, const double2*, float2*);
+hipError_t bse_launch_rows_from_reference(hipStream_t, int64_t, int, int, int, const double2*,
+                                          int64_t, double2*, int64_t);
+int bse_apply_pick_tk(int, int, int);
+int bse_assemble_mfma_supported(int, int, int);
+hipError_t bse_launch_assemble_mfma(hipStream_t, int, int, int, int, int, int, int, int64_t,
+                                    const double2*, const double2*, const double2*, const double2*,
+                                    const double*, double, double, void*);
+hipError_t bse_launch_apply(hipStream_t, int, int, int, int, int, int, int, const double2*,
+                            const double2*, const double2*, const double2*, const double2*,
+                            const double2*, const double2*, const double*, double, double, int,
+                            const double2*, int64_t, double2*, int64_t);
+hipError_t bse_launch_small_rr(hipStream_t, int, int, const double2*, const double2*, double,
+                               double*, double2*, int*);
+hipError_t bse_launch_residual(hipStream_t, int64_t, int, const double2*, const double2*, int64_t,
+                               double2*, double*);
+hipError_t bse_launch_coldot(hipStream_t, int64_t, int, const double2*, const double2*, int64_t,
+                             double*);
+}
+
+static const double kC = (1.602176634 * 1000.0) / 8.8541878176;   // e^2/eps0, eV*A
+static const int kPanel = 32;                                      // hetrd panel width
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+enum Stage { ST_TB, ST_TABLES, ST_ASM, ST_TRD, ST_TEVAL, ST_TVEC, ST_BT, ST_REF, ST_SPEC, ST_N };
+
+}  // namespace
+
+struct bse_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+    std::vector<DevBuf> bufs;   // indexed by slot id
+    // problem
+    bool have_problem = false;
+    bse_system sys{};
+    int norb = 0, nr = 0, nk = 0, ng = 0, B = 0, ns = 0;
+    int64_t N = 0;
+    double recip[9] = {0}, area = 0, volume = 0, wbar = 0, qhat[3] = {1, 0, 0};
+    std::vector<double> kpts_h, gvec_h, hr_h, rcart_h;
+    // state
+    bool have_tb = false, have_tables = false, built = false, have_eig = false, have_f = false;
+    int precision = BSE_PREC_FP64;
+    int64_t ldH = 0;
+    int64_t n_eig = 0;          // size of the last eigenproblem
+    int nvec = 0;               // eigenvectors formed
+    bool eig_on_bse = false;    // last eig was of the assembled BSE matrix
+    int TK = 1;
+    hipEvent_t ev[ST_N + 1] = {};
+    bool ev_used[ST_N] = {};
+    double stage_ms[16] = {0};
+    // optional per-launch timing of the tridiagonalisation's hemv kernel
+    bool prof = false;
+    std::vector<hipEvent_t> prof_ev;
+    double hemv_ms = 0.0, hemv_bytes = 0.0;
+    bool force_valu_assembly = false;   // BSE_ASSEMBLY=valu selects the VALU tiles
+    int asm_kernel = 0;                 // 0 VALU, 1 FP64 MFMA, 2 FP32 MFMA, 3 BF16x2 MFMA
+    int64_t hemv_launches = 0;
+    // autotuned hemv chunk length per floor(log2(trailing size)) bucket, one
+    // table per storage precision (0 = not tuned yet)
+    signed char hemv_ch[2][BSE_HEMV_BUCKETS] = {};
+    bool autotune = true;
+    hipEvent_t tev[2] = {};     // autotuner timing pair
+    // batched sweep state (owned by batch_api.hip)
+    struct BatchState* batch = nullptr;
+};
+
+namespace {
+
+enum Slot {
+    S_RCART, S_HR, S_TAU, S_KPTS, S_GVEC, S_Q, S_EPS, S_COEF, S_WTAB, S_VX, S_P, S_PV, S_RHO, S_DE,
+    S_H, S_D, S_E, S_TAUH, S_VSTORE, S_WP, S_ACOL, S_YR, S_YC, S_T12, S_PART, S_SCAL, S_COUNTER,
+    S_TALL, S_GRAM, S_E2, S_ITEMS, S_BOUNDS, S_LAMI, S_LAM, S_BSV, S_NBV, S_CLUST, S_IISCR, S_IIX,
+    S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_ALPHA2,
+    S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL,
+    S_GDHR, S_GDEPS, S_GDCOEF, S_GDP, S_GDPV, S_GDRHO, S_GDDE, S_GDW, S_GY, S_GOUT, S_NSLOTS
+};
+
+int fail(bse_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HCK(expr)                                                                       \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return fail(ctx, BSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+void* buf(bse_ctx* ctx, int slot, size_t bytes, int* status) {
+    DevBuf& b = ctx->bufs[slot];
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return b.p;
+    if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.cap = 0; }
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+        b.p = nullptr;
+        *status = fail(ctx, BSE_ERR_ALLOC, "hipMalloc failed for " + std::to_string(bytes) + " bytes");
+        return nullptr;
+    }
+    b.cap = bytes;
+    return b.p;
+}
+
+template <typename T>
+T* bufT(bse_ctx* ctx, int slot, size_t count, int* status) {
+    return (T*)buf(ctx, slot, count * sizeof(T), status);
+}
+
+#define BUF(T, slot, count)                                  \
+    ({                                                        \
+        int _s = BSE_OK;                                      \
+        T* _p = bufT<T>(ctx, slot, (size_t)(count), &_s);     \
+        if (!_p) return _s;                                   \
+        _p;                                                   \
+    })
+
+void tick(bse_ctx* ctx, int stage) { (void)hipEventRecord(ctx->ev[stage], ctx->st); }
+
+void stage_begin(bse_ctx* ctx, int stage) {
+    (void)hipEventRecord(ctx->ev[stage], ctx->st);
+    ctx->ev_used[stage] = true;
+}
+void stage_end(bse_ctx* ctx, int stage) {
+    (void)hipEventRecord(ctx->ev[ST_N], ctx->st);
+    (void)hipEventSynchronize(ctx->ev[ST_N]);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev[stage], ctx->ev[ST_N]);
+    ctx->stage_ms[stage] = ms;
+}
+
+bool finite_all(const double* p, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+// Geometry and sizes of one problem, derived on the host from the model and
+// the system parameters: Crystal_Lattice (bse.version13.cpp:85-124;
+// b_i = 2 pi (a_j x a_k) / Omega), K_points (183-242; k = ((i+s1)/n1) b1 +
+// ((j+s2)/n2) b2, index i*n2 + j), G_points (319-385; 2D parallelepiped,
+// G = 0 at index Ng/2), and the q = 0 mini-BZ average of the Keldysh W over
+// a disk of area (2 pi)^2 / (A Nk).
+struct Geom {
+    bse_system sys{};
+    int norb = 0, nr = 0, nk = 0, B = 0, ns = 0, ng = 0;
+    int64_t N = 0;
+    double recip[9] = {0}, volume = 0, area = 0, wbar = 0, qhat[3] = {1, 0, 0};
+    std::vector<double> kpts, gvec;
+};
+
+int make_geom(const bse_tb_model* m, const bse_system* s, Geom& g, std::string& msg) {
+    if (!m || !s || !m->r_cart || !m->h_r || !m->tau) { msg = "null argument"; return BSE_ERR_ARG; }
+    if (m->norb < 1 || m->norb > BSE_MAXORB) { msg = "norb must be in [1, 16]"; return BSE_ERR_UNSUPPORTED; }
+    if (m->nr < 1) { msg = "nr must be >= 1"; return BSE_ERR_ARG; }
+    if (s->n1 < 1 || s->n2 < 1) { msg = "k grid must be at least 1 x 1"; return BSE_ERR_ARG; }
+    if (s->nv < 1 || s->nc < 1 || s->n_occ < s->nv || s->n_occ + s->nc > m->norb) {
+        msg = "band window outside the Hamiltonian (nv <= n_occ, n_occ + nc <= norb)";
+        return BSE_ERR_ARG;
+    }
+    if (!(s->eps_env > 0.0) || !(s->r0 >= 0.0) || !(s->g_cut >= 0.0)) {
+        msg = "screening parameters must satisfy eps_env > 0, r0 >= 0, g_cut >= 0";
+        return BSE_ERR_ARG;
+    }
+    const size_t nh = (size_t)m->nr * m->norb * m->norb * 2;
+    if (!finite_all(m->h_r, nh) || !finite_all(m->r_cart, 3 * (size_t)m->nr) ||
+        !finite_all(m->tau, 3 * (size_t)m->norb) || !finite_all(s->bravais, 9)) {
+        msg = "non-finite model input";
+        return BSE_ERR_NUMERIC;
+    }
+    g.sys = *s;
+    g.norb = m->norb;
+    g.nr = m->nr;
+    g.nk = s->n1 * s->n2;
+    g.B = s->nv * s->nc;
+    g.ns = s->nv + s->nc;
+    g.N = (int64_t)g.nk * g.B;
+    const double* a = s->bravais;
+    auto cross = [](const double* x, const double* y, double* z) {
+        z[0] = x[1] * y[2] - x[2] * y[1];
+        z[1] = x[2] * y[0] - x[0] * y[2];
+        z[2] = x[0] * y[1] - x[1] * y[0];
+    };
+    double c0[3], c1[3], c2[3];
+    cross(a + 3, a + 6, c0);
+    cross(a + 6, a + 0, c1);
+    cross(a + 0, a + 3, c2);
+    const double det = a[0] * c0[0] + a[1] * c0[1] + a[2] * c0[2];
+    if (det == 0.0) { msg = "singular Bravais lattice"; return BSE_ERR_ARG; }
+    g.volume = fabs(det);
+    const double f = 2.0 * M_PI / g.volume;
+    for (int i = 0; i < 3; ++i) {
+        g.recip[i] = f * c0[i];
+        g.recip[3 + i] = f * c1[i];
+        g.recip[6 + i] = f * c2[i];
+    }
+    double axb[3];
+    cross(a, a + 3, axb);
+    g.area = sqrt(axb[0] * axb[0] + axb[1] * axb[1] + axb[2] * axb[2]);
+    g.kpts.assign(3 * (size_t)g.nk, 0.0);
+    for (int i = 0; i < s->n1; ++i)
+        for (int j = 0; j < s->n2; ++j) {
+            const double f1 = (i + s->shift[0]) / s->n1, f2 = (j + s->shift[1]) / s->n2;
+            double* kp = &g.kpts[3 * ((size_t)i * s->n2 + j)];
+            for (int x = 0; x < 3; ++x) kp[x] = f1 * g.recip[x] + f2 * g.recip[3 + x];
+        }
+    const double nb1 = sqrt(g.recip[0] * g.recip[0] + g.recip[1] * g.recip[1] + g.recip[2] * g.recip[2]);
+    const double nb2 = sqrt(g.recip[3] * g.recip[3] + g.recip[4] * g.recip[4] + g.recip[5] * g.recip[5]);
+    const int g1 = (int)(s->g_cut / nb1), g2 = (int)(s->g_cut / nb2);
+    g.gvec.clear();
+    for (int i = -g1; i <= g1; ++i)
+        for (int j = -g2; j <= g2; ++j)
+            for (int x = 0; x < 3; ++x) g.gvec.push_back(i * g.recip[x] + j * g.recip[3 + x]);
+    g.ng = (int)(g.gvec.size() / 3);
+    const double da = 4.0 * M_PI * M_PI / (g.area * g.nk);
+    const double rad = sqrt(da / M_PI);
+    g.wbar = (s->r0 > 0.0) ? (M_PI * kC / s->eps_env) * log1p(s->r0 * rad) / (s->r0 * da)
+                           : (M_PI * kC / s->eps_env) * rad / da;
+    double qh[3] = {s->dipole_dir[0], s->dipole_dir[1], s->dipole_dir[2]};
+    const double qn = sqrt(qh[0] * qh[0] + qh[1] * qh[1] + qh[2] * qh[2]);
+    if (!(qn > 0.0)) { qh[0] = 1.0; qh[1] = qh[2] = 0.0; }
+    else for (double& x : qh) x /= qn;
+    for (int x = 0; x < 3; ++x) g.qhat[x] = qh[x];
+    return BSE_OK;
+}
+
+}  // namespace
+
+// Chunk length for trailing size nt from an autotune table indexed by
+// floor(log2(nt)); untuned entries give the default 8.
+inline int hemv_ch_table_lookup(const signed char* table, int nt) {
+    if (!table || nt <= 0) return 8;
+    int b = 0;
+    while ((2 << b) <= nt && b < 30) ++b;
+    const int ch = table[b < BSE_HEMV_BUCKETS ? b : BSE_HEMV_BUCKETS - 1];
+    return (ch == 4 || ch == 8 || ch == 16) ? ch : 8;
+}
+
+// Releases the batched-sweep state of a context (batch_api.hip).
+void batch_destroy(bse_ctx* ctx);
+
+// Autotuner of the tridiagonalisation's hemv chunk length (bse_api.hip).
+int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, int64_t ld, double2* v,
+                  double2* Yr, double2* Yc);

@@ -48,14 +48,27 @@ __device__ inline double2 w_at(const HetrdCol& h, int64_t r, int l) {
     return cadd(h.Wp[r + (int64_t)l * h.m], cmul(h.alpha2[l], h.Vp[r + (int64_t)l * h.m]));
 }
 
-// (a) + (b): column update, norm, reflector scalars.
+// Block bodies.  Each kernel's work is written as a device function of the
+// block coordinates and of the grid it belongs to, so that the same code runs
+// as a single-matrix kernel (coordinates = blockIdx) and inside the batched
+// kernels further down, where blockIdx.z selects one matrix of an uneven batch
+// and blocks beyond that matrix's own grid return at once.
+
+// (a) + (b): column update, norm, reflector scalars.  Block 0 also clears
+// W(0:i, i) and alpha2[i] (rows above the reflector are never read, but the
+// whole column is kept defined).
 template <typename T>
-__global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict__ A)
+__device__ inline void colupd_body(const HetrdCol& h, const T* __restrict__ A, int bx, int gx)
 {
     __shared__ double sh[32];
     const int c = h.j0 + h.i;
     const int L = h.m - h.i;                 // rows c .. n-1
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = bx * blockDim.x + threadIdx.x;
+    if (bx == 0) {
+        if ((int)threadIdx.x <= h.i && (int)threadIdx.x < h.m)
+            h.Wp[threadIdx.x + (int64_t)h.i * h.m] = cmk(0, 0);
+        if (threadIdx.x == 0) h.alpha2[h.i] = cmk(0, 0);
+    }
     double nrm = 0.0;
     if (t < L) {
         const int lrow = h.i + t;
@@ -72,11 +85,11 @@ __global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict_
         if (t >= 2) nrm = cabs2(a);
     }
     nrm = block_sum(nrm, sh);
-    if (threadIdx.x == 0) h.part[blockIdx.x] = nrm;
-    if (!last_block_ticket(h.counter, gridDim.x)) return;
+    if (threadIdx.x == 0) h.part[bx] = nrm;
+    if (!last_block_ticket(h.counter, gx)) return;
     if (threadIdx.x == 0) {
         double xn2 = 0.0;
-        for (unsigned b = 0; b < gridDim.x; ++b) xn2 += h.part[b];
+        for (int b = 0; b < gx; ++b) xn2 += h.part[b];
         const double2 a0 = h.acol[0];
         h.d[c] = a0.x;
         const double2 alpha = h.acol[1];
@@ -96,6 +109,12 @@ __global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict_
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict__ A)
+{
+    colupd_body<T>(h, A, blockIdx.x, gridDim.x);
+}
+
 // Reflector vector into Vp(:, i), plus t1 = W^H v and t2 = V^H v over the
 // panel's earlier columns l < i (rows i+1 .. m-1).  Grid (row blocks, l):
 // block (rb, l) covers MAKEV_ROWS rows for one column l, so the dot products
@@ -104,10 +123,10 @@ __global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict_
 // factor.  The l = 0 blocks write the reflector.
 #define MAKEV_RPT 4
 #define MAKEV_ROWS (256 * MAKEV_RPT)
-__global__ __launch_bounds__(256) void k_makev(HetrdCol h, double2* __restrict__ t12)
+__device__ inline void makev_body(const HetrdCol& h, double2* __restrict__ t12, int rb, int l,
+                                  int nrb, int nl)
 {
     __shared__ double sh[32];
-    const int rb = blockIdx.x, l = blockIdx.y, nrb = gridDim.x;
     const double2 sc = cmk(h.scal[0], h.scal[1]);
     double2 p1 = cmk(0, 0), p2 = cmk(0, 0);
 #pragma unroll
@@ -132,7 +151,7 @@ __global__ __launch_bounds__(256) void k_makev(HetrdCol h, double2* __restrict__
             pp[((int64_t)l * nrb + rb) * 2 + 1] = p2;
         }
     }
-    if (!last_block_ticket(h.counter, gridDim.x * gridDim.y)) return;
+    if (!last_block_ticket(h.counter, nrb * nl)) return;
     const int t = threadIdx.x;
     if (t < 2 * h.i) {
         const int ll = t >> 1, which = t & 1;
@@ -147,31 +166,31 @@ __global__ __launch_bounds__(256) void k_makev(HetrdCol h, double2* __restrict__
     }
 }
 
+__global__ __launch_bounds__(256) void k_makev(HetrdCol h, double2* __restrict__ t12)
+{
+    makev_body(h, t12, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+}
+
 // (c): y = A22 v for the trailing Hermitian block (lower triangle stored).
 // Workgroup (q, bj) handles block column bj (64 columns) and block rows
 // bi in [bj + q*CH, bj + (q+1)*CH).  Row results of each 64x64 block go to
 // Yr[bi][bj]; the transposed (conjugate) contribution for column block bj is
 // accumulated across the chunk and written to Yc[bj][q].  One wave owns 16
 // columns; each load instruction reads 64 consecutive rows of one column.
-#ifndef HEMV_CH
-#define HEMV_CH 8
-#endif
-#ifndef HEMV_NT
-#define HEMV_NT 0
-#endif
-template <typename T>
-__global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict__ A, int64_t lda,
-                                                   const double2* __restrict__ v,
-                                                   double2* __restrict__ Yr,
-                                                   double2* __restrict__ Yc, int nbk, int nq)
+// The chunk length CH (block rows per workgroup) is chosen per trailing size
+// by the autotuner.
+#define HEMV_CH_MIN 4
+template <typename T, int CH>
+__device__ inline void hemv_body(int nt, const T* __restrict__ A, int64_t lda,
+                                 const double2* __restrict__ v, double2* __restrict__ Yr,
+                                 double2* __restrict__ Yc, int nbk, int nq, int q, int bj)
 {
-    const int q = blockIdx.x, bj = blockIdx.y;
-    const int bi0 = bj + q * HEMV_CH;
+    const int bi0 = bj + q * CH;
     if (bi0 >= nbk) return;
-    const int bi1 = min(nbk, bi0 + HEMV_CH);
+    const int bi1 = min(nbk, bi0 + CH);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     __shared__ double2 s_v[64];
-    __shared__ double2 s_part[4][HEMV_CH][64];
+    __shared__ double2 s_part[4][CH][64];
     const int col0 = bj * 64;
     if (threadIdx.x < 64) {
         const int cidx = col0 + threadIdx.x;
@@ -191,10 +210,7 @@ __global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict_
         if (bi > bj && (bi + 1) * 64 <= nt) {
             T a[16];
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                if (HEMV_NT) a[jj] = nt_load(Ap + (int64_t)jj * lda);
-                else a[jj] = Ap[(int64_t)jj * lda];
-            }
+            for (int jj = 0; jj < 16; ++jj) a[jj] = Ap[(int64_t)jj * lda];
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) {
                 const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
@@ -231,18 +247,26 @@ __global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict_
     }
 }
 
+template <typename T, int CH>
+__global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict__ A, int64_t lda,
+                                                   const double2* __restrict__ v,
+                                                   double2* __restrict__ Yr,
+                                                   double2* __restrict__ Yc, int nbk, int nq)
+{
+    hemv_body<T, CH>(nt, A, lda, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
+}
+
 // Reduce the hemv partials of one block row (64 rows) per workgroup, apply the
 // panel corrections (d) and the tau scaling (e), partial dot y^H v.  Sixteen
 // lane groups split the partial sums so that many loads are in flight.
 // Stores y in Wp(:, i); the last block computes alpha2[i] = -tau/2 (y^H v).
 #define RED_G 16
-__global__ __launch_bounds__(1024) void k_hemv_reduce(HetrdCol h, int nt, const double2* __restrict__ Yr,
-                                                      const double2* __restrict__ Yc, int nbk, int nq,
-                                                      const double2* __restrict__ t12)
+__device__ inline void reduce_body(const HetrdCol& h, int nt, const double2* __restrict__ Yr,
+                                   const double2* __restrict__ Yc, int nbk, int nq, int ch,
+                                   const double2* __restrict__ t12, int b, int gx)
 {
     __shared__ double2 sacc[RED_G][64];
     __shared__ double sh[32];
-    const int b = blockIdx.x;
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int r = b * 64 + lane;
     const bool valid = r < nt;
@@ -256,7 +280,7 @@ __global__ __launch_bounds__(1024) void k_hemv_reduce(HetrdCol h, int nt, const 
             acc1 = cadd(acc1, yr[(int64_t)(bj + RED_G) * 64]);
         }
         if (bj <= b) acc0 = cadd(acc0, yr[(int64_t)bj * 64]);
-        const int nqb = (int)ceil_div64(nbk - b, HEMV_CH);
+        const int nqb = (int)ceil_div64(nbk - b, ch);
         for (int q = g; q < nqb; q += RED_G) acc1 = cadd(acc1, Yc[((int64_t)b * nq + q) * 64 + lane]);
         for (int l = g; l < h.i; l += RED_G) {
             const double2 vv = h.Vp[lr + (int64_t)l * h.m];
@@ -278,35 +302,32 @@ __global__ __launch_bounds__(1024) void k_hemv_reduce(HetrdCol h, int nt, const 
     }
     dotp = block_sum2(dotp, sh);
     double2* pp = (double2*)h.part;
-    if (threadIdx.x == 0) pp[blockIdx.x] = dotp;
-    if (!last_block_ticket(h.counter, gridDim.x)) return;
+    if (threadIdx.x == 0) pp[b] = dotp;
+    if (!last_block_ticket(h.counter, gx)) return;
     if (threadIdx.x == 0) {
         double2 s = cmk(0, 0);
-        for (unsigned bb = 0; bb < gridDim.x; ++bb) s = cadd(s, pp[bb]);
+        for (int bb = 0; bb < gx; ++bb) s = cadd(s, pp[bb]);
         h.alpha2[h.i] = cscale(cmul(tau, s), -0.5);
     }
 }
 
-// Zero W(0:i, i) and alpha2 (rows above the reflector are never read, but the
-// whole column is kept defined).
-__global__ void k_wclear(HetrdCol h)
+__global__ __launch_bounds__(1024) void k_hemv_reduce(HetrdCol h, int nt, const double2* __restrict__ Yr,
+                                                      const double2* __restrict__ Yc, int nbk, int nq,
+                                                      int ch, const double2* __restrict__ t12)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r <= h.i && r < h.m) h.Wp[r + (int64_t)h.i * h.m] = cmk(0, 0);
-    if (r == 0) h.alpha2[h.i] = cmk(0, 0);
+    reduce_body(h, nt, Yr, Yc, nbk, nq, ch, t12, blockIdx.x, gridDim.x);
 }
 
 // Trailing update A22 -= V W^H + W V^H on lower 64x64 tiles, K = nb.
 // V, W rows start at panel row nb (ld m).
 template <typename T>
-__global__ __launch_bounds__(256) void k_her2k_lower(int n2, T* __restrict__ A, int64_t lda,
-                                                    const double2* __restrict__ V,
-                                                    const double2* __restrict__ W, int64_t ldv,
-                                                    int nb, const double2* __restrict__ alpha2)
+__device__ inline void her2k_valu_body(int n2, T* __restrict__ A, int64_t lda,
+                                       const double2* __restrict__ V,
+                                       const double2* __restrict__ W, int64_t ldv, int nb,
+                                       const double2* __restrict__ alpha2, int64_t idx)
 {
     int ti, tj;
     {
-        const int64_t idx = blockIdx.x;
         int64_t r = (int64_t)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
         while (r * (r + 1) / 2 > idx) --r;
         while ((r + 1) * (r + 2) / 2 <= idx) ++r;
@@ -402,16 +423,15 @@ template <> struct her2k_traits<float2> {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_her2k_mfma(int n2, T* __restrict__ A, int64_t lda,
-                                                   const double2* __restrict__ V,
-                                                   const double2* __restrict__ W, int64_t ldv,
-                                                   int nb, const double2* __restrict__ alpha2)
+__device__ inline void her2k_mfma_body(int n2, T* __restrict__ A, int64_t lda,
+                                       const double2* __restrict__ V,
+                                       const double2* __restrict__ W, int64_t ldv, int nb,
+                                       const double2* __restrict__ alpha2, int64_t idx)
 {
     typedef typename her2k_traits<T>::real R;
     typedef typename her2k_traits<T>::acc ACC;
     int ti, tj;
     {
-        const int64_t idx = blockIdx.x;
         int64_t r = (int64_t)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
         while (r * (r + 1) / 2 > idx) --r;
         while ((r + 1) * (r + 2) / 2 <= idx) ++r;
@@ -508,6 +528,24 @@ __global__ __launch_bounds__(256) void k_her2k_mfma(int n2, T* __restrict__ A, i
             }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_her2k_lower(int n2, T* __restrict__ A, int64_t lda,
+                                                    const double2* __restrict__ V,
+                                                    const double2* __restrict__ W, int64_t ldv,
+                                                    int nb, const double2* __restrict__ alpha2)
+{
+    her2k_valu_body<T>(n2, A, lda, V, W, ldv, nb, alpha2, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_her2k_mfma(int n2, T* __restrict__ A, int64_t lda,
+                                                   const double2* __restrict__ V,
+                                                   const double2* __restrict__ W, int64_t ldv,
+                                                   int nb, const double2* __restrict__ alpha2)
+{
+    her2k_mfma_body<T>(n2, A, lda, V, W, ldv, nb, alpha2, blockIdx.x);
+}
+
 __global__ void k_last_diag(int n, const void* A, int64_t lda, int fp32, double* d)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -517,14 +555,146 @@ __global__ void k_last_diag(int n, const void* A, int64_t lda, int fp32, double*
 }
 
 // ---------------------------------------------------------------------------
-// Host driver (runs on the context's stream; no host synchronisation).
+// Batched lockstep form: one launch per step serves every matrix of an uneven
+// batch.  Matrix b (blockIdx.z) has its own storage and workspace (HetrdSys);
+// column c of the reduction is processed for every matrix with c < n_b - 1,
+// and each block first checks that it lies inside its own matrix's grid.
+// The panel geometry is derived from (n, nb, c) exactly as in the
+// single-matrix driver: panel p = c / nb starts at j0 = p nb, its reflectors
+// sit at offset nb (p n - nb p (p-1)/2) of the packed store.
+// ---------------------------------------------------------------------------
+
+__host__ __device__ inline int64_t panel_voff(int n, int nb, int p) {
+    return (int64_t)nb * ((int64_t)p * n - (int64_t)nb * p * (p - 1) / 2);
+}
+
+__device__ inline bool batch_col(const HetrdSys& s, int c, int nb, HetrdCol& h) {
+    if (c >= s.n - 1) return false;
+    const int p = c / nb;
+    h.n = s.n; h.lda = s.lda; h.j0 = p * nb; h.i = c - h.j0; h.m = s.n - h.j0;
+    h.Vp = s.Vstore + panel_voff(s.n, nb, p);
+    h.Wp = s.Wp; h.alpha2 = s.alpha2; h.G = s.G; h.acol = s.acol; h.d = s.d; h.e = s.e;
+    h.tau = s.tau; h.scal = s.scal; h.part = s.part; h.counter = s.counter;
+    return true;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_colupd(const HetrdSys* __restrict__ sys, int c, int nb)
+{
+    HetrdCol h;
+    const HetrdSys s = sys[blockIdx.z];
+    if (!batch_col(s, c, nb, h)) return;
+    const int gx = (int)ceil_div64(h.m - h.i, 256);
+    if ((int)blockIdx.x >= gx) return;
+    colupd_body<T>(h, (const T*)s.A, blockIdx.x, gx);
+}
+
+__global__ __launch_bounds__(256) void kb_makev(const HetrdSys* __restrict__ sys, int c, int nb)
+{
+    HetrdCol h;
+    const HetrdSys s = sys[blockIdx.z];
+    if (!batch_col(s, c, nb, h)) return;
+    const int gx = (int)ceil_div64(h.m, MAKEV_ROWS), gy = max(h.i, 1);
+    if ((int)blockIdx.x >= gx || (int)blockIdx.y >= gy) return;
+    makev_body(h, s.t12, blockIdx.x, blockIdx.y, gx, gy);
+}
+
+template <typename T, int CH>
+__global__ __launch_bounds__(256) void kb_hemv(const HetrdSys* __restrict__ sys, int c, int nb)
+{
+    HetrdCol h;
+    const HetrdSys s = sys[blockIdx.z];
+    if (!batch_col(s, c, nb, h)) return;
+    const int nt = s.n - c - 1;
+    const int nbk = (int)ceil_div64(nt, 64), nq = (int)ceil_div64(nbk, CH);
+    if ((int)blockIdx.x >= nq || (int)blockIdx.y >= nbk) return;
+    const T* A22 = (const T*)s.A + (c + 1) + (int64_t)(c + 1) * s.lda;
+    const double2* vtr = h.Vp + (h.i + 1) + (int64_t)h.i * h.m;
+    hemv_body<T, CH>(nt, A22, s.lda, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(1024) void kb_reduce(const HetrdSys* __restrict__ sys, int c, int nb,
+                                                  int ch)
+{
+    HetrdCol h;
+    const HetrdSys s = sys[blockIdx.z];
+    if (!batch_col(s, c, nb, h)) return;
+    const int nt = s.n - c - 1;
+    const int nbk = (int)ceil_div64(nt, 64), nq = (int)ceil_div64(nbk, ch);
+    if ((int)blockIdx.x >= nbk) return;
+    reduce_body(h, nt, s.Yr, s.Yc, nbk, nq, ch, s.t12, blockIdx.x, nbk);
+}
+
+// Column c closes the matrix's current panel when c = j0 + nbp - 1.
+__device__ inline bool batch_panel_end(const HetrdSys& s, int c, int nb, int& j0, int& nbp, int& p) {
+    if (c >= s.n - 1) return false;
+    p = c / nb;
+    j0 = p * nb;
+    nbp = min(nb, s.n - 1 - j0);
+    return c == j0 + nbp - 1;
+}
+
+__global__ __launch_bounds__(64) void kb_larft(const HetrdSys* __restrict__ sys, int c, int nb)
+{
+    __shared__ double2 col[64];
+    const HetrdSys s = sys[blockIdx.z];
+    int j0, nbp, p;
+    if (!batch_panel_end(s, c, nb, j0, nbp, p)) return;
+    larft_body(nbp, s.tau + j0, s.G, s.Tall + (size_t)p * nb * nb, nb, col);
+}
+
+template <typename T, int MFMA>
+__global__ __launch_bounds__(256) void kb_her2k(const HetrdSys* __restrict__ sys, int c, int nb)
+{
+    const HetrdSys s = sys[blockIdx.z];
+    int j0, nbp, p;
+    if (!batch_panel_end(s, c, nb, j0, nbp, p)) return;
+    const int m = s.n - j0, n2 = m - nbp;
+    if (n2 <= 0) return;
+    const int64_t nt2 = ceil_div64(n2, 64);
+    if ((int64_t)blockIdx.x >= nt2 * (nt2 + 1) / 2) return;
+    T* A = (T*)s.A + (j0 + nbp) + (int64_t)(j0 + nbp) * s.lda;
+    const double2* Vp = s.Vstore + panel_voff(s.n, nb, p);
+    if (MFMA)
+        her2k_mfma_body<T>(n2, A, s.lda, Vp + nbp, s.Wp + nbp, (int64_t)m, nbp, s.alpha2, blockIdx.x);
+    else
+        her2k_valu_body<T>(n2, A, s.lda, Vp + nbp, s.Wp + nbp, (int64_t)m, nbp, s.alpha2, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Host drivers (run on the caller's stream; no host synchronisation).
 // ---------------------------------------------------------------------------
 extern "C" hipError_t bse_launch_larft(hipStream_t st, int nb, const double2* tau, const double2* G,
                                        double2* T, int ldt);
 
+// Chunk length of the hemv for trailing size nt: table indexed by
+// floor(log2(nt)) (autotuned per device), 0 entries mean the default 8.
+static inline int hemv_ch_for(const signed char* table, int nt) {
+    if (!table || nt <= 0) return 8;
+    int b = 0;
+    while ((2 << b) <= nt && b < 30) ++b;
+    const int ch = table[b < BSE_HEMV_BUCKETS ? b : BSE_HEMV_BUCKETS - 1];
+    return (ch == 4 || ch == 8 || ch == 16) ? ch : 8;
+}
+
+template <typename T>
+static void hemv_launch(hipStream_t st, int ch, int nt, const T* A, int64_t lda, const double2* v,
+                        double2* Yr, double2* Yc)
+{
+    const int nbk = (int)ceil_div64(nt, 64);
+    const int nq = (int)ceil_div64(nbk, ch);
+    if (ch == 4)
+        hipLaunchKernelGGL((k_hemv_lower<T, 4>), dim3(nq, nbk), dim3(256), 0, st, nt, A, lda, v, Yr, Yc, nbk, nq);
+    else if (ch == 16)
+        hipLaunchKernelGGL((k_hemv_lower<T, 16>), dim3(nq, nbk), dim3(256), 0, st, nt, A, lda, v, Yr, Yc, nbk, nq);
+    else
+        hipLaunchKernelGGL((k_hemv_lower<T, 8>), dim3(nq, nbk), dim3(256), 0, st, nt, A, lda, v, Yr, Yc, nbk, nq);
+}
+
 struct HetrdWork {
     int her2k_valu;      // 1: VALU trailing update (BSE_HER2K=valu), 0: MFMA
     hipEvent_t* prof_ev; // optional: 2 events per column around the hemv launch
+    const signed char* ch_table;  // hemv chunk length per log2(nt) bucket
     double2* Vstore;     // packed panel reflectors
     int64_t* voff;       // host array: offset of panel p in Vstore
     double2* Tall;       // compact-WY factors, nb x nb per panel
@@ -567,21 +737,20 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
             const int L = m - i;
             const int c = j0 + i;
             const int nt = n - c - 1;
-            hipLaunchKernelGGL(k_wclear, dim3((unsigned)ceil_div64(i + 1, 64)), dim3(64), 0, st, h);
             hipLaunchKernelGGL(k_colupd<T>, dim3((unsigned)ceil_div64(L, 256)), dim3(256), 0, st, h,
                                (const T*)A);
             hipLaunchKernelGGL(k_makev, dim3((unsigned)ceil_div64(m, MAKEV_ROWS), (unsigned)max(i, 1)),
                                dim3(256), 0, st, h, w.t12);
+            const int ch = hemv_ch_for(w.ch_table, nt);
             const int nbk = (int)ceil_div64(nt, 64);
-            const int nq = (int)ceil_div64(nbk, HEMV_CH);
+            const int nq = (int)ceil_div64(nbk, ch);
             const T* A22 = A + (c + 1) + (int64_t)(c + 1) * lda;
             const double2* vtr = h.Vp + (i + 1) + (int64_t)i * m;
             if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c], st);
-            hipLaunchKernelGGL(k_hemv_lower<T>, dim3(nq, nbk), dim3(256), 0, st, nt, A22, lda, vtr,
-                               w.Yr, w.Yc, nbk, nq);
+            hemv_launch<T>(st, ch, nt, A22, lda, vtr, w.Yr, w.Yc);
             if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c + 1], st);
             hipLaunchKernelGGL(k_hemv_reduce, dim3((unsigned)nbk), dim3(1024), 0, st, h, nt,
-                               (const double2*)w.Yr, (const double2*)w.Yc, nbk, nq,
+                               (const double2*)w.Yr, (const double2*)w.Yc, nbk, nq, ch,
                                (const double2*)w.t12);
             if ((err = hipGetLastError()) != hipSuccess) return err;
         }
@@ -610,7 +779,7 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
     return hipGetLastError();
 }
 
-// Workspace sizes for a given n, nb.
+// Workspace sizes for a given n, nb (Yc sized for the shortest hemv chunk).
 extern "C" void bse_hetrd_workspace(int n, int nb, int64_t* vstore_elems, int64_t* yr_elems,
                                     int64_t* yc_elems, int64_t* part_doubles)
 {
@@ -618,7 +787,7 @@ extern "C" void bse_hetrd_workspace(int n, int nb, int64_t* vstore_elems, int64_
     for (int j0 = 0; j0 < n - 1; j0 += nb) vs += (int64_t)(n - j0) * nb;
     *vstore_elems = vs > 0 ? vs : 1;
     const int64_t nbk = ceil_div64(n, 64);
-    const int64_t nq = ceil_div64(nbk, HEMV_CH);
+    const int64_t nq = ceil_div64(nbk, HEMV_CH_MIN);
     *yr_elems = nbk * nbk * 64;
     *yc_elems = nbk * nq * 64;
     // makev: 2 complex per (l, row block) for l < 64; reduce: 1 complex per block row
@@ -632,15 +801,102 @@ extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int64_
                                 const int64_t* voff, double2* Tall, double2* Wp, double2* alpha2,
                                 double2* G, double2* acol, double2* Yr, double2* Yc, double2* t12,
                                 double* part, double* scal, unsigned int* counter,
-                                hipEvent_t* prof_ev)
+                                hipEvent_t* prof_ev, const signed char* ch_table)
 {
     HetrdWork w;
     const char* hv = getenv("BSE_HER2K");
     w.her2k_valu = (hv != nullptr && hv[0] == 'v') ? 1 : 0;
     w.prof_ev = prof_ev;
+    w.ch_table = ch_table;
     w.Vstore = Vstore; w.voff = (int64_t*)voff; w.Tall = Tall; w.Wp = Wp; w.alpha2 = alpha2;
     w.G = G; w.acol = acol; w.Yr = Yr; w.Yc = Yc; w.t12 = t12; w.part = part; w.scal = scal;
     w.counter = counter;
     if (fp32) return hetrd_run<float2>(st, n, (float2*)A, lda, nb, d, e, tau, w);
     return hetrd_run<double2>(st, n, (double2*)A, lda, nb, d, e, tau, w);
+}
+
+// One hemv launch (autotuner timing): y partials of A (nt x nt, lower) v.
+extern "C" hipError_t bse_launch_hemv(hipStream_t st, int fp32, int ch, int nt, const void* A,
+                                      int64_t lda, const double2* v, double2* Yr, double2* Yc)
+{
+    if (nt <= 0 || (ch != 4 && ch != 8 && ch != 16)) return hipErrorInvalidValue;
+    if (fp32) hemv_launch<float2>(st, ch, nt, (const float2*)A, lda, v, Yr, Yc);
+    else hemv_launch<double2>(st, ch, nt, (const double2*)A, lda, v, Yr, Yc);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_launch_last_diag(hipStream_t st, int fp32, int n, const void* A,
+                                           int64_t lda, double* d)
+{
+    hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, A, lda, fp32, d);
+    return hipGetLastError();
+}
+
+// Column c of the batched reduction for the first nact matrices of d_sys
+// (sorted by decreasing n; n0 = n of matrix 0 fixes the grid extents).
+extern "C" hipError_t bse_hetrd_batch_column(hipStream_t st, int fp32, int c, int nb, int nact,
+                                             int n0, const void* d_sys, int ch)
+{
+    if (nact <= 0 || c >= n0 - 1) return hipSuccess;
+    const HetrdSys* sys = (const HetrdSys*)d_sys;
+    const int j0 = (c / nb) * nb, i = c - j0, m = n0 - j0, nt = n0 - c - 1;
+    const int nbk = (int)ceil_div64(nt, 64);
+    const unsigned z = (unsigned)nact;
+    if (fp32)
+        hipLaunchKernelGGL(kb_colupd<float2>, dim3((unsigned)ceil_div64(m - i, 256), 1, z), dim3(256), 0, st, sys, c, nb);
+    else
+        hipLaunchKernelGGL(kb_colupd<double2>, dim3((unsigned)ceil_div64(m - i, 256), 1, z), dim3(256), 0, st, sys, c, nb);
+    hipLaunchKernelGGL(kb_makev, dim3((unsigned)ceil_div64(m, MAKEV_ROWS), (unsigned)max(i, 1), z),
+                       dim3(256), 0, st, sys, c, nb);
+    const dim3 hg((unsigned)ceil_div64(nbk, ch), (unsigned)nbk, z);
+#define KBH(TT, CC) hipLaunchKernelGGL((kb_hemv<TT, CC>), hg, dim3(256), 0, st, sys, c, nb)
+    if (fp32) {
+        if (ch == 4) KBH(float2, 4); else if (ch == 16) KBH(float2, 16); else KBH(float2, 8);
+    } else {
+        if (ch == 4) KBH(double2, 4); else if (ch == 16) KBH(double2, 16); else KBH(double2, 8);
+    }
+#undef KBH
+    hipLaunchKernelGGL(kb_reduce, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch);
+    return hipGetLastError();
+}
+
+// Panel close at column c: compact-WY factor and trailing update for every
+// matrix whose current panel ends at c.  max_tiles: largest lower-tile count
+// among them (grid extent).
+extern "C" hipError_t bse_hetrd_batch_panel(hipStream_t st, int fp32, int c, int nb, int nact,
+                                            const void* d_sys, int64_t max_tiles, int her2k_valu)
+{
+    if (nact <= 0) return hipSuccess;
+    const HetrdSys* sys = (const HetrdSys*)d_sys;
+    const unsigned z = (unsigned)nact;
+    hipLaunchKernelGGL(kb_larft, dim3(1, 1, z), dim3(64), 0, st, sys, c, nb);
+    if (max_tiles > 0) {
+        const dim3 g((unsigned)max_tiles, 1, z);
+        if (fp32) {
+            if (her2k_valu) hipLaunchKernelGGL((kb_her2k<float2, 0>), g, dim3(256), 0, st, sys, c, nb);
+            else hipLaunchKernelGGL((kb_her2k<float2, 1>), g, dim3(256), 0, st, sys, c, nb);
+        } else {
+            if (her2k_valu) hipLaunchKernelGGL((kb_her2k<double2, 0>), g, dim3(256), 0, st, sys, c, nb);
+            else hipLaunchKernelGGL((kb_her2k<double2, 1>), g, dim3(256), 0, st, sys, c, nb);
+        }
+    }
+    return hipGetLastError();
+}
+
+// One trailing-update launch on an n2 x n2 lower block (autotuner timing).
+extern "C" hipError_t bse_launch_her2k(hipStream_t st, int fp32, int valu, int n2, void* A,
+                                       int64_t lda, const double2* V, const double2* W,
+                                       int64_t ldv, int nb, const double2* alpha2)
+{
+    if (n2 <= 0) return hipSuccess;
+    const int64_t nt2 = ceil_div64(n2, 64);
+    const dim3 g((unsigned)(nt2 * (nt2 + 1) / 2));
+    if (fp32) {
+        if (valu) hipLaunchKernelGGL(k_her2k_lower<float2>, g, dim3(256), 0, st, n2, (float2*)A, lda, V, W, ldv, nb, alpha2);
+        else hipLaunchKernelGGL(k_her2k_mfma<float2>, g, dim3(256), 0, st, n2, (float2*)A, lda, V, W, ldv, nb, alpha2);
+    } else {
+        if (valu) hipLaunchKernelGGL(k_her2k_lower<double2>, g, dim3(256), 0, st, n2, (double2*)A, lda, V, W, ldv, nb, alpha2);
+        else hipLaunchKernelGGL(k_her2k_mfma<double2>, g, dim3(256), 0, st, n2, (double2*)A, lda, V, W, ldv, nb, alpha2);
+    }
+    return hipGetLastError();
 }

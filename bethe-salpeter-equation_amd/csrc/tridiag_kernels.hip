@@ -35,12 +35,19 @@ __device__ inline int sturm_count(const double* __restrict__ d, const double* __
 
 // items: int4 (block start, block size, local index, block id); bounds per
 // block id: (lo, hi).  One 8-lane group per item.
+// prm (may be null): device copy of (pivmin, tnorm) that replaces the
+// by-value arguments, for launches recorded before those are known.
 __global__ __launch_bounds__(256) void k_bisect(int nitems, const int4* __restrict__ items,
                                                const double2* __restrict__ bounds,
                                                const double* __restrict__ d,
                                                const double* __restrict__ e2, double pivmin,
-                                               double tnorm, double* __restrict__ lam)
+                                               double tnorm, const double* __restrict__ prm,
+                                               double* __restrict__ lam)
 {
+    if (prm) {
+        pivmin = prm[0];
+        tnorm = prm[1];
+    }
     const int gid = (blockIdx.x * blockDim.x + threadIdx.x);
     const int item = gid >> 3, g = gid & 7;
     const int lane = threadIdx.x & 63;
@@ -93,11 +100,13 @@ __device__ inline double hash_uniform(uint64_t a, uint64_t b) {
 __global__ __launch_bounds__(64) void k_inverse_iter(
     int bsz, int cnt, int m0, const double* __restrict__ lam, const int* __restrict__ bsv,
     const int* __restrict__ nbv, const double* __restrict__ d, const double* __restrict__ e,
-    double tnorm, double* __restrict__ u0, double* __restrict__ u1, double* __restrict__ u2,
-    double* __restrict__ mul, double* __restrict__ pf, double* __restrict__ x)
+    double tnorm, const double* __restrict__ prm, double* __restrict__ u0,
+    double* __restrict__ u1, double* __restrict__ u2, double* __restrict__ mul,
+    double* __restrict__ pf, double* __restrict__ x)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
+    if (prm) tnorm = prm[1];
     const double lm = lam[m0 + t];
     const int bs = bsv[m0 + t], nb = nbv[m0 + t];
     const int64_t S = bsz;
@@ -178,12 +187,15 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
 // Modified Gram-Schmidt (two passes) inside clusters of close eigenvalues.
 // clusters: int2 (first batch index, size); one workgroup per cluster.  All
 // members of a cluster share the block (bs, nb).
+// ncl (may be null): device count of valid clusters; blocks beyond it return.
 __global__ __launch_bounds__(256) void k_cluster_mgs(int bsz, const int2* __restrict__ clusters,
-                                                     int m0, const int* __restrict__ bsv,
+                                                     const int* __restrict__ ncl, int m0,
+                                                     const int* __restrict__ bsv,
                                                      const int* __restrict__ nbv,
                                                      double* __restrict__ x)
 {
     __shared__ double sh[32];
+    if (ncl && (int)blockIdx.x >= *ncl) return;
     const int2 cl = clusters[blockIdx.x];
     const int t0 = cl.x, csz = cl.y;
     const int bs = bsv[m0 + t0], nb = nbv[m0 + t0];
@@ -269,7 +281,7 @@ extern "C" hipError_t bse_launch_bisect(hipStream_t st, int nitems, const int4* 
 {
     const int64_t threads = (int64_t)nitems * 8;
     hipLaunchKernelGGL(k_bisect, dim3((unsigned)ceil_div64(threads, 256)), dim3(256), 0, st,
-                       nitems, items, bounds, d, e2, pivmin, tnorm, lam);
+                       nitems, items, bounds, d, e2, pivmin, tnorm, (const double*)nullptr, lam);
     return hipGetLastError();
 }
 
@@ -281,8 +293,8 @@ extern "C" hipError_t bse_launch_inverse_iter(hipStream_t st, int bsz, int cnt, 
 {
     const int64_t A = (int64_t)n * bsz;
     hipLaunchKernelGGL(k_inverse_iter, dim3((unsigned)ceil_div64(cnt, 64)), dim3(64), 0, st, bsz,
-                       cnt, m0, lam, bsv, nbv, d, e, tnorm, scratch, scratch + A, scratch + 2 * A,
-                       scratch + 3 * A, scratch + 4 * A, x);
+                       cnt, m0, lam, bsv, nbv, d, e, tnorm, (const double*)nullptr, scratch,
+                       scratch + A, scratch + 2 * A, scratch + 3 * A, scratch + 4 * A, x);
     return hipGetLastError();
 }
 
@@ -290,7 +302,8 @@ extern "C" hipError_t bse_launch_cluster_mgs(hipStream_t st, int bsz, int ncl, c
                                              int m0, const int* bsv, const int* nbv, double* x)
 {
     if (ncl <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cluster_mgs, dim3(ncl), dim3(256), 0, st, bsz, clusters, m0, bsv, nbv, x);
+    hipLaunchKernelGGL(k_cluster_mgs, dim3(ncl), dim3(256), 0, st, bsz, clusters, (const int*)nullptr,
+                       m0, bsv, nbv, x);
     return hipGetLastError();
 }
 
@@ -311,5 +324,203 @@ extern "C" hipError_t bse_launch_contract(hipStream_t st, int bsz, int cnt, int 
 {
     hipLaunchKernelGGL(k_contract, dim3((unsigned)ceil_div64(cnt, 256)), dim3(256), 0, st, bsz, cnt,
                        m0, bsv, nbv, x, yc, f);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Device-only bookkeeping of the tridiagonal stage, for pipelines that run
+// without host round trips (the batched sweep inside one graph).  Same rules
+// as the host bookkeeping of the single-matrix path: split where
+// e_i^2 <= eps^2 |d_i d_{i+1}| + safmin, eigenvalues of every unreduced block
+// by bisection, ascending sort, clusters of eigenvalues closer than
+// 1e-7 ||T|| inside one block.  One workgroup per matrix, n <= TRI_DEV_MAXN.
+// ---------------------------------------------------------------------------
+#define TRI_DEV_MAXN 4096
+
+__device__ inline double block_max(double v, double* sh /* >= 16 */) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = sh[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) t = fmax(t, sh[i]);
+    return t;
+}
+
+// Norm, pivot threshold, splits (e and e2 updated in place), one bisection
+// item per row (block start, block size, local index, 0) and a common
+// Gershgorin interval bounds[0].  prm <- (pivmin, tnorm).
+__global__ __launch_bounds__(1024) void k_tri_prepare(int n, const double* __restrict__ d,
+                                                      double* __restrict__ e,
+                                                      double* __restrict__ e2,
+                                                      int4* __restrict__ items,
+                                                      double2* __restrict__ bounds,
+                                                      double* __restrict__ prm)
+{
+    __shared__ double sh[32];
+    __shared__ unsigned char s_split[TRI_DEV_MAXN];
+    __shared__ int s_bs[TRI_DEV_MAXN];
+    const int tid = threadIdx.x, bd = blockDim.x;
+    double tn = 0.0, em = 0.0;
+    for (int i = tid; i < n; i += bd) {
+        double r = fabs(d[i]);
+        if (i > 0) r += fabs(e[i - 1]);
+        if (i < n - 1) {
+            r += fabs(e[i]);
+            em = fmax(em, e[i] * e[i]);
+        }
+        tn = fmax(tn, r);
+    }
+    tn = block_max(tn, sh);
+    em = block_max(em, sh);
+    const double tnorm = (tn == 0.0) ? 1.0 : tn;
+    const double pivmin = DBL_MIN * fmax(1.0, em);
+    __syncthreads();
+    for (int i = tid; i < n - 1; i += bd) {
+        const double ee = e[i] * e[i];
+        const bool sp = fabs(d[i] * d[i + 1]) * DBL_EPSILON * DBL_EPSILON + DBL_MIN >= ee;
+        s_split[i] = sp ? 1 : 0;
+        e2[i] = sp ? 0.0 : ee;
+    }
+    __syncthreads();
+    double lo = DBL_MAX, hi = -DBL_MAX;
+    for (int i = tid; i < n; i += bd) {
+        double r = 0.0;
+        if (i > 0 && !s_split[i - 1]) r += fabs(e[i - 1]);
+        if (i < n - 1 && !s_split[i]) r += fabs(e[i]);
+        lo = fmin(lo, d[i] - r);
+        hi = fmax(hi, d[i] + r);
+    }
+    hi = block_max(hi, sh);
+    lo = -block_max(-lo, sh);
+    __syncthreads();
+    for (int i = tid; i < n - 1; i += bd)
+        if (s_split[i]) e[i] = 0.0;
+    if (tid == 0) {
+        int bs = 0;
+        for (int i = 0; i < n; ++i) {
+            if (i > 0 && s_split[i - 1]) bs = i;
+            s_bs[i] = bs;
+        }
+        const double fud = 2.1 * DBL_EPSILON * tnorm * n + 4.2 * pivmin;
+        bounds[0] = make_double2(lo - fud, hi + fud);
+        prm[0] = pivmin;
+        prm[1] = tnorm;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += bd) {
+        const int bs = s_bs[i];
+        int be = i + 1;
+        while (be < n && !s_split[be - 1]) ++be;
+        items[i] = make_int4(bs, be - bs, i - bs, 0);
+    }
+}
+
+// Ascending sort of the bisection results (ties by row), block data of each
+// sorted eigenvalue, and the cluster list (first index, size) with its count.
+__global__ __launch_bounds__(1024) void k_tri_sort(int n, const double* __restrict__ lami,
+                                                   const int4* __restrict__ items,
+                                                   const double* __restrict__ prm,
+                                                   double* __restrict__ lam, int* __restrict__ bsv,
+                                                   int* __restrict__ nbv, int2* __restrict__ clusters,
+                                                   int* __restrict__ ncl)
+{
+    __shared__ double key[TRI_DEV_MAXN];
+    __shared__ int idx[TRI_DEV_MAXN];
+    __shared__ int s_ncl;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int i = tid; i < P; i += bd) {
+        key[i] = (i < n) ? lami[i] : DBL_MAX;
+        idx[i] = i;
+    }
+    if (tid == 0) s_ncl = 0;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < P; i += bd) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool asc = (i & k) == 0;
+                    const bool gt = key[i] > key[l] || (key[i] == key[l] && idx[i] > idx[l]);
+                    if (gt == asc) {
+                        const double tk = key[i]; key[i] = key[l]; key[l] = tk;
+                        const int ti = idx[i]; idx[i] = idx[l]; idx[l] = ti;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    int myb[TRI_DEV_MAXN / 1024];
+    for (int i = tid, q = 0; i < n; i += bd, ++q) {
+        const int4 it = items[idx[i]];
+        lam[i] = key[i];
+        bsv[i] = it.x;
+        nbv[i] = it.y;
+        myb[q] = it.x;
+    }
+    __syncthreads();
+    for (int i = tid, q = 0; i < n; i += bd, ++q) idx[i] = myb[q];   // idx <- block start
+    __syncthreads();
+    const double tolc = 1e-7 * prm[1];
+    for (int i = tid; i < n; i += bd) {
+        const bool start = (i == 0) || idx[i] != idx[i - 1] || key[i] - key[i - 1] > tolc;
+        if (!start) continue;
+        int j = i + 1;
+        while (j < n && idx[j] == idx[j - 1] && key[j] - key[j - 1] <= tolc) ++j;
+        if (j - i > 1) {
+            const int slot = atomicAdd(&s_ncl, 1);
+            clusters[slot] = make_int2(i, j - i);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) *ncl = s_ncl;
+}
+
+extern "C" hipError_t bse_launch_tri_prepare(hipStream_t st, int n, const double* d, double* e,
+                                             double* e2, int4* items, double2* bounds, double* prm)
+{
+    if (n < 1 || n > TRI_DEV_MAXN) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tri_prepare, dim3(1), dim3(1024), 0, st, n, d, e, e2, items, bounds, prm);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_launch_tri_sort(hipStream_t st, int n, const double* lami,
+                                          const int4* items, const double* prm, double* lam,
+                                          int* bsv, int* nbv, int2* clusters, int* ncl)
+{
+    if (n < 1 || n > TRI_DEV_MAXN) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tri_sort, dim3(1), dim3(1024), 0, st, n, lami, items, prm, lam, bsv, nbv,
+                       clusters, ncl);
+    return hipGetLastError();
+}
+
+// Device-parameter forms of the bisection, inverse iteration and cluster
+// orthogonalisation (all n eigenpairs of one matrix in one batch of n).
+extern "C" hipError_t bse_launch_tri_vectors_dev(hipStream_t st, int n, const int4* items,
+                                                 const double2* bounds, const double* d,
+                                                 const double* e, const double* e2,
+                                                 const double* prm, double* lami,
+                                                 const double* lam, const int* bsv, const int* nbv,
+                                                 const int2* clusters, const int* ncl,
+                                                 double* scratch /* 5 n^2 */, double* x /* n^2 */,
+                                                 int stage)
+{
+    if (stage == 0) {
+        const int64_t threads = (int64_t)n * 8;
+        hipLaunchKernelGGL(k_bisect, dim3((unsigned)ceil_div64(threads, 256)), dim3(256), 0, st, n,
+                           items, bounds, d, e2, 0.0, 1.0, prm, lami);
+    } else {
+        const int64_t A = (int64_t)n * n;
+        hipLaunchKernelGGL(k_inverse_iter, dim3((unsigned)ceil_div64(n, 64)), dim3(64), 0, st, n, n,
+                           0, lam, bsv, nbv, d, e, 1.0, prm, scratch, scratch + A, scratch + 2 * A,
+                           scratch + 3 * A, scratch + 4 * A, x);
+        const int maxcl = (n + 1) / 2;
+        hipLaunchKernelGGL(k_cluster_mgs, dim3((unsigned)maxcl), dim3(256), 0, st, n, clusters, ncl,
+                           0, bsv, nbv, x);
+    }
     return hipGetLastError();
 }

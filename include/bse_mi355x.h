@@ -174,6 +174,43 @@ int  bse_stage_times(const bse_ctx* ctx, double* ms, int n);
  * trailing block), [2] launches, for the last bse_eigh. */
 int  bse_profile(bse_ctx* ctx, int enable, double* out);
 
+/* ---- K11: batched sweep of many small systems ------------------------------
+ * Many independent excitonic problems of uneven size (a parameter sweep) run
+ * as ONE graph launch: per-system TB states, tables and assembly, one
+ * lockstep Householder reduction whose every step serves all systems at
+ * once, then per-system tridiagonal eigenpairs, oscillator strengths,
+ * lowest-exciton vectors with optional FP64 refinement and the spectrum.
+ * Tile shapes (hemv chunk length per trailing size, trailing-update kernel)
+ * are picked by the library's autotuner on the batch itself.
+ * No counterpart in the reference (which handles one system per run); each
+ * system's results follow bse_build + bse_eigh + bse_spectrum.
+ * Limits: 2 <= N <= 4096 per system, 1 <= n_lowest <= 16.               */
+typedef struct {
+    int32_t precision;     /* BSE_PREC_*: assembly and storage precision       */
+    int32_t n_lowest;      /* excitons returned (and refined) per system       */
+    int32_t refine;        /* 1: FP64 Rayleigh-Ritz of the lowest n_lowest      */
+    int32_t n_omega;       /* spectrum points per system (0: no spectrum)      */
+    const double* omega;   /* n_omega frequencies in eV (host)                 */
+    double eta;            /* Lorentzian broadening in eV                      */
+} bse_batch_options;
+
+/* Upload all systems, size the device arena, autotune and record the graph.
+ * models[i] / systems[i] describe system i (as bse_set_problem).          */
+int bse_batch_prepare(bse_ctx* ctx, int n_systems, const bse_tb_model* models,
+                      const bse_system* systems, const bse_batch_options* opt);
+/* One launch of the recorded graph (inputs stay resident between runs).
+ * total_ms: launch-to-completion time (HIP events); system_ms (may be NULL):
+ * per system, time from the launch to that system's last kernel (device
+ * wall clock), in the order given to bse_batch_prepare.                    */
+int bse_batch_run(bse_ctx* ctx, double* total_ms, double* system_ms);
+/* Results of the last run, in input order: evals n_systems x n_lowest
+ * (row per system), eps_re / eps_im n_systems x n_omega (may be NULL).    */
+int bse_batch_results(bse_ctx* ctx, double* evals, double* eps_re, double* eps_im);
+/* info[0] graph nodes, [1] device arena bytes, [2] largest N, [3] systems,
+ * [4] hemv chunk length chosen for the largest trailing size,
+ * [5] trailing-update kernel (0 MFMA, 1 VALU).                             */
+int bse_batch_info(const bse_ctx* ctx, int64_t info[6]);
+
 #ifdef __cplusplus
 }
 #endif
