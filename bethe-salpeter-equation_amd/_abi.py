@@ -26,6 +26,7 @@ ABI_SYMBOLS = (
     "bse_eigh", "bse_get_eigenvalues", "bse_get_eigenvectors", "bse_oscillator",
     "bse_spectrum", "bse_eigh_dense", "bse_stage_times", "bse_profile", "bse_apply", "bse_grad_energies",
     "bse_batch_prepare", "bse_batch_run", "bse_batch_results", "bse_batch_info",
+    "bse_grad_spectrum",
 )
 
 
@@ -113,6 +114,7 @@ def load_library(path: pathlib.Path | str | None = None):
         "bse_batch_run": (i32, [vp, dp, dp]),
         "bse_batch_results": (i32, [vp, dp, dp, dp]),
         "bse_batch_info": (i32, [vp, C.POINTER(C.c_int64)]),
+        "bse_grad_spectrum": (i32, [vp, dp, i32, C.c_double, dp, C.POINTER(C.c_int), dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -229,6 +231,21 @@ class Context:
         self._check(self.lib.bse_grad_energies(self.h, int(n_exc), w, _dptr(out)),
                     "bse_grad_energies")
         return out.T.copy()
+
+    def grad_spectrum(self, omegas, eta, target, which=None):
+        """(L, dL/dtheta[5]) of the spectrum misfit (needs eigh(-1))."""
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        tg = np.ascontiguousarray(target, dtype=np.float64)
+        if tg.shape != om.shape:
+            raise ValueError("target must have one value per frequency")
+        w = None
+        if which is not None:
+            w = (C.c_int * 5)(*[1 if x else 0 for x in which])
+        L = C.c_double(0.0)
+        dL = np.zeros(5)
+        self._check(self.lib.bse_grad_spectrum(self.h, _dptr(om), om.size, float(eta), _dptr(tg), w,
+                                               C.byref(L), _dptr(dL)), "bse_grad_spectrum")
+        return L.value, dL
 
     def eigh(self, n_vectors=-1, refine=False):
         self._check(self.lib.bse_eigh(self.h, int(n_vectors), int(bool(refine))), "bse_eigh")

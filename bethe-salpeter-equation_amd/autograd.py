@@ -1,8 +1,11 @@
-"""torch.autograd binding of the Hellmann-Feynman gradient kernel.
+"""torch.autograd binding of the Hellmann-Feynman gradient kernels.
 
 ``exciton_energies(theta, base, n_exc)`` returns the lowest n_exc exciton
 energies of ``base`` with the parameters theta = (hopping scale s, scissor,
-r0, eps_env, exchange field h) applied, as a differentiable torch tensor.
+r0, eps_env, exchange field h) applied, as a differentiable torch tensor;
+``spectrum_misfit(theta, base, target)`` returns the spectrum misfit
+L = sum_w (Im eps(w) - target(w))^2 with its gradient from
+``bse_grad_spectrum``.
 Forward runs the FP64 GPU path (bse_build + bse_eigh); backward contracts the
 incoming gradient with dE/dtheta from ``bse_grad_energies`` (no derivative
 matrix is stored).  torch is used for the autograd graph only; all compute
@@ -61,3 +64,39 @@ def exciton_energies(theta, base: BSESystem, n_exc: int = 8, ctx: _abi.Context |
             return grad_E.to(g.dtype) @ g
 
     return _ExcitonEnergies.apply(theta)
+
+
+def spectrum_misfit_and_gradient(ctx: _abi.Context, base: BSESystem, theta, target,
+                                 omegas=None, eta=None):
+    """(L, dL/dtheta) of L = sum_w (Im eps(w) - target(w))^2 at theta (FP64 path,
+    all eigenvectors)."""
+    s = apply_theta(base, theta)
+    om = s.omegas() if omegas is None else np.asarray(omegas, dtype=np.float64)
+    et = s.eta if eta is None else float(eta)
+    ctx.set_system(s).build("fp64")
+    ctx.eigh(-1)
+    L, g = ctx.grad_spectrum(om, et, target)
+    g[0] = g[0] / float(theta[0])
+    return L, g
+
+
+def spectrum_misfit(theta, base: BSESystem, target, ctx: _abi.Context | None = None,
+                    omegas=None, eta=None):
+    """Differentiable spectrum misfit L(theta) (torch scalar)."""
+    import torch
+
+    class _SpectrumMisfit(torch.autograd.Function):
+        @staticmethod
+        def forward(fctx, th):
+            c = ctx if ctx is not None else _abi.Context(0)
+            L, g = spectrum_misfit_and_gradient(c, base, th.detach().cpu().numpy().astype(np.float64),
+                                                target, omegas, eta)
+            fctx.save_for_backward(torch.as_tensor(g, dtype=th.dtype))
+            return torch.as_tensor(L, dtype=th.dtype)
+
+        @staticmethod
+        def backward(fctx, grad_L):
+            (g,) = fctx.saved_tensors
+            return grad_L.to(g.dtype) * g
+
+    return _SpectrumMisfit.apply(theta)

@@ -14,6 +14,10 @@
 // row and a residue class of columns and accumulates its share of Y in
 // registers, so the only LDS traffic is the tile's pair intermediates and the
 // X rows of the current column tile.
+//
+// Trace mode (Bm != null, no vectors): instead of H X the kernel accumulates
+// tr(H Bm) = sum_rs H_rs Bm_sr over the same recomputed tiles and adds each
+// workgroup's share to tr_part[blockIdx.x] (fixed order, deterministic).
 #include "bse_common.hpp"
 
 #define APPLY_KMAX 32   // vectors per pass
@@ -32,6 +36,9 @@ struct ApplyArgs {
     double alpha, beta;
     const double2* X;
     double2* Y;
+    const double2* Bm;   // trace mode: N x N, ld ldb
+    int64_t ldb;
+    double2* tr_part;
 };
 
 __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
@@ -70,6 +77,7 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
             sPV[e] = (row < a.N) ? a.PVr[row * no + e % no] : cmk(0, 0);
         }
     }
+    double2 tacc = cmk(0, 0);
     double2 acc[APPLY_KMAX / 2];
 #pragma unroll
     for (int v = 0; v < APPLY_KMAX / 2; ++v) acc[v] = cmk(0, 0);
@@ -95,7 +103,7 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
                 sP[e] = (col < a.N) ? a.Pc[col * no + e % no] : cmk(0, 0);
             }
         }
-        for (int e = tid; e < TR * a.k; e += blockDim.x) {
+        for (int e = tid; a.Bm == nullptr && e < TR * a.k; e += blockDim.x) {
             const int s2 = e % TR, v = e / TR;
             const int64_t col = (int64_t)kp0 * B + s2;
             sX[s2 * a.k + v] = (col < a.N) ? a.X[col + (int64_t)v * a.ldx] : cmk(0, 0);
@@ -146,7 +154,13 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
             sT[e] = h;
         }
         __syncthreads();
-        if (grp < ngrp) {
+        if (a.Bm != nullptr) {
+            for (int e = tid; e < TR * TR; e += blockDim.x) {
+                const int r = e / TR, s2 = e % TR;
+                const int64_t gr = (int64_t)k0 * B + r, gc = (int64_t)kp0 * B + s2;
+                if (gr < a.N && gc < a.N) cfma(tacc, sT[e], a.Bm[gc + gr * a.ldb]);
+            }
+        } else if (grp < ngrp) {
             const double2* trow = sT + rr * TR;
 #pragma unroll
             for (int vi = 0; vi < APPLY_KMAX / 2; ++vi) {
@@ -157,6 +171,12 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
                 acc[vi] = s2;
             }
         }
+    }
+    if (a.Bm != nullptr) {
+        __shared__ double sh[32];
+        tacc = block_sum2(tacc, sh);
+        if (tid == 0) a.tr_part[blockIdx.x] = cadd(a.tr_part[blockIdx.x], tacc);
+        return;
     }
     if (grp < ngrp && grow < a.N) {
 #pragma unroll
@@ -192,7 +212,8 @@ extern "C" hipError_t bse_launch_apply(hipStream_t st, int nk, int n1, int n2, i
                                        const double2* Vl, const double2* Vr, const double2* wtab,
                                        const double2* PVr, const double2* Pc, const double* dE,
                                        double alpha, double beta, int k, const double2* X,
-                                       int64_t ldx, double2* Y, int64_t ldy)
+                                       int64_t ldx, double2* Y, int64_t ldy, const double2* Bm,
+                                       int64_t ldb, double2* tr_part)
 {
     ApplyArgs a;
     a.nk = nk; a.n1 = n1; a.n2 = n2; a.nv = nv; a.nc = nc; a.norb = norb; a.TK = TK;
@@ -203,6 +224,14 @@ extern "C" hipError_t bse_launch_apply(hipStream_t st, int nk, int n1, int n2, i
     // each of the 256/TR vector groups holds at most APPLY_KMAX/2 accumulators
     if (TR > 256 || (int64_t)APPLY_KMAX > (int64_t)(256 / TR) * (APPLY_KMAX / 2))
         return hipErrorInvalidValue;
+    a.Bm = Bm; a.ldb = ldb; a.tr_part = tr_part;
+    if (Bm != nullptr) {
+        if (tr_part == nullptr) return hipErrorInvalidValue;
+        a.k = 0; a.X = nullptr; a.ldx = 0; a.Y = nullptr; a.ldy = 0;
+        const size_t lds = apply_lds_elems(TK, nv, nc, norb, 0) * sizeof(double2);
+        hipLaunchKernelGGL(k_apply, dim3((unsigned)ceil_div64(nk, TK)), dim3(256), lds, st, a);
+        return hipGetLastError();
+    }
     for (int v0 = 0; v0 < k; v0 += APPLY_KMAX) {
         a.k = (k - v0 < APPLY_KMAX) ? (k - v0) : APPLY_KMAX;
         a.X = X + (int64_t)v0 * ldx; a.ldx = ldx;

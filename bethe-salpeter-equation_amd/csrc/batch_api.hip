@@ -219,7 +219,7 @@ static hipError_t record_apply(BatchState* b, int s, hipStream_t st, int k, cons
     if ((e = bse_launch_zero_words(st, 2 * g.N * k, Y))) return e;
     return bse_launch_apply(st, g.nk, sy.n1, sy.n2, sy.nv, sy.nc, g.norb,
                             bse_apply_pick_tk(sy.nv, sy.nc, g.norb), d.coef, d.coef, d.coef, d.coef,
-                            d.wtab, d.PV, d.P, d.dE, alpha, beta, k, X, g.N, Y, g.N);
+                            d.wtab, d.PV, d.P, d.dE, alpha, beta, k, X, g.N, Y, g.N, nullptr, 0, nullptr);
 }
 
 // Per-system work after the reduction: tridiagonal eigenpairs, oscillator

@@ -389,3 +389,66 @@ extern "C" hipError_t bse_launch_stamp(hipStream_t st, unsigned long long* dst)
     hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, st, dst);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Spectrum-misfit gradient factors.  With g_m(w) = 1/(w + i eta - E_m), the
+// misfit residual r_w = Im eps(w) - target(w) and oscillator strengths f_m:
+//   Fbr[m, w] = conj(f_m) g_m(w) r_w,   Fac[m, w] = conj(f_m g_m(w)),
+//   sa[m] = f_m sum_w r_w g_m(w),       sb[m] = conj(f_m) sum_w r_w g_m(w).
+// One thread per exciton m; column-major N x nw outputs (coalesced per w).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_spec_factors(int N, int nw, const double* __restrict__ omega,
+                                                      double eta, const double* __restrict__ E,
+                                                      const double2* __restrict__ f,
+                                                      const double* __restrict__ r,
+                                                      double2* __restrict__ Fbr,
+                                                      double2* __restrict__ Fac,
+                                                      double2* __restrict__ sa,
+                                                      double2* __restrict__ sb)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= N) return;
+    const double2 fm = f[m], fc = cconj(fm);
+    const double em = E[m];
+    double2 acc = cmk(0, 0);
+    for (int w = 0; w < nw; ++w) {
+        const double dw = omega[w] - em;
+        const double den = dw * dw + eta * eta;
+        const double2 g = cmk(dw / den, -eta / den);
+        const double rw = r[w];
+        const int64_t o = m + (int64_t)w * N;
+        Fbr[o] = cscale(cmul(fc, g), rw);
+        Fac[o] = cconj(cmul(fm, g));
+        acc = cadd(acc, cscale(g, rw));
+    }
+    sa[m] = cmul(fm, acc);
+    sb[m] = cmul(fc, acc);
+}
+
+// out[0] = sum_i B(i, i) (fixed order; one block of 1024 threads).
+__global__ __launch_bounds__(1024) void k_trace(int64_t N, const double2* __restrict__ B, int64_t ldb,
+                                                double2* __restrict__ out)
+{
+    __shared__ double sh[32];
+    double2 acc = cmk(0, 0);
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) acc = cadd(acc, B[i + i * ldb]);
+    acc = block_sum2(acc, sh);
+    if (threadIdx.x == 0) out[0] = cadd(out[0], acc);
+}
+
+extern "C" hipError_t bse_launch_spec_factors(hipStream_t st, int N, int nw, const double* omega,
+                                              double eta, const double* E, const double2* f,
+                                              const double* r, double2* Fbr, double2* Fac,
+                                              double2* sa, double2* sb)
+{
+    hipLaunchKernelGGL(k_spec_factors, dim3((unsigned)ceil_div64(N, 256)), dim3(256), 0, st, N, nw,
+                       omega, eta, E, f, r, Fbr, Fac, sa, sb);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_launch_trace(hipStream_t st, int64_t N, const double2* B, int64_t ldb,
+                                       double2* out)
+{
+    hipLaunchKernelGGL(k_trace, dim3(1), dim3(1024), 0, st, N, B, ldb, out);
+    return hipGetLastError();
+}

@@ -551,7 +551,7 @@ static int apply_H(bse_ctx* ctx, int k, const double2* X, int64_t ldx, double2* 
                          bse_apply_pick_tk(s.nv, s.nc, ctx->norb), coef, coef, coef, coef,
                          (const double2*)ctx->bufs[S_WTAB].p, (const double2*)ctx->bufs[S_PV].p,
                          (const double2*)ctx->bufs[S_P].p, (const double*)ctx->bufs[S_DE].p, alpha,
-                         beta, k, X, ldx, Y, ldy));
+                         beta, k, X, ldx, Y, ldy, nullptr, 0, nullptr));
     return BSE_OK;
 }
 
@@ -625,7 +625,8 @@ extern "C" int bse_apply(bse_ctx* ctx, int k, const double* x, int64_t ldx, doub
 // energies and coefficients (parallel-transport gauge); the gauge part of dc
 // drops out of A^H dH A.
 // ---------------------------------------------------------------------------
-static int grad_tb_apply(bse_ctx* ctx, int which, int k, const double2* A, double2* Y) {
+static int grad_tb_apply(bse_ctx* ctx, int which, int k, const double2* A, double2* Y,
+                         const double2* Bm = nullptr, double2* trp = nullptr) {
     const bse_system& s = ctx->sys;
     const int no = ctx->norb, nr = ctx->nr;
     std::vector<double> dh((size_t)nr * no * no * 2, 0.0);
@@ -666,23 +667,24 @@ static int grad_tb_apply(bse_ctx* ctx, int which, int k, const double2* A, doubl
     const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
     const double beta = s.f_s * s.exchange_scale / (ctx->area * ctx->nk);
     const int tk = bse_apply_pick_tk(s.nv, s.nc, no);
-    HCK(hipMemsetAsync(Y, 0, sizeof(double2) * ctx->N * k, ctx->st));
+    if (!Bm) HCK(hipMemsetAsync(Y, 0, sizeof(double2) * ctx->N * k, ctx->st));
     const double2* fac[4][4] = {{dcoef, coef, coef, coef}, {coef, dcoef, coef, coef},
                                 {coef, coef, dcoef, coef}, {coef, coef, coef, dcoef}};
     for (int t = 0; t < 4; ++t)
         HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no, tk, fac[t][0], fac[t][1],
                              fac[t][2], fac[t][3], wt, nullptr, nullptr, nullptr, alpha, 0.0, k, A,
-                             ctx->N, Y, ctx->N));
+                             ctx->N, Y, ctx->N, Bm, ctx->N, trp));
     // exchange: beta (dP V P^H + P V dP^H); diagonal d(eps_c - eps_v)
     HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no, tk, coef, coef, coef, coef, wt,
-                         dPV, (const double2*)ctx->bufs[S_P].p, ddE, 0.0, beta, k, A, ctx->N, Y, ctx->N));
+                         dPV, (const double2*)ctx->bufs[S_P].p, ddE, 0.0, beta, k, A, ctx->N, Y, ctx->N, Bm, ctx->N, trp));
     HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no, tk, coef, coef, coef, coef, wt,
                          (const double2*)ctx->bufs[S_PV].p, dP, nullptr, 0.0, beta, k, A, ctx->N, Y,
-                         ctx->N));
+                         ctx->N, Bm, ctx->N, trp));
     return BSE_OK;
 }
 
-static int grad_w_apply(bse_ctx* ctx, int which, int k, const double2* A, double2* Y) {
+static int grad_w_apply(bse_ctx* ctx, int which, int k, const double2* A, double2* Y,
+                        const double2* Bm = nullptr, double2* trp = nullptr) {
     const bse_system& s = ctx->sys;
     const int no = ctx->norb;
     const int64_t nw = (int64_t)(2 * s.n1 - 1) * (2 * s.n2 - 1) * no * no;
@@ -704,10 +706,10 @@ static int grad_w_apply(bse_ctx* ctx, int which, int k, const double2* A, double
                            which == 2 ? 1 : 2, dW));
     const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
     const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
-    HCK(hipMemsetAsync(Y, 0, sizeof(double2) * ctx->N * k, ctx->st));
+    if (!Bm) HCK(hipMemsetAsync(Y, 0, sizeof(double2) * ctx->N * k, ctx->st));
     HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no,
                          bse_apply_pick_tk(s.nv, s.nc, no), coef, coef, coef, coef, dW, nullptr,
-                         nullptr, nullptr, alpha, 0.0, k, A, ctx->N, Y, ctx->N));
+                         nullptr, nullptr, alpha, 0.0, k, A, ctx->N, Y, ctx->N, Bm, ctx->N, trp));
     return BSE_OK;
 }
 
@@ -730,6 +732,100 @@ extern "C" int bse_grad_energies(bse_ctx* ctx, int n_exc, const int* which, doub
         HCK(bse_launch_coldot(ctx->st, N, n_exc, A, Y, N, out));
         HCK(hipMemcpyAsync(dst, out, sizeof(double) * n_exc, hipMemcpyDeviceToHost, ctx->st));
         HCK(hipStreamSynchronize(ctx->st));
+    }
+    return BSE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// K10 (spectrum): gradient of the misfit L = sum_w (Im eps(w) - target(w))^2.
+// With G(z) = (z - H)^-1 = A diag(g) A^H and Im eps = -pref Im(rho^H G rho):
+//   dL/dtheta = -2 pref Im[ tr(dH Bm) + drho^H A sb + sa^T A^H drho ],
+//   Bm = Xr U^H,  Xr = A Fbr,  U = A Fac   (factors from k_spec_factors),
+// i.e. the sum over frequencies collapses into one N x N matrix Bm formed by
+// three GEMMs, and tr(dH Bm) is accumulated over dH tiles recomputed on the
+// fly (no stored derivative matrix).  Needs every eigenvector (bse_eigh with
+// n_vectors = -1).
+// ---------------------------------------------------------------------------
+extern "C" int bse_grad_spectrum(bse_ctx* ctx, const double* omega, int n_omega, double eta,
+                                 const double* target, const int* which, double* L, double* dL) {
+    if (!ctx || !omega || !target || !dL || n_omega <= 0 || !(eta > 0.0)) return BSE_ERR_ARG;
+    if (!ctx->have_eig || !ctx->eig_on_bse || !ctx->have_f)
+        return fail(ctx, BSE_ERR_STATE, "run bse_eigh on a built system first");
+    if (ctx->nvec != ctx->n_eig)
+        return fail(ctx, BSE_ERR_STATE, "the spectrum gradient needs every eigenvector (bse_eigh(ctx, -1, 0))");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->st;
+    const int64_t N = ctx->N;
+    const int nw = n_omega;
+    const double2* A = (const double2*)ctx->bufs[S_Z].p;
+    const double* E = (const double*)ctx->bufs[S_EVAL].p;
+    const double2* f = (const double2*)ctx->bufs[S_F].p;
+    // spectrum and misfit residual
+    double* d_v = BUF(double, S_SGV, 3 * (size_t)nw + 8 * (size_t)N);
+    double* d_om = d_v;
+    double* d_re = d_v + nw;
+    double* d_im = d_v + 2 * (size_t)nw;
+    double2* sa = (double2*)(d_v + 3 * (size_t)nw);
+    double2* sb = sa + N;
+    double2* p1 = sb + N;
+    double2* q = p1 + N;
+    HCK(hipMemcpyAsync(d_om, omega, sizeof(double) * nw, hipMemcpyHostToDevice, st));
+    const double pref = kC / (ctx->volume * ctx->nk);
+    HCK(bse_launch_dielectric(st, nw, d_om, eta, (int)N, E, f, pref, d_re, d_im));
+    std::vector<double> im(nw), r(nw);
+    HCK(hipMemcpyAsync(im.data(), d_im, sizeof(double) * nw, hipMemcpyDeviceToHost, st));
+    HCK(hipStreamSynchronize(st));
+    double loss = 0.0;
+    for (int w = 0; w < nw; ++w) {
+        r[w] = im[w] - target[w];
+        loss += r[w] * r[w];
+    }
+    if (L) *L = loss;
+    double* d_r = d_re;   // reuse: the real part is not needed
+    HCK(hipMemcpyAsync(d_r, r.data(), sizeof(double) * nw, hipMemcpyHostToDevice, st));
+    double2* Fbr = BUF(double2, S_SGFB, (size_t)N * nw);
+    double2* Fac = BUF(double2, S_SGFA, (size_t)N * nw);
+    double2* Xr = BUF(double2, S_SGX, (size_t)N * nw);
+    double2* U = BUF(double2, S_SGU, (size_t)N * nw);
+    double2* Bm = BUF(double2, S_SGB, (size_t)N * N);
+    HCK(bse_launch_spec_factors(st, (int)N, nw, d_om, eta, E, f, d_r, Fbr, Fac, sa, sb));
+    HCK(bse_zgemm(st, 0, 0, (int)N, nw, (int)N, 1, 0, A, N, Fbr, N, 0, 0, Xr, N, nullptr, 0));
+    HCK(bse_zgemm(st, 0, 0, (int)N, nw, (int)N, 1, 0, A, N, Fac, N, 0, 0, U, N, nullptr, 0));
+    HCK(bse_zgemm(st, 0, 1, (int)N, (int)N, nw, 1, 0, Xr, N, U, N, 0, 0, Bm, N, nullptr, 0));
+    HCK(bse_zgemm(st, 0, 0, (int)N, 1, (int)N, 1, 0, A, N, sb, N, 0, 0, p1, N, nullptr, 0));
+    const int tk = bse_apply_pick_tk(ctx->sys.nv, ctx->sys.nc, ctx->norb);
+    const int nblk = (int)ceil_div64(ctx->nk, tk);
+    double2* part = BUF(double2, S_SGPART, std::max(nblk, 1));
+    std::vector<double2> hpart(std::max(nblk, 1));
+    std::vector<double2> hv(3 * (size_t)N);
+    for (int t = 0; t < 5; ++t) {
+        if (which && !which[t]) { dL[t] = 0.0; continue; }
+        HCK(hipMemsetAsync(part, 0, sizeof(double2) * std::max(nblk, 1), st));
+        int rc = BSE_OK;
+        if (t == 1) HCK(bse_launch_trace(st, N, Bm, N, part));                       // dH = I
+        else if (t == 0 || t == 4) rc = grad_tb_apply(ctx, t, 0, nullptr, nullptr, Bm, part);
+        else rc = grad_w_apply(ctx, t, 0, nullptr, nullptr, Bm, part);
+        if (rc) return rc;
+        HCK(hipMemcpyAsync(hpart.data(), part, sizeof(double2) * std::max(nblk, 1), hipMemcpyDeviceToHost, st));
+        HCK(hipStreamSynchronize(st));
+        double2 term = cmk(0, 0);
+        for (int b = 0; b < std::max(nblk, 1); ++b) term = cadd(term, hpart[b]);
+        if (t == 0 || t == 4) {
+            // dipole derivative terms: drho^H (A sb) + sa^T (A^H drho)
+            const double2* drho = (const double2*)ctx->bufs[S_GDRHO].p;
+            HCK(bse_zgemm(st, 1, 0, (int)N, 1, (int)N, 1, 0, A, N, drho, N, 0, 0, q, N, nullptr, 0));
+            HCK(hipMemcpyAsync(hv.data(), drho, sizeof(double2) * N, hipMemcpyDeviceToHost, st));
+            HCK(hipMemcpyAsync(hv.data() + N, p1, sizeof(double2) * N, hipMemcpyDeviceToHost, st));
+            HCK(hipMemcpyAsync(hv.data() + 2 * N, q, sizeof(double2) * N, hipMemcpyDeviceToHost, st));
+            std::vector<double2> hsa(N);
+            HCK(hipMemcpyAsync(hsa.data(), sa, sizeof(double2) * N, hipMemcpyDeviceToHost, st));
+            HCK(hipStreamSynchronize(st));
+            for (int64_t i = 0; i < N; ++i) {
+                cfmac(term, hv[i], hv[N + i]);              // conj(drho_i) (A sb)_i
+                cfma(term, hsa[i], hv[2 * N + i]);          // sa_i (A^H drho)_i
+            }
+        }
+        dL[t] = -2.0 * pref * term.y;
     }
     return BSE_OK;
 }

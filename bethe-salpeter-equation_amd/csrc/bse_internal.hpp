@@ -56,6 +56,10 @@ hipError_t bse_launch_apply_q(hipStream_t, int, int, const double2*, const doubl
 hipError_t bse_launch_copy_words(hipStream_t, int64_t, const void*, void*);
 hipError_t bse_launch_zero_words(hipStream_t, int64_t, void*);
 hipError_t bse_launch_stamp(hipStream_t, unsigned long long*);
+hipError_t bse_launch_spec_factors(hipStream_t, int, int, const double*, double, const double*,
+                                   const double2*, const double*, double2*, double2*, double2*,
+                                   double2*);
+hipError_t bse_launch_trace(hipStream_t, int64_t, const double2*, int64_t, double2*);
 hipError_t bse_launch_bisect(hipStream_t, int, const int4*, const double2*, const double*,
                              const double*, double, double, double*);
 hipError_t bse_launch_inverse_iter(hipStream_t, int, int, int, const double*, const int*,
@@ -85,7 +89,8 @@ hipError_t bse_launch_assemble_mfma(hipStream_t, int, int, int, int, int, int, i
 hipError_t bse_launch_apply(hipStream_t, int, int, int, int, int, int, int, const double2*,
                             const double2*, const double2*, const double2*, const double2*,
                             const double2*, const double2*, const double*, double, double, int,
-                            const double2*, int64_t, double2*, int64_t);
+                            const double2*, int64_t, double2*, int64_t, const double2*, int64_t,
+                            double2*);
 hipError_t bse_launch_small_rr(hipStream_t, int, int, const double2*, const double2*, double,
                                double*, double2*, int*);
 hipError_t bse_launch_residual(hipStream_t, int64_t, int, const double2*, const double2*, int64_t,
@@ -155,7 +160,8 @@ enum Slot {
     S_TALL, S_GRAM, S_E2, S_ITEMS, S_BOUNDS, S_LAMI, S_LAM, S_BSV, S_NBV, S_CLUST, S_IISCR, S_IIX,
     S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_ALPHA2,
     S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL,
-    S_GDHR, S_GDEPS, S_GDCOEF, S_GDP, S_GDPV, S_GDRHO, S_GDDE, S_GDW, S_GY, S_GOUT, S_NSLOTS
+    S_GDHR, S_GDEPS, S_GDCOEF, S_GDP, S_GDPV, S_GDRHO, S_GDDE, S_GDW, S_GY, S_GOUT,
+    S_SGFB, S_SGFA, S_SGX, S_SGU, S_SGB, S_SGV, S_SGPART, S_NSLOTS
 };
 
 int fail(bse_ctx* c, int code, const std::string& msg) {

@@ -155,6 +155,16 @@ int  bse_spectrum(bse_ctx* ctx, const double* omega, int n_omega, double eta,
  * No counterpart in the reference (new capability). */
 int  bse_grad_energies(bse_ctx* ctx, int n_exc, const int* which, double* dE);
 
+/* K10 (spectrum): L = sum_w (Im eps(w) - target(w))^2 and dL/dtheta for the
+ * same five parameters, from the resolvent identity
+ *   dL/dtheta = -2 pref Im[tr(dH B) + drho^H G rho + rho^H G drho],
+ * B = A K^T A^H with the frequency sum folded in (three GEMMs), dH tiles
+ * recomputed on the fly.  Requires bse_eigh(ctx, -1, 0) (all eigenvectors).
+ * which: 5 flags (NULL = all); L (may be NULL); dL: 5 doubles.
+ * No counterpart in the reference (new capability). */
+int  bse_grad_spectrum(bse_ctx* ctx, const double* omega, int n_omega, double eta,
+                       const double* target, const int* which, double* L, double* dL);
+
 /* Dense Hermitian eig of any matrix (drop-in for arma::eig_sym, zheevd):
  * a: complex n x n (lower triangle read), doubles, host or device.
  * evals: n ascending; z: complex n x n_vectors (may be NULL).             */

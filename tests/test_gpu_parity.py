@@ -264,3 +264,51 @@ def test_torch_autograd_function(ctx):
     E.sum().backward()
     _, g = ag.energies_and_gradients(ctx, base, ag.default_theta(base), 8)
     assert np.allclose(th.grad.numpy(), g.sum(axis=0), rtol=1e-12, atol=1e-14)
+
+
+def _misfit_target(s, seed=5):
+    p = O.problem_from_system(s)
+    rng = np.random.default_rng(seed)
+    base_eps = O.solve(p)["eps"].imag
+    return p, base_eps * (1 + 0.1 * rng.normal(size=base_eps.size))
+
+
+def test_spectrum_gradient_matches_oracle(ctx):
+    s = bse_amd.random_multiorbital(3).with_params(n_omega=256)
+    p, target = _misfit_target(s)
+    g_ref, L_ref = O.spectrum_l2_gradient(p, target, params=PARAMS)
+    ctx.set_system(s).build("fp64").eigh(-1)
+    L, g = ctx.grad_spectrum(p.omegas, p.eta, target)
+    assert abs(L - L_ref) <= 1e-10 * abs(L_ref)
+    assert np.allclose(g, g_ref, rtol=1e-8, atol=1e-10 * np.abs(g_ref).max()), (g, g_ref)
+
+
+def test_spectrum_gradient_matches_gpu_finite_differences(ctx):
+    from bse_amd import autograd as ag
+    base = bse_amd.random_multiorbital(3).with_params(n_omega=256)
+    _, target = _misfit_target(base, seed=9)
+    th0 = ag.default_theta(base)
+    L0, g = ag.spectrum_misfit_and_gradient(ctx, base, th0, target)
+    # eta = 50 meV Lorentzians: central differences need h = 1e-5 to keep the
+    # O(h^2) truncation error below 1e-4 relative
+    h = 1e-5
+    for t in range(5):
+        tp, tm = th0.copy(), th0.copy()
+        tp[t] += h
+        tm[t] -= h
+        Lp, _ = ag.spectrum_misfit_and_gradient(ctx, base, tp, target)
+        Lm, _ = ag.spectrum_misfit_and_gradient(ctx, base, tm, target)
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(g[t] - fd) <= 1e-4 * abs(fd) + 1e-10 * abs(L0), (PARAMS[t], g[t], fd)
+
+
+def test_torch_spectrum_misfit(ctx):
+    import torch
+    from bse_amd import autograd as ag
+    base = bse_amd.random_multiorbital(2).with_params(n_omega=128)
+    _, target = _misfit_target(base, seed=2)
+    th = torch.tensor(ag.default_theta(base), dtype=torch.float64, requires_grad=True)
+    L = ag.spectrum_misfit(th, base, target, ctx=ctx)
+    L.backward()
+    _, g = ag.spectrum_misfit_and_gradient(ctx, base, ag.default_theta(base), target)
+    assert np.allclose(th.grad.numpy(), g, rtol=1e-12, atol=1e-14)
