@@ -398,28 +398,20 @@ __device__ inline void her2k_valu_body(int n2, T* __restrict__ A, int64_t lda,
 //   Re(P Q^H) = Pcat Qcat^T,   Im(P Q^H) = Pimc Qcat^T.
 // The kernel computes the transposed products Qcat Pcat^T so that the 16 lanes
 // holding one MFMA output column map to 16 consecutive rows of the
-// column-major matrix (coalesced read-modify-write).  FP64 storage uses
-// v_mfma_f64_16x16x4 with FP64 operands; FP32 storage uses v_mfma_f32_16x16x4
-// with the FP64 panels rounded to FP32 when staged into LDS.
+// column-major matrix (coalesced read-modify-write).  Both storage types use
+// v_mfma_f64_16x16x4 with the FP64 panels and FP64 accumulation, so an FP32
+// matrix is rounded once per update, exactly as the FP64-VALU form does; an
+// FP32 MFMA update (FP32 operands and accumulation) loses enough accuracy over
+// N/nb updates to slow the FP64 refinement of the lowest excitons.
 typedef double her2k_d4 __attribute__((ext_vector_type(4)));
-typedef float her2k_f4 __attribute__((ext_vector_type(4)));
 
-template <typename T> struct her2k_traits;
-template <> struct her2k_traits<double2> {
+struct her2k_f64 {
     typedef double real;
     typedef her2k_d4 acc;
     __device__ static inline acc mfma(real a, real b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
     __device__ static inline int row(int lane, int r) { return mfma16_row_f64(lane, r); }
-};
-template <> struct her2k_traits<float2> {
-    typedef float real;
-    typedef her2k_f4 acc;
-    __device__ static inline acc mfma(real a, real b, acc c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    __device__ static inline int row(int lane, int r) { return mfma16_row_f32(lane, r); }
 };
 
 template <typename T>
@@ -428,8 +420,8 @@ __device__ inline void her2k_mfma_body(int n2, T* __restrict__ A, int64_t lda,
                                        const double2* __restrict__ W, int64_t ldv, int nb,
                                        const double2* __restrict__ alpha2, int64_t idx)
 {
-    typedef typename her2k_traits<T>::real R;
-    typedef typename her2k_traits<T>::acc ACC;
+    typedef her2k_f64::real R;
+    typedef her2k_f64::acc ACC;
     int ti, tj;
     {
         int64_t r = (int64_t)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
@@ -501,13 +493,13 @@ __device__ inline void her2k_mfma_body(int n2, T* __restrict__ A, int64_t lda,
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    accR[a][b] = her2k_traits<T>::mfma(aQ[b], bP[a], accR[a][b]);
-                    accI[a][b] = her2k_traits<T>::mfma(aQ[b], bPi[a], accI[a][b]);
+                    accR[a][b] = her2k_f64::mfma(aQ[b], bP[a], accR[a][b]);
+                    accI[a][b] = her2k_f64::mfma(aQ[b], bPi[a], accI[a][b]);
                 }
         }
         __syncthreads();
     }
-    // lane holds D'[i][j = lane % 16] with i = her2k_traits<T>::row(lane, reg):
+    // lane holds D'[i][j = lane % 16] with i = her2k_f64::row(lane, reg):
     // column i, row j of the subtile
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -516,7 +508,7 @@ __device__ inline void her2k_mfma_body(int n2, T* __restrict__ A, int64_t lda,
 #pragma unroll
             for (int reg = 0; reg < 4; ++reg) {
                 const int gr = r0 + rowbase + a * 16 + (lane & 15);
-                const int gc = c0 + colbase + b * 16 + her2k_traits<T>::row(lane, reg);
+                const int gc = c0 + colbase + b * 16 + her2k_f64::row(lane, reg);
                 if (gr < n2 && gc < n2 && gr >= gc) {
                     T* p = A + gr + (int64_t)gc * lda;
                     double2 x = cplx<T>::load(p);
