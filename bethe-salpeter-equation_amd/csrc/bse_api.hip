@@ -30,6 +30,8 @@ extern "C" int bse_create(bse_ctx** out, int device) {
     }
     for (int i = 0; i <= ST_N; ++i) (void)hipEventCreate(&ctx->ev[i]);
     for (int i = 0; i < 2; ++i) (void)hipEventCreate(&ctx->tev[i]);
+    const char* lp = getenv("BSE_LDPAD");
+    if (lp) ctx->ld_pad = std::max(0, atoi(lp));
     const char* at = getenv("BSE_AUTOTUNE");
     ctx->autotune = !(at && at[0] == '0');
     const char* am = getenv("BSE_ASSEMBLY");
@@ -198,8 +200,11 @@ extern "C" int bse_build(bse_ctx* ctx, int precision) {
     const bse_system& s = ctx->sys;
     const int fp32 = precision != BSE_PREC_FP64;
     const int64_t N = ctx->N;
-    ctx->ldH = N;
-    void* H = buf(ctx, S_H, (size_t)N * N * (fp32 ? sizeof(float2) : sizeof(double2)), &rc);
+    // leading dimension padded by ld_pad elements (BSE_LDPAD): a column stride
+    // with a large power-of-two factor maps the hemv's column reads onto few
+    // HBM channels
+    ctx->ldH = N + ctx->ld_pad;
+    void* H = buf(ctx, S_H, (size_t)ctx->ldH * N * (fp32 ? sizeof(float2) : sizeof(double2)), &rc);
     if (!H) return rc;
     const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
     const double beta = s.f_s * s.exchange_scale / (ctx->area * ctx->nk);

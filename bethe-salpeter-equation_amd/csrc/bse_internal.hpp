@@ -147,6 +147,7 @@ struct bse_ctx {
     // table per storage precision (0 = not tuned yet)
     signed char hemv_ch[2][BSE_HEMV_BUCKETS] = {};
     bool autotune = true;
+    int ld_pad = 0;             // padding of the assembled matrix's leading dimension
     hipEvent_t tev[2] = {};     // autotuner timing pair
     // batched sweep state (owned by batch_api.hip)
     struct BatchState* batch = nullptr;
