@@ -74,6 +74,49 @@ __device__ inline double2 nt_load(const double2* p) {
     return make_double2(v.x, v.y);
 }
 
+// Loads through the global address space (global_load_* rather than flat
+// loads, which also count against the LDS wait counter).
+typedef __attribute__((address_space(1))) const bse_f2v bse_gf2v;
+typedef __attribute__((address_space(1))) const bse_d2v bse_gd2v;
+__device__ inline float2 gload(const float2* p) {
+    const bse_f2v v = *(bse_gf2v*)p;
+    return make_float2(v.x, v.y);
+}
+__device__ inline double2 gload(const double2* p) {
+    const bse_d2v v = *(bse_gd2v*)p;
+    return make_double2(v.x, v.y);
+}
+
+// Buffer loads: a wave-uniform base in a buffer descriptor (SGPRs), a
+// per-lane byte offset (one VGPR) and a uniform byte offset (SGPR), so a
+// wave reading 16 columns of a column-major matrix keeps one address VGPR
+// instead of sixteen 64-bit addresses.  Descriptor word 3 = 0x00020000
+// (gfx9 family); loads past num_records return zero.
+typedef int bse_i2v __attribute__((ext_vector_type(2)));
+typedef int bse_i4v __attribute__((ext_vector_type(4)));
+__device__ inline __amdgpu_buffer_rsrc_t bse_rsrc(const void* base, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
+}
+// Bit casts of single vector elements go through a scalar argument: a
+// __builtin_bit_cast applied directly to an element access (v.y) was compiled
+// as a cast of the vector's low 32 bits, dropping the imaginary part.
+__device__ inline float bse_i2f(int x) { return __builtin_bit_cast(float, x); }
+template <typename T> struct bufld;
+template <> struct bufld<float2> {
+    __device__ static inline float2 ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+        const bse_i2v v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        const int re = v[0], im = v[1];
+        return make_float2(bse_i2f(re), bse_i2f(im));
+    }
+};
+template <> struct bufld<double2> {
+    __device__ static inline double2 ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+        const bse_i4v v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        const bse_i2v lo = {v.x, v.y}, hi = {v.z, v.w};
+        return make_double2(__builtin_bit_cast(double, lo), __builtin_bit_cast(double, hi));
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Wave-level reductions (64-lane wavefronts on CDNA4).
 // ---------------------------------------------------------------------------
@@ -147,7 +190,7 @@ __host__ __device__ inline int64_t ceil_div64(int64_t a, int64_t b) { return (a 
 //   T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i),  T(i, i) = tau_i,
 // from the Gram column G(l, i) = V(:, l)^H v_i (ld 64).  One block of >= nb
 // threads (nb <= 64); col is a 64-entry shared scratch.
-__device__ inline void larft_body(int nb, const double2* __restrict__ tau,
+__device__ __attribute__((always_inline)) inline void larft_body(int nb, const double2* __restrict__ tau,
                                   const double2* __restrict__ G, double2* __restrict__ T, int ldt,
                                   double2* col)
 {
@@ -190,3 +233,15 @@ struct HetrdSys {
     double2* Yc;
     double2* t12;
 };
+
+// Hemv variant for trailing size nt from an autotune table indexed by
+// floor(log2(nt)): code = ch + 32 u2 (chunk length ch in {4, 8, 16}, u2 = two
+// row blocks per step); untuned entries give ch = 8, u2 = 0.
+inline int hemv_ch_table_lookup(const signed char* table, int nt) {
+    if (!table || nt <= 0) return 8;
+    int b = 0;
+    while ((2 << b) <= nt && b < 30) ++b;
+    const int code = table[b < BSE_HEMV_BUCKETS ? b : BSE_HEMV_BUCKETS - 1];
+    const int ch = code & 31, u2 = code >> 5;
+    return ((ch == 4 || ch == 8 || ch == 16) && (u2 == 0 || u2 == 1)) ? code : 8;
+}

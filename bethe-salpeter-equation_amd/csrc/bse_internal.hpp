@@ -315,16 +315,6 @@ int make_geom(const bse_tb_model* m, const bse_system* s, Geom& g, std::string& 
 
 }  // namespace
 
-// Chunk length for trailing size nt from an autotune table indexed by
-// floor(log2(nt)); untuned entries give the default 8.
-inline int hemv_ch_table_lookup(const signed char* table, int nt) {
-    if (!table || nt <= 0) return 8;
-    int b = 0;
-    while ((2 << b) <= nt && b < 30) ++b;
-    const int ch = table[b < BSE_HEMV_BUCKETS ? b : BSE_HEMV_BUCKETS - 1];
-    return (ch == 4 || ch == 8 || ch == 16) ? ch : 8;
-}
-
 // Releases the batched-sweep state of a context (batch_api.hip).
 void batch_destroy(bse_ctx* ctx);
 

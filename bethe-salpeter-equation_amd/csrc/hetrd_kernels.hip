@@ -58,7 +58,7 @@ __device__ inline double2 w_at(const HetrdCol& h, int64_t r, int l) {
 // W(0:i, i) and alpha2[i] (rows above the reflector are never read, but the
 // whole column is kept defined).
 template <typename T>
-__device__ inline void colupd_body(const HetrdCol& h, const T* __restrict__ A, int bx, int gx)
+__device__ __attribute__((always_inline)) inline void colupd_body(const HetrdCol& h, const T* __restrict__ A, int bx, int gx)
 {
     __shared__ double sh[32];
     const int c = h.j0 + h.i;
@@ -69,16 +69,40 @@ __device__ inline void colupd_body(const HetrdCol& h, const T* __restrict__ A, i
             h.Wp[threadIdx.x + (int64_t)h.i * h.m] = cmk(0, 0);
         if (threadIdx.x == 0) h.alpha2[h.i] = cmk(0, 0);
     }
+    // row i of the panel's V and W (the same for every thread), staged once
+    __shared__ double2 s_wc[64], s_vc[64], s_a2[64];
+    if ((int)threadIdx.x < h.i) {
+        const int l = threadIdx.x;
+        s_vc[l] = h.Vp[h.i + (int64_t)l * h.m];
+        s_wc[l] = w_at(h, h.i, l);
+        s_a2[l] = h.alpha2[l];
+    }
+    __syncthreads();
     double nrm = 0.0;
     if (t < L) {
         const int lrow = h.i + t;
         double2 a = cplx<T>::load(A + (c + t) + (int64_t)c * h.lda);
-        for (int l = 0; l < h.i; ++l) {
-            const double2 vr = h.Vp[lrow + (int64_t)l * h.m];
-            const double2 wr = w_at(h, lrow, l);
-            const double2 wc = w_at(h, h.i, l);
-            const double2 vc = h.Vp[h.i + (int64_t)l * h.m];
-            a = csub(a, cadd(cmul(vr, cconj(wc)), cmul(wr, cconj(vc))));
+        const double2* vp = h.Vp + lrow;
+        const double2* wp = h.Wp + lrow;
+        int l = 0;
+        // four columns per step: eight independent loads before any use
+        for (; l + 4 <= h.i; l += 4) {
+            double2 vr[4], wq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                vr[u] = vp[(int64_t)(l + u) * h.m];
+                wq[u] = wp[(int64_t)(l + u) * h.m];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double2 wr = cadd(wq[u], cmul(s_a2[l + u], vr[u]));
+                a = csub(a, cadd(cmul(vr[u], cconj(s_wc[l + u])), cmul(wr, cconj(s_vc[l + u]))));
+            }
+        }
+        for (; l < h.i; ++l) {
+            const double2 vr = vp[(int64_t)l * h.m];
+            const double2 wr = cadd(wp[(int64_t)l * h.m], cmul(s_a2[l], vr));
+            a = csub(a, cadd(cmul(vr, cconj(s_wc[l])), cmul(wr, cconj(s_vc[l]))));
         }
         if (t == 0) a.y = 0.0;
         h.acol[t] = a;
@@ -87,9 +111,10 @@ __device__ inline void colupd_body(const HetrdCol& h, const T* __restrict__ A, i
     nrm = block_sum(nrm, sh);
     if (threadIdx.x == 0) h.part[bx] = nrm;
     if (!last_block_ticket(h.counter, gx)) return;
+    double xp = 0.0;
+    for (int b = threadIdx.x; b < gx; b += blockDim.x) xp += h.part[b];
+    const double xn2 = block_sum(xp, sh);
     if (threadIdx.x == 0) {
-        double xn2 = 0.0;
-        for (int b = 0; b < gx; ++b) xn2 += h.part[b];
         const double2 a0 = h.acol[0];
         h.d[c] = a0.x;
         const double2 alpha = h.acol[1];
@@ -123,7 +148,7 @@ __global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict_
 // factor.  The l = 0 blocks write the reflector.
 #define MAKEV_RPT 4
 #define MAKEV_ROWS (256 * MAKEV_RPT)
-__device__ inline void makev_body(const HetrdCol& h, double2* __restrict__ t12, int rb, int l,
+__device__ __attribute__((always_inline)) inline void makev_body(const HetrdCol& h, double2* __restrict__ t12, int rb, int l,
                                   int nrb, int nl)
 {
     __shared__ double sh[32];
@@ -155,8 +180,16 @@ __device__ inline void makev_body(const HetrdCol& h, double2* __restrict__ t12, 
     const int t = threadIdx.x;
     if (t < 2 * h.i) {
         const int ll = t >> 1, which = t & 1;
-        double2 s = cmk(0, 0);
-        for (int b = 0; b < nrb; ++b) s = cadd(s, pp[((int64_t)ll * nrb + b) * 2 + which]);
+        double2 s = cmk(0, 0), s2 = cmk(0, 0);
+        int b = 0;
+        for (; b + 2 <= nrb; b += 2) {
+            const double2 x0 = pp[((int64_t)ll * nrb + b) * 2 + which];
+            const double2 x1 = pp[((int64_t)ll * nrb + b + 1) * 2 + which];
+            s = cadd(s, x0);
+            s2 = cadd(s2, x1);
+        }
+        if (b < nrb) s = cadd(s, pp[((int64_t)ll * nrb + b) * 2 + which]);
+        s = cadd(s, s2);
         if (which) {
             t12[64 + ll] = s;
             h.G[ll + (int64_t)h.i * 64] = s;
@@ -180,8 +213,8 @@ __global__ __launch_bounds__(256) void k_makev(HetrdCol h, double2* __restrict__
 // The chunk length CH (block rows per workgroup) is chosen per trailing size
 // by the autotuner.
 #define HEMV_CH_MIN 4
-template <typename T, int CH>
-__device__ inline void hemv_body(int nt, const T* __restrict__ A, int64_t lda,
+template <typename T, int CH, int U2>
+__device__ __attribute__((always_inline)) inline void hemv_body(int nt, const T* __restrict__ A, int64_t lda,
                                  const double2* __restrict__ v, double2* __restrict__ Yr,
                                  double2* __restrict__ Yc, int nbk, int nq, int q, int bj)
 {
@@ -197,24 +230,86 @@ __device__ inline void hemv_body(int nt, const T* __restrict__ A, int64_t lda,
         s_v[threadIdx.x] = (cidx < nt) ? v[cidx] : cmk(0, 0);
     }
     __syncthreads();
+    const double2* sv = s_v + w * 16;         // this wave's 16 entries of v (LDS)
     double2 cacc[16];
 #pragma unroll
     for (int jj = 0; jj < 16; ++jj) cacc[jj] = cmk(0, 0);
     const int cw = col0 + w * 16;             // first column of this wave
-    for (int bi = bi0; bi < bi1; ++bi) {
+    // interior blocks are read through a buffer descriptor based at this
+    // wave's first column: lane offset row * sizeof(T), column offset
+    // jj * lda * sizeof(T) in a scalar register (16 columns < 2^31 bytes)
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const __amdgpu_buffer_rsrc_t rs = bse_rsrc(A + (int64_t)(col0 + wu * 16) * lda, 0x7fffffff);
+    const int cstride = (int)(lda * (int64_t)sizeof(T));
+    const bool pack_ok = (lda & 1) == 0;      // 16-byte alignment of row pairs
+    for (int bi = bi0; bi < bi1;) {
         const int row = bi * 64 + lane;
-        const bool rvalid = row < nt;
-        const double2 vr = rvalid ? v[row] : cmk(0, 0);
-        double2 racc = cmk(0, 0);
         const T* Ap = A + row + (int64_t)cw * lda;
-        if (bi > bj && (bi + 1) * 64 <= nt) {
-            T a[16];
+        const bool full0 = bi > bj && (bi + 1) * 64 <= nt;
+        const int vo = row * (int)sizeof(T);
+        if (U2 && sizeof(T) == 8 && pack_ok && full0 && bi + 1 < bi1 && (bi + 2) * 64 <= nt) {
+            // FP32 storage, two interior row blocks: lane l reads the two
+            // consecutive rows 2l, 2l+1 of the 128-row slab in one 16-byte
+            // load (1 KB per wave instruction)
+            const int r2 = bi * 64 + 2 * lane;
+            bse_i4v q[16];
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) a[jj] = Ap[(int64_t)jj * lda];
+            for (int jj = 0; jj < 16; ++jj)
+                q[jj] = __builtin_amdgcn_raw_buffer_load_b128(rs, r2 * (int)sizeof(T), jj * cstride, 0);
+            const double2 v0 = v[r2], v1 = v[r2 + 1];
+            double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const int q0 = q[jj][0], q1 = q[jj][1], q2 = q[jj][2], q3 = q[jj][3];
+                const double2 x = cmk((double)bse_i2f(q0), (double)bse_i2f(q1));
+                const double2 y = cmk((double)bse_i2f(q2), (double)bse_i2f(q3));
+                cfma(r0, x, sv[jj]);
+                cfmac(cacc[jj], x, v0);
+                cfma(r1, y, sv[jj]);
+                cfmac(cacc[jj], y, v1);
+            }
+            // rows 2l, 2l+1 of the slab: block bi for l < 32, bi+1 above
+            const int hb = lane >> 5, rl = (2 * lane) & 63;
+            s_part[w][bi + hb - bi0][rl] = r0;
+            s_part[w][bi + hb - bi0][rl + 1] = r1;
+            bi += 2;
+            continue;
+        }
+        if (U2 && sizeof(T) == 16 && full0 && bi + 1 < bi1 && (bi + 2) * 64 <= nt) {
+            // two interior row blocks: 32 loads in flight per lane
+            T a[16], b[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                a[jj] = bufld<T>::ld(rs, vo, jj * cstride);
+                b[jj] = bufld<T>::ld(rs, vo + 64 * (int)sizeof(T), jj * cstride);
+            }
+            const double2 vr0 = v[row], vr1 = v[row + 64];
+            double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) {
                 const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
-                cfma(racc, x, s_v[w * 16 + jj]);
+                const double2 y = cmk((double)b[jj].x, (double)b[jj].y);
+                cfma(r0, x, sv[jj]);
+                cfmac(cacc[jj], x, vr0);
+                cfma(r1, y, sv[jj]);
+                cfmac(cacc[jj], y, vr1);
+            }
+            s_part[w][bi - bi0][lane] = r0;
+            s_part[w][bi + 1 - bi0][lane] = r1;
+            bi += 2;
+            continue;
+        }
+        const bool rvalid = row < nt;
+        const double2 vr = rvalid ? v[row] : cmk(0, 0);
+        double2 racc = cmk(0, 0);
+        if (full0) {
+            T a[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) a[jj] = bufld<T>::ld(rs, vo, jj * cstride);
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
+                cfma(racc, x, sv[jj]);
                 cfmac(cacc[jj], x, vr);
             }
         } else {
@@ -224,14 +319,15 @@ __device__ inline void hemv_body(int nt, const T* __restrict__ A, int64_t lda,
                 if (!rvalid || col >= nt || row < col) continue;
                 double2 a = cplx<T>::load(Ap + (int64_t)jj * lda);
                 if (row == col) {
-                    cfma(racc, cmk(a.x, 0.0), s_v[w * 16 + jj]);
+                    cfma(racc, cmk(a.x, 0.0), sv[jj]);
                 } else {
-                    cfma(racc, a, s_v[w * 16 + jj]);
+                    cfma(racc, a, sv[jj]);
                     cfmac(cacc[jj], a, vr);
                 }
             }
         }
         s_part[w][bi - bi0][lane] = racc;
+        ++bi;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < (bi1 - bi0) * 64; e += 256) {
@@ -247,13 +343,18 @@ __device__ inline void hemv_body(int nt, const T* __restrict__ A, int64_t lda,
     }
 }
 
-template <typename T, int CH>
-__global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict__ A, int64_t lda,
-                                                   const double2* __restrict__ v,
-                                                   double2* __restrict__ Yr,
-                                                   double2* __restrict__ Yc, int nbk, int nq)
+// Occupancy matters more than registers for this streaming kernel: without a
+// bound the compiler spends ~250 VGPRs and runs one wave per SIMD.  U2 (two
+// row blocks per step) holds 16 more loads per lane and gets one wave less.
+#ifndef HEMV_WPE
+#define HEMV_WPE 3
+#endif
+template <typename T, int CH, int U2>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U2 && HEMV_WPE > 1 ? HEMV_WPE - 1 : HEMV_WPE, 8)))
+void k_hemv_lower(int nt, const T* __restrict__ A, int64_t lda, const double2* __restrict__ v,
+                  double2* __restrict__ Yr, double2* __restrict__ Yc, int nbk, int nq)
 {
-    hemv_body<T, CH>(nt, A, lda, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    hemv_body<T, CH, U2>(nt, A, lda, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
 }
 
 // Reduce the hemv partials of one block row (64 rows) per workgroup, apply the
@@ -261,7 +362,7 @@ __global__ __launch_bounds__(256) void k_hemv_lower(int nt, const T* __restrict_
 // lane groups split the partial sums so that many loads are in flight.
 // Stores y in Wp(:, i); the last block computes alpha2[i] = -tau/2 (y^H v).
 #define RED_G 16
-__device__ inline void reduce_body(const HetrdCol& h, int nt, const double2* __restrict__ Yr,
+__device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol& h, int nt, const double2* __restrict__ Yr,
                                    const double2* __restrict__ Yc, int nbk, int nq, int ch,
                                    const double2* __restrict__ t12, int b, int gx)
 {
@@ -273,15 +374,25 @@ __device__ inline void reduce_body(const HetrdCol& h, int nt, const double2* __r
     const int lr = h.i + 1 + r;                         // panel-local row
     double2 acc0 = cmk(0, 0), acc1 = cmk(0, 0);
     if (valid) {
+        // partials of this block row: Yr[b][0..b] and Yc[b][0..nqb); groups of
+        // four independent loads are issued before any of them is summed
         const double2* yr = Yr + (int64_t)b * nbk * 64 + lane;
         int bj = g;
-        for (; bj + RED_G <= b; bj += 2 * RED_G) {
-            acc0 = cadd(acc0, yr[(int64_t)bj * 64]);
-            acc1 = cadd(acc1, yr[(int64_t)(bj + RED_G) * 64]);
+        for (; bj + 3 * RED_G <= b; bj += 4 * RED_G) {
+            const double2 x0 = yr[(int64_t)bj * 64], x1 = yr[(int64_t)(bj + RED_G) * 64];
+            const double2 x2 = yr[(int64_t)(bj + 2 * RED_G) * 64], x3 = yr[(int64_t)(bj + 3 * RED_G) * 64];
+            acc0 = cadd(acc0, cadd(x0, x1));
+            acc1 = cadd(acc1, cadd(x2, x3));
         }
-        if (bj <= b) acc0 = cadd(acc0, yr[(int64_t)bj * 64]);
+        for (; bj <= b; bj += RED_G) acc0 = cadd(acc0, yr[(int64_t)bj * 64]);
         const int nqb = (int)ceil_div64(nbk - b, ch);
-        for (int q = g; q < nqb; q += RED_G) acc1 = cadd(acc1, Yc[((int64_t)b * nq + q) * 64 + lane]);
+        const double2* yc = Yc + (int64_t)b * nq * 64 + lane;
+        int q = g;
+        for (; q + RED_G < nqb; q += 2 * RED_G) {
+            const double2 x0 = yc[(int64_t)q * 64], x1 = yc[(int64_t)(q + RED_G) * 64];
+            acc1 = cadd(acc1, cadd(x0, x1));
+        }
+        if (q < nqb) acc1 = cadd(acc1, yc[(int64_t)q * 64]);
         for (int l = g; l < h.i; l += RED_G) {
             const double2 vv = h.Vp[lr + (int64_t)l * h.m];
             const double2 ww = w_at(h, lr, l);
@@ -304,11 +415,10 @@ __device__ inline void reduce_body(const HetrdCol& h, int nt, const double2* __r
     double2* pp = (double2*)h.part;
     if (threadIdx.x == 0) pp[b] = dotp;
     if (!last_block_ticket(h.counter, gx)) return;
-    if (threadIdx.x == 0) {
-        double2 s = cmk(0, 0);
-        for (int bb = 0; bb < gx; ++bb) s = cadd(s, pp[bb]);
-        h.alpha2[h.i] = cscale(cmul(tau, s), -0.5);
-    }
+    double2 sp = cmk(0, 0);
+    for (int bb = threadIdx.x; bb < gx; bb += blockDim.x) sp = cadd(sp, pp[bb]);
+    const double2 s = block_sum2(sp, sh);
+    if (threadIdx.x == 0) h.alpha2[h.i] = cscale(cmul(tau, s), -0.5);
 }
 
 __global__ __launch_bounds__(1024) void k_hemv_reduce(HetrdCol h, int nt, const double2* __restrict__ Yr,
@@ -321,7 +431,7 @@ __global__ __launch_bounds__(1024) void k_hemv_reduce(HetrdCol h, int nt, const 
 // Trailing update A22 -= V W^H + W V^H on lower 64x64 tiles, K = nb.
 // V, W rows start at panel row nb (ld m).
 template <typename T>
-__device__ inline void her2k_valu_body(int n2, T* __restrict__ A, int64_t lda,
+__device__ __attribute__((always_inline)) inline void her2k_valu_body(int n2, T* __restrict__ A, int64_t lda,
                                        const double2* __restrict__ V,
                                        const double2* __restrict__ W, int64_t ldv, int nb,
                                        const double2* __restrict__ alpha2, int64_t idx)
@@ -415,7 +525,7 @@ struct her2k_f64 {
 };
 
 template <typename T>
-__device__ inline void her2k_mfma_body(int n2, T* __restrict__ A, int64_t lda,
+__device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n2, T* __restrict__ A, int64_t lda,
                                        const double2* __restrict__ V,
                                        const double2* __restrict__ W, int64_t ldv, int nb,
                                        const double2* __restrict__ alpha2, int64_t idx)
@@ -591,8 +701,9 @@ __global__ __launch_bounds__(256) void kb_makev(const HetrdSys* __restrict__ sys
     makev_body(h, s.t12, blockIdx.x, blockIdx.y, gx, gy);
 }
 
-template <typename T, int CH>
-__global__ __launch_bounds__(256) void kb_hemv(const HetrdSys* __restrict__ sys, int c, int nb)
+template <typename T, int CH, int U2>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U2 && HEMV_WPE > 1 ? HEMV_WPE - 1 : HEMV_WPE, 8)))
+void kb_hemv(const HetrdSys* __restrict__ sys, int c, int nb)
 {
     HetrdCol h;
     const HetrdSys s = sys[blockIdx.z];
@@ -602,7 +713,7 @@ __global__ __launch_bounds__(256) void kb_hemv(const HetrdSys* __restrict__ sys,
     if ((int)blockIdx.x >= nq || (int)blockIdx.y >= nbk) return;
     const T* A22 = (const T*)s.A + (c + 1) + (int64_t)(c + 1) * s.lda;
     const double2* vtr = h.Vp + (h.i + 1) + (int64_t)h.i * h.m;
-    hemv_body<T, CH>(nt, A22, s.lda, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    hemv_body<T, CH, U2>(nt, A22, s.lda, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
 }
 
 __global__ __launch_bounds__(1024) void kb_reduce(const HetrdSys* __restrict__ sys, int c, int nb,
@@ -659,28 +770,27 @@ __global__ __launch_bounds__(256) void kb_her2k(const HetrdSys* __restrict__ sys
 extern "C" hipError_t bse_launch_larft(hipStream_t st, int nb, const double2* tau, const double2* G,
                                        double2* T, int ldt);
 
-// Chunk length of the hemv for trailing size nt: table indexed by
-// floor(log2(nt)) (autotuned per device), 0 entries mean the default 8.
+// Hemv variant code for trailing size nt: ch + 32 u2 (chunk length ch in
+// {4, 8, 16}, u2 = two row blocks per step), from a table indexed by
+// floor(log2(nt)) (autotuned per device); 0 entries mean the default 8.
 static inline int hemv_ch_for(const signed char* table, int nt) {
-    if (!table || nt <= 0) return 8;
-    int b = 0;
-    while ((2 << b) <= nt && b < 30) ++b;
-    const int ch = table[b < BSE_HEMV_BUCKETS ? b : BSE_HEMV_BUCKETS - 1];
-    return (ch == 4 || ch == 8 || ch == 16) ? ch : 8;
+    return hemv_ch_table_lookup(table, nt);
 }
 
 template <typename T>
-static void hemv_launch(hipStream_t st, int ch, int nt, const T* A, int64_t lda, const double2* v,
+static void hemv_launch(hipStream_t st, int code, int nt, const T* A, int64_t lda, const double2* v,
                         double2* Yr, double2* Yc)
 {
+    const int ch = code & 31, u2 = code >> 5;
     const int nbk = (int)ceil_div64(nt, 64);
     const int nq = (int)ceil_div64(nbk, ch);
-    if (ch == 4)
-        hipLaunchKernelGGL((k_hemv_lower<T, 4>), dim3(nq, nbk), dim3(256), 0, st, nt, A, lda, v, Yr, Yc, nbk, nq);
-    else if (ch == 16)
-        hipLaunchKernelGGL((k_hemv_lower<T, 16>), dim3(nq, nbk), dim3(256), 0, st, nt, A, lda, v, Yr, Yc, nbk, nq);
-    else
-        hipLaunchKernelGGL((k_hemv_lower<T, 8>), dim3(nq, nbk), dim3(256), 0, st, nt, A, lda, v, Yr, Yc, nbk, nq);
+#define HL(CC, UU) hipLaunchKernelGGL((k_hemv_lower<T, CC, UU>), dim3(nq, nbk), dim3(256), 0, st, nt, A, lda, v, Yr, Yc, nbk, nq)
+    if (u2) {
+        if (ch == 4) HL(4, 1); else if (ch == 16) HL(16, 1); else HL(8, 1);
+    } else {
+        if (ch == 4) HL(4, 0); else if (ch == 16) HL(16, 0); else HL(8, 0);
+    }
+#undef HL
 }
 
 struct HetrdWork {
@@ -733,13 +843,14 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
                                (const T*)A);
             hipLaunchKernelGGL(k_makev, dim3((unsigned)ceil_div64(m, MAKEV_ROWS), (unsigned)max(i, 1)),
                                dim3(256), 0, st, h, w.t12);
-            const int ch = hemv_ch_for(w.ch_table, nt);
+            const int code = hemv_ch_for(w.ch_table, nt);
+            const int ch = code & 31;
             const int nbk = (int)ceil_div64(nt, 64);
             const int nq = (int)ceil_div64(nbk, ch);
             const T* A22 = A + (c + 1) + (int64_t)(c + 1) * lda;
             const double2* vtr = h.Vp + (i + 1) + (int64_t)i * m;
             if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c], st);
-            hemv_launch<T>(st, ch, nt, A22, lda, vtr, w.Yr, w.Yc);
+            hemv_launch<T>(st, code, nt, A22, lda, vtr, w.Yr, w.Yc);
             if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c + 1], st);
             hipLaunchKernelGGL(k_hemv_reduce, dim3((unsigned)nbk), dim3(1024), 0, st, h, nt,
                                (const double2*)w.Yr, (const double2*)w.Yc, nbk, nq, ch,
@@ -808,12 +919,13 @@ extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int64_
 }
 
 // One hemv launch (autotuner timing): y partials of A (nt x nt, lower) v.
-extern "C" hipError_t bse_launch_hemv(hipStream_t st, int fp32, int ch, int nt, const void* A,
+extern "C" hipError_t bse_launch_hemv(hipStream_t st, int fp32, int code, int nt, const void* A,
                                       int64_t lda, const double2* v, double2* Yr, double2* Yc)
 {
-    if (nt <= 0 || (ch != 4 && ch != 8 && ch != 16)) return hipErrorInvalidValue;
-    if (fp32) hemv_launch<float2>(st, ch, nt, (const float2*)A, lda, v, Yr, Yc);
-    else hemv_launch<double2>(st, ch, nt, (const double2*)A, lda, v, Yr, Yc);
+    const int ch = code & 31, u2 = code >> 5;
+    if (nt <= 0 || (ch != 4 && ch != 8 && ch != 16) || u2 > 1) return hipErrorInvalidValue;
+    if (fp32) hemv_launch<float2>(st, code, nt, (const float2*)A, lda, v, Yr, Yc);
+    else hemv_launch<double2>(st, code, nt, (const double2*)A, lda, v, Yr, Yc);
     return hipGetLastError();
 }
 
@@ -827,8 +939,9 @@ extern "C" hipError_t bse_launch_last_diag(hipStream_t st, int fp32, int n, cons
 // Column c of the batched reduction for the first nact matrices of d_sys
 // (sorted by decreasing n; n0 = n of matrix 0 fixes the grid extents).
 extern "C" hipError_t bse_hetrd_batch_column(hipStream_t st, int fp32, int c, int nb, int nact,
-                                             int n0, const void* d_sys, int ch)
+                                             int n0, const void* d_sys, int code)
 {
+    const int ch = code & 31, u2 = code >> 5;
     if (nact <= 0 || c >= n0 - 1) return hipSuccess;
     const HetrdSys* sys = (const HetrdSys*)d_sys;
     const int j0 = (c / nb) * nb, i = c - j0, m = n0 - j0, nt = n0 - c - 1;
@@ -841,11 +954,13 @@ extern "C" hipError_t bse_hetrd_batch_column(hipStream_t st, int fp32, int c, in
     hipLaunchKernelGGL(kb_makev, dim3((unsigned)ceil_div64(m, MAKEV_ROWS), (unsigned)max(i, 1), z),
                        dim3(256), 0, st, sys, c, nb);
     const dim3 hg((unsigned)ceil_div64(nbk, ch), (unsigned)nbk, z);
-#define KBH(TT, CC) hipLaunchKernelGGL((kb_hemv<TT, CC>), hg, dim3(256), 0, st, sys, c, nb)
+#define KBH(TT, CC, UU) hipLaunchKernelGGL((kb_hemv<TT, CC, UU>), hg, dim3(256), 0, st, sys, c, nb)
     if (fp32) {
-        if (ch == 4) KBH(float2, 4); else if (ch == 16) KBH(float2, 16); else KBH(float2, 8);
+        if (u2) { if (ch == 4) KBH(float2, 4, 1); else if (ch == 16) KBH(float2, 16, 1); else KBH(float2, 8, 1); }
+        else { if (ch == 4) KBH(float2, 4, 0); else if (ch == 16) KBH(float2, 16, 0); else KBH(float2, 8, 0); }
     } else {
-        if (ch == 4) KBH(double2, 4); else if (ch == 16) KBH(double2, 16); else KBH(double2, 8);
+        if (u2) { if (ch == 4) KBH(double2, 4, 1); else if (ch == 16) KBH(double2, 16, 1); else KBH(double2, 8, 1); }
+        else { if (ch == 4) KBH(double2, 4, 0); else if (ch == 16) KBH(double2, 16, 0); else KBH(double2, 8, 0); }
     }
 #undef KBH
     hipLaunchKernelGGL(kb_reduce, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch);

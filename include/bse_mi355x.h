@@ -217,7 +217,8 @@ int bse_batch_run(bse_ctx* ctx, double* total_ms, double* system_ms);
  * (row per system), eps_re / eps_im n_systems x n_omega (may be NULL).    */
 int bse_batch_results(bse_ctx* ctx, double* evals, double* eps_re, double* eps_im);
 /* info[0] graph nodes, [1] device arena bytes, [2] largest N, [3] systems,
- * [4] hemv chunk length chosen for the largest trailing size,
+ * [4] hemv variant chosen for the largest trailing size (chunk length in
+ *     block rows, plus 32 for the two-row-blocks-per-step form),
  * [5] trailing-update kernel (0 MFMA, 1 VALU).                             */
 int bse_batch_info(const bse_ctx* ctx, int64_t info[6]);
 
