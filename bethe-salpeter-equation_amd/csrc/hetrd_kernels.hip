@@ -19,8 +19,22 @@
 // and the V/W panels are always FP64.  Reflectors go to a separate packed
 // FP64 store so the back-transform is exact FP64 even for FP32 storage.
 //
-// All reductions are deterministic: per-block partials reduced in a fixed
-// order by the last block to finish ("ticket" counter).
+// All reductions are deterministic and need no atomics: every kernel of the
+// column step writes per-block partials, and every block of the kernel that
+// consumes the result sums them itself, in a fixed order (identical values
+// in every block).  A single "last block" counter would serialise one atomic
+// per block across the chip's XCDs, which costs more than the redundant sums.
+//
+// Per column c = j0 + i the kernels are
+//   colupd(i):  alpha2[i-1] from the reduce(i-1) partials; column update;
+//               norm partials                                  -> region A
+//   makev(i):   reflector scalars from region A; v_i; partial W^H v, V^H v
+//                                                              -> region B
+//   hemv(i):    y partials                                     -> Yr, Yc
+//   reduce(i):  W^H v, V^H v from region B; W(:, i); partial y^H v
+//                                                              -> region C
+// and at the panel end panel_close (alpha2 of the last column from region C,
+// compact-WY factor) before the trailing update.
 #include "bse_common.hpp"
 #include <cstdlib>
 
@@ -43,6 +57,23 @@ struct HetrdCol {
     unsigned int* counter;
 };
 
+// Partial-sum regions inside HetrdCol::part (doubles), fixed per matrix size.
+struct PartLayout {
+    int64_t a, b, c;
+};
+__host__ __device__ inline PartLayout part_layout(int n) {
+    PartLayout L;
+    const int64_t na = ceil_div64(n, 256) + 1;              // colupd blocks
+    const int64_t nb = 4 * 64 * (ceil_div64(n, 1024) + 1);  // 2 dots x complex x l x makev row blocks
+    L.a = 0;
+    L.b = na;
+    L.c = na + nb;                                          // + 2 (ceil(n/64) + 1): reduce blocks
+    return L;
+}
+__host__ __device__ inline int64_t part_doubles(int n) {
+    return part_layout(n).c + 2 * (ceil_div64(n, 64) + 1);
+}
+
 // W(r, l) of the panel, with the alpha2 v term applied on the fly.
 __device__ inline double2 w_at(const HetrdCol& h, int64_t r, int l) {
     return cadd(h.Wp[r + (int64_t)l * h.m], cmul(h.alpha2[l], h.Vp[r + (int64_t)l * h.m]));
@@ -54,28 +85,48 @@ __device__ inline double2 w_at(const HetrdCol& h, int64_t r, int l) {
 // kernels further down, where blockIdx.z selects one matrix of an uneven batch
 // and blocks beyond that matrix's own grid return at once.
 
-// (a) + (b): column update, norm, reflector scalars.  Block 0 also clears
-// W(0:i, i) and alpha2[i] (rows above the reflector are never read, but the
-// whole column is kept defined).
+// Sum of the complex partials p[0..cnt) in a fixed order, in every thread.
+__device__ inline double2 block_sum_partials(const double2* __restrict__ p, int cnt, double* sh) {
+    double2 x = cmk(0, 0);
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) x = cadd(x, p[k]);
+    return block_sum2(x, sh);
+}
+
+// (a): column update and its norm partials.  alpha2 of the previous column
+// (i - 1) is finished here from the reduce partials (prev_gx of them) by every
+// block; block 0 stores it and clears W(0:i, i) and alpha2[i] (rows above the
+// reflector are never read, but the whole column is kept defined).
 template <typename T>
-__device__ __attribute__((always_inline)) inline void colupd_body(const HetrdCol& h, const T* __restrict__ A, int bx, int gx)
+__device__ __attribute__((always_inline)) inline void colupd_body(const HetrdCol& h, const T* __restrict__ A, int bx, int gx,
+                                                                   int prev_gx)
 {
     __shared__ double sh[32];
+    __shared__ double2 s_wc[64], s_vc[64], s_a2[64];
     const int c = h.j0 + h.i;
     const int L = h.m - h.i;                 // rows c .. n-1
     const int t = bx * blockDim.x + threadIdx.x;
+    const PartLayout pl = part_layout(h.n);
+    double2 a2prev = cmk(0, 0);
+    if (h.i > 0) {
+        const double2 s = block_sum_partials((const double2*)(h.part + pl.c), prev_gx, sh);
+        a2prev = cscale(cmul(h.tau[c - 1], s), -0.5);
+    }
     if (bx == 0) {
         if ((int)threadIdx.x <= h.i && (int)threadIdx.x < h.m)
             h.Wp[threadIdx.x + (int64_t)h.i * h.m] = cmk(0, 0);
-        if (threadIdx.x == 0) h.alpha2[h.i] = cmk(0, 0);
+        if (threadIdx.x == 0) {
+            h.alpha2[h.i] = cmk(0, 0);
+            if (h.i > 0) h.alpha2[h.i - 1] = a2prev;
+        }
     }
     // row i of the panel's V and W (the same for every thread), staged once
-    __shared__ double2 s_wc[64], s_vc[64], s_a2[64];
     if ((int)threadIdx.x < h.i) {
         const int l = threadIdx.x;
-        s_vc[l] = h.Vp[h.i + (int64_t)l * h.m];
-        s_wc[l] = w_at(h, h.i, l);
-        s_a2[l] = h.alpha2[l];
+        const double2 a2 = (l == h.i - 1) ? a2prev : h.alpha2[l];
+        const double2 vc = h.Vp[h.i + (int64_t)l * h.m];
+        s_vc[l] = vc;
+        s_wc[l] = cadd(h.Wp[h.i + (int64_t)l * h.m], cmul(a2, vc));
+        s_a2[l] = a2;
     }
     __syncthreads();
     double nrm = 0.0;
@@ -85,16 +136,16 @@ __device__ __attribute__((always_inline)) inline void colupd_body(const HetrdCol
         const double2* vp = h.Vp + lrow;
         const double2* wp = h.Wp + lrow;
         int l = 0;
-        // four columns per step: eight independent loads before any use
-        for (; l + 4 <= h.i; l += 4) {
-            double2 vr[4], wq[4];
+        // eight columns per step: sixteen independent loads before any use
+        for (; l + 8 <= h.i; l += 8) {
+            double2 vr[8], wq[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 vr[u] = vp[(int64_t)(l + u) * h.m];
                 wq[u] = wp[(int64_t)(l + u) * h.m];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 const double2 wr = cadd(wq[u], cmul(s_a2[l + u], vr[u]));
                 a = csub(a, cadd(cmul(vr[u], cconj(s_wc[l + u])), cmul(wr, cconj(s_vc[l + u]))));
             }
@@ -109,50 +160,63 @@ __device__ __attribute__((always_inline)) inline void colupd_body(const HetrdCol
         if (t >= 2) nrm = cabs2(a);
     }
     nrm = block_sum(nrm, sh);
-    if (threadIdx.x == 0) h.part[bx] = nrm;
-    if (!last_block_ticket(h.counter, gx)) return;
+    if (threadIdx.x == 0) h.part[pl.a + bx] = nrm;
+}
+
+// (b): reflector scalars from the colupd partials, computed identically by
+// every block of the next kernel: beta = -sign(Re alpha) ||x||,
+// tau = (beta - alpha)/beta, scale 1/(alpha - beta).
+struct Refl {
+    double d, e;
+    double2 tau, sc;
+};
+__device__ inline Refl reflector_scalars(const HetrdCol& h, int colupd_gx, double* sh) {
+    const PartLayout pl = part_layout(h.n);
     double xp = 0.0;
-    for (int b = threadIdx.x; b < gx; b += blockDim.x) xp += h.part[b];
+    for (int b = threadIdx.x; b < colupd_gx; b += blockDim.x) xp += h.part[pl.a + b];
     const double xn2 = block_sum(xp, sh);
-    if (threadIdx.x == 0) {
-        const double2 a0 = h.acol[0];
-        h.d[c] = a0.x;
-        const double2 alpha = h.acol[1];
-        if (xn2 == 0.0 && alpha.y == 0.0) {
-            h.e[c] = alpha.x;
-            h.tau[c] = cmk(0, 0);
-            h.scal[0] = 0.0; h.scal[1] = 0.0; h.scal[2] = 0.0; h.scal[3] = 0.0;
-        } else {
-            const double nr = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-            const double beta = (alpha.x >= 0.0) ? -nr : nr;
-            const double2 tau = cmk((beta - alpha.x) / beta, -alpha.y / beta);
-            const double2 sc = cdiv(cmk(1, 0), cmk(alpha.x - beta, alpha.y));
-            h.e[c] = beta;
-            h.tau[c] = tau;
-            h.scal[0] = sc.x; h.scal[1] = sc.y; h.scal[2] = tau.x; h.scal[3] = tau.y;
-        }
+    Refl R;
+    R.d = h.acol[0].x;
+    const double2 alpha = h.acol[1];
+    if (xn2 == 0.0 && alpha.y == 0.0) {
+        R.e = alpha.x;
+        R.tau = cmk(0, 0);
+        R.sc = cmk(0, 0);
+    } else {
+        const double nr = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        const double beta = (alpha.x >= 0.0) ? -nr : nr;
+        R.e = beta;
+        R.tau = cmk((beta - alpha.x) / beta, -alpha.y / beta);
+        R.sc = cdiv(cmk(1, 0), cmk(alpha.x - beta, alpha.y));
     }
+    return R;
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict__ A)
+__global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict__ A, int prev_gx)
 {
-    colupd_body<T>(h, A, blockIdx.x, gridDim.x);
+    colupd_body<T>(h, A, blockIdx.x, gridDim.x, prev_gx);
 }
 
-// Reflector vector into Vp(:, i), plus t1 = W^H v and t2 = V^H v over the
-// panel's earlier columns l < i (rows i+1 .. m-1).  Grid (row blocks, l):
+// Reflector vector into Vp(:, i), plus partial t1 = W^H v and t2 = V^H v over
+// the panel's earlier columns l < i (rows i+1 .. m-1).  Grid (row blocks, l):
 // block (rb, l) covers MAKEV_ROWS rows for one column l, so the dot products
-// of all l run concurrently; the last block reduces the partials in a fixed
-// order.  t2 also fills column i of the Gram matrix used for the compact-WY
-// factor.  The l = 0 blocks write the reflector.
+// of all l run concurrently.  Every block first forms the reflector scalars
+// from the colupd partials; block (0, 0) stores d, e, tau and the scale, and
+// the l = 0 blocks write the reflector.
 #define MAKEV_RPT 4
 #define MAKEV_ROWS (256 * MAKEV_RPT)
-__device__ __attribute__((always_inline)) inline void makev_body(const HetrdCol& h, double2* __restrict__ t12, int rb, int l,
-                                  int nrb, int nl)
+__device__ __attribute__((always_inline)) inline void makev_body(const HetrdCol& h, int rb, int l, int nrb, int colupd_gx)
 {
     __shared__ double sh[32];
-    const double2 sc = cmk(h.scal[0], h.scal[1]);
+    const int c = h.j0 + h.i;
+    const Refl R = reflector_scalars(h, colupd_gx, sh);
+    if (rb == 0 && l == 0 && threadIdx.x == 0) {
+        h.d[c] = R.d;
+        h.e[c] = R.e;
+        h.tau[c] = R.tau;
+        h.scal[0] = R.sc.x; h.scal[1] = R.sc.y; h.scal[2] = R.tau.x; h.scal[3] = R.tau.y;
+    }
     double2 p1 = cmk(0, 0), p2 = cmk(0, 0);
 #pragma unroll
     for (int j = 0; j < MAKEV_RPT; ++j) {
@@ -160,48 +224,27 @@ __device__ __attribute__((always_inline)) inline void makev_body(const HetrdCol&
         if (r >= h.m) continue;
         double2 vv = cmk(0, 0);
         if (r == h.i + 1) vv = cmk(1, 0);
-        else if (r > h.i + 1) vv = cmul(h.acol[r - h.i], sc);
+        else if (r > h.i + 1) vv = cmul(h.acol[r - h.i], R.sc);
         if (l == 0) h.Vp[r + (int64_t)h.i * h.m] = vv;
         if (l < h.i && r > h.i) {
             cfmac(p1, w_at(h, r, l), vv);
             cfmac(p2, h.Vp[r + (int64_t)l * h.m], vv);
         }
     }
-    double2* pp = (double2*)h.part;
     if (h.i > 0) {
         p1 = block_sum2(p1, sh);
         p2 = block_sum2(p2, sh);
         if (threadIdx.x == 0) {
+            double2* pp = (double2*)(h.part + part_layout(h.n).b);
             pp[((int64_t)l * nrb + rb) * 2 + 0] = p1;
             pp[((int64_t)l * nrb + rb) * 2 + 1] = p2;
         }
     }
-    if (!last_block_ticket(h.counter, nrb * nl)) return;
-    const int t = threadIdx.x;
-    if (t < 2 * h.i) {
-        const int ll = t >> 1, which = t & 1;
-        double2 s = cmk(0, 0), s2 = cmk(0, 0);
-        int b = 0;
-        for (; b + 2 <= nrb; b += 2) {
-            const double2 x0 = pp[((int64_t)ll * nrb + b) * 2 + which];
-            const double2 x1 = pp[((int64_t)ll * nrb + b + 1) * 2 + which];
-            s = cadd(s, x0);
-            s2 = cadd(s2, x1);
-        }
-        if (b < nrb) s = cadd(s, pp[((int64_t)ll * nrb + b) * 2 + which]);
-        s = cadd(s, s2);
-        if (which) {
-            t12[64 + ll] = s;
-            h.G[ll + (int64_t)h.i * 64] = s;
-        } else {
-            t12[ll] = s;
-        }
-    }
 }
 
-__global__ __launch_bounds__(256) void k_makev(HetrdCol h, double2* __restrict__ t12)
+__global__ __launch_bounds__(256) void k_makev(HetrdCol h, int colupd_gx)
 {
-    makev_body(h, t12, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+    makev_body(h, blockIdx.x, blockIdx.y, gridDim.x, colupd_gx);
 }
 
 // (c): y = A22 v for the trailing Hermitian block (lower triangle stored).
@@ -360,24 +403,59 @@ void k_hemv_lower(int nt, const T* __restrict__ A, int64_t lda, const double2* _
 // Reduce the hemv partials of one block row (64 rows) per workgroup, apply the
 // panel corrections (d) and the tau scaling (e), partial dot y^H v.  Sixteen
 // lane groups split the partial sums so that many loads are in flight.
-// Stores y in Wp(:, i); the last block computes alpha2[i] = -tau/2 (y^H v).
+// t1 = W^H v and t2 = V^H v are first summed from the makev partials by every
+// block (block 0 also stores t2 as column i of the Gram matrix).  Stores y in
+// Wp(:, i) and the partial y^H v in region C; alpha2[i] = -tau/2 (y^H v) is
+// finished by the next kernel.
 #define RED_G 16
-__device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol& h, int nt, const double2* __restrict__ Yr,
-                                   const double2* __restrict__ Yc, int nbk, int nq, int ch,
-                                   const double2* __restrict__ t12, int b, int gx)
+#define RED_STAGE 2048        // makev partials staged in LDS (32 KB)
+__device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol& h, int nt,
+                                                                   const double2* __restrict__ Yr,
+                                                                   const double2* __restrict__ Yc,
+                                                                   int nbk, int nq, int ch,
+                                                                   int makev_nrb, int b)
 {
     __shared__ double2 sacc[RED_G][64];
+    __shared__ double2 s_t[2][64];
+    __shared__ double2 s_pp[RED_STAGE];
     __shared__ double sh[32];
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const PartLayout pl = part_layout(h.n);
+    if (h.i > 0) {
+        const double2* pp = (const double2*)(h.part + pl.b);
+        const int npp = 2 * h.i * makev_nrb;
+        const bool staged = npp <= RED_STAGE;
+        if (staged)
+            for (int e2 = threadIdx.x; e2 < npp; e2 += blockDim.x) s_pp[e2] = pp[e2];
+        __syncthreads();
+        if ((int)threadIdx.x < 2 * h.i) {
+            const int ll = threadIdx.x >> 1, which = threadIdx.x & 1;
+            double2 sum = cmk(0, 0);
+            for (int rb = 0; rb < makev_nrb; ++rb) {
+                const int64_t e2 = ((int64_t)ll * makev_nrb + rb) * 2 + which;
+                sum = cadd(sum, staged ? s_pp[e2] : pp[e2]);
+            }
+            s_t[which][ll] = sum;
+            if (b == 0 && which) h.G[ll + (int64_t)h.i * 64] = sum;
+        }
+        __syncthreads();
+    }
     const int r = b * 64 + lane;
     const bool valid = r < nt;
     const int lr = h.i + 1 + r;                         // panel-local row
     double2 acc0 = cmk(0, 0), acc1 = cmk(0, 0);
     if (valid) {
-        // partials of this block row: Yr[b][0..b] and Yc[b][0..nqb); groups of
-        // four independent loads are issued before any of them is summed
+        // partials of this block row: Yr[b][0..b] and Yc[b][0..nqb), several
+        // independent loads issued before any of them is summed
         const double2* yr = Yr + (int64_t)b * nbk * 64 + lane;
         int bj = g;
+        for (; bj + 7 * RED_G <= b; bj += 8 * RED_G) {
+            double2 x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = yr[(int64_t)(bj + u * RED_G) * 64];
+            acc0 = cadd(acc0, cadd(cadd(x[0], x[1]), cadd(x[2], x[3])));
+            acc1 = cadd(acc1, cadd(cadd(x[4], x[5]), cadd(x[6], x[7])));
+        }
         for (; bj + 3 * RED_G <= b; bj += 4 * RED_G) {
             const double2 x0 = yr[(int64_t)bj * 64], x1 = yr[(int64_t)(bj + RED_G) * 64];
             const double2 x2 = yr[(int64_t)(bj + 2 * RED_G) * 64], x3 = yr[(int64_t)(bj + 3 * RED_G) * 64];
@@ -396,13 +474,13 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
         for (int l = g; l < h.i; l += RED_G) {
             const double2 vv = h.Vp[lr + (int64_t)l * h.m];
             const double2 ww = w_at(h, lr, l);
-            acc0 = csub(acc0, cadd(cmul(vv, t12[l]), cmul(ww, t12[64 + l])));
+            acc0 = csub(acc0, cadd(cmul(vv, s_t[0][l]), cmul(ww, s_t[1][l])));
         }
     }
     sacc[g][lane] = cadd(acc0, acc1);
     __syncthreads();
     double2 dotp = cmk(0, 0);
-    const double2 tau = cmk(h.scal[2], h.scal[3]);
+    const double2 tau = h.tau[h.j0 + h.i];
     if (g == 0 && valid) {
         double2 y = cmk(0, 0);
 #pragma unroll
@@ -412,20 +490,32 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
         cfmac(dotp, y, h.Vp[lr + (int64_t)h.i * h.m]);
     }
     dotp = block_sum2(dotp, sh);
-    double2* pp = (double2*)h.part;
-    if (threadIdx.x == 0) pp[b] = dotp;
-    if (!last_block_ticket(h.counter, gx)) return;
-    double2 sp = cmk(0, 0);
-    for (int bb = threadIdx.x; bb < gx; bb += blockDim.x) sp = cadd(sp, pp[bb]);
-    const double2 s = block_sum2(sp, sh);
-    if (threadIdx.x == 0) h.alpha2[h.i] = cscale(cmul(tau, s), -0.5);
+    if (threadIdx.x == 0) ((double2*)(h.part + pl.c))[b] = dotp;
 }
 
 __global__ __launch_bounds__(1024) void k_hemv_reduce(HetrdCol h, int nt, const double2* __restrict__ Yr,
                                                       const double2* __restrict__ Yc, int nbk, int nq,
-                                                      int ch, const double2* __restrict__ t12)
+                                                      int ch, int makev_nrb)
 {
-    reduce_body(h, nt, Yr, Yc, nbk, nq, ch, t12, blockIdx.x, gridDim.x);
+    reduce_body(h, nt, Yr, Yc, nbk, nq, ch, makev_nrb, blockIdx.x);
+}
+
+// Panel close: alpha2 of the panel's last column (from its reduce partials,
+// red_gx of them) and the compact-WY factor of the panel.  One block of 64.
+__device__ __attribute__((always_inline)) inline void panel_close_body(const HetrdCol& h, int nbp, int red_gx,
+                                                                        double2* __restrict__ T, int ldt)
+{
+    __shared__ double sh[32];
+    __shared__ double2 col[64];
+    const int cl = h.j0 + nbp - 1;
+    const double2 s = block_sum_partials((const double2*)(h.part + part_layout(h.n).c), red_gx, sh);
+    if (threadIdx.x == 0) h.alpha2[nbp - 1] = cscale(cmul(h.tau[cl], s), -0.5);
+    larft_body(nbp, h.tau + h.j0, h.G, T, ldt, col);
+}
+
+__global__ __launch_bounds__(64) void k_panel_close(HetrdCol h, int nbp, int red_gx, double2* T, int ldt)
+{
+    panel_close_body(h, nbp, red_gx, T, ldt);
 }
 
 // Trailing update A22 -= V W^H + W V^H on lower 64x64 tiles, K = nb.
@@ -688,7 +778,8 @@ __global__ __launch_bounds__(256) void kb_colupd(const HetrdSys* __restrict__ sy
     if (!batch_col(s, c, nb, h)) return;
     const int gx = (int)ceil_div64(h.m - h.i, 256);
     if ((int)blockIdx.x >= gx) return;
-    colupd_body<T>(h, (const T*)s.A, blockIdx.x, gx);
+    const int prev_gx = (h.i > 0) ? (int)ceil_div64(s.n - c, 64) : 0;
+    colupd_body<T>(h, (const T*)s.A, blockIdx.x, gx, prev_gx);
 }
 
 __global__ __launch_bounds__(256) void kb_makev(const HetrdSys* __restrict__ sys, int c, int nb)
@@ -698,7 +789,7 @@ __global__ __launch_bounds__(256) void kb_makev(const HetrdSys* __restrict__ sys
     if (!batch_col(s, c, nb, h)) return;
     const int gx = (int)ceil_div64(h.m, MAKEV_ROWS), gy = max(h.i, 1);
     if ((int)blockIdx.x >= gx || (int)blockIdx.y >= gy) return;
-    makev_body(h, s.t12, blockIdx.x, blockIdx.y, gx, gy);
+    makev_body(h, blockIdx.x, blockIdx.y, gx, (int)ceil_div64(h.m - h.i, 256));
 }
 
 template <typename T, int CH, int U2>
@@ -725,7 +816,7 @@ __global__ __launch_bounds__(1024) void kb_reduce(const HetrdSys* __restrict__ s
     const int nt = s.n - c - 1;
     const int nbk = (int)ceil_div64(nt, 64), nq = (int)ceil_div64(nbk, ch);
     if ((int)blockIdx.x >= nbk) return;
-    reduce_body(h, nt, s.Yr, s.Yc, nbk, nq, ch, s.t12, blockIdx.x, nbk);
+    reduce_body(h, nt, s.Yr, s.Yc, nbk, nq, ch, (int)ceil_div64(h.m, MAKEV_ROWS), blockIdx.x);
 }
 
 // Column c closes the matrix's current panel when c = j0 + nbp - 1.
@@ -739,11 +830,12 @@ __device__ inline bool batch_panel_end(const HetrdSys& s, int c, int nb, int& j0
 
 __global__ __launch_bounds__(64) void kb_larft(const HetrdSys* __restrict__ sys, int c, int nb)
 {
-    __shared__ double2 col[64];
     const HetrdSys s = sys[blockIdx.z];
     int j0, nbp, p;
     if (!batch_panel_end(s, c, nb, j0, nbp, p)) return;
-    larft_body(nbp, s.tau + j0, s.G, s.Tall + (size_t)p * nb * nb, nb, col);
+    HetrdCol h;
+    batch_col(s, c, nb, h);
+    panel_close_body(h, nbp, (int)ceil_div64(s.n - c - 1, 64), s.Tall + (size_t)p * nb * nb, nb);
 }
 
 template <typename T, int MFMA>
@@ -839,10 +931,11 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
             const int L = m - i;
             const int c = j0 + i;
             const int nt = n - c - 1;
+            const int prev_gx = (i > 0) ? (int)ceil_div64(n - c, 64) : 0;
             hipLaunchKernelGGL(k_colupd<T>, dim3((unsigned)ceil_div64(L, 256)), dim3(256), 0, st, h,
-                               (const T*)A);
+                               (const T*)A, prev_gx);
             hipLaunchKernelGGL(k_makev, dim3((unsigned)ceil_div64(m, MAKEV_ROWS), (unsigned)max(i, 1)),
-                               dim3(256), 0, st, h, w.t12);
+                               dim3(256), 0, st, h, (int)ceil_div64(L, 256));
             const int code = hemv_ch_for(w.ch_table, nt);
             const int ch = code & 31;
             const int nbk = (int)ceil_div64(nt, 64);
@@ -854,13 +947,15 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
             if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c + 1], st);
             hipLaunchKernelGGL(k_hemv_reduce, dim3((unsigned)nbk), dim3(1024), 0, st, h, nt,
                                (const double2*)w.Yr, (const double2*)w.Yc, nbk, nq, ch,
-                               (const double2*)w.t12);
+                               (int)ceil_div64(m, MAKEV_ROWS));
             if ((err = hipGetLastError()) != hipSuccess) return err;
         }
-        // compact-WY factor of this panel from G(l, i) = V(:, l)^H v_i
-        if ((err = bse_launch_larft(st, nbp, tau + j0, w.G, w.Tall + (size_t)p * nb * nb, nb)) !=
-            hipSuccess)
-            return err;
+        // alpha2 of the last column and the compact-WY factor of this panel
+        // from G(l, i) = V(:, l)^H v_i
+        h.i = nbp - 1;
+        hipLaunchKernelGGL(k_panel_close, dim3(1), dim3(64), 0, st, h, nbp,
+                           (int)ceil_div64(n - (j0 + nbp - 1) - 1, 64), w.Tall + (size_t)p * nb * nb, nb);
+        if ((err = hipGetLastError()) != hipSuccess) return err;
         const int n2 = m - nbp;
         if (n2 > 0) {
             const int64_t nt2 = ceil_div64(n2, 64);
@@ -893,9 +988,7 @@ extern "C" void bse_hetrd_workspace(int n, int nb, int64_t* vstore_elems, int64_
     const int64_t nq = ceil_div64(nbk, HEMV_CH_MIN);
     *yr_elems = nbk * nbk * 64;
     *yc_elems = nbk * nq * 64;
-    // makev: 2 complex per (l, row block) for l < 64; reduce: 1 complex per block row
-    const int64_t nblk = ceil_div64(n, 64) + 1;
-    *part_doubles = 2 * 2 * 64 * nblk + 64;
+    *part_doubles = ::part_doubles(n);
 }
 
 // nb <= 64.  Tall receives one nb x nb compact-WY factor per panel.
