@@ -314,6 +314,42 @@ int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, int64_t ld, doub
     return BSE_OK;
 }
 
+// Trailing-update kernel choice (FP64 MFMA or FP64 VALU, which have the same
+// peak on this part): both timed once per context and storage precision on a
+// scratch matrix carved from the hemv partial buffer (contents irrelevant).
+// BSE_HER2K=valu / mfma forces the choice.
+static int autotune_her2k(bse_ctx* ctx, int fp32, void* scratch, size_t scratch_bytes,
+                          double2* V, double2* W, int64_t vw_elems, double2* alpha2) {
+    if (ctx->her2k_valu[fp32] >= 0) return BSE_OK;
+    const char* hv = getenv("BSE_HER2K");
+    if (hv && (hv[0] == 'v' || hv[0] == 'm')) { ctx->her2k_valu[fp32] = hv[0] == 'v'; return BSE_OK; }
+    const int nb = kPanel;
+    const size_t es = fp32 ? sizeof(float2) : sizeof(double2);
+    int n2 = (int)std::sqrt((double)scratch_bytes / (double)es);
+    n2 = std::min(4096, (n2 / 64) * 64);
+    while (n2 > 0 && (int64_t)n2 * nb > vw_elems) n2 -= 64;
+    if (!ctx->autotune || n2 < 512) { ctx->her2k_valu[fp32] = 1; return BSE_OK; }
+    hipStream_t st = ctx->st;
+    HCK(hipMemsetAsync(scratch, 0, (size_t)n2 * n2 * es, st));
+    HCK(hipMemsetAsync(V, 0, sizeof(double2) * n2 * nb, st));
+    HCK(hipMemsetAsync(W, 0, sizeof(double2) * n2 * nb, st));
+    HCK(hipMemsetAsync(alpha2, 0, sizeof(double2) * 64, st));
+    float best = 0.f;
+    int choice = 1;
+    for (int valu = 1; valu >= 0; --valu) {
+        HCK(bse_launch_her2k(st, fp32, valu, n2, scratch, n2, V, W, n2, nb, alpha2));
+        HCK(hipEventRecord(ctx->tev[0], st));
+        for (int r = 0; r < 4; ++r) HCK(bse_launch_her2k(st, fp32, valu, n2, scratch, n2, V, W, n2, nb, alpha2));
+        HCK(hipEventRecord(ctx->tev[1], st));
+        HCK(hipEventSynchronize(ctx->tev[1]));
+        float ms = 0.f;
+        HCK(hipEventElapsedTime(&ms, ctx->tev[0], ctx->tev[1]));
+        if (valu == 1 || ms < 0.97f * best) { best = ms; choice = valu; }
+    }
+    ctx->her2k_valu[fp32] = choice;
+    return BSE_OK;
+}
+
 static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32,
                         const double2* rho, int n_vectors)
 {
@@ -356,11 +392,15 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32
     double2* Gram = BUF(double2, S_GRAM, (size_t)64 * 64);
     double2* alpha2 = BUF(double2, S_ALPHA2, 64);
     {
-        const int rc = autotune_hemv(ctx, fp32, n, H, ld, acol, Yr, Yc);
+        int rc = autotune_hemv(ctx, fp32, n, H, ld, acol, Yr, Yc);
+        if (rc) return rc;
+        rc = autotune_her2k(ctx, fp32, Yr, sizeof(double2) * (size_t)yr_el, Vst, Wp,
+                            std::min<int64_t>(vs_el, (int64_t)n * nb), alpha2);
         if (rc) return rc;
     }
     HCK(bse_hetrd(st, fp32, n, H, ld, nb, d, e, tauh, Vst, voff.data(), Tall, Wp, alpha2, Gram, acol,
-                  Yr, Yc, t12, part, scal, counter, pev, ctx->hemv_ch[fp32]));
+                  Yr, Yc, t12, part, scal, counter, pev, ctx->hemv_ch[fp32],
+                  ctx->her2k_valu[fp32] > 0 ? 1 : 0));
     if (pev) {
         HCK(hipStreamSynchronize(st));
         double ms = 0.0, bytes = 0.0;

@@ -35,7 +35,7 @@ void bse_hetrd_workspace(int, int, int64_t*, int64_t*, int64_t*, int64_t*);
 hipError_t bse_hetrd(hipStream_t, int, int, void*, int64_t, int, double*, double*, double2*,
                      double2*, const int64_t*, double2*, double2*, double2*, double2*, double2*,
                      double2*, double2*, double2*, double*, double*, unsigned int*, hipEvent_t*,
-                     const signed char*);
+                     const signed char*, int);
 hipError_t bse_launch_hemv(hipStream_t, int, int, int, const void*, int64_t, const double2*,
                            double2*, double2*);
 hipError_t bse_launch_last_diag(hipStream_t, int, int, const void*, int64_t, double*);
@@ -146,6 +146,8 @@ struct bse_ctx {
     // autotuned hemv chunk length per floor(log2(trailing size)) bucket, one
     // table per storage precision (0 = not tuned yet)
     signed char hemv_ch[2][BSE_HEMV_BUCKETS] = {};
+    // trailing-update kernel per storage precision: -1 not tuned, 0 MFMA, 1 VALU
+    int her2k_valu[2] = {-1, -1};
     bool autotune = true;
     int ld_pad = 0;             // padding of the assembled matrix's leading dimension
     hipEvent_t tev[2] = {};     // autotuner timing pair

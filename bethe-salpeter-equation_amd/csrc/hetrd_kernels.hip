@@ -886,7 +886,7 @@ static void hemv_launch(hipStream_t st, int code, int nt, const T* A, int64_t ld
 }
 
 struct HetrdWork {
-    int her2k_valu;      // 1: VALU trailing update (BSE_HER2K=valu), 0: MFMA
+    int her2k_valu;      // 1: FP64 VALU trailing update, 0: FP64 MFMA
     hipEvent_t* prof_ev; // optional: 2 events per column around the hemv launch
     const signed char* ch_table;  // hemv chunk length per log2(nt) bucket
     double2* Vstore;     // packed panel reflectors
@@ -997,11 +997,10 @@ extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int64_
                                 const int64_t* voff, double2* Tall, double2* Wp, double2* alpha2,
                                 double2* G, double2* acol, double2* Yr, double2* Yc, double2* t12,
                                 double* part, double* scal, unsigned int* counter,
-                                hipEvent_t* prof_ev, const signed char* ch_table)
+                                hipEvent_t* prof_ev, const signed char* ch_table, int her2k_valu)
 {
     HetrdWork w;
-    const char* hv = getenv("BSE_HER2K");
-    w.her2k_valu = (hv != nullptr && hv[0] == 'v') ? 1 : 0;
+    w.her2k_valu = her2k_valu;
     w.prof_ev = prof_ev;
     w.ch_table = ch_table;
     w.Vstore = Vstore; w.voff = (int64_t*)voff; w.Tall = Tall; w.Wp = Wp; w.alpha2 = alpha2;
