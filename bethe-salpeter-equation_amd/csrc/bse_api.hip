@@ -278,7 +278,8 @@ extern "C" int bse_get_dipoles(bse_ctx* ctx, double* rho) {
 // every floor(log2(nt)) bucket up to the current size that is not tuned yet,
 // time chunk lengths 4, 8 and 16 on a trailing block of the matrix about to
 // be reduced (the kernel only reads it) and keep the fastest, also with the
-// two-row-blocks-per-step variant (code ch + 32).  The table is
+// two-row-blocks-per-step variant (code ch + 32) and the variant with
+// per-slab cross-lane column sums (code ch + 64).  The table is
 // per context and storage precision; BSE_AUTOTUNE=0 keeps the default 8.
 // ---------------------------------------------------------------------------
 int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, int64_t ld, double2* v,
@@ -299,7 +300,7 @@ int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, int64_t ld, doub
         const int reps = nt > 4096 ? 3 : 10;
         float best = 0.f;
         int best_ch = 8;
-        for (int ch : {8, 4, 16, 8 + 32, 4 + 32, 16 + 32}) {
+        for (int ch : {8, 4, 16, 8 + 32, 4 + 32, 16 + 32, 8 + 64, 4 + 64, 16 + 64}) {
             HCK(bse_launch_hemv(st, fp32, ch, nt, A22, ld, v, Yr, Yc));
             HCK(hipEventRecord(ctx->tev[0], st));
             for (int r = 0; r < reps; ++r) HCK(bse_launch_hemv(st, fp32, ch, nt, A22, ld, v, Yr, Yc));

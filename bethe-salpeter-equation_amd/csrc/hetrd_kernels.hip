@@ -386,18 +386,158 @@ __device__ __attribute__((always_inline)) inline void hemv_body(int nt, const T*
     }
 }
 
+// Column sums of sixteen per-lane complex values over the 64 lanes of a wave
+// by reduce-scatter: after exchanges at lane distances 32, 16, 8, 4 every lane
+// keeps one column, col = 8 b5 + 4 b4 + 2 b3 + b2 (b_k = bit k of the lane),
+// and two more exchanges (2, 1) complete its sum in all four lanes holding
+// it.  17 complex exchanges instead of 16 x 6 for separate wave sums, and no
+// per-column accumulators outlive the slab.
+__device__ inline double2 shfl_xor_c(double2 v, int m) {
+    return cmk(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+__device__ __attribute__((always_inline)) inline double2 colsum16(const double2 (&c)[16], int lane) {
+    double2 a[8], b[4], e[2];
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        a[k] = cadd(b5 ? c[k + 8] : c[k], shfl_xor_c(b5 ? c[k] : c[k + 8], 32));
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        b[k] = cadd(b4 ? a[k + 4] : a[k], shfl_xor_c(b4 ? a[k] : a[k + 4], 16));
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+        e[k] = cadd(b3 ? b[k + 2] : b[k], shfl_xor_c(b3 ? b[k] : b[k + 2], 8));
+    double2 d = cadd(b2 ? e[1] : e[0], shfl_xor_c(b2 ? e[0] : e[1], 4));
+    d = cadd(d, shfl_xor_c(d, 2));
+    d = cadd(d, shfl_xor_c(d, 1));
+    return d;
+}
+__device__ inline int colsum16_col(int lane) {
+    return 8 * ((lane >> 5) & 1) + 4 * ((lane >> 4) & 1) + 2 * ((lane >> 3) & 1) + ((lane >> 2) & 1);
+}
+
+// Hemv variant with few live registers: the column partials of each slab (128
+// rows for FP32 storage read as row pairs, 64 rows otherwise) are reduced
+// across the wave at once and added to one FP64 accumulator per lane, so the
+// kernel keeps four or more waves per SIMD while a full slab of loads is in
+// flight.  Same partial outputs as hemv_body.
+template <typename T, int CH>
+__device__ __attribute__((always_inline)) inline void hemv_bfly_body(int nt, const T* __restrict__ A, int64_t lda,
+                                                                      const double2* __restrict__ v,
+                                                                      double2* __restrict__ Yr,
+                                                                      double2* __restrict__ Yc, int nbk,
+                                                                      int nq, int q, int bj)
+{
+    const int bi0 = bj + q * CH;
+    if (bi0 >= nbk) return;
+    const int bi1 = min(nbk, bi0 + CH);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ double2 s_v[64];
+    __shared__ double2 s_part[4][CH][64];
+    const int col0 = bj * 64;
+    if (threadIdx.x < 64) {
+        const int cidx = col0 + threadIdx.x;
+        s_v[threadIdx.x] = (cidx < nt) ? v[cidx] : cmk(0, 0);
+    }
+    __syncthreads();
+    const double2* sv = s_v + w * 16;
+    double2 colacc = cmk(0, 0);
+    const int cw = col0 + w * 16;
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const __amdgpu_buffer_rsrc_t rs = bse_rsrc(A + (int64_t)(col0 + wu * 16) * lda, 0x7fffffff);
+    const int cstride = (int)(lda * (int64_t)sizeof(T));
+    const bool pack_ok = (lda & 1) == 0;
+    for (int bi = bi0; bi < bi1;) {
+        const bool full0 = bi > bj && (bi + 1) * 64 <= nt;
+        double2 c[16];
+        if (sizeof(T) == 8 && pack_ok && full0 && bi + 1 < bi1 && (bi + 2) * 64 <= nt) {
+            const int r2 = bi * 64 + 2 * lane;
+            bse_i4v qv[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj)
+                qv[jj] = __builtin_amdgcn_raw_buffer_load_b128(rs, r2 * (int)sizeof(T), jj * cstride, 0);
+            const double2 v0 = v[r2], v1 = v[r2 + 1];
+            double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const int q0 = qv[jj][0], q1 = qv[jj][1], q2 = qv[jj][2], q3 = qv[jj][3];
+                const double2 x = cmk((double)bse_i2f(q0), (double)bse_i2f(q1));
+                const double2 y = cmk((double)bse_i2f(q2), (double)bse_i2f(q3));
+                const double2 s = sv[jj];
+                cfma(r0, x, s);
+                cfma(r1, y, s);
+                double2 cc = cmk(0, 0);
+                cfmac(cc, x, v0);
+                cfmac(cc, y, v1);
+                c[jj] = cc;
+            }
+            colacc = cadd(colacc, colsum16(c, lane));
+            const int hb = lane >> 5, rl = (2 * lane) & 63;
+            s_part[w][bi + hb - bi0][rl] = r0;
+            s_part[w][bi + hb - bi0][rl + 1] = r1;
+            bi += 2;
+            continue;
+        }
+        const int row = bi * 64 + lane;
+        const bool rvalid = row < nt;
+        const double2 vr = rvalid ? v[row] : cmk(0, 0);
+        double2 racc = cmk(0, 0);
+        if (full0) {
+            T a[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) a[jj] = bufld<T>::ld(rs, row * (int)sizeof(T), jj * cstride);
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
+                cfma(racc, x, sv[jj]);
+                double2 cc = cmk(0, 0);
+                cfmac(cc, x, vr);
+                c[jj] = cc;
+            }
+        } else {
+            const T* Ap = A + row + (int64_t)cw * lda;
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                c[jj] = cmk(0, 0);
+                const int col = cw + jj;
+                if (!rvalid || col >= nt || row < col) continue;
+                const double2 x = cplx<T>::load(Ap + (int64_t)jj * lda);
+                if (row == col) {
+                    cfma(racc, cmk(x.x, 0.0), sv[jj]);
+                } else {
+                    cfma(racc, x, sv[jj]);
+                    cfmac(c[jj], x, vr);
+                }
+            }
+        }
+        colacc = cadd(colacc, colsum16(c, lane));
+        s_part[w][bi - bi0][lane] = racc;
+        ++bi;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < (bi1 - bi0) * 64; e += 256) {
+        const int bb = e >> 6, ln = e & 63;
+        const double2 s = cadd(cadd(s_part[0][bb][ln], s_part[1][bb][ln]),
+                               cadd(s_part[2][bb][ln], s_part[3][bb][ln]));
+        Yr[((int64_t)(bi0 + bb) * nbk + bj) * 64 + ln] = s;
+    }
+    if ((lane & 3) == 0) Yc[((int64_t)bj * nq + q) * 64 + w * 16 + colsum16_col(lane)] = colacc;
+}
+
 // Occupancy matters more than registers for this streaming kernel: without a
 // bound the compiler spends ~250 VGPRs and runs one wave per SIMD.  U2 (two
 // row blocks per step) holds 16 more loads per lane and gets one wave less.
 #ifndef HEMV_WPE
 #define HEMV_WPE 3
 #endif
+#define HEMV_WPE_FOR(U2) ((U2) == 2 ? HEMV_WPE + 1 : ((U2) && HEMV_WPE > 1 ? HEMV_WPE - 1 : HEMV_WPE))
 template <typename T, int CH, int U2>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U2 && HEMV_WPE > 1 ? HEMV_WPE - 1 : HEMV_WPE, 8)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HEMV_WPE_FOR(U2), 8)))
 void k_hemv_lower(int nt, const T* __restrict__ A, int64_t lda, const double2* __restrict__ v,
                   double2* __restrict__ Yr, double2* __restrict__ Yc, int nbk, int nq)
 {
-    hemv_body<T, CH, U2>(nt, A, lda, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    if (U2 == 2) hemv_bfly_body<T, CH>(nt, A, lda, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    else hemv_body<T, CH, U2>(nt, A, lda, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
 }
 
 // Reduce the hemv partials of one block row (64 rows) per workgroup, apply the
@@ -793,7 +933,7 @@ __global__ __launch_bounds__(256) void kb_makev(const HetrdSys* __restrict__ sys
 }
 
 template <typename T, int CH, int U2>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U2 && HEMV_WPE > 1 ? HEMV_WPE - 1 : HEMV_WPE, 8)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HEMV_WPE_FOR(U2), 8)))
 void kb_hemv(const HetrdSys* __restrict__ sys, int c, int nb)
 {
     HetrdCol h;
@@ -804,7 +944,8 @@ void kb_hemv(const HetrdSys* __restrict__ sys, int c, int nb)
     if ((int)blockIdx.x >= nq || (int)blockIdx.y >= nbk) return;
     const T* A22 = (const T*)s.A + (c + 1) + (int64_t)(c + 1) * s.lda;
     const double2* vtr = h.Vp + (h.i + 1) + (int64_t)h.i * h.m;
-    hemv_body<T, CH, U2>(nt, A22, s.lda, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    if (U2 == 2) hemv_bfly_body<T, CH>(nt, A22, s.lda, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    else hemv_body<T, CH, U2>(nt, A22, s.lda, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
 }
 
 __global__ __launch_bounds__(1024) void kb_reduce(const HetrdSys* __restrict__ sys, int c, int nb,
@@ -877,7 +1018,9 @@ static void hemv_launch(hipStream_t st, int code, int nt, const T* A, int64_t ld
     const int nbk = (int)ceil_div64(nt, 64);
     const int nq = (int)ceil_div64(nbk, ch);
 #define HL(CC, UU) hipLaunchKernelGGL((k_hemv_lower<T, CC, UU>), dim3(nq, nbk), dim3(256), 0, st, nt, A, lda, v, Yr, Yc, nbk, nq)
-    if (u2) {
+    if (u2 == 2) {
+        if (ch == 4) HL(4, 2); else if (ch == 16) HL(16, 2); else HL(8, 2);
+    } else if (u2) {
         if (ch == 4) HL(4, 1); else if (ch == 16) HL(16, 1); else HL(8, 1);
     } else {
         if (ch == 4) HL(4, 0); else if (ch == 16) HL(16, 0); else HL(8, 0);
@@ -1015,7 +1158,7 @@ extern "C" hipError_t bse_launch_hemv(hipStream_t st, int fp32, int code, int nt
                                       int64_t lda, const double2* v, double2* Yr, double2* Yc)
 {
     const int ch = code & 31, u2 = code >> 5;
-    if (nt <= 0 || (ch != 4 && ch != 8 && ch != 16) || u2 > 1) return hipErrorInvalidValue;
+    if (nt <= 0 || (ch != 4 && ch != 8 && ch != 16) || u2 > 2) return hipErrorInvalidValue;
     if (fp32) hemv_launch<float2>(st, code, nt, (const float2*)A, lda, v, Yr, Yc);
     else hemv_launch<double2>(st, code, nt, (const double2*)A, lda, v, Yr, Yc);
     return hipGetLastError();
@@ -1048,10 +1191,12 @@ extern "C" hipError_t bse_hetrd_batch_column(hipStream_t st, int fp32, int c, in
     const dim3 hg((unsigned)ceil_div64(nbk, ch), (unsigned)nbk, z);
 #define KBH(TT, CC, UU) hipLaunchKernelGGL((kb_hemv<TT, CC, UU>), hg, dim3(256), 0, st, sys, c, nb)
     if (fp32) {
-        if (u2) { if (ch == 4) KBH(float2, 4, 1); else if (ch == 16) KBH(float2, 16, 1); else KBH(float2, 8, 1); }
+        if (u2 == 2) { if (ch == 4) KBH(float2, 4, 2); else if (ch == 16) KBH(float2, 16, 2); else KBH(float2, 8, 2); }
+        else if (u2) { if (ch == 4) KBH(float2, 4, 1); else if (ch == 16) KBH(float2, 16, 1); else KBH(float2, 8, 1); }
         else { if (ch == 4) KBH(float2, 4, 0); else if (ch == 16) KBH(float2, 16, 0); else KBH(float2, 8, 0); }
     } else {
-        if (u2) { if (ch == 4) KBH(double2, 4, 1); else if (ch == 16) KBH(double2, 16, 1); else KBH(double2, 8, 1); }
+        if (u2 == 2) { if (ch == 4) KBH(double2, 4, 2); else if (ch == 16) KBH(double2, 16, 2); else KBH(double2, 8, 2); }
+        else if (u2) { if (ch == 4) KBH(double2, 4, 1); else if (ch == 16) KBH(double2, 16, 1); else KBH(double2, 8, 1); }
         else { if (ch == 4) KBH(double2, 4, 0); else if (ch == 16) KBH(double2, 16, 0); else KBH(double2, 8, 0); }
     }
 #undef KBH
