@@ -81,6 +81,31 @@ def cpu_baseline(pkg, n_sample: int, n_full: int):
                         f"{t2 - t1:.2f} s; scaled to N={n_f} as build*(N/n)^2 + eig*(N/n)^3"))
 
 
+def copy_calibration(args):
+    """Plain device copy of known size (4 GiB read + 4 GiB written), for scaling
+    the FETCH_SIZE / WRITE_SIZE counters of the profiled hemv kernel."""
+    import torch
+    nbytes = 1 << 32
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "copies": 3}))
+
+
+def measured_traffic(mixed: str, n: int):
+    """HBM bytes per hemv launch from committed rocprofv3 counter runs, if any."""
+    p = ROOT / "profiles" / "hemv_traffic_n20736.json"
+    if not p.exists():
+        return None, None
+    t = json.loads(p.read_text())
+    if t.get("N") != n or t.get("storage") != ("fp32" if mixed != "fp64" else "fp64"):
+        return None, None
+    return t["hbm_bytes_per_launch"], t
+
+
 def sweep_bench(args, pkg):
     """Batched sweep: 64 two-band honeycomb systems on seeded uneven k-grids
     (12..40)^2, FP32-storage assembly + FP64 refinement of the lowest 16, 2048-point
@@ -145,10 +170,14 @@ def main():
     ap.add_argument("--no-refine", dest="refine", action="store_false")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample-grid", type=int, default=13)
-    ap.add_argument("--workload", default="n20736", choices=["n20736", "sweep"],
-                    help="n20736: the headline single system; sweep: the 64-system batch")
+    ap.add_argument("--cpu-sample-grid", type=int, default=20)
+    ap.add_argument("--workload", default="n20736", choices=["n20736", "sweep", "copy"],
+                    help="n20736: the headline single system; sweep: the 64-system batch; "
+                         "copy: plain copy for counter calibration")
     args = ap.parse_args()
+    if args.workload == "copy":
+        copy_calibration(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -207,8 +236,10 @@ def main():
         log(f"fp64 path: {t_fp64:.3f} s, stages {stages_fp64}")
 
     ach = prof["hemv_bytes"] / (prof["hemv_ms"] * 1e-3) / 1e9 if prof["hemv_ms"] > 0 else 0.0
+    traffic, traffic_src = measured_traffic(args.mixed, N)
     roof = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
-                traffic=None, kernel=f"k_hemv_lower<{'float2' if args.mixed != 'fp64' else 'double2'}>",
+                traffic=traffic, traffic_source=(traffic_src or {}).get("source"),
+                kernel=f"k_hemv_lower<{'float2' if args.mixed != 'fp64' else 'double2'}>",
                 launches=prof["hemv_launches"],
                 algorithmic_bytes_per_launch=prof["hemv_bytes"] / max(1, prof["hemv_launches"]),
                 avg_launch_us=prof["hemv_ms"] * 1e3 / max(1, prof["hemv_launches"]))
