@@ -12,7 +12,7 @@ algebra (index order, conjugations, signs, the k - k' + G argument of W):
 * diagonal model W_GG'(k2-k1) = delta_GG' W(|k2-k1+G|) (2697-2728) with the
   mini-BZ average replacing the single Q + G = 0 term;
 * the direct row of 2850-2874 (temporary_vector2), kept in its textbook
-  orientation (the reference stores its conjugate through .t(), Q23);
+  orientation (the reference stores its conjugate through .t(), 2878);
 * the exchange of 2796-2830 in its Hermitian form sum_{G != G0} v rho_r rho_s^*.
 
 Pure Python loops: use only for N up to a few tens.

@@ -11,8 +11,11 @@ with point-orbital pair densities:
                     Armadillo eig_sym; ascending, unit columns)
   point-orbital M . function_building_exponential_factor 1459-1501
   W / v tables .... pull_coulomb_potentials 2547-2728 (model W, q=0 average)
-  assembly ........ pull_resonant_part_and_rcv 2767-2928 (textbook orientation,
-                    Hermitian exchange; see Q1/Q2/Q23 of the defect register)
+  assembly ........ pull_resonant_part_and_rcv 2767-2928 in textbook orientation
+                    (the reference stores the complex conjugate of the direct
+                    term through .t() at 2878), with the exchange kept after
+                    the direct rows are written (the reference's row
+                    assignment at 2876 overwrites it) and in its Hermitian form
   eig ............. common_diagonalization 3318-3345 (zheevd, sort, normalise)
   oscillator ...... pull_excitonic_oscillator_force 3486-3502
   spectrum ........ pull_macroscopic_bse_dielectric_function 3525-3657
