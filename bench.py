@@ -106,6 +106,44 @@ def measured_traffic(mixed: str, n: int):
     return t["hbm_bytes_per_launch"], t
 
 
+def grad_bench(args, pkg):
+    """Differentiation on the N = 20736 system: Hellmann-Feynman gradients of
+    the lowest 8 exciton energies and of the spectrum misfit against a seeded
+    synthetic target (2048 points, eta = 50 meV) for the five parameters,
+    after the FP64 path with all eigenvectors."""
+    from bse_amd import _abi
+    s = pkg.random_multiorbital(args.grid)
+    ctx = _abi.Context(0)
+    ctx.set_system(s)
+    t = time.perf_counter()
+    ctx.build("fp64")
+    ctx.eigh(-1)
+    t_eig = time.perf_counter() - t
+    om = s.omegas()
+    eps = ctx.spectrum(om, s.eta)
+    rng = np.random.Generator(np.random.PCG64(2048))
+    target = eps.imag * (1.0 + 0.1 * rng.standard_normal(om.size))
+    t = time.perf_counter()
+    dE = ctx.grad_energies(8)
+    t_ge = time.perf_counter() - t
+    t = time.perf_counter()
+    L, dL = ctx.grad_spectrum(om, s.eta, target)
+    t_gs = time.perf_counter() - t
+    log(f"eig {t_eig:.2f} s, energy gradients {t_ge:.3f} s, spectrum gradient {t_gs:.3f} s")
+    res = {
+        "metric": "N=20736 Hellmann-Feynman gradients (lowest 8 energies + spectrum misfit, 5 parameters)",
+        "value": t_ge + t_gs, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": (t_ge + t_gs) * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded PCG64 random TB model)",
+        "config": {"workload": f"random_multiorbital_{args.grid}x{args.grid}_gradients", "N": s.n_bse,
+                   "n_exc": 8, "n_omega": int(om.size), "params": list(pkg.GRAD_PARAMS)},
+        "fp64_eig_all_vectors_s": t_eig, "energy_gradients_s": t_ge, "spectrum_gradient_s": t_gs,
+        "dE_dscissor": dE[:, 1].tolist(), "misfit": L, "dL": dL.tolist(),
+    }
+    print(json.dumps(res))
+    ctx.close()
+
+
 def sweep_bench(args, pkg):
     """Batched sweep: 64 two-band honeycomb systems on seeded uneven k-grids
     (12..40)^2, FP32-storage assembly + FP64 refinement of the lowest 16, 2048-point
@@ -171,9 +209,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-grid", type=int, default=20)
-    ap.add_argument("--workload", default="n20736", choices=["n20736", "sweep", "copy"],
+    ap.add_argument("--workload", default="n20736", choices=["n20736", "sweep", "grad", "copy"],
                     help="n20736: the headline single system; sweep: the 64-system batch; "
-                         "copy: plain copy for counter calibration")
+                         "grad: gradients on the N = 20736 system; copy: plain copy for "
+                         "counter calibration")
     args = ap.parse_args()
     if args.workload == "copy":
         copy_calibration(args)
@@ -187,6 +226,9 @@ def main():
     from bse_amd import _abi
     if args.workload == "sweep":
         sweep_bench(args, pkg)
+        return
+    if args.workload == "grad":
+        grad_bench(args, pkg)
         return
     s = pkg.random_multiorbital(args.grid)
     N = s.n_bse

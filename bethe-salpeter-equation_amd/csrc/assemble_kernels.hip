@@ -24,7 +24,8 @@
 
 struct AsmArgs {
     int nk, n1, n2, nv, nc, norb, TK;
-    int64_t N, ld;
+    int64_t N;
+    int nt;              // tile rows of the tiled lower storage
     const double2* Cl;   // coef table [k][slot][orb] for the row (left) conduction factor
     const double2* Cr;   // ... column (right) conduction factor
     const double2* Vl;   // ... row valence factor
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(256) void k_assemble_tile(AsmArgs a, T* __restrict_
             h = cadd(h, cscale(x, a.beta));
         }
         if (grow == gcol && a.dE != nullptr) h.x += a.dE[grow];
-        cplx<T>::store(H + grow + gcol * a.ld, h);
+        if (grow >= gcol) cplx<T>::store(H + tix(grow, gcol, a.nt), h);
     }
 }
 
@@ -182,7 +183,9 @@ extern "C" hipError_t bse_launch_assemble(hipStream_t st, int store_fp32, int nk
 {
     AsmArgs a;
     a.nk = nk; a.n1 = n1; a.n2 = n2; a.nv = nv; a.nc = nc; a.norb = norb; a.TK = TK;
-    a.N = (int64_t)nk * nv * nc; a.ld = ld;
+    a.N = (int64_t)nk * nv * nc;
+    a.nt = tile_count(a.N);
+    (void)ld;            // the matrix is stored in tiles (bse_common.hpp)
     a.Cl = Cl; a.Cr = Cr; a.Vl = Vl; a.Vr = Vr; a.wtab = wtab; a.PVr = PVr; a.Pc = Pc;
     a.dE = dE; a.alpha = alpha; a.beta = beta;
     if (store_fp32) return launch_assemble<float2>(st, a, (float2*)H, lower_only);
@@ -190,17 +193,18 @@ extern "C" hipError_t bse_launch_assemble(hipStream_t st, int store_fp32, int nk
 }
 
 // ---------------------------------------------------------------------------
-// Copy-out: internal k-major lower-triangle storage -> full Hermitian matrix
-// in reference or internal order.  dst element type matches src.
+// Copy-out: internal k-major tiled lower storage -> full Hermitian matrix
+// (column-major, ldd) in reference or internal order.  dst element type
+// matches src.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_expand(int64_t N, int B, int nk, const T* __restrict__ src,
-                                                int64_t lds, T* __restrict__ dst, int64_t ldd,
-                                                int ref_order)
+                                                T* __restrict__ dst, int64_t ldd, int ref_order)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // destination row
     const int64_t s = blockIdx.y;                                     // destination column
     if (r >= N) return;
+    const int nt = tile_count(N);
     // map destination (reference) indices to internal ones
     int64_t ri = r, si = s;
     if (ref_order) {
@@ -209,9 +213,9 @@ __global__ __launch_bounds__(256) void k_expand(int64_t N, int B, int nk, const 
     }
     T v;
     if (ri >= si) {
-        v = src[ri + si * lds];
+        v = src[tix(ri, si, nt)];
     } else {
-        T u = src[si + ri * lds];
+        T u = src[tix(si, ri, nt)];
         v = u;
         v.y = -u.y;
     }
@@ -220,15 +224,77 @@ __global__ __launch_bounds__(256) void k_expand(int64_t N, int B, int nk, const 
 }
 
 extern "C" hipError_t bse_launch_expand(hipStream_t st, int fp32, int64_t N, int B, int nk,
-                                        const void* src, int64_t lds, void* dst, int64_t ldd,
-                                        int ref_order)
+                                        const void* src, void* dst, int64_t ldd, int ref_order)
 {
     dim3 grid((unsigned)ceil_div64(N, 256), (unsigned)N);
     if (fp32)
         hipLaunchKernelGGL(k_expand<float2>, grid, dim3(256), 0, st, N, B, nk,
-                           (const float2*)src, lds, (float2*)dst, ldd, ref_order);
+                           (const float2*)src, (float2*)dst, ldd, ref_order);
     else
         hipLaunchKernelGGL(k_expand<double2>, grid, dim3(256), 0, st, N, B, nk,
-                           (const double2*)src, lds, (double2*)dst, ldd, ref_order);
+                           (const double2*)src, (double2*)dst, ldd, ref_order);
+    return hipGetLastError();
+}
+
+// Lower triangle of a column-major FP64 matrix (n x n, lds) into tiled lower
+// storage of FP64 or FP32 elements (dense eigensolver input).  Each
+// workgroup fills one tile; padding rows and columns beyond n are zero.
+template <typename T>
+__global__ __launch_bounds__(256) void k_to_tiles(int64_t n, const double2* __restrict__ src,
+                                                  int64_t lds, T* __restrict__ dst)
+{
+    const int nt = tile_count(n);
+    int ti, tj;
+    tri_tile(blockIdx.x, ti, tj);
+    T* t = dst + tile_pos(ti, tj, nt) * (BSE_TILE * BSE_TILE);
+    for (int e = threadIdx.x; e < BSE_TILE * BSE_TILE; e += blockDim.x) {
+        const int64_t r = (int64_t)ti * 64 + (e & 63), c = (int64_t)tj * 64 + (e >> 6);
+        double2 x = cmk(0, 0);
+        if (r < n && c < n && r >= c) x = src[r + c * lds];
+        cplx<T>::store(t + e, x);
+    }
+}
+
+extern "C" hipError_t bse_launch_to_tiles(hipStream_t st, int fp32, int64_t n, const double2* src,
+                                          int64_t lds, void* dst)
+{
+    const int64_t nt = tile_count(n);
+    const unsigned g = (unsigned)(nt * (nt + 1) / 2);
+    if (fp32) hipLaunchKernelGGL(k_to_tiles<float2>, dim3(g), dim3(256), 0, st, n, src, lds, (float2*)dst);
+    else hipLaunchKernelGGL(k_to_tiles<double2>, dim3(g), dim3(256), 0, st, n, src, lds, (double2*)dst);
+    return hipGetLastError();
+}
+
+// Storage invariant of the tiled lower layout: padding entries (rows or
+// columns >= n) and the strict upper half of every diagonal tile are zero
+// (the tridiagonalisation reads whole tiles without masks).  Nothing else
+// writes these entries, so clearing them once after allocation is enough.
+// Blocks 0 .. nt-1 clear the upper halves of the diagonal tiles, blocks
+// nt .. 2nt-1 the rows >= n of the last tile row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tile_pad_zero(int64_t n, T* __restrict__ dst)
+{
+    const int nt = tile_count(n);
+    const int b = blockIdx.x;
+    const bool diag = b < nt;
+    const int ti = diag ? b : nt - 1, tj = diag ? b : b - nt;
+    T* t = dst + tile_pos(ti, tj, nt) * (BSE_TILE * BSE_TILE);
+    T z;
+    z.x = 0;
+    z.y = 0;
+    for (int e = threadIdx.x; e < BSE_TILE * BSE_TILE; e += blockDim.x) {
+        const int r = e & 63, c = e >> 6;
+        const bool clear = diag ? (r < c || (int64_t)ti * 64 + r >= n) : ((int64_t)ti * 64 + r >= n);
+        if (clear) t[e] = z;
+    }
+}
+
+extern "C" hipError_t bse_launch_tile_pad_zero(hipStream_t st, int fp32, int64_t n, void* dst)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t nt = tile_count(n);
+    const unsigned g = (unsigned)((n & 63) ? 2 * nt : nt);
+    if (fp32) hipLaunchKernelGGL(k_tile_pad_zero<float2>, dim3(g), dim3(256), 0, st, n, (float2*)dst);
+    else hipLaunchKernelGGL(k_tile_pad_zero<double2>, dim3(g), dim3(256), 0, st, n, (double2*)dst);
     return hipGetLastError();
 }

@@ -99,7 +99,8 @@ __device__ inline mf_f4 mm16_bf16x2(const float* a, const float* b, mf_f4 acc, f
 
 struct MfArgs {
     int nk, n1, n2, nv, nc, norb, TK;
-    int64_t N, ld;
+    int64_t N;
+    int nt;              // tile rows of the tiled lower storage
     const double2* coef;
     const double2* wtab;
     const double2* PV;
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(MfArgs a, typename mf_t<P
             for (int j = 0; j < no; ++j) cfma(x, sPV[rl * 8 + j], cconj(sP[cl * 8 + j]));
             h = cadd(h, cscale(x, a.beta));
             if (grow == gcol) { h.x += a.dE[grow]; h.y = 0.0; }
-            cplx<OUT>::store(H + grow + gcol * a.ld, h);
+            if (grow >= gcol) cplx<OUT>::store(H + tix(grow, gcol, a.nt), h);
         }
     }
 }
@@ -260,7 +261,9 @@ extern "C" hipError_t bse_launch_assemble_mfma(hipStream_t st, int prec, int nk,
     MfArgs a;
     a.nk = nk; a.n1 = n1; a.n2 = n2; a.nv = nv; a.nc = nc; a.norb = norb;
     a.TK = bse_assemble_mfma_tk(nv, nc);
-    a.N = (int64_t)nk * nv * nc; a.ld = ld;
+    a.N = (int64_t)nk * nv * nc;
+    a.nt = tile_count(a.N);
+    (void)ld;            // the matrix is stored in tiles (bse_common.hpp)
     a.coef = coef; a.wtab = wtab; a.PV = PV; a.P = P; a.dE = dE; a.alpha = alpha; a.beta = beta;
     const int64_t nt = ceil_div64(nk, a.TK);
     const dim3 grid((unsigned)(nt * (nt + 1) / 2));

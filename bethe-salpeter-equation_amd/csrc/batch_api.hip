@@ -118,7 +118,7 @@ void carve(Bump& bp, const SysHost& h, int fp32, int nlow, int n_omega, SysDev& 
     d.PV = bp.take<double2>((size_t)N * no);
     d.rho = bp.take<double2>(N);
     d.dE = bp.take<double>(N);
-    d.H = bp.take<char>((size_t)N * N * (fp32 ? sizeof(float2) : sizeof(double2)));
+    d.H = bp.take<char>((size_t)tiled_elems(N) * (fp32 ? sizeof(float2) : sizeof(double2)));
     int64_t vs_el, yr_el, yc_el, part_d;
     bse_hetrd_workspace(n, nb, &vs_el, &yr_el, &yc_el, &part_d);
     const int npan = (int)ceil_div64(std::max(n - 1, 1), nb);
@@ -199,6 +199,7 @@ static hipError_t record_front(BatchState* b, int s, hipStream_t st) {
         return e;
     const double alpha = -sy.direct_scale / (g.area * g.nk);
     const double beta = sy.f_s * sy.exchange_scale / (g.area * g.nk);
+    if ((e = bse_launch_tile_pad_zero(st, b->fp32, g.N, d.H))) return e;
     if (bse_assemble_mfma_supported(sy.nv, sy.nc, no))
         return bse_launch_assemble_mfma(st, b->precision, g.nk, sy.n1, sy.n2, sy.nv, sy.nc, no, g.N,
                                         d.coef, d.wtab, d.PV, d.P, d.dE, alpha, beta, d.H);
@@ -232,7 +233,7 @@ static hipError_t record_tail(BatchState* b, int s, hipStream_t st) {
     const int n = (int)g.N, k = b->nlow, nb = kPanel;
     const int64_t N = g.N;
     hipError_t e;
-    if ((e = bse_launch_last_diag(st, b->fp32, n, d.H, N, d.d))) return e;
+    if ((e = bse_launch_last_diag(st, b->fp32, n, d.H, d.d))) return e;
     if ((e = bse_launch_tri_prepare(st, n, d.d, d.e, d.e2, d.items, d.bounds, d.prm))) return e;
     if ((e = bse_launch_tri_vectors_dev(st, n, d.items, d.bounds, d.d, d.e, d.e2, d.prm, d.lami,
                                         d.lam, d.bsv, d.nbv, d.clusters, d.ncl, d.iiscr, d.iix, 0)))
@@ -290,21 +291,20 @@ static int batch_autotune(bse_ctx* ctx, BatchState* b) {
     const int64_t N = n;
     hipStream_t st = ctx->st;
     const size_t es = b->fp32 ? sizeof(float2) : sizeof(double2);
-    HCK(hipMemsetAsync(d.H, 0, (size_t)N * N * es, st));
+    HCK(hipMemsetAsync(d.H, 0, (size_t)tiled_elems(N) * es, st));
     HCK(hipMemsetAsync(d.Vst, 0, sizeof(double2) * (size_t)N * nb, st));
     HCK(hipMemsetAsync(d.Wp, 0, sizeof(double2) * (size_t)N * nb, st));
     HCK(hipMemsetAsync(d.alpha2, 0, sizeof(double2) * 64, st));
-    int rc = autotune_hemv(ctx, b->fp32, n, d.H, N, d.acol, d.Yr, d.Yc);
+    int rc = autotune_hemv(ctx, b->fp32, n, d.H, d.acol, d.Yr, d.Yc);
     if (rc) return rc;
-    const int n2 = n - nb;
     b->her2k_valu = 0;
-    if (ctx->autotune && n2 > 0) {
+    if (ctx->autotune && n > nb) {
         float best = 0.f;
         for (int valu = 0; valu < 2; ++valu) {
-            HCK(bse_launch_her2k(st, b->fp32, valu, n2, d.H, N, d.Vst, d.Wp, N, nb, d.alpha2));
+            HCK(bse_launch_her2k(st, b->fp32, valu, n, nb, d.H, d.Vst, d.Wp, N, 0, nb, d.alpha2));
             HCK(hipEventRecord(ctx->tev[0], st));
             for (int r = 0; r < 5; ++r)
-                HCK(bse_launch_her2k(st, b->fp32, valu, n2, d.H, N, d.Vst, d.Wp, N, nb, d.alpha2));
+                HCK(bse_launch_her2k(st, b->fp32, valu, n, nb, d.H, d.Vst, d.Wp, N, 0, nb, d.alpha2));
             HCK(hipEventRecord(ctx->tev[1], st));
             HCK(hipEventSynchronize(ctx->tev[1]));
             float ms = 0.f;
@@ -364,8 +364,9 @@ static int batch_capture(bse_ctx* ctx, BatchState* b) {
                 const int p = c / nb, j0 = p * nb, nbp = std::min(nb, nsz[s] - 1 - j0);
                 if (c != j0 + nbp - 1) continue;
                 any_end = true;
-                const int64_t nt2 = ceil_div64(nsz[s] - j0 - nbp, 64);
-                max_tiles = std::max(max_tiles, nt2 * (nt2 + 1) / 2);
+                if (j0 + nbp >= nsz[s]) continue;
+                const int64_t nT = tile_count(nsz[s]) - ((j0 + nbp) >> 6);
+                max_tiles = std::max(max_tiles, nT * (nT + 1) / 2);
             }
             if (any_end) BCK(bse_hetrd_batch_panel(st, b->fp32, c, nb, nact, b->d_hsys, max_tiles, b->her2k_valu));
             for (int s = 0; s < nact; ++s) {
@@ -482,7 +483,7 @@ extern "C" int bse_batch_prepare(bse_ctx* ctx, int n_systems, const bse_tb_model
         HCK(hipMemcpyAsync(d.kpts, h.g.kpts.data(), sizeof(double) * h.g.kpts.size(), hipMemcpyHostToDevice, st));
         HCK(hipMemcpyAsync(d.gvec, h.g.gvec.data(), sizeof(double) * h.g.gvec.size(), hipMemcpyHostToDevice, st));
         HetrdSys& x = hsys[s];
-        x.n = (int)h.g.N; x.lda = h.g.N; x.A = d.H; x.Vstore = d.Vst; x.Tall = d.Tall; x.Wp = d.Wp;
+        x.n = (int)h.g.N; x.A = d.H; x.Vstore = d.Vst; x.Tall = d.Tall; x.Wp = d.Wp;
         x.alpha2 = d.alpha2; x.G = d.G; x.acol = d.acol; x.d = d.d; x.e = d.e; x.tau = d.tauh;
         x.scal = d.scal; x.part = d.part; x.counter = d.counter; x.Yr = d.Yr; x.Yc = d.Yc; x.t12 = d.t12;
     }

@@ -30,8 +30,6 @@ extern "C" int bse_create(bse_ctx** out, int device) {
     }
     for (int i = 0; i <= ST_N; ++i) (void)hipEventCreate(&ctx->ev[i]);
     for (int i = 0; i < 2; ++i) (void)hipEventCreate(&ctx->tev[i]);
-    const char* lp = getenv("BSE_LDPAD");
-    if (lp) ctx->ld_pad = std::max(0, atoi(lp));
     const char* at = getenv("BSE_AUTOTUNE");
     ctx->autotune = !(at && at[0] == '0');
     const char* am = getenv("BSE_ASSEMBLY");
@@ -200,20 +198,19 @@ extern "C" int bse_build(bse_ctx* ctx, int precision) {
     const bse_system& s = ctx->sys;
     const int fp32 = precision != BSE_PREC_FP64;
     const int64_t N = ctx->N;
-    // leading dimension padded by ld_pad elements (BSE_LDPAD): a column stride
-    // with a large power-of-two factor maps the hemv's column reads onto few
-    // HBM channels
-    ctx->ldH = N + ctx->ld_pad;
-    void* H = buf(ctx, S_H, (size_t)ctx->ldH * N * (fp32 ? sizeof(float2) : sizeof(double2)), &rc);
+    // tiled lower storage: 64 x 64 tiles of the lower triangle, each column-major,
+    // ordered by tile column and then tile row (one tile column is contiguous)
+    void* H = buf(ctx, S_H, (size_t)tiled_elems(N) * (fp32 ? sizeof(float2) : sizeof(double2)), &rc);
     if (!H) return rc;
     const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
     const double beta = s.f_s * s.exchange_scale / (ctx->area * ctx->nk);
     const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
     stage_begin(ctx, ST_ASM);
+    HCK(bse_launch_tile_pad_zero(ctx->st, fp32, N, H));
     if (!ctx->force_valu_assembly && bse_assemble_mfma_supported(s.nv, s.nc, ctx->norb)) {
         // matrix cores: FP64 MFMA (FP64 storage), FP32 MFMA or compensated BF16 MFMA (FP32 storage)
         HCK(bse_launch_assemble_mfma(ctx->st, precision, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb,
-                                     ctx->ldH, coef, (const double2*)ctx->bufs[S_WTAB].p,
+                                     N, coef, (const double2*)ctx->bufs[S_WTAB].p,
                                      (const double2*)ctx->bufs[S_PV].p,
                                      (const double2*)ctx->bufs[S_P].p,
                                      (const double*)ctx->bufs[S_DE].p, alpha, beta, H));
@@ -221,7 +218,7 @@ extern "C" int bse_build(bse_ctx* ctx, int precision) {
     } else {
         // small band windows (e.g. 1v+1c): FP64 VALU tiles, FP64 or FP32 storage
         HCK(bse_launch_assemble(ctx->st, fp32, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb, ctx->TK,
-                                ctx->ldH, coef, coef, coef, coef, (const double2*)ctx->bufs[S_WTAB].p,
+                                N, coef, coef, coef, coef, (const double2*)ctx->bufs[S_WTAB].p,
                                 (const double2*)ctx->bufs[S_PV].p, (const double2*)ctx->bufs[S_P].p,
                                 (const double*)ctx->bufs[S_DE].p, alpha, beta, H, 1));
         ctx->asm_kernel = 0;
@@ -248,7 +245,7 @@ extern "C" int bse_get_matrix(bse_ctx* ctx, void* dst, int64_t ld, int dst_on_de
         out = buf(ctx, S_Z, (size_t)ld * N * es, &rc);
         if (!out) return rc;
     }
-    HCK(bse_launch_expand(ctx->st, fp32, N, ctx->B, ctx->nk, ctx->bufs[S_H].p, ctx->ldH, out, ld,
+    HCK(bse_launch_expand(ctx->st, fp32, N, ctx->B, ctx->nk, ctx->bufs[S_H].p, out, ld,
                           order == BSE_ORDER_REFERENCE));
     if (!dst_on_device) HCK(hipMemcpyAsync(dst, out, (size_t)ld * N * es, hipMemcpyDeviceToHost, ctx->st));
     HCK(hipStreamSynchronize(ctx->st));
@@ -277,16 +274,18 @@ extern "C" int bse_get_dipoles(bse_ctx* ctx, double* rho) {
 // Autotuner for the tridiagonalisation's Hermitian matrix-vector kernel: for
 // every floor(log2(nt)) bucket up to the current size that is not tuned yet,
 // time chunk lengths 4, 8 and 16 on a trailing block of the matrix about to
-// be reduced (the kernel only reads it) and keep the fastest, also with the
-// two-row-blocks-per-step variant (code ch + 32) and the variant with
-// per-slab cross-lane column sums (code ch + 64).  The table is
-// per context and storage precision; BSE_AUTOTUNE=0 keeps the default 8.
+// be reduced (the kernel only reads it) and keep the fastest, each as the
+// plain variant (code ch), the two-row-blocks-per-step variant (ch + 32) and
+// the variant with per-slab cross-lane column sums (ch + 64), with
+// non-temporal or default-policy loads (+ 128).  Repeated launches over the
+// same block see the cache reuse that consecutive columns see.  The table is
+// per context and storage precision; BSE_AUTOTUNE=0 keeps the default
+// (ch 8, column sums, non-temporal).
 // ---------------------------------------------------------------------------
-int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, int64_t ld, double2* v,
-                  double2* Yr, double2* Yc) {
+int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, double2* v, double2* Yr,
+                  double2* Yc) {
     if (!ctx->autotune || n < 128) return BSE_OK;
-    signed char* tab = ctx->hemv_ch[fp32];
-    const size_t es = fp32 ? sizeof(float2) : sizeof(double2);
+    unsigned char* tab = ctx->hemv_ch[fp32];
     hipStream_t st = ctx->st;
     bool zeroed = false;
     for (int b = 6; b < BSE_HEMV_BUCKETS && (1 << b) <= n - 1; ++b) {
@@ -296,21 +295,23 @@ int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, int64_t ld, doub
             zeroed = true;
         }
         const int nt = std::min(3 << (b - 1), n - 1);
-        const void* A22 = (const char*)H + es * ((size_t)(n - nt) + (size_t)(n - nt) * ld);
+        const int s0 = n - nt;                 // trailing block A[s0:, s0:]
         const int reps = nt > 4096 ? 3 : 10;
         float best = 0.f;
-        int best_ch = 8;
-        for (int ch : {8, 4, 16, 8 + 32, 4 + 32, 16 + 32, 8 + 64, 4 + 64, 16 + 64}) {
-            HCK(bse_launch_hemv(st, fp32, ch, nt, A22, ld, v, Yr, Yc));
+        int best_ch = 8 + 64;
+        for (int k = 0; k < 18; ++k) {
+            static const int base[9] = {8 + 64, 4 + 64, 16 + 64, 8 + 32, 4 + 32, 16 + 32, 8, 4, 16};
+            const int ch = base[k % 9] + (k >= 9 ? 128 : 0);
+            HCK(bse_launch_hemv(st, fp32, ch, n, s0, H, v, Yr, Yc));
             HCK(hipEventRecord(ctx->tev[0], st));
-            for (int r = 0; r < reps; ++r) HCK(bse_launch_hemv(st, fp32, ch, nt, A22, ld, v, Yr, Yc));
+            for (int r = 0; r < reps; ++r) HCK(bse_launch_hemv(st, fp32, ch, n, s0, H, v, Yr, Yc));
             HCK(hipEventRecord(ctx->tev[1], st));
             HCK(hipEventSynchronize(ctx->tev[1]));
             float ms = 0.f;
             HCK(hipEventElapsedTime(&ms, ctx->tev[0], ctx->tev[1]));
             if (best == 0.f || ms < 0.97f * best) { best = ms; best_ch = ch; }
         }
-        tab[b] = (signed char)best_ch;
+        tab[b] = (unsigned char)best_ch;
     }
     return BSE_OK;
 }
@@ -326,21 +327,22 @@ static int autotune_her2k(bse_ctx* ctx, int fp32, void* scratch, size_t scratch_
     if (hv && (hv[0] == 'v' || hv[0] == 'm')) { ctx->her2k_valu[fp32] = hv[0] == 'v'; return BSE_OK; }
     const int nb = kPanel;
     const size_t es = fp32 ? sizeof(float2) : sizeof(double2);
-    int n2 = (int)std::sqrt((double)scratch_bytes / (double)es);
-    n2 = std::min(4096, (n2 / 64) * 64);
-    while (n2 > 0 && (int64_t)n2 * nb > vw_elems) n2 -= 64;
+    // a tiled n2 x n2 matrix updated below row and column nb (one panel)
+    int n2 = 4096;
+    while (n2 > 0 && ((size_t)tiled_elems(n2) * es > scratch_bytes || (int64_t)n2 * nb > vw_elems))
+        n2 -= 64;
     if (!ctx->autotune || n2 < 512) { ctx->her2k_valu[fp32] = 1; return BSE_OK; }
     hipStream_t st = ctx->st;
-    HCK(hipMemsetAsync(scratch, 0, (size_t)n2 * n2 * es, st));
+    HCK(hipMemsetAsync(scratch, 0, (size_t)tiled_elems(n2) * es, st));
     HCK(hipMemsetAsync(V, 0, sizeof(double2) * n2 * nb, st));
     HCK(hipMemsetAsync(W, 0, sizeof(double2) * n2 * nb, st));
     HCK(hipMemsetAsync(alpha2, 0, sizeof(double2) * 64, st));
     float best = 0.f;
     int choice = 1;
     for (int valu = 1; valu >= 0; --valu) {
-        HCK(bse_launch_her2k(st, fp32, valu, n2, scratch, n2, V, W, n2, nb, alpha2));
+        HCK(bse_launch_her2k(st, fp32, valu, n2, nb, scratch, V, W, n2, 0, nb, alpha2));
         HCK(hipEventRecord(ctx->tev[0], st));
-        for (int r = 0; r < 4; ++r) HCK(bse_launch_her2k(st, fp32, valu, n2, scratch, n2, V, W, n2, nb, alpha2));
+        for (int r = 0; r < 4; ++r) HCK(bse_launch_her2k(st, fp32, valu, n2, nb, scratch, V, W, n2, 0, nb, alpha2));
         HCK(hipEventRecord(ctx->tev[1], st));
         HCK(hipEventSynchronize(ctx->tev[1]));
         float ms = 0.f;
@@ -351,7 +353,7 @@ static int autotune_her2k(bse_ctx* ctx, int fp32, void* scratch, size_t scratch_
     return BSE_OK;
 }
 
-static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32,
+static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
                         const double2* rho, int n_vectors)
 {
     const int n = (int)n64;
@@ -393,13 +395,13 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int64_t ld, int fp32
     double2* Gram = BUF(double2, S_GRAM, (size_t)64 * 64);
     double2* alpha2 = BUF(double2, S_ALPHA2, 64);
     {
-        int rc = autotune_hemv(ctx, fp32, n, H, ld, acol, Yr, Yc);
+        int rc = autotune_hemv(ctx, fp32, n, H, acol, Yr, Yc);
         if (rc) return rc;
         rc = autotune_her2k(ctx, fp32, Yr, sizeof(double2) * (size_t)yr_el, Vst, Wp,
                             std::min<int64_t>(vs_el, (int64_t)n * nb), alpha2);
         if (rc) return rc;
     }
-    HCK(bse_hetrd(st, fp32, n, H, ld, nb, d, e, tauh, Vst, voff.data(), Tall, Wp, alpha2, Gram, acol,
+    HCK(bse_hetrd(st, fp32, n, H, nb, d, e, tauh, Vst, voff.data(), Tall, Wp, alpha2, Gram, acol,
                   Yr, Yc, t12, part, scal, counter, pev, ctx->hemv_ch[fp32],
                   ctx->her2k_valu[fp32] > 0 ? 1 : 0));
     if (pev) {
@@ -886,7 +888,7 @@ extern "C" int bse_eigh(bse_ctx* ctx, int n_vectors, int refine) {
     (void)hipSetDevice(ctx->device);
     const int fp32 = ctx->precision != BSE_PREC_FP64;
     ctx->eig_on_bse = true;
-    int rc = eig_pipeline(ctx, ctx->N, ctx->bufs[S_H].p, ctx->ldH, fp32,
+    int rc = eig_pipeline(ctx, ctx->N, ctx->bufs[S_H].p, fp32,
                           (const double2*)ctx->bufs[S_RHO].p, n_vectors);
     if (rc || !refine) return rc;
     stage_begin(ctx, ST_REF);
@@ -956,20 +958,24 @@ extern "C" int bse_eigh_dense(bse_ctx* ctx, int n, const double* a, int64_t lda,
     (void)hipSetDevice(ctx->device);
     const int fp32 = precision == BSE_PREC_FP32;
     int rc = BSE_OK;
-    // stage the FP64 input; FP32 storage is converted on the device
-    double2* stage = fp32 ? BUF(double2, S_STAGE, (size_t)n * n) : BUF(double2, S_H, (size_t)n * n);
-    HCK(hipMemcpy2DAsync(stage, sizeof(double2) * n, a, sizeof(double2) * lda, sizeof(double2) * n, n,
-                         a_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->st));
-    void* Hm = stage;
-    if (fp32) {
-        float2* H32 = BUF(float2, S_H, (size_t)n * n);
-        HCK(bse_launch_convert_to_fp32(ctx->st, (int64_t)n * n, stage, H32));
-        Hm = H32;
+    // the lower triangle of the FP64 input goes to tiled storage (FP64 or FP32);
+    // a device input is read in place
+    const double2* src = (const double2*)a;
+    int64_t lds = lda;
+    if (!a_on_device) {
+        double2* stage = BUF(double2, S_STAGE, (size_t)n * n);
+        HCK(hipMemcpy2DAsync(stage, sizeof(double2) * n, a, sizeof(double2) * lda, sizeof(double2) * n,
+                             n, hipMemcpyHostToDevice, ctx->st));
+        src = stage;
+        lds = n;
     }
+    void* Hm = buf(ctx, S_H, (size_t)tiled_elems(n) * (fp32 ? sizeof(float2) : sizeof(double2)), &rc);
+    if (!Hm) return rc;
+    HCK(bse_launch_to_tiles(ctx->st, fp32, n, src, lds, Hm));
     ctx->built = false;
     ctx->have_tables = false;
     ctx->eig_on_bse = false;
-    rc = eig_pipeline(ctx, n, Hm, n, fp32, nullptr, n_vectors);
+    rc = eig_pipeline(ctx, n, Hm, fp32, nullptr, n_vectors);
     if (rc) return rc;
     if (evals) HCK(hipMemcpy(evals, ctx->bufs[S_EVAL].p, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (z && ctx->nvec > 0) {

@@ -101,17 +101,22 @@ __device__ inline __amdgpu_buffer_rsrc_t bse_rsrc(const void* base, int bytes) {
 // __builtin_bit_cast applied directly to an element access (v.y) was compiled
 // as a cast of the vector's low 32 bits, dropping the imaginary part.
 __device__ inline float bse_i2f(int x) { return __builtin_bit_cast(float, x); }
+// Buffer loads of one complex element; AUX is the cache-policy operand of the
+// load (BSE_LOAD_STREAM marks data that is read once: non-temporal).
+#define BSE_LOAD_STREAM 2
 template <typename T> struct bufld;
 template <> struct bufld<float2> {
+    template <int AUX = 0>
     __device__ static inline float2 ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-        const bse_i2v v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        const bse_i2v v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX);
         const int re = v[0], im = v[1];
         return make_float2(bse_i2f(re), bse_i2f(im));
     }
 };
 template <> struct bufld<double2> {
+    template <int AUX = 0>
     __device__ static inline double2 ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-        const bse_i4v v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        const bse_i4v v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, AUX);
         const bse_i2v lo = {v.x, v.y}, hi = {v.z, v.w};
         return make_double2(__builtin_bit_cast(double, lo), __builtin_bit_cast(double, hi));
     }
@@ -184,6 +189,38 @@ __device__ inline int mfma16_row_f64(int lane, int r) { return (lane >> 4) + 4 *
 
 __host__ __device__ inline int64_t ceil_div64(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// ---------------------------------------------------------------------------
+// Storage of the assembled N x N Hermitian matrix: lower triangle only, in
+// 64 x 64 tiles that are contiguous in memory (column-major inside a tile),
+// tiles ordered by tile column and, inside a column, by tile row from the
+// diagonal down.  A tile column is one contiguous run, so a workgroup reading
+// a tile reads 32 KB (FP32) or 64 KB (FP64) of consecutive memory, and the
+// upper triangle is not stored (half the memory of a full square).
+// ---------------------------------------------------------------------------
+#define BSE_TILE 64
+__host__ __device__ inline int tile_count(int64_t n) { return (int)ceil_div64(n, BSE_TILE); }
+// Position of tile (ti, tj), ti >= tj, among nt x nt lower tiles.
+__host__ __device__ inline int64_t tile_pos(int ti, int tj, int nt) {
+    return (int64_t)tj * nt - (int64_t)tj * (tj - 1) / 2 + (ti - tj);
+}
+// Element offset of (r, c), r >= c (or any (r, c) inside a diagonal tile).
+__host__ __device__ inline int64_t tix(int64_t r, int64_t c, int nt) {
+    return tile_pos((int)(r >> 6), (int)(c >> 6), nt) * (BSE_TILE * BSE_TILE) + (r & 63) + ((c & 63) << 6);
+}
+// Elements of the tiled lower storage of an n x n matrix.
+__host__ __device__ inline int64_t tiled_elems(int64_t n) {
+    const int64_t nt = tile_count(n);
+    return nt * (nt + 1) / 2 * BSE_TILE * BSE_TILE;
+}
+// Lower tile k of an m x m tile triangle, enumerated row by row: (ti, tj).
+__host__ __device__ inline void tri_tile(int64_t k, int& ti, int& tj) {
+    int64_t r = (int64_t)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
+    while (r * (r + 1) / 2 > k) --r;
+    while ((r + 1) * (r + 2) / 2 <= k) ++r;
+    ti = (int)r;
+    tj = (int)(k - r * (r + 1) / 2);
+}
+
 // Compact-WY factor of one Householder panel (LAPACK larft, forward,
 // columnwise): Q = H_0 H_1 ... H_{nb-1} = I - V T V^H with T upper
 // triangular, built column by column as
@@ -215,8 +252,8 @@ __device__ __attribute__((always_inline)) inline void larft_body(int nb, const d
 // workspace of one matrix of an uneven batch; filled on the host).
 struct HetrdSys {
     int n;
-    int64_t lda;
-    void* A;
+    void* A;          // tiled lower storage
+
     double2* Vstore;
     double2* Tall;
     double2* Wp;
@@ -235,14 +272,15 @@ struct HetrdSys {
 };
 
 // Hemv variant for trailing size nt from an autotune table indexed by
-// floor(log2(nt)): code = ch + 32 u2 (chunk length ch in {4, 8, 16}; u2 = 1:
-// two row blocks per step, u2 = 2: per-slab cross-lane column sums); untuned
-// entries give ch = 8, u2 = 0.
-inline int hemv_ch_table_lookup(const signed char* table, int nt) {
-    if (!table || nt <= 0) return 8;
+// floor(log2(nt)): code = ch + 32 u2 + 128 t (chunk length ch in {4, 8, 16};
+// u2 = 1: two row blocks per step, u2 = 2: per-slab cross-lane column sums;
+// t = 1: default cache policy instead of non-temporal loads); untuned entries
+// give ch = 8, u2 = 2, t = 0.
+inline int hemv_ch_table_lookup(const unsigned char* table, int nt) {
+    if (!table || nt <= 0) return 8 + 64;
     int b = 0;
     while ((2 << b) <= nt && b < 30) ++b;
     const int code = table[b < BSE_HEMV_BUCKETS ? b : BSE_HEMV_BUCKETS - 1];
-    const int ch = code & 31, u2 = code >> 5;
-    return ((ch == 4 || ch == 8 || ch == 16) && u2 >= 0 && u2 <= 2) ? code : 8;
+    const int ch = code & 31, u2 = (code >> 5) & 3;
+    return ((ch == 4 || ch == 8 || ch == 16) && u2 <= 2) ? code : 8 + 64;
 }

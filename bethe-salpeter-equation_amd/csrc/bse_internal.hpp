@@ -29,20 +29,21 @@ hipError_t bse_launch_assemble(hipStream_t, int, int, int, int, int, int, int, i
                                const double2*, const double2*, const double2*, const double2*,
                                const double2*, const double2*, const double2*, const double*,
                                double, double, void*, int);
-hipError_t bse_launch_expand(hipStream_t, int, int64_t, int, int, const void*, int64_t, void*,
-                             int64_t, int);
+hipError_t bse_launch_expand(hipStream_t, int, int64_t, int, int, const void*, void*, int64_t, int);
+hipError_t bse_launch_to_tiles(hipStream_t, int, int64_t, const double2*, int64_t, void*);
+hipError_t bse_launch_tile_pad_zero(hipStream_t, int, int64_t, void*);
 void bse_hetrd_workspace(int, int, int64_t*, int64_t*, int64_t*, int64_t*);
-hipError_t bse_hetrd(hipStream_t, int, int, void*, int64_t, int, double*, double*, double2*,
+hipError_t bse_hetrd(hipStream_t, int, int, void*, int, double*, double*, double2*,
                      double2*, const int64_t*, double2*, double2*, double2*, double2*, double2*,
                      double2*, double2*, double2*, double*, double*, unsigned int*, hipEvent_t*,
-                     const signed char*, int);
-hipError_t bse_launch_hemv(hipStream_t, int, int, int, const void*, int64_t, const double2*,
-                           double2*, double2*);
-hipError_t bse_launch_last_diag(hipStream_t, int, int, const void*, int64_t, double*);
+                     const unsigned char*, int);
+hipError_t bse_launch_hemv(hipStream_t, int, int, int, int, const void*, const double2*, double2*,
+                           double2*);
+hipError_t bse_launch_last_diag(hipStream_t, int, int, const void*, double*);
 hipError_t bse_hetrd_batch_column(hipStream_t, int, int, int, int, int, const void*, int);
 hipError_t bse_hetrd_batch_panel(hipStream_t, int, int, int, int, const void*, int64_t, int);
-hipError_t bse_launch_her2k(hipStream_t, int, int, int, void*, int64_t, const double2*,
-                            const double2*, int64_t, int, const double2*);
+hipError_t bse_launch_her2k(hipStream_t, int, int, int, int, void*, const double2*,
+                            const double2*, int64_t, int, int, const double2*);
 hipError_t bse_launch_tri_prepare(hipStream_t, int, const double*, double*, double*, int4*,
                                   double2*, double*);
 hipError_t bse_launch_tri_sort(hipStream_t, int, const double*, const int4*, const double*,
@@ -128,7 +129,6 @@ struct bse_ctx {
     // state
     bool have_tb = false, have_tables = false, built = false, have_eig = false, have_f = false;
     int precision = BSE_PREC_FP64;
-    int64_t ldH = 0;
     int64_t n_eig = 0;          // size of the last eigenproblem
     int nvec = 0;               // eigenvectors formed
     bool eig_on_bse = false;    // last eig was of the assembled BSE matrix
@@ -145,11 +145,10 @@ struct bse_ctx {
     int64_t hemv_launches = 0;
     // autotuned hemv chunk length per floor(log2(trailing size)) bucket, one
     // table per storage precision (0 = not tuned yet)
-    signed char hemv_ch[2][BSE_HEMV_BUCKETS] = {};
+    unsigned char hemv_ch[2][BSE_HEMV_BUCKETS] = {};
     // trailing-update kernel per storage precision: -1 not tuned, 0 MFMA, 1 VALU
     int her2k_valu[2] = {-1, -1};
     bool autotune = true;
-    int ld_pad = 0;             // padding of the assembled matrix's leading dimension
     hipEvent_t tev[2] = {};     // autotuner timing pair
     // batched sweep state (owned by batch_api.hip)
     struct BatchState* batch = nullptr;
@@ -321,5 +320,5 @@ int make_geom(const bse_tb_model* m, const bse_system* s, Geom& g, std::string& 
 void batch_destroy(bse_ctx* ctx);
 
 // Autotuner of the tridiagonalisation's hemv chunk length (bse_api.hip).
-int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, int64_t ld, double2* v,
-                  double2* Yr, double2* Yc);
+int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, double2* v, double2* Yr,
+                  double2* Yc);

@@ -39,8 +39,7 @@
 #include <cstdlib>
 
 struct HetrdCol {
-    int n;            // matrix size
-    int64_t lda;
+    int n;            // matrix size (tiled lower storage, bse_common.hpp)
     int j0;           // panel start column
     int i;            // column inside the panel
     int m;            // n - j0 (panel rows)
@@ -132,7 +131,7 @@ __device__ __attribute__((always_inline)) inline void colupd_body(const HetrdCol
     double nrm = 0.0;
     if (t < L) {
         const int lrow = h.i + t;
-        double2 a = cplx<T>::load(A + (c + t) + (int64_t)c * h.lda);
+        double2 a = cplx<T>::load(A + tix(c + t, c, tile_count(h.n)));
         const double2* vp = h.Vp + lrow;
         const double2* wp = h.Wp + lrow;
         int l = 0;
@@ -247,63 +246,96 @@ __global__ __launch_bounds__(256) void k_makev(HetrdCol h, int colupd_gx)
     makev_body(h, blockIdx.x, blockIdx.y, gridDim.x, colupd_gx);
 }
 
-// (c): y = A22 v for the trailing Hermitian block (lower triangle stored).
-// Workgroup (q, bj) handles block column bj (64 columns) and block rows
-// bi in [bj + q*CH, bj + (q+1)*CH).  Row results of each 64x64 block go to
-// Yr[bi][bj]; the transposed (conjugate) contribution for column block bj is
-// accumulated across the chunk and written to Yc[bj][q].  One wave owns 16
-// columns; each load instruction reads 64 consecutive rows of one column.
-// The chunk length CH (block rows per workgroup) is chosen per trailing size
-// by the autotuner.
+// (c): y = A22 v for the trailing Hermitian block A22 = A[s0:, s0:] (s0 =
+// c + 1), over the 64 x 64 storage tiles that intersect it: tile column
+// TJ = tc + bj, tile rows TI = tc + bi (tc = s0 / 64).  Workgroup (q, bj)
+// handles tile rows bi in [bj + q*CH, bj + (q+1)*CH) of tile column bj; the
+// tiles of one tile column are consecutive in memory, so a workgroup streams
+// one contiguous run.  Row results of each tile go to Yr[bi][bj]; the
+// transposed (conjugate) contribution for tile column bj is accumulated across
+// the chunk and written to Yc[bj][q].  One wave owns 16 columns of the tile.
+// The chunk length CH is chosen per trailing size by the autotuner.
+//
+// Every tile is read whole and without masks.  This relies on the storage
+// invariant that padding entries (rows or columns >= n) and the strict upper
+// half of each diagonal tile are zero, so they add nothing; entries of the
+// first tile row and column that lie before s0 are finite and meet v = 0 (rows
+// and columns of v outside [s0, n) are zero), and the column sums they produce
+// belong to columns < s0, which the reduction never reads.  A diagonal tile
+// adds a(r, r) v(r) to both the row and the column sum, so Re a(r, r) v(r) is
+// taken off the row sum once (which also drops any imaginary part of the
+// diagonal).
 #define HEMV_CH_MIN 4
-template <typename T, int CH, int U2>
-__device__ __attribute__((always_inline)) inline void hemv_body(int nt, const T* __restrict__ A, int64_t lda,
-                                 const double2* __restrict__ v, double2* __restrict__ Yr,
-                                 double2* __restrict__ Yc, int nbk, int nq, int q, int bj)
+// Variant code: ch + 32 u2 + 128 t (chunk length ch in {4, 8, 16}; u2 = 1 two
+// tile rows per step, u2 = 2 per-slab cross-lane column sums; t = 1 default
+// cache policy for the matrix loads, t = 0 non-temporal: a large trailing
+// block is streamed once per column and cannot stay in the caches, a small
+// one can).  AUX is the resulting load cache-policy operand.
+#define HEMV_CODE_DEFAULT (8 + 64)
+__host__ __device__ inline bool hemv_code_valid(int code) {
+    const int ch = code & 31, u2 = (code >> 5) & 3;
+    return code >= 0 && code < 256 && (ch == 4 || ch == 8 || ch == 16) && u2 <= 2;
+}
+
+__device__ inline double2 hemv_v(const double2* __restrict__ v, int g, int s0, int n) {
+    return (g >= s0 && g < n) ? v[g - s0] : cmk(0, 0);
+}
+
+// Re a(row, row) of a diagonal tile from this wave's 16 loaded columns (zero
+// in lanes whose diagonal column belongs to another wave).
+template <typename T>
+__device__ __attribute__((always_inline)) inline double hemv_diag_re(const T (&a)[16], int lane, int w) {
+    double d = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) d = (w * 16 + jj == lane) ? (double)a[jj].x : d;
+    return d;
+}
+
+template <typename T, int CH, int U2, int AUX>
+__device__ __attribute__((always_inline)) inline void hemv_body(int n, int s0, const T* __restrict__ A,
+                                                                 const double2* __restrict__ v,
+                                                                 double2* __restrict__ Yr,
+                                                                 double2* __restrict__ Yc, int nbk,
+                                                                 int nq, int q, int bj)
 {
     const int bi0 = bj + q * CH;
     if (bi0 >= nbk) return;
     const int bi1 = min(nbk, bi0 + CH);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntile = tile_count(n), tc = s0 >> 6, TJ = tc + bj;
     __shared__ double2 s_v[64];
     __shared__ double2 s_part[4][CH][64];
-    const int col0 = bj * 64;
-    if (threadIdx.x < 64) {
-        const int cidx = col0 + threadIdx.x;
-        s_v[threadIdx.x] = (cidx < nt) ? v[cidx] : cmk(0, 0);
-    }
+    if (threadIdx.x < 64) s_v[threadIdx.x] = hemv_v(v, TJ * 64 + threadIdx.x, s0, n);
     __syncthreads();
     const double2* sv = s_v + w * 16;         // this wave's 16 entries of v (LDS)
     double2 cacc[16];
 #pragma unroll
     for (int jj = 0; jj < 16; ++jj) cacc[jj] = cmk(0, 0);
-    const int cw = col0 + w * 16;             // first column of this wave
-    // interior blocks are read through a buffer descriptor based at this
-    // wave's first column: lane offset row * sizeof(T), column offset
-    // jj * lda * sizeof(T) in a scalar register (16 columns < 2^31 bytes)
+    // the chunk's tiles are one contiguous run starting at tile (tc+bi0, TJ);
+    // descriptor base = that run + this wave's first column, lane offset
+    // (tile, row) and column offset jj * 64 elements in a scalar register
     const int wu = __builtin_amdgcn_readfirstlane(w);
-    const __amdgpu_buffer_rsrc_t rs = bse_rsrc(A + (int64_t)(col0 + wu * 16) * lda, 0x7fffffff);
-    const int cstride = (int)(lda * (int64_t)sizeof(T));
-    const bool pack_ok = (lda & 1) == 0;      // 16-byte alignment of row pairs
+    const T* run = A + tile_pos(tc + bi0, TJ, ntile) * (BSE_TILE * BSE_TILE);
+    const __amdgpu_buffer_rsrc_t rs = bse_rsrc(run + wu * 16 * 64, 0x7fffffff);
+    const int cstride = 64 * (int)sizeof(T);
     for (int bi = bi0; bi < bi1;) {
-        const int row = bi * 64 + lane;
-        const T* Ap = A + row + (int64_t)cw * lda;
-        const bool full0 = bi > bj && (bi + 1) * 64 <= nt;
-        const int vo = row * (int)sizeof(T);
-        if (U2 && sizeof(T) == 8 && pack_ok && full0 && bi + 1 < bi1 && (bi + 2) * 64 <= nt) {
-            // FP32 storage, two interior row blocks: lane l reads the two
-            // consecutive rows 2l, 2l+1 of the 128-row slab in one 16-byte
-            // load (1 KB per wave instruction)
-            const int r2 = bi * 64 + 2 * lane;
-            bse_i4v q[16];
+        const int TI = tc + bi;
+        const bool diag = TI == TJ;
+        if (U2 && sizeof(T) == 8 && !diag && bi + 1 < bi1) {
+            // FP32 storage, two off-diagonal tiles: lane l reads the
+            // consecutive rows 2l, 2l+1 of the 128-row slab (tile TI for
+            // l < 32, TI+1 above) in one 16-byte load
+            const int hb = lane >> 5, rl = (2 * lane) & 63;
+            const int vo2 = ((bi - bi0 + hb) * 4096 + rl) * (int)sizeof(T);
+            bse_i4v qv[16];
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj)
-                q[jj] = __builtin_amdgcn_raw_buffer_load_b128(rs, r2 * (int)sizeof(T), jj * cstride, 0);
-            const double2 v0 = v[r2], v1 = v[r2 + 1];
+            for (int jj = 0; jj < 16; ++jj) qv[jj] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo2, jj * cstride, AUX);
+            const int r2 = (TI + hb) * 64 + rl;
+            const double2 v0 = hemv_v(v, r2, s0, n), v1 = hemv_v(v, r2 + 1, s0, n);
             double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) {
-                const int q0 = q[jj][0], q1 = q[jj][1], q2 = q[jj][2], q3 = q[jj][3];
+                const int q0 = qv[jj][0], q1 = qv[jj][1], q2 = qv[jj][2], q3 = qv[jj][3];
                 const double2 x = cmk((double)bse_i2f(q0), (double)bse_i2f(q1));
                 const double2 y = cmk((double)bse_i2f(q2), (double)bse_i2f(q3));
                 cfma(r0, x, sv[jj]);
@@ -311,22 +343,22 @@ __device__ __attribute__((always_inline)) inline void hemv_body(int nt, const T*
                 cfma(r1, y, sv[jj]);
                 cfmac(cacc[jj], y, v1);
             }
-            // rows 2l, 2l+1 of the slab: block bi for l < 32, bi+1 above
-            const int hb = lane >> 5, rl = (2 * lane) & 63;
             s_part[w][bi + hb - bi0][rl] = r0;
             s_part[w][bi + hb - bi0][rl + 1] = r1;
             bi += 2;
             continue;
         }
-        if (U2 && sizeof(T) == 16 && full0 && bi + 1 < bi1 && (bi + 2) * 64 <= nt) {
-            // two interior row blocks: 32 loads in flight per lane
+        const int vo = ((bi - bi0) * 4096 + lane) * (int)sizeof(T);
+        const int row = TI * 64 + lane;
+        if (U2 && sizeof(T) == 16 && !diag && bi + 1 < bi1) {
+            // two off-diagonal tiles: 32 loads in flight per lane
             T a[16], b[16];
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) {
-                a[jj] = bufld<T>::ld(rs, vo, jj * cstride);
-                b[jj] = bufld<T>::ld(rs, vo + 64 * (int)sizeof(T), jj * cstride);
+                a[jj] = bufld<T>::template ld<AUX>(rs, vo, jj * cstride);
+                b[jj] = bufld<T>::template ld<AUX>(rs, vo + 4096 * (int)sizeof(T), jj * cstride);
             }
-            const double2 vr0 = v[row], vr1 = v[row + 64];
+            const double2 vr0 = hemv_v(v, row, s0, n), vr1 = hemv_v(v, row + 64, s0, n);
             double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) {
@@ -342,33 +374,18 @@ __device__ __attribute__((always_inline)) inline void hemv_body(int nt, const T*
             bi += 2;
             continue;
         }
-        const bool rvalid = row < nt;
-        const double2 vr = rvalid ? v[row] : cmk(0, 0);
+        const double2 vr = hemv_v(v, row, s0, n);
         double2 racc = cmk(0, 0);
-        if (full0) {
-            T a[16];
+        T a[16];
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) a[jj] = bufld<T>::ld(rs, vo, jj * cstride);
+        for (int jj = 0; jj < 16; ++jj) a[jj] = bufld<T>::template ld<AUX>(rs, vo, jj * cstride);
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
-                cfma(racc, x, sv[jj]);
-                cfmac(cacc[jj], x, vr);
-            }
-        } else {
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                const int col = cw + jj;
-                if (!rvalid || col >= nt || row < col) continue;
-                double2 a = cplx<T>::load(Ap + (int64_t)jj * lda);
-                if (row == col) {
-                    cfma(racc, cmk(a.x, 0.0), sv[jj]);
-                } else {
-                    cfma(racc, a, sv[jj]);
-                    cfmac(cacc[jj], a, vr);
-                }
-            }
+        for (int jj = 0; jj < 16; ++jj) {
+            const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
+            cfma(racc, x, sv[jj]);
+            cfmac(cacc[jj], x, vr);
         }
+        if (diag) racc = csub(racc, cscale(vr, hemv_diag_re(a, lane, w)));
         s_part[w][bi - bi0][lane] = racc;
         ++bi;
     }
@@ -416,13 +433,14 @@ __device__ inline int colsum16_col(int lane) {
     return 8 * ((lane >> 5) & 1) + 4 * ((lane >> 4) & 1) + 2 * ((lane >> 3) & 1) + ((lane >> 2) & 1);
 }
 
+
 // Hemv variant with few live registers: the column partials of each slab (128
 // rows for FP32 storage read as row pairs, 64 rows otherwise) are reduced
 // across the wave at once and added to one FP64 accumulator per lane, so the
 // kernel keeps four or more waves per SIMD while a full slab of loads is in
-// flight.  Same partial outputs as hemv_body.
-template <typename T, int CH>
-__device__ __attribute__((always_inline)) inline void hemv_bfly_body(int nt, const T* __restrict__ A, int64_t lda,
+// flight.  Same tiles, masks and partial outputs as hemv_body.
+template <typename T, int CH, int AUX>
+__device__ __attribute__((always_inline)) inline void hemv_bfly_body(int n, int s0, const T* __restrict__ A,
                                                                       const double2* __restrict__ v,
                                                                       double2* __restrict__ Yr,
                                                                       double2* __restrict__ Yc, int nbk,
@@ -432,84 +450,65 @@ __device__ __attribute__((always_inline)) inline void hemv_bfly_body(int nt, con
     if (bi0 >= nbk) return;
     const int bi1 = min(nbk, bi0 + CH);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntile = tile_count(n), tc = s0 >> 6, TJ = tc + bj;
     __shared__ double2 s_v[64];
     __shared__ double2 s_part[4][CH][64];
-    const int col0 = bj * 64;
-    if (threadIdx.x < 64) {
-        const int cidx = col0 + threadIdx.x;
-        s_v[threadIdx.x] = (cidx < nt) ? v[cidx] : cmk(0, 0);
-    }
+    if (threadIdx.x < 64) s_v[threadIdx.x] = hemv_v(v, TJ * 64 + threadIdx.x, s0, n);
     __syncthreads();
     const double2* sv = s_v + w * 16;
     double2 colacc = cmk(0, 0);
-    const int cw = col0 + w * 16;
     const int wu = __builtin_amdgcn_readfirstlane(w);
-    const __amdgpu_buffer_rsrc_t rs = bse_rsrc(A + (int64_t)(col0 + wu * 16) * lda, 0x7fffffff);
-    const int cstride = (int)(lda * (int64_t)sizeof(T));
-    const bool pack_ok = (lda & 1) == 0;
+    const T* run = A + tile_pos(tc + bi0, TJ, ntile) * (BSE_TILE * BSE_TILE);
+    const __amdgpu_buffer_rsrc_t rs = bse_rsrc(run + wu * 16 * 64, 0x7fffffff);
+    const int cstride = 64 * (int)sizeof(T);
     for (int bi = bi0; bi < bi1;) {
-        const bool full0 = bi > bj && (bi + 1) * 64 <= nt;
+        const int TI = tc + bi;
+        const bool diag = TI == TJ;
         double2 c[16];
-        if (sizeof(T) == 8 && pack_ok && full0 && bi + 1 < bi1 && (bi + 2) * 64 <= nt) {
-            const int r2 = bi * 64 + 2 * lane;
+        if (sizeof(T) == 8 && !diag && bi + 1 < bi1) {
+            const int hb = lane >> 5, rl = (2 * lane) & 63;
+            const int vo2 = ((bi - bi0 + hb) * 4096 + rl) * (int)sizeof(T);
             bse_i4v qv[16];
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj)
-                qv[jj] = __builtin_amdgcn_raw_buffer_load_b128(rs, r2 * (int)sizeof(T), jj * cstride, 0);
-            const double2 v0 = v[r2], v1 = v[r2 + 1];
+            for (int jj = 0; jj < 16; ++jj) qv[jj] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo2, jj * cstride, AUX);
+            const int r2 = (TI + hb) * 64 + rl;
+            const double2 v0 = hemv_v(v, r2, s0, n), v1 = hemv_v(v, r2 + 1, s0, n);
             double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) {
                 const int q0 = qv[jj][0], q1 = qv[jj][1], q2 = qv[jj][2], q3 = qv[jj][3];
                 const double2 x = cmk((double)bse_i2f(q0), (double)bse_i2f(q1));
                 const double2 y = cmk((double)bse_i2f(q2), (double)bse_i2f(q3));
-                const double2 s = sv[jj];
-                cfma(r0, x, s);
-                cfma(r1, y, s);
+                const double2 sj = sv[jj];
+                cfma(r0, x, sj);
+                cfma(r1, y, sj);
                 double2 cc = cmk(0, 0);
                 cfmac(cc, x, v0);
                 cfmac(cc, y, v1);
                 c[jj] = cc;
             }
             colacc = cadd(colacc, colsum16(c, lane));
-            const int hb = lane >> 5, rl = (2 * lane) & 63;
             s_part[w][bi + hb - bi0][rl] = r0;
             s_part[w][bi + hb - bi0][rl + 1] = r1;
             bi += 2;
             continue;
         }
-        const int row = bi * 64 + lane;
-        const bool rvalid = row < nt;
-        const double2 vr = rvalid ? v[row] : cmk(0, 0);
+        const int row = TI * 64 + lane;
+        const double2 vr = hemv_v(v, row, s0, n);
         double2 racc = cmk(0, 0);
-        if (full0) {
-            T a[16];
+        const int vo = ((bi - bi0) * 4096 + lane) * (int)sizeof(T);
+        T a[16];
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) a[jj] = bufld<T>::ld(rs, row * (int)sizeof(T), jj * cstride);
+        for (int jj = 0; jj < 16; ++jj) a[jj] = bufld<T>::template ld<AUX>(rs, vo, jj * cstride);
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
-                cfma(racc, x, sv[jj]);
-                double2 cc = cmk(0, 0);
-                cfmac(cc, x, vr);
-                c[jj] = cc;
-            }
-        } else {
-            const T* Ap = A + row + (int64_t)cw * lda;
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                c[jj] = cmk(0, 0);
-                const int col = cw + jj;
-                if (!rvalid || col >= nt || row < col) continue;
-                const double2 x = cplx<T>::load(Ap + (int64_t)jj * lda);
-                if (row == col) {
-                    cfma(racc, cmk(x.x, 0.0), sv[jj]);
-                } else {
-                    cfma(racc, x, sv[jj]);
-                    cfmac(c[jj], x, vr);
-                }
-            }
+        for (int jj = 0; jj < 16; ++jj) {
+            const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
+            cfma(racc, x, sv[jj]);
+            double2 cc = cmk(0, 0);
+            cfmac(cc, x, vr);
+            c[jj] = cc;
         }
+        if (diag) racc = csub(racc, cscale(vr, hemv_diag_re(a, lane, w)));
         colacc = cadd(colacc, colsum16(c, lane));
         s_part[w][bi - bi0][lane] = racc;
         ++bi;
@@ -524,21 +523,30 @@ __device__ __attribute__((always_inline)) inline void hemv_bfly_body(int nt, con
     if ((lane & 3) == 0) Yc[((int64_t)bj * nq + q) * 64 + w * 16 + colsum16_col(lane)] = colacc;
 }
 
-// Occupancy matters more than registers for this streaming kernel: without a
-// bound the compiler spends ~250 VGPRs and runs one wave per SIMD.  U2 (two
-// row blocks per step) holds 16 more loads per lane and gets one wave less.
-#ifndef HEMV_WPE
-#define HEMV_WPE 3
+// Occupancy bound of the hemv kernels: two waves per SIMD (up to 256 VGPRs)
+// with 16 to 32 loads in flight per lane; tighter bounds make the compiler
+// spill the column accumulators.
+#ifndef HEMV_WPE_FOR
+#define HEMV_WPE_FOR(U2) 2
 #endif
-#define HEMV_WPE_FOR(U2) ((U2) == 2 ? HEMV_WPE + 1 : ((U2) && HEMV_WPE > 1 ? HEMV_WPE - 1 : HEMV_WPE))
-template <typename T, int CH, int U2>
+template <typename T, int CH, int U2, int AUX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HEMV_WPE_FOR(U2), 8)))
-void k_hemv_lower(int nt, const T* __restrict__ A, int64_t lda, const double2* __restrict__ v,
+void k_hemv_lower(int n, int s0, const T* __restrict__ A, const double2* __restrict__ v,
                   double2* __restrict__ Yr, double2* __restrict__ Yc, int nbk, int nq)
 {
-    if (U2 == 2) hemv_bfly_body<T, CH>(nt, A, lda, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
-    else hemv_body<T, CH, U2>(nt, A, lda, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    if (U2 == 2) hemv_bfly_body<T, CH, AUX>(n, s0, A, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    else hemv_body<T, CH, U2, AUX>(n, s0, A, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
 }
+
+// Instantiates M(CH, U2, AUX) for the variant of a (valid) code.
+#define HEMV_DISPATCH_CH(ch_, UU, AX, M) \
+    { if (ch_ == 4) M(4, UU, AX); else if (ch_ == 16) M(16, UU, AX); else M(8, UU, AX); }
+#define HEMV_DISPATCH_U2(ch_, u2_, AX, M) \
+    { if (u2_ == 2) HEMV_DISPATCH_CH(ch_, 2, AX, M) else if (u2_ == 1) HEMV_DISPATCH_CH(ch_, 1, AX, M) \
+      else HEMV_DISPATCH_CH(ch_, 0, AX, M) }
+#define HEMV_DISPATCH(code, M) \
+    { const int ch_ = (code) & 31, u2_ = ((code) >> 5) & 3; \
+      if ((code) & 128) HEMV_DISPATCH_U2(ch_, u2_, 0, M) else HEMV_DISPATCH_U2(ch_, u2_, BSE_LOAD_STREAM, M) }
 
 // Reduce the hemv partials of one block row (64 rows) per workgroup, apply the
 // panel corrections (d) and the tau scaling (e), partial dot y^H v.  Sixteen
@@ -549,7 +557,7 @@ void k_hemv_lower(int nt, const T* __restrict__ A, int64_t lda, const double2* _
 // finished by the next kernel.
 #define RED_G 16
 #define RED_STAGE 2048        // makev partials staged in LDS (32 KB)
-__device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol& h, int nt,
+__device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol& h,
                                                                    const double2* __restrict__ Yr,
                                                                    const double2* __restrict__ Yc,
                                                                    int nbk, int nq, int ch,
@@ -580,9 +588,11 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
         }
         __syncthreads();
     }
-    const int r = b * 64 + lane;
-    const bool valid = r < nt;
-    const int lr = h.i + 1 + r;                         // panel-local row
+    // block b covers storage tile row (s0 / 64) + b of the trailing block
+    const int s0 = h.j0 + h.i + 1;
+    const int grow = ((s0 >> 6) + b) * 64 + lane;       // matrix row
+    const bool valid = grow >= s0 && grow < h.n;
+    const int lr = grow - h.j0;                         // panel-local row
     double2 acc0 = cmk(0, 0), acc1 = cmk(0, 0);
     if (valid) {
         // partials of this block row: Yr[b][0..b] and Yc[b][0..nqb), several
@@ -633,11 +643,11 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
     if (threadIdx.x == 0) ((double2*)(h.part + pl.c))[b] = dotp;
 }
 
-__global__ __launch_bounds__(1024) void k_hemv_reduce(HetrdCol h, int nt, const double2* __restrict__ Yr,
+__global__ __launch_bounds__(1024) void k_hemv_reduce(HetrdCol h, const double2* __restrict__ Yr,
                                                       const double2* __restrict__ Yc, int nbk, int nq,
                                                       int ch, int makev_nrb)
 {
-    reduce_body(h, nt, Yr, Yc, nbk, nq, ch, makev_nrb, blockIdx.x);
+    reduce_body(h, Yr, Yc, nbk, nq, ch, makev_nrb, blockIdx.x);
 }
 
 // Panel close: alpha2 of the panel's last column (from its reduce partials,
@@ -658,25 +668,24 @@ __global__ __launch_bounds__(64) void k_panel_close(HetrdCol h, int nbp, int red
     panel_close_body(h, nbp, red_gx, T, ldt);
 }
 
-// Trailing update A22 -= V W^H + W V^H on lower 64x64 tiles, K = nb.
-// V, W rows start at panel row nb (ld m).
+// Trailing update A22 -= V W^H + W V^H, K = nb, for the trailing block
+// A[s:, s:] (s = j0 + nb) over the 64 x 64 storage tiles that intersect it:
+// block idx is tile (tcs + ti, tcs + tj) with (ti, tj) the idx-th lower tile
+// (tcs = s / 64); rows and columns below s are masked.  V and W are the panel
+// (global row j0 at index 0, ld ldv).
 template <typename T>
-__device__ __attribute__((always_inline)) inline void her2k_valu_body(int n2, T* __restrict__ A, int64_t lda,
+__device__ __attribute__((always_inline)) inline void her2k_valu_body(int n, int s, T* __restrict__ A,
                                        const double2* __restrict__ V,
-                                       const double2* __restrict__ W, int64_t ldv, int nb,
+                                       const double2* __restrict__ W, int64_t ldv, int j0, int nb,
                                        const double2* __restrict__ alpha2, int64_t idx)
 {
     int ti, tj;
-    {
-        int64_t r = (int64_t)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
-        while (r * (r + 1) / 2 > idx) --r;
-        while ((r + 1) * (r + 2) / 2 <= idx) ++r;
-        ti = (int)r;
-        tj = (int)(idx - r * (r + 1) / 2);
-    }
+    tri_tile(idx, ti, tj);
+    const int tcs = s >> 6;
+    T* tile = A + tile_pos(tcs + ti, tcs + tj, tile_count(n)) * (BSE_TILE * BSE_TILE);
     __shared__ double2 sVr[8][64], sWr[8][64], sVc[8][64], sWc[8][64];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;   // 16 x 16 threads
-    const int r0 = ti * 64, c0 = tj * 64;
+    const int r0 = (tcs + ti) * 64, c0 = (tcs + tj) * 64;
     double2 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -688,13 +697,14 @@ __device__ __attribute__((always_inline)) inline void her2k_valu_body(int n2, T*
             const int l = l0 + ll;
             const bool lok = l < nb;
             const int gr = r0 + rr, gc = c0 + rr;
+            const bool rok = lok && gr >= s && gr < n, cok = lok && gc >= s && gc < n;
             const double2 a2 = lok ? alpha2[l] : cmk(0, 0);
-            const double2 vr = (lok && gr < n2) ? V[gr + (int64_t)l * ldv] : cmk(0, 0);
-            const double2 vc = (lok && gc < n2) ? V[gc + (int64_t)l * ldv] : cmk(0, 0);
+            const double2 vr = rok ? V[(gr - j0) + (int64_t)l * ldv] : cmk(0, 0);
+            const double2 vc = cok ? V[(gc - j0) + (int64_t)l * ldv] : cmk(0, 0);
             sVr[ll][rr] = vr;
-            sWr[ll][rr] = (lok && gr < n2) ? cadd(W[gr + (int64_t)l * ldv], cmul(a2, vr)) : cmk(0, 0);
+            sWr[ll][rr] = rok ? cadd(W[(gr - j0) + (int64_t)l * ldv], cmul(a2, vr)) : cmk(0, 0);
             sVc[ll][rr] = vc;
-            sWc[ll][rr] = (lok && gc < n2) ? cadd(W[gc + (int64_t)l * ldv], cmul(a2, vc)) : cmk(0, 0);
+            sWc[ll][rr] = cok ? cadd(W[(gc - j0) + (int64_t)l * ldv], cmul(a2, vc)) : cmk(0, 0);
         }
         __syncthreads();
 #pragma unroll
@@ -722,8 +732,8 @@ __device__ __attribute__((always_inline)) inline void her2k_valu_body(int n2, T*
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int gr = r0 + tx + 16 * a, gc = c0 + ty + 16 * b;
-            if (gr < n2 && gc < n2 && gr >= gc) {
-                T* p = A + gr + (int64_t)gc * lda;
+            if (gc >= s && gr < n && gr >= gc) {
+                T* p = tile + (tx + 16 * a) + (ty + 16 * b) * 64;
                 double2 x = cplx<T>::load(p);
                 x = csub(x, acc[a][b]);
                 if (gr == gc) x.y = 0.0;
@@ -755,25 +765,21 @@ struct her2k_f64 {
 };
 
 template <typename T>
-__device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n2, T* __restrict__ A, int64_t lda,
+__device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int s, T* __restrict__ A,
                                        const double2* __restrict__ V,
-                                       const double2* __restrict__ W, int64_t ldv, int nb,
+                                       const double2* __restrict__ W, int64_t ldv, int j0, int nb,
                                        const double2* __restrict__ alpha2, int64_t idx)
 {
     typedef her2k_f64::real R;
     typedef her2k_f64::acc ACC;
     int ti, tj;
-    {
-        int64_t r = (int64_t)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
-        while (r * (r + 1) / 2 > idx) --r;
-        while ((r + 1) * (r + 2) / 2 <= idx) ++r;
-        ti = (int)r;
-        tj = (int)(idx - r * (r + 1) / 2);
-    }
+    tri_tile(idx, ti, tj);
+    const int tcs = s >> 6;
+    T* tile = A + tile_pos(tcs + ti, tcs + tj, tile_count(n)) * (BSE_TILE * BSE_TILE);
     // chunk of 16 complex K columns: real and imaginary parts, [kcol][row]
     __shared__ R sPr[16][64], sPi[16][64], sQr[16][64], sQi[16][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int r0 = ti * 64, c0 = tj * 64;
+    const int r0 = (tcs + ti) * 64, c0 = (tcs + tj) * 64;
     const int rowbase = (wv >> 1) * 32, colbase = (wv & 1) * 32;
     ACC accR[2][2], accI[2][2];
 #pragma unroll
@@ -794,14 +800,14 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n2, T*
                 const int l = first ? kk : kk - nb;
                 const double2 a2 = alpha2[l];
                 const int gr = r0 + rr, gc = c0 + rr;
-                if (gr < n2) {
-                    const double2 v = V[gr + (int64_t)l * ldv];
-                    const double2 w = cadd(W[gr + (int64_t)l * ldv], cmul(a2, v));
+                if (gr >= s && gr < n) {
+                    const double2 v = V[(gr - j0) + (int64_t)l * ldv];
+                    const double2 w = cadd(W[(gr - j0) + (int64_t)l * ldv], cmul(a2, v));
                     pv = first ? v : w;
                 }
-                if (gc < n2) {
-                    const double2 v = V[gc + (int64_t)l * ldv];
-                    const double2 w = cadd(W[gc + (int64_t)l * ldv], cmul(a2, v));
+                if (gc >= s && gc < n) {
+                    const double2 v = V[(gc - j0) + (int64_t)l * ldv];
+                    const double2 w = cadd(W[(gc - j0) + (int64_t)l * ldv], cmul(a2, v));
                     qv = first ? w : v;
                 }
             }
@@ -847,10 +853,11 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n2, T*
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int reg = 0; reg < 4; ++reg) {
-                const int gr = r0 + rowbase + a * 16 + (lane & 15);
-                const int gc = c0 + colbase + b * 16 + her2k_f64::row(lane, reg);
-                if (gr < n2 && gc < n2 && gr >= gc) {
-                    T* p = A + gr + (int64_t)gc * lda;
+                const int lr = rowbase + a * 16 + (lane & 15);
+                const int lc = colbase + b * 16 + her2k_f64::row(lane, reg);
+                const int gr = r0 + lr, gc = c0 + lc;
+                if (gc >= s && gr < n && gr >= gc) {
+                    T* p = tile + lr + lc * 64;
                     double2 x = cplx<T>::load(p);
                     x.x -= (double)accR[a][b][reg];
                     x.y -= (double)accI[a][b][reg];
@@ -861,28 +868,29 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n2, T*
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_her2k_lower(int n2, T* __restrict__ A, int64_t lda,
+__global__ __launch_bounds__(256) void k_her2k_lower(int n, int s, T* __restrict__ A,
                                                     const double2* __restrict__ V,
                                                     const double2* __restrict__ W, int64_t ldv,
-                                                    int nb, const double2* __restrict__ alpha2)
+                                                    int j0, int nb, const double2* __restrict__ alpha2)
 {
-    her2k_valu_body<T>(n2, A, lda, V, W, ldv, nb, alpha2, blockIdx.x);
+    her2k_valu_body<T>(n, s, A, V, W, ldv, j0, nb, alpha2, blockIdx.x);
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_her2k_mfma(int n2, T* __restrict__ A, int64_t lda,
+__global__ __launch_bounds__(256) void k_her2k_mfma(int n, int s, T* __restrict__ A,
                                                    const double2* __restrict__ V,
                                                    const double2* __restrict__ W, int64_t ldv,
-                                                   int nb, const double2* __restrict__ alpha2)
+                                                   int j0, int nb, const double2* __restrict__ alpha2)
 {
-    her2k_mfma_body<T>(n2, A, lda, V, W, ldv, nb, alpha2, blockIdx.x);
+    her2k_mfma_body<T>(n, s, A, V, W, ldv, j0, nb, alpha2, blockIdx.x);
 }
 
-__global__ void k_last_diag(int n, const void* A, int64_t lda, int fp32, double* d)
+__global__ void k_last_diag(int n, const void* A, int fp32, double* d)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        if (fp32) d[n - 1] = (double)((const float2*)A)[(n - 1) + (int64_t)(n - 1) * lda].x;
-        else d[n - 1] = ((const double2*)A)[(n - 1) + (int64_t)(n - 1) * lda].x;
+        const int64_t o = tix(n - 1, n - 1, tile_count(n));
+        if (fp32) d[n - 1] = (double)((const float2*)A)[o].x;
+        else d[n - 1] = ((const double2*)A)[o].x;
     }
 }
 
@@ -903,7 +911,7 @@ __host__ __device__ inline int64_t panel_voff(int n, int nb, int p) {
 __device__ inline bool batch_col(const HetrdSys& s, int c, int nb, HetrdCol& h) {
     if (c >= s.n - 1) return false;
     const int p = c / nb;
-    h.n = s.n; h.lda = s.lda; h.j0 = p * nb; h.i = c - h.j0; h.m = s.n - h.j0;
+    h.n = s.n; h.j0 = p * nb; h.i = c - h.j0; h.m = s.n - h.j0;
     h.Vp = s.Vstore + panel_voff(s.n, nb, p);
     h.Wp = s.Wp; h.alpha2 = s.alpha2; h.G = s.G; h.acol = s.acol; h.d = s.d; h.e = s.e;
     h.tau = s.tau; h.scal = s.scal; h.part = s.part; h.counter = s.counter;
@@ -918,7 +926,7 @@ __global__ __launch_bounds__(256) void kb_colupd(const HetrdSys* __restrict__ sy
     if (!batch_col(s, c, nb, h)) return;
     const int gx = (int)ceil_div64(h.m - h.i, 256);
     if ((int)blockIdx.x >= gx) return;
-    const int prev_gx = (h.i > 0) ? (int)ceil_div64(s.n - c, 64) : 0;
+    const int prev_gx = (h.i > 0) ? tile_count(s.n) - (c >> 6) : 0;
     colupd_body<T>(h, (const T*)s.A, blockIdx.x, gx, prev_gx);
 }
 
@@ -932,20 +940,19 @@ __global__ __launch_bounds__(256) void kb_makev(const HetrdSys* __restrict__ sys
     makev_body(h, blockIdx.x, blockIdx.y, gx, (int)ceil_div64(h.m - h.i, 256));
 }
 
-template <typename T, int CH, int U2>
+template <typename T, int CH, int U2, int AUX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HEMV_WPE_FOR(U2), 8)))
 void kb_hemv(const HetrdSys* __restrict__ sys, int c, int nb)
 {
     HetrdCol h;
     const HetrdSys s = sys[blockIdx.z];
     if (!batch_col(s, c, nb, h)) return;
-    const int nt = s.n - c - 1;
-    const int nbk = (int)ceil_div64(nt, 64), nq = (int)ceil_div64(nbk, CH);
+    const int nbk = tile_count(s.n) - ((c + 1) >> 6), nq = (int)ceil_div64(nbk, CH);
     if ((int)blockIdx.x >= nq || (int)blockIdx.y >= nbk) return;
-    const T* A22 = (const T*)s.A + (c + 1) + (int64_t)(c + 1) * s.lda;
+    const T* A = (const T*)s.A;
     const double2* vtr = h.Vp + (h.i + 1) + (int64_t)h.i * h.m;
-    if (U2 == 2) hemv_bfly_body<T, CH>(nt, A22, s.lda, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
-    else hemv_body<T, CH, U2>(nt, A22, s.lda, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    if (U2 == 2) hemv_bfly_body<T, CH, AUX>(s.n, c + 1, A, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    else hemv_body<T, CH, U2, AUX>(s.n, c + 1, A, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
 }
 
 __global__ __launch_bounds__(1024) void kb_reduce(const HetrdSys* __restrict__ sys, int c, int nb,
@@ -954,10 +961,9 @@ __global__ __launch_bounds__(1024) void kb_reduce(const HetrdSys* __restrict__ s
     HetrdCol h;
     const HetrdSys s = sys[blockIdx.z];
     if (!batch_col(s, c, nb, h)) return;
-    const int nt = s.n - c - 1;
-    const int nbk = (int)ceil_div64(nt, 64), nq = (int)ceil_div64(nbk, ch);
+    const int nbk = tile_count(s.n) - ((c + 1) >> 6), nq = (int)ceil_div64(nbk, ch);
     if ((int)blockIdx.x >= nbk) return;
-    reduce_body(h, nt, s.Yr, s.Yc, nbk, nq, ch, (int)ceil_div64(h.m, MAKEV_ROWS), blockIdx.x);
+    reduce_body(h, s.Yr, s.Yc, nbk, nq, ch, (int)ceil_div64(h.m, MAKEV_ROWS), blockIdx.x);
 }
 
 // Column c closes the matrix's current panel when c = j0 + nbp - 1.
@@ -976,7 +982,7 @@ __global__ __launch_bounds__(64) void kb_larft(const HetrdSys* __restrict__ sys,
     if (!batch_panel_end(s, c, nb, j0, nbp, p)) return;
     HetrdCol h;
     batch_col(s, c, nb, h);
-    panel_close_body(h, nbp, (int)ceil_div64(s.n - c - 1, 64), s.Tall + (size_t)p * nb * nb, nb);
+    panel_close_body(h, nbp, tile_count(s.n) - ((c + 1) >> 6), s.Tall + (size_t)p * nb * nb, nb);
 }
 
 template <typename T, int MFMA>
@@ -985,16 +991,15 @@ __global__ __launch_bounds__(256) void kb_her2k(const HetrdSys* __restrict__ sys
     const HetrdSys s = sys[blockIdx.z];
     int j0, nbp, p;
     if (!batch_panel_end(s, c, nb, j0, nbp, p)) return;
-    const int m = s.n - j0, n2 = m - nbp;
-    if (n2 <= 0) return;
-    const int64_t nt2 = ceil_div64(n2, 64);
-    if ((int64_t)blockIdx.x >= nt2 * (nt2 + 1) / 2) return;
-    T* A = (T*)s.A + (j0 + nbp) + (int64_t)(j0 + nbp) * s.lda;
+    const int m = s.n - j0, st = j0 + nbp;
+    if (st >= s.n) return;
+    const int64_t nT = tile_count(s.n) - (st >> 6);
+    if ((int64_t)blockIdx.x >= nT * (nT + 1) / 2) return;
     const double2* Vp = s.Vstore + panel_voff(s.n, nb, p);
     if (MFMA)
-        her2k_mfma_body<T>(n2, A, s.lda, Vp + nbp, s.Wp + nbp, (int64_t)m, nbp, s.alpha2, blockIdx.x);
+        her2k_mfma_body<T>(s.n, st, (T*)s.A, Vp, s.Wp, (int64_t)m, j0, nbp, s.alpha2, blockIdx.x);
     else
-        her2k_valu_body<T>(n2, A, s.lda, Vp + nbp, s.Wp + nbp, (int64_t)m, nbp, s.alpha2, blockIdx.x);
+        her2k_valu_body<T>(s.n, st, (T*)s.A, Vp, s.Wp, (int64_t)m, j0, nbp, s.alpha2, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1006,32 +1011,29 @@ extern "C" hipError_t bse_launch_larft(hipStream_t st, int nb, const double2* ta
 // Hemv variant code for trailing size nt: ch + 32 u2 (chunk length ch in
 // {4, 8, 16}, u2 = two row blocks per step), from a table indexed by
 // floor(log2(nt)) (autotuned per device); 0 entries mean the default 8.
-static inline int hemv_ch_for(const signed char* table, int nt) {
+static inline int hemv_ch_for(const unsigned char* table, int nt) {
     return hemv_ch_table_lookup(table, nt);
 }
 
+// Number of storage tile rows of the trailing block A[s0:, s0:].
+static inline int hemv_nbk(int n, int s0) { return tile_count(n) - (s0 >> 6); }
+
 template <typename T>
-static void hemv_launch(hipStream_t st, int code, int nt, const T* A, int64_t lda, const double2* v,
+static void hemv_launch(hipStream_t st, int code, int n, int s0, const T* A, const double2* v,
                         double2* Yr, double2* Yc)
 {
-    const int ch = code & 31, u2 = code >> 5;
-    const int nbk = (int)ceil_div64(nt, 64);
+    const int ch = code & 31;
+    const int nbk = hemv_nbk(n, s0);
     const int nq = (int)ceil_div64(nbk, ch);
-#define HL(CC, UU) hipLaunchKernelGGL((k_hemv_lower<T, CC, UU>), dim3(nq, nbk), dim3(256), 0, st, nt, A, lda, v, Yr, Yc, nbk, nq)
-    if (u2 == 2) {
-        if (ch == 4) HL(4, 2); else if (ch == 16) HL(16, 2); else HL(8, 2);
-    } else if (u2) {
-        if (ch == 4) HL(4, 1); else if (ch == 16) HL(16, 1); else HL(8, 1);
-    } else {
-        if (ch == 4) HL(4, 0); else if (ch == 16) HL(16, 0); else HL(8, 0);
-    }
+#define HL(CC, UU, AX) hipLaunchKernelGGL((k_hemv_lower<T, CC, UU, AX>), dim3(nq, nbk), dim3(256), 0, st, n, s0, A, v, Yr, Yc, nbk, nq)
+    HEMV_DISPATCH(code, HL)
 #undef HL
 }
 
 struct HetrdWork {
     int her2k_valu;      // 1: FP64 VALU trailing update, 0: FP64 MFMA
     hipEvent_t* prof_ev; // optional: 2 events per column around the hemv launch
-    const signed char* ch_table;  // hemv chunk length per log2(nt) bucket
+    const unsigned char* ch_table;  // hemv variant code per log2(nt) bucket
     double2* Vstore;     // packed panel reflectors
     int64_t* voff;       // host array: offset of panel p in Vstore
     double2* Tall;       // compact-WY factors, nb x nb per panel
@@ -1048,12 +1050,13 @@ struct HetrdWork {
 };
 
 template <typename T>
-static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, double* d, double* e,
+static hipError_t hetrd_run(hipStream_t st, int n, T* A, int nb, double* d, double* e,
                             double2* tau, const HetrdWork& w)
 {
     hipError_t err;
+    const int ntile = tile_count(n);
     if (n == 1) {
-        hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, (const void*)A, lda,
+        hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, (const void*)A,
                            (int)(sizeof(T) == sizeof(float2)), d);
         return hipGetLastError();
     }
@@ -1062,7 +1065,7 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
         const int nbp = min(nb, n - 1 - j0);
         const int m = n - j0;
         HetrdCol h;
-        h.n = n; h.lda = lda; h.j0 = j0; h.m = m;
+        h.n = n; h.j0 = j0; h.m = m;
         h.Vp = w.Vstore + w.voff[p];
         h.Wp = w.Wp;
         h.alpha2 = w.alpha2;
@@ -1074,21 +1077,20 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
             const int L = m - i;
             const int c = j0 + i;
             const int nt = n - c - 1;
-            const int prev_gx = (i > 0) ? (int)ceil_div64(n - c, 64) : 0;
+            const int prev_gx = (i > 0) ? ntile - (c >> 6) : 0;
             hipLaunchKernelGGL(k_colupd<T>, dim3((unsigned)ceil_div64(L, 256)), dim3(256), 0, st, h,
                                (const T*)A, prev_gx);
             hipLaunchKernelGGL(k_makev, dim3((unsigned)ceil_div64(m, MAKEV_ROWS), (unsigned)max(i, 1)),
                                dim3(256), 0, st, h, (int)ceil_div64(L, 256));
             const int code = hemv_ch_for(w.ch_table, nt);
             const int ch = code & 31;
-            const int nbk = (int)ceil_div64(nt, 64);
+            const int nbk = hemv_nbk(n, c + 1);
             const int nq = (int)ceil_div64(nbk, ch);
-            const T* A22 = A + (c + 1) + (int64_t)(c + 1) * lda;
             const double2* vtr = h.Vp + (i + 1) + (int64_t)i * m;
             if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c], st);
-            hemv_launch<T>(st, code, nt, A22, lda, vtr, w.Yr, w.Yc);
+            hemv_launch<T>(st, code, n, c + 1, (const T*)A, vtr, w.Yr, w.Yc);
             if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c + 1], st);
-            hipLaunchKernelGGL(k_hemv_reduce, dim3((unsigned)nbk), dim3(1024), 0, st, h, nt,
+            hipLaunchKernelGGL(k_hemv_reduce, dim3((unsigned)nbk), dim3(1024), 0, st, h,
                                (const double2*)w.Yr, (const double2*)w.Yc, nbk, nq, ch,
                                (int)ceil_div64(m, MAKEV_ROWS));
             if ((err = hipGetLastError()) != hipSuccess) return err;
@@ -1097,25 +1099,24 @@ static hipError_t hetrd_run(hipStream_t st, int n, T* A, int64_t lda, int nb, do
         // from G(l, i) = V(:, l)^H v_i
         h.i = nbp - 1;
         hipLaunchKernelGGL(k_panel_close, dim3(1), dim3(64), 0, st, h, nbp,
-                           (int)ceil_div64(n - (j0 + nbp - 1) - 1, 64), w.Tall + (size_t)p * nb * nb, nb);
+                           hemv_nbk(n, j0 + nbp), w.Tall + (size_t)p * nb * nb, nb);
         if ((err = hipGetLastError()) != hipSuccess) return err;
-        const int n2 = m - nbp;
-        if (n2 > 0) {
-            const int64_t nt2 = ceil_div64(n2, 64);
+        const int s2 = j0 + nbp;
+        if (s2 < n) {
+            const int64_t nT = ntile - (s2 >> 6);
+            const dim3 g((unsigned)(nT * (nT + 1) / 2));
             if (w.her2k_valu)
-                hipLaunchKernelGGL(k_her2k_lower<T>, dim3((unsigned)(nt2 * (nt2 + 1) / 2)), dim3(256),
-                                   0, st, n2, A + (j0 + nbp) + (int64_t)(j0 + nbp) * lda, lda,
-                                   (const double2*)(h.Vp + nbp), (const double2*)(w.Wp + nbp),
-                                   (int64_t)m, nbp, (const double2*)w.alpha2);
+                hipLaunchKernelGGL(k_her2k_lower<T>, g, dim3(256), 0, st, n, s2, A,
+                                   (const double2*)h.Vp, (const double2*)w.Wp, (int64_t)m, j0, nbp,
+                                   (const double2*)w.alpha2);
             else
-                hipLaunchKernelGGL(k_her2k_mfma<T>, dim3((unsigned)(nt2 * (nt2 + 1) / 2)), dim3(256),
-                                   0, st, n2, A + (j0 + nbp) + (int64_t)(j0 + nbp) * lda, lda,
-                                   (const double2*)(h.Vp + nbp), (const double2*)(w.Wp + nbp),
-                                   (int64_t)m, nbp, (const double2*)w.alpha2);
+                hipLaunchKernelGGL(k_her2k_mfma<T>, g, dim3(256), 0, st, n, s2, A,
+                                   (const double2*)h.Vp, (const double2*)w.Wp, (int64_t)m, j0, nbp,
+                                   (const double2*)w.alpha2);
         }
         if ((err = hipGetLastError()) != hipSuccess) return err;
     }
-    hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, (const void*)A, lda,
+    hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, (const void*)A,
                        (int)(sizeof(T) == sizeof(float2)), d);
     return hipGetLastError();
 }
@@ -1134,13 +1135,14 @@ extern "C" void bse_hetrd_workspace(int n, int nb, int64_t* vstore_elems, int64_
     *part_doubles = ::part_doubles(n);
 }
 
-// nb <= 64.  Tall receives one nb x nb compact-WY factor per panel.
-extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int64_t lda, int nb,
+// nb <= 64, A in tiled lower storage.  Tall receives one nb x nb compact-WY
+// factor per panel.
+extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int nb,
                                 double* d, double* e, double2* tau, double2* Vstore,
                                 const int64_t* voff, double2* Tall, double2* Wp, double2* alpha2,
                                 double2* G, double2* acol, double2* Yr, double2* Yc, double2* t12,
                                 double* part, double* scal, unsigned int* counter,
-                                hipEvent_t* prof_ev, const signed char* ch_table, int her2k_valu)
+                                hipEvent_t* prof_ev, const unsigned char* ch_table, int her2k_valu)
 {
     HetrdWork w;
     w.her2k_valu = her2k_valu;
@@ -1149,25 +1151,24 @@ extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int64_
     w.Vstore = Vstore; w.voff = (int64_t*)voff; w.Tall = Tall; w.Wp = Wp; w.alpha2 = alpha2;
     w.G = G; w.acol = acol; w.Yr = Yr; w.Yc = Yc; w.t12 = t12; w.part = part; w.scal = scal;
     w.counter = counter;
-    if (fp32) return hetrd_run<float2>(st, n, (float2*)A, lda, nb, d, e, tau, w);
-    return hetrd_run<double2>(st, n, (double2*)A, lda, nb, d, e, tau, w);
+    if (fp32) return hetrd_run<float2>(st, n, (float2*)A, nb, d, e, tau, w);
+    return hetrd_run<double2>(st, n, (double2*)A, nb, d, e, tau, w);
 }
 
-// One hemv launch (autotuner timing): y partials of A (nt x nt, lower) v.
-extern "C" hipError_t bse_launch_hemv(hipStream_t st, int fp32, int code, int nt, const void* A,
-                                      int64_t lda, const double2* v, double2* Yr, double2* Yc)
+// One hemv launch (autotuner timing): y partials of A[s0:, s0:] v for A in
+// tiled lower storage of size n (v indexed from row s0).
+extern "C" hipError_t bse_launch_hemv(hipStream_t st, int fp32, int code, int n, int s0,
+                                      const void* A, const double2* v, double2* Yr, double2* Yc)
 {
-    const int ch = code & 31, u2 = code >> 5;
-    if (nt <= 0 || (ch != 4 && ch != 8 && ch != 16) || u2 > 2) return hipErrorInvalidValue;
-    if (fp32) hemv_launch<float2>(st, code, nt, (const float2*)A, lda, v, Yr, Yc);
-    else hemv_launch<double2>(st, code, nt, (const double2*)A, lda, v, Yr, Yc);
+    if (s0 < 0 || s0 >= n || !hemv_code_valid(code)) return hipErrorInvalidValue;
+    if (fp32) hemv_launch<float2>(st, code, n, s0, (const float2*)A, v, Yr, Yc);
+    else hemv_launch<double2>(st, code, n, s0, (const double2*)A, v, Yr, Yc);
     return hipGetLastError();
 }
 
-extern "C" hipError_t bse_launch_last_diag(hipStream_t st, int fp32, int n, const void* A,
-                                           int64_t lda, double* d)
+extern "C" hipError_t bse_launch_last_diag(hipStream_t st, int fp32, int n, const void* A, double* d)
 {
-    hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, A, lda, fp32, d);
+    hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, A, fp32, d);
     return hipGetLastError();
 }
 
@@ -1176,11 +1177,12 @@ extern "C" hipError_t bse_launch_last_diag(hipStream_t st, int fp32, int n, cons
 extern "C" hipError_t bse_hetrd_batch_column(hipStream_t st, int fp32, int c, int nb, int nact,
                                              int n0, const void* d_sys, int code)
 {
-    const int ch = code & 31, u2 = code >> 5;
+    if (!hemv_code_valid(code)) return hipErrorInvalidValue;
+    const int ch = code & 31;
     if (nact <= 0 || c >= n0 - 1) return hipSuccess;
     const HetrdSys* sys = (const HetrdSys*)d_sys;
-    const int j0 = (c / nb) * nb, i = c - j0, m = n0 - j0, nt = n0 - c - 1;
-    const int nbk = (int)ceil_div64(nt, 64);
+    const int j0 = (c / nb) * nb, i = c - j0, m = n0 - j0;
+    const int nbk = hemv_nbk(n0, c + 1);
     const unsigned z = (unsigned)nact;
     if (fp32)
         hipLaunchKernelGGL(kb_colupd<float2>, dim3((unsigned)ceil_div64(m - i, 256), 1, z), dim3(256), 0, st, sys, c, nb);
@@ -1189,17 +1191,11 @@ extern "C" hipError_t bse_hetrd_batch_column(hipStream_t st, int fp32, int c, in
     hipLaunchKernelGGL(kb_makev, dim3((unsigned)ceil_div64(m, MAKEV_ROWS), (unsigned)max(i, 1), z),
                        dim3(256), 0, st, sys, c, nb);
     const dim3 hg((unsigned)ceil_div64(nbk, ch), (unsigned)nbk, z);
-#define KBH(TT, CC, UU) hipLaunchKernelGGL((kb_hemv<TT, CC, UU>), hg, dim3(256), 0, st, sys, c, nb)
-    if (fp32) {
-        if (u2 == 2) { if (ch == 4) KBH(float2, 4, 2); else if (ch == 16) KBH(float2, 16, 2); else KBH(float2, 8, 2); }
-        else if (u2) { if (ch == 4) KBH(float2, 4, 1); else if (ch == 16) KBH(float2, 16, 1); else KBH(float2, 8, 1); }
-        else { if (ch == 4) KBH(float2, 4, 0); else if (ch == 16) KBH(float2, 16, 0); else KBH(float2, 8, 0); }
-    } else {
-        if (u2 == 2) { if (ch == 4) KBH(double2, 4, 2); else if (ch == 16) KBH(double2, 16, 2); else KBH(double2, 8, 2); }
-        else if (u2) { if (ch == 4) KBH(double2, 4, 1); else if (ch == 16) KBH(double2, 16, 1); else KBH(double2, 8, 1); }
-        else { if (ch == 4) KBH(double2, 4, 0); else if (ch == 16) KBH(double2, 16, 0); else KBH(double2, 8, 0); }
-    }
-#undef KBH
+#define KBF(CC, UU, AX) hipLaunchKernelGGL((kb_hemv<float2, CC, UU, AX>), hg, dim3(256), 0, st, sys, c, nb)
+#define KBD(CC, UU, AX) hipLaunchKernelGGL((kb_hemv<double2, CC, UU, AX>), hg, dim3(256), 0, st, sys, c, nb)
+    if (fp32) HEMV_DISPATCH(code, KBF) else HEMV_DISPATCH(code, KBD)
+#undef KBF
+#undef KBD
     hipLaunchKernelGGL(kb_reduce, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch);
     return hipGetLastError();
 }
@@ -1227,20 +1223,22 @@ extern "C" hipError_t bse_hetrd_batch_panel(hipStream_t st, int fp32, int c, int
     return hipGetLastError();
 }
 
-// One trailing-update launch on an n2 x n2 lower block (autotuner timing).
-extern "C" hipError_t bse_launch_her2k(hipStream_t st, int fp32, int valu, int n2, void* A,
-                                       int64_t lda, const double2* V, const double2* W,
-                                       int64_t ldv, int nb, const double2* alpha2)
+// One trailing-update launch on A[s:, s:] of a tiled lower matrix of size n
+// (autotuner timing); V, W start at global row j0 with leading dimension ldv.
+extern "C" hipError_t bse_launch_her2k(hipStream_t st, int fp32, int valu, int n, int s, void* A,
+                                       const double2* V, const double2* W, int64_t ldv, int j0,
+                                       int nb, const double2* alpha2)
 {
-    if (n2 <= 0) return hipSuccess;
-    const int64_t nt2 = ceil_div64(n2, 64);
-    const dim3 g((unsigned)(nt2 * (nt2 + 1) / 2));
+    if (s >= n) return hipSuccess;
+    if (s < j0 || ldv < n - j0) return hipErrorInvalidValue;
+    const int64_t nT = tile_count(n) - (s >> 6);
+    const dim3 g((unsigned)(nT * (nT + 1) / 2));
     if (fp32) {
-        if (valu) hipLaunchKernelGGL(k_her2k_lower<float2>, g, dim3(256), 0, st, n2, (float2*)A, lda, V, W, ldv, nb, alpha2);
-        else hipLaunchKernelGGL(k_her2k_mfma<float2>, g, dim3(256), 0, st, n2, (float2*)A, lda, V, W, ldv, nb, alpha2);
+        if (valu) hipLaunchKernelGGL(k_her2k_lower<float2>, g, dim3(256), 0, st, n, s, (float2*)A, V, W, ldv, j0, nb, alpha2);
+        else hipLaunchKernelGGL(k_her2k_mfma<float2>, g, dim3(256), 0, st, n, s, (float2*)A, V, W, ldv, j0, nb, alpha2);
     } else {
-        if (valu) hipLaunchKernelGGL(k_her2k_lower<double2>, g, dim3(256), 0, st, n2, (double2*)A, lda, V, W, ldv, nb, alpha2);
-        else hipLaunchKernelGGL(k_her2k_mfma<double2>, g, dim3(256), 0, st, n2, (double2*)A, lda, V, W, ldv, nb, alpha2);
+        if (valu) hipLaunchKernelGGL(k_her2k_lower<double2>, g, dim3(256), 0, st, n, s, (double2*)A, V, W, ldv, j0, nb, alpha2);
+        else hipLaunchKernelGGL(k_her2k_mfma<double2>, g, dim3(256), 0, st, n, s, (double2*)A, V, W, ldv, j0, nb, alpha2);
     }
     return hipGetLastError();
 }

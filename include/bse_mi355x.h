@@ -218,7 +218,9 @@ int bse_batch_run(bse_ctx* ctx, double* total_ms, double* system_ms);
 int bse_batch_results(bse_ctx* ctx, double* evals, double* eps_re, double* eps_im);
 /* info[0] graph nodes, [1] device arena bytes, [2] largest N, [3] systems,
  * [4] hemv variant chosen for the largest trailing size (chunk length in
- *     block rows, plus 32 for the two-row-blocks-per-step form),
+ *     tile rows; plus 32 for the two-row-blocks-per-step form or 64 for the
+ *     per-slab column-sum form; plus 128 for default-policy instead of
+ *     non-temporal matrix loads),
  * [5] trailing-update kernel (0 MFMA, 1 VALU).                             */
 int bse_batch_info(const bse_ctx* ctx, int64_t info[6]);
 
