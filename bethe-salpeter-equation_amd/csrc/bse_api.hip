@@ -401,9 +401,10 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
                             std::min<int64_t>(vs_el, (int64_t)n * nb), alpha2);
         if (rc) return rc;
     }
-    HCK(bse_hetrd(st, fp32, n, H, nb, d, e, tauh, Vst, voff.data(), Tall, Wp, alpha2, Gram, acol,
-                  Yr, Yc, t12, part, scal, counter, pev, ctx->hemv_ch[fp32],
-                  ctx->her2k_valu[fp32] > 0 ? 1 : 0));
+    double* hsys = BUF(double, S_HSYS, 32);   // one HetrdSys (device copy)
+    HCK(bse_hetrd(st, fp32, n, H, nb, d, e, tauh, Vst, Tall, Wp, alpha2, Gram, acol, Yr, Yc, t12,
+                  part, scal, counter, pev, ctx->hemv_ch[fp32], ctx->her2k_valu[fp32] > 0 ? 1 : 0,
+                  hsys));
     if (pev) {
         HCK(hipStreamSynchronize(st));
         double ms = 0.0, bytes = 0.0;

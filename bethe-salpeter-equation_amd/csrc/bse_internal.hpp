@@ -33,10 +33,9 @@ hipError_t bse_launch_expand(hipStream_t, int, int64_t, int, int, const void*, v
 hipError_t bse_launch_to_tiles(hipStream_t, int, int64_t, const double2*, int64_t, void*);
 hipError_t bse_launch_tile_pad_zero(hipStream_t, int, int64_t, void*);
 void bse_hetrd_workspace(int, int, int64_t*, int64_t*, int64_t*, int64_t*);
-hipError_t bse_hetrd(hipStream_t, int, int, void*, int, double*, double*, double2*,
-                     double2*, const int64_t*, double2*, double2*, double2*, double2*, double2*,
-                     double2*, double2*, double2*, double*, double*, unsigned int*, hipEvent_t*,
-                     const unsigned char*, int);
+hipError_t bse_hetrd(hipStream_t, int, int, void*, int, double*, double*, double2*, double2*,
+                     double2*, double2*, double2*, double2*, double2*, double2*, double2*, double2*,
+                     double*, double*, unsigned int*, hipEvent_t*, const unsigned char*, int, void*);
 hipError_t bse_launch_hemv(hipStream_t, int, int, int, int, const void*, const double2*, double2*,
                            double2*);
 hipError_t bse_launch_last_diag(hipStream_t, int, int, const void*, double*);
@@ -159,6 +158,7 @@ namespace {
 enum Slot {
     S_RCART, S_HR, S_TAU, S_KPTS, S_GVEC, S_Q, S_EPS, S_COEF, S_WTAB, S_VX, S_P, S_PV, S_RHO, S_DE,
     S_H, S_D, S_E, S_TAUH, S_VSTORE, S_WP, S_ACOL, S_YR, S_YC, S_T12, S_PART, S_SCAL, S_COUNTER,
+    S_HSYS,
     S_TALL, S_GRAM, S_E2, S_ITEMS, S_BOUNDS, S_LAMI, S_LAM, S_BSV, S_NBV, S_CLUST, S_IISCR, S_IIX,
     S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_ALPHA2,
     S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL,

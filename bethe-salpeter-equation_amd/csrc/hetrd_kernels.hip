@@ -28,11 +28,17 @@
 // Per column c = j0 + i the kernels are
 //   colupd(i):  alpha2[i-1] from the reduce(i-1) partials; column update;
 //               norm partials                                  -> region A
-//   makev(i):   reflector scalars from region A; v_i; partial W^H v, V^H v
-//                                                              -> region B
-//   hemv(i):    y partials                                     -> Yr, Yc
-//   reduce(i):  W^H v, V^H v from region B; W(:, i); partial y^H v
-//                                                              -> region C
+//   hemv(i):    partials of A22 x (x: the updated column below its first
+//               subdiagonal entry, before scaling)             -> Yr, Yc;
+//               the first workgroup of each tile column also forms partial
+//               W^H x, V^H x                                   -> region B
+//   reduce(i):  reflector scalars from region A; v_i = e_1 + sc x (stored);
+//               A22 v = sc (A22 x) + A22 e_1; W^H v, V^H v from region B;
+//               W(:, i); partial y^H v                         -> region C
+// The reflector's scale sc = 1/(alpha - beta) depends on the column norm (a
+// grid-wide sum), so the product is formed with the unscaled column and
+// rescaled afterwards: the hemv needs no scalars and starts streaming at
+// once.
 // and at the panel end panel_close (alpha2 of the last column from region C,
 // compact-WY factor) before the trailing update.
 #include "bse_common.hpp"
@@ -62,8 +68,8 @@ struct PartLayout {
 };
 __host__ __device__ inline PartLayout part_layout(int n) {
     PartLayout L;
-    const int64_t na = ceil_div64(n, 256) + 1;              // colupd blocks
-    const int64_t nb = 4 * 64 * (ceil_div64(n, 1024) + 1);  // 2 dots x complex x l x makev row blocks
+    const int64_t na = ceil_div64(n, 64) + 1;               // colupd blocks
+    const int64_t nb = 4 * 32 * (ceil_div64(n, 64) + 1);    // 2 dots x complex x l < 32 x tile columns
     L.a = 0;
     L.b = na;
     L.c = na + nb;                                          // + 2 (ceil(n/64) + 1): reduce blocks
@@ -72,6 +78,10 @@ __host__ __device__ inline PartLayout part_layout(int n) {
 __host__ __device__ inline int64_t part_doubles(int n) {
     return part_layout(n).c + 2 * (ceil_div64(n, 64) + 1);
 }
+
+// Workgroups of the column update of column c = j0 + i (64 rows each): the
+// number of norm partials in region A.
+__host__ __device__ inline int colupd_blocks(int m, int i) { return (int)ceil_div64(m - i, 64); }
 
 // W(r, l) of the panel, with the alpha2 v term applied on the fly.
 __device__ inline double2 w_at(const HetrdCol& h, int64_t r, int l) {
@@ -94,21 +104,41 @@ __device__ inline double2 block_sum_partials(const double2* __restrict__ p, int 
 // (a): column update and its norm partials.  alpha2 of the previous column
 // (i - 1) is finished here from the reduce partials (prev_gx of them) by every
 // block; block 0 stores it and clears W(0:i, i) and alpha2[i] (rows above the
-// reflector are never read, but the whole column is kept defined).
+// reflector are never read, but the whole column is kept defined).  A block
+// covers 64 rows; its four waves split the panel columns (l = w + 4k) so that
+// each lane issues all of its loads at once, before the partial alpha2 sum.
+#define COLUPD_ROWS 64
 template <typename T>
-__device__ __attribute__((always_inline)) inline void colupd_body(const HetrdCol& h, const T* __restrict__ A, int bx, int gx,
+__device__ __attribute__((always_inline)) inline void colupd_body(const HetrdCol& h, const T* __restrict__ A, int bx,
                                                                    int prev_gx)
 {
     __shared__ double sh[32];
-    __shared__ double2 s_wc[64], s_vc[64], s_a2[64];
+    __shared__ double2 s_wc[32], s_vc[32], s_a2[32];
+    __shared__ double2 s_part[4][COLUPD_ROWS];
     const int c = h.j0 + h.i;
     const int L = h.m - h.i;                 // rows c .. n-1
-    const int t = bx * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int t = bx * COLUPD_ROWS + lane;
+    const bool row_ok = t < L;
+    const int64_t lrow = h.i + t;
     const PartLayout pl = part_layout(h.n);
+    double2 vr[8], wq[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int l = grp + 4 * k;
+        vr[k] = cmk(0, 0);
+        wq[k] = cmk(0, 0);
+        if (row_ok && l < h.i) {
+            vr[k] = h.Vp[lrow + (int64_t)l * h.m];
+            wq[k] = h.Wp[lrow + (int64_t)l * h.m];
+        }
+    }
+    double2 a = cmk(0, 0);
+    if (grp == 0 && row_ok) a = cplx<T>::load(A + tix(c + t, c, tile_count(h.n)));
     double2 a2prev = cmk(0, 0);
     if (h.i > 0) {
-        const double2 s = block_sum_partials((const double2*)(h.part + pl.c), prev_gx, sh);
-        a2prev = cscale(cmul(h.tau[c - 1], s), -0.5);
+        const double2 sp = block_sum_partials((const double2*)(h.part + pl.c), prev_gx, sh);
+        a2prev = cscale(cmul(h.tau[c - 1], sp), -0.5);
     }
     if (bx == 0) {
         if ((int)threadIdx.x <= h.i && (int)threadIdx.x < h.m)
@@ -128,32 +158,20 @@ __device__ __attribute__((always_inline)) inline void colupd_body(const HetrdCol
         s_a2[l] = a2;
     }
     __syncthreads();
+    double2 acc = cmk(0, 0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int l = grp + 4 * k;
+        if (l < h.i) {
+            const double2 wr = cadd(wq[k], cmul(s_a2[l], vr[k]));
+            acc = cadd(acc, cadd(cmul(vr[k], cconj(s_wc[l])), cmul(wr, cconj(s_vc[l]))));
+        }
+    }
+    s_part[grp][lane] = acc;
+    __syncthreads();
     double nrm = 0.0;
-    if (t < L) {
-        const int lrow = h.i + t;
-        double2 a = cplx<T>::load(A + tix(c + t, c, tile_count(h.n)));
-        const double2* vp = h.Vp + lrow;
-        const double2* wp = h.Wp + lrow;
-        int l = 0;
-        // eight columns per step: sixteen independent loads before any use
-        for (; l + 8 <= h.i; l += 8) {
-            double2 vr[8], wq[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                vr[u] = vp[(int64_t)(l + u) * h.m];
-                wq[u] = wp[(int64_t)(l + u) * h.m];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const double2 wr = cadd(wq[u], cmul(s_a2[l + u], vr[u]));
-                a = csub(a, cadd(cmul(vr[u], cconj(s_wc[l + u])), cmul(wr, cconj(s_vc[l + u]))));
-            }
-        }
-        for (; l < h.i; ++l) {
-            const double2 vr = vp[(int64_t)l * h.m];
-            const double2 wr = cadd(wp[(int64_t)l * h.m], cmul(s_a2[l], vr));
-            a = csub(a, cadd(cmul(vr, cconj(s_wc[l])), cmul(wr, cconj(s_vc[l]))));
-        }
+    if (grp == 0 && row_ok) {
+        a = csub(a, cadd(cadd(s_part[0][lane], s_part[1][lane]), cadd(s_part[2][lane], s_part[3][lane])));
         if (t == 0) a.y = 0.0;
         h.acol[t] = a;
         if (t >= 2) nrm = cabs2(a);
@@ -191,60 +209,6 @@ __device__ inline Refl reflector_scalars(const HetrdCol& h, int colupd_gx, doubl
     return R;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_colupd(HetrdCol h, const T* __restrict__ A, int prev_gx)
-{
-    colupd_body<T>(h, A, blockIdx.x, gridDim.x, prev_gx);
-}
-
-// Reflector vector into Vp(:, i), plus partial t1 = W^H v and t2 = V^H v over
-// the panel's earlier columns l < i (rows i+1 .. m-1).  Grid (row blocks, l):
-// block (rb, l) covers MAKEV_ROWS rows for one column l, so the dot products
-// of all l run concurrently.  Every block first forms the reflector scalars
-// from the colupd partials; block (0, 0) stores d, e, tau and the scale, and
-// the l = 0 blocks write the reflector.
-#define MAKEV_RPT 4
-#define MAKEV_ROWS (256 * MAKEV_RPT)
-__device__ __attribute__((always_inline)) inline void makev_body(const HetrdCol& h, int rb, int l, int nrb, int colupd_gx)
-{
-    __shared__ double sh[32];
-    const int c = h.j0 + h.i;
-    const Refl R = reflector_scalars(h, colupd_gx, sh);
-    if (rb == 0 && l == 0 && threadIdx.x == 0) {
-        h.d[c] = R.d;
-        h.e[c] = R.e;
-        h.tau[c] = R.tau;
-        h.scal[0] = R.sc.x; h.scal[1] = R.sc.y; h.scal[2] = R.tau.x; h.scal[3] = R.tau.y;
-    }
-    double2 p1 = cmk(0, 0), p2 = cmk(0, 0);
-#pragma unroll
-    for (int j = 0; j < MAKEV_RPT; ++j) {
-        const int r = rb * MAKEV_ROWS + threadIdx.x + 256 * j;
-        if (r >= h.m) continue;
-        double2 vv = cmk(0, 0);
-        if (r == h.i + 1) vv = cmk(1, 0);
-        else if (r > h.i + 1) vv = cmul(h.acol[r - h.i], R.sc);
-        if (l == 0) h.Vp[r + (int64_t)h.i * h.m] = vv;
-        if (l < h.i && r > h.i) {
-            cfmac(p1, w_at(h, r, l), vv);
-            cfmac(p2, h.Vp[r + (int64_t)l * h.m], vv);
-        }
-    }
-    if (h.i > 0) {
-        p1 = block_sum2(p1, sh);
-        p2 = block_sum2(p2, sh);
-        if (threadIdx.x == 0) {
-            double2* pp = (double2*)(h.part + part_layout(h.n).b);
-            pp[((int64_t)l * nrb + rb) * 2 + 0] = p1;
-            pp[((int64_t)l * nrb + rb) * 2 + 1] = p2;
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_makev(HetrdCol h, int colupd_gx)
-{
-    makev_body(h, blockIdx.x, blockIdx.y, gridDim.x, colupd_gx);
-}
 
 // (c): y = A22 v for the trailing Hermitian block A22 = A[s0:, s0:] (s0 =
 // c + 1), over the 64 x 64 storage tiles that intersect it: tile column
@@ -277,132 +241,6 @@ __host__ __device__ inline bool hemv_code_valid(int code) {
     return code >= 0 && code < 256 && (ch == 4 || ch == 8 || ch == 16) && u2 <= 2;
 }
 
-__device__ inline double2 hemv_v(const double2* __restrict__ v, int g, int s0, int n) {
-    return (g >= s0 && g < n) ? v[g - s0] : cmk(0, 0);
-}
-
-// Re a(row, row) of a diagonal tile from this wave's 16 loaded columns (zero
-// in lanes whose diagonal column belongs to another wave).
-template <typename T>
-__device__ __attribute__((always_inline)) inline double hemv_diag_re(const T (&a)[16], int lane, int w) {
-    double d = 0.0;
-#pragma unroll
-    for (int jj = 0; jj < 16; ++jj) d = (w * 16 + jj == lane) ? (double)a[jj].x : d;
-    return d;
-}
-
-template <typename T, int CH, int U2, int AUX>
-__device__ __attribute__((always_inline)) inline void hemv_body(int n, int s0, const T* __restrict__ A,
-                                                                 const double2* __restrict__ v,
-                                                                 double2* __restrict__ Yr,
-                                                                 double2* __restrict__ Yc, int nbk,
-                                                                 int nq, int q, int bj)
-{
-    const int bi0 = bj + q * CH;
-    if (bi0 >= nbk) return;
-    const int bi1 = min(nbk, bi0 + CH);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int ntile = tile_count(n), tc = s0 >> 6, TJ = tc + bj;
-    __shared__ double2 s_v[64];
-    __shared__ double2 s_part[4][CH][64];
-    if (threadIdx.x < 64) s_v[threadIdx.x] = hemv_v(v, TJ * 64 + threadIdx.x, s0, n);
-    __syncthreads();
-    const double2* sv = s_v + w * 16;         // this wave's 16 entries of v (LDS)
-    double2 cacc[16];
-#pragma unroll
-    for (int jj = 0; jj < 16; ++jj) cacc[jj] = cmk(0, 0);
-    // the chunk's tiles are one contiguous run starting at tile (tc+bi0, TJ);
-    // descriptor base = that run + this wave's first column, lane offset
-    // (tile, row) and column offset jj * 64 elements in a scalar register
-    const int wu = __builtin_amdgcn_readfirstlane(w);
-    const T* run = A + tile_pos(tc + bi0, TJ, ntile) * (BSE_TILE * BSE_TILE);
-    const __amdgpu_buffer_rsrc_t rs = bse_rsrc(run + wu * 16 * 64, 0x7fffffff);
-    const int cstride = 64 * (int)sizeof(T);
-    for (int bi = bi0; bi < bi1;) {
-        const int TI = tc + bi;
-        const bool diag = TI == TJ;
-        if (U2 && sizeof(T) == 8 && !diag && bi + 1 < bi1) {
-            // FP32 storage, two off-diagonal tiles: lane l reads the
-            // consecutive rows 2l, 2l+1 of the 128-row slab (tile TI for
-            // l < 32, TI+1 above) in one 16-byte load
-            const int hb = lane >> 5, rl = (2 * lane) & 63;
-            const int vo2 = ((bi - bi0 + hb) * 4096 + rl) * (int)sizeof(T);
-            bse_i4v qv[16];
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) qv[jj] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo2, jj * cstride, AUX);
-            const int r2 = (TI + hb) * 64 + rl;
-            const double2 v0 = hemv_v(v, r2, s0, n), v1 = hemv_v(v, r2 + 1, s0, n);
-            double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                const int q0 = qv[jj][0], q1 = qv[jj][1], q2 = qv[jj][2], q3 = qv[jj][3];
-                const double2 x = cmk((double)bse_i2f(q0), (double)bse_i2f(q1));
-                const double2 y = cmk((double)bse_i2f(q2), (double)bse_i2f(q3));
-                cfma(r0, x, sv[jj]);
-                cfmac(cacc[jj], x, v0);
-                cfma(r1, y, sv[jj]);
-                cfmac(cacc[jj], y, v1);
-            }
-            s_part[w][bi + hb - bi0][rl] = r0;
-            s_part[w][bi + hb - bi0][rl + 1] = r1;
-            bi += 2;
-            continue;
-        }
-        const int vo = ((bi - bi0) * 4096 + lane) * (int)sizeof(T);
-        const int row = TI * 64 + lane;
-        if (U2 && sizeof(T) == 16 && !diag && bi + 1 < bi1) {
-            // two off-diagonal tiles: 32 loads in flight per lane
-            T a[16], b[16];
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                a[jj] = bufld<T>::template ld<AUX>(rs, vo, jj * cstride);
-                b[jj] = bufld<T>::template ld<AUX>(rs, vo + 4096 * (int)sizeof(T), jj * cstride);
-            }
-            const double2 vr0 = hemv_v(v, row, s0, n), vr1 = hemv_v(v, row + 64, s0, n);
-            double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
-                const double2 y = cmk((double)b[jj].x, (double)b[jj].y);
-                cfma(r0, x, sv[jj]);
-                cfmac(cacc[jj], x, vr0);
-                cfma(r1, y, sv[jj]);
-                cfmac(cacc[jj], y, vr1);
-            }
-            s_part[w][bi - bi0][lane] = r0;
-            s_part[w][bi + 1 - bi0][lane] = r1;
-            bi += 2;
-            continue;
-        }
-        const double2 vr = hemv_v(v, row, s0, n);
-        double2 racc = cmk(0, 0);
-        T a[16];
-#pragma unroll
-        for (int jj = 0; jj < 16; ++jj) a[jj] = bufld<T>::template ld<AUX>(rs, vo, jj * cstride);
-#pragma unroll
-        for (int jj = 0; jj < 16; ++jj) {
-            const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
-            cfma(racc, x, sv[jj]);
-            cfmac(cacc[jj], x, vr);
-        }
-        if (diag) racc = csub(racc, cscale(vr, hemv_diag_re(a, lane, w)));
-        s_part[w][bi - bi0][lane] = racc;
-        ++bi;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < (bi1 - bi0) * 64; e += 256) {
-        const int bb = e >> 6, ln = e & 63;
-        const double2 s = cadd(cadd(s_part[0][bb][ln], s_part[1][bb][ln]),
-                               cadd(s_part[2][bb][ln], s_part[3][bb][ln]));
-        Yr[((int64_t)(bi0 + bb) * nbk + bj) * 64 + ln] = s;
-    }
-#pragma unroll
-    for (int jj = 0; jj < 16; ++jj) {
-        const double x = wave_sum(cacc[jj].x), y = wave_sum(cacc[jj].y);
-        if (lane == 0) Yc[((int64_t)bj * nq + q) * 64 + w * 16 + jj] = cmk(x, y);
-    }
-}
-
 // Column sums of sixteen per-lane complex values over the 64 lanes of a wave
 // by reduce-scatter: after exchanges at lane distances 32, 16, 8, 4 every lane
 // keeps one column, col = 8 b5 + 4 b4 + 2 b3 + b2 (b_k = bit k of the lane),
@@ -433,6 +271,200 @@ __device__ inline int colsum16_col(int lane) {
     return 8 * ((lane >> 5) & 1) + 4 * ((lane >> 4) & 1) + 2 * ((lane >> 3) & 1) + ((lane >> 2) & 1);
 }
 
+// The vector of the product: an explicit v (indexed from s0), or the updated
+// column c = s0 - 1 below its first subdiagonal entry, x(g) = acol(g - c)
+// for g > s0 and x(s0) = 0; zero outside [s0, n).
+struct HemvVec {
+    const double2* v;
+    const double2* acol;
+    __device__ inline double2 at(int g, int s0, int n) const {
+        if (v) return (g >= s0 && g < n) ? v[g - s0] : cmk(0, 0);
+        return (g > s0 && g < n) ? acol[g - s0 + 1] : cmk(0, 0);
+    }
+};
+
+// Reflector scalars of column c = j0 + i from the colupd norm partials,
+// formed by one workgroup of the hemv (which does not need them itself):
+// d, e, tau, and scal = (sc, tau) for the reduce.
+__device__ inline void hemv_scalars(const HetrdCol& h)
+{
+    __shared__ double sh[32];
+    const Refl R = reflector_scalars(h, colupd_blocks(h.m, h.i), sh);
+    if (threadIdx.x == 0) {
+        const int c = h.j0 + h.i;
+        h.d[c] = R.d;
+        h.e[c] = R.e;
+        h.tau[c] = R.tau;
+        h.scal[0] = R.sc.x; h.scal[1] = R.sc.y; h.scal[2] = R.tau.x; h.scal[3] = R.tau.y;
+    }
+}
+
+// Partial dots W(:, l)^H x and V(:, l)^H x (l < i) over the 64 rows of
+// storage tile row TJ = tc + t: region B entry ((l nbk + t) 2 + {0: W, 1: V}).
+// Done by the first nbk workgroups in dispatch order, before their own tiles,
+// so that this work never lands in the tail of the grid.  Wave w takes
+// l = w + 4k (k < 8, so i <= 32), and one cross-lane reduce-scatter sums the
+// 16 products of a lane.
+__device__ __attribute__((always_inline)) inline void hemv_dots(const HetrdCol& h, const HemvVec& vs,
+                                                                 int TJ, int bj, int nbk)
+{
+    const int c = h.j0 + h.i, s0 = c + 1, n = h.n;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = TJ * 64 + lane;
+    if (h.i == 0) return;
+    const bool ok = g > s0 && g < n;
+    const double2 vg = vs.at(g, s0, n);
+    const int64_t lr = g - h.j0;
+    double2 cc[16];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int l = w + 4 * k;
+        cc[k] = cmk(0, 0);
+        cc[8 + k] = cmk(0, 0);
+        if (ok && l < h.i) {
+            const double2 vv = h.Vp[lr + (int64_t)l * h.m];
+            const double2 ww = cadd(h.Wp[lr + (int64_t)l * h.m], cmul(h.alpha2[l], vv));
+            cfmac(cc[k], ww, vg);
+            cfmac(cc[8 + k], vv, vg);
+        }
+    }
+    const double2 sum = colsum16(cc, lane);
+    const int col = colsum16_col(lane), l = w + 4 * (col & 7);
+    if ((lane & 3) == 0 && l < h.i) {
+        double2* pp = (double2*)(h.part + part_layout(n).b);
+        pp[((int64_t)l * nbk + bj) * 2 + (col >> 3)] = sum;
+    }
+}
+
+// Re a(row, row) of a diagonal tile from this wave's 16 loaded columns (zero
+// in lanes whose diagonal column belongs to another wave).
+template <typename T>
+__device__ __attribute__((always_inline)) inline double hemv_diag_re(const T (&a)[16], int lane, int w) {
+    double d = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) d = (w * 16 + jj == lane) ? (double)a[jj].x : d;
+    return d;
+}
+
+template <typename T, int CH, int U2, int AUX>
+__device__ __attribute__((always_inline)) inline void hemv_body(int n, int s0, const T* __restrict__ A,
+                                                                 const double2* __restrict__ v,
+                                                                 double2* __restrict__ Yr,
+                                                                 double2* __restrict__ Yc, int nbk,
+                                                                 int nq, int q, int bj,
+                                                                 const HetrdCol& h)
+{
+    HemvVec vs;
+    vs.v = v;
+    vs.acol = h.acol;
+    if (!v && q == 0 && bj == 0) hemv_scalars(h);
+    if (!v && bj * nq + q < nbk) hemv_dots(h, vs, (s0 >> 6) + bj * nq + q, bj * nq + q, nbk);
+    const int bi0 = bj + q * CH;
+    if (bi0 >= nbk) return;
+    const int bi1 = min(nbk, bi0 + CH);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntile = tile_count(n), tc = s0 >> 6, TJ = tc + bj;
+    __shared__ double2 s_v[64];
+    __shared__ double2 s_part[4][CH][64];
+    if (threadIdx.x < 64) s_v[threadIdx.x] = vs.at(TJ * 64 + threadIdx.x, s0, n);
+    __syncthreads();
+    const double2* sv = s_v + w * 16;         // this wave's 16 entries of v (LDS)
+    double2 cacc[16];
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) cacc[jj] = cmk(0, 0);
+    // the chunk's tiles are one contiguous run starting at tile (tc+bi0, TJ);
+    // descriptor base = that run + this wave's first column, lane offset
+    // (tile, row) and column offset jj * 64 elements in a scalar register
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const T* run = A + tile_pos(tc + bi0, TJ, ntile) * (BSE_TILE * BSE_TILE);
+    const __amdgpu_buffer_rsrc_t rs = bse_rsrc(run + wu * 16 * 64, 0x7fffffff);
+    const int cstride = 64 * (int)sizeof(T);
+    for (int bi = bi0; bi < bi1;) {
+        const int TI = tc + bi;
+        const bool diag = TI == TJ;
+        if (U2 && sizeof(T) == 8 && !diag && bi + 1 < bi1) {
+            // FP32 storage, two off-diagonal tiles: lane l reads the
+            // consecutive rows 2l, 2l+1 of the 128-row slab (tile TI for
+            // l < 32, TI+1 above) in one 16-byte load
+            const int hb = lane >> 5, rl = (2 * lane) & 63;
+            const int vo2 = ((bi - bi0 + hb) * 4096 + rl) * (int)sizeof(T);
+            bse_i4v qv[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) qv[jj] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo2, jj * cstride, AUX);
+            const int r2 = (TI + hb) * 64 + rl;
+            const double2 v0 = vs.at(r2, s0, n), v1 = vs.at(r2 + 1, s0, n);
+            double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const int q0 = qv[jj][0], q1 = qv[jj][1], q2 = qv[jj][2], q3 = qv[jj][3];
+                const double2 x = cmk((double)bse_i2f(q0), (double)bse_i2f(q1));
+                const double2 y = cmk((double)bse_i2f(q2), (double)bse_i2f(q3));
+                cfma(r0, x, sv[jj]);
+                cfmac(cacc[jj], x, v0);
+                cfma(r1, y, sv[jj]);
+                cfmac(cacc[jj], y, v1);
+            }
+            s_part[w][bi + hb - bi0][rl] = r0;
+            s_part[w][bi + hb - bi0][rl + 1] = r1;
+            bi += 2;
+            continue;
+        }
+        const int vo = ((bi - bi0) * 4096 + lane) * (int)sizeof(T);
+        const int row = TI * 64 + lane;
+        if (U2 && sizeof(T) == 16 && !diag && bi + 1 < bi1) {
+            // two off-diagonal tiles: 32 loads in flight per lane
+            T a[16], b[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                a[jj] = bufld<T>::template ld<AUX>(rs, vo, jj * cstride);
+                b[jj] = bufld<T>::template ld<AUX>(rs, vo + 4096 * (int)sizeof(T), jj * cstride);
+            }
+            const double2 vr0 = vs.at(row, s0, n), vr1 = vs.at(row + 64, s0, n);
+            double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
+                const double2 y = cmk((double)b[jj].x, (double)b[jj].y);
+                cfma(r0, x, sv[jj]);
+                cfmac(cacc[jj], x, vr0);
+                cfma(r1, y, sv[jj]);
+                cfmac(cacc[jj], y, vr1);
+            }
+            s_part[w][bi - bi0][lane] = r0;
+            s_part[w][bi + 1 - bi0][lane] = r1;
+            bi += 2;
+            continue;
+        }
+        const double2 vr = vs.at(row, s0, n);
+        double2 racc = cmk(0, 0);
+        T a[16];
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) a[jj] = bufld<T>::template ld<AUX>(rs, vo, jj * cstride);
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
+            cfma(racc, x, sv[jj]);
+            cfmac(cacc[jj], x, vr);
+        }
+        if (diag) racc = csub(racc, cscale(vr, hemv_diag_re(a, lane, w)));
+        s_part[w][bi - bi0][lane] = racc;
+        ++bi;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < (bi1 - bi0) * 64; e += 256) {
+        const int bb = e >> 6, ln = e & 63;
+        const double2 s = cadd(cadd(s_part[0][bb][ln], s_part[1][bb][ln]),
+                               cadd(s_part[2][bb][ln], s_part[3][bb][ln]));
+        Yr[((int64_t)(bi0 + bb) * nbk + bj) * 64 + ln] = s;
+    }
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) {
+        const double x = wave_sum(cacc[jj].x), y = wave_sum(cacc[jj].y);
+        if (lane == 0) Yc[((int64_t)bj * nq + q) * 64 + w * 16 + jj] = cmk(x, y);
+    }
+}
+
+
 
 // Hemv variant with few live registers: the column partials of each slab (128
 // rows for FP32 storage read as row pairs, 64 rows otherwise) are reduced
@@ -444,8 +476,14 @@ __device__ __attribute__((always_inline)) inline void hemv_bfly_body(int n, int 
                                                                       const double2* __restrict__ v,
                                                                       double2* __restrict__ Yr,
                                                                       double2* __restrict__ Yc, int nbk,
-                                                                      int nq, int q, int bj)
+                                                                      int nq, int q, int bj,
+                                                                      const HetrdCol& h)
 {
+    HemvVec vs;
+    vs.v = v;
+    vs.acol = h.acol;
+    if (!v && q == 0 && bj == 0) hemv_scalars(h);
+    if (!v && bj * nq + q < nbk) hemv_dots(h, vs, (s0 >> 6) + bj * nq + q, bj * nq + q, nbk);
     const int bi0 = bj + q * CH;
     if (bi0 >= nbk) return;
     const int bi1 = min(nbk, bi0 + CH);
@@ -453,7 +491,7 @@ __device__ __attribute__((always_inline)) inline void hemv_bfly_body(int n, int 
     const int ntile = tile_count(n), tc = s0 >> 6, TJ = tc + bj;
     __shared__ double2 s_v[64];
     __shared__ double2 s_part[4][CH][64];
-    if (threadIdx.x < 64) s_v[threadIdx.x] = hemv_v(v, TJ * 64 + threadIdx.x, s0, n);
+    if (threadIdx.x < 64) s_v[threadIdx.x] = vs.at(TJ * 64 + threadIdx.x, s0, n);
     __syncthreads();
     const double2* sv = s_v + w * 16;
     double2 colacc = cmk(0, 0);
@@ -472,7 +510,7 @@ __device__ __attribute__((always_inline)) inline void hemv_bfly_body(int n, int 
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) qv[jj] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo2, jj * cstride, AUX);
             const int r2 = (TI + hb) * 64 + rl;
-            const double2 v0 = hemv_v(v, r2, s0, n), v1 = hemv_v(v, r2 + 1, s0, n);
+            const double2 v0 = vs.at(r2, s0, n), v1 = vs.at(r2 + 1, s0, n);
             double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) {
@@ -494,7 +532,7 @@ __device__ __attribute__((always_inline)) inline void hemv_bfly_body(int n, int 
             continue;
         }
         const int row = TI * 64 + lane;
-        const double2 vr = hemv_v(v, row, s0, n);
+        const double2 vr = vs.at(row, s0, n);
         double2 racc = cmk(0, 0);
         const int vo = ((bi - bi0) * 4096 + lane) * (int)sizeof(T);
         T a[16];
@@ -532,10 +570,10 @@ __device__ __attribute__((always_inline)) inline void hemv_bfly_body(int n, int 
 template <typename T, int CH, int U2, int AUX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HEMV_WPE_FOR(U2), 8)))
 void k_hemv_lower(int n, int s0, const T* __restrict__ A, const double2* __restrict__ v,
-                  double2* __restrict__ Yr, double2* __restrict__ Yc, int nbk, int nq)
+                  double2* __restrict__ Yr, double2* __restrict__ Yc, int nbk, int nq, HetrdCol h)
 {
-    if (U2 == 2) hemv_bfly_body<T, CH, AUX>(n, s0, A, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
-    else hemv_body<T, CH, U2, AUX>(n, s0, A, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    if (U2 == 2) hemv_bfly_body<T, CH, AUX>(n, s0, A, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y, h);
+    else hemv_body<T, CH, U2, AUX>(n, s0, A, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y, h);
 }
 
 // Instantiates M(CH, U2, AUX) for the variant of a (valid) code.
@@ -548,52 +586,42 @@ void k_hemv_lower(int n, int s0, const T* __restrict__ A, const double2* __restr
     { const int ch_ = (code) & 31, u2_ = ((code) >> 5) & 3; \
       if ((code) & 128) HEMV_DISPATCH_U2(ch_, u2_, 0, M) else HEMV_DISPATCH_U2(ch_, u2_, BSE_LOAD_STREAM, M) }
 
-// Reduce the hemv partials of one block row (64 rows) per workgroup, apply the
-// panel corrections (d) and the tau scaling (e), partial dot y^H v.  Sixteen
-// lane groups split the partial sums so that many loads are in flight.
-// t1 = W^H v and t2 = V^H v are first summed from the makev partials by every
-// block (block 0 also stores t2 as column i of the Gram matrix).  Stores y in
-// Wp(:, i) and the partial y^H v in region C; alpha2[i] = -tau/2 (y^H v) is
-// finished by the next kernel.
+// Reduce the hemv partials of one block row (64 rows) per workgroup and
+// finish the reflector step of column c = j0 + i, with sc = 1/(alpha - beta)
+// and tau from the hemv (scal, tau[c]).  t1 = W^H v and t2 = V^H v come from
+// the hemv's region-B partials of x (one per tile column; sixteen lanes per
+// sum and a fixed butterfly):
+//   t1 = conj(W(s0, :)) + sc (W^H x),   t2 = conj(V(s0, :)) + sc (V^H x)
+// (block 0 also stores t2 as column i of the Gram matrix).  Per row r >= s0:
+//   v(r) = [r == s0] + sc x(r)                  -> V(:, i)
+//   y(r) = tau (sc (A22 x)(r) + a(r, s0) - V(r, :) t1 - W(r, :) t2)
+// (the panel corrections (d) and the tau scaling (e)), stored in Wp(:, i),
+// with the partial y^H v in region C; alpha2[i] = -tau/2 (y^H v) is finished
+// by the next kernel.  Sixteen lane groups split the partial sums so that
+// many loads are in flight; every load that does not depend on t1, t2 is
+// issued before the one barrier that publishes them.
 #define RED_G 16
-#define RED_STAGE 2048        // makev partials staged in LDS (32 KB)
-__device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol& h,
+template <typename T>
+__device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol& h, const T* __restrict__ A,
                                                                    const double2* __restrict__ Yr,
                                                                    const double2* __restrict__ Yc,
-                                                                   int nbk, int nq, int ch,
-                                                                   int makev_nrb, int b)
+                                                                   int nbk, int nq, int ch, int b)
 {
     __shared__ double2 sacc[RED_G][64];
     __shared__ double2 s_t[2][64];
-    __shared__ double2 s_pp[RED_STAGE];
     __shared__ double sh[32];
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int c = h.j0 + h.i, s0 = c + 1;
     const PartLayout pl = part_layout(h.n);
-    if (h.i > 0) {
-        const double2* pp = (const double2*)(h.part + pl.b);
-        const int npp = 2 * h.i * makev_nrb;
-        const bool staged = npp <= RED_STAGE;
-        if (staged)
-            for (int e2 = threadIdx.x; e2 < npp; e2 += blockDim.x) s_pp[e2] = pp[e2];
-        __syncthreads();
-        if ((int)threadIdx.x < 2 * h.i) {
-            const int ll = threadIdx.x >> 1, which = threadIdx.x & 1;
-            double2 sum = cmk(0, 0);
-            for (int rb = 0; rb < makev_nrb; ++rb) {
-                const int64_t e2 = ((int64_t)ll * makev_nrb + rb) * 2 + which;
-                sum = cadd(sum, staged ? s_pp[e2] : pp[e2]);
-            }
-            s_t[which][ll] = sum;
-            if (b == 0 && which) h.G[ll + (int64_t)h.i * 64] = sum;
-        }
-        __syncthreads();
-    }
+    const double2 sc = cmk(h.scal[0], h.scal[1]);
+    const double2 tau = h.tau[c];
     // block b covers storage tile row (s0 / 64) + b of the trailing block
-    const int s0 = h.j0 + h.i + 1;
     const int grow = ((s0 >> 6) + b) * 64 + lane;       // matrix row
     const bool valid = grow >= s0 && grow < h.n;
     const int lr = grow - h.j0;                         // panel-local row
     double2 acc0 = cmk(0, 0), acc1 = cmk(0, 0);
+    double2 vv[2] = {cmk(0, 0), cmk(0, 0)}, ww[2] = {cmk(0, 0), cmk(0, 0)};
+    double2 a_rs0 = cmk(0, 0), xr = cmk(0, 0);
     if (valid) {
         // partials of this block row: Yr[b][0..b] and Yc[b][0..nqb), several
         // independent loads issued before any of them is summed
@@ -621,33 +649,72 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
             acc1 = cadd(acc1, cadd(x0, x1));
         }
         if (q < nqb) acc1 = cadd(acc1, yc[(int64_t)q * 64]);
-        for (int l = g; l < h.i; l += RED_G) {
-            const double2 vv = h.Vp[lr + (int64_t)l * h.m];
-            const double2 ww = w_at(h, lr, l);
-            acc0 = csub(acc0, cadd(cmul(vv, s_t[0][l]), cmul(ww, s_t[1][l])));
+        // panel rows of this row for the corrections (l = g, g + 16 < i <= 32)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int l = g + u * RED_G;
+            if (l < h.i) {
+                vv[u] = h.Vp[lr + (int64_t)l * h.m];
+                ww[u] = w_at(h, lr, l);
+            }
+        }
+        if (g == 0) {
+            // A22 e_1: column s0 of the stored lower triangle (real diagonal)
+            a_rs0 = cplx<T>::load(A + tix(grow, s0, tile_count(h.n)));
+            if (grow == s0) a_rs0.y = 0.0;
+            if (grow > s0) xr = h.acol[grow - c];
         }
     }
-    sacc[g][lane] = cadd(acc0, acc1);
+    // t1, t2 from the region-B partials
+    {
+        const int pr = threadIdx.x >> 4, k = threadIdx.x & 15;   // (l, which) = (pr / 2, pr % 2)
+        const int ll = pr >> 1, which = pr & 1;
+        double2 psum = cmk(0, 0), e1 = cmk(0, 0);
+        if (ll < h.i) {
+            const double2* pp = (const double2*)(h.part + pl.b);
+            for (int rb = k; rb < nbk; rb += 16) psum = cadd(psum, pp[((int64_t)ll * nbk + rb) * 2 + which]);
+            if (k == 0) {
+                const int64_t r0 = s0 - h.j0;
+                e1 = which ? h.Vp[r0 + (int64_t)ll * h.m] : w_at(h, r0, ll);
+            }
+        }
+#pragma unroll
+        for (int m = 8; m >= 1; m >>= 1) psum = cadd(psum, shfl_xor_c(psum, m));
+        if (k == 0 && ll < h.i) {
+            const double2 t = cadd(cconj(e1), cmul(sc, psum));
+            s_t[which][ll] = t;
+            if (b == 0 && which) h.G[ll + (int64_t)h.i * 64] = t;
+        }
+    }
+    __syncthreads();
+    double2 corr = cmk(0, 0);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int l = g + u * RED_G;
+        if (valid && l < h.i) corr = cadd(corr, cadd(cmul(vv[u], s_t[0][l]), cmul(ww[u], s_t[1][l])));
+    }
+    sacc[g][lane] = csub(cadd(cmul(sc, cadd(acc0, acc1)), a_rs0), corr);
     __syncthreads();
     double2 dotp = cmk(0, 0);
-    const double2 tau = h.tau[h.j0 + h.i];
-    if (g == 0 && valid) {
-        double2 y = cmk(0, 0);
+    if (g == 0 && grow >= h.j0 && grow < h.n) {
+        double2 vr = cmk(0, 0);
+        if (valid) {
+            double2 y = cmk(0, 0);
 #pragma unroll
-        for (int k = 0; k < RED_G; ++k) y = cadd(y, sacc[k][lane]);
-        y = cmul(y, tau);
-        h.Wp[lr + (int64_t)h.i * h.m] = y;
-        cfmac(dotp, y, h.Vp[lr + (int64_t)h.i * h.m]);
+            for (int kk = 0; kk < RED_G; ++kk) y = cadd(y, sacc[kk][lane]);
+            y = cmul(y, tau);
+            h.Wp[lr + (int64_t)h.i * h.m] = y;
+            vr = grow == s0 ? cmk(1, 0) : cmul(xr, sc);
+            cfmac(dotp, y, vr);
+        }
+        h.Vp[lr + (int64_t)h.i * h.m] = vr;
     }
+    // panel rows above the first tile row of the trailing block: V(:, i) = 0
+    if (b == 0)
+        for (int r = threadIdx.x; r < ((s0 >> 6) << 6) - h.j0; r += blockDim.x)
+            h.Vp[r + (int64_t)h.i * h.m] = cmk(0, 0);
     dotp = block_sum2(dotp, sh);
     if (threadIdx.x == 0) ((double2*)(h.part + pl.c))[b] = dotp;
-}
-
-__global__ __launch_bounds__(1024) void k_hemv_reduce(HetrdCol h, const double2* __restrict__ Yr,
-                                                      const double2* __restrict__ Yc, int nbk, int nq,
-                                                      int ch, int makev_nrb)
-{
-    reduce_body(h, Yr, Yc, nbk, nq, ch, makev_nrb, blockIdx.x);
 }
 
 // Panel close: alpha2 of the panel's last column (from its reduce partials,
@@ -663,10 +730,6 @@ __device__ __attribute__((always_inline)) inline void panel_close_body(const Het
     larft_body(nbp, h.tau + h.j0, h.G, T, ldt, col);
 }
 
-__global__ __launch_bounds__(64) void k_panel_close(HetrdCol h, int nbp, int red_gx, double2* T, int ldt)
-{
-    panel_close_body(h, nbp, red_gx, T, ldt);
-}
 
 // Trailing update A22 -= V W^H + W V^H, K = nb, for the trailing block
 // A[s:, s:] (s = j0 + nb) over the 64 x 64 storage tiles that intersect it:
@@ -924,21 +987,11 @@ __global__ __launch_bounds__(256) void kb_colupd(const HetrdSys* __restrict__ sy
     HetrdCol h;
     const HetrdSys s = sys[blockIdx.z];
     if (!batch_col(s, c, nb, h)) return;
-    const int gx = (int)ceil_div64(h.m - h.i, 256);
-    if ((int)blockIdx.x >= gx) return;
+    if ((int)blockIdx.x >= colupd_blocks(h.m, h.i)) return;
     const int prev_gx = (h.i > 0) ? tile_count(s.n) - (c >> 6) : 0;
-    colupd_body<T>(h, (const T*)s.A, blockIdx.x, gx, prev_gx);
+    colupd_body<T>(h, (const T*)s.A, blockIdx.x, prev_gx);
 }
 
-__global__ __launch_bounds__(256) void kb_makev(const HetrdSys* __restrict__ sys, int c, int nb)
-{
-    HetrdCol h;
-    const HetrdSys s = sys[blockIdx.z];
-    if (!batch_col(s, c, nb, h)) return;
-    const int gx = (int)ceil_div64(h.m, MAKEV_ROWS), gy = max(h.i, 1);
-    if ((int)blockIdx.x >= gx || (int)blockIdx.y >= gy) return;
-    makev_body(h, blockIdx.x, blockIdx.y, gx, (int)ceil_div64(h.m - h.i, 256));
-}
 
 template <typename T, int CH, int U2, int AUX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HEMV_WPE_FOR(U2), 8)))
@@ -950,11 +1003,11 @@ void kb_hemv(const HetrdSys* __restrict__ sys, int c, int nb)
     const int nbk = tile_count(s.n) - ((c + 1) >> 6), nq = (int)ceil_div64(nbk, CH);
     if ((int)blockIdx.x >= nq || (int)blockIdx.y >= nbk) return;
     const T* A = (const T*)s.A;
-    const double2* vtr = h.Vp + (h.i + 1) + (int64_t)h.i * h.m;
-    if (U2 == 2) hemv_bfly_body<T, CH, AUX>(s.n, c + 1, A, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
-    else hemv_body<T, CH, U2, AUX>(s.n, c + 1, A, vtr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y);
+    if (U2 == 2) hemv_bfly_body<T, CH, AUX>(s.n, c + 1, A, nullptr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y, h);
+    else hemv_body<T, CH, U2, AUX>(s.n, c + 1, A, nullptr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y, h);
 }
 
+template <typename T>
 __global__ __launch_bounds__(1024) void kb_reduce(const HetrdSys* __restrict__ sys, int c, int nb,
                                                   int ch)
 {
@@ -963,7 +1016,7 @@ __global__ __launch_bounds__(1024) void kb_reduce(const HetrdSys* __restrict__ s
     if (!batch_col(s, c, nb, h)) return;
     const int nbk = tile_count(s.n) - ((c + 1) >> 6), nq = (int)ceil_div64(nbk, ch);
     if ((int)blockIdx.x >= nbk) return;
-    reduce_body(h, s.Yr, s.Yc, nbk, nq, ch, (int)ceil_div64(h.m, MAKEV_ROWS), blockIdx.x);
+    reduce_body<T>(h, (const T*)s.A, s.Yr, s.Yc, nbk, nq, ch, blockIdx.x);
 }
 
 // Column c closes the matrix's current panel when c = j0 + nbp - 1.
@@ -1018,107 +1071,18 @@ static inline int hemv_ch_for(const unsigned char* table, int nt) {
 // Number of storage tile rows of the trailing block A[s0:, s0:].
 static inline int hemv_nbk(int n, int s0) { return tile_count(n) - (s0 >> 6); }
 
+// v given: plain product (autotuner timing); v null: the reflector step of
+// column h.j0 + h.i (s0 = that column + 1) is fused in.
 template <typename T>
 static void hemv_launch(hipStream_t st, int code, int n, int s0, const T* A, const double2* v,
-                        double2* Yr, double2* Yc)
+                        double2* Yr, double2* Yc, const HetrdCol& h)
 {
     const int ch = code & 31;
     const int nbk = hemv_nbk(n, s0);
     const int nq = (int)ceil_div64(nbk, ch);
-#define HL(CC, UU, AX) hipLaunchKernelGGL((k_hemv_lower<T, CC, UU, AX>), dim3(nq, nbk), dim3(256), 0, st, n, s0, A, v, Yr, Yc, nbk, nq)
+#define HL(CC, UU, AX) hipLaunchKernelGGL((k_hemv_lower<T, CC, UU, AX>), dim3(nq, nbk), dim3(256), 0, st, n, s0, A, v, Yr, Yc, nbk, nq, h)
     HEMV_DISPATCH(code, HL)
 #undef HL
-}
-
-struct HetrdWork {
-    int her2k_valu;      // 1: FP64 VALU trailing update, 0: FP64 MFMA
-    hipEvent_t* prof_ev; // optional: 2 events per column around the hemv launch
-    const unsigned char* ch_table;  // hemv variant code per log2(nt) bucket
-    double2* Vstore;     // packed panel reflectors
-    int64_t* voff;       // host array: offset of panel p in Vstore
-    double2* Tall;       // compact-WY factors, nb x nb per panel
-    double2* Wp;         // n x nb
-    double2* alpha2;     // nb
-    double2* G;          // 64 x 64
-    double2* acol;       // n
-    double2* Yr;         // nbk^2 x 64
-    double2* Yc;         // nbk * nq * 64
-    double2* t12;        // 128
-    double* part;        // reduction partials
-    double* scal;        // 8
-    unsigned int* counter;
-};
-
-template <typename T>
-static hipError_t hetrd_run(hipStream_t st, int n, T* A, int nb, double* d, double* e,
-                            double2* tau, const HetrdWork& w)
-{
-    hipError_t err;
-    const int ntile = tile_count(n);
-    if (n == 1) {
-        hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, (const void*)A,
-                           (int)(sizeof(T) == sizeof(float2)), d);
-        return hipGetLastError();
-    }
-    int p = 0;
-    for (int j0 = 0; j0 < n - 1; j0 += nb, ++p) {
-        const int nbp = min(nb, n - 1 - j0);
-        const int m = n - j0;
-        HetrdCol h;
-        h.n = n; h.j0 = j0; h.m = m;
-        h.Vp = w.Vstore + w.voff[p];
-        h.Wp = w.Wp;
-        h.alpha2 = w.alpha2;
-        h.G = w.G;
-        h.acol = w.acol; h.d = d; h.e = e; h.tau = tau; h.scal = w.scal; h.part = w.part;
-        h.counter = w.counter;
-        for (int i = 0; i < nbp; ++i) {
-            h.i = i;
-            const int L = m - i;
-            const int c = j0 + i;
-            const int nt = n - c - 1;
-            const int prev_gx = (i > 0) ? ntile - (c >> 6) : 0;
-            hipLaunchKernelGGL(k_colupd<T>, dim3((unsigned)ceil_div64(L, 256)), dim3(256), 0, st, h,
-                               (const T*)A, prev_gx);
-            hipLaunchKernelGGL(k_makev, dim3((unsigned)ceil_div64(m, MAKEV_ROWS), (unsigned)max(i, 1)),
-                               dim3(256), 0, st, h, (int)ceil_div64(L, 256));
-            const int code = hemv_ch_for(w.ch_table, nt);
-            const int ch = code & 31;
-            const int nbk = hemv_nbk(n, c + 1);
-            const int nq = (int)ceil_div64(nbk, ch);
-            const double2* vtr = h.Vp + (i + 1) + (int64_t)i * m;
-            if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c], st);
-            hemv_launch<T>(st, code, n, c + 1, (const T*)A, vtr, w.Yr, w.Yc);
-            if (w.prof_ev) (void)hipEventRecord(w.prof_ev[2 * c + 1], st);
-            hipLaunchKernelGGL(k_hemv_reduce, dim3((unsigned)nbk), dim3(1024), 0, st, h,
-                               (const double2*)w.Yr, (const double2*)w.Yc, nbk, nq, ch,
-                               (int)ceil_div64(m, MAKEV_ROWS));
-            if ((err = hipGetLastError()) != hipSuccess) return err;
-        }
-        // alpha2 of the last column and the compact-WY factor of this panel
-        // from G(l, i) = V(:, l)^H v_i
-        h.i = nbp - 1;
-        hipLaunchKernelGGL(k_panel_close, dim3(1), dim3(64), 0, st, h, nbp,
-                           hemv_nbk(n, j0 + nbp), w.Tall + (size_t)p * nb * nb, nb);
-        if ((err = hipGetLastError()) != hipSuccess) return err;
-        const int s2 = j0 + nbp;
-        if (s2 < n) {
-            const int64_t nT = ntile - (s2 >> 6);
-            const dim3 g((unsigned)(nT * (nT + 1) / 2));
-            if (w.her2k_valu)
-                hipLaunchKernelGGL(k_her2k_lower<T>, g, dim3(256), 0, st, n, s2, A,
-                                   (const double2*)h.Vp, (const double2*)w.Wp, (int64_t)m, j0, nbp,
-                                   (const double2*)w.alpha2);
-            else
-                hipLaunchKernelGGL(k_her2k_mfma<T>, g, dim3(256), 0, st, n, s2, A,
-                                   (const double2*)h.Vp, (const double2*)w.Wp, (int64_t)m, j0, nbp,
-                                   (const double2*)w.alpha2);
-        }
-        if ((err = hipGetLastError()) != hipSuccess) return err;
-    }
-    hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, (const void*)A,
-                       (int)(sizeof(T) == sizeof(float2)), d);
-    return hipGetLastError();
 }
 
 // Workspace sizes for a given n, nb (Yc sized for the shortest hemv chunk).
@@ -1135,24 +1099,55 @@ extern "C" void bse_hetrd_workspace(int n, int nb, int64_t* vstore_elems, int64_
     *part_doubles = ::part_doubles(n);
 }
 
-// nb <= 64, A in tiled lower storage.  Tall receives one nb x nb compact-WY
-// factor per panel.
+extern "C" hipError_t bse_hetrd_batch_panel(hipStream_t st, int fp32, int c, int nb, int nact,
+                                            const void* d_sys, int64_t max_tiles, int her2k_valu);
+static hipError_t column_launch(hipStream_t st, int fp32, int c, int nb, int nact, int n0,
+                                const void* d_sys, int code, hipEvent_t e0, hipEvent_t e1);
+
+static_assert(sizeof(HetrdSys) <= 256, "HetrdSys must fit the 256-byte device slot");
+
+// Single-matrix reduction, nb <= 32, A in tiled lower storage: the batched
+// kernels with one matrix, whose description (HetrdSys) is copied to d_sys
+// (at least 256 bytes of device memory) first, so that every launch carries
+// only a pointer and the column index (large kernel arguments cost about a
+// microsecond per launch).  Tall receives one nb x nb compact-WY factor per panel;
+// prof_ev (optional) gets 2 events per column around the hemv launch.
 extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int nb,
                                 double* d, double* e, double2* tau, double2* Vstore,
-                                const int64_t* voff, double2* Tall, double2* Wp, double2* alpha2,
+                                double2* Tall, double2* Wp, double2* alpha2,
                                 double2* G, double2* acol, double2* Yr, double2* Yc, double2* t12,
                                 double* part, double* scal, unsigned int* counter,
-                                hipEvent_t* prof_ev, const unsigned char* ch_table, int her2k_valu)
+                                hipEvent_t* prof_ev, const unsigned char* ch_table, int her2k_valu,
+                                void* d_sys)
 {
-    HetrdWork w;
-    w.her2k_valu = her2k_valu;
-    w.prof_ev = prof_ev;
-    w.ch_table = ch_table;
-    w.Vstore = Vstore; w.voff = (int64_t*)voff; w.Tall = Tall; w.Wp = Wp; w.alpha2 = alpha2;
-    w.G = G; w.acol = acol; w.Yr = Yr; w.Yc = Yc; w.t12 = t12; w.part = part; w.scal = scal;
-    w.counter = counter;
-    if (fp32) return hetrd_run<float2>(st, n, (float2*)A, nb, d, e, tau, w);
-    return hetrd_run<double2>(st, n, (double2*)A, nb, d, e, tau, w);
+    if (nb < 1 || nb > 32 || n < 1 || !d_sys) return hipErrorInvalidValue;
+    hipError_t err;
+    if (n > 1) {
+        HetrdSys x;
+        x.n = n; x.A = A; x.Vstore = Vstore; x.Tall = Tall; x.Wp = Wp; x.alpha2 = alpha2; x.G = G;
+        x.acol = acol; x.d = d; x.e = e; x.tau = tau; x.scal = scal; x.part = part;
+        x.counter = counter; x.Yr = Yr; x.Yc = Yc; x.t12 = t12;
+        if ((err = hipMemcpyAsync(d_sys, &x, sizeof(x), hipMemcpyHostToDevice, st)) != hipSuccess)
+            return err;
+        const int ntile = tile_count(n);
+        for (int c = 0; c < n - 1; ++c) {
+            const int code = hemv_ch_for(ch_table, n - c - 1);
+            err = column_launch(st, fp32, c, nb, 1, n, d_sys, code, prof_ev ? prof_ev[2 * c] : nullptr,
+                                prof_ev ? prof_ev[2 * c + 1] : nullptr);
+            if (err != hipSuccess) return err;
+            const int j0 = (c / nb) * nb, nbp = min(nb, n - 1 - j0);
+            if (c != j0 + nbp - 1) continue;
+            int64_t tiles = 0;
+            if (j0 + nbp < n) {
+                const int64_t nT = ntile - ((j0 + nbp) >> 6);
+                tiles = nT * (nT + 1) / 2;
+            }
+            if ((err = bse_hetrd_batch_panel(st, fp32, c, nb, 1, d_sys, tiles, her2k_valu)) != hipSuccess)
+                return err;
+        }
+    }
+    hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, (const void*)A, fp32, d);
+    return hipGetLastError();
 }
 
 // One hemv launch (autotuner timing): y partials of A[s0:, s0:] v for A in
@@ -1161,8 +1156,10 @@ extern "C" hipError_t bse_launch_hemv(hipStream_t st, int fp32, int code, int n,
                                       const void* A, const double2* v, double2* Yr, double2* Yc)
 {
     if (s0 < 0 || s0 >= n || !hemv_code_valid(code)) return hipErrorInvalidValue;
-    if (fp32) hemv_launch<float2>(st, code, n, s0, (const float2*)A, v, Yr, Yc);
-    else hemv_launch<double2>(st, code, n, s0, (const double2*)A, v, Yr, Yc);
+    if (!v) return hipErrorInvalidValue;
+    HetrdCol h0 = {};
+    if (fp32) hemv_launch<float2>(st, code, n, s0, (const float2*)A, v, Yr, Yc, h0);
+    else hemv_launch<double2>(st, code, n, s0, (const double2*)A, v, Yr, Yc, h0);
     return hipGetLastError();
 }
 
@@ -1173,11 +1170,12 @@ extern "C" hipError_t bse_launch_last_diag(hipStream_t st, int fp32, int n, cons
 }
 
 // Column c of the batched reduction for the first nact matrices of d_sys
-// (sorted by decreasing n; n0 = n of matrix 0 fixes the grid extents).
-extern "C" hipError_t bse_hetrd_batch_column(hipStream_t st, int fp32, int c, int nb, int nact,
-                                             int n0, const void* d_sys, int code)
+// (sorted by decreasing n; n0 = n of matrix 0 fixes the grid extents).  The
+// optional events bracket the hemv launch.
+static hipError_t column_launch(hipStream_t st, int fp32, int c, int nb, int nact, int n0,
+                                const void* d_sys, int code, hipEvent_t e0, hipEvent_t e1)
 {
-    if (!hemv_code_valid(code)) return hipErrorInvalidValue;
+    if (!hemv_code_valid(code) || nb < 1 || nb > 32) return hipErrorInvalidValue;
     const int ch = code & 31;
     if (nact <= 0 || c >= n0 - 1) return hipSuccess;
     const HetrdSys* sys = (const HetrdSys*)d_sys;
@@ -1185,19 +1183,26 @@ extern "C" hipError_t bse_hetrd_batch_column(hipStream_t st, int fp32, int c, in
     const int nbk = hemv_nbk(n0, c + 1);
     const unsigned z = (unsigned)nact;
     if (fp32)
-        hipLaunchKernelGGL(kb_colupd<float2>, dim3((unsigned)ceil_div64(m - i, 256), 1, z), dim3(256), 0, st, sys, c, nb);
+        hipLaunchKernelGGL(kb_colupd<float2>, dim3((unsigned)colupd_blocks(m, i), 1, z), dim3(256), 0, st, sys, c, nb);
     else
-        hipLaunchKernelGGL(kb_colupd<double2>, dim3((unsigned)ceil_div64(m - i, 256), 1, z), dim3(256), 0, st, sys, c, nb);
-    hipLaunchKernelGGL(kb_makev, dim3((unsigned)ceil_div64(m, MAKEV_ROWS), (unsigned)max(i, 1), z),
-                       dim3(256), 0, st, sys, c, nb);
+        hipLaunchKernelGGL(kb_colupd<double2>, dim3((unsigned)colupd_blocks(m, i), 1, z), dim3(256), 0, st, sys, c, nb);
     const dim3 hg((unsigned)ceil_div64(nbk, ch), (unsigned)nbk, z);
+    if (e0) (void)hipEventRecord(e0, st);
 #define KBF(CC, UU, AX) hipLaunchKernelGGL((kb_hemv<float2, CC, UU, AX>), hg, dim3(256), 0, st, sys, c, nb)
 #define KBD(CC, UU, AX) hipLaunchKernelGGL((kb_hemv<double2, CC, UU, AX>), hg, dim3(256), 0, st, sys, c, nb)
     if (fp32) HEMV_DISPATCH(code, KBF) else HEMV_DISPATCH(code, KBD)
 #undef KBF
 #undef KBD
-    hipLaunchKernelGGL(kb_reduce, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch);
+    if (e1) (void)hipEventRecord(e1, st);
+    if (fp32) hipLaunchKernelGGL(kb_reduce<float2>, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch);
+    else hipLaunchKernelGGL(kb_reduce<double2>, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch);
     return hipGetLastError();
+}
+
+extern "C" hipError_t bse_hetrd_batch_column(hipStream_t st, int fp32, int c, int nb, int nact,
+                                             int n0, const void* d_sys, int code)
+{
+    return column_launch(st, fp32, c, nb, nact, n0, d_sys, code, nullptr, nullptr);
 }
 
 // Panel close at column c: compact-WY factor and trailing update for every
