@@ -276,7 +276,8 @@ extern "C" int bse_get_dipoles(bse_ctx* ctx, double* rho) {
 // time chunk lengths 4, 8 and 16 on a trailing block of the matrix about to
 // be reduced (the kernel only reads it) and keep the fastest, each as the
 // plain variant (code ch), the two-row-blocks-per-step variant (ch + 32) and
-// the variant with per-slab cross-lane column sums (ch + 64), with
+// the variant with per-slab cross-lane column sums (ch + 64), the variant
+// with waves owning rows (ch + 96, chunks of 2 ch tiles), with
 // non-temporal or default-policy loads (+ 128).  Repeated launches over the
 // same block see the cache reuse that consecutive columns see.  The table is
 // per context and storage precision; BSE_AUTOTUNE=0 keeps the default
@@ -299,9 +300,10 @@ int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, double2* v, doub
         const int reps = nt > 4096 ? 3 : 10;
         float best = 0.f;
         int best_ch = 8 + 64;
-        for (int k = 0; k < 18; ++k) {
-            static const int base[9] = {8 + 64, 4 + 64, 16 + 64, 8 + 32, 4 + 32, 16 + 32, 8, 4, 16};
-            const int ch = base[k % 9] + (k >= 9 ? 128 : 0);
+        for (int k = 0; k < 24; ++k) {
+            static const int base[12] = {8 + 64, 4 + 64, 16 + 64, 8 + 32, 4 + 32, 16 + 32, 8, 4, 16,
+                                         4 + 96, 8 + 96, 16 + 96};
+            const int ch = base[k % 12] + (k >= 12 ? 128 : 0);
             HCK(bse_launch_hemv(st, fp32, ch, n, s0, H, v, Yr, Yc));
             HCK(hipEventRecord(ctx->tev[0], st));
             for (int r = 0; r < reps; ++r) HCK(bse_launch_hemv(st, fp32, ch, n, s0, H, v, Yr, Yc));

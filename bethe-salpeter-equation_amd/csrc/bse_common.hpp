@@ -272,15 +272,15 @@ struct HetrdSys {
 };
 
 // Hemv variant for trailing size nt from an autotune table indexed by
-// floor(log2(nt)): code = ch + 32 u2 + 128 t (chunk length ch in {4, 8, 16};
-// u2 = 1: two row blocks per step, u2 = 2: per-slab cross-lane column sums;
-// t = 1: default cache policy instead of non-temporal loads); untuned entries
-// give ch = 8, u2 = 2, t = 0.
+// floor(log2(nt)): code = ch + 32 u2 + 128 t (ch in {4, 8, 16}; u2 = 1: two
+// row blocks per step, u2 = 2: per-slab cross-lane column sums, u2 = 3: waves
+// owning rows, chunks of 2 ch tiles; t = 1: default cache policy instead of
+// non-temporal loads); untuned entries give ch = 8, u2 = 2, t = 0.
 inline int hemv_ch_table_lookup(const unsigned char* table, int nt) {
     if (!table || nt <= 0) return 8 + 64;
     int b = 0;
     while ((2 << b) <= nt && b < 30) ++b;
     const int code = table[b < BSE_HEMV_BUCKETS ? b : BSE_HEMV_BUCKETS - 1];
-    const int ch = code & 31, u2 = (code >> 5) & 3;
-    return ((ch == 4 || ch == 8 || ch == 16) && u2 <= 2) ? code : 8 + 64;
+    const int ch = code & 31;
+    return (ch == 4 || ch == 8 || ch == 16) ? code : 8 + 64;
 }

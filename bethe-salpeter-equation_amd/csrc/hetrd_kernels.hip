@@ -215,7 +215,9 @@ __device__ inline Refl reflector_scalars(const HetrdCol& h, int colupd_gx, doubl
 // TJ = tc + bj, tile rows TI = tc + bi (tc = s0 / 64).  Workgroup (q, bj)
 // handles tile rows bi in [bj + q*CH, bj + (q+1)*CH) of tile column bj; the
 // tiles of one tile column are consecutive in memory, so a workgroup streams
-// one contiguous run.  Row results of each tile go to Yr[bi][bj]; the
+// one contiguous run.  Row results of each tile go to Yr[bj][bi] (so that
+// the rows of one workgroup are written as one contiguous block: scattered
+// 1 KB writes cost the streaming reads about a seventh of their rate); the
 // transposed (conjugate) contribution for tile column bj is accumulated across
 // the chunk and written to Yc[bj][q].  One wave owns 16 columns of the tile.
 // The chunk length CH is chosen per trailing size by the autotuner.
@@ -230,16 +232,22 @@ __device__ inline Refl reflector_scalars(const HetrdCol& h, int colupd_gx, doubl
 // taken off the row sum once (which also drops any imaginary part of the
 // diagonal).
 #define HEMV_CH_MIN 4
-// Variant code: ch + 32 u2 + 128 t (chunk length ch in {4, 8, 16}; u2 = 1 two
-// tile rows per step, u2 = 2 per-slab cross-lane column sums; t = 1 default
-// cache policy for the matrix loads, t = 0 non-temporal: a large trailing
-// block is streamed once per column and cannot stay in the caches, a small
-// one can).  AUX is the resulting load cache-policy operand.
+// Variant code: ch + 32 u2 + 128 t (ch in {4, 8, 16}; u2 = 1 two tile rows
+// per step, u2 = 2 per-slab cross-lane column sums, u2 = 3 the same over
+// blocks of four tile columns, which cuts the row partials fourfold; t = 1
+// default cache policy
+// for the matrix loads, t = 0 non-temporal: a large trailing block is
+// streamed once per column and cannot stay in the caches, a small one can).
+// AUX is the resulting load cache-policy operand.
 #define HEMV_CODE_DEFAULT (8 + 64)
 __host__ __device__ inline bool hemv_code_valid(int code) {
-    const int ch = code & 31, u2 = (code >> 5) & 3;
-    return code >= 0 && code < 256 && (ch == 4 || ch == 8 || ch == 16) && u2 <= 2;
+    const int ch = code & 31;
+    return code >= 0 && code < 256 && (ch == 4 || ch == 8 || ch == 16);
 }
+// Tile rows per workgroup chunk, and tile columns per workgroup, of a
+// variant code.
+__host__ __device__ inline int hemv_chunk(int code) { return code & 31; }
+__host__ __device__ inline int hemv_rblk(int code) { return ((code >> 5) & 3) == 3 ? 4 : 1; }
 
 // Column sums of sixteen per-lane complex values over the 64 lanes of a wave
 // by reduce-scatter: after exchanges at lane distances 32, 16, 8, 4 every lane
@@ -455,7 +463,7 @@ __device__ __attribute__((always_inline)) inline void hemv_body(int n, int s0, c
         const int bb = e >> 6, ln = e & 63;
         const double2 s = cadd(cadd(s_part[0][bb][ln], s_part[1][bb][ln]),
                                cadd(s_part[2][bb][ln], s_part[3][bb][ln]));
-        Yr[((int64_t)(bi0 + bb) * nbk + bj) * 64 + ln] = s;
+        Yr[((int64_t)bj * nbk + bi0 + bb) * 64 + ln] = s;
     }
 #pragma unroll
     for (int jj = 0; jj < 16; ++jj) {
@@ -471,94 +479,116 @@ __device__ __attribute__((always_inline)) inline void hemv_body(int n, int s0, c
 // across the wave at once and added to one FP64 accumulator per lane, so the
 // kernel keeps four or more waves per SIMD while a full slab of loads is in
 // flight.  Same tiles, masks and partial outputs as hemv_body.
-template <typename T, int CH, int AUX>
+template <typename T, int CH, int R, int AUX>
 __device__ __attribute__((always_inline)) inline void hemv_bfly_body(int n, int s0, const T* __restrict__ A,
                                                                       const double2* __restrict__ v,
                                                                       double2* __restrict__ Yr,
                                                                       double2* __restrict__ Yc, int nbk,
-                                                                      int nq, int q, int bj,
+                                                                      int nq, int q, int pb, int npb,
                                                                       const HetrdCol& h)
 {
     HemvVec vs;
     vs.v = v;
     vs.acol = h.acol;
-    if (!v && q == 0 && bj == 0) hemv_scalars(h);
-    if (!v && bj * nq + q < nbk) hemv_dots(h, vs, (s0 >> 6) + bj * nq + q, bj * nq + q, nbk);
-    const int bi0 = bj + q * CH;
-    if (bi0 >= nbk) return;
-    const int bi1 = min(nbk, bi0 + CH);
+    if (!v && q == 0 && pb == 0) hemv_scalars(h);
+    if (!v)
+        for (int t = pb * nq + q; t < nbk; t += nq * npb) hemv_dots(h, vs, (s0 >> 6) + t, t, nbk);
+    // workgroup (q, pb): tile columns bj = pb R + r (r < R), tile rows
+    // [row0, row1) with row0 = pb R + q CH; lower tiles only (bi >= bj)
+    const int row0 = pb * R + q * CH;
+    if (row0 >= nbk) return;
+    const int row1 = min(nbk, row0 + CH);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int ntile = tile_count(n), tc = s0 >> 6, TJ = tc + bj;
-    __shared__ double2 s_v[64];
+    const int ntile = tile_count(n), tc = s0 >> 6;
+    __shared__ double2 s_v[R][64];
     __shared__ double2 s_part[4][CH][64];
-    if (threadIdx.x < 64) s_v[threadIdx.x] = vs.at(TJ * 64 + threadIdx.x, s0, n);
-    __syncthreads();
-    const double2* sv = s_v + w * 16;
-    double2 colacc = cmk(0, 0);
-    const int wu = __builtin_amdgcn_readfirstlane(w);
-    const T* run = A + tile_pos(tc + bi0, TJ, ntile) * (BSE_TILE * BSE_TILE);
-    const __amdgpu_buffer_rsrc_t rs = bse_rsrc(run + wu * 16 * 64, 0x7fffffff);
-    const int cstride = 64 * (int)sizeof(T);
-    for (int bi = bi0; bi < bi1;) {
-        const int TI = tc + bi;
-        const bool diag = TI == TJ;
-        double2 c[16];
-        if (sizeof(T) == 8 && !diag && bi + 1 < bi1) {
-            const int hb = lane >> 5, rl = (2 * lane) & 63;
-            const int vo2 = ((bi - bi0 + hb) * 4096 + rl) * (int)sizeof(T);
-            bse_i4v qv[16];
+    for (int e = threadIdx.x; e < R * 64; e += 256)
+        s_v[e >> 6][e & 63] = vs.at((tc + pb * R + (e >> 6)) * 64 + (e & 63), s0, n);
+    if (R > 1) {
+        // row sums accumulate over the R tile columns: this wave's slots start at zero
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) qv[jj] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo2, jj * cstride, AUX);
-            const int r2 = (TI + hb) * 64 + rl;
-            const double2 v0 = vs.at(r2, s0, n), v1 = vs.at(r2 + 1, s0, n);
-            double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                const int q0 = qv[jj][0], q1 = qv[jj][1], q2 = qv[jj][2], q3 = qv[jj][3];
-                const double2 x = cmk((double)bse_i2f(q0), (double)bse_i2f(q1));
-                const double2 y = cmk((double)bse_i2f(q2), (double)bse_i2f(q3));
-                const double2 sj = sv[jj];
-                cfma(r0, x, sj);
-                cfma(r1, y, sj);
-                double2 cc = cmk(0, 0);
-                cfmac(cc, x, v0);
-                cfmac(cc, y, v1);
-                c[jj] = cc;
-            }
-            colacc = cadd(colacc, colsum16(c, lane));
-            s_part[w][bi + hb - bi0][rl] = r0;
-            s_part[w][bi + hb - bi0][rl + 1] = r1;
-            bi += 2;
-            continue;
-        }
-        const int row = TI * 64 + lane;
-        const double2 vr = vs.at(row, s0, n);
-        double2 racc = cmk(0, 0);
-        const int vo = ((bi - bi0) * 4096 + lane) * (int)sizeof(T);
-        T a[16];
-#pragma unroll
-        for (int jj = 0; jj < 16; ++jj) a[jj] = bufld<T>::template ld<AUX>(rs, vo, jj * cstride);
-#pragma unroll
-        for (int jj = 0; jj < 16; ++jj) {
-            const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
-            cfma(racc, x, sv[jj]);
-            double2 cc = cmk(0, 0);
-            cfmac(cc, x, vr);
-            c[jj] = cc;
-        }
-        if (diag) racc = csub(racc, cscale(vr, hemv_diag_re(a, lane, w)));
-        colacc = cadd(colacc, colsum16(c, lane));
-        s_part[w][bi - bi0][lane] = racc;
-        ++bi;
+        for (int bb = 0; bb < CH; ++bb) s_part[w][bb][lane] = cmk(0, 0);
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < (bi1 - bi0) * 64; e += 256) {
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    const int cstride = 64 * (int)sizeof(T);
+#pragma unroll 1
+    for (int r = 0; r < R; ++r) {
+        double2 colacc = cmk(0, 0);
+        const int bj = pb * R + r, TJ = tc + bj;
+        const int bstart = max(row0, bj);
+        if (bj >= nbk || bstart >= row1) continue;
+        const double2* sv = s_v[r] + w * 16;
+        const T* run = A + tile_pos(tc + bstart, TJ, ntile) * (BSE_TILE * BSE_TILE);
+        const __amdgpu_buffer_rsrc_t rs = bse_rsrc(run + wu * 16 * 64, 0x7fffffff);
+        for (int bi = bstart; bi < row1;) {
+            const int TI = tc + bi;
+            const bool diag = TI == TJ;
+            double2 c[16];
+            if (sizeof(T) == 8 && !diag && bi + 1 < row1) {
+                const int hb = lane >> 5, rl = (2 * lane) & 63;
+                const int vo2 = ((bi - bstart + hb) * 4096 + rl) * (int)sizeof(T);
+                bse_i4v qv[16];
+#pragma unroll
+                for (int jj = 0; jj < 16; ++jj) qv[jj] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo2, jj * cstride, AUX);
+                const int r2 = (TI + hb) * 64 + rl;
+                const double2 v0 = vs.at(r2, s0, n), v1 = vs.at(r2 + 1, s0, n);
+                double2 r0 = cmk(0, 0), r1 = cmk(0, 0);
+#pragma unroll
+                for (int jj = 0; jj < 16; ++jj) {
+                    const int q0 = qv[jj][0], q1 = qv[jj][1], q2 = qv[jj][2], q3 = qv[jj][3];
+                    const double2 x = cmk((double)bse_i2f(q0), (double)bse_i2f(q1));
+                    const double2 y = cmk((double)bse_i2f(q2), (double)bse_i2f(q3));
+                    const double2 sj = sv[jj];
+                    cfma(r0, x, sj);
+                    cfma(r1, y, sj);
+                    double2 cc = cmk(0, 0);
+                    cfmac(cc, x, v0);
+                    cfmac(cc, y, v1);
+                    c[jj] = cc;
+                }
+                colacc = cadd(colacc, colsum16(c, lane));
+                double2* sp = &s_part[w][bi + hb - row0][rl];
+                if (R > 1) {
+                    sp[0] = cadd(sp[0], r0);
+                    sp[1] = cadd(sp[1], r1);
+                } else {
+                    sp[0] = r0;
+                    sp[1] = r1;
+                }
+                bi += 2;
+                continue;
+            }
+            const int row = TI * 64 + lane;
+            const double2 vr = vs.at(row, s0, n);
+            double2 racc = cmk(0, 0);
+            const int vo = ((bi - bstart) * 4096 + lane) * (int)sizeof(T);
+            T a[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) a[jj] = bufld<T>::template ld<AUX>(rs, vo, jj * cstride);
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const double2 x = cmk((double)a[jj].x, (double)a[jj].y);
+                cfma(racc, x, sv[jj]);
+                double2 cc = cmk(0, 0);
+                cfmac(cc, x, vr);
+                c[jj] = cc;
+            }
+            if (diag) racc = csub(racc, cscale(vr, hemv_diag_re(a, lane, w)));
+            colacc = cadd(colacc, colsum16(c, lane));
+            double2* sp = &s_part[w][bi - row0][lane];
+            *sp = (R > 1) ? cadd(*sp, racc) : racc;
+            ++bi;
+        }
+        if ((lane & 3) == 0) Yc[((int64_t)bj * nq + q) * 64 + w * 16 + colsum16_col(lane)] = colacc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < (row1 - row0) * 64; e += 256) {
         const int bb = e >> 6, ln = e & 63;
         const double2 s = cadd(cadd(s_part[0][bb][ln], s_part[1][bb][ln]),
                                cadd(s_part[2][bb][ln], s_part[3][bb][ln]));
-        Yr[((int64_t)(bi0 + bb) * nbk + bj) * 64 + ln] = s;
+        Yr[((int64_t)pb * nbk + row0 + bb) * 64 + ln] = s;
     }
-    if ((lane & 3) == 0) Yc[((int64_t)bj * nq + q) * 64 + w * 16 + colsum16_col(lane)] = colacc;
 }
 
 // Occupancy bound of the hemv kernels: two waves per SIMD (up to 256 VGPRs)
@@ -572,16 +602,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HEMV_WPE_FO
 void k_hemv_lower(int n, int s0, const T* __restrict__ A, const double2* __restrict__ v,
                   double2* __restrict__ Yr, double2* __restrict__ Yc, int nbk, int nq, HetrdCol h)
 {
-    if (U2 == 2) hemv_bfly_body<T, CH, AUX>(n, s0, A, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y, h);
-    else hemv_body<T, CH, U2, AUX>(n, s0, A, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y, h);
+    if (U2 == 3) hemv_bfly_body<T, CH, 4, AUX>(n, s0, A, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y, gridDim.y, h);
+    else if (U2 == 2) hemv_bfly_body<T, CH, 1, AUX>(n, s0, A, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y, gridDim.y, h);
+    else hemv_body<T, CH, U2 & 1, AUX>(n, s0, A, v, Yr, Yc, nbk, nq, blockIdx.x, blockIdx.y, h);
 }
 
 // Instantiates M(CH, U2, AUX) for the variant of a (valid) code.
 #define HEMV_DISPATCH_CH(ch_, UU, AX, M) \
     { if (ch_ == 4) M(4, UU, AX); else if (ch_ == 16) M(16, UU, AX); else M(8, UU, AX); }
 #define HEMV_DISPATCH_U2(ch_, u2_, AX, M) \
-    { if (u2_ == 2) HEMV_DISPATCH_CH(ch_, 2, AX, M) else if (u2_ == 1) HEMV_DISPATCH_CH(ch_, 1, AX, M) \
-      else HEMV_DISPATCH_CH(ch_, 0, AX, M) }
+    { if (u2_ == 3) HEMV_DISPATCH_CH(ch_, 3, AX, M) else if (u2_ == 2) HEMV_DISPATCH_CH(ch_, 2, AX, M) \
+      else if (u2_ == 1) HEMV_DISPATCH_CH(ch_, 1, AX, M) else HEMV_DISPATCH_CH(ch_, 0, AX, M) }
 #define HEMV_DISPATCH(code, M) \
     { const int ch_ = (code) & 31, u2_ = ((code) >> 5) & 3; \
       if ((code) & 128) HEMV_DISPATCH_U2(ch_, u2_, 0, M) else HEMV_DISPATCH_U2(ch_, u2_, BSE_LOAD_STREAM, M) }
@@ -605,7 +636,7 @@ template <typename T>
 __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol& h, const T* __restrict__ A,
                                                                    const double2* __restrict__ Yr,
                                                                    const double2* __restrict__ Yc,
-                                                                   int nbk, int nq, int ch, int b)
+                                                                   int nbk, int nq, int ch, int rbk, int b)
 {
     __shared__ double2 sacc[RED_G][64];
     __shared__ double2 s_t[2][64];
@@ -623,25 +654,28 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
     double2 vv[2] = {cmk(0, 0), cmk(0, 0)}, ww[2] = {cmk(0, 0), cmk(0, 0)};
     double2 a_rs0 = cmk(0, 0), xr = cmk(0, 0);
     if (valid) {
-        // partials of this block row: Yr[b][0..b] and Yc[b][0..nqb), several
-        // independent loads issued before any of them is summed
-        const double2* yr = Yr + (int64_t)b * nbk * 64 + lane;
+        // partials of this block row: Yr[pb][b] (pb <= b / rbk, one per
+        // block of rbk tile columns) and Yc[b][0..nqb), several independent
+        // loads issued before any of them is summed
+        const double2* yr = Yr + (int64_t)b * 64 + lane;
+        const int64_t ys = (int64_t)nbk * 64;
+        const int pmax = b / rbk;
         int bj = g;
-        for (; bj + 7 * RED_G <= b; bj += 8 * RED_G) {
+        for (; bj + 7 * RED_G <= pmax; bj += 8 * RED_G) {
             double2 x[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = yr[(int64_t)(bj + u * RED_G) * 64];
+            for (int u = 0; u < 8; ++u) x[u] = yr[(bj + u * RED_G) * ys];
             acc0 = cadd(acc0, cadd(cadd(x[0], x[1]), cadd(x[2], x[3])));
             acc1 = cadd(acc1, cadd(cadd(x[4], x[5]), cadd(x[6], x[7])));
         }
-        for (; bj + 3 * RED_G <= b; bj += 4 * RED_G) {
-            const double2 x0 = yr[(int64_t)bj * 64], x1 = yr[(int64_t)(bj + RED_G) * 64];
-            const double2 x2 = yr[(int64_t)(bj + 2 * RED_G) * 64], x3 = yr[(int64_t)(bj + 3 * RED_G) * 64];
+        for (; bj + 3 * RED_G <= pmax; bj += 4 * RED_G) {
+            const double2 x0 = yr[bj * ys], x1 = yr[(bj + RED_G) * ys];
+            const double2 x2 = yr[(bj + 2 * RED_G) * ys], x3 = yr[(bj + 3 * RED_G) * ys];
             acc0 = cadd(acc0, cadd(x0, x1));
             acc1 = cadd(acc1, cadd(x2, x3));
         }
-        for (; bj <= b; bj += RED_G) acc0 = cadd(acc0, yr[(int64_t)bj * 64]);
-        const int nqb = (int)ceil_div64(nbk - b, ch);
+        for (; bj <= pmax; bj += RED_G) acc0 = cadd(acc0, yr[bj * ys]);
+        const int nqb = (int)ceil_div64(nbk - pmax * rbk, ch);
         const double2* yc = Yc + (int64_t)b * nq * 64 + lane;
         int q = g;
         for (; q + RED_G < nqb; q += 2 * RED_G) {
@@ -1000,23 +1034,25 @@ void kb_hemv(const HetrdSys* __restrict__ sys, int c, int nb)
     HetrdCol h;
     const HetrdSys s = sys[blockIdx.z];
     if (!batch_col(s, c, nb, h)) return;
+    constexpr int RB = U2 == 3 ? 4 : 1;
     const int nbk = tile_count(s.n) - ((c + 1) >> 6), nq = (int)ceil_div64(nbk, CH);
-    if ((int)blockIdx.x >= nq || (int)blockIdx.y >= nbk) return;
+    const int npb = (int)ceil_div64(nbk, RB);
+    if ((int)blockIdx.x >= nq || (int)blockIdx.y >= npb) return;
     const T* A = (const T*)s.A;
-    if (U2 == 2) hemv_bfly_body<T, CH, AUX>(s.n, c + 1, A, nullptr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y, h);
+    if (U2 >= 2) hemv_bfly_body<T, CH, RB, AUX>(s.n, c + 1, A, nullptr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y, npb, h);
     else hemv_body<T, CH, U2, AUX>(s.n, c + 1, A, nullptr, s.Yr, s.Yc, nbk, nq, blockIdx.x, blockIdx.y, h);
 }
 
 template <typename T>
 __global__ __launch_bounds__(1024) void kb_reduce(const HetrdSys* __restrict__ sys, int c, int nb,
-                                                  int ch)
+                                                  int ch, int rbk)
 {
     HetrdCol h;
     const HetrdSys s = sys[blockIdx.z];
     if (!batch_col(s, c, nb, h)) return;
     const int nbk = tile_count(s.n) - ((c + 1) >> 6), nq = (int)ceil_div64(nbk, ch);
     if ((int)blockIdx.x >= nbk) return;
-    reduce_body<T>(h, (const T*)s.A, s.Yr, s.Yc, nbk, nq, ch, blockIdx.x);
+    reduce_body<T>(h, (const T*)s.A, s.Yr, s.Yc, nbk, nq, ch, rbk, blockIdx.x);
 }
 
 // Column c closes the matrix's current panel when c = j0 + nbp - 1.
@@ -1077,10 +1113,10 @@ template <typename T>
 static void hemv_launch(hipStream_t st, int code, int n, int s0, const T* A, const double2* v,
                         double2* Yr, double2* Yc, const HetrdCol& h)
 {
-    const int ch = code & 31;
+    const int ch = hemv_chunk(code);
     const int nbk = hemv_nbk(n, s0);
-    const int nq = (int)ceil_div64(nbk, ch);
-#define HL(CC, UU, AX) hipLaunchKernelGGL((k_hemv_lower<T, CC, UU, AX>), dim3(nq, nbk), dim3(256), 0, st, n, s0, A, v, Yr, Yc, nbk, nq, h)
+    const int nq = (int)ceil_div64(nbk, ch), npb = (int)ceil_div64(nbk, hemv_rblk(code));
+#define HL(CC, UU, AX) hipLaunchKernelGGL((k_hemv_lower<T, CC, UU, AX>), dim3(nq, npb), dim3(256), 0, st, n, s0, A, v, Yr, Yc, nbk, nq, h)
     HEMV_DISPATCH(code, HL)
 #undef HL
 }
@@ -1176,7 +1212,7 @@ static hipError_t column_launch(hipStream_t st, int fp32, int c, int nb, int nac
                                 const void* d_sys, int code, hipEvent_t e0, hipEvent_t e1)
 {
     if (!hemv_code_valid(code) || nb < 1 || nb > 32) return hipErrorInvalidValue;
-    const int ch = code & 31;
+    const int ch = hemv_chunk(code);
     if (nact <= 0 || c >= n0 - 1) return hipSuccess;
     const HetrdSys* sys = (const HetrdSys*)d_sys;
     const int j0 = (c / nb) * nb, i = c - j0, m = n0 - j0;
@@ -1186,7 +1222,7 @@ static hipError_t column_launch(hipStream_t st, int fp32, int c, int nb, int nac
         hipLaunchKernelGGL(kb_colupd<float2>, dim3((unsigned)colupd_blocks(m, i), 1, z), dim3(256), 0, st, sys, c, nb);
     else
         hipLaunchKernelGGL(kb_colupd<double2>, dim3((unsigned)colupd_blocks(m, i), 1, z), dim3(256), 0, st, sys, c, nb);
-    const dim3 hg((unsigned)ceil_div64(nbk, ch), (unsigned)nbk, z);
+    const dim3 hg((unsigned)ceil_div64(nbk, ch), (unsigned)ceil_div64(nbk, hemv_rblk(code)), z);
     if (e0) (void)hipEventRecord(e0, st);
 #define KBF(CC, UU, AX) hipLaunchKernelGGL((kb_hemv<float2, CC, UU, AX>), hg, dim3(256), 0, st, sys, c, nb)
 #define KBD(CC, UU, AX) hipLaunchKernelGGL((kb_hemv<double2, CC, UU, AX>), hg, dim3(256), 0, st, sys, c, nb)
@@ -1194,8 +1230,9 @@ static hipError_t column_launch(hipStream_t st, int fp32, int c, int nb, int nac
 #undef KBF
 #undef KBD
     if (e1) (void)hipEventRecord(e1, st);
-    if (fp32) hipLaunchKernelGGL(kb_reduce<float2>, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch);
-    else hipLaunchKernelGGL(kb_reduce<double2>, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch);
+    const int rbk = hemv_rblk(code);
+    if (fp32) hipLaunchKernelGGL(kb_reduce<float2>, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch, rbk);
+    else hipLaunchKernelGGL(kb_reduce<double2>, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch, rbk);
     return hipGetLastError();
 }
 
