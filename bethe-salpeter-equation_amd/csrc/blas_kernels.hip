@@ -344,6 +344,129 @@ extern "C" hipError_t bse_launch_apply_q(hipStream_t st, int n, int nb, const do
 }
 
 // ---------------------------------------------------------------------------
+// The same application for a large matrix and at most 16 vectors, spread over
+// the whole device: per panel (V: m x nbp, ld m; X: the rows j0.. of the
+// vectors, ld ldx)
+//   (1) k_pq_dots:   partial P_g = V_g^H X_g over row blocks g of PQ_ROWS rows
+//   (2) k_pq_small:  W = T^(H) sum_g P_g (one workgroup; fixed summation order)
+//   (3) k_pq_update: X -= V W
+// Three short launches per panel instead of four GEMM launches; every pass
+// reads V once at streaming rate.
+// ---------------------------------------------------------------------------
+#define PQ_ROWS 256
+#define PQ_CHUNK 64
+#define PQ_MAXK 16
+__global__ __launch_bounds__(256) void k_pq_dots(int m, int nbp, int k, const double2* __restrict__ V,
+                                                 const double2* __restrict__ X, int64_t ldx,
+                                                 double2* __restrict__ P)
+{
+    __shared__ double2 sV[PQ_CHUNK][APQ_NB + 1];
+    __shared__ double2 sX[PQ_CHUNK][PQ_MAXK];
+    const int r0 = blockIdx.x * PQ_ROWS;
+    const int t = threadIdx.x;
+    double2 acc[2] = {cmk(0, 0), cmk(0, 0)};
+    for (int c0 = r0; c0 < min(m, r0 + PQ_ROWS); c0 += PQ_CHUNK) {
+        for (int e = t; e < PQ_CHUNK * APQ_NB; e += 256) {
+            const int rr = e & (PQ_CHUNK - 1), l = e / PQ_CHUNK;
+            const int r = c0 + rr;
+            sV[rr][l] = (r < m && l < nbp) ? V[r + (int64_t)l * m] : cmk(0, 0);
+        }
+        for (int e = t; e < PQ_CHUNK * PQ_MAXK; e += 256) {
+            const int rr = e & (PQ_CHUNK - 1), j = e / PQ_CHUNK;
+            const int r = c0 + rr;
+            sX[rr][j] = (r < m && j < k) ? X[r + (int64_t)j * ldx] : cmk(0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int o = t + 256 * u, l = o & 31, j = o >> 5;
+#pragma unroll 8
+            for (int rr = 0; rr < PQ_CHUNK; ++rr) cfmac(acc[u], sV[rr][l], sX[rr][j]);
+        }
+        __syncthreads();
+    }
+    P[(int64_t)blockIdx.x * 512 + t] = acc[0];
+    P[(int64_t)blockIdx.x * 512 + t + 256] = acc[1];
+}
+
+__global__ __launch_bounds__(512) void k_pq_small(int nbp, int k, int G, const double2* __restrict__ P,
+                                                  const double2* __restrict__ T, int ldt, int adjoint,
+                                                  double2* __restrict__ W)
+{
+    __shared__ double2 sW[512];
+    const int o = threadIdx.x, l = o & 31, j = o >> 5;
+    double2 a = cmk(0, 0), b = cmk(0, 0);
+    int g = 0;
+    for (; g + 1 < G; g += 2) {
+        a = cadd(a, P[(int64_t)g * 512 + o]);
+        b = cadd(b, P[(int64_t)(g + 1) * 512 + o]);
+    }
+    if (g < G) a = cadd(a, P[(int64_t)g * 512 + o]);
+    sW[o] = cadd(a, b);
+    __syncthreads();
+    double2 s = cmk(0, 0);
+    if (l < nbp && j < k) {
+        if (adjoint) {
+            for (int q = 0; q <= l; ++q) cfmac(s, T[q + (int64_t)l * ldt], sW[q + 32 * j]);
+        } else {
+            for (int q = l; q < nbp; ++q) cfma(s, T[l + (int64_t)q * ldt], sW[q + 32 * j]);
+        }
+    }
+    W[o] = s;
+}
+
+__global__ __launch_bounds__(256) void k_pq_update(int m, int nbp, int k, const double2* __restrict__ V,
+                                                   const double2* __restrict__ W, double2* __restrict__ X,
+                                                   int64_t ldx)
+{
+    __shared__ double2 sW[512];
+    for (int e = threadIdx.x; e < 512; e += 256) sW[e] = W[e];
+    __syncthreads();
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= m) return;
+    double2 v[APQ_NB];
+#pragma unroll
+    for (int l = 0; l < APQ_NB; ++l) v[l] = (l < nbp) ? V[r + (int64_t)l * m] : cmk(0, 0);
+    for (int j = 0; j < k; ++j) {
+        double2 s = cmk(0, 0);
+#pragma unroll
+        for (int l = 0; l < APQ_NB; ++l) cfma(s, v[l], sW[l + 32 * j]);
+        X[r + (int64_t)j * ldx] = csub(X[r + (int64_t)j * ldx], s);
+    }
+}
+
+// Workspace (complex elements) of bse_apply_q_thin for size n.
+extern "C" int64_t bse_apply_q_thin_workspace(int n) { return (ceil_div64(n, PQ_ROWS) + 1) * 512 + 512; }
+
+// X <- Q^H X (adjoint = 1) or Q X (adjoint = 0) for k <= 16 columns (ld ldx),
+// panels of the packed reflector store as in bse_launch_apply_q.  ws: at
+// least bse_apply_q_thin_workspace(n) complex elements.
+extern "C" hipError_t bse_apply_q_thin(hipStream_t st, int n, int nb, const double2* Vstore,
+                                       const double2* Tall, int k, double2* X, int64_t ldx, int adjoint,
+                                       double2* ws)
+{
+    if (nb > APQ_NB || k <= 0 || k > PQ_MAXK) return hipErrorInvalidValue;
+    if (n < 2) return hipSuccess;
+    const int npan = (int)ceil_div64(n - 1, nb);
+    double2* W = ws;
+    double2* P = ws + 512;
+    for (int pp = 0; pp < npan; ++pp) {
+        const int p = adjoint ? pp : npan - 1 - pp;
+        const int j0 = p * nb, m = n - j0, nbp = min(nb, n - 1 - j0);
+        const double2* V = Vstore + (int64_t)nb * ((int64_t)p * n - (int64_t)nb * p * (p - 1) / 2);
+        const double2* T = Tall + (size_t)p * nb * nb;
+        const int G = (int)ceil_div64(m, PQ_ROWS);
+        hipLaunchKernelGGL(k_pq_dots, dim3((unsigned)G), dim3(256), 0, st, m, nbp, k, V,
+                           (const double2*)(X + j0), ldx, P);
+        hipLaunchKernelGGL(k_pq_small, dim3(1), dim3(512), 0, st, nbp, k, G, (const double2*)P, T, nb,
+                           adjoint, W);
+        hipLaunchKernelGGL(k_pq_update, dim3((unsigned)ceil_div64(m, 256)), dim3(256), 0, st, m, nbp, k, V,
+                           (const double2*)W, X + j0, ldx);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Copies, fills and time stamps as kernels, so that sequences recorded into a
 // graph contain kernel nodes only.
 // ---------------------------------------------------------------------------

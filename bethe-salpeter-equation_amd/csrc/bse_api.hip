@@ -511,7 +511,10 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
         double2* w1 = BUF(double2, S_W1, (size_t)nb * 64);
         double2* w2 = BUF(double2, S_W2, (size_t)nb * 64);
         HCK(hipMemcpyAsync(y, rho, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
-        for (int p = 0; p < npan; ++p) {
+        if (bse_apply_q_thin_workspace(n) <= sk_el) {
+            // one vector: the device-wide thin application (three short kernels per panel)
+            HCK(bse_apply_q_thin(st, n, nb, Vst, Tall, 1, y, n, 1, skw));
+        } else for (int p = 0; p < npan; ++p) {
             const int j0 = p * nb, m = n - j0, nbp = std::min(nb, n - 1 - j0);
             const double2* Vp = Vst + voff[p];
             const double2* Tp = Tall + (size_t)p * nb * nb;
@@ -564,7 +567,9 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
 
     // ---------------- K7: back-transform Z <- Q Z
     stage_begin(ctx, ST_BT);
-    if (nstore > 0) {
+    if (nstore > 0 && nstore <= 16 && bse_apply_q_thin_workspace(n) <= sk_el) {
+        HCK(bse_apply_q_thin(st, n, nb, Vst, Tall, nstore, Z, n, 0, skw));
+    } else if (nstore > 0) {
         double2* w1 = BUF(double2, S_W1, (size_t)nb * nstore);
         double2* w2 = BUF(double2, S_W2, (size_t)nb * nstore);
         for (int p = npan - 1; p >= 0; --p) {

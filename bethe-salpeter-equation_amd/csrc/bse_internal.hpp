@@ -53,6 +53,9 @@ hipError_t bse_launch_tri_vectors_dev(hipStream_t, int, const int4*, const doubl
                                       const int*, double*, double*, int);
 hipError_t bse_launch_apply_q(hipStream_t, int, int, const double2*, const double2*, int, double2*,
                               int64_t, int);
+int64_t bse_apply_q_thin_workspace(int);
+hipError_t bse_apply_q_thin(hipStream_t, int, int, const double2*, const double2*, int, double2*, int64_t,
+                            int, double2*);
 hipError_t bse_launch_copy_words(hipStream_t, int64_t, const void*, void*);
 hipError_t bse_launch_zero_words(hipStream_t, int64_t, void*);
 hipError_t bse_launch_stamp(hipStream_t, unsigned long long*);

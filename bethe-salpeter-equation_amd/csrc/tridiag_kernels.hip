@@ -4,6 +4,9 @@
 // Eigenvalues: Sturm-count bisection (Golub & Van Loan, sec. 8.4.1) run as an
 // 8-way multisection -- 8 lanes of a wavefront evaluate the count at 8
 // interior points of the current interval, so each round narrows it 9-fold.
+// The recurrence divides by a reciprocal refined with two Newton steps
+// instead of the correctly rounded division sequence (a relative error of a
+// few ulps per step, well inside the backward error the count tolerates).
 // The matrix is first split at negligible off-diagonals
 // (e_i^2 <= eps^2 |d_i d_{i+1}| + safmin) and each unreduced block is solved
 // on its own, so exactly degenerate eigenvalues of the BSE matrix (which show
@@ -18,6 +21,15 @@
 #include "bse_common.hpp"
 #include <cfloat>
 
+// 1/b from the hardware reciprocal and two Newton steps (|b| >= pivmin is a
+// normal number here).
+__device__ inline double fast_rcp(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    r = fma(fma(-b, r, 1.0), r, r);
+    return r;
+}
+
 // Count of eigenvalues of block [bs, bs+nbk) strictly below x.
 __device__ inline int sturm_count(const double* __restrict__ d, const double* __restrict__ e2,
                                   int bs, int nbk, double x, double pivmin)
@@ -26,7 +38,7 @@ __device__ inline int sturm_count(const double* __restrict__ d, const double* __
     if (fabs(q) <= pivmin) q = -pivmin;
     int cnt = q < 0.0;
     for (int i = 1; i < nbk; ++i) {
-        q = (d[bs + i] - x) - e2[bs + i - 1] / q;
+        q = (d[bs + i] - x) - e2[bs + i - 1] * fast_rcp(q);
         if (fabs(q) <= pivmin) q = -pivmin;
         cnt += q < 0.0;
     }
