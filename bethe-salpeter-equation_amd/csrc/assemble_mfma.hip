@@ -22,6 +22,7 @@
 // (rank-norb, sum_j PV[r,j] conj(P[s,j])) and the diagonal are added in the
 // epilogue with VALU FP64 arithmetic.
 #include "bse_common.hpp"
+#include <algorithm>
 
 #define PREC_F64 0
 #define PREC_F32 1
@@ -230,6 +231,224 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(MfArgs a, typename mf_t<P
             if (grow >= gcol) cplx<OUT>::store(H + tix(grow, gcol, a.nt), h);
         }
     }
+}
+
+extern "C" int bse_assemble_mfma_tk(int nv, int nc);
+extern "C" int bse_assemble_mfma_supported(int nv, int nc, int norb);
+// ---------------------------------------------------------------------------
+// Matrix-free FP64 product Y = H X for k <= 16 vectors on matrix cores (the
+// FP64 refinement of the mixed path).  Workgroup (ti, g) owns the TR rows of
+// k-point block ti and the column blocks tj = g, g + ncg, ...: for each it
+// forms the TR x TR tile of H exactly as the FP64 assembly does (same two
+// chained MFMA contractions per (k, k') pair, exchange and diagonal in FP64)
+// but into LDS, and multiplies it by the tile's rows of X with
+// v_mfma_f64_16x16x4 (wave w: rows 16w .. 16w+15, all 16 vectors):
+//   Yr += Hr Xr - Hi Xi,   Yi += Hr Xi + Hi Xr.
+// Partial results per column group go to Yp[g] (N x 16); k_apply_sum adds
+// them in a fixed order.  Requires TR <= 64 (bse_assemble_mfma_tk).
+// ---------------------------------------------------------------------------
+#define AM_K 16
+__global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __restrict__ X, int64_t ldx, int k,
+                                                    int ncg, double2* __restrict__ Yp)
+{
+    extern __shared__ double2 smem[];
+    const int B = a.nc * a.nv, ns = a.nv + a.nc, no = a.norb, TK = a.TK;
+    const int TR = TK * B;
+    const int ti = blockIdx.x, g = blockIdx.y;
+    const int nt = (int)ceil_div64(a.nk, TK);
+    const int k0 = ti * TK;
+    double2* sCr = smem;                       // [TK][ns][8]
+    double2* sCc = sCr + TK * ns * 8;          // [TK][ns][8]
+    double2* sPV = sCc + TK * ns * 8;          // [TR][8]
+    double2* sP = sPV + TR * 8;                // [TR][8]
+    double* sT = (double*)(sP + TR * 8);       // per-wave 16x17 transposition
+    double* sHr = sT + 4 * 272;                // [64][65]
+    double* sHi = sHr + 64 * 65;
+    double* sXr = sHi + 64 * 65;               // [64][16]
+    double* sXi = sXr + 64 * 16;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int e = tid; e < TK * ns * 8; e += 256) {
+        const int i = e & 7, sl = (e >> 3) % ns, kk = e / (8 * ns);
+        const int kq = k0 + kk;
+        sCr[e] = (kq < a.nk && i < no) ? a.coef[((size_t)kq * ns + sl) * no + i] : cmk(0, 0);
+    }
+    for (int e = tid; e < TR * 8; e += 256) {
+        const int i = e & 7, rr = e >> 3;
+        const int64_t row = (int64_t)k0 * B + rr;
+        sPV[e] = (row < a.N && i < no) ? a.PV[row * no + i] : cmk(0, 0);
+    }
+    // rows and columns of the 64 x 64 LDS tile beyond TR stay zero
+    for (int e = tid; e < 64 * 65; e += 256) { sHr[e] = 0.0; sHi[e] = 0.0; }
+    double* tw = sT + wv * 272;
+    const int li = lane & 15, lq = lane >> 4;
+    const int w2 = 2 * a.n2 - 1;
+    mf_d4 yr = mf_d4{0, 0, 0, 0}, yi = mf_d4{0, 0, 0, 0};
+    for (int tj = g; tj < nt; tj += ncg) {
+        const int kp0 = tj * TK;
+        __syncthreads();   // previous tile fully consumed
+        for (int e = tid; e < TK * ns * 8; e += 256) {
+            const int i = e & 7, sl = (e >> 3) % ns, kk = e / (8 * ns);
+            const int kp = kp0 + kk;
+            sCc[e] = (kp < a.nk && i < no) ? a.coef[((size_t)kp * ns + sl) * no + i] : cmk(0, 0);
+        }
+        for (int e = tid; e < TR * 8; e += 256) {
+            const int i = e & 7, rr = e >> 3;
+            const int64_t col = (int64_t)kp0 * B + rr;
+            sP[e] = (col < a.N && i < no) ? a.P[col * no + i] : cmk(0, 0);
+        }
+        for (int e = tid; e < 64 * AM_K; e += 256) {
+            const int rr = e >> 4, j = e & 15;
+            const int64_t col = (int64_t)kp0 * B + rr;
+            double2 x = cmk(0, 0);
+            if (rr < TR && col < a.N && j < k) x = X[col + (int64_t)j * ldx];
+            sXr[e] = x.x;
+            sXi[e] = x.y;
+        }
+        __syncthreads();
+        for (int p = wv; p < TK * TK; p += 4) {
+            const int kk = p / TK, kq = p % TK;
+            const int kx = k0 + kk, kp = kp0 + kq;
+            const bool pok = kx < a.nk && kp < a.nk;
+            const double2* wq = a.wtab + (pok ? ((size_t)((kx / a.n2 - kp / a.n2 + a.n1 - 1) * w2 +
+                                                          (kx % a.n2 - kp % a.n2 + a.n2 - 1))) * no * no : 0);
+            double a1[4], b1[4];
+            const int rho = li, c = rho / a.nc, cp = rho % a.nc;
+            const bool rok = rho < a.nc * a.nc;
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                const int kap = 4 * ks + lq;
+                const int i = kap & 7;
+                double2 x = cmk(0, 0);
+                if (pok && rok && i < no)
+                    x = cmulc(sCr[(kk * ns + a.nv + c) * 8 + i], sCc[(kq * ns + a.nv + cp) * 8 + i]);
+                a1[ks] = kap < 8 ? x.x : x.y;
+                const int jp = li;
+                const int jj = jp & 7, ii = kap & 7;
+                double wv2 = 0.0;
+                if (pok && ii < no && jj < no) {
+                    const double2 w = wq[ii * no + jj];
+                    if (kap < 8) wv2 = (jp < 8) ? w.x : w.y;
+                    else wv2 = (jp < 8) ? -w.y : w.x;
+                }
+                b1[ks] = wv2;
+            }
+            mf_d4 d1 = mm16(a1, b1, mf_d4{0, 0, 0, 0});
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int reg = 0; reg < 4; ++reg) tw[mfma16_row_f64(lane, reg) * 17 + li] = d1[reg];
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            double a2[4], b2r[4], b2i[4];
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) a2[ks] = tw[li * 17 + 4 * ks + lq];
+            __builtin_amdgcn_wave_barrier();
+            const int sg = li, v = sg / a.nv, vp = sg % a.nv;
+            const bool sok = sg < a.nv * a.nv;
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                const int kap = 4 * ks + lq;
+                const int j = kap & 7;
+                double2 y = cmk(0, 0);
+                if (pok && sok && j < no)
+                    y = cmul(sCr[(kk * ns + v) * 8 + j], cconj(sCc[(kq * ns + vp) * 8 + j]));
+                b2r[ks] = kap < 8 ? y.x : -y.y;
+                b2i[ks] = kap < 8 ? y.y : y.x;
+            }
+            const mf_d4 dr = mm16(a2, b2r, mf_d4{0, 0, 0, 0});
+            const mf_d4 di = mm16(a2, b2i, mf_d4{0, 0, 0, 0});
+#pragma unroll
+            for (int reg = 0; reg < 4; ++reg) {
+                const int rh = mfma16_row_f64(lane, reg);
+                if (rh >= a.nc * a.nc || !sok) continue;
+                const int cc = rh / a.nc, ccp = rh % a.nc;
+                const int rl = kk * B + cc * a.nv + v, cl = kq * B + ccp * a.nv + vp;
+                const int64_t grow = (int64_t)k0 * B + rl, gcol = (int64_t)kp0 * B + cl;
+                double2 hv = cmk(0, 0);
+                if (pok) {
+                    hv = cmk(a.alpha * dr[reg], a.alpha * di[reg]);
+                    double2 x = cmk(0, 0);
+                    for (int j = 0; j < no; ++j) cfma(x, sPV[rl * 8 + j], cconj(sP[cl * 8 + j]));
+                    hv = cadd(hv, cscale(x, a.beta));
+                    if (grow == gcol) { hv.x += a.dE[grow]; hv.y = 0.0; }
+                }
+                sHr[rl * 65 + cl] = hv.x;
+                sHi[rl * 65 + cl] = hv.y;
+            }
+        }
+        __syncthreads();
+        // Y rows 16 wv .. 16 wv + 15 += H(rows, 0:64) X(0:64, 0:16)
+        const int hr0 = (16 * wv + li) * 65;
+#pragma unroll 4
+        for (int kc = 0; kc < 64; kc += 4) {
+            const double hr = sHr[hr0 + kc + lq], hi = sHi[hr0 + kc + lq];
+            const double xr = sXr[(kc + lq) * 16 + li], xi = sXi[(kc + lq) * 16 + li];
+            yr = __builtin_amdgcn_mfma_f64_16x16x4f64(hr, xr, yr, 0, 0, 0);
+            yr = __builtin_amdgcn_mfma_f64_16x16x4f64(-hi, xi, yr, 0, 0, 0);
+            yi = __builtin_amdgcn_mfma_f64_16x16x4f64(hr, xi, yi, 0, 0, 0);
+            yi = __builtin_amdgcn_mfma_f64_16x16x4f64(hi, xr, yi, 0, 0, 0);
+        }
+    }
+    // lane holds Y[row 16 wv + mfma16_row_f64(lane, reg)][vector li]
+#pragma unroll
+    for (int reg = 0; reg < 4; ++reg) {
+        const int rl = 16 * wv + mfma16_row_f64(lane, reg);
+        const int64_t grow = (int64_t)k0 * B + rl;
+        if (rl < TR && grow < a.N) Yp[((int64_t)g * a.N + grow) * AM_K + li] = cmk(yr[reg], yi[reg]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_apply_sum(int64_t N, int k, int ncg, const double2* __restrict__ Yp,
+                                                   double2* __restrict__ Y, int64_t ldy)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= N * AM_K) return;
+    const int64_t r = e / AM_K;
+    const int j = (int)(e % AM_K);
+    if (j >= k) return;
+    double2 s = cmk(0, 0);
+    for (int gg = 0; gg < ncg; ++gg) s = cadd(s, Yp[(int64_t)gg * N * AM_K + e]);
+    Y[r + (int64_t)j * ldy] = s;
+}
+
+static size_t apply_mfma_lds_bytes(int TK, int nv, int nc) {
+    const int ns = nv + nc, B = nv * nc;
+    return ((size_t)2 * TK * ns * 8 + (size_t)2 * TK * B * 8) * sizeof(double2) +
+           ((size_t)4 * 272 + 2 * 64 * 65 + 2 * 64 * 16) * sizeof(double);
+}
+
+// Column groups per row block of the matrix-core product (about four
+// workgroups per compute unit), and the size of its partial buffer.
+extern "C" int bse_apply_mfma_groups(int nk, int nv, int nc) {
+    const int nt = (int)ceil_div64(nk, bse_assemble_mfma_tk(nv, nc));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(nt, ceil_div64(1024, nt)));
+}
+extern "C" int64_t bse_apply_mfma_workspace(int nk, int nv, int nc) {
+    return (int64_t)bse_apply_mfma_groups(nk, nv, nc) * nk * nv * nc * AM_K;
+}
+
+extern "C" hipError_t bse_launch_apply_mfma(hipStream_t st, int nk, int n1, int n2, int nv, int nc,
+                                            int norb, const double2* coef, const double2* wtab,
+                                            const double2* PV, const double2* P, const double* dE,
+                                            double alpha, double beta, int k, const double2* X,
+                                            int64_t ldx, double2* Y, int64_t ldy, double2* ws)
+{
+    if (!bse_assemble_mfma_supported(nv, nc, norb) || k <= 0 || k > AM_K) return hipErrorInvalidValue;
+    MfArgs a;
+    a.nk = nk; a.n1 = n1; a.n2 = n2; a.nv = nv; a.nc = nc; a.norb = norb;
+    a.TK = bse_assemble_mfma_tk(nv, nc);
+    if (a.TK * nv * nc > 64) return hipErrorInvalidValue;
+    a.N = (int64_t)nk * nv * nc;
+    a.nt = tile_count(a.N);
+    a.coef = coef; a.wtab = wtab; a.PV = PV; a.P = P; a.dE = dE; a.alpha = alpha; a.beta = beta;
+    const int nt = (int)ceil_div64(nk, a.TK);
+    const int ncg = bse_apply_mfma_groups(nk, nv, nc);
+    const size_t lds = apply_mfma_lds_bytes(a.TK, nv, nc);
+    hipLaunchKernelGGL(k_apply_mfma, dim3((unsigned)nt, (unsigned)ncg), dim3(256), lds, st, a, X, ldx, k, ncg,
+                       ws);
+    hipLaunchKernelGGL(k_apply_sum, dim3((unsigned)ceil_div64(a.N * AM_K, 256)), dim3(256), 0, st, a.N, k,
+                       ncg, (const double2*)ws, Y, ldy);
+    return hipGetLastError();
 }
 
 extern "C" int bse_assemble_mfma_supported(int nv, int nc, int norb) {

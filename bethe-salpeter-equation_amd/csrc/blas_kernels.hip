@@ -353,7 +353,7 @@ extern "C" hipError_t bse_launch_apply_q(hipStream_t st, int n, int nb, const do
 // Three short launches per panel instead of four GEMM launches; every pass
 // reads V once at streaming rate.
 // ---------------------------------------------------------------------------
-#define PQ_ROWS 256
+#define PQ_ROWS 128
 #define PQ_CHUNK 64
 #define PQ_MAXK 16
 __global__ __launch_bounds__(256) void k_pq_dots(int m, int nbp, int k, const double2* __restrict__ V,
@@ -389,21 +389,31 @@ __global__ __launch_bounds__(256) void k_pq_dots(int m, int nbp, int k, const do
     P[(int64_t)blockIdx.x * 512 + t + 256] = acc[1];
 }
 
-__global__ __launch_bounds__(512) void k_pq_small(int nbp, int k, int G, const double2* __restrict__ P,
-                                                  const double2* __restrict__ T, int ldt, int adjoint,
-                                                  double2* __restrict__ W)
+__global__ __launch_bounds__(1024) void k_pq_small(int nbp, int k, int G, const double2* __restrict__ P,
+                                                   const double2* __restrict__ T, int ldt, int adjoint,
+                                                   double2* __restrict__ W)
 {
-    __shared__ double2 sW[512];
-    const int o = threadIdx.x, l = o & 31, j = o >> 5;
-    double2 a = cmk(0, 0), b = cmk(0, 0);
-    int g = 0;
-    for (; g + 1 < G; g += 2) {
-        a = cadd(a, P[(int64_t)g * 512 + o]);
-        b = cadd(b, P[(int64_t)(g + 1) * 512 + o]);
+    // two threads per output, each summing every other partial with eight
+    // independent loads in flight; fixed order, identical on every run
+    __shared__ double2 sW[512], sH[512];
+    const int o = threadIdx.x & 511, half = threadIdx.x >> 9, l = o & 31, j = o >> 5;
+    double2 acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = cmk(0, 0);
+    int g = half;
+    for (; g + 14 < G; g += 16) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] = cadd(acc[u], P[(int64_t)(g + 2 * u) * 512 + o]);
     }
-    if (g < G) a = cadd(a, P[(int64_t)g * 512 + o]);
-    sW[o] = cadd(a, b);
+    double2 rem = cmk(0, 0);
+    for (; g < G; g += 2) rem = cadd(rem, P[(int64_t)g * 512 + o]);
+    const double2 part = cadd(cadd(cadd(cadd(acc[0], acc[1]), cadd(acc[2], acc[3])),
+                                   cadd(cadd(acc[4], acc[5]), cadd(acc[6], acc[7]))), rem);
+    if (half) sH[o] = part;
     __syncthreads();
+    if (!half) sW[o] = cadd(part, sH[o]);
+    __syncthreads();
+    if (half) return;
     double2 s = cmk(0, 0);
     if (l < nbp && j < k) {
         if (adjoint) {
@@ -458,7 +468,7 @@ extern "C" hipError_t bse_apply_q_thin(hipStream_t st, int n, int nb, const doub
         const int G = (int)ceil_div64(m, PQ_ROWS);
         hipLaunchKernelGGL(k_pq_dots, dim3((unsigned)G), dim3(256), 0, st, m, nbp, k, V,
                            (const double2*)(X + j0), ldx, P);
-        hipLaunchKernelGGL(k_pq_small, dim3(1), dim3(512), 0, st, nbp, k, G, (const double2*)P, T, nb,
+        hipLaunchKernelGGL(k_pq_small, dim3(1), dim3(1024), 0, st, nbp, k, G, (const double2*)P, T, nb,
                            adjoint, W);
         hipLaunchKernelGGL(k_pq_update, dim3((unsigned)ceil_div64(m, 256)), dim3(256), 0, st, m, nbp, k, V,
                            (const double2*)W, X + j0, ldx);

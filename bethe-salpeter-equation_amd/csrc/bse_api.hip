@@ -603,6 +603,15 @@ static int apply_H(bse_ctx* ctx, int k, const double2* X, int64_t ldx, double2* 
     const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
     const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
     const double beta = s.f_s * s.exchange_scale / (ctx->area * ctx->nk);
+    if (k <= 16 && !ctx->force_valu_assembly && bse_assemble_mfma_supported(s.nv, s.nc, ctx->norb)) {
+        // matrix cores: tiles of H formed in LDS and multiplied at once
+        double2* ws = BUF(double2, S_APPLYP, bse_apply_mfma_workspace(ctx->nk, s.nv, s.nc));
+        HCK(bse_launch_apply_mfma(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb, coef,
+                                  (const double2*)ctx->bufs[S_WTAB].p, (const double2*)ctx->bufs[S_PV].p,
+                                  (const double2*)ctx->bufs[S_P].p, (const double*)ctx->bufs[S_DE].p,
+                                  alpha, beta, k, X, ldx, Y, ldy, ws));
+        return BSE_OK;
+    }
     HCK(hipMemsetAsync(Y, 0, sizeof(double2) * ldy * k, ctx->st));
     HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb,
                          bse_apply_pick_tk(s.nv, s.nc, ctx->norb), coef, coef, coef, coef,

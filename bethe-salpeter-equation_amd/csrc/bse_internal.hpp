@@ -86,6 +86,11 @@ hipError_t bse_launch_rows_from_reference(hipStream_t, int64_t, int, int, int, c
                                           int64_t, double2*, int64_t);
 int bse_apply_pick_tk(int, int, int);
 int bse_assemble_mfma_supported(int, int, int);
+int bse_apply_mfma_groups(int, int, int);
+int64_t bse_apply_mfma_workspace(int, int, int);
+hipError_t bse_launch_apply_mfma(hipStream_t, int, int, int, int, int, int, const double2*, const double2*,
+                                 const double2*, const double2*, const double*, double, double, int,
+                                 const double2*, int64_t, double2*, int64_t, double2*);
 hipError_t bse_launch_assemble_mfma(hipStream_t, int, int, int, int, int, int, int, int64_t,
                                     const double2*, const double2*, const double2*, const double2*,
                                     const double*, double, double, void*);
@@ -166,7 +171,7 @@ enum Slot {
     S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_ALPHA2,
     S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL,
     S_GDHR, S_GDEPS, S_GDCOEF, S_GDP, S_GDPV, S_GDRHO, S_GDDE, S_GDW, S_GY, S_GOUT,
-    S_SGFB, S_SGFA, S_SGX, S_SGU, S_SGB, S_SGV, S_SGPART, S_NSLOTS
+    S_SGFB, S_SGFA, S_SGX, S_SGU, S_SGB, S_SGV, S_SGPART, S_APPLYP, S_NSLOTS
 };
 
 int fail(bse_ctx* c, int code, const std::string& msg) {
