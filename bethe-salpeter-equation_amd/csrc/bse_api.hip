@@ -270,6 +270,11 @@ extern "C" int bse_get_dipoles(bse_ctx* ctx, double* rho) {
 // type FP64 (fp32 = 0) or FP32 complex.  rho (may be null): contraction vector
 // for the oscillator strengths.
 // ---------------------------------------------------------------------------
+static bool hemv_code_ok(int code) {
+    const int ch = code & 31;
+    return code > 0 && code < 256 && (ch == 4 || ch == 8 || ch == 16);
+}
+
 // ---------------------------------------------------------------------------
 // Autotuner for the tridiagonalisation's Hermitian matrix-vector kernel: for
 // every floor(log2(nt)) bucket up to the current size that is not tuned yet,
@@ -285,8 +290,15 @@ extern "C" int bse_get_dipoles(bse_ctx* ctx, double* rho) {
 // ---------------------------------------------------------------------------
 int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, double2* v, double2* Yr,
                   double2* Yc) {
-    if (!ctx->autotune || n < 128) return BSE_OK;
     unsigned char* tab = ctx->hemv_ch[fp32];
+    // BSE_HEMV_CODE=<code> pins one variant for every trailing size (for
+    // measurements of a single variant)
+    const char* hc = getenv("BSE_HEMV_CODE");
+    if (hc && hemv_code_ok(atoi(hc))) {
+        for (int b = 0; b < BSE_HEMV_BUCKETS; ++b) tab[b] = (unsigned char)atoi(hc);
+        return BSE_OK;
+    }
+    if (!ctx->autotune || n < 128) return BSE_OK;
     hipStream_t st = ctx->st;
     bool zeroed = false;
     for (int b = 6; b < BSE_HEMV_BUCKETS && (1 << b) <= n - 1; ++b) {

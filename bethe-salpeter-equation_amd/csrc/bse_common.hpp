@@ -231,6 +231,34 @@ __device__ __attribute__((always_inline)) inline void larft_body(int nb, const d
                                   const double2* __restrict__ G, double2* __restrict__ T, int ldt,
                                   double2* col)
 {
+    if (nb <= 32) {
+        // the recurrence runs in LDS (global round trips per column would
+        // dominate); T is written out once at the end
+        __shared__ double2 sT[32][33], sG[32][33], stau[32];
+        for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+            const int r = e & 31, c = e >> 5;
+            sT[r][c] = cmk(0, 0);
+            sG[r][c] = (r < nb && c < nb) ? G[r + c * 64] : cmk(0, 0);
+        }
+        if ((int)threadIdx.x < nb) stau[threadIdx.x] = tau[threadIdx.x];
+        __syncthreads();
+        for (int i = 0; i < nb; ++i) {
+            const double2 ti = stau[i];
+            const int r = threadIdx.x;
+            double2 cv = cmk(0, 0);
+            if (r < i) {
+                double2 sacc = cmk(0, 0);
+                for (int l = r; l < i; ++l) cfma(sacc, sT[r][l], sG[l][i]);
+                cv = cscale(cmul(ti, sacc), -1.0);
+            }
+            __syncthreads();
+            if (r < i) sT[r][i] = cv;
+            if (r == 0) sT[i][i] = ti;
+            __syncthreads();
+        }
+        for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) T[(e % nb) + (e / nb) * ldt] = sT[e % nb][e / nb];
+        return;
+    }
     for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) T[(e % nb) + (e / nb) * ldt] = cmk(0, 0);
     __syncthreads();
     for (int i = 0; i < nb; ++i) {
