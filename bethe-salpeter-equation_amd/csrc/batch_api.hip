@@ -300,17 +300,17 @@ static int batch_autotune(bse_ctx* ctx, BatchState* b) {
     b->her2k_valu = 0;
     if (ctx->autotune && n > nb) {
         float best = 0.f;
-        for (int valu = 0; valu < 2; ++valu) {
-            HCK(bse_launch_her2k(st, b->fp32, valu, n, nb, d.H, d.Vst, d.Wp, N, 0, nb, d.alpha2));
+        for (int kind : {0, 1, 2}) {
+            HCK(bse_launch_her2k(st, b->fp32, kind, n, nb, d.H, d.Vst, d.Wp, N, 0, nb, d.alpha2));
             HCK(hipEventRecord(ctx->tev[0], st));
             for (int r = 0; r < 5; ++r)
-                HCK(bse_launch_her2k(st, b->fp32, valu, n, nb, d.H, d.Vst, d.Wp, N, 0, nb, d.alpha2));
+                HCK(bse_launch_her2k(st, b->fp32, kind, n, nb, d.H, d.Vst, d.Wp, N, 0, nb, d.alpha2));
             HCK(hipEventRecord(ctx->tev[1], st));
             HCK(hipEventSynchronize(ctx->tev[1]));
             float ms = 0.f;
             HCK(hipEventElapsedTime(&ms, ctx->tev[0], ctx->tev[1]));
-            if (valu == 0 || ms < 0.97f * best) {
-                if (valu == 1) b->her2k_valu = 1;
+            if (kind == 0 || ms < 0.97f * best) {
+                b->her2k_valu = kind;
                 best = ms;
             }
         }
@@ -358,7 +358,7 @@ static int batch_capture(bse_ctx* ctx, BatchState* b) {
             const int ch = hemv_ch_table_lookup(ctx->hemv_ch[b->fp32], n0 - c - 1);
             if (c == 0) b->ch_top = ch;
             BCK(bse_hetrd_batch_column(st, b->fp32, c, nb, nact, n0, b->d_hsys, ch));
-            int64_t max_tiles = 0;
+            int64_t max_nt = 0;
             bool any_end = false;
             for (int s = 0; s < nact; ++s) {
                 const int p = c / nb, j0 = p * nb, nbp = std::min(nb, nsz[s] - 1 - j0);
@@ -366,9 +366,9 @@ static int batch_capture(bse_ctx* ctx, BatchState* b) {
                 any_end = true;
                 if (j0 + nbp >= nsz[s]) continue;
                 const int64_t nT = tile_count(nsz[s]) - ((j0 + nbp) >> 6);
-                max_tiles = std::max(max_tiles, nT * (nT + 1) / 2);
+                max_nt = std::max(max_nt, nT);
             }
-            if (any_end) BCK(bse_hetrd_batch_panel(st, b->fp32, c, nb, nact, b->d_hsys, max_tiles, b->her2k_valu));
+            if (any_end) BCK(bse_hetrd_batch_panel(st, b->fp32, c, nb, nact, b->d_hsys, max_nt, b->her2k_valu));
             for (int s = 0; s < nact; ++s) {
                 if (nsz[s] - 2 != c) continue;
                 hipEvent_t fork = b->evs[ei++];

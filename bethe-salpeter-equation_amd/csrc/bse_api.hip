@@ -330,15 +330,19 @@ int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, double2* v, doub
     return BSE_OK;
 }
 
-// Trailing-update kernel choice (FP64 MFMA or FP64 VALU, which have the same
-// peak on this part): both timed once per context and storage precision on a
-// scratch matrix carved from the hemv partial buffer (contents irrelevant).
-// BSE_HER2K=valu / mfma forces the choice.
+// Trailing-update kernel choice (FP64 VALU, FP64 MFMA on 64 x 64 tiles or on
+// 128 x 128 macro tiles; VALU and MFMA have the same FP64 peak on this part):
+// all timed once per context and storage precision on a scratch matrix carved
+// from the hemv partial buffer (contents irrelevant).  BSE_HER2K=valu / mfma /
+// mfma2 forces the choice.
 static int autotune_her2k(bse_ctx* ctx, int fp32, void* scratch, size_t scratch_bytes,
                           double2* V, double2* W, int64_t vw_elems, double2* alpha2) {
     if (ctx->her2k_valu[fp32] >= 0) return BSE_OK;
     const char* hv = getenv("BSE_HER2K");
-    if (hv && (hv[0] == 'v' || hv[0] == 'm')) { ctx->her2k_valu[fp32] = hv[0] == 'v'; return BSE_OK; }
+    if (hv && (!strcmp(hv, "valu") || !strcmp(hv, "mfma") || !strcmp(hv, "mfma2"))) {
+        ctx->her2k_valu[fp32] = !strcmp(hv, "valu") ? 1 : (!strcmp(hv, "mfma2") ? 2 : 0);
+        return BSE_OK;
+    }
     const int nb = kPanel;
     const size_t es = fp32 ? sizeof(float2) : sizeof(double2);
     // a tiled n2 x n2 matrix updated below row and column nb (one panel)
@@ -353,15 +357,15 @@ static int autotune_her2k(bse_ctx* ctx, int fp32, void* scratch, size_t scratch_
     HCK(hipMemsetAsync(alpha2, 0, sizeof(double2) * 64, st));
     float best = 0.f;
     int choice = 1;
-    for (int valu = 1; valu >= 0; --valu) {
-        HCK(bse_launch_her2k(st, fp32, valu, n2, nb, scratch, V, W, n2, 0, nb, alpha2));
+    for (int kind : {1, 2, 0}) {
+        HCK(bse_launch_her2k(st, fp32, kind, n2, nb, scratch, V, W, n2, 0, nb, alpha2));
         HCK(hipEventRecord(ctx->tev[0], st));
-        for (int r = 0; r < 4; ++r) HCK(bse_launch_her2k(st, fp32, valu, n2, nb, scratch, V, W, n2, 0, nb, alpha2));
+        for (int r = 0; r < 4; ++r) HCK(bse_launch_her2k(st, fp32, kind, n2, nb, scratch, V, W, n2, 0, nb, alpha2));
         HCK(hipEventRecord(ctx->tev[1], st));
         HCK(hipEventSynchronize(ctx->tev[1]));
         float ms = 0.f;
         HCK(hipEventElapsedTime(&ms, ctx->tev[0], ctx->tev[1]));
-        if (valu == 1 || ms < 0.97f * best) { best = ms; choice = valu; }
+        if (kind == 1 || ms < 0.97f * best) { best = ms; choice = kind; }
     }
     ctx->her2k_valu[fp32] = choice;
     return BSE_OK;
@@ -417,7 +421,7 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
     }
     double* hsys = BUF(double, S_HSYS, 32);   // one HetrdSys (device copy)
     HCK(bse_hetrd(st, fp32, n, H, nb, d, e, tauh, Vst, Tall, Wp, alpha2, Gram, acol, Yr, Yc, t12,
-                  part, scal, counter, pev, ctx->hemv_ch[fp32], ctx->her2k_valu[fp32] > 0 ? 1 : 0,
+                  part, scal, counter, pev, ctx->hemv_ch[fp32], ctx->her2k_valu[fp32] >= 0 ? ctx->her2k_valu[fp32] : 1,
                   hsys));
     if (pev) {
         HCK(hipStreamSynchronize(st));

@@ -153,7 +153,8 @@ struct bse_ctx {
     // autotuned hemv chunk length per floor(log2(trailing size)) bucket, one
     // table per storage precision (0 = not tuned yet)
     unsigned char hemv_ch[2][BSE_HEMV_BUCKETS] = {};
-    // trailing-update kernel per storage precision: -1 not tuned, 0 MFMA, 1 VALU
+    // trailing-update kernel per storage precision: -1 not tuned, 0 FP64 MFMA
+    // (64 x 64 tiles), 1 FP64 VALU, 2 FP64 MFMA (128 x 128 macro tiles)
     int her2k_valu[2] = {-1, -1};
     bool autotune = true;
     hipEvent_t tev[2] = {};     // autotuner timing pair

@@ -964,6 +964,138 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int
             }
 }
 
+// The same MFMA update over 128 x 128 macro tiles (2 x 2 storage tiles):
+// every V/W row staged in LDS feeds twice as many products, which halves the
+// panel traffic from L2 per flop (the 64 x 64 form is limited by it).  Wave
+// w = 2 rh + ch computes rows 64 rh.. and columns 64 ch.. of the macro tile
+// (4 x 4 MFMA subtiles); on a diagonal macro tile the upper storage tile
+// (rh = 0, ch = 1) is skipped.  idx runs over the lower macro tiles of the
+// trailing block A[s:, s:] (tile rows and columns tcs + 2 mi, tcs + 2 mj).
+template <typename T>
+__device__ __attribute__((always_inline)) inline void her2k_mfma2_body(int n, int s, T* __restrict__ A,
+                                       const double2* __restrict__ V,
+                                       const double2* __restrict__ W, int64_t ldv, int j0, int nb,
+                                       const double2* __restrict__ alpha2, int64_t idx)
+{
+    typedef her2k_f64::real R;
+    typedef her2k_f64::acc ACC;
+    int mi, mj;
+    tri_tile(idx, mi, mj);
+    const int tcs = s >> 6, ntile = tile_count(n);
+    const int R0 = (tcs + 2 * mi) * 64, C0 = (tcs + 2 * mj) * 64;
+    __shared__ R sPr[16][128], sPi[16][128], sQr[16][128], sQi[16][128];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rh = wv >> 1, chf = wv & 1;
+    const bool idle = (mi == mj && rh == 0 && chf == 1);
+    ACC accR[4][4], accI[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            accR[a][b] = ACC{0, 0, 0, 0};
+            accI[a][b] = ACC{0, 0, 0, 0};
+        }
+    const int K2 = 2 * nb;
+    for (int l0 = 0; l0 < K2; l0 += 16) {
+        for (int e = threadIdx.x; e < 16 * 128; e += 256) {
+            const int rr = e & 127, kc = e >> 7;
+            const int kk = l0 + kc;
+            double2 pv = cmk(0, 0), qv = cmk(0, 0);
+            if (kk < K2) {
+                const bool first = kk < nb;
+                const int l = first ? kk : kk - nb;
+                const double2 a2 = alpha2[l];
+                const int gr = R0 + rr, gc = C0 + rr;
+                if (gr >= s && gr < n) {
+                    const double2 v = V[(gr - j0) + (int64_t)l * ldv];
+                    const double2 w = cadd(W[(gr - j0) + (int64_t)l * ldv], cmul(a2, v));
+                    pv = first ? v : w;
+                }
+                if (gc >= s && gc < n) {
+                    const double2 v = V[(gc - j0) + (int64_t)l * ldv];
+                    const double2 w = cadd(W[(gc - j0) + (int64_t)l * ldv], cmul(a2, v));
+                    qv = first ? w : v;
+                }
+            }
+            sPr[kc][rr] = (R)pv.x;
+            sPi[kc][rr] = (R)pv.y;
+            sQr[kc][rr] = (R)qv.x;
+            sQi[kc][rr] = (R)qv.y;
+        }
+        __syncthreads();
+        if (!idle) {
+#pragma unroll 2
+            for (int ks = 0; ks < 8; ++ks) {
+                const int kap = ks * 4 + (lane >> 4);
+                const int kc = kap & 15;
+                const bool re = kap < 16;
+                R bP[4], bPi[4], aQ[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int row = 64 * rh + a * 16 + (lane & 15);
+                    const R pr = sPr[kc][row], pi = sPi[kc][row];
+                    bP[a] = re ? pr : pi;
+                    bPi[a] = re ? pi : -pr;
+                }
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int col = 64 * chf + b * 16 + (lane & 15);
+                    aQ[b] = re ? sQr[kc][col] : sQi[kc][col];
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        accR[a][b] = her2k_f64::mfma(aQ[b], bP[a], accR[a][b]);
+                        accI[a][b] = her2k_f64::mfma(aQ[b], bPi[a], accI[a][b]);
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    if (idle) return;
+    // lane holds D'[i][j = lane % 16] with i = her2k_f64::row(lane, reg):
+    // column i, row j of the subtile
+    const int TI = tcs + 2 * mi + rh, TJ = tcs + 2 * mj + chf;
+    if (TI >= ntile || TJ >= ntile) return;
+    T* tile = A + tile_pos(TI, TJ, ntile) * (BSE_TILE * BSE_TILE);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int reg = 0; reg < 4; ++reg) {
+                const int lr = a * 16 + (lane & 15);
+                const int lc = b * 16 + her2k_f64::row(lane, reg);
+                const int gr = TI * 64 + lr, gc = TJ * 64 + lc;
+                if (gc >= s && gr < n && gr >= gc) {
+                    T* p = tile + lr + lc * 64;
+                    double2 x = cplx<T>::load(p);
+                    x.x -= (double)accR[a][b][reg];
+                    x.y -= (double)accI[a][b][reg];
+                    if (gr == gc) x.y = 0.0;
+                    cplx<T>::store(p, x);
+                }
+            }
+}
+
+// Number of work items (blocks) of a trailing update over nT storage tile
+// rows and columns: lower 64 x 64 tiles, or lower 128 x 128 macro tiles
+// (kind 2).
+__host__ __device__ inline int64_t her2k_blocks(int64_t nT, int kind) {
+    const int64_t m = (kind == 2) ? (nT + 1) / 2 : nT;
+    return m * (m + 1) / 2;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_her2k_mfma2(int n, int s, T* __restrict__ A,
+                                                    const double2* __restrict__ V,
+                                                    const double2* __restrict__ W, int64_t ldv,
+                                                    int j0, int nb, const double2* __restrict__ alpha2)
+{
+    her2k_mfma2_body<T>(n, s, A, V, W, ldv, j0, nb, alpha2, blockIdx.x);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_her2k_lower(int n, int s, T* __restrict__ A,
                                                     const double2* __restrict__ V,
@@ -1074,7 +1206,9 @@ __global__ __launch_bounds__(64) void kb_larft(const HetrdSys* __restrict__ sys,
     panel_close_body(h, nbp, tile_count(s.n) - ((c + 1) >> 6), s.Tall + (size_t)p * nb * nb, nb);
 }
 
-template <typename T, int MFMA>
+// KIND: 0 FP64 MFMA on 64 x 64 tiles, 1 FP64 VALU, 2 FP64 MFMA on 128 x 128
+// macro tiles.
+template <typename T, int KIND>
 __global__ __launch_bounds__(256) void kb_her2k(const HetrdSys* __restrict__ sys, int c, int nb)
 {
     const HetrdSys s = sys[blockIdx.z];
@@ -1083,10 +1217,12 @@ __global__ __launch_bounds__(256) void kb_her2k(const HetrdSys* __restrict__ sys
     const int m = s.n - j0, st = j0 + nbp;
     if (st >= s.n) return;
     const int64_t nT = tile_count(s.n) - (st >> 6);
-    if ((int64_t)blockIdx.x >= nT * (nT + 1) / 2) return;
+    if ((int64_t)blockIdx.x >= her2k_blocks(nT, KIND)) return;
     const double2* Vp = s.Vstore + panel_voff(s.n, nb, p);
-    if (MFMA)
+    if (KIND == 0)
         her2k_mfma_body<T>(s.n, st, (T*)s.A, Vp, s.Wp, (int64_t)m, j0, nbp, s.alpha2, blockIdx.x);
+    else if (KIND == 2)
+        her2k_mfma2_body<T>(s.n, st, (T*)s.A, Vp, s.Wp, (int64_t)m, j0, nbp, s.alpha2, blockIdx.x);
     else
         her2k_valu_body<T>(s.n, st, (T*)s.A, Vp, s.Wp, (int64_t)m, j0, nbp, s.alpha2, blockIdx.x);
 }
@@ -1136,7 +1272,7 @@ extern "C" void bse_hetrd_workspace(int n, int nb, int64_t* vstore_elems, int64_
 }
 
 extern "C" hipError_t bse_hetrd_batch_panel(hipStream_t st, int fp32, int c, int nb, int nact,
-                                            const void* d_sys, int64_t max_tiles, int her2k_valu);
+                                            const void* d_sys, int64_t max_nt, int her2k_kind);
 static hipError_t column_launch(hipStream_t st, int fp32, int c, int nb, int nact, int n0,
                                 const void* d_sys, int code, hipEvent_t e0, hipEvent_t e1);
 
@@ -1153,7 +1289,7 @@ extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int nb
                                 double2* Tall, double2* Wp, double2* alpha2,
                                 double2* G, double2* acol, double2* Yr, double2* Yc, double2* t12,
                                 double* part, double* scal, unsigned int* counter,
-                                hipEvent_t* prof_ev, const unsigned char* ch_table, int her2k_valu,
+                                hipEvent_t* prof_ev, const unsigned char* ch_table, int her2k_kind,
                                 void* d_sys)
 {
     if (nb < 1 || nb > 32 || n < 1 || !d_sys) return hipErrorInvalidValue;
@@ -1173,12 +1309,8 @@ extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int nb
             if (err != hipSuccess) return err;
             const int j0 = (c / nb) * nb, nbp = min(nb, n - 1 - j0);
             if (c != j0 + nbp - 1) continue;
-            int64_t tiles = 0;
-            if (j0 + nbp < n) {
-                const int64_t nT = ntile - ((j0 + nbp) >> 6);
-                tiles = nT * (nT + 1) / 2;
-            }
-            if ((err = bse_hetrd_batch_panel(st, fp32, c, nb, 1, d_sys, tiles, her2k_valu)) != hipSuccess)
+            const int64_t nT = (j0 + nbp < n) ? ntile - ((j0 + nbp) >> 6) : 0;
+            if ((err = bse_hetrd_batch_panel(st, fp32, c, nb, 1, d_sys, nT, her2k_kind)) != hipSuccess)
                 return err;
         }
     }
@@ -1243,44 +1375,46 @@ extern "C" hipError_t bse_hetrd_batch_column(hipStream_t st, int fp32, int c, in
 }
 
 // Panel close at column c: compact-WY factor and trailing update for every
-// matrix whose current panel ends at c.  max_tiles: largest lower-tile count
-// among them (grid extent).
+// matrix whose current panel ends at c.  max_nt: largest trailing tile count
+// among them (grid extent); her2k_kind: 0 FP64 MFMA 64 x 64, 1 FP64 VALU,
+// 2 FP64 MFMA 128 x 128.
 extern "C" hipError_t bse_hetrd_batch_panel(hipStream_t st, int fp32, int c, int nb, int nact,
-                                            const void* d_sys, int64_t max_tiles, int her2k_valu)
+                                            const void* d_sys, int64_t max_nt, int her2k_kind)
 {
     if (nact <= 0) return hipSuccess;
+    if (her2k_kind < 0 || her2k_kind > 2) return hipErrorInvalidValue;
     const HetrdSys* sys = (const HetrdSys*)d_sys;
     const unsigned z = (unsigned)nact;
     hipLaunchKernelGGL(kb_larft, dim3(1, 1, z), dim3(64), 0, st, sys, c, nb);
-    if (max_tiles > 0) {
-        const dim3 g((unsigned)max_tiles, 1, z);
+    if (max_nt > 0) {
+        const dim3 g((unsigned)her2k_blocks(max_nt, her2k_kind), 1, z);
+#define KBK(TT, KK) hipLaunchKernelGGL((kb_her2k<TT, KK>), g, dim3(256), 0, st, sys, c, nb)
         if (fp32) {
-            if (her2k_valu) hipLaunchKernelGGL((kb_her2k<float2, 0>), g, dim3(256), 0, st, sys, c, nb);
-            else hipLaunchKernelGGL((kb_her2k<float2, 1>), g, dim3(256), 0, st, sys, c, nb);
+            if (her2k_kind == 0) KBK(float2, 0); else if (her2k_kind == 2) KBK(float2, 2); else KBK(float2, 1);
         } else {
-            if (her2k_valu) hipLaunchKernelGGL((kb_her2k<double2, 0>), g, dim3(256), 0, st, sys, c, nb);
-            else hipLaunchKernelGGL((kb_her2k<double2, 1>), g, dim3(256), 0, st, sys, c, nb);
+            if (her2k_kind == 0) KBK(double2, 0); else if (her2k_kind == 2) KBK(double2, 2); else KBK(double2, 1);
         }
+#undef KBK
     }
     return hipGetLastError();
 }
 
 // One trailing-update launch on A[s:, s:] of a tiled lower matrix of size n
 // (autotuner timing); V, W start at global row j0 with leading dimension ldv.
-extern "C" hipError_t bse_launch_her2k(hipStream_t st, int fp32, int valu, int n, int s, void* A,
+extern "C" hipError_t bse_launch_her2k(hipStream_t st, int fp32, int kind, int n, int s, void* A,
                                        const double2* V, const double2* W, int64_t ldv, int j0,
                                        int nb, const double2* alpha2)
 {
     if (s >= n) return hipSuccess;
-    if (s < j0 || ldv < n - j0) return hipErrorInvalidValue;
+    if (s < j0 || ldv < n - j0 || kind < 0 || kind > 2) return hipErrorInvalidValue;
     const int64_t nT = tile_count(n) - (s >> 6);
-    const dim3 g((unsigned)(nT * (nT + 1) / 2));
+    const dim3 g((unsigned)her2k_blocks(nT, kind));
+#define KH(K, TT) hipLaunchKernelGGL(K<TT>, g, dim3(256), 0, st, n, s, (TT*)A, V, W, ldv, j0, nb, alpha2)
     if (fp32) {
-        if (valu) hipLaunchKernelGGL(k_her2k_lower<float2>, g, dim3(256), 0, st, n, s, (float2*)A, V, W, ldv, j0, nb, alpha2);
-        else hipLaunchKernelGGL(k_her2k_mfma<float2>, g, dim3(256), 0, st, n, s, (float2*)A, V, W, ldv, j0, nb, alpha2);
+        if (kind == 1) KH(k_her2k_lower, float2); else if (kind == 2) KH(k_her2k_mfma2, float2); else KH(k_her2k_mfma, float2);
     } else {
-        if (valu) hipLaunchKernelGGL(k_her2k_lower<double2>, g, dim3(256), 0, st, n, s, (double2*)A, V, W, ldv, j0, nb, alpha2);
-        else hipLaunchKernelGGL(k_her2k_mfma<double2>, g, dim3(256), 0, st, n, s, (double2*)A, V, W, ldv, j0, nb, alpha2);
+        if (kind == 1) KH(k_her2k_lower, double2); else if (kind == 2) KH(k_her2k_mfma2, double2); else KH(k_her2k_mfma, double2);
     }
+#undef KH
     return hipGetLastError();
 }

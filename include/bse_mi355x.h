@@ -221,7 +221,8 @@ int bse_batch_results(bse_ctx* ctx, double* evals, double* eps_re, double* eps_i
  *     tile rows; plus 32 for the two-row-blocks-per-step form or 64 for the
  *     per-slab column-sum form; plus 128 for default-policy instead of
  *     non-temporal matrix loads),
- * [5] trailing-update kernel (0 MFMA, 1 VALU).                             */
+ * [5] trailing-update kernel (0 FP64 MFMA on 64 x 64 tiles, 1 FP64 VALU,
+ *     2 FP64 MFMA on 128 x 128 tiles).                                  */
 int bse_batch_info(const bse_ctx* ctx, int64_t info[6]);
 
 #ifdef __cplusplus
