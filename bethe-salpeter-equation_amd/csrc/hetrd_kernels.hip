@@ -705,8 +705,17 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
         const int ll = pr >> 1, which = pr & 1;
         double2 psum = cmk(0, 0), e1 = cmk(0, 0);
         if (ll < h.i) {
-            const double2* pp = (const double2*)(h.part + pl.b);
-            for (int rb = k; rb < nbk; rb += 16) psum = cadd(psum, pp[((int64_t)ll * nbk + rb) * 2 + which]);
+            // eight independent loads per step (the partial count is nbk)
+            const double2* pp = (const double2*)(h.part + pl.b) + (int64_t)ll * nbk * 2 + which;
+            int rb = k;
+            for (; rb + 7 * 16 < nbk; rb += 8 * 16) {
+                double2 x[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) x[u] = pp[(int64_t)(rb + 16 * u) * 2];
+                psum = cadd(psum, cadd(cadd(cadd(x[0], x[1]), cadd(x[2], x[3])),
+                                       cadd(cadd(x[4], x[5]), cadd(x[6], x[7]))));
+            }
+            for (; rb < nbk; rb += 16) psum = cadd(psum, pp[(int64_t)rb * 2]);
             if (k == 0) {
                 const int64_t r0 = s0 - h.j0;
                 e1 = which ? h.Vp[r0 + (int64_t)ll * h.m] : w_at(h, r0, ll);
