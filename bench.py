@@ -201,7 +201,7 @@ def sweep_bench(args, pkg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=36, help="k-grid n (N = 16 n^2)")
     ap.add_argument("--mixed", default="bf16x2", choices=["fp32", "bf16x2"])

@@ -135,6 +135,15 @@ __device__ __attribute__((always_inline)) inline void colupd_body(const HetrdCol
     }
     double2 a = cmk(0, 0);
     if (grp == 0 && row_ok) a = cplx<T>::load(A + tix(c + t, c, tile_count(h.n)));
+    // row i of the panel's V and W (the same for every thread): loaded before
+    // the partial sum of alpha2[i-1], which only W(i, i-1) needs
+    double2 rv = cmk(0, 0), rw = cmk(0, 0), ra = cmk(0, 0);
+    if ((int)threadIdx.x < h.i) {
+        const int l = threadIdx.x;
+        rv = h.Vp[h.i + (int64_t)l * h.m];
+        rw = h.Wp[h.i + (int64_t)l * h.m];
+        if (l < h.i - 1) ra = h.alpha2[l];
+    }
     double2 a2prev = cmk(0, 0);
     if (h.i > 0) {
         const double2 sp = block_sum_partials((const double2*)(h.part + pl.c), prev_gx, sh);
@@ -148,13 +157,11 @@ __device__ __attribute__((always_inline)) inline void colupd_body(const HetrdCol
             if (h.i > 0) h.alpha2[h.i - 1] = a2prev;
         }
     }
-    // row i of the panel's V and W (the same for every thread), staged once
     if ((int)threadIdx.x < h.i) {
         const int l = threadIdx.x;
-        const double2 a2 = (l == h.i - 1) ? a2prev : h.alpha2[l];
-        const double2 vc = h.Vp[h.i + (int64_t)l * h.m];
-        s_vc[l] = vc;
-        s_wc[l] = cadd(h.Wp[h.i + (int64_t)l * h.m], cmul(a2, vc));
+        const double2 a2 = (l == h.i - 1) ? a2prev : ra;
+        s_vc[l] = rv;
+        s_wc[l] = cadd(rw, cmul(a2, rv));
         s_a2[l] = a2;
     }
     __syncthreads();
