@@ -166,9 +166,32 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
     // records the largest component so the final pass also fixes the sign.
     for (int k = 0; k < nb; ++k) x[(int64_t)(bs + k) * S + t] = hash_uniform(m0 + t, k);
     double s = 1.0, big = 0.0;
+    // The solves walk the column in steps of IIB rows: all loads of a step are
+    // issued before its recurrence runs (the compiler cannot move them above
+    // the stores into x), so each step costs one memory round trip.
+    constexpr int IIB = 8;
     for (int it = 0; it < 3; ++it) {
         double cur = x[(int64_t)bs * S + t] * s;
-        for (int k = 0; k < nb - 1; ++k) {
+        int k = 0;
+        for (; k + IIB <= nb - 1; k += IIB) {
+            double xn[IIB], mu[IIB], pv[IIB];
+#pragma unroll
+            for (int u = 0; u < IIB; ++u) {
+                const int64_t gi = bs + k + u;
+                xn[u] = x[(gi + 1) * S + t];
+                mu[u] = mul[gi * S + t];
+                pv[u] = pf[gi * S + t];
+            }
+#pragma unroll
+            for (int u = 0; u < IIB; ++u) {
+                double nxt = xn[u] * s;
+                if (pv[u] != 0.0) { const double tmp = cur; cur = nxt; nxt = tmp; }
+                nxt -= mu[u] * cur;
+                x[(bs + k + u) * S + t] = cur;
+                cur = nxt;
+            }
+        }
+        for (; k < nb - 1; ++k) {
             const int64_t gi = bs + k;
             double nxt = x[(gi + 1) * S + t] * s;
             if (pf[gi * S + t] != 0.0) { const double tmp = cur; cur = nxt; nxt = tmp; }
@@ -181,8 +204,29 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
         x[gl * S + t] = x1;
         double x2 = 0.0, nrm = x1 * x1;
         big = x1;
-        for (int k = nb - 2; k >= 0; --k) {
-            const int64_t gi = bs + k;
+        int kb = nb - 2;
+        for (; kb - IIB + 1 >= 0; kb -= IIB) {
+            double xv[IIB], a0[IIB], a1[IIB], a2[IIB];
+#pragma unroll
+            for (int u = 0; u < IIB; ++u) {
+                const int64_t gi = bs + kb - u;
+                xv[u] = x[gi * S + t];
+                a0[u] = u0[gi * S + t];
+                a1[u] = u1[gi * S + t];
+                a2[u] = u2[gi * S + t];
+            }
+#pragma unroll
+            for (int u = 0; u < IIB; ++u) {
+                const double xk = (xv[u] - a1[u] * x1 - a2[u] * x2) / a0[u];
+                x[(bs + kb - u) * S + t] = xk;
+                nrm += xk * xk;
+                if (fabs(xk) > fabs(big)) big = xk;
+                x2 = x1;
+                x1 = xk;
+            }
+        }
+        for (; kb >= 0; --kb) {
+            const int64_t gi = bs + kb;
             const double xk = (x[gi * S + t] - u1[gi * S + t] * x1 - u2[gi * S + t] * x2) / u0[gi * S + t];
             x[gi * S + t] = xk;
             nrm += xk * xk;
