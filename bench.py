@@ -17,7 +17,8 @@ assembly of the N x N TDA matrix, and its eigendecomposition.
 Prints one JSON line.  The roofline object is for the dominant kernel, the
 tridiagonalisation's Hermitian matrix-vector product, whose launches are
 timed with HIP events on the library's stream; its algorithmic bytes are the
-lower triangle of every trailing block (s_c * nt (nt+1)/2 per launch).
+lower triangle of every trailing block (s_e * m (m+1)/2 per launch for a
+trailing block of order m, s_e = 8 B for FP32 complex storage).
 The CPU baseline times the FP64 oracle (numpy/LAPACK zheevd) on a reduced
 k-grid of the same model and scales it to N = 20736 (eig ~ N^3, build ~ N^2).
 """
@@ -280,8 +281,10 @@ def main():
     ach = prof["hemv_bytes"] / (prof["hemv_ms"] * 1e-3) / 1e9 if prof["hemv_ms"] > 0 else 0.0
     traffic, traffic_src = measured_traffic(args.mixed, N)
     roof = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
-                traffic=traffic, traffic_source=(traffic_src or {}).get("source"),
-                kernel=f"k_hemv_lower<{'float2' if args.mixed != 'fp64' else 'double2'}>",
+                traffic=traffic, traffic_source=(traffic_src or {}).get("source") or
+                ("none: rocprofv3 --pmc FETCH_SIZE ends in a host segfault inside the profiler "
+                 "at the first dispatch on this ROCm image, so no counter figure exists"),
+                kernel="kb_hemv<float2, *> (every variant the tuner picks)",
                 launches=prof["hemv_launches"],
                 algorithmic_bytes_per_launch=prof["hemv_bytes"] / max(1, prof["hemv_launches"]),
                 avg_launch_us=prof["hemv_ms"] * 1e3 / max(1, prof["hemv_launches"]))

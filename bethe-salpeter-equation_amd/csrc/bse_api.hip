@@ -275,6 +275,20 @@ static bool hemv_code_ok(int code) {
     return code > 0 && code < 256 && (ch == 4 || ch == 8 || ch == 16);
 }
 
+// Partial-sum vectors one hemv launch of variant `code` writes for the
+// trailing block A[s0:, s0:]: a row partial per (tile row, block of rbk tile
+// columns) and a column partial per (tile row, chunk of ch tile rows below it).
+static int64_t hemv_partials(int code, int n, int s0) {
+    const int ch = code & 31, rbk = ((code >> 5) & 3) == 3 ? 4 : 1;
+    const int nbk = (n + 63) / 64 - (s0 >> 6);
+    int64_t cnt = 0;
+    for (int b = 0; b < nbk; ++b) {
+        const int pmax = b / rbk;
+        cnt += pmax + 1 + (nbk - pmax * rbk + ch - 1) / ch;
+    }
+    return cnt;
+}
+
 // ---------------------------------------------------------------------------
 // Autotuner for the tridiagonalisation's Hermitian matrix-vector kernel: for
 // every floor(log2(nt)) bucket up to the current size that is not tuned yet,
@@ -286,7 +300,9 @@ static bool hemv_code_ok(int code) {
 // non-temporal or default-policy loads (+ 128).  Repeated launches over the
 // same block see the cache reuse that consecutive columns see.  The table is
 // per context and storage precision; BSE_AUTOTUNE=0 keeps the default
-// (ch 8, column sums, non-temporal).
+// (ch 8, column sums, non-temporal).  Each candidate's time includes the
+// reduce's read of the partial sums it writes (a variant that writes more of
+// them makes the column step's reduce slower).
 // ---------------------------------------------------------------------------
 int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, double2* v, double2* Yr,
                   double2* Yc) {
@@ -326,6 +342,11 @@ int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, double2* v, doub
                 HCK(hipEventElapsedTime(&ms, ctx->tev[0], ctx->tev[1]));
                 tk[k] = (pass == 0) ? ms : std::min(tk[k], ms);
             }
+        // each candidate is charged for the partial sums that the column
+        // step's reduce reads back (64 double2 per partial), at the 2.5 TB/s
+        // that the reduce sustains on them
+        for (int k = 0; k < 24; ++k)
+            tk[k] += (float)((double)reps * 1024.0 * (double)hemv_partials(base[k % 12], n, s0) / 2.5e9);
         float best = 0.f;
         int best_ch = 8 + 64;
         for (int k = 0; k < 24; ++k)
