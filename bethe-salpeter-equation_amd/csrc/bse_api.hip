@@ -292,8 +292,8 @@ static int64_t hemv_partials(int code, int n, int s0) {
 // ---------------------------------------------------------------------------
 // Autotuner for the tridiagonalisation's Hermitian matrix-vector kernel: for
 // every floor(log2(nt)) bucket up to the current size that is not tuned yet,
-// time chunk lengths 4, 8 and 16 on a trailing block of the matrix about to
-// be reduced (the kernel only reads it) and keep the fastest, each as the
+// time chunk lengths 4, 8 and 16 on two trailing blocks of the matrix about
+// to be reduced (the kernel only reads it) and keep the fastest, each as the
 // plain variant (code ch), the two-row-blocks-per-step variant (ch + 32) and
 // the variant with per-slab cross-lane column sums (ch + 64), the variant
 // with waves owning rows (ch + 96, chunks of 2 ch tiles), with
@@ -323,35 +323,57 @@ int autotune_hemv(bse_ctx* ctx, int fp32, int n, const void* H, double2* v, doub
             HCK(hipMemsetAsync(v, 0, sizeof(double2) * n, st));
             zeroed = true;
         }
-        const int nt = std::min(3 << (b - 1), n - 1);
-        const int s0 = n - nt;                 // trailing block A[s0:, s0:]
-        // two timing passes over the candidates, each keeping its faster one
-        const int reps = nt > 4096 ? 5 : 16;
+        // the bucket holds trailing orders [2^b, top]; every candidate is
+        // timed at a quarter and at three quarters of that range (a variant
+        // with long chunks can win at the top of a bucket and underfill the
+        // chip lower in it) and the per-launch times are summed
+        const int top = std::min((2 << b) - 1, n - 1);
+        const int span = top - (1 << b);
+        const int sizes[2] = {(1 << b) + span / 4, (1 << b) + (3 * span) / 4};
         static const int base[12] = {8 + 64, 4 + 64, 16 + 64, 8 + 32, 4 + 32, 16 + 32, 8, 4, 16,
                                      4 + 96, 8 + 96, 16 + 96};
         float tk[24];
-        for (int pass = 0; pass < 2; ++pass)
-            for (int k = 0; k < 24; ++k) {
-                const int ch = base[k % 12] + (k >= 12 ? 128 : 0);
-                HCK(bse_launch_hemv(st, fp32, ch, n, s0, H, v, Yr, Yc));
-                HCK(hipEventRecord(ctx->tev[0], st));
-                for (int r = 0; r < reps; ++r) HCK(bse_launch_hemv(st, fp32, ch, n, s0, H, v, Yr, Yc));
-                HCK(hipEventRecord(ctx->tev[1], st));
-                HCK(hipEventSynchronize(ctx->tev[1]));
-                float ms = 0.f;
-                HCK(hipEventElapsedTime(&ms, ctx->tev[0], ctx->tev[1]));
-                tk[k] = (pass == 0) ? ms : std::min(tk[k], ms);
-            }
-        // each candidate is charged for the partial sums that the column
-        // step's reduce reads back (64 double2 per partial), at the 2.5 TB/s
-        // that the reduce sustains on them
-        for (int k = 0; k < 24; ++k)
-            tk[k] += (float)((double)reps * 1024.0 * (double)hemv_partials(base[k % 12], n, s0) / 2.5e9);
+        for (int k = 0; k < 24; ++k) tk[k] = 0.f;
+        for (int si = 0; si < 2; ++si) {
+            const int nt = sizes[si];
+            const int s0 = n - nt;             // trailing block A[s0:, s0:]
+            const int reps = nt > 4096 ? 5 : 16;
+            // two timing passes over the candidates, each keeping its faster one
+            float tp[24];
+            for (int pass = 0; pass < 2; ++pass)
+                for (int k = 0; k < 24; ++k) {
+                    const int ch = base[k % 12] + (k >= 12 ? 128 : 0);
+                    HCK(bse_launch_hemv(st, fp32, ch, n, s0, H, v, Yr, Yc));
+                    HCK(hipEventRecord(ctx->tev[0], st));
+                    for (int r = 0; r < reps; ++r) HCK(bse_launch_hemv(st, fp32, ch, n, s0, H, v, Yr, Yc));
+                    HCK(hipEventRecord(ctx->tev[1], st));
+                    HCK(hipEventSynchronize(ctx->tev[1]));
+                    float ms = 0.f;
+                    HCK(hipEventElapsedTime(&ms, ctx->tev[0], ctx->tev[1]));
+                    ms /= (float)reps;
+                    tp[k] = (pass == 0) ? ms : std::min(tp[k], ms);
+                }
+            // each candidate is charged for the partial sums that the column
+            // step's reduce reads back (64 double2 per partial), at the
+            // 2.5 TB/s that the reduce sustains on them
+            for (int k = 0; k < 24; ++k)
+                tk[k] += tp[k] + (float)(1024.0 * (double)hemv_partials(base[k % 12], n, s0) / 2.5e9);
+        }
         float best = 0.f;
         int best_ch = 8 + 64;
         for (int k = 0; k < 24; ++k)
             if (best == 0.f || tk[k] < 0.97f * best) { best = tk[k]; best_ch = base[k % 12] + (k >= 12 ? 128 : 0); }
         tab[b] = (unsigned char)best_ch;
+        if (getenv("BSE_VERBOSE")) {
+            float second = 0.f;
+            int second_ch = 0;
+            for (int k = 0; k < 24; ++k) {
+                const int code = base[k % 12] + (k >= 12 ? 128 : 0);
+                if (code != best_ch && (second == 0.f || tk[k] < second)) { second = tk[k]; second_ch = code; }
+            }
+            fprintf(stderr, "hemv tuner fp32=%d m=%d,%d: code %d %.4f ms (next %d %.4f)\n", fp32, sizes[0],
+                    sizes[1], best_ch, best, second_ch, second);
+        }
     }
     return BSE_OK;
 }
