@@ -902,34 +902,53 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int
             accR[a][b] = ACC{0, 0, 0, 0};
             accI[a][b] = ACC{0, 0, 0, 0};
         }
-    const int K2 = 2 * nb;                    // complex K columns of [V W]
-    for (int l0 = 0; l0 < K2; l0 += 16) {
-        for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-            const int rr = e & 63, kc = e >> 6;
-            const int kk = l0 + kc;           // column of P = [V W], Q = [W V]
-            double2 pv = cmk(0, 0), qv = cmk(0, 0);
+    const int K2 = 2 * nb;                    // complex K columns of P = [V W''], Q = [W V]
+    // Only Re(alpha2) enters the Hermitian update: with W' = W + V a,
+    //   V W'^H + W' V^H = V W^H + W V^H + 2 Re(a) V V^H = P Q^H
+    // for P = [V, W + 2 Re(a) V] and Q = [W, V]; the column side is raw.
+    // Each thread stages elements e = tid + 256 u (u < 4) of a chunk; the raw
+    // operands of the next chunk are loaded into registers while the current
+    // chunk's products run.
+    const int rr = threadIdx.x & 63;
+    const int gr = r0 + rr, gc = c0 + rr;
+    const bool rok = gr >= s && gr < n, cok = gc >= s && gc < n;
+    const int64_t ro = (int64_t)(gr - j0), co = (int64_t)(gc - j0);
+    double2 pvr[4], pwr[4], pc[4];           // row V, row W, column operand
+    auto fetch = [&](int l0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kk = l0 + (threadIdx.x >> 6) + 4 * u;
+            pvr[u] = cmk(0, 0);
+            pwr[u] = cmk(0, 0);
+            pc[u] = cmk(0, 0);
             if (kk < K2) {
                 const bool first = kk < nb;
-                const int l = first ? kk : kk - nb;
-                const double2 a2 = alpha2[l];
-                const int gr = r0 + rr, gc = c0 + rr;
-                if (gr >= s && gr < n) {
-                    const double2 v = V[(gr - j0) + (int64_t)l * ldv];
-                    const double2 w = cadd(W[(gr - j0) + (int64_t)l * ldv], cmul(a2, v));
-                    pv = first ? v : w;
+                const int64_t lo = (int64_t)(first ? kk : kk - nb) * ldv;
+                if (rok) {
+                    pvr[u] = V[ro + lo];
+                    if (!first) pwr[u] = W[ro + lo];
                 }
-                if (gc >= s && gc < n) {
-                    const double2 v = V[(gc - j0) + (int64_t)l * ldv];
-                    const double2 w = cadd(W[(gc - j0) + (int64_t)l * ldv], cmul(a2, v));
-                    qv = first ? w : v;
-                }
+                if (cok) pc[u] = first ? W[co + lo] : V[co + lo];
+            }
+        }
+    };
+    fetch(0);
+    for (int l0 = 0; l0 < K2; l0 += 16) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kc = (threadIdx.x >> 6) + 4 * u, kk = l0 + kc;
+            double2 pv = pvr[u];
+            if (kk < K2 && kk >= nb) {
+                const double ar = 2.0 * alpha2[kk - nb].x;
+                pv = cmk(fma(ar, pvr[u].x, pwr[u].x), fma(ar, pvr[u].y, pwr[u].y));
             }
             sPr[kc][rr] = (R)pv.x;
             sPi[kc][rr] = (R)pv.y;
-            sQr[kc][rr] = (R)qv.x;
-            sQi[kc][rr] = (R)qv.y;
+            sQr[kc][rr] = (R)pc[u].x;
+            sQi[kc][rr] = (R)pc[u].y;
         }
         __syncthreads();
+        if (l0 + 16 < K2) fetch(l0 + 16);
         // real K index kappa in [0, 32): < 16 real parts, >= 16 imaginary parts
 #pragma unroll
         for (int ks = 0; ks < 8; ++ks) {
