@@ -635,10 +635,15 @@ void k_hemv_lower(int n, int s0, const T* __restrict__ A, const double2* __restr
 //   y(r) = tau (sc (A22 x)(r) + a(r, s0) - V(r, :) t1 - W(r, :) t2)
 // (the panel corrections (d) and the tau scaling (e)), stored in Wp(:, i),
 // with the partial y^H v in region C; alpha2[i] = -tau/2 (y^H v) is finished
-// by the next kernel.  Sixteen lane groups split the partial sums so that
+// by the next kernel.  RED_G lane groups split the partial sums so that
 // many loads are in flight; every load that does not depend on t1, t2 is
 // issued before the one barrier that publishes them.
-#define RED_G 16
+// Lane groups of the reduce (workgroup of 64 RED_G threads).  Eight keep the
+// workgroup at 512 threads, so that two fit a CU and a trailing block of up
+// to 512 tile rows reduces in one round (one 1024-thread workgroup per CU
+// left half of the blocks for a second round above 256 tile rows).
+#define RED_G 8
+#define RED_NL (32 / RED_G)   // panel columns per lane group (i <= 32)
 template <typename T>
 __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol& h, const T* __restrict__ A,
                                                                    const double2* __restrict__ Yr,
@@ -658,7 +663,9 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
     const bool valid = grow >= s0 && grow < h.n;
     const int lr = grow - h.j0;                         // panel-local row
     double2 acc0 = cmk(0, 0), acc1 = cmk(0, 0);
-    double2 vv[2] = {cmk(0, 0), cmk(0, 0)}, ww[2] = {cmk(0, 0), cmk(0, 0)};
+    double2 vv[RED_NL], ww[RED_NL];
+#pragma unroll
+    for (int u = 0; u < RED_NL; ++u) vv[u] = ww[u] = cmk(0, 0);
     double2 a_rs0 = cmk(0, 0), xr = cmk(0, 0);
     if (valid) {
         // partials of this block row: Yr[pb][b] (pb <= b / rbk, one per
@@ -690,9 +697,9 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
             acc1 = cadd(acc1, cadd(x0, x1));
         }
         if (q < nqb) acc1 = cadd(acc1, yc[(int64_t)q * 64]);
-        // panel rows of this row for the corrections (l = g, g + 16 < i <= 32)
+        // panel rows of this row for the corrections (l = g + RED_G u < i <= 32)
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < RED_NL; ++u) {
             const int l = g + u * RED_G;
             if (l < h.i) {
                 vv[u] = h.Vp[lr + (int64_t)l * h.m];
@@ -707,8 +714,8 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
         }
     }
     // t1, t2 from the region-B partials
-    {
-        const int pr = threadIdx.x >> 4, k = threadIdx.x & 15;   // (l, which) = (pr / 2, pr % 2)
+    for (int pr = threadIdx.x >> 4; pr < 64; pr += (int)(blockDim.x >> 4)) {
+        const int k = threadIdx.x & 15;           // (l, which) = (pr / 2, pr % 2)
         const int ll = pr >> 1, which = pr & 1;
         double2 psum = cmk(0, 0), e1 = cmk(0, 0);
         if (ll < h.i) {
@@ -739,7 +746,7 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
     __syncthreads();
     double2 corr = cmk(0, 0);
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < RED_NL; ++u) {
         const int l = g + u * RED_G;
         if (valid && l < h.i) corr = cadd(corr, cadd(cmul(vv[u], s_t[0][l]), cmul(ww[u], s_t[1][l])));
     }
@@ -1211,7 +1218,7 @@ void kb_hemv(const HetrdSys* __restrict__ sys, int c, int nb)
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void kb_reduce(const HetrdSys* __restrict__ sys, int c, int nb,
+__global__ __launch_bounds__(64 * RED_G) void kb_reduce(const HetrdSys* __restrict__ sys, int c, int nb,
                                                   int ch, int rbk)
 {
     HetrdCol h;
@@ -1398,8 +1405,8 @@ static hipError_t column_launch(hipStream_t st, int fp32, int c, int nb, int nac
 #undef KBD
     if (e1) (void)hipEventRecord(e1, st);
     const int rbk = hemv_rblk(code);
-    if (fp32) hipLaunchKernelGGL(kb_reduce<float2>, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch, rbk);
-    else hipLaunchKernelGGL(kb_reduce<double2>, dim3((unsigned)nbk, 1, z), dim3(1024), 0, st, sys, c, nb, ch, rbk);
+    if (fp32) hipLaunchKernelGGL(kb_reduce<float2>, dim3((unsigned)nbk, 1, z), dim3(64 * RED_G), 0, st, sys, c, nb, ch, rbk);
+    else hipLaunchKernelGGL(kb_reduce<double2>, dim3((unsigned)nbk, 1, z), dim3(64 * RED_G), 0, st, sys, c, nb, ch, rbk);
     return hipGetLastError();
 }
 
