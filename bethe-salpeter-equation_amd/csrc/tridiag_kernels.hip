@@ -30,14 +30,52 @@ __device__ inline double fast_rcp(double b) {
     return r;
 }
 
-// Count of eigenvalues of block [bs, bs+nbk) strictly below x.
+// Count of eigenvalues of block [bs, bs+nbk) strictly below x.  The
+// recurrence is a chain of dependent operations; its inputs are read eight
+// steps at a time, the next eight while the current ones are consumed.
 __device__ inline int sturm_count(const double* __restrict__ d, const double* __restrict__ e2,
                                   int bs, int nbk, double x, double pivmin)
 {
+    constexpr int SB = 8;
     double q = d[bs] - x;
     if (fabs(q) <= pivmin) q = -pivmin;
     int cnt = q < 0.0;
-    for (int i = 1; i < nbk; ++i) {
+    int i = 1;
+    if (i + SB <= nbk) {
+        double dv[SB], ev[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            dv[u] = d[bs + i + u];
+            ev[u] = e2[bs + i + u - 1];
+        }
+        for (; i + 2 * SB <= nbk; i += SB) {
+            double dn[SB], en[SB];
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                dn[u] = d[bs + i + SB + u];
+                en[u] = e2[bs + i + SB + u - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                q = (dv[u] - x) - ev[u] * fast_rcp(q);
+                if (fabs(q) <= pivmin) q = -pivmin;
+                cnt += q < 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                dv[u] = dn[u];
+                ev[u] = en[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            q = (dv[u] - x) - ev[u] * fast_rcp(q);
+            if (fabs(q) <= pivmin) q = -pivmin;
+            cnt += q < 0.0;
+        }
+        i += SB;
+    }
+    for (; i < nbk; ++i) {
         q = (d[bs + i] - x) - e2[bs + i - 1] * fast_rcp(q);
         if (fabs(q) <= pivmin) q = -pivmin;
         cnt += q < 0.0;
