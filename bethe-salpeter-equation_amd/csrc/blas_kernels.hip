@@ -348,7 +348,7 @@ extern "C" hipError_t bse_launch_apply_q(hipStream_t st, int n, int nb, const do
 // the whole device: per panel (V: m x nbp, ld m; X: the rows j0.. of the
 // vectors, ld ldx)
 //   (1) k_pq_dots:   partial P_g = V_g^H X_g over row blocks g of PQ_ROWS rows
-//   (2) k_pq_small:  W = T^(H) sum_g P_g (one workgroup; fixed summation order)
+//   (2) k_pq_small:  W = T^(H) sum_g P_g (one workgroup per vector; fixed summation order)
 //   (3) k_pq_update: X -= V W
 // Three short launches per panel instead of four GEMM launches; every pass
 // reads V once at streaming rate.
@@ -389,37 +389,45 @@ __global__ __launch_bounds__(256) void k_pq_dots(int m, int nbp, int k, const do
     P[(int64_t)blockIdx.x * 512 + t + 256] = acc[1];
 }
 
-__global__ __launch_bounds__(1024) void k_pq_small(int nbp, int k, int G, const double2* __restrict__ P,
-                                                   const double2* __restrict__ T, int ldt, int adjoint,
-                                                   double2* __restrict__ W)
+// One workgroup per vector column j < k: thread (l, part) sums entry (l, j)
+// over the partials g = part, part + 8, ... (eight independent loads in
+// flight), the eight part sums are added in a fixed order, then
+// W(:, j) = T^(H) (sum) for the nbp reflector rows.  Fixed summation order,
+// identical on every run.
+__global__ __launch_bounds__(256) void k_pq_small(int nbp, int k, int G, const double2* __restrict__ P,
+                                                  const double2* __restrict__ T, int ldt, int adjoint,
+                                                  double2* __restrict__ W)
 {
-    // two threads per output, each summing every other partial with eight
-    // independent loads in flight; fixed order, identical on every run
-    __shared__ double2 sW[512], sH[512];
-    const int o = threadIdx.x & 511, half = threadIdx.x >> 9, l = o & 31, j = o >> 5;
+    __shared__ double2 sP[8][32];
+    __shared__ double2 sW[32];
+    const int j = blockIdx.x, l = threadIdx.x & 31, part = threadIdx.x >> 5;
+    if (j >= k) return;
+    const int o = l + 32 * j;
     double2 acc[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc[u] = cmk(0, 0);
-    int g = half;
-    for (; g + 14 < G; g += 16) {
+    int g = part;
+    if (l < nbp) {
+        for (; g + 56 < G; g += 64) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc[u] = cadd(acc[u], P[(int64_t)(g + 2 * u) * 512 + o]);
+            for (int u = 0; u < 8; ++u) acc[u] = cadd(acc[u], P[(int64_t)(g + 8 * u) * 512 + o]);
+        }
+        for (; g < G; g += 8) acc[0] = cadd(acc[0], P[(int64_t)g * 512 + o]);
     }
-    double2 rem = cmk(0, 0);
-    for (; g < G; g += 2) rem = cadd(rem, P[(int64_t)g * 512 + o]);
-    const double2 part = cadd(cadd(cadd(cadd(acc[0], acc[1]), cadd(acc[2], acc[3])),
-                                   cadd(cadd(acc[4], acc[5]), cadd(acc[6], acc[7]))), rem);
-    if (half) sH[o] = part;
+    sP[part][l] = cadd(cadd(cadd(acc[0], acc[1]), cadd(acc[2], acc[3])),
+                       cadd(cadd(acc[4], acc[5]), cadd(acc[6], acc[7])));
     __syncthreads();
-    if (!half) sW[o] = cadd(part, sH[o]);
+    if (threadIdx.x < 32)
+        sW[l] = cadd(cadd(cadd(sP[0][l], sP[1][l]), cadd(sP[2][l], sP[3][l])),
+                     cadd(cadd(sP[4][l], sP[5][l]), cadd(sP[6][l], sP[7][l])));
     __syncthreads();
-    if (half) return;
+    if (threadIdx.x >= 32) return;
     double2 s = cmk(0, 0);
-    if (l < nbp && j < k) {
+    if (l < nbp) {
         if (adjoint) {
-            for (int q = 0; q <= l; ++q) cfmac(s, T[q + (int64_t)l * ldt], sW[q + 32 * j]);
+            for (int q = 0; q <= l; ++q) cfmac(s, T[q + (int64_t)l * ldt], sW[q]);
         } else {
-            for (int q = l; q < nbp; ++q) cfma(s, T[l + (int64_t)q * ldt], sW[q + 32 * j]);
+            for (int q = l; q < nbp; ++q) cfma(s, T[l + (int64_t)q * ldt], sW[q]);
         }
     }
     W[o] = s;
@@ -468,7 +476,7 @@ extern "C" hipError_t bse_apply_q_thin(hipStream_t st, int n, int nb, const doub
         const int G = (int)ceil_div64(m, PQ_ROWS);
         hipLaunchKernelGGL(k_pq_dots, dim3((unsigned)G), dim3(256), 0, st, m, nbp, k, V,
                            (const double2*)(X + j0), ldx, P);
-        hipLaunchKernelGGL(k_pq_small, dim3(1), dim3(1024), 0, st, nbp, k, G, (const double2*)P, T, nb,
+        hipLaunchKernelGGL(k_pq_small, dim3((unsigned)k), dim3(256), 0, st, nbp, k, G, (const double2*)P, T, nb,
                            adjoint, W);
         hipLaunchKernelGGL(k_pq_update, dim3((unsigned)ceil_div64(m, 256)), dim3(256), 0, st, m, nbp, k, V,
                            (const double2*)W, X + j0, ldx);

@@ -31,12 +31,12 @@ __device__ inline double fast_rcp(double b) {
 }
 
 // Count of eigenvalues of block [bs, bs+nbk) strictly below x.  The
-// recurrence is a chain of dependent operations; its inputs are read eight
-// steps at a time, the next eight while the current ones are consumed.
+// recurrence is a chain of dependent operations; its inputs are read sixteen
+// steps at a time, the next sixteen while the current ones are consumed.
 __device__ inline int sturm_count(const double* __restrict__ d, const double* __restrict__ e2,
                                   int bs, int nbk, double x, double pivmin)
 {
-    constexpr int SB = 8;
+    constexpr int SB = 16;
     double q = d[bs] - x;
     if (fabs(q) <= pivmin) q = -pivmin;
     int cnt = q < 0.0;
