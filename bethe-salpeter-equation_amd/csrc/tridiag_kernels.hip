@@ -145,6 +145,13 @@ __device__ inline double hash_uniform(uint64_t a, uint64_t b) {
     return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
+// Rows per memory round trip of the inverse iteration: with one thread per
+// eigenvalue the kernel runs about one wave per CU, so its rate is the number
+// of loads each wave keeps in flight.
+#ifndef IIB
+#define IIB 16
+#endif
+
 // Inverse iteration.  Thread t of the batch owns sorted eigenvalue m0 + t in
 // block (bs, nb).  Scratch arrays are indexed [global row * bsz + t].
 __global__ __launch_bounds__(64) void k_inverse_iter(
@@ -165,14 +172,13 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
         x[(int64_t)bs * S + t] = 1.0;
         return;
     }
-    // LU with partial pivoting of T_block - lam I.
+    // LU with partial pivoting of T_block - lam I.  The step takes e(k),
+    // d(k+1), e(k+1); they are read IIB steps at a time (every thread of the
+    // block reads the same ones) so that the chain does not wait on them.
     double cur_d = d[bs] - lm;
     double cur_s = e[bs];
-    for (int k = 0; k < nb - 1; ++k) {
+    auto lu_step = [&](int k, double sub, double nd, double ns) {
         const int64_t gi = bs + k;
-        const double sub = e[gi];
-        const double nd = d[gi + 1] - lm;
-        const double ns = (k + 1 < nb - 1) ? e[gi + 1] : 0.0;
         double a0, a1, a2, m, fl;
         if (fabs(cur_d) >= fabs(sub)) {
             fl = 0.0;
@@ -193,6 +199,22 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
         u2[gi * S + t] = a2;
         mul[gi * S + t] = m;
         pf[gi * S + t] = fl;
+    };
+    {
+        int k = 0;
+        for (; k + IIB <= nb - 1; k += IIB) {
+            double ev[IIB + 1], dv[IIB];
+#pragma unroll
+            for (int u = 0; u < IIB; ++u) {
+                ev[u] = e[bs + k + u];
+                dv[u] = d[bs + k + u + 1] - lm;
+            }
+            ev[IIB] = (k + IIB < nb - 1) ? e[bs + k + IIB] : 0.0;
+#pragma unroll
+            for (int u = 0; u < IIB; ++u) lu_step(k + u, ev[u], dv[u], ev[u + 1]);
+        }
+        for (; k < nb - 1; ++k)
+            lu_step(k, e[bs + k], d[bs + k + 1] - lm, (k + 1 < nb - 1) ? e[bs + k + 1] : 0.0);
     }
     {
         double a0 = cur_d;
@@ -207,7 +229,6 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
     // The solves walk the column in steps of IIB rows: all loads of a step are
     // issued before its recurrence runs (the compiler cannot move them above
     // the stores into x), so each step costs one memory round trip.
-    constexpr int IIB = 8;
     for (int it = 0; it < 3; ++it) {
         double cur = x[(int64_t)bs * S + t] * s;
         int k = 0;
