@@ -483,7 +483,7 @@ extern "C" int bse_batch_prepare(bse_ctx* ctx, int n_systems, const bse_tb_model
         HCK(hipMemcpyAsync(d.kpts, h.g.kpts.data(), sizeof(double) * h.g.kpts.size(), hipMemcpyHostToDevice, st));
         HCK(hipMemcpyAsync(d.gvec, h.g.gvec.data(), sizeof(double) * h.g.gvec.size(), hipMemcpyHostToDevice, st));
         HetrdSys& x = hsys[s];
-        x.n = (int)h.g.N; x.A = d.H; x.Vstore = d.Vst; x.Tall = d.Tall; x.Wp = d.Wp;
+        x.n = (int)h.g.N; x.g_stride = 0; x.A = d.H; x.Vstore = d.Vst; x.Tall = d.Tall; x.Wp = d.Wp;
         x.alpha2 = d.alpha2; x.G = d.G; x.acol = d.acol; x.d = d.d; x.e = d.e; x.tau = d.tauh;
         x.scal = d.scal; x.part = d.part; x.counter = d.counter; x.Yr = d.Yr; x.Yc = d.Yc; x.t12 = d.t12;
     }

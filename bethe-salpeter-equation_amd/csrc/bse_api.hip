@@ -458,7 +458,7 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
         pev = ctx->prof_ev.data();
     }
     double2* Tall = BUF(double2, S_TALL, (size_t)std::max(npan, 1) * nb * nb);
-    double2* Gram = BUF(double2, S_GRAM, (size_t)64 * 64);
+    double2* Gram = BUF(double2, S_GRAM, (size_t)std::max(npan, 1) * 64 * 64);   // one block per panel
     double2* alpha2 = BUF(double2, S_ALPHA2, 64);
     {
         int rc = autotune_hemv(ctx, fp32, n, H, acol, Yr, Yc);

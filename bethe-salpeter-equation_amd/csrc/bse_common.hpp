@@ -280,6 +280,11 @@ __device__ __attribute__((always_inline)) inline void larft_body(int nb, const d
 // workspace of one matrix of an uneven batch; filled on the host).
 struct HetrdSys {
     int n;
+    // 0: one 64 x 64 Gram block (G) reused by every panel and the panel's
+    // compact-WY factor formed when the panel closes; > 0: one Gram block per
+    // panel at this stride (elements) and every factor formed after the
+    // reduction (off the column-step chain)
+    int g_stride;
     void* A;          // tiled lower storage
 
     double2* Vstore;

@@ -775,16 +775,31 @@ __device__ __attribute__((always_inline)) inline void reduce_body(const HetrdCol
 }
 
 // Panel close: alpha2 of the panel's last column (from its reduce partials,
-// red_gx of them) and the compact-WY factor of the panel.  One block of 64.
+// red_gx of them) and, if form_t, the compact-WY factor of the panel.  One
+// block of 64.
 __device__ __attribute__((always_inline)) inline void panel_close_body(const HetrdCol& h, int nbp, int red_gx,
-                                                                        double2* __restrict__ T, int ldt)
+                                                                        double2* __restrict__ T, int ldt,
+                                                                        bool form_t)
 {
     __shared__ double sh[32];
     __shared__ double2 col[64];
     const int cl = h.j0 + nbp - 1;
     const double2 s = block_sum_partials((const double2*)(h.part + part_layout(h.n).c), red_gx, sh);
     if (threadIdx.x == 0) h.alpha2[nbp - 1] = cscale(cmul(h.tau[cl], s), -0.5);
-    larft_body(nbp, h.tau + h.j0, h.G, T, ldt, col);
+    if (form_t) larft_body(nbp, h.tau + h.j0, h.G, T, ldt, col);
+}
+
+// Compact-WY factors of every panel of an n x n reduction after it (panel p
+// = blockIdx.x, Gram block at G + p g_stride).  One block of 64 per panel.
+__global__ __launch_bounds__(64) void k_larft_all(int n, int nb, const double2* __restrict__ tau,
+                                                  const double2* __restrict__ G, int64_t g_stride,
+                                                  double2* __restrict__ Tall)
+{
+    __shared__ double2 col[64];
+    const int p = blockIdx.x, j0 = p * nb;
+    if (j0 >= n - 1) return;
+    const int nbp = min(nb, n - 1 - j0);
+    larft_body(nbp, tau + j0, G + (int64_t)p * g_stride, Tall + (size_t)p * nb * nb, nb, col);
 }
 
 
@@ -1184,7 +1199,7 @@ __device__ inline bool batch_col(const HetrdSys& s, int c, int nb, HetrdCol& h) 
     const int p = c / nb;
     h.n = s.n; h.j0 = p * nb; h.i = c - h.j0; h.m = s.n - h.j0;
     h.Vp = s.Vstore + panel_voff(s.n, nb, p);
-    h.Wp = s.Wp; h.alpha2 = s.alpha2; h.G = s.G; h.acol = s.acol; h.d = s.d; h.e = s.e;
+    h.Wp = s.Wp; h.alpha2 = s.alpha2; h.G = s.G + (int64_t)p * s.g_stride; h.acol = s.acol; h.d = s.d; h.e = s.e;
     h.tau = s.tau; h.scal = s.scal; h.part = s.part; h.counter = s.counter;
     return true;
 }
@@ -1245,7 +1260,7 @@ __global__ __launch_bounds__(64) void kb_larft(const HetrdSys* __restrict__ sys,
     if (!batch_panel_end(s, c, nb, j0, nbp, p)) return;
     HetrdCol h;
     batch_col(s, c, nb, h);
-    panel_close_body(h, nbp, tile_count(s.n) - ((c + 1) >> 6), s.Tall + (size_t)p * nb * nb, nb);
+    panel_close_body(h, nbp, tile_count(s.n) - ((c + 1) >> 6), s.Tall + (size_t)p * nb * nb, nb, s.g_stride == 0);
 }
 
 // KIND: 0 FP64 MFMA on 64 x 64 tiles, 1 FP64 VALU, 2 FP64 MFMA on 128 x 128
@@ -1324,8 +1339,10 @@ static_assert(sizeof(HetrdSys) <= 256, "HetrdSys must fit the 256-byte device sl
 // kernels with one matrix, whose description (HetrdSys) is copied to d_sys
 // (at least 256 bytes of device memory) first, so that every launch carries
 // only a pointer and the column index (large kernel arguments cost about a
-// microsecond per launch).  Tall receives one nb x nb compact-WY factor per panel;
-// prof_ev (optional) gets 2 events per column around the hemv launch.
+// microsecond per launch).  Tall receives one nb x nb compact-WY factor per panel,
+// formed in one launch after the last column from the per-panel Gram blocks
+// (G: ceil((n-1)/nb) blocks of 64 x 64); prof_ev (optional) gets 2 events per
+// column around the hemv launch.
 extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int nb,
                                 double* d, double* e, double2* tau, double2* Vstore,
                                 double2* Tall, double2* Wp, double2* alpha2,
@@ -1338,7 +1355,7 @@ extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int nb
     hipError_t err;
     if (n > 1) {
         HetrdSys x;
-        x.n = n; x.A = A; x.Vstore = Vstore; x.Tall = Tall; x.Wp = Wp; x.alpha2 = alpha2; x.G = G;
+        x.n = n; x.g_stride = 64 * 64; x.A = A; x.Vstore = Vstore; x.Tall = Tall; x.Wp = Wp; x.alpha2 = alpha2; x.G = G;
         x.acol = acol; x.d = d; x.e = e; x.tau = tau; x.scal = scal; x.part = part;
         x.counter = counter; x.Yr = Yr; x.Yc = Yc; x.t12 = t12;
         if ((err = hipMemcpyAsync(d_sys, &x, sizeof(x), hipMemcpyHostToDevice, st)) != hipSuccess)
@@ -1355,6 +1372,9 @@ extern "C" hipError_t bse_hetrd(hipStream_t st, int fp32, int n, void* A, int nb
             if ((err = bse_hetrd_batch_panel(st, fp32, c, nb, 1, d_sys, nT, her2k_kind)) != hipSuccess)
                 return err;
         }
+        const int npan = (int)ceil_div64(n - 1, nb);
+        hipLaunchKernelGGL(k_larft_all, dim3((unsigned)npan), dim3(64), 0, st, n, nb, (const double2*)tau,
+                           (const double2*)G, (int64_t)64 * 64, Tall);
     }
     hipLaunchKernelGGL(k_last_diag, dim3(1), dim3(64), 0, st, n, (const void*)A, fp32, d);
     return hipGetLastError();
