@@ -30,17 +30,20 @@
 //               norm partials                                  -> region A
 //   hemv(i):    partials of A22 x (x: the updated column below its first
 //               subdiagonal entry, before scaling)             -> Yr, Yc;
-//               the first workgroup of each tile column also forms partial
-//               W^H x, V^H x                                   -> region B
-//   reduce(i):  reflector scalars from region A; v_i = e_1 + sc x (stored);
+//               the first workgroups also form partial W^H x,
+//               V^H x (region B), and workgroup (0, 0) the
+//               reflector scalars from region A                -> d, e, tau
+//   reduce(i):  v_i = e_1 + sc x (stored);
 //               A22 v = sc (A22 x) + A22 e_1; W^H v, V^H v from region B;
 //               W(:, i); partial y^H v                         -> region C
 // The reflector's scale sc = 1/(alpha - beta) depends on the column norm (a
 // grid-wide sum), so the product is formed with the unscaled column and
 // rescaled afterwards: the hemv needs no scalars and starts streaming at
 // once.
-// and at the panel end panel_close (alpha2 of the last column from region C,
-// compact-WY factor) before the trailing update.
+// and at the panel end panel_close (alpha2 of the last column from region C;
+// in the batched path also the compact-WY factor, which the single-matrix
+// path forms for all panels after the last column) before the trailing
+// update.
 #include "bse_common.hpp"
 #include <cstdlib>
 
