@@ -239,13 +239,16 @@ extern "C" int bse_assemble_mfma_supported(int nv, int nc, int norb);
 // Matrix-free FP64 product Y = H X for k <= 16 vectors on matrix cores (the
 // FP64 refinement of the mixed path).  Workgroup (ti, g) owns the TR rows of
 // k-point block ti and the column blocks tj = g, g + ncg, ...: for each it
-// forms the TR x TR tile of H exactly as the FP64 assembly does (same two
-// chained MFMA contractions per (k, k') pair, exchange and diagonal in FP64)
+// forms the TR x TR tile of the screened direct term of H exactly as the
+// FP64 assembly does (same two chained MFMA contractions per (k, k') pair)
 // but into LDS, and multiplies it by the tile's rows of X with
 // v_mfma_f64_16x16x4 (wave w: rows 16w .. 16w+15, all 16 vectors):
 //   Yr += Hr Xr - Hi Xi,   Yi += Hr Xi + Hi Xr.
-// Partial results per column group go to Yp[g] (N x 16); k_apply_sum adds
-// them in a fixed order.  Requires TR <= 64 (bse_assemble_mfma_tk).
+// Partial results per column group go to Yp[g] (N x 16).  The exchange term
+// has rank norb, H^x = beta PV P^H, so it is applied as PV (P^H X) outside
+// the tile loop: k_apply_px / k_apply_pxsum form Z = P^H X (norb x k, fixed
+// summation order), and k_apply_sum adds, in a fixed order, the partials,
+// beta PV Z and the diagonal dE X.  Requires TR <= 64 (bse_assemble_mfma_tk).
 // ---------------------------------------------------------------------------
 #define AM_K 16
 __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __restrict__ X, int64_t ldx, int k,
@@ -259,9 +262,7 @@ __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __r
     const int k0 = ti * TK;
     double2* sCr = smem;                       // [TK][ns][8]
     double2* sCc = sCr + TK * ns * 8;          // [TK][ns][8]
-    double2* sPV = sCc + TK * ns * 8;          // [TR][8]
-    double2* sP = sPV + TR * 8;                // [TR][8]
-    double* sT = (double*)(sP + TR * 8);       // per-wave 16x17 transposition
+    double* sT = (double*)(sCc + TK * ns * 8); // per-wave 16x17 transposition
     double* sHr = sT + 4 * 272;                // [64][65]
     double* sHi = sHr + 64 * 65;
     double* sXr = sHi + 64 * 65;               // [64][16]
@@ -271,11 +272,6 @@ __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __r
         const int i = e & 7, sl = (e >> 3) % ns, kk = e / (8 * ns);
         const int kq = k0 + kk;
         sCr[e] = (kq < a.nk && i < no) ? a.coef[((size_t)kq * ns + sl) * no + i] : cmk(0, 0);
-    }
-    for (int e = tid; e < TR * 8; e += 256) {
-        const int i = e & 7, rr = e >> 3;
-        const int64_t row = (int64_t)k0 * B + rr;
-        sPV[e] = (row < a.N && i < no) ? a.PV[row * no + i] : cmk(0, 0);
     }
     // rows and columns of the 64 x 64 LDS tile beyond TR stay zero
     for (int e = tid; e < 64 * 65; e += 256) { sHr[e] = 0.0; sHi[e] = 0.0; }
@@ -290,11 +286,6 @@ __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __r
             const int i = e & 7, sl = (e >> 3) % ns, kk = e / (8 * ns);
             const int kp = kp0 + kk;
             sCc[e] = (kp < a.nk && i < no) ? a.coef[((size_t)kp * ns + sl) * no + i] : cmk(0, 0);
-        }
-        for (int e = tid; e < TR * 8; e += 256) {
-            const int i = e & 7, rr = e >> 3;
-            const int64_t col = (int64_t)kp0 * B + rr;
-            sP[e] = (col < a.N && i < no) ? a.P[col * no + i] : cmk(0, 0);
         }
         for (int e = tid; e < 64 * AM_K; e += 256) {
             const int rr = e >> 4, j = e & 15;
@@ -367,10 +358,7 @@ __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __r
                 double2 hv = cmk(0, 0);
                 if (pok) {
                     hv = cmk(a.alpha * dr[reg], a.alpha * di[reg]);
-                    double2 x = cmk(0, 0);
-                    for (int j = 0; j < no; ++j) cfma(x, sPV[rl * 8 + j], cconj(sP[cl * 8 + j]));
-                    hv = cadd(hv, cscale(x, a.beta));
-                    if (grow == gcol) { hv.x += a.dE[grow]; hv.y = 0.0; }
+                    if (grow == gcol) hv.y = 0.0;   // the direct term's diagonal is real
                 }
                 sHr[rl * 65 + cl] = hv.x;
                 sHi[rl * 65 + cl] = hv.y;
@@ -398,7 +386,40 @@ __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __r
     }
 }
 
-__global__ __launch_bounds__(256) void k_apply_sum(int64_t N, int k, int ncg, const double2* __restrict__ Yp,
+// Z partials: block g sums conj(P(r, i)) X(r, j) over its rows (two row
+// phases per (i, j)); 128 entries (i < 8, j < 16) per block.
+#define APX_BLOCKS 256
+__global__ __launch_bounds__(256) void k_apply_px(int64_t N, int no, int k, const double2* __restrict__ P,
+                                                  const double2* __restrict__ X, int64_t ldx,
+                                                  double2* __restrict__ Zp)
+{
+    __shared__ double2 s[256];
+    const int t = threadIdx.x, i = t & 7, j = (t >> 3) & 15, h = t >> 7;
+    const int64_t per = ceil_div64(N, (int64_t)gridDim.x);
+    const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(N, r0 + per);
+    double2 acc = cmk(0, 0);
+    if (i < no && j < k)
+        for (int64_t r = r0 + h; r < r1; r += 2) cfmac(acc, P[r * no + i], X[r + (int64_t)j * ldx]);
+    s[t] = acc;
+    __syncthreads();
+    if (h == 0) Zp[(int64_t)blockIdx.x * 128 + t] = cadd(s[t], s[t + 128]);
+}
+
+// Z(i, j) = sum over the G partials in order (one thread per entry).
+__global__ __launch_bounds__(128) void k_apply_pxsum(int G, const double2* __restrict__ Zp,
+                                                     double2* __restrict__ Z)
+{
+    double2 z = cmk(0, 0);
+    for (int g = 0; g < G; ++g) z = cadd(z, Zp[(int64_t)g * 128 + threadIdx.x]);
+    Z[threadIdx.x] = z;
+}
+
+// Y(r, j) = sum_g Yp[g](r, j) + beta sum_i PV(r, i) Z(i, j) + dE(r) X(r, j).
+__global__ __launch_bounds__(256) void k_apply_sum(int64_t N, int k, int ncg, int no,
+                                                   const double2* __restrict__ Yp,
+                                                   const double2* __restrict__ PV,
+                                                   const double2* __restrict__ Z, const double* __restrict__ dE,
+                                                   double beta, const double2* __restrict__ X, int64_t ldx,
                                                    double2* __restrict__ Y, int64_t ldy)
 {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -408,12 +429,18 @@ __global__ __launch_bounds__(256) void k_apply_sum(int64_t N, int k, int ncg, co
     if (j >= k) return;
     double2 s = cmk(0, 0);
     for (int gg = 0; gg < ncg; ++gg) s = cadd(s, Yp[(int64_t)gg * N * AM_K + e]);
+    double2 x = cmk(0, 0);
+    for (int i = 0; i < no; ++i) cfma(x, PV[r * no + i], Z[i + 8 * j]);
+    const double2 xr = X[r + (int64_t)j * ldx];
+    s = cadd(s, cscale(x, beta));
+    s = cadd(s, cscale(xr, dE[r]));
     Y[r + (int64_t)j * ldy] = s;
 }
 
 static size_t apply_mfma_lds_bytes(int TK, int nv, int nc) {
     const int ns = nv + nc, B = nv * nc;
-    return ((size_t)2 * TK * ns * 8 + (size_t)2 * TK * B * 8) * sizeof(double2) +
+    (void)B;
+    return ((size_t)2 * TK * ns * 8) * sizeof(double2) +
            ((size_t)4 * 272 + 2 * 64 * 65 + 2 * 64 * 16) * sizeof(double);
 }
 
@@ -424,7 +451,7 @@ extern "C" int bse_apply_mfma_groups(int nk, int nv, int nc) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(nt, ceil_div64(1024, nt)));
 }
 extern "C" int64_t bse_apply_mfma_workspace(int nk, int nv, int nc) {
-    return (int64_t)bse_apply_mfma_groups(nk, nv, nc) * nk * nv * nc * AM_K;
+    return (int64_t)bse_apply_mfma_groups(nk, nv, nc) * nk * nv * nc * AM_K + (APX_BLOCKS + 1) * 128;
 }
 
 extern "C" hipError_t bse_launch_apply_mfma(hipStream_t st, int nk, int n1, int n2, int nv, int nc,
@@ -444,10 +471,14 @@ extern "C" hipError_t bse_launch_apply_mfma(hipStream_t st, int nk, int n1, int 
     const int nt = (int)ceil_div64(nk, a.TK);
     const int ncg = bse_apply_mfma_groups(nk, nv, nc);
     const size_t lds = apply_mfma_lds_bytes(a.TK, nv, nc);
+    double2* Zp = ws + (int64_t)ncg * a.N * AM_K;   // exchange: Z = P^H X
+    double2* Z = Zp + (int64_t)APX_BLOCKS * 128;
+    hipLaunchKernelGGL(k_apply_px, dim3(APX_BLOCKS), dim3(256), 0, st, a.N, norb, k, P, X, ldx, Zp);
+    hipLaunchKernelGGL(k_apply_pxsum, dim3(1), dim3(128), 0, st, APX_BLOCKS, (const double2*)Zp, Z);
     hipLaunchKernelGGL(k_apply_mfma, dim3((unsigned)nt, (unsigned)ncg), dim3(256), lds, st, a, X, ldx, k, ncg,
                        ws);
     hipLaunchKernelGGL(k_apply_sum, dim3((unsigned)ceil_div64(a.N * AM_K, 256)), dim3(256), 0, st, a.N, k,
-                       ncg, (const double2*)ws, Y, ldy);
+                       ncg, norb, (const double2*)ws, PV, (const double2*)Z, dE, beta, X, ldx, Y, ldy);
     return hipGetLastError();
 }
 
