@@ -21,14 +21,16 @@ __device__ void sd_jacobi(int n, double2 (*A)[SD_MAX + 1], double2 (*V)[SD_MAX +
     __shared__ double rc[SD_MAX / 2], rs[SD_MAX / 2];
     __shared__ double2 ru[SD_MAX / 2];
     __shared__ double sh[32];
-    __shared__ int order[SD_MAX];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int m = (n + 1) & ~1;   // even number of slots; slot n (if n odd) is a dummy
     for (int e = tid; e < SD_MAX * SD_MAX; e += nt) {
         const int i = e / SD_MAX, j = e % SD_MAX;
         V[i][j] = cmk(i == j ? 1.0 : 0.0, 0.0);
     }
-    for (int i = tid; i < m; i += nt) order[i] = i;
+    // round-robin order at step r of a sweep: slot 0 holds 0, slot i >= 1
+    // holds ((i - 1 - r) mod (m - 1)) + 1 (slots 1.. shift right by one per
+    // step; after m - 1 steps the order is the identity again)
+    auto slot = [m](int i, int r) { return i == 0 ? 0 : ((i - 1 - r) % (m - 1) + (m - 1)) % (m - 1) + 1; };
     __syncthreads();
     double fro = 0.0;
     for (int e = tid; e < n * n; e += nt) fro += cabs2(A[e / n][e % n]);
@@ -44,7 +46,7 @@ __device__ void sd_jacobi(int n, double2 (*A)[SD_MAX + 1], double2 (*V)[SD_MAX +
         for (int step = 0; step < m - 1; ++step) {
             // pairs from the round-robin order: (order[t], order[m-1-t])
             if (tid < m / 2) {
-                int p = order[tid], q = order[m - 1 - tid];
+                int p = slot(tid, step), q = slot(m - 1 - tid, step);
                 if (p > q) { const int x = p; p = q; q = x; }
                 pr[tid][0] = p;
                 pr[tid][1] = q;
@@ -81,7 +83,8 @@ __device__ void sd_jacobi(int n, double2 (*A)[SD_MAX + 1], double2 (*V)[SD_MAX +
                 V[i][q] = cadd(cscale(vip, s), cmul(viq, cu));
             }
             __syncthreads();
-            // rows: J^H (A J)
+            // rows: J^H (A J); the rotated pair's off-diagonal entries are set
+            // to zero and its diagonal to real (each entry has one writer)
             for (int e = tid; e < (m / 2) * n; e += nt) {
                 const int t = e / n, j = e % n;
                 const int p = pr[t][0], q = pr[t][1];
@@ -89,25 +92,12 @@ __device__ void sd_jacobi(int n, double2 (*A)[SD_MAX + 1], double2 (*V)[SD_MAX +
                 const double c = rc[t], s = rs[t];
                 const double2 uc = cconj(ru[t]);
                 const double2 bpj = A[p][j], bqj = A[q][j];
-                A[p][j] = csub(cscale(bpj, c), cmul(bqj, cscale(uc, s)));
-                A[q][j] = cadd(cscale(bpj, s), cmul(bqj, cscale(uc, c)));
-            }
-            __syncthreads();
-            // clean the rotated pair and rotate the round-robin order (slot 0 fixed)
-            if (tid < m / 2) {
-                const int p = pr[tid][0], q = pr[tid][1];
-                if (q < n && rs[tid] != 0.0) {
-                    A[p][q] = cmk(0, 0);
-                    A[q][p] = cmk(0, 0);
-                    A[p][p].y = 0.0;
-                    A[q][q].y = 0.0;
-                }
-            }
-            __syncthreads();
-            if (tid == 0) {
-                const int last = order[m - 1];
-                for (int i = m - 1; i > 1; --i) order[i] = order[i - 1];
-                order[1] = last;
+                double2 np = csub(cscale(bpj, c), cmul(bqj, cscale(uc, s)));
+                double2 nq = cadd(cscale(bpj, s), cmul(bqj, cscale(uc, c)));
+                if (j == q) { np = cmk(0, 0); nq.y = 0.0; }
+                if (j == p) { nq = cmk(0, 0); np.y = 0.0; }
+                A[p][j] = np;
+                A[q][j] = nq;
             }
             __syncthreads();
         }
@@ -137,7 +127,7 @@ __device__ void sd_sort(int n, double2 (*A)[SD_MAX + 1], double* w, int* perm)
 // Rayleigh-Ritz with SVQB: inputs G, Hs (m x m, ld m, device), outputs the
 // lowest k Ritz values (evals) and the m x k coefficient matrix C (ld m).
 // Returns the rank kept in info[0].
-__global__ __launch_bounds__(256) void k_small_rr(int m, int k, const double2* __restrict__ G,
+__global__ __launch_bounds__(512) void k_small_rr(int m, int k, const double2* __restrict__ G,
                                                  const double2* __restrict__ Hs, double tol,
                                                  double* __restrict__ evals,
                                                  double2* __restrict__ C, int* __restrict__ info)
@@ -217,7 +207,7 @@ extern "C" hipError_t bse_launch_small_rr(hipStream_t st, int m, int k, const do
                                           double2* C, int* info)
 {
     if (m > SD_MAX || k > m) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_small_rr, dim3(1), dim3(256), 0, st, m, k, G, Hs, tol, evals, C, info);
+    hipLaunchKernelGGL(k_small_rr, dim3(1), dim3(512), 0, st, m, k, G, Hs, tol, evals, C, info);
     return hipGetLastError();
 }
 
