@@ -272,17 +272,22 @@ extern "C" hipError_t bse_launch_rows_from_reference(hipStream_t st, int64_t N, 
 // Y <- Q Y (adjoint = 0; decreasing order, T), with Q = prod_p (I - V_p T_p V_p^H)
 // from the packed FP64 reflector store (panel p at nb (p n - nb p (p-1)/2),
 // ld n - p nb) and the per-panel compact-WY factors (nb x nb, ld nb).
-// One workgroup per vector walks all panels; for each it forms
-// w1 = V^H y (wave sums), w2 = T^(H) w1 and y -= V w2.  For the small
+// One workgroup (APQ_THREADS threads; 512 keep the 32 partial dots of a
+// thread in registers, 1024 spill them) per vector walks all panels; for
+// each it forms w1 = V^H y (wave sums), w2 = T^(H) w1 and y -= V w2.  For the small
 // matrices of the batched sweep, where one vector is a few MB of reflector
 // reads, this replaces three GEMM launches per panel.  nb <= 32.
 // ---------------------------------------------------------------------------
 #define APQ_NB 32
-__global__ __launch_bounds__(256) void k_apply_q(int n, int nb, const double2* __restrict__ Vstore,
+#ifndef APQ_THREADS
+#define APQ_THREADS 512
+#endif
+__global__ __launch_bounds__(APQ_THREADS) void k_apply_q(int n, int nb, const double2* __restrict__ Vstore,
                                                  const double2* __restrict__ Tall,
                                                  double2* __restrict__ Y, int64_t ldy, int adjoint)
 {
-    __shared__ double2 s_red[4][APQ_NB];
+    constexpr int NW = APQ_THREADS / 64;
+    __shared__ double2 s_red[NW][APQ_NB];
     __shared__ double2 s_w1[APQ_NB], s_w2[APQ_NB];
     double2* y = Y + (int64_t)blockIdx.x * ldy;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -295,7 +300,7 @@ __global__ __launch_bounds__(256) void k_apply_q(int n, int nb, const double2* _
         double2 acc[APQ_NB];
 #pragma unroll
         for (int l = 0; l < APQ_NB; ++l) acc[l] = cmk(0, 0);
-        for (int r = threadIdx.x; r < m; r += 256) {
+        for (int r = threadIdx.x; r < m; r += APQ_THREADS) {
             const double2 yr = y[j0 + r];
 #pragma unroll
             for (int l = 0; l < APQ_NB; ++l)
@@ -309,7 +314,9 @@ __global__ __launch_bounds__(256) void k_apply_q(int n, int nb, const double2* _
         __syncthreads();
         if (threadIdx.x < APQ_NB) {
             const int l = threadIdx.x;
-            s_w1[l] = cadd(cadd(s_red[0][l], s_red[1][l]), cadd(s_red[2][l], s_red[3][l]));
+            double2 t = s_red[0][l];
+            for (int ww = 1; ww < NW; ++ww) t = cadd(t, s_red[ww][l]);
+            s_w1[l] = t;
         }
         __syncthreads();
         if (threadIdx.x < nbp) {
@@ -323,7 +330,7 @@ __global__ __launch_bounds__(256) void k_apply_q(int n, int nb, const double2* _
             s_w2[l] = s;
         }
         __syncthreads();
-        for (int r = threadIdx.x; r < m; r += 256) {
+        for (int r = threadIdx.x; r < m; r += APQ_THREADS) {
             double2 s = cmk(0, 0);
             for (int l = 0; l < nbp; ++l) cfma(s, V[r + (int64_t)l * m], s_w2[l]);
             y[j0 + r] = csub(y[j0 + r], s);
@@ -338,7 +345,7 @@ extern "C" hipError_t bse_launch_apply_q(hipStream_t st, int n, int nb, const do
 {
     if (nb > APQ_NB || nvec <= 0) return hipErrorInvalidValue;
     if (n < 2) return hipSuccess;
-    hipLaunchKernelGGL(k_apply_q, dim3((unsigned)nvec), dim3(256), 0, st, n, nb, Vstore, Tall, Y,
+    hipLaunchKernelGGL(k_apply_q, dim3((unsigned)nvec), dim3(APQ_THREADS), 0, st, n, nb, Vstore, Tall, Y,
                        ldy, adjoint);
     return hipGetLastError();
 }
