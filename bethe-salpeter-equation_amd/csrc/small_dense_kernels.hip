@@ -14,12 +14,16 @@
 #define SD_MAX 32
 
 // Jacobi on the LDS matrix A (ld SD_MAX), eigenvectors into V.  n <= SD_MAX.
-// On exit the diagonal of A holds the eigenvalues (unsorted).
+// On exit the diagonal of A holds the eigenvalues (unsorted).  One thread per
+// (pair, row) task: blockDim.x >= SD_JACOBI_THREADS.  A step reads A and
+// writes the rotated columns into A2, then reads A2 and writes the rotated
+// rows back into A, so no phase reads an entry another thread writes; every
+// thread forms its pair's rotation from A itself and keeps it for both
+// phases (two barriers per step).
+#define SD_JACOBI_THREADS ((SD_MAX / 2) * SD_MAX)
 __device__ void sd_jacobi(int n, double2 (*A)[SD_MAX + 1], double2 (*V)[SD_MAX + 1])
 {
-    __shared__ int pr[SD_MAX / 2][2];
-    __shared__ double rc[SD_MAX / 2], rs[SD_MAX / 2];
-    __shared__ double2 ru[SD_MAX / 2];
+    __shared__ double2 A2[SD_MAX][SD_MAX + 1];
     __shared__ double sh[32];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int m = (n + 1) & ~1;   // even number of slots; slot n (if n odd) is a dummy
@@ -35,69 +39,72 @@ __device__ void sd_jacobi(int n, double2 (*A)[SD_MAX + 1], double2 (*V)[SD_MAX +
     double fro = 0.0;
     for (int e = tid; e < n * n; e += nt) fro += cabs2(A[e / n][e % n]);
     fro = block_sum(fro, sh);
+    const bool act = tid < (m / 2) * n;
+    const int t = act ? tid / n : 0, i = act ? tid % n : 0;
     for (int sweep = 0; sweep < 30; ++sweep) {
         double off = 0.0;
         for (int e = tid; e < n * n; e += nt) {
-            const int i = e / n, j = e % n;
-            if (i != j) off += cabs2(A[i][j]);
+            const int r = e / n, j = e % n;
+            if (r != j) off += cabs2(A[r][j]);
         }
         off = block_sum(off, sh);
         if (off <= 1e-30 * fro) break;
         for (int step = 0; step < m - 1; ++step) {
-            // pairs from the round-robin order: (order[t], order[m-1-t])
-            if (tid < m / 2) {
-                int p = slot(tid, step), q = slot(m - 1 - tid, step);
-                if (p > q) { const int x = p; p = q; q = x; }
-                pr[tid][0] = p;
-                pr[tid][1] = q;
-                double c = 1.0, s = 0.0;
-                double2 u = cmk(1.0, 0.0);
-                if (q < n) {
-                    const double2 apq = A[p][q];
-                    const double absa = sqrt(cabs2(apq));
-                    if (absa > 0.0) {
-                        const double app = A[p][p].x, aqq = A[q][q].x;
-                        u = cmk(apq.x / absa, -apq.y / absa);
-                        const double theta = (aqq - app) / (2.0 * absa);
-                        double t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
-                        if (theta < 0.0) t = -t;
-                        c = 1.0 / sqrt(1.0 + t * t);
-                        s = t * c;
-                    }
+            // pair t of this step: (slot t, slot m-1-t); rotation from A
+            int p = slot(t, step), q = slot(m - 1 - t, step);
+            if (p > q) { const int x = p; p = q; q = x; }
+            double c = 1.0, s = 0.0;
+            double2 u = cmk(1.0, 0.0);
+            if (act && q < n) {
+                const double2 apq = A[p][q];
+                const double absa = sqrt(cabs2(apq));
+                if (absa > 0.0) {
+                    const double app = A[p][p].x, aqq = A[q][q].x;
+                    u = cmk(apq.x / absa, -apq.y / absa);
+                    const double theta = (aqq - app) / (2.0 * absa);
+                    double tt = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    if (theta < 0.0) tt = -tt;
+                    c = 1.0 / sqrt(1.0 + tt * tt);
+                    s = tt * c;
                 }
-                rc[tid] = c; rs[tid] = s; ru[tid] = u;
+            }
+            const bool rot = act && q < n && s != 0.0;
+            // columns: (A J)(i, {p, q}) into A2, V J in place, with
+            // J = [[c, s], [-s u, c u]] on (p, q); other pairs copied
+            if (act) {
+                const double2 aip = A[i][p];
+                if (rot) {
+                    const double2 aiq = A[i][q];
+                    const double2 su = cscale(u, s), cu = cscale(u, c);
+                    A2[i][p] = csub(cscale(aip, c), cmul(aiq, su));
+                    A2[i][q] = cadd(cscale(aip, s), cmul(aiq, cu));
+                    const double2 vip = V[i][p], viq = V[i][q];
+                    V[i][p] = csub(cscale(vip, c), cmul(viq, su));
+                    V[i][q] = cadd(cscale(vip, s), cmul(viq, cu));
+                } else {
+                    A2[i][p] = aip;
+                    if (q < n) A2[i][q] = A[i][q];
+                }
             }
             __syncthreads();
-            // columns: A J, V J  with J = [[c, s], [-s u, c u]] on (p, q)
-            for (int e = tid; e < (m / 2) * n; e += nt) {
-                const int t = e / n, i = e % n;
-                const int p = pr[t][0], q = pr[t][1];
-                if (q >= n || rs[t] == 0.0) continue;
-                const double c = rc[t], s = rs[t];
-                const double2 su = cscale(ru[t], s), cu = cscale(ru[t], c);
-                const double2 aip = A[i][p], aiq = A[i][q];
-                A[i][p] = csub(cscale(aip, c), cmul(aiq, su));
-                A[i][q] = cadd(cscale(aip, s), cmul(aiq, cu));
-                const double2 vip = V[i][p], viq = V[i][q];
-                V[i][p] = csub(cscale(vip, c), cmul(viq, su));
-                V[i][q] = cadd(cscale(vip, s), cmul(viq, cu));
-            }
-            __syncthreads();
-            // rows: J^H (A J); the rotated pair's off-diagonal entries are set
-            // to zero and its diagonal to real (each entry has one writer)
-            for (int e = tid; e < (m / 2) * n; e += nt) {
-                const int t = e / n, j = e % n;
-                const int p = pr[t][0], q = pr[t][1];
-                if (q >= n || rs[t] == 0.0) continue;
-                const double c = rc[t], s = rs[t];
-                const double2 uc = cconj(ru[t]);
-                const double2 bpj = A[p][j], bqj = A[q][j];
-                double2 np = csub(cscale(bpj, c), cmul(bqj, cscale(uc, s)));
-                double2 nq = cadd(cscale(bpj, s), cmul(bqj, cscale(uc, c)));
-                if (j == q) { np = cmk(0, 0); nq.y = 0.0; }
-                if (j == p) { nq = cmk(0, 0); np.y = 0.0; }
-                A[p][j] = np;
-                A[q][j] = nq;
+            // rows: J^H (A J) back into A; the rotated pair's off-diagonal
+            // entries are set to zero and its diagonal to real
+            if (act) {
+                const int j = i;
+                const double2 bpj = A2[p][j];
+                if (rot) {
+                    const double2 bqj = A2[q][j];
+                    const double2 uc = cconj(u);
+                    double2 np = csub(cscale(bpj, c), cmul(bqj, cscale(uc, s)));
+                    double2 nq = cadd(cscale(bpj, s), cmul(bqj, cscale(uc, c)));
+                    if (j == q) { np = cmk(0, 0); nq.y = 0.0; }
+                    if (j == p) { nq = cmk(0, 0); np.y = 0.0; }
+                    A[p][j] = np;
+                    A[q][j] = nq;
+                } else {
+                    A[p][j] = bpj;
+                    if (q < n) A[q][j] = A2[q][j];
+                }
             }
             __syncthreads();
         }
@@ -108,6 +115,7 @@ __device__ void sd_jacobi(int n, double2 (*A)[SD_MAX + 1], double2 (*V)[SD_MAX +
 // permutation to perm[0..n) (shared).
 __device__ void sd_sort(int n, double2 (*A)[SD_MAX + 1], double* w, int* perm)
 {
+    __shared__ double tmp[SD_MAX];
     if (threadIdx.x == 0) {
         for (int i = 0; i < n; ++i) { perm[i] = i; w[i] = A[i][i].x; }
         for (int i = 1; i < n; ++i) {
@@ -117,7 +125,6 @@ __device__ void sd_sort(int n, double2 (*A)[SD_MAX + 1], double* w, int* perm)
             while (j >= 0 && w[perm[j]] > x) { perm[j + 1] = perm[j]; --j; }
             perm[j + 1] = pi;
         }
-        double tmp[SD_MAX];
         for (int i = 0; i < n; ++i) tmp[i] = w[perm[i]];
         for (int i = 0; i < n; ++i) w[i] = tmp[i];
     }
@@ -127,7 +134,7 @@ __device__ void sd_sort(int n, double2 (*A)[SD_MAX + 1], double* w, int* perm)
 // Rayleigh-Ritz with SVQB: inputs G, Hs (m x m, ld m, device), outputs the
 // lowest k Ritz values (evals) and the m x k coefficient matrix C (ld m).
 // Returns the rank kept in info[0].
-__global__ __launch_bounds__(512) void k_small_rr(int m, int k, const double2* __restrict__ G,
+__global__ __launch_bounds__(SD_JACOBI_THREADS) void k_small_rr(int m, int k, const double2* __restrict__ G,
                                                  const double2* __restrict__ Hs, double tol,
                                                  double* __restrict__ evals,
                                                  double2* __restrict__ C, int* __restrict__ info)
@@ -207,7 +214,7 @@ extern "C" hipError_t bse_launch_small_rr(hipStream_t st, int m, int k, const do
                                           double2* C, int* info)
 {
     if (m > SD_MAX || k > m) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_small_rr, dim3(1), dim3(512), 0, st, m, k, G, Hs, tol, evals, C, info);
+    hipLaunchKernelGGL(k_small_rr, dim3(1), dim3(SD_JACOBI_THREADS), 0, st, m, k, G, Hs, tol, evals, C, info);
     return hipGetLastError();
 }
 
