@@ -28,8 +28,15 @@ extern "C" int bse_create(bse_ctx** out, int device) {
         delete ctx;
         return BSE_ERR_HIP;
     }
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->st);
+        delete ctx;
+        return BSE_ERR_HIP;
+    }
     for (int i = 0; i <= ST_N; ++i) (void)hipEventCreate(&ctx->ev[i]);
     for (int i = 0; i < 2; ++i) (void)hipEventCreate(&ctx->tev[i]);
+    (void)hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming);
     const char* at = getenv("BSE_AUTOTUNE");
     ctx->autotune = !(at && at[0] == '0');
     const char* am = getenv("BSE_ASSEMBLY");
@@ -42,6 +49,7 @@ extern "C" int bse_destroy(bse_ctx* ctx) {
     if (!ctx) return BSE_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->st);
+    (void)hipStreamSynchronize(ctx->side);
     batch_destroy(ctx);
     for (auto& b : ctx->bufs)
         if (b.p) (void)hipFree(b.p);
@@ -50,6 +58,9 @@ extern "C" int bse_destroy(bse_ctx* ctx) {
     for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
     for (int i = 0; i < 2; ++i)
         if (ctx->tev[i]) (void)hipEventDestroy(ctx->tev[i]);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->st);
     delete ctx;
     return BSE_OK;
@@ -499,6 +510,32 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
     HCK(hipStreamSynchronize(st));
     if (!finite_all(hd.data(), hd.size()) || !finite_all(he.data(), he.size()))
         return fail(ctx, BSE_ERR_NUMERIC, "non-finite tridiagonal (input matrix not finite?)");
+    // ---------------- y = Q^H rho for the oscillator contraction, on the side
+    // stream: it needs only the reflectors, so it overlaps the host's block
+    // bookkeeping and the bisection; st waits for it before the first
+    // contraction.  Nothing on st touches y, w1, w2 or skw in between.
+    double2* y = nullptr;
+    if (rho) {
+        y = BUF(double2, S_Y, n);
+        double2* w1 = BUF(double2, S_W1, (size_t)nb * 64);
+        double2* w2 = BUF(double2, S_W2, (size_t)nb * 64);
+        hipStream_t ss = ctx->side;
+        HCK(hipEventRecord(ctx->fork_ev, st));
+        HCK(hipStreamWaitEvent(ss, ctx->fork_ev, 0));
+        HCK(hipMemcpyAsync(y, rho, sizeof(double2) * n, hipMemcpyDeviceToDevice, ss));
+        if (bse_apply_q_thin_workspace(n) <= sk_el) {
+            // one vector: the device-wide thin application (three short kernels per panel)
+            HCK(bse_apply_q_thin(ss, n, nb, Vst, Tall, 1, y, n, 1, skw));
+        } else for (int p = 0; p < npan; ++p) {
+            const int j0 = p * nb, m = n - j0, nbp = std::min(nb, n - 1 - j0);
+            const double2* Vp = Vst + voff[p];
+            const double2* Tp = Tall + (size_t)p * nb * nb;
+            HCK(bse_zgemm(ss, 1, 0, nbp, 1, m, 1, 0, Vp, m, y + j0, n, 0, 0, w1, nbp, skw, sk_el));
+            HCK(bse_zgemm(ss, 1, 0, nbp, 1, nbp, 1, 0, Tp, nb, w1, nbp, 0, 0, w2, nbp, nullptr, 0));
+            HCK(bse_zgemm(ss, 0, 0, m, 1, nbp, -1, 0, Vp, m, w2, nbp, 1, 0, y + j0, n, nullptr, 0));
+        }
+        HCK(hipEventRecord(ctx->join_ev, ss));
+    }
     double tnorm = 0.0, emax2 = 0.0;
     for (int i = 0; i < n; ++i) {
         double r = fabs(hd[i]);
@@ -568,26 +605,6 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
     HCK(hipMemcpyAsync(d_nbv, nbv.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     stage_end(ctx, ST_TEVAL);
 
-    // ---------------- y = Q^H rho for the oscillator contraction
-    double2* y = nullptr;
-    if (rho) {
-        y = BUF(double2, S_Y, n);
-        double2* w1 = BUF(double2, S_W1, (size_t)nb * 64);
-        double2* w2 = BUF(double2, S_W2, (size_t)nb * 64);
-        HCK(hipMemcpyAsync(y, rho, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
-        if (bse_apply_q_thin_workspace(n) <= sk_el) {
-            // one vector: the device-wide thin application (three short kernels per panel)
-            HCK(bse_apply_q_thin(st, n, nb, Vst, Tall, 1, y, n, 1, skw));
-        } else for (int p = 0; p < npan; ++p) {
-            const int j0 = p * nb, m = n - j0, nbp = std::min(nb, n - 1 - j0);
-            const double2* Vp = Vst + voff[p];
-            const double2* Tp = Tall + (size_t)p * nb * nb;
-            HCK(bse_zgemm(st, 1, 0, nbp, 1, m, 1, 0, Vp, m, y + j0, n, 0, 0, w1, nbp, skw, sk_el));
-            HCK(bse_zgemm(st, 1, 0, nbp, 1, nbp, 1, 0, Tp, nb, w1, nbp, 0, 0, w2, nbp, nullptr, 0));
-            HCK(bse_zgemm(st, 0, 0, m, 1, nbp, -1, 0, Vp, m, w2, nbp, 1, 0, y + j0, n, nullptr, 0));
-        }
-    }
-
     // ---------------- K6: tridiagonal eigenvectors by batches
     stage_begin(ctx, ST_TVEC);
     const int nstore = (n_vectors < 0) ? n : std::min(n_vectors, n);
@@ -602,6 +619,7 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
     if (nstore > 0) Z = BUF(double2, S_Z, (size_t)n * nstore);
     double2* fvec = rho ? BUF(double2, S_F, n) : nullptr;
     int2* d_cl = BUF(int2, S_CLUST, bsz);
+    if (rho) HCK(hipStreamWaitEvent(st, ctx->join_ev, 0));   // y = Q^H rho is ready
     int m0 = 0;
     while (m0 < n) {
         int m1 = std::min(n, m0 + bsz);

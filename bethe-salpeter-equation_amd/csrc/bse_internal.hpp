@@ -158,6 +158,10 @@ struct bse_ctx {
     int her2k_valu[2] = {-1, -1};
     bool autotune = true;
     hipEvent_t tev[2] = {};     // autotuner timing pair
+    // second stream (default priority): Q^H rho runs on it while the host
+    // sorts the tridiagonal blocks and the bisection runs on st
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     // batched sweep state (owned by batch_api.hip)
     struct BatchState* batch = nullptr;
 };
