@@ -358,105 +358,146 @@ extern "C" hipError_t bse_launch_apply_q(hipStream_t st, int n, int nb, const do
 //   (2) k_pq_small:  W = T^(H) sum_g P_g (one workgroup per vector; fixed summation order)
 //   (3) k_pq_update: X -= V W
 // Three short launches per panel instead of four GEMM launches; every pass
-// reads V once at streaming rate.
+// reads V once.  The launches are latency-bound rather than bandwidth-bound
+// (a panel's V is at most 32 columns): the row blocks are 64 rows, giving up
+// to n/64 workgroups; every load of a block is issued before the first LDS
+// store; T is staged in LDS; and the update runs one wave per workgroup.
 // ---------------------------------------------------------------------------
-#define PQ_ROWS 128
-#define PQ_CHUNK 64
+#define PQ_ROWS 64
 #define PQ_MAXK 16
+// One block of PQ_ROWS rows per workgroup.  Thread t owns the entries
+// (l, j) = (t % 32, t / 32) and (l, j + 8) of the 32 x 16 partial; even and
+// odd rows go to separate sums (four independent FMA chains per thread),
+// added at the end in a fixed order.
 __global__ __launch_bounds__(256) void k_pq_dots(int m, int nbp, int k, const double2* __restrict__ V,
                                                  const double2* __restrict__ X, int64_t ldx,
                                                  double2* __restrict__ P)
 {
-    __shared__ double2 sV[PQ_CHUNK][APQ_NB + 1];
-    __shared__ double2 sX[PQ_CHUNK][PQ_MAXK];
+    __shared__ double2 sV[PQ_ROWS][APQ_NB + 1];
+    __shared__ double2 sX[PQ_ROWS][PQ_MAXK];
     const int r0 = blockIdx.x * PQ_ROWS;
     const int t = threadIdx.x;
-    double2 acc[2] = {cmk(0, 0), cmk(0, 0)};
-    for (int c0 = r0; c0 < min(m, r0 + PQ_ROWS); c0 += PQ_CHUNK) {
-        for (int e = t; e < PQ_CHUNK * APQ_NB; e += 256) {
-            const int rr = e & (PQ_CHUNK - 1), l = e / PQ_CHUNK;
-            const int r = c0 + rr;
-            sV[rr][l] = (r < m && l < nbp) ? V[r + (int64_t)l * m] : cmk(0, 0);
-        }
-        for (int e = t; e < PQ_CHUNK * PQ_MAXK; e += 256) {
-            const int rr = e & (PQ_CHUNK - 1), j = e / PQ_CHUNK;
-            const int r = c0 + rr;
-            sX[rr][j] = (r < m && j < k) ? X[r + (int64_t)j * ldx] : cmk(0, 0);
-        }
-        __syncthreads();
+    // every load is issued before the first LDS store: addresses are clamped
+    // into the block and out-of-range entries replaced by zero afterwards, so
+    // no load sits behind a branch
+    constexpr int NV = PQ_ROWS * APQ_NB / 256, NX = PQ_ROWS * PQ_MAXK / 256;
+    double2 rv[NV], rx[NX];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int o = t + 256 * u, l = o & 31, j = o >> 5;
-#pragma unroll 8
-            for (int rr = 0; rr < PQ_CHUNK; ++rr) cfmac(acc[u], sV[rr][l], sX[rr][j]);
-        }
-        __syncthreads();
+    for (int i = 0; i < NV; ++i) {
+        const int e = t + 256 * i, rr = e & (PQ_ROWS - 1), l = e / PQ_ROWS, r = r0 + rr;
+        const double2 a = V[min(r, m - 1) + (int64_t)min(l, nbp - 1) * m];
+        rv[i] = (r < m && l < nbp) ? a : cmk(0, 0);
     }
-    P[(int64_t)blockIdx.x * 512 + t] = acc[0];
-    P[(int64_t)blockIdx.x * 512 + t + 256] = acc[1];
+#pragma unroll
+    for (int i = 0; i < NX; ++i) {
+        const int e = t + 256 * i, rr = e & (PQ_ROWS - 1), j = e / PQ_ROWS, r = r0 + rr;
+        const double2 a = X[min(r, m - 1) + (int64_t)min(j, k - 1) * ldx];
+        rx[i] = (r < m && j < k) ? a : cmk(0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int e = t + 256 * i;
+        sV[e & (PQ_ROWS - 1)][e / PQ_ROWS] = rv[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NX; ++i) {
+        const int e = t + 256 * i;
+        sX[e & (PQ_ROWS - 1)][e / PQ_ROWS] = rx[i];
+    }
+    __syncthreads();
+    const int l = t & 31, j = t >> 5;
+    double2 a0 = cmk(0, 0), a1 = cmk(0, 0), b0 = cmk(0, 0), b1 = cmk(0, 0);
+#pragma unroll 8
+    for (int rr = 0; rr < PQ_ROWS; rr += 2) {
+        const double2 v0 = sV[rr][l], v1 = sV[rr + 1][l];
+        cfmac(a0, v0, sX[rr][j]);
+        cfmac(a1, v0, sX[rr][j + 8]);
+        cfmac(b0, v1, sX[rr + 1][j]);
+        cfmac(b1, v1, sX[rr + 1][j + 8]);
+    }
+    P[(int64_t)blockIdx.x * 512 + t] = cadd(a0, b0);
+    P[(int64_t)blockIdx.x * 512 + t + 256] = cadd(a1, b1);
 }
 
-// One workgroup per vector column j < k: thread (l, part) sums entry (l, j)
-// over the partials g = part, part + 8, ... (eight independent loads in
-// flight), the eight part sums are added in a fixed order, then
+// One workgroup per vector column j < k: thread (l, part), part < 16, sums
+// entry (l, j) over the partials g = part, part + 16, ... (eight independent
+// loads in flight), the sixteen part sums are added in a fixed order, then
 // W(:, j) = T^(H) (sum) for the nbp reflector rows.  Fixed summation order,
 // identical on every run.
-__global__ __launch_bounds__(256) void k_pq_small(int nbp, int k, int G, const double2* __restrict__ P,
+__global__ __launch_bounds__(512) void k_pq_small(int nbp, int k, int G, const double2* __restrict__ P,
                                                   const double2* __restrict__ T, int ldt, int adjoint,
                                                   double2* __restrict__ W)
 {
-    __shared__ double2 sP[8][32];
+    __shared__ double2 sP[16][32];
     __shared__ double2 sW[32];
+    __shared__ double2 sT[32][33];   // sT[b][a] = T(a, b)
     const int j = blockIdx.x, l = threadIdx.x & 31, part = threadIdx.x >> 5;
     if (j >= k) return;
     const int o = l + 32 * j;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int e = threadIdx.x + 512 * i, a = e & 31, b = e >> 5;
+        const double2 x = T[min(a, nbp - 1) + (int64_t)min(b, nbp - 1) * ldt];
+        sT[b][a] = (a < nbp && b < nbp) ? x : cmk(0, 0);
+    }
     double2 acc[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc[u] = cmk(0, 0);
     int g = part;
     if (l < nbp) {
-        for (; g + 56 < G; g += 64) {
+        for (; g + 112 < G; g += 128) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) acc[u] = cadd(acc[u], P[(int64_t)(g + 8 * u) * 512 + o]);
+            for (int u = 0; u < 8; ++u) acc[u] = cadd(acc[u], P[(int64_t)(g + 16 * u) * 512 + o]);
         }
-        for (; g < G; g += 8) acc[0] = cadd(acc[0], P[(int64_t)g * 512 + o]);
+        for (; g < G; g += 16) acc[0] = cadd(acc[0], P[(int64_t)g * 512 + o]);
     }
     sP[part][l] = cadd(cadd(cadd(acc[0], acc[1]), cadd(acc[2], acc[3])),
                        cadd(cadd(acc[4], acc[5]), cadd(acc[6], acc[7])));
     __syncthreads();
     if (threadIdx.x < 32)
-        sW[l] = cadd(cadd(cadd(sP[0][l], sP[1][l]), cadd(sP[2][l], sP[3][l])),
-                     cadd(cadd(sP[4][l], sP[5][l]), cadd(sP[6][l], sP[7][l])));
+        sW[l] = cadd(cadd(cadd(cadd(sP[0][l], sP[1][l]), cadd(sP[2][l], sP[3][l])),
+                          cadd(cadd(sP[4][l], sP[5][l]), cadd(sP[6][l], sP[7][l]))),
+                     cadd(cadd(cadd(sP[8][l], sP[9][l]), cadd(sP[10][l], sP[11][l])),
+                          cadd(cadd(sP[12][l], sP[13][l]), cadd(sP[14][l], sP[15][l]))));
     __syncthreads();
     if (threadIdx.x >= 32) return;
     double2 s = cmk(0, 0);
     if (l < nbp) {
         if (adjoint) {
-            for (int q = 0; q <= l; ++q) cfmac(s, T[q + (int64_t)l * ldt], sW[q]);
+            for (int q = 0; q <= l; ++q) cfmac(s, sT[l][q], sW[q]);
         } else {
-            for (int q = l; q < nbp; ++q) cfma(s, T[l + (int64_t)q * ldt], sW[q]);
+            for (int q = l; q < nbp; ++q) cfma(s, sT[q][l], sW[q]);
         }
     }
     W[o] = s;
 }
 
-__global__ __launch_bounds__(256) void k_pq_update(int m, int nbp, int k, const double2* __restrict__ V,
-                                                   const double2* __restrict__ W, double2* __restrict__ X,
-                                                   int64_t ldx)
+#define PQU_THREADS 64
+// X -= V W for one row per thread; all k values of X are loaded before the
+// first store, so the loads overlap.
+__global__ __launch_bounds__(PQU_THREADS) void k_pq_update(int m, int nbp, int k, const double2* __restrict__ V,
+                                                           const double2* __restrict__ W, double2* __restrict__ X,
+                                                           int64_t ldx)
 {
-    __shared__ double2 sW[512];
-    for (int e = threadIdx.x; e < 512; e += 256) sW[e] = W[e];
+    __shared__ double2 sW[32 * PQ_MAXK];
+    for (int e = threadIdx.x; e < 32 * k; e += PQU_THREADS) sW[e] = W[e];
     __syncthreads();
-    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int r = blockIdx.x * PQU_THREADS + threadIdx.x;
     if (r >= m) return;
     double2 v[APQ_NB];
 #pragma unroll
     for (int l = 0; l < APQ_NB; ++l) v[l] = (l < nbp) ? V[r + (int64_t)l * m] : cmk(0, 0);
-    for (int j = 0; j < k; ++j) {
-        double2 s = cmk(0, 0);
+    double2 x[PQ_MAXK];
 #pragma unroll
-        for (int l = 0; l < APQ_NB; ++l) cfma(s, v[l], sW[l + 32 * j]);
-        X[r + (int64_t)j * ldx] = csub(X[r + (int64_t)j * ldx], s);
+    for (int j = 0; j < PQ_MAXK; ++j) x[j] = (j < k) ? X[r + (int64_t)j * ldx] : cmk(0, 0);
+#pragma unroll
+    for (int j = 0; j < PQ_MAXK; ++j) {
+        if (j < k) {
+            double2 s = cmk(0, 0);
+#pragma unroll
+            for (int l = 0; l < APQ_NB; ++l) cfma(s, v[l], sW[l + 32 * j]);
+            X[r + (int64_t)j * ldx] = csub(x[j], s);
+        }
     }
 }
 
@@ -483,10 +524,10 @@ extern "C" hipError_t bse_apply_q_thin(hipStream_t st, int n, int nb, const doub
         const int G = (int)ceil_div64(m, PQ_ROWS);
         hipLaunchKernelGGL(k_pq_dots, dim3((unsigned)G), dim3(256), 0, st, m, nbp, k, V,
                            (const double2*)(X + j0), ldx, P);
-        hipLaunchKernelGGL(k_pq_small, dim3((unsigned)k), dim3(256), 0, st, nbp, k, G, (const double2*)P, T, nb,
+        hipLaunchKernelGGL(k_pq_small, dim3((unsigned)k), dim3(512), 0, st, nbp, k, G, (const double2*)P, T, nb,
                            adjoint, W);
-        hipLaunchKernelGGL(k_pq_update, dim3((unsigned)ceil_div64(m, 256)), dim3(256), 0, st, m, nbp, k, V,
-                           (const double2*)W, X + j0, ldx);
+        hipLaunchKernelGGL(k_pq_update, dim3((unsigned)ceil_div64(m, PQU_THREADS)), dim3(PQU_THREADS), 0, st, m,
+                           nbp, k, V, (const double2*)W, X + j0, ldx);
     }
     return hipGetLastError();
 }
