@@ -639,7 +639,13 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
             HCK(hipMemcpyAsync(d_cl, cls.data(), sizeof(int2) * cls.size(), hipMemcpyHostToDevice, st));
             HCK(bse_launch_cluster_mgs(st, bsz, (int)cls.size(), d_cl, m0, d_bsv, d_nbv, xb));
         }
-        if (rho) HCK(bse_launch_contract(st, bsz, cnt, m0, d_bsv, d_nbv, xb, y, fvec));
+        if (rho) {
+            // the inverse-iteration scratch is idle until the next batch: it holds the segment partials
+            if (bse_contract_split_workspace(cnt) <= iscr)
+                HCK(bse_launch_contract_split(st, bsz, cnt, m0, d_bsv, d_nbv, xb, y, fvec, scr));
+            else
+                HCK(bse_launch_contract(st, bsz, cnt, m0, d_bsv, d_nbv, xb, y, fvec));
+        }
         const int ncommit = std::max(0, std::min(m1, nstore) - m0);
         if (ncommit > 0) HCK(bse_launch_commit(st, n, bsz, cnt, ncommit, m0, d_bsv, d_nbv, xb, Z, n, m0));
         HCK(hipStreamSynchronize(st));   // keeps d_cl and the host vectors alive

@@ -74,6 +74,9 @@ hipError_t bse_launch_commit(hipStream_t, int, int, int, int, int, const int*, c
                              const double*, double2*, int64_t, int);
 hipError_t bse_launch_contract(hipStream_t, int, int, int, const int*, const int*, const double*,
                                const double2*, double2*);
+hipError_t bse_launch_contract_split(hipStream_t, int, int, int, const int*, const int*, const double*,
+                                     const double2*, double2*, double*);
+int64_t bse_contract_split_workspace(int);
 hipError_t bse_zgemm(hipStream_t, int, int, int, int, int, double, double, const double2*, int64_t,
                      const double2*, int64_t, double, double, double2*, int64_t, double2*, int64_t);
 hipError_t bse_launch_larft(hipStream_t, int, const double2*, const double2*, double2*, int);

@@ -390,6 +390,51 @@ __global__ __launch_bounds__(256) void k_contract(int bsz, int cnt, int m0,
     f[m0 + t] = acc;
 }
 
+// The same contraction for large blocks: the walk over each eigenvector is
+// cut into CT_SEG segments, one per grid row, each item's segment partial
+// written to part (CT_SEG x cnt), then k_contract_sum adds the partials in
+// segment order (fixed, identical on every run).  One wave per workgroup, so
+// a block of order 20736 runs 324 x 8 workgroups instead of 81.
+#define CT_SEG 8
+__global__ __launch_bounds__(64) void k_contract_part(int bsz, int cnt, int m0, const int* __restrict__ bsv,
+                                                      const int* __restrict__ nbv, const double* __restrict__ x,
+                                                      const double2* __restrict__ yc, double2* __restrict__ part)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= cnt) return;
+    const int bs = bsv[m0 + t], nb = nbv[m0 + t];
+    const int k0 = (int)((int64_t)nb * blockIdx.y / CT_SEG), k1 = (int)((int64_t)nb * (blockIdx.y + 1) / CT_SEG);
+    double2 a0 = cmk(0, 0), a1 = cmk(0, 0);
+    int k = k0;
+#pragma unroll 4
+    for (; k + 1 < k1; k += 2) {
+        const double v0 = x[(int64_t)(bs + k) * bsz + t], v1 = x[(int64_t)(bs + k + 1) * bsz + t];
+        const double2 y0 = yc[bs + k], y1 = yc[bs + k + 1];
+        a0.x += v0 * y0.x;
+        a0.y -= v0 * y0.y;
+        a1.x += v1 * y1.x;
+        a1.y -= v1 * y1.y;
+    }
+    if (k < k1) {
+        const double v0 = x[(int64_t)(bs + k) * bsz + t];
+        const double2 y0 = yc[bs + k];
+        a0.x += v0 * y0.x;
+        a0.y -= v0 * y0.y;
+    }
+    part[(int64_t)blockIdx.y * cnt + t] = cadd(a0, a1);
+}
+
+__global__ __launch_bounds__(256) void k_contract_sum(int cnt, int m0, const double2* __restrict__ part,
+                                                      double2* __restrict__ f)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= cnt) return;
+    double2 acc = part[t];
+#pragma unroll
+    for (int s = 1; s < CT_SEG; ++s) acc = cadd(acc, part[(int64_t)s * cnt + t]);
+    f[m0 + t] = acc;
+}
+
 extern "C" hipError_t bse_launch_bisect(hipStream_t st, int nitems, const int4* items,
                                         const double2* bounds, const double* d, const double* e2,
                                         double pivmin, double tnorm, double* lam)
@@ -441,6 +486,21 @@ extern "C" hipError_t bse_launch_contract(hipStream_t st, int bsz, int cnt, int 
                        m0, bsv, nbv, x, yc, f);
     return hipGetLastError();
 }
+
+// ws: at least 2 * CT_SEG * cnt doubles (the segment partials).
+extern "C" hipError_t bse_launch_contract_split(hipStream_t st, int bsz, int cnt, int m0, const int* bsv,
+                                                const int* nbv, const double* x, const double2* yc,
+                                                double2* f, double* ws)
+{
+    if (cnt <= 0) return hipSuccess;
+    double2* part = reinterpret_cast<double2*>(ws);
+    hipLaunchKernelGGL(k_contract_part, dim3((unsigned)ceil_div64(cnt, 64), CT_SEG), dim3(64), 0, st, bsz, cnt,
+                       m0, bsv, nbv, x, yc, part);
+    hipLaunchKernelGGL(k_contract_sum, dim3((unsigned)ceil_div64(cnt, 256)), dim3(256), 0, st, cnt, m0,
+                       (const double2*)part, f);
+    return hipGetLastError();
+}
+extern "C" int64_t bse_contract_split_workspace(int cnt) { return (int64_t)2 * CT_SEG * cnt; }
 
 // ---------------------------------------------------------------------------
 // Device-only bookkeeping of the tridiagonal stage, for pipelines that run
