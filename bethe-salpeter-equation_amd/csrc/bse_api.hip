@@ -995,6 +995,8 @@ extern "C" int bse_eigh(bse_ctx* ctx, int n_vectors, int refine) {
     ctx->eig_on_bse = true;
     int rc = eig_pipeline(ctx, ctx->N, ctx->bufs[S_H].p, fp32,
                           (const double2*)ctx->bufs[S_RHO].p, n_vectors);
+    // an early error return may leave Q^H rho running on the side stream
+    if (rc) (void)hipStreamSynchronize(ctx->side);
     if (rc || !refine) return rc;
     stage_begin(ctx, ST_REF);
     rc = refine_lowest(ctx, n_vectors, 2);
