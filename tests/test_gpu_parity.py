@@ -145,6 +145,30 @@ def test_dense_eigh_fp32_storage(ctx):
     ref = np.linalg.eigvalsh(a)
     assert np.abs(ev - ref).max() < 1e-5 * np.abs(ref).max()
     assert z.shape == (n, 16)
+    # the 16 vectors come through the thin back-transform: residual at the
+    # FP32 rounding of the stored matrix, orthonormal to FP64 reflector accuracy
+    scale = np.abs(ref).max()
+    assert np.abs(a @ z - z * ev[None, :16]).max() < 1e-4 * scale
+    assert np.abs(z.conj().T @ z - np.eye(16)).max() < 1e-10
+
+
+@pytest.mark.parametrize("k", [1, 3, 5, 16])
+def test_dense_eigh_few_vectors(ctx, k):
+    """k <= 16 vectors use the thin back-transform (three kernels per panel,
+    any k): residual and orthonormality at FP64 accuracy, order 333 (not a
+    multiple of the 64-row blocks)."""
+    print(f"dense eigh n=333 k={k}", flush=True)
+    rng = np.random.default_rng(333 + k)
+    n = 333
+    a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    a = 0.5 * (a + a.conj().T)
+    ev, z = ctx.eigh_dense(a, "fp64", k)
+    ref = np.linalg.eigvalsh(a)
+    scale = np.abs(ref).max()
+    assert z.shape == (n, k)
+    assert np.abs(ev - ref).max() < 1e-12 * scale * n ** 0.5
+    assert np.abs(a @ z - z * ev[None, :k]).max() < 1e-11 * scale * n ** 0.5
+    assert np.abs(z.conj().T @ z - np.eye(k)).max() < 1e-10
 
 
 @pytest.mark.parametrize("idx", [0, 1, 2])
