@@ -25,7 +25,7 @@ sys.path.insert(0, str(ROOT / "oracle"))
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
     config.addinivalue_line("markers", "slow: long CPU test")
 
 
