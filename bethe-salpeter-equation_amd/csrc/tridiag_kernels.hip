@@ -153,7 +153,10 @@ __device__ inline double hash_uniform(uint64_t a, uint64_t b) {
 #endif
 
 // Inverse iteration.  Thread t of the batch owns sorted eigenvalue m0 + t in
-// block (bs, nb).  Scratch arrays are indexed [global row * bsz + t].
+// block (bs, nb).  Scratch arrays are indexed [global row * bsz + t]; the
+// pivot flags take one byte each at the start of the pf region.  The kernel
+// is bound by its scratch traffic, so the flags are bytes and the random
+// start vector is generated inside the first forward pass, never stored.
 __global__ __launch_bounds__(64) void k_inverse_iter(
     int bsz, int cnt, int m0, const double* __restrict__ lam, const int* __restrict__ bsv,
     const int* __restrict__ nbv, const double* __restrict__ d, const double* __restrict__ e,
@@ -168,6 +171,7 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
     const int bs = bsv[m0 + t], nb = nbv[m0 + t];
     const int64_t S = bsz;
     const double tiny = DBL_EPSILON * tnorm + DBL_MIN;
+    unsigned char* __restrict__ pb = reinterpret_cast<unsigned char*>(pf);
     if (nb == 1) {
         x[(int64_t)bs * S + t] = 1.0;
         return;
@@ -179,15 +183,16 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
     double cur_s = e[bs];
     auto lu_step = [&](int k, double sub, double nd, double ns) {
         const int64_t gi = bs + k;
-        double a0, a1, a2, m, fl;
+        double a0, a1, a2, m;
+        unsigned char fl;
         if (fabs(cur_d) >= fabs(sub)) {
-            fl = 0.0;
+            fl = 0;
             m = (cur_d != 0.0) ? sub / cur_d : 0.0;
             a0 = cur_d; a1 = cur_s; a2 = 0.0;
             cur_d = nd - m * cur_s;
             cur_s = ns;
         } else {
-            fl = 1.0;
+            fl = 1;
             m = cur_d / sub;
             a0 = sub; a1 = nd; a2 = ns;
             cur_d = cur_s - m * nd;
@@ -198,7 +203,7 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
         u1[gi * S + t] = a1;
         u2[gi * S + t] = a2;
         mul[gi * S + t] = m;
-        pf[gi * S + t] = fl;
+        pb[gi * S + t] = fl;
     };
     {
         int k = 0;
@@ -224,27 +229,35 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
     // Random start; three solves.  The normalisation of each iterate is folded
     // into the next forward pass (scale s), and the last back substitution
     // records the largest component so the final pass also fixes the sign.
-    for (int k = 0; k < nb; ++k) x[(int64_t)(bs + k) * S + t] = hash_uniform(m0 + t, k);
+    // random start x(k) = hash_uniform(m0 + t, k), read in pass 0 instead of x
     double s = 1.0, big = 0.0;
     // The solves walk the column in steps of IIB rows: all loads of a step are
     // issued before its recurrence runs (the compiler cannot move them above
     // the stores into x), so each step costs one memory round trip.
     for (int it = 0; it < 3; ++it) {
-        double cur = x[(int64_t)bs * S + t] * s;
+        const bool first = (it == 0);
+        double cur = (first ? hash_uniform(m0 + t, 0) : x[(int64_t)bs * S + t]) * s;
         int k = 0;
         for (; k + IIB <= nb - 1; k += IIB) {
-            double xn[IIB], mu[IIB], pv[IIB];
+            double xn[IIB], mu[IIB];
+            unsigned char pv[IIB];
+            if (first) {
+#pragma unroll
+                for (int u = 0; u < IIB; ++u) xn[u] = hash_uniform(m0 + t, k + u + 1);
+            } else {
+#pragma unroll
+                for (int u = 0; u < IIB; ++u) xn[u] = x[(bs + k + u + 1) * S + t];
+            }
 #pragma unroll
             for (int u = 0; u < IIB; ++u) {
                 const int64_t gi = bs + k + u;
-                xn[u] = x[(gi + 1) * S + t];
                 mu[u] = mul[gi * S + t];
-                pv[u] = pf[gi * S + t];
+                pv[u] = pb[gi * S + t];
             }
 #pragma unroll
             for (int u = 0; u < IIB; ++u) {
                 double nxt = xn[u] * s;
-                if (pv[u] != 0.0) { const double tmp = cur; cur = nxt; nxt = tmp; }
+                if (pv[u] != 0) { const double tmp = cur; cur = nxt; nxt = tmp; }
                 nxt -= mu[u] * cur;
                 x[(bs + k + u) * S + t] = cur;
                 cur = nxt;
@@ -252,8 +265,8 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
         }
         for (; k < nb - 1; ++k) {
             const int64_t gi = bs + k;
-            double nxt = x[(gi + 1) * S + t] * s;
-            if (pf[gi * S + t] != 0.0) { const double tmp = cur; cur = nxt; nxt = tmp; }
+            double nxt = (first ? hash_uniform(m0 + t, k + 1) : x[(gi + 1) * S + t]) * s;
+            if (pb[gi * S + t] != 0) { const double tmp = cur; cur = nxt; nxt = tmp; }
             nxt -= mul[gi * S + t] * cur;
             x[gi * S + t] = cur;
             cur = nxt;
