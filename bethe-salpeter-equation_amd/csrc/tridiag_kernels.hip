@@ -1,9 +1,10 @@
 // K6: eigenpairs of the real symmetric tridiagonal T (diagonal d, off-diagonal
 // e) produced by K5.
 //
-// Eigenvalues: Sturm-count bisection (Golub & Van Loan, sec. 8.4.1) run as an
-// 8-way multisection -- 8 lanes of a wavefront evaluate the count at 8
-// interior points of the current interval, so each round narrows it 9-fold.
+// Eigenvalues: Sturm-count bisection (Golub & Van Loan, sec. 8.4.1) run as a
+// (W+1)-way multisection -- W lanes of a wavefront (W = 2, 4 or 8, fewer for
+// larger matrices) evaluate the count at W interior points of the current
+// interval, so each round narrows it (W+1)-fold.
 // The recurrence divides by a reciprocal refined with one Newton step
 // instead of the correctly rounded division sequence: the hardware
 // reciprocal is good to 4.6e-8 relative, one step brings it to 2.2e-15
@@ -84,9 +85,10 @@ __device__ inline int sturm_count(const double* __restrict__ d, const double* __
 }
 
 // items: int4 (block start, block size, local index, block id); bounds per
-// block id: (lo, hi).  One 8-lane group per item.
+// block id: (lo, hi).  One W-lane group per item: (W + 1)-way multisection.
 // prm (may be null): device copy of (pivmin, tnorm) that replaces the
 // by-value arguments, for launches recorded before those are known.
+template <int W>
 __global__ __launch_bounds__(256) void k_bisect(int nitems, const int4* __restrict__ items,
                                                const double2* __restrict__ bounds,
                                                const double* __restrict__ d,
@@ -99,9 +101,10 @@ __global__ __launch_bounds__(256) void k_bisect(int nitems, const int4* __restri
         tnorm = prm[1];
     }
     const int gid = (blockIdx.x * blockDim.x + threadIdx.x);
-    const int item = gid >> 3, g = gid & 7;
+    static_assert(W == 2 || W == 4 || W == 8, "lanes per item");
+    const int item = gid / W, g = gid % W;
     const int lane = threadIdx.x & 63;
-    const int base = lane & ~7;
+    const int base = lane & ~(W - 1);
     const bool active = item < nitems;
     int4 it = make_int4(0, 1, 0, 0);
     double lo = 0.0, hi = 0.0;
@@ -116,22 +119,24 @@ __global__ __launch_bounds__(256) void k_bisect(int nitems, const int4* __restri
     }
     const bool work = active && it.y > 1;
     const double eps = DBL_EPSILON;
-    for (int round = 0; round < 40; ++round) {
+    // the interval shrinks (W + 1)-fold per round from at most 2 |T| to
+    // eps |T|: 17 rounds for W = 8, 23 for W = 4, 34 for W = 2
+    for (int round = 0; round < 80; ++round) {
         const double tol = 2.0 * eps * fmax(fabs(lo), fabs(hi)) + eps * tnorm;
         const bool conv = !work || (hi - lo) <= tol;
         // group-uniform; the whole wave keeps looping until every group converged
         const unsigned long long notdone = __ballot(!conv);
         if (notdone == 0ull) break;
-        const double x = lo + (double)(g + 1) * (hi - lo) / 9.0;
+        const double x = lo + (double)(g + 1) * (hi - lo) / (double)(W + 1);
         int cnt = 0;
         if (!conv) cnt = sturm_count(d, e2, it.x, it.y, x, pivmin);
         const unsigned long long below = __ballot(!conv && cnt <= it.z);
-        const int nlo = __popcll((below >> base) & 0xffull);
+        const int nlo = __popcll((below >> base) & ((1ull << W) - 1));
         const double xlo = __shfl(x, base + max(nlo - 1, 0), 64);
-        const double xhi = __shfl(x, base + min(nlo, 7), 64);
+        const double xhi = __shfl(x, base + min(nlo, W - 1), 64);
         if (!conv) {
             if (nlo > 0) lo = xlo;
-            if (nlo < 8) hi = xhi;
+            if (nlo < W) hi = xhi;
         }
     }
     if (work && g == 0) lam[item] = 0.5 * (lo + hi);
@@ -452,9 +457,21 @@ extern "C" hipError_t bse_launch_bisect(hipStream_t st, int nitems, const int4* 
                                         const double2* bounds, const double* d, const double* e2,
                                         double pivmin, double tnorm, double* lam)
 {
-    const int64_t threads = (int64_t)nitems * 8;
-    hipLaunchKernelGGL(k_bisect, dim3((unsigned)ceil_div64(threads, 256)), dim3(256), 0, st,
-                       nitems, items, bounds, d, e2, pivmin, tnorm, (const double*)nullptr, lam);
+    // Fewer lanes per eigenvalue do less redundant Sturm work (W lanes run
+    // W * log_{W+1}(2/eps) counts) at the price of more rounds; once the
+    // items alone give a wave per CU, W = 2 is fastest (eigenvalue stage at
+    // order 20736: 32.7 ms against 37.2 ms for W = 4 and 38.2 ms for W = 8).
+    const int W = nitems >= 8192 ? 2 : nitems >= 4096 ? 4 : 8;
+    const int64_t threads = (int64_t)nitems * W;
+    if (W == 2)
+        hipLaunchKernelGGL(k_bisect<2>, dim3((unsigned)ceil_div64(threads, 256)), dim3(256), 0, st,
+                           nitems, items, bounds, d, e2, pivmin, tnorm, (const double*)nullptr, lam);
+    else if (W == 4)
+        hipLaunchKernelGGL(k_bisect<4>, dim3((unsigned)ceil_div64(threads, 256)), dim3(256), 0, st,
+                           nitems, items, bounds, d, e2, pivmin, tnorm, (const double*)nullptr, lam);
+    else
+        hipLaunchKernelGGL(k_bisect<8>, dim3((unsigned)ceil_div64(threads, 256)), dim3(256), 0, st,
+                           nitems, items, bounds, d, e2, pivmin, tnorm, (const double*)nullptr, lam);
     return hipGetLastError();
 }
 
@@ -699,7 +716,7 @@ extern "C" hipError_t bse_launch_tri_vectors_dev(hipStream_t st, int n, const in
 {
     if (stage == 0) {
         const int64_t threads = (int64_t)n * 8;
-        hipLaunchKernelGGL(k_bisect, dim3((unsigned)ceil_div64(threads, 256)), dim3(256), 0, st, n,
+        hipLaunchKernelGGL(k_bisect<8>, dim3((unsigned)ceil_div64(threads, 256)), dim3(256), 0, st, n,
                            items, bounds, d, e2, 0.0, 1.0, prm, lami);
     } else {
         const int64_t A = (int64_t)n * n;

@@ -171,6 +171,22 @@ def test_dense_eigh_few_vectors(ctx, k):
     assert np.abs(z.conj().T @ z - np.eye(k)).max() < 1e-10
 
 
+@pytest.mark.parametrize("n", [4096, 8192])
+def test_dense_eigh_laplacian_closed_form(ctx, n):
+    """Orders 4096 and 8192 run the bisection with 4 and 2 lanes per
+    eigenvalue: the 1-D Laplacian tridiag(-1, 2, -1) has the eigenvalues
+    2 - 2 cos(k pi / (n + 1)), k = 1..n."""
+    print(f"dense eigh laplacian n={n}", flush=True)
+    a = np.zeros((n, n), dtype=np.complex128, order="F")
+    i = np.arange(n)
+    a[i, i] = 2.0
+    a[i[:-1], i[:-1] + 1] = -1.0
+    a[i[:-1] + 1, i[:-1]] = -1.0
+    ev, _ = ctx.eigh_dense(a, "fp64", 1)
+    ref = 2.0 - 2.0 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1))
+    assert np.abs(ev - ref).max() < 1e-12 * 4.0 * n ** 0.5
+
+
 @pytest.mark.parametrize("idx", [0, 1, 2])
 def test_bse_eig_and_spectrum_fp64(ctx, idx):
     s = small_systems()[idx]
