@@ -13,4 +13,4 @@ The directory name is not a Python identifier; load it with
 """
 from . import lattice, tight_binding, system  # noqa: F401
 from .system import (BSESystem, honeycomb_plain, honeycomb_magnetic,  # noqa: F401
-                     random_multiorbital, honeycomb_sweep, GRAD_PARAMS)
+                     random_multiorbital, honeycomb_sweep, cubic_3d, random_cubic_3d, GRAD_PARAMS)

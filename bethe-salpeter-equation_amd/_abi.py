@@ -26,7 +26,7 @@ ABI_SYMBOLS = (
     "bse_eigh", "bse_get_eigenvalues", "bse_get_eigenvectors", "bse_oscillator",
     "bse_spectrum", "bse_eigh_dense", "bse_stage_times", "bse_profile", "bse_apply", "bse_grad_energies",
     "bse_batch_prepare", "bse_batch_run", "bse_batch_results", "bse_batch_info",
-    "bse_grad_spectrum",
+    "bse_grad_spectrum", "bse_refine_info", "bse_set_problem_klist", "bse_tb_states_at",
 )
 
 
@@ -45,7 +45,9 @@ class SystemC(C.Structure):
                 ("nc", C.c_int32), ("scissor", C.c_double), ("q", C.c_double * 3),
                 ("r0", C.c_double), ("eps_env", C.c_double), ("g_cut", C.c_double),
                 ("f_s", C.c_double), ("exchange_scale", C.c_double),
-                ("direct_scale", C.c_double), ("dipole_dir", C.c_double * 3)]
+                ("direct_scale", C.c_double), ("dipole_dir", C.c_double * 3),
+                ("dimension", C.c_int32), ("n3", C.c_int32), ("shift3", C.c_double),
+                ("radius", C.c_double)]
 
 
 class BatchOptionsC(C.Structure):
@@ -64,7 +66,7 @@ def _system_to_c(s):
     sc = SystemC()
     sc.bravais[:] = list(np.asarray(m.lattice.bravais, dtype=np.float64).T.ravel())
     sc.n1, sc.n2 = s.grid.n1, s.grid.n2
-    sc.shift[:] = list(s.grid.shift)
+    sc.shift[:] = [float(x) for x in list(s.grid.shift)[:2]]
     sc.n_occ, sc.nv, sc.nc = s.n_occ, s.nv, s.nc
     sc.scissor = s.scissor
     sc.q[:] = list(s.q_opt)
@@ -72,7 +74,20 @@ def _system_to_c(s):
     sc.f_s = s.spin_factor
     sc.exchange_scale, sc.direct_scale = s.exchange_scale, s.direct_scale
     sc.dipole_dir[:] = [1.0, 0.0, 0.0]
+    sc.dimension = int(getattr(s, "dimension", 2))
+    sc.n3 = int(getattr(s.grid, "n3", 1))
+    sc.shift3 = float(getattr(s.grid, "shift3", 0.0))
+    sc.radius = float(getattr(s, "radius", 0.0))
     return mc, sc, [r_cart, h_r, tau]
+
+
+def _model_to_c(m):
+    r_cart = np.ascontiguousarray(m.r_cart, dtype=np.float64)
+    h_r = np.ascontiguousarray(m.h_r, dtype=np.complex128)
+    tau = np.ascontiguousarray(m.tau, dtype=np.float64)
+    mc = TBModelC(m.norb, m.nr, _dptr(r_cart), h_r.ctypes.data_as(C.POINTER(C.c_double)),
+                  _dptr(tau))
+    return mc, [r_cart, h_r, tau]
 
 
 _lib = None
@@ -115,6 +130,10 @@ def load_library(path: pathlib.Path | str | None = None):
         "bse_batch_results": (i32, [vp, dp, dp, dp]),
         "bse_batch_info": (i32, [vp, C.POINTER(C.c_int64)]),
         "bse_grad_spectrum": (i32, [vp, dp, i32, C.c_double, dp, C.POINTER(C.c_int), dp, dp]),
+        "bse_refine_info": (i32, [vp, dp, i32, C.POINTER(C.c_int)]),
+        "bse_set_problem_klist": (i32, [vp, C.POINTER(TBModelC), C.POINTER(SystemC), dp, i32]),
+        "bse_tb_states_at": (i32, [vp, C.POINTER(TBModelC), dp, i32, i32, i32, i32, C.c_double,
+                                   dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -175,6 +194,41 @@ class Context:
         self._check(self.lib.bse_set_problem(self.h, C.byref(mc), C.byref(sc)), "bse_set_problem")
         self.system = s
         return self
+
+    def set_system_klist(self, s, k_cryst):
+        """Upload a system whose k-points are the caller's list (crystal
+        coordinates, 3 x nk, any order of a complete regular grid); the BSE
+        index k of every reference-order output is the list position."""
+        mc, sc, keep = _system_to_c(s)
+        kc = np.asfortranarray(np.asarray(k_cryst, dtype=np.float64).reshape(3, -1))
+        self._keep = keep + [kc]
+        self._check(self.lib.bse_set_problem_klist(self.h, C.byref(mc), C.byref(sc), _dptr(kc),
+                                                   kc.shape[1]), "bse_set_problem_klist")
+        self.system = s
+        return self
+
+    def tb_states_at(self, kpts, n_occ=0, nv=0, nc=0, scissor=0.0, model=None):
+        """States at Cartesian k (3 x nk): every band ascending when nv = nc = 0
+        (pull_ks_states), else the subset window (pull_ks_states_subset).
+        Returns (eps [nk, ns], coef [nk, ns, norb])."""
+        kp = np.asfortranarray(np.asarray(kpts, dtype=np.float64).reshape(3, -1))
+        nk = kp.shape[1]
+        mc_p, keep = None, []
+        if model is not None:
+            mc, keep = _model_to_c(model)
+            mc_p = C.byref(mc)
+            norb = model.norb
+        else:
+            norb = self.system.norb
+        ns = norb if (nv == 0 and nc == 0) else nv + nc
+        eps = np.zeros((nk, ns))
+        coef = np.zeros((nk, ns, norb), dtype=np.complex128)
+        self._check(self.lib.bse_tb_states_at(self.h, mc_p, _dptr(kp), nk, int(n_occ), int(nv), int(nc),
+                                              float(scissor), _dptr(eps),
+                                              coef.ctypes.data_as(C.POINTER(C.c_double))),
+                    "bse_tb_states_at")
+        del keep
+        return eps, coef
 
     def sizes(self):
         out = (C.c_int64 * 4)()
@@ -251,6 +305,15 @@ class Context:
         self._check(self.lib.bse_eigh(self.h, int(n_vectors), int(bool(refine))), "bse_eigh")
         self.n_vectors = n_vectors
         return self
+
+    def refine_info(self, n):
+        """(residual norms in eV of the first n refined vectors, rounds, status)."""
+        out = np.zeros(max(int(n), 1))
+        rounds = C.c_int(0)
+        rc = self.lib.bse_refine_info(self.h, _dptr(out), int(n), C.byref(rounds))
+        if rc == -1:
+            self._check(rc, "bse_refine_info")
+        return out[:n], rounds.value, rc
 
     def eigenvalues(self):
         n = self.sizes()["n"] if self.system is not None else self._n_dense

@@ -23,7 +23,7 @@
 #define APPLY_KMAX 32   // vectors per pass
 
 struct ApplyArgs {
-    int nk, n1, n2, nv, nc, norb, TK, k;
+    int nk, n1, n2, n3, nv, nc, norb, TK, k;
     int64_t N, ldx, ldy;
     const double2* Cl;
     const double2* Cr;
@@ -83,7 +83,6 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
     for (int v = 0; v < APPLY_KMAX / 2; ++v) acc[v] = cmk(0, 0);
 
     const int nt = (int)ceil_div64(a.nk, TK);
-    const int w2 = 2 * a.n2 - 1;
     for (int tj = 0; tj < nt; ++tj) {
         const int kp0 = tj * TK;
         __syncthreads();
@@ -119,9 +118,7 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a)
             const int k = k0 + pk, kp = kp0 + pq;
             double2 s2 = cmk(0, 0);
             if (k < a.nk && kp < a.nk) {
-                const int d1 = k / a.n2 - kp / a.n2 + (a.n1 - 1);
-                const int d2 = k % a.n2 - kp % a.n2 + (a.n2 - 1);
-                const double2* wq = a.wtab + ((size_t)d1 * w2 + d2) * no * no;
+                const double2* wq = a.wtab + (size_t)ktable_index(k, kp, a.n1, a.n2, a.n3) * no * no;
                 const double2* cl = sCl + (pk * a.nc + cq) * no;
                 const double2* cr = sCr + (pq * a.nc + cp) * no;
                 for (int i = 0; i < no; ++i) cfma(s2, cmulc(cl[i], cr[i]), wq[i * no + j]);
@@ -207,7 +204,7 @@ extern "C" int bse_apply_pick_tk(int nv, int nc, int norb) {
     return tk;
 }
 
-extern "C" hipError_t bse_launch_apply(hipStream_t st, int nk, int n1, int n2, int nv, int nc,
+extern "C" hipError_t bse_launch_apply(hipStream_t st, int nk, int n1, int n2, int n3, int nv, int nc,
                                        int norb, int TK, const double2* Cl, const double2* Cr,
                                        const double2* Vl, const double2* Vr, const double2* wtab,
                                        const double2* PVr, const double2* Pc, const double* dE,
@@ -216,7 +213,7 @@ extern "C" hipError_t bse_launch_apply(hipStream_t st, int nk, int n1, int n2, i
                                        int64_t ldb, double2* tr_part)
 {
     ApplyArgs a;
-    a.nk = nk; a.n1 = n1; a.n2 = n2; a.nv = nv; a.nc = nc; a.norb = norb; a.TK = TK;
+    a.nk = nk; a.n1 = n1; a.n2 = n2; a.n3 = n3; a.nv = nv; a.nc = nc; a.norb = norb; a.TK = TK;
     a.N = (int64_t)nk * nv * nc;
     a.Cl = Cl; a.Cr = Cr; a.Vl = Vl; a.Vr = Vr; a.wtab = wtab; a.PVr = PVr; a.Pc = Pc; a.dE = dE;
     a.alpha = alpha; a.beta = beta;

@@ -23,14 +23,14 @@
 #include "bse_common.hpp"
 
 struct AsmArgs {
-    int nk, n1, n2, nv, nc, norb, TK;
+    int nk, n1, n2, n3, nv, nc, norb, TK;
     int64_t N;
     int nt;              // tile rows of the tiled lower storage
     const double2* Cl;   // coef table [k][slot][orb] for the row (left) conduction factor
     const double2* Cr;   // ... column (right) conduction factor
     const double2* Vl;   // ... row valence factor
     const double2* Vr;   // ... column valence factor
-    const double2* wtab; // (2n1-1) x (2n2-1) x norb x norb
+    const double2* wtab; // (2n1-1) x (2n2-1) x (2n3-1) x norb x norb
     const double2* PVr;  // N x norb exchange row factor (may be null)
     const double2* Pc;   // N x norb exchange column factor
     const double* dE;    // N diagonal (may be null)
@@ -95,7 +95,6 @@ __global__ __launch_bounds__(256) void k_assemble_tile(AsmArgs a, T* __restrict_
 
     // Pair intermediates Apair[p][c][c'][j], p = kk*TK + kq.
     const int nA = TK * TK * a.nc * a.nc * no;
-    const int w2 = 2 * a.n2 - 1;
     for (int e = tid; e < nA; e += blockDim.x) {
         const int j = e % no;
         const int cp = (e / no) % a.nc;
@@ -105,9 +104,7 @@ __global__ __launch_bounds__(256) void k_assemble_tile(AsmArgs a, T* __restrict_
         const int k = k0 + kk, kp = kp0 + kq;
         double2 acc = cmk(0, 0);
         if (k < a.nk && kp < a.nk) {
-            const int d1 = k / a.n2 - kp / a.n2 + (a.n1 - 1);
-            const int d2 = k % a.n2 - kp % a.n2 + (a.n2 - 1);
-            const double2* wq = a.wtab + ((size_t)d1 * w2 + d2) * no * no;
+            const double2* wq = a.wtab + (size_t)ktable_index(k, kp, a.n1, a.n2, a.n3) * no * no;
             const double2* cl = sCl + (kk * a.nc + c) * no;
             const double2* cr = sCr + (kq * a.nc + cp) * no;
             for (int i = 0; i < no; ++i) {
@@ -174,7 +171,7 @@ static hipError_t launch_assemble(hipStream_t st, const AsmArgs& a, T* H, int lo
 }
 
 extern "C" hipError_t bse_launch_assemble(hipStream_t st, int store_fp32, int nk, int n1, int n2,
-                                          int nv, int nc, int norb, int TK, int64_t ld,
+                                          int n3, int nv, int nc, int norb, int TK, int64_t ld,
                                           const double2* Cl, const double2* Cr,
                                           const double2* Vl, const double2* Vr,
                                           const double2* wtab, const double2* PVr,
@@ -182,7 +179,7 @@ extern "C" hipError_t bse_launch_assemble(hipStream_t st, int store_fp32, int nk
                                           double beta, void* H, int lower_only)
 {
     AsmArgs a;
-    a.nk = nk; a.n1 = n1; a.n2 = n2; a.nv = nv; a.nc = nc; a.norb = norb; a.TK = TK;
+    a.nk = nk; a.n1 = n1; a.n2 = n2; a.n3 = n3; a.nv = nv; a.nc = nc; a.norb = norb; a.TK = TK;
     a.N = (int64_t)nk * nv * nc;
     a.nt = tile_count(a.N);
     (void)ld;            // the matrix is stored in tiles (bse_common.hpp)
@@ -197,9 +194,11 @@ extern "C" hipError_t bse_launch_assemble(hipStream_t st, int store_fp32, int nk
 // (column-major, ldd) in reference or internal order.  dst element type
 // matches src.
 // ---------------------------------------------------------------------------
+// kinv (may be null): caller's k-list position -> grid index.
 template <typename T>
 __global__ __launch_bounds__(256) void k_expand(int64_t N, int B, int nk, const T* __restrict__ src,
-                                                T* __restrict__ dst, int64_t ldd, int ref_order)
+                                                T* __restrict__ dst, int64_t ldd, int ref_order,
+                                                const int* __restrict__ kinv)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // destination row
     const int64_t s = blockIdx.y;                                     // destination column
@@ -208,8 +207,9 @@ __global__ __launch_bounds__(256) void k_expand(int64_t N, int B, int nk, const 
     // map destination (reference) indices to internal ones
     int64_t ri = r, si = s;
     if (ref_order) {
-        ri = (r % nk) * B + r / nk;
-        si = (s % nk) * B + s / nk;
+        const int kr = (int)(r % nk), ks = (int)(s % nk);
+        ri = (int64_t)(kinv ? kinv[kr] : kr) * B + r / nk;
+        si = (int64_t)(kinv ? kinv[ks] : ks) * B + s / nk;
     }
     T v;
     if (ri >= si) {
@@ -224,15 +224,16 @@ __global__ __launch_bounds__(256) void k_expand(int64_t N, int B, int nk, const 
 }
 
 extern "C" hipError_t bse_launch_expand(hipStream_t st, int fp32, int64_t N, int B, int nk,
-                                        const void* src, void* dst, int64_t ldd, int ref_order)
+                                        const void* src, void* dst, int64_t ldd, int ref_order,
+                                        const int* kinv)
 {
     dim3 grid((unsigned)ceil_div64(N, 256), (unsigned)N);
     if (fp32)
         hipLaunchKernelGGL(k_expand<float2>, grid, dim3(256), 0, st, N, B, nk,
-                           (const float2*)src, (float2*)dst, ldd, ref_order);
+                           (const float2*)src, (float2*)dst, ldd, ref_order, kinv);
     else
         hipLaunchKernelGGL(k_expand<double2>, grid, dim3(256), 0, st, N, B, nk,
-                           (const double2*)src, (double2*)dst, ldd, ref_order);
+                           (const double2*)src, (double2*)dst, ldd, ref_order, kinv);
     return hipGetLastError();
 }
 

@@ -99,7 +99,7 @@ __device__ inline mf_f4 mm16_bf16x2(const float* a, const float* b, mf_f4 acc, f
 }
 
 struct MfArgs {
-    int nk, n1, n2, nv, nc, norb, TK;
+    int nk, n1, n2, n3, nv, nc, norb, TK;
     int64_t N;
     int nt;              // tile rows of the tiled lower storage
     const double2* coef;
@@ -150,13 +150,11 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(MfArgs a, typename mf_t<P
     __syncthreads();
     R* tw = sT + wv * (PREC == PREC_BF16 ? 544 : 272);
     const int li = lane & 15, lq = lane >> 4;
-    const int w2 = 2 * a.n2 - 1;
     for (int p = wv; p < TK * TK; p += 4) {
         const int kk = p / TK, kq = p % TK;
         const int k = k0 + kk, kp = kp0 + kq;
         if (k >= a.nk || kp >= a.nk) continue;
-        const double2* wq = a.wtab + ((size_t)((k / a.n2 - kp / a.n2 + a.n1 - 1) * w2 +
-                                               (k % a.n2 - kp % a.n2 + a.n2 - 1))) * no * no;
+        const double2* wq = a.wtab + (size_t)ktable_index(k, kp, a.n1, a.n2, a.n3) * no * no;
         // ---- step 1 operands: A1[rho][kappa] (rho = c*nc + c'), B1[kappa][j']
         R a1[4], b1[4];
         const int rho = li, c = rho / a.nc, cp = rho % a.nc;
@@ -277,7 +275,6 @@ __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __r
     for (int e = tid; e < 64 * 65; e += 256) { sHr[e] = 0.0; sHi[e] = 0.0; }
     double* tw = sT + wv * 272;
     const int li = lane & 15, lq = lane >> 4;
-    const int w2 = 2 * a.n2 - 1;
     mf_d4 yr = mf_d4{0, 0, 0, 0}, yi = mf_d4{0, 0, 0, 0};
     for (int tj = g; tj < nt; tj += ncg) {
         const int kp0 = tj * TK;
@@ -300,8 +297,7 @@ __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __r
             const int kk = p / TK, kq = p % TK;
             const int kx = k0 + kk, kp = kp0 + kq;
             const bool pok = kx < a.nk && kp < a.nk;
-            const double2* wq = a.wtab + (pok ? ((size_t)((kx / a.n2 - kp / a.n2 + a.n1 - 1) * w2 +
-                                                          (kx % a.n2 - kp % a.n2 + a.n2 - 1))) * no * no : 0);
+            const double2* wq = a.wtab + (pok ? (size_t)ktable_index(kx, kp, a.n1, a.n2, a.n3) * no * no : 0);
             double a1[4], b1[4];
             const int rho = li, c = rho / a.nc, cp = rho % a.nc;
             const bool rok = rho < a.nc * a.nc;
@@ -454,7 +450,7 @@ extern "C" int64_t bse_apply_mfma_workspace(int nk, int nv, int nc) {
     return (int64_t)bse_apply_mfma_groups(nk, nv, nc) * nk * nv * nc * AM_K + (APX_BLOCKS + 1) * 128;
 }
 
-extern "C" hipError_t bse_launch_apply_mfma(hipStream_t st, int nk, int n1, int n2, int nv, int nc,
+extern "C" hipError_t bse_launch_apply_mfma(hipStream_t st, int nk, int n1, int n2, int n3, int nv, int nc,
                                             int norb, const double2* coef, const double2* wtab,
                                             const double2* PV, const double2* P, const double* dE,
                                             double alpha, double beta, int k, const double2* X,
@@ -462,7 +458,7 @@ extern "C" hipError_t bse_launch_apply_mfma(hipStream_t st, int nk, int n1, int 
 {
     if (!bse_assemble_mfma_supported(nv, nc, norb) || k <= 0 || k > AM_K) return hipErrorInvalidValue;
     MfArgs a;
-    a.nk = nk; a.n1 = n1; a.n2 = n2; a.nv = nv; a.nc = nc; a.norb = norb;
+    a.nk = nk; a.n1 = n1; a.n2 = n2; a.n3 = n3; a.nv = nv; a.nc = nc; a.norb = norb;
     a.TK = bse_assemble_mfma_tk(nv, nc);
     if (a.TK * nv * nc > 64) return hipErrorInvalidValue;
     a.N = (int64_t)nk * nv * nc;
@@ -501,7 +497,7 @@ static size_t mfma_lds_bytes(int TK, int nv, int nc, int prec) {
 }
 
 extern "C" hipError_t bse_launch_assemble_mfma(hipStream_t st, int prec, int nk, int n1, int n2,
-                                               int nv, int nc, int norb, int64_t ld,
+                                               int n3, int nv, int nc, int norb, int64_t ld,
                                                const double2* coef, const double2* wtab,
                                                const double2* PV, const double2* P,
                                                const double* dE, double alpha, double beta,
@@ -509,7 +505,7 @@ extern "C" hipError_t bse_launch_assemble_mfma(hipStream_t st, int prec, int nk,
 {
     if (!bse_assemble_mfma_supported(nv, nc, norb)) return hipErrorInvalidValue;
     MfArgs a;
-    a.nk = nk; a.n1 = n1; a.n2 = n2; a.nv = nv; a.nc = nc; a.norb = norb;
+    a.nk = nk; a.n1 = n1; a.n2 = n2; a.n3 = n3; a.nv = nv; a.nc = nc; a.norb = norb;
     a.TK = bse_assemble_mfma_tk(nv, nc);
     a.N = (int64_t)nk * nv * nc;
     a.nt = tile_count(a.N);

@@ -39,7 +39,7 @@ struct SysDev {
     unsigned int* counter;
     int4* items;
     double2* bounds;
-    int *bsv, *nbv, *ncl;
+    int *bsv, *nbv, *ncl, *members;
     int2* clusters;
     double *iiscr, *iix;
     double2 *y, *f, *Z;
@@ -116,7 +116,7 @@ void carve(Bump& bp, const SysHost& h, int fp32, int nlow, int n_omega, SysDev& 
     d.gvec = bp.take<double>(3 * (size_t)g.ng);
     d.eps = bp.take<double>((size_t)g.nk * g.ns);
     d.coef = bp.take<double2>((size_t)g.nk * g.ns * no);
-    d.wtab = bp.take<double2>((size_t)(2 * g.sys.n1 - 1) * (2 * g.sys.n2 - 1) * no * no);
+    d.wtab = bp.take<double2>((size_t)ktable_size(g.sys.n1, g.sys.n2, g.n3) * no * no);
     d.vx = bp.take<double2>((size_t)no * no);
     d.P = bp.take<double2>((size_t)N * no);
     d.PV = bp.take<double2>((size_t)N * no);
@@ -150,6 +150,7 @@ void carve(Bump& bp, const SysHost& h, int fp32, int nlow, int n_omega, SysDev& 
     d.bsv = bp.take<int>(N);
     d.nbv = bp.take<int>(N);
     d.clusters = bp.take<int2>(N);
+    d.members = bp.take<int>(N);
     d.ncl = bp.take<int>(4);
     d.iiscr = bp.take<double>((size_t)5 * N * N);
     d.iix = bp.take<double>((size_t)N * N);
@@ -193,21 +194,20 @@ static hipError_t record_front(BatchState* b, int s, hipStream_t st) {
     if ((e = bse_launch_tb_states(st, g.nk, no, g.nr, d.rcart, d.hr, d.kpts, sy.q, sy.n_occ, sy.nv,
                                   sy.nc, sy.scissor, nullptr, d.eps, d.coef, nullptr, nullptr)))
         return e;
-    const double bvec[6] = {g.recip[0], g.recip[1], g.recip[2], g.recip[3], g.recip[4], g.recip[5]};
-    if ((e = bse_launch_w_table(st, sy.n1, sy.n2, no, g.ng, d.gvec, bvec, d.tau, sy.r0, sy.eps_env,
-                                0.5 * kC, g.wbar, 0, d.wtab)))
+    if ((e = bse_launch_w_table(st, sy.n1, sy.n2, g.n3, g.dim, no, g.ng, d.gvec, g.recip, d.tau, sy.r0,
+                                sy.eps_env, sy.radius, kC, g.wbar, 0, d.wtab)))
         return e;
-    if ((e = bse_launch_v_exchange(st, no, g.ng, d.gvec, d.tau, 0.5 * kC, d.vx))) return e;
+    if ((e = bse_launch_v_exchange(st, no, g.ng, g.dim, d.gvec, d.tau, kC, sy.radius, d.vx))) return e;
     if ((e = bse_launch_transitions(st, g.nk, sy.nv, sy.nc, no, d.eps, d.coef, nullptr, nullptr,
                                     d.tau, g.qhat, d.vx, d.P, d.PV, d.rho, d.dE)))
         return e;
-    const double alpha = -sy.direct_scale / (g.area * g.nk);
-    const double beta = sy.f_s * sy.exchange_scale / (g.area * g.nk);
+    const double alpha = -sy.direct_scale / (g.cell * g.nk);
+    const double beta = sy.f_s * sy.exchange_scale / (g.cell * g.nk);
     if ((e = bse_launch_tile_pad_zero(st, b->fp32, g.N, d.H))) return e;
     if (bse_assemble_mfma_supported(sy.nv, sy.nc, no))
-        return bse_launch_assemble_mfma(st, b->precision, g.nk, sy.n1, sy.n2, sy.nv, sy.nc, no, g.N,
+        return bse_launch_assemble_mfma(st, b->precision, g.nk, sy.n1, sy.n2, g.n3, sy.nv, sy.nc, no, g.N,
                                         d.coef, d.wtab, d.PV, d.P, d.dE, alpha, beta, d.H);
-    return bse_launch_assemble(st, b->fp32, g.nk, sy.n1, sy.n2, sy.nv, sy.nc, no, h.TK, g.N, d.coef,
+    return bse_launch_assemble(st, b->fp32, g.nk, sy.n1, sy.n2, g.n3, sy.nv, sy.nc, no, h.TK, g.N, d.coef,
                                d.coef, d.coef, d.coef, d.wtab, d.PV, d.P, d.dE, alpha, beta, d.H, 1);
 }
 
@@ -218,11 +218,11 @@ static hipError_t record_apply(BatchState* b, int s, hipStream_t st, int k, cons
     const SysDev& d = b->ds[s];
     const Geom& g = h.g;
     const bse_system& sy = g.sys;
-    const double alpha = -sy.direct_scale / (g.area * g.nk);
-    const double beta = sy.f_s * sy.exchange_scale / (g.area * g.nk);
+    const double alpha = -sy.direct_scale / (g.cell * g.nk);
+    const double beta = sy.f_s * sy.exchange_scale / (g.cell * g.nk);
     hipError_t e;
     if ((e = bse_launch_zero_words(st, 2 * g.N * k, Y))) return e;
-    return bse_launch_apply(st, g.nk, sy.n1, sy.n2, sy.nv, sy.nc, g.norb,
+    return bse_launch_apply(st, g.nk, sy.n1, sy.n2, g.n3, sy.nv, sy.nc, g.norb,
                             bse_apply_pick_tk(sy.nv, sy.nc, g.norb), d.coef, d.coef, d.coef, d.coef,
                             d.wtab, d.PV, d.P, d.dE, alpha, beta, k, X, g.N, Y, g.N, nullptr, 0, nullptr);
 }
@@ -240,13 +240,15 @@ static hipError_t record_tail(BatchState* b, int s, hipStream_t st) {
     if ((e = bse_launch_last_diag(st, b->fp32, n, d.H, d.d))) return e;
     if ((e = bse_launch_tri_prepare(st, n, d.d, d.e, d.e2, d.items, d.bounds, d.prm))) return e;
     if ((e = bse_launch_tri_vectors_dev(st, n, d.items, d.bounds, d.d, d.e, d.e2, d.prm, d.lami,
-                                        d.lam, d.bsv, d.nbv, d.clusters, d.ncl, d.iiscr, d.iix, 0)))
+                                        d.lam, d.bsv, d.nbv, d.clusters, d.members, d.ncl, d.iiscr,
+                                        d.iix, 0)))
         return e;
     if ((e = bse_launch_tri_sort(st, n, d.lami, d.items, d.prm, d.lam, d.bsv, d.nbv, d.clusters,
-                                 d.ncl)))
+                                 d.members, d.ncl)))
         return e;
     if ((e = bse_launch_tri_vectors_dev(st, n, d.items, d.bounds, d.d, d.e, d.e2, d.prm, d.lami,
-                                        d.lam, d.bsv, d.nbv, d.clusters, d.ncl, d.iiscr, d.iix, 1)))
+                                        d.lam, d.bsv, d.nbv, d.clusters, d.members, d.ncl, d.iiscr,
+                                        d.iix, 1)))
         return e;
     // oscillator strengths of all excitons: f_m = sum_i x_m[i] conj((Q^H rho)_i)
     if ((e = bse_launch_copy_words(st, 2 * N, d.rho, d.y))) return e;
@@ -261,7 +263,7 @@ static hipError_t record_tail(BatchState* b, int s, hipStream_t st) {
         if ((e = bse_launch_copy_words(st, 2 * N * k, d.Z, S))) return e;
         if ((e = record_apply(b, s, st, k, S, HS))) return e;
         for (int it = 0; it < kRefineIters; ++it) {
-            if ((e = bse_launch_residual(st, N, k, S, HS, N, S + N * k, d.small + k))) return e;
+            if ((e = bse_launch_residual(st, N, k, S, HS, N, S + N * k, d.small + k, nullptr))) return e;
             if ((e = record_apply(b, s, st, k, S + N * k, HS + N * k))) return e;
             if ((e = bse_zgemm(st, 1, 0, m, m, n, 1, 0, S, N, S, N, 0, 0, d.RG, m, d.skw, d.sk_el))) return e;
             if ((e = bse_zgemm(st, 1, 0, m, m, n, 1, 0, S, N, HS, N, 0, 0, d.RHG, m, d.skw, d.sk_el))) return e;

@@ -203,27 +203,30 @@ extern "C" hipError_t bse_launch_dielectric(hipStream_t st, int nw, const double
 }
 
 // Permute rows of a complex N x ncol matrix from internal k-major order to the
-// reference order: dst[ref(r'), j] = src[r', j], ref(r') = (r' % B) * nk + r' / B.
+// reference order: dst[ref(r'), j] = src[r', j], ref(r') = (r' % B) * nk +
+// kpos(r' / B), where kpos maps a grid index to the caller's k-list position
+// (kmap, or the identity when kmap is null).
 __global__ __launch_bounds__(256) void k_rows_to_reference(int64_t N, int B, int nk, int ncol,
                                                           const double2* __restrict__ src,
                                                           int64_t lds, double2* __restrict__ dst,
-                                                          int64_t ldd)
+                                                          int64_t ldd, const int* __restrict__ kmap)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (r >= N || j >= ncol) return;
-    const int64_t rr = (r % B) * nk + r / B;
+    const int kg = (int)(r / B);
+    const int64_t rr = (r % B) * nk + (kmap ? kmap[kg] : kg);
     dst[rr + (int64_t)j * ldd] = src[r + (int64_t)j * lds];
 }
 
 extern "C" hipError_t bse_launch_rows_to_reference(hipStream_t st, int64_t N, int B, int nk,
                                                    int ncol, const double2* src, int64_t lds,
-                                                   double2* dst, int64_t ldd)
+                                                   double2* dst, int64_t ldd, const int* kmap)
 {
     if (ncol <= 0) return hipSuccess;
     dim3 grid((unsigned)ceil_div64(N, 256), (unsigned)ncol);
     hipLaunchKernelGGL(k_rows_to_reference, grid, dim3(256), 0, st, N, B, nk, ncol, src, lds, dst,
-                       ldd);
+                       ldd, kmap);
     return hipGetLastError();
 }
 
@@ -246,23 +249,24 @@ extern "C" hipError_t bse_launch_convert_to_fp32(hipStream_t st, int64_t n, cons
 __global__ __launch_bounds__(256) void k_rows_from_reference(int64_t N, int B, int nk, int ncol,
                                                             const double2* __restrict__ src,
                                                             int64_t lds, double2* __restrict__ dst,
-                                                            int64_t ldd)
+                                                            int64_t ldd, const int* __restrict__ kmap)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (r >= N || j >= ncol) return;
-    const int64_t rr = (r % B) * nk + r / B;
+    const int kg = (int)(r / B);
+    const int64_t rr = (r % B) * nk + (kmap ? kmap[kg] : kg);
     dst[r + (int64_t)j * ldd] = src[rr + (int64_t)j * lds];
 }
 
 extern "C" hipError_t bse_launch_rows_from_reference(hipStream_t st, int64_t N, int B, int nk,
                                                      int ncol, const double2* src, int64_t lds,
-                                                     double2* dst, int64_t ldd)
+                                                     double2* dst, int64_t ldd, const int* kmap)
 {
     if (ncol <= 0) return hipSuccess;
     dim3 grid((unsigned)ceil_div64(N, 256), (unsigned)ncol);
     hipLaunchKernelGGL(k_rows_from_reference, grid, dim3(256), 0, st, N, B, nk, ncol, src, lds, dst,
-                       ldd);
+                       ldd, kmap);
     return hipGetLastError();
 }
 

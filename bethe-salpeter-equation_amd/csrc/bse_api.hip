@@ -96,13 +96,11 @@ extern "C" int bse_problem_sizes(const bse_ctx* ctx, int64_t out[4]) {
 // Problem upload: geometry (Crystal_Lattice 85-124, K_points 183-242,
 // G_points 319-385) is derived on the host and copied to the device.
 // ---------------------------------------------------------------------------
-extern "C" int bse_set_problem(bse_ctx* ctx, const bse_tb_model* m, const bse_system* s) {
-    if (!ctx) return BSE_ERR_ARG;
-    Geom g;
-    std::string msg;
-    const int vrc = make_geom(m, s, g, msg);
-    if (vrc) return fail(ctx, vrc, msg);
+// Upload of a problem whose geometry g is derived; kpos (may be null): the
+// caller's k-list position of each grid index.
+static int load_problem(bse_ctx* ctx, const bse_tb_model* m, const Geom& g, const int* kpos) {
     (void)hipSetDevice(ctx->device);
+    const bse_system* s = &g.sys;
     ctx->sys = g.sys;
     ctx->norb = g.norb;
     ctx->nr = g.nr;
@@ -113,6 +111,9 @@ extern "C" int bse_set_problem(bse_ctx* ctx, const bse_tb_model* m, const bse_sy
     for (int i = 0; i < 9; ++i) ctx->recip[i] = g.recip[i];
     ctx->volume = g.volume;
     ctx->area = g.area;
+    ctx->cell = g.cell;
+    ctx->n3 = g.n3;
+    ctx->dim = g.dim;
     ctx->kpts_h = g.kpts;
     ctx->gvec_h = g.gvec;
     ctx->ng = g.ng;
@@ -134,11 +135,124 @@ extern "C" int bse_set_problem(bse_ctx* ctx, const bse_tb_model* m, const bse_sy
     HCK(hipMemcpyAsync(d_k, ctx->kpts_h.data(), 3 * sizeof(double) * ctx->nk, hipMemcpyHostToDevice, ctx->st));
     HCK(hipMemcpyAsync(d_g, ctx->gvec_h.data(), 3 * sizeof(double) * ctx->ng, hipMemcpyHostToDevice, ctx->st));
     HCK(hipMemcpyAsync(d_q, s->q, 3 * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    ctx->kmap_h.clear();
+    ctx->kinv_h.clear();
+    if (kpos) {
+        ctx->kmap_h.assign(kpos, kpos + ctx->nk);
+        ctx->kinv_h.assign(ctx->nk, 0);
+        for (int k = 0; k < ctx->nk; ++k) ctx->kinv_h[kpos[k]] = k;
+        int* d_kmap = BUF(int, S_KMAP, ctx->nk);
+        int* d_kinv = BUF(int, S_KINV, ctx->nk);
+        HCK(hipMemcpyAsync(d_kmap, ctx->kmap_h.data(), sizeof(int) * ctx->nk, hipMemcpyHostToDevice, ctx->st));
+        HCK(hipMemcpyAsync(d_kinv, ctx->kinv_h.data(), sizeof(int) * ctx->nk, hipMemcpyHostToDevice, ctx->st));
+    }
     for (int x = 0; x < 3; ++x) ctx->qhat[x] = g.qhat[x];
     HCK(hipStreamSynchronize(ctx->st));
     ctx->have_problem = true;
     ctx->have_tb = ctx->have_tables = ctx->built = ctx->have_eig = ctx->have_f = false;
     return BSE_OK;
+}
+
+// Device maps of the caller's k order (null for the plain grid order).
+static const int* kmap_dev(const bse_ctx* ctx) {
+    return ctx->kmap_h.empty() ? nullptr : (const int*)ctx->bufs[S_KMAP].p;
+}
+static const int* kinv_dev(const bse_ctx* ctx) {
+    return ctx->kinv_h.empty() ? nullptr : (const int*)ctx->bufs[S_KINV].p;
+}
+
+extern "C" int bse_set_problem(bse_ctx* ctx, const bse_tb_model* m, const bse_system* s) {
+    if (!ctx) return BSE_ERR_ARG;
+    Geom g;
+    std::string msg;
+    const int vrc = make_geom(m, s, g, msg);
+    if (vrc) return fail(ctx, vrc, msg);
+    return load_problem(ctx, m, g, nullptr);
+}
+
+// A k-list in crystal coordinates as a regular grid: along every axis the
+// fractional parts (k - floor k) must take n equally spaced values v0 +
+// j/n, the offset v0 n is the grid shift in steps, and the n1 n2 n3 index
+// triples must each appear once.  2D systems need a zero third coordinate
+// (mod 1).  grid[f] receives the grid index of list entry f.
+static bool klist_to_grid(const double* kc, int nk, int dim, int n[3], double shift[3],
+                          std::vector<int>& grid, std::string& msg) {
+    const double tol = 1e-7;
+    std::vector<int> idx[3];
+    for (int a = 0; a < 3; ++a) {
+        std::vector<double> fr(nk);
+        for (int f = 0; f < nk; ++f) {
+            double x = kc[3 * (size_t)f + a] - floor(kc[3 * (size_t)f + a]);
+            if (x > 1.0 - tol) x = 0.0;
+            fr[f] = x;
+        }
+        if (a == 2 && dim == 2) {
+            for (int f = 0; f < nk; ++f)
+                if (fr[f] > tol) { msg = "2D k-list with a component along b3"; return false; }
+            n[2] = 1;
+            shift[2] = 0.0;
+            idx[2].assign(nk, 0);
+            continue;
+        }
+        std::vector<double> vals(fr);
+        std::sort(vals.begin(), vals.end());
+        std::vector<double> u;
+        for (double v : vals)
+            if (u.empty() || v - u.back() > tol) u.push_back(v);
+        const int na = (int)u.size();
+        for (int j = 0; j < na; ++j)
+            if (fabs(u[j] - u[0] - (double)j / na) > tol) {
+                msg = "k-list coordinates are not equally spaced along an axis";
+                return false;
+            }
+        n[a] = na;
+        shift[a] = u[0] * na;
+        idx[a].resize(nk);
+        for (int f = 0; f < nk; ++f) {
+            const double t = (fr[f] - u[0]) * na;
+            const int j = (int)lround(t);
+            if (fabs(t - j) > 1e-5 || j < 0 || j >= na) { msg = "k-list point off the grid"; return false; }
+            idx[a][f] = j;
+        }
+    }
+    if ((int64_t)n[0] * n[1] * n[2] != nk) { msg = "k-list is not a complete grid"; return false; }
+    grid.assign(nk, 0);
+    std::vector<char> seen(nk, 0);
+    for (int f = 0; f < nk; ++f) {
+        const int gi = (idx[0][f] * n[1] + idx[1][f]) * n[2] + idx[2][f];
+        if (seen[gi]) { msg = "k-list repeats a grid point"; return false; }
+        seen[gi] = 1;
+        grid[f] = gi;
+    }
+    return true;
+}
+
+extern "C" int bse_set_problem_klist(bse_ctx* ctx, const bse_tb_model* m, const bse_system* s,
+                                     const double* k_cryst, int nk) {
+    if (!ctx) return BSE_ERR_ARG;
+    if (!s || !k_cryst || nk < 1) return fail(ctx, BSE_ERR_ARG, "null k-list");
+    if (!finite_all(k_cryst, 3 * (size_t)nk)) return fail(ctx, BSE_ERR_NUMERIC, "non-finite k-list");
+    const int dim = s->dimension == 3 ? 3 : 2;
+    int n[3];
+    double sh[3];
+    std::vector<int> grid;
+    std::string msg;
+    if (!klist_to_grid(k_cryst, nk, dim, n, sh, grid, msg)) return fail(ctx, BSE_ERR_UNSUPPORTED, msg);
+    bse_system s2 = *s;
+    s2.n1 = n[0];
+    s2.n2 = n[1];
+    s2.n3 = n[2];
+    s2.shift[0] = sh[0];
+    s2.shift[1] = sh[1];
+    s2.shift3 = sh[2];
+    Geom g;
+    const int vrc = make_geom(m, &s2, g, msg);
+    if (vrc) return fail(ctx, vrc, msg);
+    std::vector<int> kpos(nk);
+    for (int f = 0; f < nk; ++f) kpos[grid[f]] = f;
+    bool ident = true;
+    for (int k = 0; k < nk; ++k) ident = ident && kpos[k] == k;
+    return load_problem(ctx, m, g, ident ? nullptr : kpos.data());
 }
 
 // ---------------------------------------------------------------------------
@@ -162,12 +276,72 @@ extern "C" int bse_tb_states(bse_ctx* ctx, double* eps, double* coef) {
     (void)hipSetDevice(ctx->device);
     int rc = run_tb(ctx);
     if (rc) return rc;
-    if (eps)
-        HCK(hipMemcpyAsync(eps, ctx->bufs[S_EPS].p, sizeof(double) * ctx->nk * ctx->ns,
-                           hipMemcpyDeviceToHost, ctx->st));
-    if (coef)
-        HCK(hipMemcpyAsync(coef, ctx->bufs[S_COEF].p, sizeof(double2) * ctx->nk * ctx->ns * ctx->norb,
-                           hipMemcpyDeviceToHost, ctx->st));
+    const size_t ne = (size_t)ctx->nk * ctx->ns, nc = ne * ctx->norb;
+    std::vector<double> he(eps ? ne : 0);
+    std::vector<double2> hc(coef ? nc : 0);
+    if (eps) HCK(hipMemcpyAsync(he.data(), ctx->bufs[S_EPS].p, sizeof(double) * ne, hipMemcpyDeviceToHost, ctx->st));
+    if (coef) HCK(hipMemcpyAsync(hc.data(), ctx->bufs[S_COEF].p, sizeof(double2) * nc, hipMemcpyDeviceToHost, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    // rows in the caller's k order
+    const size_t rs = (size_t)ctx->ns, rc2 = rs * ctx->norb;
+    for (int k = 0; k < ctx->nk; ++k) {
+        const size_t kd = ctx->kmap_h.empty() ? (size_t)k : (size_t)ctx->kmap_h[k];
+        if (eps) std::memcpy(eps + kd * rs, he.data() + (size_t)k * rs, sizeof(double) * rs);
+        if (coef) std::memcpy(coef + 2 * kd * rc2, hc.data() + (size_t)k * rc2, sizeof(double2) * rc2);
+    }
+    return BSE_OK;
+}
+
+// States at caller-given k (pull_ks_states / pull_ks_states_subset for a
+// list of k).  The valence half of K1+K2 is asked for at k itself (q = 0).
+extern "C" int bse_tb_states_at(bse_ctx* ctx, const bse_tb_model* m, const double* kpts, int nk,
+                                int n_occ, int nv, int nc, double scissor, double* eps, double* coef) {
+    if (!ctx || !kpts || nk < 1) return BSE_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    int norb, nr;
+    const double* d_rc;
+    const double2* d_hr;
+    if (m) {
+        if (!m->r_cart || !m->h_r || !m->tau) return fail(ctx, BSE_ERR_ARG, "null model array");
+        if (m->norb < 1 || m->norb > BSE_MAXORB) return fail(ctx, BSE_ERR_UNSUPPORTED, "norb must be in [1, 16]");
+        if (m->nr < 1) return fail(ctx, BSE_ERR_ARG, "nr must be >= 1");
+        norb = m->norb;
+        nr = m->nr;
+        const size_t nh = (size_t)nr * norb * norb * 2;
+        if (!finite_all(m->h_r, nh) || !finite_all(m->r_cart, 3 * (size_t)nr))
+            return fail(ctx, BSE_ERR_NUMERIC, "non-finite model input");
+        double* r = BUF(double, S_TBR, 3 * (size_t)nr);
+        double2* h = BUF(double2, S_TBH, (size_t)nr * norb * norb);
+        HCK(hipMemcpyAsync(r, m->r_cart, 3 * sizeof(double) * nr, hipMemcpyHostToDevice, ctx->st));
+        HCK(hipMemcpyAsync(h, m->h_r, nh * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+        d_rc = r;
+        d_hr = h;
+    } else {
+        if (!ctx->have_problem) return fail(ctx, BSE_ERR_STATE, "no model given and no problem loaded");
+        norb = ctx->norb;
+        nr = ctx->nr;
+        d_rc = (const double*)ctx->bufs[S_RCART].p;
+        d_hr = (const double2*)ctx->bufs[S_HR].p;
+    }
+    const bool all = nv == 0 && nc == 0;
+    if (all) {
+        n_occ = 0;
+        nc = norb;
+        scissor = 0.0;
+    } else if (nv < 0 || nc < 0 || n_occ < nv || n_occ + nc > norb) {
+        return fail(ctx, BSE_ERR_ARG, "band window outside the Hamiltonian (nv <= n_occ, n_occ + nc <= norb)");
+    }
+    if (!finite_all(kpts, 3 * (size_t)nk)) return fail(ctx, BSE_ERR_NUMERIC, "non-finite k");
+    const int ns = nv + nc;
+    double* d_k = BUF(double, S_TBK, 3 * (size_t)nk);
+    double* d_e = BUF(double, S_TBE, (size_t)nk * ns);
+    double2* d_c = BUF(double2, S_TBC, (size_t)nk * ns * norb);
+    HCK(hipMemcpyAsync(d_k, kpts, 3 * sizeof(double) * nk, hipMemcpyHostToDevice, ctx->st));
+    const double q0[3] = {0.0, 0.0, 0.0};
+    HCK(bse_launch_tb_states(ctx->st, nk, norb, nr, d_rc, d_hr, d_k, q0, n_occ, nv, nc, scissor, nullptr,
+                             d_e, d_c, nullptr, nullptr));
+    if (eps) HCK(hipMemcpyAsync(eps, d_e, sizeof(double) * nk * ns, hipMemcpyDeviceToHost, ctx->st));
+    if (coef) HCK(hipMemcpyAsync(coef, d_c, sizeof(double2) * nk * ns * norb, hipMemcpyDeviceToHost, ctx->st));
     HCK(hipStreamSynchronize(ctx->st));
     return BSE_OK;
 }
@@ -175,7 +349,7 @@ extern "C" int bse_tb_states(bse_ctx* ctx, double* eps, double* coef) {
 static int run_tables(bse_ctx* ctx) {
     const bse_system& s = ctx->sys;
     const int no = ctx->norb;
-    const int64_t nw = (int64_t)(2 * s.n1 - 1) * (2 * s.n2 - 1) * no * no;
+    const int64_t nw = ktable_size(s.n1, s.n2, ctx->n3) * no * no;
     double2* wtab = BUF(double2, S_WTAB, nw);
     double2* vx = BUF(double2, S_VX, (size_t)no * no);
     double2* P = BUF(double2, S_P, (size_t)ctx->N * no);
@@ -185,10 +359,9 @@ static int run_tables(bse_ctx* ctx) {
     const double* gvec = (const double*)ctx->bufs[S_GVEC].p;
     const double* tau = (const double*)ctx->bufs[S_TAU].p;
     stage_begin(ctx, ST_TABLES);
-    double bvec[6] = {ctx->recip[0], ctx->recip[1], ctx->recip[2], ctx->recip[3], ctx->recip[4], ctx->recip[5]};
-    HCK(bse_launch_w_table(ctx->st, s.n1, s.n2, no, ctx->ng, gvec, bvec, tau, s.r0, s.eps_env,
-                           0.5 * kC, ctx->wbar, 0, wtab));
-    HCK(bse_launch_v_exchange(ctx->st, no, ctx->ng, gvec, tau, 0.5 * kC, vx));
+    HCK(bse_launch_w_table(ctx->st, s.n1, s.n2, ctx->n3, ctx->dim, no, ctx->ng, gvec, ctx->recip, tau,
+                           s.r0, s.eps_env, s.radius, kC, ctx->wbar, 0, wtab));
+    HCK(bse_launch_v_exchange(ctx->st, no, ctx->ng, ctx->dim, gvec, tau, kC, s.radius, vx));
     HCK(bse_launch_transitions(ctx->st, ctx->nk, s.nv, s.nc, no, (const double*)ctx->bufs[S_EPS].p,
                                (const double2*)ctx->bufs[S_COEF].p, nullptr, nullptr, tau, ctx->qhat,
                                vx, P, PV, rho, dE));
@@ -213,14 +386,14 @@ extern "C" int bse_build(bse_ctx* ctx, int precision) {
     // ordered by tile column and then tile row (one tile column is contiguous)
     void* H = buf(ctx, S_H, (size_t)tiled_elems(N) * (fp32 ? sizeof(float2) : sizeof(double2)), &rc);
     if (!H) return rc;
-    const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
-    const double beta = s.f_s * s.exchange_scale / (ctx->area * ctx->nk);
+    const double alpha = -s.direct_scale / (ctx->cell * ctx->nk);
+    const double beta = s.f_s * s.exchange_scale / (ctx->cell * ctx->nk);
     const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
     stage_begin(ctx, ST_ASM);
     HCK(bse_launch_tile_pad_zero(ctx->st, fp32, N, H));
     if (!ctx->force_valu_assembly && bse_assemble_mfma_supported(s.nv, s.nc, ctx->norb)) {
         // matrix cores: FP64 MFMA (FP64 storage), FP32 MFMA or compensated BF16 MFMA (FP32 storage)
-        HCK(bse_launch_assemble_mfma(ctx->st, precision, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb,
+        HCK(bse_launch_assemble_mfma(ctx->st, precision, ctx->nk, s.n1, s.n2, ctx->n3, s.nv, s.nc, ctx->norb,
                                      N, coef, (const double2*)ctx->bufs[S_WTAB].p,
                                      (const double2*)ctx->bufs[S_PV].p,
                                      (const double2*)ctx->bufs[S_P].p,
@@ -228,7 +401,7 @@ extern "C" int bse_build(bse_ctx* ctx, int precision) {
         ctx->asm_kernel = 1 + precision;
     } else {
         // small band windows (e.g. 1v+1c): FP64 VALU tiles, FP64 or FP32 storage
-        HCK(bse_launch_assemble(ctx->st, fp32, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb, ctx->TK,
+        HCK(bse_launch_assemble(ctx->st, fp32, ctx->nk, s.n1, s.n2, ctx->n3, s.nv, s.nc, ctx->norb, ctx->TK,
                                 N, coef, coef, coef, coef, (const double2*)ctx->bufs[S_WTAB].p,
                                 (const double2*)ctx->bufs[S_PV].p, (const double2*)ctx->bufs[S_P].p,
                                 (const double*)ctx->bufs[S_DE].p, alpha, beta, H, 1));
@@ -257,7 +430,7 @@ extern "C" int bse_get_matrix(bse_ctx* ctx, void* dst, int64_t ld, int dst_on_de
         if (!out) return rc;
     }
     HCK(bse_launch_expand(ctx->st, fp32, N, ctx->B, ctx->nk, ctx->bufs[S_H].p, out, ld,
-                          order == BSE_ORDER_REFERENCE));
+                          order == BSE_ORDER_REFERENCE, kinv_dev(ctx)));
     if (!dst_on_device) HCK(hipMemcpyAsync(dst, out, (size_t)ld * N * es, hipMemcpyDeviceToHost, ctx->st));
     HCK(hipStreamSynchronize(ctx->st));
     ctx->have_eig = false;
@@ -270,7 +443,7 @@ extern "C" int bse_get_dipoles(bse_ctx* ctx, double* rho) {
     (void)hipSetDevice(ctx->device);
     double2* tmp = BUF(double2, S_W1, ctx->N);
     HCK(bse_launch_rows_to_reference(ctx->st, ctx->N, ctx->B, ctx->nk, 1,
-                                     (const double2*)ctx->bufs[S_RHO].p, ctx->N, tmp, ctx->N));
+                                     (const double2*)ctx->bufs[S_RHO].p, ctx->N, tmp, ctx->N, kmap_dev(ctx)));
     HCK(hipMemcpyAsync(rho, tmp, sizeof(double2) * ctx->N, hipMemcpyDeviceToHost, ctx->st));
     HCK(hipStreamSynchronize(ctx->st));
     return BSE_OK;
@@ -430,8 +603,11 @@ static int autotune_her2k(bse_ctx* ctx, int fp32, void* scratch, size_t scratch_
     return BSE_OK;
 }
 
+// guard_tol > 0: eigenvalues within guard_tol above the last requested one
+// are formed as well, up to max_vectors in total (ctx->nvec reports the count).
 static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
-                        const double2* rho, int n_vectors)
+                        const double2* rho, int n_vectors, double guard_tol = 0.0,
+                        int max_vectors = 0)
 {
     const int n = (int)n64;
     const int nb = kPanel;
@@ -607,8 +783,38 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
 
     // ---------------- K6: tridiagonal eigenvectors by batches
     stage_begin(ctx, ST_TVEC);
-    const int nstore = (n_vectors < 0) ? n : std::min(n_vectors, n);
-    const double tolc = 1e-7 * tnorm;
+    int nstore = (n_vectors < 0) ? n : std::min(n_vectors, n);
+    if (guard_tol < 0.0) guard_tol = 1e-4 * tnorm;
+    if (nstore > 0 && nstore < n && guard_tol > 0.0) {
+        // guard vectors: eigenvalues just above the last requested one that
+        // lie within guard_tol of it (a cluster straddling the cut) are formed
+        // too, so that a refinement sees the whole cluster
+        while (nstore < n && nstore < max_vectors && lam[nstore] - lam[nstore - 1] <= guard_tol) ++nstore;
+    }
+    ctx->hnorm = tnorm;
+    const double tolc = 1e-6 * tnorm;
+    // Clusters of close eigenvalues inside one unreduced block.  The global
+    // ascending order interleaves the blocks, so each block's eigenvalues are
+    // walked on their own: memb = sorted positions ordered by (block start,
+    // position); a cluster is a run of memb with gaps <= tolc, given as
+    // (offset into memb, size).  Its members span [memb[off], memb[off+size-1]]
+    // in the global order.
+    std::vector<int> memb(n);
+    for (int i = 0; i < n; ++i) memb[i] = i;
+    std::stable_sort(memb.begin(), memb.end(), [&](int a, int b) { return bsv[a] < bsv[b]; });
+    std::vector<int2> clall;
+    std::vector<int> nocut(n + 1, 0);   // difference array: cut points inside a cluster's span
+    for (int q = 0; q < n;) {
+        int j = q + 1;
+        while (j < n && bsv[memb[j]] == bsv[memb[q]] && lam[memb[j]] - lam[memb[j - 1]] <= tolc) ++j;
+        if (j - q > 1) {
+            clall.push_back(make_int2(q, j - q));
+            ++nocut[memb[q] + 1];
+            --nocut[memb[j - 1] + 1];
+        }
+        q = j;
+    }
+    for (int i = 1; i <= n; ++i) nocut[i] += nocut[i - 1];
     // one thread per eigenvalue; as many per batch as 24 GB of scratch allows
     const int64_t cap = std::max<int64_t>(4096, (int64_t)(24e9 / (48.0 * n)));
     int bsz = (int)std::min<int64_t>(n, cap);
@@ -618,26 +824,27 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
     double2* Z = nullptr;
     if (nstore > 0) Z = BUF(double2, S_Z, (size_t)n * nstore);
     double2* fvec = rho ? BUF(double2, S_F, n) : nullptr;
-    int2* d_cl = BUF(int2, S_CLUST, bsz);
+    int2* d_cl = BUF(int2, S_CLUST, std::max<size_t>(clall.size(), 1));
+    int* d_memb = BUF(int, S_MEMB, n);
+    HCK(hipMemcpyAsync(d_memb, memb.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     if (rho) HCK(hipStreamWaitEvent(st, ctx->join_ev, 0));   // y = Q^H rho is ready
     int m0 = 0;
+    size_t ci = 0;   // clusters are visited in order of their first member
+    std::vector<int2> clall_sorted = clall;
+    std::sort(clall_sorted.begin(), clall_sorted.end(),
+              [&](int2 a, int2 b) { return memb[a.x] < memb[b.x]; });
     while (m0 < n) {
         int m1 = std::min(n, m0 + bsz);
         // do not split a cluster across batches
-        while (m1 < n && m1 > m0 + 1 && bsv[m1] == bsv[m1 - 1] && lam[m1] - lam[m1 - 1] <= tolc) --m1;
+        while (m1 < n && m1 > m0 && nocut[m1] > 0) --m1;
+        if (m1 == m0) return fail(ctx, BSE_ERR_UNSUPPORTED, "eigenvalue cluster larger than the inverse-iteration batch");
         const int cnt = m1 - m0;
         std::vector<int2> cls;
-        int i = m0;
-        while (i < m1) {
-            int j = i + 1;
-            while (j < m1 && bsv[j] == bsv[i] && lam[j] - lam[j - 1] <= tolc) ++j;
-            if (j - i > 1) cls.push_back(make_int2(i - m0, j - i));
-            i = j;
-        }
+        while (ci < clall_sorted.size() && memb[clall_sorted[ci].x] < m1) cls.push_back(clall_sorted[ci++]);
         HCK(bse_launch_inverse_iter(st, bsz, cnt, m0, d_lam, d_bsv, d_nbv, d, e, tnorm, scr, n, xb));
         if (!cls.empty()) {
             HCK(hipMemcpyAsync(d_cl, cls.data(), sizeof(int2) * cls.size(), hipMemcpyHostToDevice, st));
-            HCK(bse_launch_cluster_mgs(st, bsz, (int)cls.size(), d_cl, m0, d_bsv, d_nbv, xb));
+            HCK(bse_launch_cluster_mgs(st, bsz, (int)cls.size(), d_cl, m0, d_bsv, d_nbv, d_memb, xb));
         }
         if (rho) {
             // the inverse-iteration scratch is idle until the next batch: it holds the segment partials
@@ -689,19 +896,24 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
 static int apply_H(bse_ctx* ctx, int k, const double2* X, int64_t ldx, double2* Y, int64_t ldy) {
     const bse_system& s = ctx->sys;
     const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
-    const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
-    const double beta = s.f_s * s.exchange_scale / (ctx->area * ctx->nk);
-    if (k <= 16 && !ctx->force_valu_assembly && bse_assemble_mfma_supported(s.nv, s.nc, ctx->norb)) {
-        // matrix cores: tiles of H formed in LDS and multiplied at once
+    const double alpha = -s.direct_scale / (ctx->cell * ctx->nk);
+    const double beta = s.f_s * s.exchange_scale / (ctx->cell * ctx->nk);
+    if (!ctx->force_valu_assembly && bse_assemble_mfma_supported(s.nv, s.nc, ctx->norb)) {
+        // matrix cores: tiles of H formed in LDS and multiplied at once, at
+        // most 16 columns per launch
         double2* ws = BUF(double2, S_APPLYP, bse_apply_mfma_workspace(ctx->nk, s.nv, s.nc));
-        HCK(bse_launch_apply_mfma(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb, coef,
-                                  (const double2*)ctx->bufs[S_WTAB].p, (const double2*)ctx->bufs[S_PV].p,
-                                  (const double2*)ctx->bufs[S_P].p, (const double*)ctx->bufs[S_DE].p,
-                                  alpha, beta, k, X, ldx, Y, ldy, ws));
+        for (int c0 = 0; c0 < k; c0 += 16) {
+            const int kc = std::min(16, k - c0);
+            HCK(bse_launch_apply_mfma(ctx->st, ctx->nk, s.n1, s.n2, ctx->n3, s.nv, s.nc, ctx->norb, coef,
+                                      (const double2*)ctx->bufs[S_WTAB].p, (const double2*)ctx->bufs[S_PV].p,
+                                      (const double2*)ctx->bufs[S_P].p, (const double*)ctx->bufs[S_DE].p,
+                                      alpha, beta, kc, X + (int64_t)c0 * ldx, ldx, Y + (int64_t)c0 * ldy,
+                                      ldy, ws));
+        }
         return BSE_OK;
     }
     HCK(hipMemsetAsync(Y, 0, sizeof(double2) * ldy * k, ctx->st));
-    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, ctx->norb,
+    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, ctx->n3, s.nv, s.nc, ctx->norb,
                          bse_apply_pick_tk(s.nv, s.nc, ctx->norb), coef, coef, coef, coef,
                          (const double2*)ctx->bufs[S_WTAB].p, (const double2*)ctx->bufs[S_PV].p,
                          (const double2*)ctx->bufs[S_P].p, (const double*)ctx->bufs[S_DE].p, alpha,
@@ -709,9 +921,22 @@ static int apply_H(bse_ctx* ctx, int k, const double2* X, int64_t ldx, double2* 
     return BSE_OK;
 }
 
-static int refine_lowest(bse_ctx* ctx, int k, int iters) {
+// Stopping rule of the refinement.  Each round is a Rayleigh-Ritz step on
+// [X, R]; it stops when every requested vector's residual
+// ||H x - theta x|| is below kRefineTol * ||H|| (||H|| bounded by the
+// Gershgorin norm of the tridiagonal), when a round takes less than 20% off
+// the largest residual (slow convergence, or the FP64 rounding floor of the
+// matrix-free product), or after kRefineMaxRounds rounds.  The result is accepted if the largest
+// residual is below kRefineAccept * ||H||: for a Hermitian H each refined
+// value theta_j then lies within ||r_j|| of an eigenvalue.
+static const double kRefineTol = 1e-11, kRefineAccept = 1e-7;
+static const int kRefineMaxRounds = 10;
+
+// k_req requested vectors, kw >= k_req formed ones (guard vectors of a cluster
+// that straddles the cut); the block refined is all kw, 2 kw <= 40.
+static int refine_lowest(bse_ctx* ctx, int k_req, int kw) {
     const int64_t N = ctx->N;
-    const int m = 2 * k;
+    const int k = kw, m = 2 * k;
     hipStream_t st = ctx->st;
     double2* Z = (double2*)ctx->bufs[S_Z].p;               // N x nvec, internal order
     double2* S = BUF(double2, S_RS, (size_t)N * m);        // [X | R]
@@ -720,14 +945,42 @@ static int refine_lowest(bse_ctx* ctx, int k, int iters) {
     double2* Gm = BUF(double2, S_RG, (size_t)m * m);
     double2* Hm = BUF(double2, S_RHG, (size_t)m * m);
     double2* Cm = BUF(double2, S_RC, (size_t)m * k);
-    double* small = BUF(double, S_RSMALL, 4 * k + 8);      // evals k, rnorm k, info
+    // small: [0, k) Ritz values, [k, 2k) residual norms, [2k] rank (int),
+    // [2k+8, 3k+8) Rayleigh quotients
+    double* small = BUF(double, S_RSMALL, 4 * (size_t)k + 16);
+    double* rq = small + 2 * k + 8;
     double2* skw = (double2*)ctx->bufs[S_SPLITK].p;
     const int64_t sk_el = (int64_t)(ctx->bufs[S_SPLITK].cap / sizeof(double2));
+    const double hn = ctx->hnorm > 0.0 ? ctx->hnorm : 1.0;
     HCK(hipMemcpyAsync(S, Z, sizeof(double2) * N * k, hipMemcpyDeviceToDevice, st));
+    HCK(hipMemsetAsync(small + 2 * k, 0, sizeof(double) * 8, st));
     int rc = apply_H(ctx, k, S, N, HS, N);
     if (rc) return rc;
-    for (int it = 0; it < iters; ++it) {
-        HCK(bse_launch_residual(st, N, k, S, HS, N, S + N * k, small + k));
+    std::vector<double> hsm(3 * (size_t)k + 8);
+    double prev = 0.0;
+    int rounds = 0;
+    ctx->ref_status = BSE_OK;
+    for (;; ++rounds) {
+        // residuals and Rayleigh quotients of the current X; one host read per
+        // round carries them and the rank kept by the previous Rayleigh-Ritz
+        HCK(bse_launch_residual(st, N, k, S, HS, N, S + N * k, small + k, rq));
+        HCK(hipMemcpyAsync(hsm.data(), small + k, sizeof(double) * (k + 8), hipMemcpyDeviceToHost, st));
+        HCK(hipStreamSynchronize(st));
+        int rank = 0;
+        std::memcpy(&rank, &hsm[k], sizeof(int));
+        if (rounds > 0 && rank < k)
+            return fail(ctx, BSE_ERR_NUMERIC, "refinement basis lost rank (" + std::to_string(rank) +
+                                                  " of " + std::to_string(k) + ")");
+        double rmax = 0.0;
+        for (int j = 0; j < k_req; ++j) {
+            if (!std::isfinite(hsm[j])) return fail(ctx, BSE_ERR_NUMERIC, "non-finite refinement residual");
+            rmax = std::max(rmax, hsm[j]);
+        }
+        ctx->ref_rnorm.assign(hsm.begin(), hsm.begin() + k);
+        const bool done = rmax <= kRefineTol * hn || rounds >= kRefineMaxRounds ||
+                          (rounds >= 2 && rmax > 0.8 * prev);
+        prev = rmax;
+        if (done) break;
         rc = apply_H(ctx, k, S + N * k, N, HS + N * k, N);
         if (rc) return rc;
         HCK(bse_zgemm(st, 1, 0, m, m, (int)N, 1, 0, S, N, S, N, 0, 0, Gm, m, skw, sk_el));
@@ -738,12 +991,21 @@ static int refine_lowest(bse_ctx* ctx, int k, int iters) {
         HCK(hipMemcpyAsync(S, T, sizeof(double2) * N * k, hipMemcpyDeviceToDevice, st));
         HCK(hipMemcpyAsync(HS, T + N * k, sizeof(double2) * N * k, hipMemcpyDeviceToDevice, st));
     }
+    ctx->ref_rounds = rounds;
+    double rmax = 0.0;
+    for (int j = 0; j < k_req; ++j) rmax = std::max(rmax, ctx->ref_rnorm[j]);
     HCK(hipMemcpyAsync(Z, S, sizeof(double2) * N * k, hipMemcpyDeviceToDevice, st));
-    HCK(hipMemcpyAsync(ctx->bufs[S_EVAL].p, small, sizeof(double) * k, hipMemcpyDeviceToDevice, st));
+    // refined eigenvalues: the Rayleigh quotients of the final vectors
+    HCK(hipMemcpyAsync(ctx->bufs[S_EVAL].p, rq, sizeof(double) * k, hipMemcpyDeviceToDevice, st));
     // oscillator strengths of the refined vectors: f^T = rho^H X
     HCK(bse_zgemm(st, 1, 0, 1, k, (int)N, 1, 0, (const double2*)ctx->bufs[S_RHO].p, N, S, N, 0, 0,
                   (double2*)ctx->bufs[S_F].p, 1, skw, sk_el));
     HCK(hipStreamSynchronize(st));
+    if (rmax > kRefineAccept * hn) {
+        ctx->ref_status = BSE_ERR_NUMERIC;
+        return fail(ctx, BSE_ERR_NUMERIC, "refinement did not converge: largest residual " +
+                                              std::to_string(rmax) + " eV");
+    }
     return BSE_OK;
 }
 
@@ -760,10 +1022,10 @@ extern "C" int bse_apply(bse_ctx* ctx, int k, const double* x, int64_t ldx, doub
     HCK(hipMemcpy2DAsync(tmp, sizeof(double2) * N, x, sizeof(double2) * ldx, sizeof(double2) * N, k,
                          hipMemcpyHostToDevice, ctx->st));
     // reference order -> internal order: xin[r'] = x[ref(r')]
-    HCK(bse_launch_rows_from_reference(ctx->st, N, ctx->B, ctx->nk, k, tmp, N, xin, N));
+    HCK(bse_launch_rows_from_reference(ctx->st, N, ctx->B, ctx->nk, k, tmp, N, xin, N, kmap_dev(ctx)));
     int rc = apply_H(ctx, k, xin, N, yout, N);
     if (rc) return rc;
-    HCK(bse_launch_rows_to_reference(ctx->st, N, ctx->B, ctx->nk, k, yout, N, tmp, N));
+    HCK(bse_launch_rows_to_reference(ctx->st, N, ctx->B, ctx->nk, k, yout, N, tmp, N, kmap_dev(ctx)));
     HCK(hipMemcpy2DAsync(y, sizeof(double2) * ldy, tmp, sizeof(double2) * N, sizeof(double2) * N, k,
                          hipMemcpyDeviceToHost, ctx->st));
     HCK(hipStreamSynchronize(ctx->st));
@@ -818,20 +1080,20 @@ static int grad_tb_apply(bse_ctx* ctx, int which, int k, const double2* A, doubl
                                (const double2*)ctx->bufs[S_VX].p, dP, dPV, drho, ddE));
     const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
     const double2* wt = (const double2*)ctx->bufs[S_WTAB].p;
-    const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
-    const double beta = s.f_s * s.exchange_scale / (ctx->area * ctx->nk);
+    const double alpha = -s.direct_scale / (ctx->cell * ctx->nk);
+    const double beta = s.f_s * s.exchange_scale / (ctx->cell * ctx->nk);
     const int tk = bse_apply_pick_tk(s.nv, s.nc, no);
     if (!Bm) HCK(hipMemsetAsync(Y, 0, sizeof(double2) * ctx->N * k, ctx->st));
     const double2* fac[4][4] = {{dcoef, coef, coef, coef}, {coef, dcoef, coef, coef},
                                 {coef, coef, dcoef, coef}, {coef, coef, coef, dcoef}};
     for (int t = 0; t < 4; ++t)
-        HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no, tk, fac[t][0], fac[t][1],
+        HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, ctx->n3, s.nv, s.nc, no, tk, fac[t][0], fac[t][1],
                              fac[t][2], fac[t][3], wt, nullptr, nullptr, nullptr, alpha, 0.0, k, A,
                              ctx->N, Y, ctx->N, Bm, ctx->N, trp));
     // exchange: beta (dP V P^H + P V dP^H); diagonal d(eps_c - eps_v)
-    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no, tk, coef, coef, coef, coef, wt,
+    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, ctx->n3, s.nv, s.nc, no, tk, coef, coef, coef, coef, wt,
                          dPV, (const double2*)ctx->bufs[S_P].p, ddE, 0.0, beta, k, A, ctx->N, Y, ctx->N, Bm, ctx->N, trp));
-    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no, tk, coef, coef, coef, coef, wt,
+    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, ctx->n3, s.nv, s.nc, no, tk, coef, coef, coef, coef, wt,
                          (const double2*)ctx->bufs[S_PV].p, dP, nullptr, 0.0, beta, k, A, ctx->N, Y,
                          ctx->N, Bm, ctx->N, trp));
     return BSE_OK;
@@ -841,27 +1103,29 @@ static int grad_w_apply(bse_ctx* ctx, int which, int k, const double2* A, double
                         const double2* Bm = nullptr, double2* trp = nullptr) {
     const bse_system& s = ctx->sys;
     const int no = ctx->norb;
-    const int64_t nw = (int64_t)(2 * s.n1 - 1) * (2 * s.n2 - 1) * no * no;
+    const int64_t nw = ktable_size(s.n1, s.n2, ctx->n3) * no * no;
     double2* dW = BUF(double2, S_GDW, nw);
-    // derivative of the mini-BZ average
-    const double da = 4.0 * M_PI * M_PI / (ctx->area * ctx->nk);
+    // derivative of the mini-BZ average (the 3D potential has no r0)
+    const double da = 4.0 * M_PI * M_PI / (ctx->cell * ctx->nk);
     const double rad = sqrt(da / M_PI);
     double dwbar;
-    if (which == 2) {
+    if (which == 2 && ctx->dim == 3) {
+        dwbar = 0.0;
+    } else if (which == 2) {
         const double r0 = s.r0;
         dwbar = (r0 > 0.0) ? (M_PI * kC / (s.eps_env * da)) * (rad / ((1.0 + r0 * rad) * r0) - log1p(r0 * rad) / (r0 * r0))
                            : -(M_PI * kC / (s.eps_env * da)) * 0.5 * rad * rad;
     } else {
         dwbar = -ctx->wbar / s.eps_env;
     }
-    double bvec[6] = {ctx->recip[0], ctx->recip[1], ctx->recip[2], ctx->recip[3], ctx->recip[4], ctx->recip[5]};
-    HCK(bse_launch_w_table(ctx->st, s.n1, s.n2, no, ctx->ng, (const double*)ctx->bufs[S_GVEC].p, bvec,
-                           (const double*)ctx->bufs[S_TAU].p, s.r0, s.eps_env, 0.5 * kC, dwbar,
+    HCK(bse_launch_w_table(ctx->st, s.n1, s.n2, ctx->n3, ctx->dim, no, ctx->ng,
+                           (const double*)ctx->bufs[S_GVEC].p, ctx->recip,
+                           (const double*)ctx->bufs[S_TAU].p, s.r0, s.eps_env, s.radius, kC, dwbar,
                            which == 2 ? 1 : 2, dW));
     const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
-    const double alpha = -s.direct_scale / (ctx->area * ctx->nk);
+    const double alpha = -s.direct_scale / (ctx->cell * ctx->nk);
     if (!Bm) HCK(hipMemsetAsync(Y, 0, sizeof(double2) * ctx->N * k, ctx->st));
-    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, s.nv, s.nc, no,
+    HCK(bse_launch_apply(ctx->st, ctx->nk, s.n1, s.n2, ctx->n3, s.nv, s.nc, no,
                          bse_apply_pick_tk(s.nv, s.nc, no), coef, coef, coef, coef, dW, nullptr,
                          nullptr, nullptr, alpha, 0.0, k, A, ctx->N, Y, ctx->N, Bm, ctx->N, trp));
     return BSE_OK;
@@ -990,18 +1254,36 @@ extern "C" int bse_eigh(bse_ctx* ctx, int n_vectors, int refine) {
     if (ctx->have_eig) return fail(ctx, BSE_ERR_STATE, "matrix already reduced; build again");
     if (refine && (n_vectors <= 0 || n_vectors > 16))
         return fail(ctx, BSE_ERR_UNSUPPORTED, "refinement needs 1 <= n_vectors <= 16");
+    if (refine && n_vectors > ctx->N)
+        return fail(ctx, BSE_ERR_ARG, "refinement needs n_vectors <= N");
     (void)hipSetDevice(ctx->device);
     const int fp32 = ctx->precision != BSE_PREC_FP64;
     ctx->eig_on_bse = true;
+    ctx->ref_rnorm.clear();
+    ctx->ref_rounds = 0;
+    ctx->ref_status = BSE_OK;
+    // with refinement, up to 4 guard vectors join a cluster that straddles
+    // the cut (eigenvalues within 1e-4 ||T|| of the last requested one), as
+    // long as the Rayleigh-Ritz basis [X, R] stays within 40 columns and N
+    const int maxv = refine ? (int)std::min<int64_t>(std::min(n_vectors + 4, 20), ctx->N) : 0;
     int rc = eig_pipeline(ctx, ctx->N, ctx->bufs[S_H].p, fp32,
-                          (const double2*)ctx->bufs[S_RHO].p, n_vectors);
+                          (const double2*)ctx->bufs[S_RHO].p, n_vectors,
+                          refine ? -1.0 : 0.0, maxv);
     // an early error return may leave Q^H rho running on the side stream
     if (rc) (void)hipStreamSynchronize(ctx->side);
     if (rc || !refine) return rc;
     stage_begin(ctx, ST_REF);
-    rc = refine_lowest(ctx, n_vectors, 2);
+    rc = refine_lowest(ctx, n_vectors, ctx->nvec);
     stage_end(ctx, ST_REF);
     return rc;
+}
+
+extern "C" int bse_refine_info(const bse_ctx* ctx, double* rnorm, int n, int* rounds) {
+    if (!ctx || n < 0) return BSE_ERR_ARG;
+    if (n > (int)ctx->ref_rnorm.size()) return BSE_ERR_ARG;
+    for (int j = 0; j < n; ++j) rnorm[j] = ctx->ref_rnorm[j];
+    if (rounds) *rounds = ctx->ref_rounds;
+    return ctx->ref_status;
 }
 
 extern "C" int bse_get_eigenvalues(bse_ctx* ctx, double* evals) {
@@ -1024,7 +1306,8 @@ extern "C" int bse_get_eigenvectors(bse_ctx* ctx, double* z, int64_t ld, int nve
     const int nk = ctx->eig_on_bse ? ctx->nk : (int)n;
     double2* dst = (double2*)z;
     if (!dst_on_device) dst = BUF(double2, S_W2, (size_t)ld * nvec);
-    HCK(bse_launch_rows_to_reference(ctx->st, n, B, nk, nvec, Z, n, dst, ld));
+    HCK(bse_launch_rows_to_reference(ctx->st, n, B, nk, nvec, Z, n, dst, ld,
+                                     ctx->eig_on_bse ? kmap_dev(ctx) : nullptr));
     if (!dst_on_device)
         HCK(hipMemcpyAsync(z, dst, sizeof(double2) * ld * nvec, hipMemcpyDeviceToHost, ctx->st));
     HCK(hipStreamSynchronize(ctx->st));

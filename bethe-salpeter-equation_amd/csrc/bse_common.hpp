@@ -190,6 +190,31 @@ __device__ inline int mfma16_row_f64(int lane, int r) { return (lane >> 4) + 4 *
 __host__ __device__ inline int64_t ceil_div64(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---------------------------------------------------------------------------
+// k grid n1 x n2 x n3 (n3 = 1 for 2D systems), k index (i n2 + j) n3 + l.
+// The interaction table holds W_ij(Q) for every grid difference
+// Q = (di/n1) b1 + (dj/n2) b2 + (dl/n3) b3, |di| < n1, ..., ordered as a
+// mixed-radix number with digits di + n1 - 1, dj + n2 - 1, dl + n3 - 1 of
+// widths 2 n1 - 1, 2 n2 - 1, 2 n3 - 1.  kcode(k) writes (i, j, l) in that
+// radix, so table_index(k, k') = kcode(k) - kcode(k') + kcode_origin is the
+// entry of k - k' (each digit difference stays inside its width, so the
+// subtraction never borrows).  For n3 = 1 this is (di + n1 - 1)(2 n2 - 1) +
+// dj + n2 - 1.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int kcode(int k, int n2, int n3) {
+    const int l = k % n3, ij = k / n3;
+    return ((ij / n2) * (2 * n2 - 1) + ij % n2) * (2 * n3 - 1) + l;
+}
+__host__ __device__ inline int kcode_origin(int n1, int n2, int n3) {
+    return ((n1 - 1) * (2 * n2 - 1) + (n2 - 1)) * (2 * n3 - 1) + (n3 - 1);
+}
+__host__ __device__ inline int ktable_index(int k, int kp, int n1, int n2, int n3) {
+    return kcode(k, n2, n3) - kcode(kp, n2, n3) + kcode_origin(n1, n2, n3);
+}
+__host__ __device__ inline int64_t ktable_size(int n1, int n2, int n3) {
+    return (int64_t)(2 * n1 - 1) * (2 * n2 - 1) * (2 * n3 - 1);
+}
+
+// ---------------------------------------------------------------------------
 // Storage of the assembled N x N Hermitian matrix: lower triangle only, in
 // 64 x 64 tiles that are contiguous in memory (column-major inside a tile),
 // tiles ordered by tile column and, inside a column, by tile row from the

@@ -16,20 +16,22 @@ extern "C" {
 hipError_t bse_launch_tb_states(hipStream_t, int, int, int, const double*, const double2*,
                                 const double*, const double*, int, int, int, double,
                                 const double2*, double*, double2*, double*, double2*);
-hipError_t bse_launch_w_table(hipStream_t, int, int, int, int, const double*, const double*,
-                              const double*, double, double, double, double, int, double2*);
-hipError_t bse_launch_v_exchange(hipStream_t, int, int, const double*, const double*, double,
-                                 double2*);
+hipError_t bse_launch_w_table(hipStream_t, int, int, int, int, int, int, const double*,
+                              const double*, const double*, double, double, double, double, double,
+                              int, double2*);
+hipError_t bse_launch_v_exchange(hipStream_t, int, int, int, const double*, const double*, double,
+                                 double, double2*);
 hipError_t bse_launch_transitions(hipStream_t, int, int, int, int, const double*, const double2*,
                                   const double*, const double2*, const double*, const double*,
                                   const double2*, double2*, double2*, double2*, double*);
 size_t bse_assemble_lds_bytes(int, int, int, int);
 int bse_assemble_pick_tk(int, int, int);
-hipError_t bse_launch_assemble(hipStream_t, int, int, int, int, int, int, int, int, int64_t,
+hipError_t bse_launch_assemble(hipStream_t, int, int, int, int, int, int, int, int, int, int64_t,
                                const double2*, const double2*, const double2*, const double2*,
                                const double2*, const double2*, const double2*, const double*,
                                double, double, void*, int);
-hipError_t bse_launch_expand(hipStream_t, int, int64_t, int, int, const void*, void*, int64_t, int);
+hipError_t bse_launch_expand(hipStream_t, int, int64_t, int, int, const void*, void*, int64_t, int,
+                             const int*);
 hipError_t bse_launch_to_tiles(hipStream_t, int, int64_t, const double2*, int64_t, void*);
 hipError_t bse_launch_tile_pad_zero(hipStream_t, int, int64_t, void*);
 void bse_hetrd_workspace(int, int, int64_t*, int64_t*, int64_t*, int64_t*);
@@ -46,11 +48,11 @@ hipError_t bse_launch_her2k(hipStream_t, int, int, int, int, void*, const double
 hipError_t bse_launch_tri_prepare(hipStream_t, int, const double*, double*, double*, int4*,
                                   double2*, double*);
 hipError_t bse_launch_tri_sort(hipStream_t, int, const double*, const int4*, const double*,
-                               double*, int*, int*, int2*, int*);
+                               double*, int*, int*, int2*, int*, int*);
 hipError_t bse_launch_tri_vectors_dev(hipStream_t, int, const int4*, const double2*, const double*,
                                       const double*, const double*, const double*, double*,
                                       const double*, const int*, const int*, const int2*,
-                                      const int*, double*, double*, int);
+                                      const int*, const int*, double*, double*, int);
 hipError_t bse_launch_apply_q(hipStream_t, int, int, const double2*, const double2*, int, double2*,
                               int64_t, int);
 int64_t bse_apply_q_thin_workspace(int);
@@ -69,7 +71,7 @@ hipError_t bse_launch_inverse_iter(hipStream_t, int, int, int, const double*, co
                                    const int*, const double*, const double*, double, double*, int,
                                    double*);
 hipError_t bse_launch_cluster_mgs(hipStream_t, int, int, const int2*, int, const int*, const int*,
-                                  double*);
+                                  const int*, double*);
 hipError_t bse_launch_commit(hipStream_t, int, int, int, int, int, const int*, const int*,
                              const double*, double2*, int64_t, int);
 hipError_t bse_launch_contract(hipStream_t, int, int, int, const int*, const int*, const double*,
@@ -83,21 +85,21 @@ hipError_t bse_launch_larft(hipStream_t, int, const double2*, const double2*, do
 hipError_t bse_launch_dielectric(hipStream_t, int, const double*, double, int, const double*,
                                  const double2*, double, double*, double*);
 hipError_t bse_launch_rows_to_reference(hipStream_t, int64_t, int, int, int, const double2*,
-                                        int64_t, double2*, int64_t);
+                                        int64_t, double2*, int64_t, const int*);
 hipError_t bse_launch_convert_to_fp32(hipStream_t, int64_t, const double2*, float2*);
 hipError_t bse_launch_rows_from_reference(hipStream_t, int64_t, int, int, int, const double2*,
-                                          int64_t, double2*, int64_t);
+                                          int64_t, double2*, int64_t, const int*);
 int bse_apply_pick_tk(int, int, int);
 int bse_assemble_mfma_supported(int, int, int);
 int bse_apply_mfma_groups(int, int, int);
 int64_t bse_apply_mfma_workspace(int, int, int);
-hipError_t bse_launch_apply_mfma(hipStream_t, int, int, int, int, int, int, const double2*, const double2*,
+hipError_t bse_launch_apply_mfma(hipStream_t, int, int, int, int, int, int, int, const double2*, const double2*,
                                  const double2*, const double2*, const double*, double, double, int,
                                  const double2*, int64_t, double2*, int64_t, double2*);
-hipError_t bse_launch_assemble_mfma(hipStream_t, int, int, int, int, int, int, int, int64_t,
+hipError_t bse_launch_assemble_mfma(hipStream_t, int, int, int, int, int, int, int, int, int64_t,
                                     const double2*, const double2*, const double2*, const double2*,
                                     const double*, double, double, void*);
-hipError_t bse_launch_apply(hipStream_t, int, int, int, int, int, int, int, const double2*,
+hipError_t bse_launch_apply(hipStream_t, int, int, int, int, int, int, int, int, const double2*,
                             const double2*, const double2*, const double2*, const double2*,
                             const double2*, const double2*, const double*, double, double, int,
                             const double2*, int64_t, double2*, int64_t, const double2*, int64_t,
@@ -105,7 +107,7 @@ hipError_t bse_launch_apply(hipStream_t, int, int, int, int, int, int, int, cons
 hipError_t bse_launch_small_rr(hipStream_t, int, int, const double2*, const double2*, double,
                                double*, double2*, int*);
 hipError_t bse_launch_residual(hipStream_t, int64_t, int, const double2*, const double2*, int64_t,
-                               double2*, double*);
+                               double2*, double*, double*);
 hipError_t bse_launch_coldot(hipStream_t, int64_t, int, const double2*, const double2*, int64_t,
                              double*);
 }
@@ -132,9 +134,9 @@ struct bse_ctx {
     // problem
     bool have_problem = false;
     bse_system sys{};
-    int norb = 0, nr = 0, nk = 0, ng = 0, B = 0, ns = 0;
+    int norb = 0, nr = 0, nk = 0, ng = 0, B = 0, ns = 0, n3 = 1, dim = 2;
     int64_t N = 0;
-    double recip[9] = {0}, area = 0, volume = 0, wbar = 0, qhat[3] = {1, 0, 0};
+    double recip[9] = {0}, area = 0, volume = 0, cell = 0, wbar = 0, qhat[3] = {1, 0, 0};
     std::vector<double> kpts_h, gvec_h, hr_h, rcart_h;
     // state
     bool have_tb = false, have_tables = false, built = false, have_eig = false, have_f = false;
@@ -165,6 +167,15 @@ struct bse_ctx {
     // sorts the tridiagonal blocks and the bisection runs on st
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // caller's k order (bse_set_problem_klist): list position of each grid
+    // index and its inverse; empty for the plain grid
+    std::vector<int> kmap_h, kinv_h;
+    // Gershgorin norm of the last tridiagonal (= an upper bound of ||H||)
+    double hnorm = 0.0;
+    // last refinement: residual norms ||H x_j - theta_j x_j|| (eV) of the
+    // formed vectors, rounds run, and whether the stopping rule was met
+    std::vector<double> ref_rnorm;
+    int ref_rounds = 0, ref_status = 0;
     // batched sweep state (owned by batch_api.hip)
     struct BatchState* batch = nullptr;
 };
@@ -179,7 +190,7 @@ enum Slot {
     S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_ALPHA2,
     S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL,
     S_GDHR, S_GDEPS, S_GDCOEF, S_GDP, S_GDPV, S_GDRHO, S_GDDE, S_GDW, S_GY, S_GOUT,
-    S_SGFB, S_SGFA, S_SGX, S_SGU, S_SGB, S_SGV, S_SGPART, S_APPLYP, S_NSLOTS
+    S_SGFB, S_SGFA, S_SGX, S_SGU, S_SGB, S_SGV, S_SGPART, S_APPLYP, S_MEMB, S_KMAP, S_KINV, S_TBK, S_TBH, S_TBR, S_TBE, S_TBC, S_NSLOTS
 };
 
 int fail(bse_ctx* c, int code, const std::string& msg) {
@@ -250,8 +261,11 @@ bool finite_all(const double* p, size_t n) {
 struct Geom {
     bse_system sys{};
     int norb = 0, nr = 0, nk = 0, B = 0, ns = 0, ng = 0;
+    int n3 = 1, dim = 2;
     int64_t N = 0;
-    double recip[9] = {0}, volume = 0, area = 0, wbar = 0, qhat[3] = {1, 0, 0};
+    // cell: the measure of the kernel prefactor 1/(cell Nk) -- the in-plane
+    // area for 2D potentials, the volume for 3D ones
+    double recip[9] = {0}, volume = 0, area = 0, cell = 0, wbar = 0, qhat[3] = {1, 0, 0};
     std::vector<double> kpts, gvec;
 };
 
@@ -259,7 +273,15 @@ int make_geom(const bse_tb_model* m, const bse_system* s, Geom& g, std::string& 
     if (!m || !s || !m->r_cart || !m->h_r || !m->tau) { msg = "null argument"; return BSE_ERR_ARG; }
     if (m->norb < 1 || m->norb > BSE_MAXORB) { msg = "norb must be in [1, 16]"; return BSE_ERR_UNSUPPORTED; }
     if (m->nr < 1) { msg = "nr must be >= 1"; return BSE_ERR_ARG; }
-    if (s->n1 < 1 || s->n2 < 1) { msg = "k grid must be at least 1 x 1"; return BSE_ERR_ARG; }
+    if (s->n1 < 1 || s->n2 < 1 || s->n3 < 0) { msg = "k grid must be at least 1 x 1"; return BSE_ERR_ARG; }
+    if (s->dimension != 0 && s->dimension != 2 && s->dimension != 3) {
+        msg = "dimension must be 2 or 3";
+        return BSE_ERR_ARG;
+    }
+    const int dim = s->dimension == 3 ? 3 : 2;
+    const int n3 = s->n3 > 0 ? s->n3 : 1;
+    if (dim == 2 && n3 != 1) { msg = "a 2D system has n3 = 1"; return BSE_ERR_ARG; }
+    if (!(s->radius >= 0.0)) { msg = "radius must be >= 0"; return BSE_ERR_ARG; }
     if (s->nv < 1 || s->nc < 1 || s->n_occ < s->nv || s->n_occ + s->nc > m->norb) {
         msg = "band window outside the Hamiltonian (nv <= n_occ, n_occ + nc <= norb)";
         return BSE_ERR_ARG;
@@ -275,9 +297,13 @@ int make_geom(const bse_tb_model* m, const bse_system* s, Geom& g, std::string& 
         return BSE_ERR_NUMERIC;
     }
     g.sys = *s;
+    g.sys.n3 = n3;
+    g.sys.dimension = dim;
+    g.n3 = n3;
+    g.dim = dim;
     g.norb = m->norb;
     g.nr = m->nr;
-    g.nk = s->n1 * s->n2;
+    g.nk = s->n1 * s->n2 * n3;
     g.B = s->nv * s->nc;
     g.ns = s->nv + s->nc;
     g.N = (int64_t)g.nk * g.B;
@@ -303,25 +329,46 @@ int make_geom(const bse_tb_model* m, const bse_system* s, Geom& g, std::string& 
     double axb[3];
     cross(a, a + 3, axb);
     g.area = sqrt(axb[0] * axb[0] + axb[1] * axb[1] + axb[2] * axb[2]);
+    g.cell = dim == 3 ? g.volume : g.area;
+    // k = ((i+s1)/n1) b1 + ((j+s2)/n2) b2 + ((l+s3)/n3) b3, index (i n2 + j) n3 + l
     g.kpts.assign(3 * (size_t)g.nk, 0.0);
     for (int i = 0; i < s->n1; ++i)
-        for (int j = 0; j < s->n2; ++j) {
-            const double f1 = (i + s->shift[0]) / s->n1, f2 = (j + s->shift[1]) / s->n2;
-            double* kp = &g.kpts[3 * ((size_t)i * s->n2 + j)];
-            for (int x = 0; x < 3; ++x) kp[x] = f1 * g.recip[x] + f2 * g.recip[3 + x];
-        }
-    const double nb1 = sqrt(g.recip[0] * g.recip[0] + g.recip[1] * g.recip[1] + g.recip[2] * g.recip[2]);
-    const double nb2 = sqrt(g.recip[3] * g.recip[3] + g.recip[4] * g.recip[4] + g.recip[5] * g.recip[5]);
-    const int g1 = (int)(s->g_cut / nb1), g2 = (int)(s->g_cut / nb2);
+        for (int j = 0; j < s->n2; ++j)
+            for (int l = 0; l < n3; ++l) {
+                const double f1 = (i + s->shift[0]) / s->n1, f2 = (j + s->shift[1]) / s->n2,
+                             f3 = dim == 3 ? (l + s->shift3) / n3 : 0.0;
+                double* kp = &g.kpts[3 * (((size_t)i * s->n2 + j) * n3 + l)];
+                for (int x = 0; x < 3; ++x)
+                    kp[x] = f1 * g.recip[x] + f2 * g.recip[3 + x] + f3 * g.recip[6 + x];
+            }
+    // G-list: the parallelepiped |i| <= int(G_cut / |b_i|) over b1, b2 (and b3
+    // in 3D), lexicographic, so G = 0 sits at index Ng/2 (334-351)
+    int gm[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) {
+        const double* b = g.recip + 3 * d;
+        gm[d] = (int)(s->g_cut / sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]));
+    }
     g.gvec.clear();
-    for (int i = -g1; i <= g1; ++i)
-        for (int j = -g2; j <= g2; ++j)
-            for (int x = 0; x < 3; ++x) g.gvec.push_back(i * g.recip[x] + j * g.recip[3 + x]);
+    for (int i = -gm[0]; i <= gm[0]; ++i)
+        for (int j = -gm[1]; j <= gm[1]; ++j)
+            for (int l = -gm[2]; l <= gm[2]; ++l)
+                for (int x = 0; x < 3; ++x)
+                    g.gvec.push_back(i * g.recip[x] + j * g.recip[3 + x] + l * g.recip[6 + x]);
     g.ng = (int)(g.gvec.size() / 3);
-    const double da = 4.0 * M_PI * M_PI / (g.area * g.nk);
-    const double rad = sqrt(da / M_PI);
-    g.wbar = (s->r0 > 0.0) ? (M_PI * kC / s->eps_env) * log1p(s->r0 * rad) / (s->r0 * da)
-                           : (M_PI * kC / s->eps_env) * rad / da;
+    // q = 0 term: mean of W over the mini Brillouin zone around Q = 0, a disk
+    // of area (2 pi)^2/(A Nk) (2D) or a ball of volume (2 pi)^3/(Omega Nk) (3D)
+    if (dim == 2) {
+        const double da = 4.0 * M_PI * M_PI / (g.area * g.nk);
+        const double rad = sqrt(da / M_PI);
+        g.wbar = (s->r0 > 0.0) ? (M_PI * kC / s->eps_env) * log1p(s->r0 * rad) / (s->r0 * da)
+                               : (M_PI * kC / s->eps_env) * rad / da;
+    } else {
+        const double vs = 8.0 * M_PI * M_PI * M_PI / (g.volume * g.nk);
+        const double rad = cbrt(3.0 * vs / (4.0 * M_PI));
+        const double r = s->radius;
+        const double integ = (r > 0.0) ? rad - sqrt(r) * atan(rad / sqrt(r)) : rad;
+        g.wbar = 4.0 * M_PI * kC * integ / (s->eps_env * vs);
+    }
     double qh[3] = {s->dipole_dir[0], s->dipole_dir[1], s->dipole_dir[2]};
     const double qn = sqrt(qh[0] * qh[0] + qh[1] * qh[1] + qh[2] * qh[2]);
     if (!(qn > 0.0)) { qh[0] = 1.0; qh[1] = qh[2] = 0.0; }

@@ -11,7 +11,7 @@
 //   P^H Hs P = Y diag(l) Y^H, coefficients C = P Y of the lowest k Ritz pairs.
 #include "bse_common.hpp"
 
-#define SD_MAX 32
+#define SD_MAX 40
 
 // Jacobi on the LDS matrix A (ld SD_MAX), eigenvectors into V.  n <= SD_MAX.
 // On exit the diagonal of A holds the eigenvalues (unsorted).  One thread per
@@ -220,10 +220,12 @@ extern "C" hipError_t bse_launch_small_rr(hipStream_t st, int m, int k, const do
 
 // R = Y - X diag(lam) with lam_j = Re(x_j^H y_j) / (x_j^H x_j), then scale each
 // residual column to unit norm (columns with zero residual stay zero).  One
-// workgroup per column.
+// workgroup per column.  rnorm_j = ||y_j - lam_j x_j|| / ||x_j||; rq (may be
+// null) receives lam_j.
 __global__ __launch_bounds__(256) void k_residual(int64_t N, const double2* __restrict__ X,
                                                  const double2* __restrict__ Y, int64_t ld,
-                                                 double2* __restrict__ R, double* __restrict__ rnorm)
+                                                 double2* __restrict__ R, double* __restrict__ rnorm,
+                                                 double* __restrict__ rq)
 {
     __shared__ double sh[32];
     const int j = blockIdx.x;
@@ -247,13 +249,17 @@ __global__ __launch_bounds__(256) void k_residual(int64_t N, const double2* __re
     rr = block_sum(rr, sh);
     const double inv = (rr > 0.0) ? 1.0 / sqrt(rr) : 0.0;
     for (int64_t i = threadIdx.x; i < N; i += blockDim.x) r[i] = cscale(r[i], inv);
-    if (threadIdx.x == 0) rnorm[j] = sqrt(rr / xx);
+    if (threadIdx.x == 0) {
+        rnorm[j] = sqrt(rr / xx);
+        if (rq) rq[j] = lam;
+    }
 }
 
 extern "C" hipError_t bse_launch_residual(hipStream_t st, int64_t N, int k, const double2* X,
-                                          const double2* Y, int64_t ld, double2* R, double* rnorm)
+                                          const double2* Y, int64_t ld, double2* R, double* rnorm,
+                                          double* rq)
 {
-    hipLaunchKernelGGL(k_residual, dim3(k), dim3(256), 0, st, N, X, Y, ld, R, rnorm);
+    hipLaunchKernelGGL(k_residual, dim3(k), dim3(256), 0, st, N, X, Y, ld, R, rnorm, rq);
     return hipGetLastError();
 }
 

@@ -12,38 +12,68 @@
 // tau_j)} (G0 excluded as at 2811-2816).
 #include "bse_common.hpp"
 
-// Rytova-Keldysh W(Q) = (C/2) / (eps Q (1 + r0 Q)) and its parameter derivatives.
-__device__ inline double keldysh_w(double qn, double r0, double eps, double c_half, int mode) {
+// Screened interaction W(Q) and its parameter derivatives (mode 0 value,
+// 1 d/dr0, 2 d/deps), C = e^2/eps0 in eV*Angstrom:
+//   2D: Rytova-Keldysh W(Q) = (C/2) / (eps Q (1 + r0 Q))                 [eV A^2]
+//   3D: W(Q) = C / (eps (Q^2 + radius)), the reference's regularised bare
+//       Coulomb potential (Coulomb_Potential::pull, 1281-1285) screened by a
+//       static dielectric constant as its empirical W = eps^-1 v (2698)   [eV A^3]
+__device__ inline double screened_w(int dim, double q2, double r0, double eps, double radius,
+                                    double cc, int mode) {
+    if (dim == 3) {
+        if (mode == 1) return 0.0;
+        const double w = cc / (eps * (q2 + radius));
+        return mode == 0 ? w : -w / eps;
+    }
+    const double qn = sqrt(q2), ch = 0.5 * cc;
     const double den = eps * qn * (1.0 + r0 * qn);
-    if (mode == 0) return c_half / den;
-    if (mode == 1) return -c_half / (eps * (1.0 + r0 * qn) * (1.0 + r0 * qn));  // d/dr0
-    return -c_half / (eps * den);                                             // d/deps
+    if (mode == 0) return ch / den;
+    if (mode == 1) return -ch / (eps * (1.0 + r0 * qn) * (1.0 + r0 * qn));  // d/dr0
+    return -ch / (eps * den);                                             // d/deps
 }
 
-// Table entry (d1, d2, i, j); d = index - (n - 1).
-__global__ __launch_bounds__(256) void k_w_table(
-    int n1, int n2, int norb, int ng, const double* __restrict__ gvec /* 3 x ng */,
-    double b1x, double b1y, double b1z, double b2x, double b2y, double b2z,
-    const double* __restrict__ tau /* norb x 3 */, double r0, double eps, double c_half,
-    double wbar, int mode, double2* __restrict__ wtab)
+// Bare exchange potential v(G): 2D (C/2)/|G|, 3D C/(|G|^2 + radius).
+__device__ inline double bare_v(int dim, double g2, double radius, double cc) {
+    return dim == 3 ? cc / (g2 + radius) : 0.5 * cc / sqrt(g2);
+}
+
+struct TableGeom {
+    int n1, n2, n3, dim;
+    double b[9];         // reciprocal vectors b1 | b2 | b3 (Cartesian, 1/Angstrom)
+    double r0, eps, radius, cc, wbar;
+    int mode;
+};
+
+// Table entry (grid difference dd, i, j): dd enumerates (d1, d2, d3) in the
+// mixed radix of ktable_index, d = digit - (n - 1).
+__global__ __launch_bounds__(256) void k_w_table(TableGeom tg, int norb, int ng,
+                                                 const double* __restrict__ gvec /* 3 x ng */,
+                                                 const double* __restrict__ tau /* norb x 3 */,
+                                                 double2* __restrict__ wtab)
 {
-    const int64_t total = (int64_t)(2 * n1 - 1) * (2 * n2 - 1) * norb * norb;
+    const int64_t total = ktable_size(tg.n1, tg.n2, tg.n3) * norb * norb;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
     const int j = (int)(t % norb);
     const int i = (int)((t / norb) % norb);
     const int64_t dd = t / ((int64_t)norb * norb);
-    const int d2i = (int)(dd % (2 * n2 - 1));
-    const int d1i = (int)(dd / (2 * n2 - 1));
-    const double f1 = (double)(d1i - (n1 - 1)) / n1, f2 = (double)(d2i - (n2 - 1)) / n2;
-    const double qx = f1 * b1x + f2 * b2x, qy = f1 * b1y + f2 * b2y, qz = f1 * b1z + f2 * b2z;
+    const int w2 = 2 * tg.n2 - 1, w3 = 2 * tg.n3 - 1;
+    const int d3i = (int)(dd % w3);
+    const int d2i = (int)((dd / w3) % w2);
+    const int d1i = (int)(dd / ((int64_t)w3 * w2));
+    const double f1 = (double)(d1i - (tg.n1 - 1)) / tg.n1, f2 = (double)(d2i - (tg.n2 - 1)) / tg.n2,
+                 f3 = (double)(d3i - (tg.n3 - 1)) / tg.n3;
+    const double qx = f1 * tg.b[0] + f2 * tg.b[3] + f3 * tg.b[6];
+    const double qy = f1 * tg.b[1] + f2 * tg.b[4] + f3 * tg.b[7];
+    const double qz = f1 * tg.b[2] + f2 * tg.b[5] + f3 * tg.b[8];
     const double tx = tau[3 * i] - tau[3 * j], ty = tau[3 * i + 1] - tau[3 * j + 1],
                  tz = tau[3 * i + 2] - tau[3 * j + 2];
     double2 acc = cmk(0, 0);
     for (int g = 0; g < ng; ++g) {
         const double gx = qx + gvec[3 * g], gy = qy + gvec[3 * g + 1], gz = qz + gvec[3 * g + 2];
-        const double qn = sqrt(gx * gx + gy * gy + gz * gz);
-        const double w = (qn < 1e-12) ? wbar : keldysh_w(qn, r0, eps, c_half, mode);
+        const double q2 = gx * gx + gy * gy + gz * gz;
+        const double w = (q2 < 1e-24) ? tg.wbar
+                                      : screened_w(tg.dim, q2, tg.r0, tg.eps, tg.radius, tg.cc, tg.mode);
         double s, c;
         sincos(gx * tx + gy * ty + gz * tz, &s, &c);
         acc.x += w * c;
@@ -52,9 +82,9 @@ __global__ __launch_bounds__(256) void k_w_table(
     wtab[t] = acc;
 }
 
-// V_ij (exchange), bare 2D v(G) = (C/2)/|G|, G0 excluded.
-__global__ void k_v_exchange(int norb, int ng, const double* __restrict__ gvec,
-                             const double* __restrict__ tau, double c_half,
+// V_ij (exchange), sum over G != G0 of v(G) e^{i G.(tau_i - tau_j)}.
+__global__ void k_v_exchange(int norb, int ng, int dim, const double* __restrict__ gvec,
+                             const double* __restrict__ tau, double cc, double radius,
                              double2* __restrict__ vx)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,8 +96,7 @@ __global__ void k_v_exchange(int norb, int ng, const double* __restrict__ gvec,
     for (int g = 0; g < ng; ++g) {
         if (g == ng / 2) continue;
         const double gx = gvec[3 * g], gy = gvec[3 * g + 1], gz = gvec[3 * g + 2];
-        const double gn = sqrt(gx * gx + gy * gy + gz * gz);
-        const double v = c_half / gn;
+        const double v = bare_v(dim, gx * gx + gy * gy + gz * gz, radius, cc);
         double s, c;
         sincos(gx * tx + gy * ty + gz * tz, &s, &c);
         acc.x += v * c;
@@ -123,22 +152,28 @@ __global__ __launch_bounds__(256) void k_transitions(
     dE[r] = e[(size_t)k * ns + nv + c] - e[(size_t)k * ns + v];
 }
 
-extern "C" hipError_t bse_launch_w_table(hipStream_t st, int n1, int n2, int norb, int ng,
-                                         const double* gvec, const double* b, const double* tau,
-                                         double r0, double eps, double c_half, double wbar,
-                                         int mode, double2* wtab)
+// b: 9 doubles (b1 | b2 | b3); cc = e^2/eps0; the table has
+// ktable_size(n1, n2, n3) * norb^2 entries.
+extern "C" hipError_t bse_launch_w_table(hipStream_t st, int n1, int n2, int n3, int dim, int norb,
+                                         int ng, const double* gvec, const double* b,
+                                         const double* tau, double r0, double eps, double radius,
+                                         double cc, double wbar, int mode, double2* wtab)
 {
-    const int64_t total = (int64_t)(2 * n1 - 1) * (2 * n2 - 1) * norb * norb;
-    hipLaunchKernelGGL(k_w_table, dim3((unsigned)ceil_div64(total, 256)), dim3(256), 0, st, n1,
-                       n2, norb, ng, gvec, b[0], b[1], b[2], b[3], b[4], b[5], tau, r0, eps,
-                       c_half, wbar, mode, wtab);
+    TableGeom tg;
+    tg.n1 = n1; tg.n2 = n2; tg.n3 = n3; tg.dim = dim;
+    for (int i = 0; i < 9; ++i) tg.b[i] = b[i];
+    tg.r0 = r0; tg.eps = eps; tg.radius = radius; tg.cc = cc; tg.wbar = wbar; tg.mode = mode;
+    const int64_t total = ktable_size(n1, n2, n3) * norb * norb;
+    hipLaunchKernelGGL(k_w_table, dim3((unsigned)ceil_div64(total, 256)), dim3(256), 0, st, tg, norb,
+                       ng, gvec, tau, wtab);
     return hipGetLastError();
 }
 
-extern "C" hipError_t bse_launch_v_exchange(hipStream_t st, int norb, int ng, const double* gvec,
-                                            const double* tau, double c_half, double2* vx)
+extern "C" hipError_t bse_launch_v_exchange(hipStream_t st, int norb, int ng, int dim,
+                                            const double* gvec, const double* tau, double cc,
+                                            double radius, double2* vx)
 {
-    hipLaunchKernelGGL(k_v_exchange, dim3(1), dim3(256), 0, st, norb, ng, gvec, tau, c_half, vx);
+    hipLaunchKernelGGL(k_v_exchange, dim3(1), dim3(256), 0, st, norb, ng, dim, gvec, tau, cc, radius, vx);
     return hipGetLastError();
 }
 

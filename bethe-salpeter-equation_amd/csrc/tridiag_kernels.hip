@@ -23,6 +23,7 @@
 // re-orthogonalised by modified Gram-Schmidt (twice) in a second kernel.
 #include "bse_common.hpp"
 #include <cfloat>
+#include <climits>
 
 // 1/b from the hardware reciprocal and one Newton step (|b| >= pivmin is a
 // normal number here).
@@ -131,7 +132,11 @@ __global__ __launch_bounds__(256) void k_bisect(int nitems, const int4* __restri
         int cnt = 0;
         if (!conv) cnt = sturm_count(d, e2, it.x, it.y, x, pivmin);
         const unsigned long long below = __ballot(!conv && cnt <= it.z);
-        const int nlo = __popcll((below >> base) & ((1ull << W) - 1));
+        // the split point is the leading run of lanes whose counts mark a
+        // lower bound; the reciprocal-based recurrence does not guarantee a
+        // count monotone in x, so a set bit after a clear one is not trusted
+        const unsigned long long bits = (below >> base) & ((1ull << W) - 1);
+        const int nlo = __builtin_ctzll(~bits);
         const double xlo = __shfl(x, base + max(nlo - 1, 0), 64);
         const double xhi = __shfl(x, base + min(nlo, W - 1), 64);
         if (!conv) {
@@ -318,41 +323,51 @@ __global__ __launch_bounds__(64) void k_inverse_iter(
 }
 
 // Modified Gram-Schmidt (two passes) inside clusters of close eigenvalues.
-// clusters: int2 (first batch index, size); one workgroup per cluster.  All
-// members of a cluster share the block (bs, nb).
+// clusters: int2 (offset, size); one workgroup per cluster.  With members
+// (non-null) the cluster's eigenvalues are the sorted positions
+// members[offset .. offset + size) (ascending, all in this batch); without,
+// they are the consecutive batch items offset .. offset + size - 1.  All
+// members of a cluster share the block (bs, nb): clusters are formed inside
+// one unreduced block, where eigenvalues of other blocks may lie between them
+// in the global order.
 // ncl (may be null): device count of valid clusters; blocks beyond it return.
 __global__ __launch_bounds__(256) void k_cluster_mgs(int bsz, const int2* __restrict__ clusters,
                                                      const int* __restrict__ ncl, int m0,
                                                      const int* __restrict__ bsv,
                                                      const int* __restrict__ nbv,
+                                                     const int* __restrict__ members,
                                                      double* __restrict__ x)
 {
     __shared__ double sh[32];
     if (ncl && (int)blockIdx.x >= *ncl) return;
     const int2 cl = clusters[blockIdx.x];
-    const int t0 = cl.x, csz = cl.y;
+    const int csz = cl.y;
+    auto item = [&](int i) { return members ? members[cl.x + i] - m0 : cl.x + i; };
+    const int t0 = item(0);
     const int bs = bsv[m0 + t0], nb = nbv[m0 + t0];
     const int64_t S = bsz;
     for (int j = 0; j < csz; ++j) {
+        const int tj = item(j);
         for (int pass = 0; pass < 2; ++pass) {
             for (int i = 0; i < j; ++i) {
+                const int ti = item(i);
                 double dot = 0.0;
                 for (int k = threadIdx.x; k < nb; k += blockDim.x)
-                    dot += x[(int64_t)(bs + k) * S + t0 + i] * x[(int64_t)(bs + k) * S + t0 + j];
+                    dot += x[(int64_t)(bs + k) * S + ti] * x[(int64_t)(bs + k) * S + tj];
                 dot = block_sum(dot, sh);
                 for (int k = threadIdx.x; k < nb; k += blockDim.x)
-                    x[(int64_t)(bs + k) * S + t0 + j] -= dot * x[(int64_t)(bs + k) * S + t0 + i];
+                    x[(int64_t)(bs + k) * S + tj] -= dot * x[(int64_t)(bs + k) * S + ti];
                 __syncthreads();
             }
         }
         double nrm = 0.0;
         for (int k = threadIdx.x; k < nb; k += blockDim.x) {
-            const double v = x[(int64_t)(bs + k) * S + t0 + j];
+            const double v = x[(int64_t)(bs + k) * S + tj];
             nrm += v * v;
         }
         nrm = block_sum(nrm, sh);
         const double inv = 1.0 / sqrt(nrm);
-        for (int k = threadIdx.x; k < nb; k += blockDim.x) x[(int64_t)(bs + k) * S + t0 + j] *= inv;
+        for (int k = threadIdx.x; k < nb; k += blockDim.x) x[(int64_t)(bs + k) * S + tj] *= inv;
         __syncthreads();
     }
 }
@@ -489,11 +504,12 @@ extern "C" hipError_t bse_launch_inverse_iter(hipStream_t st, int bsz, int cnt, 
 }
 
 extern "C" hipError_t bse_launch_cluster_mgs(hipStream_t st, int bsz, int ncl, const int2* clusters,
-                                             int m0, const int* bsv, const int* nbv, double* x)
+                                             int m0, const int* bsv, const int* nbv,
+                                             const int* members, double* x)
 {
     if (ncl <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_cluster_mgs, dim3(ncl), dim3(256), 0, st, bsz, clusters, (const int*)nullptr,
-                       m0, bsv, nbv, x);
+                       m0, bsv, nbv, members, x);
     return hipGetLastError();
 }
 
@@ -538,7 +554,7 @@ extern "C" int64_t bse_contract_split_workspace(int cnt) { return (int64_t)2 * C
 // as the host bookkeeping of the single-matrix path: split where
 // e_i^2 <= eps^2 |d_i d_{i+1}| + safmin, eigenvalues of every unreduced block
 // by bisection, ascending sort, clusters of eigenvalues closer than
-// 1e-7 ||T|| inside one block.  One workgroup per matrix, n <= TRI_DEV_MAXN.
+// 1e-6 ||T|| inside one block.  One workgroup per matrix, n <= TRI_DEV_MAXN.
 // ---------------------------------------------------------------------------
 #define TRI_DEV_MAXN 4096
 
@@ -623,13 +639,16 @@ __global__ __launch_bounds__(1024) void k_tri_prepare(int n, const double* __res
 }
 
 // Ascending sort of the bisection results (ties by row), block data of each
-// sorted eigenvalue, and the cluster list (first index, size) with its count.
+// sorted eigenvalue, and the cluster list (offset into members, size) with
+// its count.  members holds the sorted positions ordered by (block start,
+// position), so each block's eigenvalues are contiguous and ascending there;
+// a cluster is a run of them closer than the tolerance.
 __global__ __launch_bounds__(1024) void k_tri_sort(int n, const double* __restrict__ lami,
                                                    const int4* __restrict__ items,
                                                    const double* __restrict__ prm,
                                                    double* __restrict__ lam, int* __restrict__ bsv,
                                                    int* __restrict__ nbv, int2* __restrict__ clusters,
-                                                   int* __restrict__ ncl)
+                                                   int* __restrict__ members, int* __restrict__ ncl)
 {
     __shared__ double key[TRI_DEV_MAXN];
     __shared__ int idx[TRI_DEV_MAXN];
@@ -668,17 +687,41 @@ __global__ __launch_bounds__(1024) void k_tri_sort(int n, const double* __restri
         myb[q] = it.x;
     }
     __syncthreads();
-    for (int i = tid, q = 0; i < n; i += bd, ++q) idx[i] = myb[q];   // idx <- block start
+    // second sort: positions by (block start, position); the block start is
+    // below n <= 4096, so the composite key fits an int
     __syncthreads();
-    const double tolc = 1e-7 * prm[1];
-    for (int i = tid; i < n; i += bd) {
-        const bool start = (i == 0) || idx[i] != idx[i - 1] || key[i] - key[i - 1] > tolc;
+    for (int i = tid, q = 0; i < P; i += bd, ++q)
+        idx[i] = (i < n) ? myb[q] * TRI_DEV_MAXN + i : INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < P; i += bd) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool asc = (i & k) == 0;
+                    if ((idx[i] > idx[l]) == asc) {
+                        const int ti = idx[i]; idx[i] = idx[l]; idx[l] = ti;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int q = tid; q < n; q += bd) members[q] = idx[q] % TRI_DEV_MAXN;
+    __syncthreads();
+    const double tolc = 1e-6 * prm[1];
+    for (int q = tid; q < n; q += bd) {
+        const int pq = idx[q] % TRI_DEV_MAXN, bq = idx[q] / TRI_DEV_MAXN;
+        const bool start = (q == 0) || idx[q - 1] / TRI_DEV_MAXN != bq ||
+                           key[pq] - key[idx[q - 1] % TRI_DEV_MAXN] > tolc;
         if (!start) continue;
-        int j = i + 1;
-        while (j < n && idx[j] == idx[j - 1] && key[j] - key[j - 1] <= tolc) ++j;
-        if (j - i > 1) {
+        int j = q + 1;
+        while (j < n && idx[j] / TRI_DEV_MAXN == bq &&
+               key[idx[j] % TRI_DEV_MAXN] - key[idx[j - 1] % TRI_DEV_MAXN] <= tolc)
+            ++j;
+        if (j - q > 1) {
             const int slot = atomicAdd(&s_ncl, 1);
-            clusters[slot] = make_int2(i, j - i);
+            clusters[slot] = make_int2(q, j - q);
         }
     }
     __syncthreads();
@@ -695,11 +738,11 @@ extern "C" hipError_t bse_launch_tri_prepare(hipStream_t st, int n, const double
 
 extern "C" hipError_t bse_launch_tri_sort(hipStream_t st, int n, const double* lami,
                                           const int4* items, const double* prm, double* lam,
-                                          int* bsv, int* nbv, int2* clusters, int* ncl)
+                                          int* bsv, int* nbv, int2* clusters, int* members, int* ncl)
 {
     if (n < 1 || n > TRI_DEV_MAXN) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_tri_sort, dim3(1), dim3(1024), 0, st, n, lami, items, prm, lam, bsv, nbv,
-                       clusters, ncl);
+                       clusters, members, ncl);
     return hipGetLastError();
 }
 
@@ -710,9 +753,9 @@ extern "C" hipError_t bse_launch_tri_vectors_dev(hipStream_t st, int n, const in
                                                  const double* e, const double* e2,
                                                  const double* prm, double* lami,
                                                  const double* lam, const int* bsv, const int* nbv,
-                                                 const int2* clusters, const int* ncl,
-                                                 double* scratch /* 5 n^2 */, double* x /* n^2 */,
-                                                 int stage)
+                                                 const int2* clusters, const int* members,
+                                                 const int* ncl, double* scratch /* 5 n^2 */,
+                                                 double* x /* n^2 */, int stage)
 {
     if (stage == 0) {
         const int64_t threads = (int64_t)n * 8;
@@ -725,7 +768,7 @@ extern "C" hipError_t bse_launch_tri_vectors_dev(hipStream_t st, int n, const in
                            scratch + 3 * A, scratch + 4 * A, x);
         const int maxcl = (n + 1) / 2;
         hipLaunchKernelGGL(k_cluster_mgs, dim3((unsigned)maxcl), dim3(256), 0, st, n, clusters, ncl,
-                           0, bsv, nbv, x);
+                           0, bsv, nbv, members, x);
     }
     return hipGetLastError();
 }

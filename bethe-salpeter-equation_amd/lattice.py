@@ -57,26 +57,36 @@ class CrystalLattice:
 
 @dataclass
 class KGrid:
-    """Monkhorst-Pack style grid k = ((i + s1)/n1) b1 + ((j + s2)/n2) b2.
+    """Monkhorst-Pack style grid k = ((i + s1)/n1) b1 + ((j + s2)/n2) b2
+    (+ ((l + s3)/n3) b3 for 3D systems).
 
-    k index = i * n2 + j (i along b1 slowest).  ``shift`` is in crystal units of
-    one grid step, so (0, 0) is Gamma-centred.  Differences k_a - k_b are then
-    integer combinations (di/n1) b1 + (dj/n2) b2, which the W table exploits.
+    k index = (i * n2 + j) * n3 + l (i along b1 slowest).  ``shift`` is in
+    crystal units of one grid step, so zeros are Gamma-centred.  Differences
+    k_a - k_b are then integer combinations (di/n1) b1 + (dj/n2) b2 (+ (dl/n3)
+    b3), which the W table exploits.
     """
 
     n1: int
     n2: int
     shift: tuple = (0.0, 0.0)
+    n3: int = 1
 
     @property
     def nk(self) -> int:
-        return self.n1 * self.n2
+        return self.n1 * self.n2 * self.n3
+
+    @property
+    def shift3(self) -> float:
+        return float(self.shift[2]) if len(self.shift) > 2 else 0.0
 
     def crystal(self) -> np.ndarray:
-        i, j = np.meshgrid(np.arange(self.n1), np.arange(self.n2), indexing="ij")
+        i, j, l = np.meshgrid(np.arange(self.n1), np.arange(self.n2), np.arange(self.n3),
+                              indexing="ij")
         c = np.zeros((3, self.nk))
         c[0] = (i.ravel() + self.shift[0]) / self.n1
         c[1] = (j.ravel() + self.shift[1]) / self.n2
+        if self.n3 > 1 or self.shift3 != 0.0:
+            c[2] = (l.ravel() + self.shift3) / self.n3
         return c
 
     def cartesian(self, lattice: CrystalLattice) -> np.ndarray:
@@ -84,9 +94,67 @@ class KGrid:
         return lattice.reciprocal @ self.crystal()
 
     def index_pairs(self):
-        """(i, j) integer coordinates of each k index."""
+        """(i, j) integer coordinates of each k index (2D grids)."""
         i, j = np.meshgrid(np.arange(self.n1), np.arange(self.n2), indexing="ij")
         return i.ravel(), j.ravel()
+
+    @classmethod
+    def from_crystal(cls, kc, dimension: int = 2, tol: float = 1e-7):
+        """(grid, order) for a k-list in crystal coordinates (3 x nk) that is a
+        complete regular grid with one common offset, in any order and with
+        any reciprocal-lattice translation of each point.  order[f] is the
+        grid index of list entry f.  Raises ValueError for anything else (the
+        rule bse_set_problem_klist applies on the device side)."""
+        kc = np.asarray(kc, dtype=np.float64).reshape(3, -1)
+        nk = kc.shape[1]
+        fr = kc - np.floor(kc)
+        fr[fr > 1.0 - tol] = 0.0
+        n, shift, idx = [1, 1, 1], [0.0, 0.0, 0.0], []
+        for a in range(3):
+            if a == 2 and dimension == 2:
+                if np.any(fr[2] > tol):
+                    raise ValueError("2D k-list with a component along b3")
+                idx.append(np.zeros(nk, dtype=np.int64))
+                continue
+            u = np.unique(np.round(fr[a] / tol) * tol)
+            vals = [u[0]]
+            for v in u[1:]:
+                if v - vals[-1] > 2 * tol:
+                    vals.append(v)
+            na = len(vals)
+            if not np.allclose(np.array(vals) - vals[0], np.arange(na) / na, atol=10 * tol):
+                raise ValueError("k-list coordinates are not equally spaced along an axis")
+            t = (fr[a] - vals[0]) * na
+            j = np.rint(t).astype(np.int64)
+            if np.abs(t - j).max() > 1e-5 or j.min() < 0 or j.max() >= na:
+                raise ValueError("k-list point off the grid")
+            n[a], shift[a] = na, vals[0] * na
+            idx.append(j)
+        if n[0] * n[1] * n[2] != nk:
+            raise ValueError("k-list is not a complete grid")
+        order = (idx[0] * n[1] + idx[1]) * n[2] + idx[2]
+        if np.unique(order).size != nk:
+            raise ValueError("k-list repeats a grid point")
+        sh = (shift[0], shift[1]) if dimension == 2 else tuple(shift)
+        return cls(n[0], n[1], sh, n[2]), order
+
+
+def read_k_list(path, number_k_points=None, shift=(0.0, 0.0, 0.0)):
+    """k-list file of K_points::push_k_points_list_values (183-242): lines of
+    ``k1 k2 k3``, reading stops after ``number_k_points`` lines (all if None);
+    ``shift`` is added to every point.  Returns 3 x nk (crystal coordinates
+    when the file holds them)."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) < 3:
+                continue
+            rows.append([float(x) for x in parts[:3]])
+            if number_k_points is not None and len(rows) >= number_k_points:
+                break
+    k = np.array(rows, dtype=np.float64).T
+    return k + np.asarray(shift, dtype=np.float64)[:, None]
 
 
 def g_list(lattice: CrystalLattice, g_cut: float, dimension: int = 2) -> np.ndarray:

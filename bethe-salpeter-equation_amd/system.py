@@ -39,6 +39,8 @@ class BSESystem:
     omega_max: float = 6.0
     name: str = "system"
     meta: dict = field(default_factory=dict)
+    dimension: int = 2              # 2: Keldysh W, 2D v; 3: C/(eps (q^2+radius)), C/(q^2+radius)
+    radius: float = 0.0             # 3D Coulomb regularisation (1281-1285), 1/Angstrom^2
 
     # ---- derived sizes -------------------------------------------------
     @property
@@ -67,11 +69,16 @@ class BSESystem:
     def volume(self) -> float:
         return self.model.lattice.volume
 
+    @property
+    def cell(self) -> float:
+        """Measure of the kernel prefactor 1/(cell Nk): area (2D) or volume (3D)."""
+        return self.volume if self.dimension == 3 else self.area
+
     def kpoints(self) -> np.ndarray:
         return self.grid.cartesian(self.model.lattice)
 
     def gvectors(self) -> np.ndarray:
-        return g_list(self.model.lattice, self.g_cut, dimension=2)
+        return g_list(self.model.lattice, self.g_cut, dimension=self.dimension)
 
     def omegas(self) -> np.ndarray:
         """omega_s = s * omega_max / n_omega, endpoint excluded (3828-3829)."""
@@ -119,6 +126,27 @@ def random_multiorbital(n: int = 36, seed: int = 20736) -> BSESystem:
     4v+4c, Keldysh r0 = 15 A.  At n = 36: N = 20736.  G_cut = 4.5 /A (5 x 5)."""
     return BSESystem(tb.random_square(8, seed=seed), KGrid(n, n), n_occ=4, nv=4, nc=4,
                      r0=15.0, eps_env=1.0, g_cut=4.5, name=f"random_multiorbital_{n}x{n}")
+
+
+def cubic_3d(n: int = 4, gap: float = 4.0, t: float = -1.0, eps_env: float = 11.6) -> BSESystem:
+    """3D two-band cubic model (CsCl arrangement, a = 3 A), 1v+1c, n x n x n grid.
+
+    The direct term uses the reference's 3D potential C/(q^2 + radius) screened
+    by a static eps (the reference's empirical eps^-1 = 1/11.6, 2698), the
+    exchange its bare form; G_cut = 2.2 /A gives the 3 x 3 x 3 G-list.
+    At n = 4: N = 64.
+    """
+    return BSESystem(tb.cubic_two_band(gap=gap, t=t), KGrid(n, n, (0.0, 0.0, 0.0), n), n_occ=1,
+                     nv=1, nc=1, r0=0.0, eps_env=eps_env, g_cut=2.2, dimension=3,
+                     name=f"cubic_3d_{n}x{n}x{n}")
+
+
+def random_cubic_3d(n: int = 3, seed: int = 3, radius: float = 0.1) -> BSESystem:
+    """Random 4-orbital simple-cubic model, 2v+2c, n x n x n grid, 3D potentials
+    with radius 0.1 /A^2 and eps = 4; G_cut 2.2 /A (3 x 3 x 3).  N = 4 n^3."""
+    return BSESystem(tb.random_cubic(4, seed=seed), KGrid(n, n, (0.0, 0.0, 0.0), n), n_occ=2,
+                     nv=2, nc=2, r0=0.0, eps_env=4.0, g_cut=2.2, dimension=3, radius=radius,
+                     name=f"random_cubic_3d_{n}x{n}x{n}")
 
 
 def honeycomb_sweep(n_systems: int = 64, seed: int = 4, n_min: int = 12,

@@ -189,6 +189,49 @@ def random_square(norb: int = 8, a: float = 3.0, sigma: float = 0.5,
                    meta=dict(seed=seed, sigma=sigma, a=a))
 
 
+def cubic_two_band(gap: float = 4.0, t: float = -1.0, a: float = 3.0) -> TBModel:
+    """3D two-band model on a simple cubic lattice (CsCl arrangement).
+
+    tau_A = 0, tau_B = (a/2, a/2, a/2); on-site +gap/2 (A), -gap/2 (B); hopping
+    t from A to each of its 8 nearest B sites, i.e. H_AB(R) = t for the eight
+    R in {0, -1}^3 (lattice units).
+    """
+    lat = CrystalLattice(np.diag([a, a, a]))
+    tau = np.zeros((2, 3))
+    tau[1] = 0.5 * a
+    blocks = {(0, 0, 0): np.array([[gap / 2, t], [t, -gap / 2]], dtype=complex)}
+    for r in [(-1, 0, 0), (0, -1, 0), (0, 0, -1), (-1, -1, 0), (-1, 0, -1), (0, -1, -1),
+              (-1, -1, -1)]:
+        h = np.zeros((2, 2), dtype=complex)
+        h[0, 1] = t
+        blocks[r] = h
+    rs, h = _complete_hermitian(blocks, 2)
+    return TBModel(lat, rs, h, tau, spinful=False, name="cubic2",
+                   meta=dict(gap=gap, t=t, a=a))
+
+
+def random_cubic(norb: int = 4, a: float = 3.0, sigma: float = 0.5, seed: int = 3,
+                 onsite_range: float = 2.0) -> TBModel:
+    """Random multi-orbital simple-cubic model (seeded PCG64), the 3D analogue
+    of random_square: tau uniform in the cell, on-site U(-2, 2) eV, complex
+    Gaussian hoppings to the six nearest cells (R = a1, a2, a3 drawn, -R by
+    Hermiticity).  Draw order: tau (norb x 3), on-site (norb), then Re and Im
+    (norb x norb) for each R."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lat = CrystalLattice(np.diag([a, a, a]))
+    tau = rng.uniform(size=(norb, 3)) * a
+    onsite = rng.uniform(-onsite_range, onsite_range, size=norb)
+    blocks = {(0, 0, 0): np.diag(onsite).astype(complex)}
+    sd = sigma / np.sqrt(2.0)
+    for r in [(1, 0, 0), (0, 1, 0), (0, 0, 1)]:
+        re = rng.normal(0.0, sd, size=(norb, norb))
+        im = rng.normal(0.0, sd, size=(norb, norb))
+        blocks[r] = re + 1j * im
+    rs, h = _complete_hermitian(blocks, norb)
+    return TBModel(lat, rs, h, tau, spinful=False, name=f"random_cubic{norb}",
+                   meta=dict(seed=seed, sigma=sigma, a=a))
+
+
 # ----------------------------------------------------------------------------
 # Reference file formats (Appendix B of the blueprint; bse.version13.cpp lines)
 # ----------------------------------------------------------------------------
@@ -208,37 +251,36 @@ def read_bravais(path) -> CrystalLattice:
 def write_hr_dat(path, model: TBModel, header: str = "written by bse_amd"):
     """Wannier90 hr.dat: header, num_wann, nrpts, ndegen (15 per line), then
     ``R1 R2 R3 l m Re Im`` with l fastest inside each R (read at 531-545)."""
-    norb, nr = model.norb, model.nr
     with open(path, "w") as f:
-        f.write(header + "\n")
-        f.write(f"{norb:12d}\n{nr:12d}\n")
-        for i in range(0, nr, 15):
-            f.write("".join(f"{1:5d}" for _ in range(min(15, nr - i))) + "\n")
-        for n in range(nr):
-            r = model.r_cryst[n]
-            for m in range(norb):
-                for l in range(norb):
-                    h = model.h_r[n, l, m]
-                    f.write(f"{r[0]:5d}{r[1]:5d}{r[2]:5d}{l + 1:5d}{m + 1:5d}"
-                            f"{h.real:22.14e}{h.imag:22.14e}\n")
+        f.write(_hr_text(model, header))
 
 
-def read_hr_dat(path):
-    """Parse hr.dat; returns (r_cryst (nr,3) int, h_r (nr,norb,norb) complex).
-
-    H(R) is divided by ndegen (Wannier90's meaning; the reference multiplies,
-    676-677).  A header/size mismatch raises instead of printing and going on
-    (the reference prints ``ERROR!!!!!!!`` and continues, 499-501).
-    """
-    with open(path) as f:
-        f.readline()
-        norb = int(f.readline().split()[0])
-        nr = int(f.readline().split()[0])
-        deg = []
-        while len(deg) < nr:
-            deg.extend(int(x) for x in f.readline().split())
-        deg = np.array(deg[:nr], dtype=np.float64)
-        data = np.loadtxt(f)
+def _read_hr_block(f):
+    """One hr block from an open file (header line, num_wann, nrpts, ndegen,
+    then nrpts*num_wann^2 lines), or None at the end of the file."""
+    head = f.readline()
+    if not head:
+        return None
+    line = f.readline()
+    while line and not line.strip():
+        line = f.readline()
+    if not line:
+        return None
+    norb = int(line.split()[0])
+    nr = int(f.readline().split()[0])
+    deg = []
+    while len(deg) < nr:
+        deg.extend(int(x) for x in f.readline().split())
+    deg = np.array(deg[:nr], dtype=np.float64)
+    rows = []
+    while len(rows) < nr * norb * norb:
+        line = f.readline()
+        if not line:
+            break
+        parts = line.split()
+        if parts:
+            rows.append([float(x) for x in parts[:7]])
+    data = np.array(rows, dtype=np.float64).reshape(-1, 7)
     if data.shape[0] != nr * norb * norb:
         raise ValueError(f"hr.dat: expected {nr * norb * norb} lines, got {data.shape[0]}")
     r = data[:, 0:3].astype(np.int64).reshape(nr, norb * norb, 3)[:, 0, :]
@@ -249,6 +291,67 @@ def read_hr_dat(path):
     h[rid, l, m] = data[:, 5] + 1j * data[:, 6]
     h /= deg[:, None, None]
     return r, h
+
+
+def read_hr_dat(path):
+    """Parse hr.dat; returns (r_cryst (nr,3) int, h_r (nr,norb,norb) complex).
+
+    H(R) is divided by ndegen (Wannier90's meaning; the reference multiplies,
+    676-677).  A header/size mismatch raises instead of printing and going on
+    (the reference prints ``ERROR!!!!!!!`` and continues, 499-501).
+    """
+    with open(path) as f:
+        blk = _read_hr_block(f)
+    if blk is None:
+        raise ValueError("hr.dat: empty file")
+    return blk
+
+
+def read_hr_dat_collinear(path):
+    """Collinear-spin hr.dat: two complete blocks, spin up then spin down
+    (Hamiltonian_TB ctor, 494-552, spinorial_calculation = 1), each with the
+    same num_wann and R list.  Returns (r_cryst, h_r) of the equivalent spinor
+    model, orbital i = 2 a + sigma: H_{2a+s, 2b+s'}(R) = delta_ss' H_s(R)_ab."""
+    with open(path) as f:
+        up = _read_hr_block(f)
+        dn = _read_hr_block(f)
+    if up is None or dn is None:
+        raise ValueError("collinear hr.dat needs two blocks (up, down)")
+    (r_u, h_u), (r_d, h_d) = up, dn
+    if h_u.shape != h_d.shape or not np.array_equal(r_u, r_d):
+        raise ValueError("collinear hr.dat: the two spin blocks differ in size or R list")
+    nr, nw, _ = h_u.shape
+    h = np.zeros((nr, 2 * nw, 2 * nw), dtype=np.complex128)
+    h[:, 0::2, 0::2] = h_u
+    h[:, 1::2, 1::2] = h_d
+    return r_u, h
+
+
+def write_hr_dat_collinear(path, model: TBModel, header: str = "written by bse_amd"):
+    """Two stacked hr blocks (up, then down) of a spin-diagonal spinor model."""
+    h = model.h_r
+    if np.abs(h[:, 0::2, 1::2]).max(initial=0.0) > 0 or np.abs(h[:, 1::2, 0::2]).max(initial=0.0) > 0:
+        raise ValueError("the model mixes spins; the collinear format cannot hold it")
+    from dataclasses import replace as _replace
+    with open(path, "w") as f:
+        for sig in (0, 1):
+            blk = _replace(model, h_r=h[:, sig::2, sig::2].copy(), tau=model.tau[sig::2])
+            f.write(_hr_text(blk, header + (" up" if sig == 0 else " down")))
+
+
+def _hr_text(model: TBModel, header: str) -> str:
+    norb, nr = model.norb, model.nr
+    out = [header + "\n", f"{norb:12d}\n{nr:12d}\n"]
+    for i in range(0, nr, 15):
+        out.append("".join(f"{1:5d}" for _ in range(min(15, nr - i))) + "\n")
+    for n in range(nr):
+        r = model.r_cryst[n]
+        for m in range(norb):
+            for l in range(norb):
+                h = model.h_r[n, l, m]
+                out.append(f"{r[0]:5d}{r[1]:5d}{r[2]:5d}{l + 1:5d}{m + 1:5d}"
+                           f"{h.real:22.14e}{h.imag:22.14e}\n")
+    return "".join(out)
 
 
 def write_centres_xyz(path, model: TBModel, atoms=None):
@@ -262,12 +365,49 @@ def write_centres_xyz(path, model: TBModel, atoms=None):
             f.write(f"{sym} {p[0]:.12f} {p[1]:.12f} {p[2]:.12f}\n")
 
 
-def read_centres_xyz(path, norb: int) -> np.ndarray:
+def read_centres_xyz(path, norb: int, number_atoms: int = 0, spin_blocks: int = 1) -> np.ndarray:
+    """Wannier centres (589-601): count line, comment line, ``X x y z`` per
+    Wannier function, then number_atoms atom lines that are skipped.  With
+    spin_blocks = 2 (collinear spin) two such blocks follow each other and the
+    result is in spinor order, row 2 a + sigma."""
+    blocks = []
     with open(path) as f:
-        f.readline()
-        f.readline()
-        tau = np.array([[float(x) for x in f.readline().split()[1:4]] for _ in range(norb)])
+        for _ in range(spin_blocks):
+            f.readline()
+            f.readline()
+            blocks.append(np.array([[float(x) for x in f.readline().split()[1:4]]
+                                    for _ in range(norb)]))
+            for _ in range(number_atoms):
+                f.readline()
+    if spin_blocks == 1:
+        return blocks[0]
+    tau = np.zeros((2 * norb, 3))
+    tau[0::2] = blocks[0]
+    tau[1::2] = blocks[1]
     return tau
+
+
+def model_from_files(hr_path, centres_path, lattice: CrystalLattice, collinear: bool = False,
+                     number_atoms: int = 0, name: str = "from_files") -> TBModel:
+    """TB model from the reference's inputs: hr.dat (one block, or two for
+    collinear spin) and centres.xyz (one or two blocks), R in lattice units."""
+    if collinear:
+        r, h = read_hr_dat_collinear(hr_path)
+        tau = read_centres_xyz(centres_path, h.shape[1] // 2, number_atoms, spin_blocks=2)
+    else:
+        r, h = read_hr_dat(hr_path)
+        tau = read_centres_xyz(centres_path, h.shape[1], number_atoms)
+    return TBModel(lattice, r, h, tau, spinful=collinear, name=name)
+
+
+def write_centres_xyz_collinear(path, model: TBModel):
+    """Two centres blocks (up, then down) of a spinor model."""
+    with open(path, "w") as f:
+        for sig in (0, 1):
+            t = model.tau[sig::2]
+            f.write(f"{len(t)}\n written by bse_amd\n")
+            for x in t:
+                f.write(f"X {x[0]:.12f} {x[1]:.12f} {x[2]:.12f}\n")
 
 
 def write_k_list(path, crystal_k: np.ndarray):

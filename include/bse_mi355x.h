@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define BSE_ABI_VERSION 1
+#define BSE_ABI_VERSION 2
 
 /* status codes */
 #define BSE_OK               0
@@ -69,11 +69,13 @@ typedef struct {
     const double* tau;     /* norb x 3 row-major orbital centres, Angstrom    */
 } bse_tb_model;
 
-/* One excitonic problem (q = 0 TDA). */
+/* One excitonic problem (q = 0 TDA).  Fields after dipole_dir are new in
+ * ABI version 2; zero-initialised they give a 2D system. */
 typedef struct {
     double bravais[9];     /* column-major 3x3, column i = a_i (93-96)        */
-    int32_t n1, n2;        /* k grid, k = ((i+s1)/n1) b1 + ((j+s2)/n2) b2,
-                              k index = i*n2 + j                              */
+    int32_t n1, n2;        /* k grid, k = ((i+s1)/n1) b1 + ((j+s2)/n2) b2
+                              (+ ((l+s3)/n3) b3 in 3D), k index
+                              (i*n2 + j)*n3 + l (K_points, 183-242)           */
     double shift[2];
     int32_t n_occ;         /* occupied bands; conduction starts at n_occ       */
     int32_t nv, nc;        /* valence / conduction bands kept                 */
@@ -85,6 +87,15 @@ typedef struct {
     double exchange_scale; /* multiplies the exchange term                    */
     double direct_scale;   /* multiplies the screened direct term            */
     double dipole_dir[3];  /* direction qhat of the optical dipole (3562)     */
+    int32_t dimension;     /* 2 (or 0): Rytova-Keldysh W and bare 2D v over the
+                              in-plane G-list, kernel prefactor 1/(A Nk);
+                              3: W = C/(eps_env (q^2 + radius)), v = C/(q^2 +
+                              radius) (Coulomb_Potential::pull, 1281-1285)
+                              over the 3D G-list (334-351), prefactor
+                              1/(Omega Nk) (2846)                             */
+    int32_t n3;            /* k grid along b3 (3D only; 0 or 1 in 2D)        */
+    double shift3;         /* grid shift along b3, in grid steps             */
+    double radius;         /* 3D regularisation of C/(q^2 + radius), 1/A^2   */
 } bse_system;
 
 /* ---- context ------------------------------------------------------------- */
@@ -98,6 +109,19 @@ int  bse_abi_version(void);
  * G_points, Hamiltonian_TB, Coulomb_Potential. */
 int  bse_set_problem(bse_ctx* ctx, const bse_tb_model* model, const bse_system* sys);
 
+/* Same, with the k-points given as a list in the caller's order (crystal
+ * coordinates, 3 x nk column-major, the k-list file of
+ * K_points::push_k_points_list_values, 183-242, with its shift already
+ * added).  The list must be a complete regular grid n1 x n2 (x n3 in 3D)
+ * with one common offset, in any order and with any reciprocal-lattice
+ * translation of each point; sys->n1, n2, n3 and the shifts are ignored and
+ * taken from the list.  Anything else returns BSE_ERR_UNSUPPORTED.  The BSE
+ * index k of every reference-order input and output (r = c*nv*Nk + v*Nk + k,
+ * bse_tb_states' rows) is then the position in this list, as in the
+ * reference. */
+int  bse_set_problem_klist(bse_ctx* ctx, const bse_tb_model* model, const bse_system* sys,
+                           const double* k_cryst, int nk);
+
 /* Sizes of the loaded problem: out[0]=Nk, out[1]=N, out[2]=norb, out[3]=Ng. */
 int  bse_problem_sizes(const bse_ctx* ctx, int64_t out[4]);
 
@@ -108,6 +132,21 @@ int  bse_problem_sizes(const bse_ctx* ctx, int64_t out[4]);
  *          conduction(bottom-up, + scissor); may be NULL
  *   coef : complex Nk x (nv+nc) x norb, [k][slot][orb]; may be NULL        */
 int  bse_tb_states(bse_ctx* ctx, double* eps, double* coef);
+
+/* K1+K2 at caller-given k: Hamiltonian_TB::pull_ks_states(k) (437, 706-777)
+ * and pull_ks_states_subset(k, nv, nc) (438, 778-821) for a list of k.
+ *   model : tight-binding model, or NULL for the loaded problem's model
+ *   kpts  : 3 x nk Cartesian k (1/A), column-major
+ *   nv = nc = 0 : every band, ascending (pull_ks_states); eps nk x norb,
+ *                 coef complex nk x norb x norb ([k][band][orb])
+ *   otherwise   : the window of pull_ks_states_subset -- valence bands
+ *                 n_occ-1, n_occ-2, ... (nv of them), then conduction bands
+ *                 n_occ, n_occ+1, ... (nc) with the scissor added; eps
+ *                 nk x (nv+nc), coef complex nk x (nv+nc) x norb
+ * Each vector is unit-norm with its largest component real and positive.
+ * Used for band structures (pull_bands, 823-875) as well. */
+int  bse_tb_states_at(bse_ctx* ctx, const bse_tb_model* model, const double* kpts, int nk,
+                      int n_occ, int nv, int nc, double scissor, double* eps, double* coef);
 
 /* K3+K4: assemble the TDA matrix on the device in the given precision.
  * Replaces Excitonic_Hamiltonian::extract_hbse_and_rcv (3041-3047), i.e.
@@ -127,8 +166,18 @@ int  bse_get_dipoles(bse_ctx* ctx, double* rho);
  * at 3330).  All N eigenvalues are computed.
  *   n_vectors : eigenvectors formed for the lowest n_vectors (-1 = all)
  *   refine    : 1 = FP64 Rayleigh-Ritz refinement of the formed vectors
- *               against the FP64 matrix recomputed on the fly           */
+ *               against the FP64 matrix recomputed on the fly, iterated
+ *               until the residuals stop decreasing (1 <= n_vectors <= 16,
+ *               n_vectors <= N; BSE_ERR_NUMERIC if it does not converge) */
 int  bse_eigh(bse_ctx* ctx, int n_vectors, int refine);
+/* Outcome of the last refinement (new capability; the reference has none).
+ * rnorm: residual norms ||H x_j - theta_j x_j|| in eV of the first n formed
+ * vectors (n <= formed vectors, which include up to 4 guard vectors when a
+ * cluster straddles the requested count); rounds: Rayleigh-Ritz rounds run.
+ * Returns the refinement's status: BSE_OK when the largest residual of the
+ * requested vectors is below 1e-7 ||H||, so that every refined value lies
+ * within its residual of an eigenvalue; BSE_ERR_NUMERIC otherwise. */
+int  bse_refine_info(const bse_ctx* ctx, double* rnorm, int n, int* rounds);
 int  bse_get_eigenvalues(bse_ctx* ctx, double* evals);
 /* complex N x nvec (ld >= N), reference order, lowest nvec columns. */
 int  bse_get_eigenvectors(bse_ctx* ctx, double* z, int64_t ld, int nvec, int dst_on_device);

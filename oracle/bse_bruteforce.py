@@ -10,7 +10,8 @@ algebra (index order, conjugations, signs, the k - k' + G argument of W):
     rho_cc(c2,c1,k2,k1,G) = sum_i conj(c_{c2,k2,i}) e^{i(G+k2-k1).tau_i} c_{c1,k1,i}
     rho_vv(v2,v1,k2,k1,G) = same with valence states taken at k - q (2836)
 * diagonal model W_GG'(k2-k1) = delta_GG' W(|k2-k1+G|) (2697-2728) with the
-  mini-BZ average replacing the single Q + G = 0 term;
+  mini-BZ average replacing the single Q + G = 0 term, over the 2D or 3D
+  G-list (319-385) and k grid;
 * the direct row of 2850-2874 (temporary_vector2), kept in its textbook
   orientation (the reference stores its conjugate through .t(), 2878);
 * the exchange of 2796-2830 in its Hermitian form sum_{G != G0} v rho_r rho_s^*.
@@ -21,23 +22,22 @@ from __future__ import annotations
 
 import numpy as np
 
-from bse_oracle import (C_EV_A, Problem, kgrid, glist2d, tb_states, keldysh,
-                        keldysh_q0_average, bare2d)
+from bse_oracle import (C_EV_A, Problem, glist, tb_states, w_model, w_q0_average, v_model)
 
 
 def assemble_bruteforce(p: Problem):
-    kc = kgrid(p.bravais, p.n1, p.n2)
+    kc = p.kpoints()
     q = np.asarray(p.q, dtype=np.float64)[:, None]
     ek, vk = tb_states(p, kc)
     eq, vq = tb_states(p, kc - q)
     nk, nc, nv = p.nk, p.nc, p.nv
     cidx = [p.n_occ + c for c in range(nc)]
     vidx = [p.n_occ - 1 - v for v in range(nv)]
-    g = glist2d(p.bravais, p.g_cut)
+    g = glist(p)
     ng = g.shape[1]
     g0 = ng // 2
     tau = p.tau
-    wbar = keldysh_q0_average(p.area, nk, p.r0, p.eps_env)
+    wbar = w_q0_average(p, p.r0, p.eps_env)
 
     def rho(left, right, kl, kr, G, vecs):
         ph = np.exp(1j * (tau @ (g[:, G] + kc[:, kl] - kc[:, kr])))
@@ -46,11 +46,11 @@ def assemble_bruteforce(p: Problem):
     def wmodel(k2, k1, G):
         qv = kc[:, k2] - kc[:, k1] + g[:, G]
         qn = np.linalg.norm(qv)
-        return wbar if qn < 1e-12 else keldysh(qn, p.r0, p.eps_env)
+        return wbar if qn < 1e-12 else w_model(p, qn, p.r0, p.eps_env)
 
     N = p.n_bse
     H = np.zeros((N, N), dtype=np.complex128)
-    factor_w = -p.direct_scale / (p.area * nk)
+    factor_w = -p.direct_scale / (p.cell * nk)
 
     def ridx(c, v, k):
         return c * nv * nk + v * nk + k
@@ -79,11 +79,11 @@ def assemble_bruteforce(p: Problem):
                 for s in range(ng):
                     ph = np.exp(1j * (tau @ g[:, s]))
                     rho_cv[ridx(c, v, k), s] = np.sum(vk[k][:, cidx[c]].conj() * ph * vq[k][:, vidx[v]])
-    fx = p.exchange_scale * p.f_s / (p.area * nk)
+    fx = p.exchange_scale * p.f_s / (p.cell * nk)
     for s in range(ng):
         if s == g0:
             continue
-        vg = bare2d(np.linalg.norm(g[:, s]))
+        vg = v_model(p, np.linalg.norm(g[:, s]))
         H += fx * vg * np.outer(rho_cv[:, s], rho_cv[:, s].conj())
 
     for c in range(nc):

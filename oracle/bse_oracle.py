@@ -6,6 +6,8 @@ restates, in numpy complex128, the reference's algorithm (bse.version13.cpp)
 with point-orbital pair densities:
 
   geometry ........ Crystal_Lattice 85-124, K_points 183-242, G_points 319-385
+  potentials ...... Coulomb_Potential::pull 1273-1309 (3D C/(q^2 + radius));
+                    the 2D Rytova-Keldysh / bare 2D models replace its 2D branch
   TB states ....... FFT 647-705, pull_ks_states 706-777, _subset 778-821
                     (numpy.linalg.eigh = LAPACK zheevd stands in for
                     Armadillo eig_sym; ascending, unit columns)
@@ -51,12 +53,13 @@ def reciprocal(bravais):
     return 2.0 * np.pi / vol * b
 
 
-def kgrid(bravais, n1, n2, shift=(0.0, 0.0)):
-    """Cartesian k (3 x nk), k index = i*n2 + j."""
+def kgrid(bravais, n1, n2, shift=(0.0, 0.0), n3=1, shift3=0.0):
+    """Cartesian k (3 x nk), k = ((i+s1)/n1) b1 + ((j+s2)/n2) b2 (+ ((l+s3)/n3) b3),
+    k index = (i*n2 + j)*n3 + l (K_points 183-242 with the grid written out)."""
     b = reciprocal(bravais)
-    i, j = np.meshgrid(np.arange(n1), np.arange(n2), indexing="ij")
-    c = np.stack([(i.ravel() + shift[0]) / n1, (j.ravel() + shift[1]) / n2,
-                  np.zeros(n1 * n2)])
+    i, j, l = np.meshgrid(np.arange(n1), np.arange(n2), np.arange(n3), indexing="ij")
+    c3 = (l.ravel() + shift3) / n3 if n3 > 1 or shift3 != 0.0 else np.zeros(l.size)
+    c = np.stack([(i.ravel() + shift[0]) / n1, (j.ravel() + shift[1]) / n2, c3])
     return b @ c
 
 
@@ -69,6 +72,23 @@ def glist2d(bravais, g_cut):
         for j in range(-n[1], n[1] + 1):
             out.append(i * b[:, 0] + j * b[:, 1])
     return np.array(out).T
+
+
+def glist3d(bravais, g_cut):
+    """Parallelepiped 3D G-list G = i b1 + j b2 + l b3, |i| <= int(G_cut/|b_i|),
+    lexicographic, G = 0 at index Ng//2 (334-351)."""
+    b = reciprocal(bravais)
+    n = [int(g_cut / np.linalg.norm(b[:, d])) for d in range(3)]
+    out = []
+    for i in range(-n[0], n[0] + 1):
+        for j in range(-n[1], n[1] + 1):
+            for l in range(-n[2], n[2] + 1):
+                out.append(i * b[:, 0] + j * b[:, 1] + l * b[:, 2])
+    return np.array(out).T
+
+
+def glist(p):
+    return glist3d(p.bravais, p.g_cut) if p.dimension == 3 else glist2d(p.bravais, p.g_cut)
 
 
 # ----------------------------------------------------------------------------
@@ -96,10 +116,23 @@ class Problem:
     q: tuple = (MINVAL, 0.0, 0.0)
     eta: float = 0.05
     omegas: np.ndarray = None
+    dimension: int = 2       # 2: Keldysh W and 2D v; 3: C/(eps (q^2 + radius)), C/(q^2 + radius)
+    n3: int = 1
+    shift: tuple = (0.0, 0.0, 0.0)
+    radius: float = 0.0
 
     @property
     def nk(self):
-        return self.n1 * self.n2
+        return self.n1 * self.n2 * self.n3
+
+    @property
+    def cell(self):
+        """Measure of the kernel prefactor 1/(cell Nk): the in-plane area for the
+        2D potentials, the volume for the 3D ones (2846)."""
+        return self.volume if self.dimension == 3 else self.area
+
+    def kpoints(self):
+        return kgrid(self.bravais, self.n1, self.n2, self.shift[:2], self.n3, self.shift[2])
 
     @property
     def norb(self):
@@ -122,12 +155,16 @@ class Problem:
 def problem_from_system(s) -> Problem:
     """Read plain arrays off a host-side system description (attributes only)."""
     m = s.model
+    g = s.grid
+    sh = tuple(g.shift) + (0.0,) * (3 - len(g.shift))
     return Problem(bravais=np.array(m.lattice.bravais), r_cart=np.array(m.r_cart),
-                   h_r=np.array(m.h_r), tau=np.array(m.tau), n1=s.grid.n1, n2=s.grid.n2,
+                   h_r=np.array(m.h_r), tau=np.array(m.tau), n1=g.n1, n2=g.n2,
                    n_occ=s.n_occ, nv=s.nv, nc=s.nc, r0=s.r0, eps_env=s.eps_env,
                    g_cut=s.g_cut, scissor=s.scissor, f_s=s.spin_factor,
                    exchange_scale=s.exchange_scale, direct_scale=s.direct_scale,
-                   q=tuple(s.q_opt), eta=s.eta, omegas=s.omegas())
+                   q=tuple(s.q_opt), eta=s.eta, omegas=s.omegas(),
+                   dimension=getattr(s, "dimension", 2), n3=getattr(g, "n3", 1), shift=sh,
+                   radius=getattr(s, "radius", 0.0))
 
 
 # ----------------------------------------------------------------------------
@@ -165,7 +202,7 @@ def band_window(p: Problem, kcart=None):
     lowest conduction band (815-816).
     """
     if kcart is None:
-        kcart = kgrid(p.bravais, p.n1, p.n2)
+        kcart = p.kpoints()
     q = np.asarray(p.q, dtype=np.float64)[:, None]
     ek, vk = tb_states(p, kcart)
     eq, vq = tb_states(p, kcart - q)
@@ -200,42 +237,78 @@ def bare2d(gn):
     return 0.5 * C_EV_A / gn
 
 
-def w_table(p: Problem, r0=None, eps_env=None):
-    """W_ij(Q) on every grid difference Q = (d1/n1) b1 + (d2/n2) b2.
+def coulomb3d(q2, radius, eps_env=1.0):
+    """C / (eps (|q|^2 + radius)), the reference's 3D potential (1281-1285)
+    screened by a static dielectric constant (eV A^3)."""
+    return C_EV_A / (eps_env * (q2 + radius))
 
-    Returns (2n1-1, 2n2-1, norb, norb) indexed [d1 + n1-1, d2 + n2-1].
-    W_ij(Q) = sum_G W(|Q+G|) e^{i(Q+G).(tau_i - tau_j)}; the single term with
-    Q + G = 0 takes the mini-BZ average.
+
+def coulomb3d_q0_average(volume, nk, radius, eps_env):
+    """Mean of C/(eps (q^2 + radius)) over the ball of volume (2 pi)^3/(Omega Nk)."""
+    vs = (2.0 * np.pi) ** 3 / (volume * nk)
+    rad = (3.0 * vs / (4.0 * np.pi)) ** (1.0 / 3.0)
+    integ = rad - np.sqrt(radius) * np.arctan(rad / np.sqrt(radius)) if radius > 0 else rad
+    return 4.0 * np.pi * C_EV_A * integ / (eps_env * vs)
+
+
+def w_model(p, qn, r0, eps_env):
+    """Screened W(|q|) of the problem's dimension (q != 0)."""
+    if p.dimension == 3:
+        return coulomb3d(qn * qn, p.radius, eps_env)
+    return keldysh(qn, r0, eps_env)
+
+
+def w_q0_average(p, r0, eps_env):
+    if p.dimension == 3:
+        return coulomb3d_q0_average(p.volume, p.nk, p.radius, eps_env)
+    return keldysh_q0_average(p.area, p.nk, r0, eps_env)
+
+
+def v_model(p, gn):
+    """Bare exchange potential v(|G|), G != 0."""
+    if p.dimension == 3:
+        return coulomb3d(gn * gn, p.radius)
+    return bare2d(gn)
+
+
+def w_table(p: Problem, r0=None, eps_env=None):
+    """W_ij(Q) on every grid difference Q = (d1/n1) b1 + (d2/n2) b2 + (d3/n3) b3.
+
+    Returns (2n1-1, 2n2-1, 2n3-1, norb, norb) indexed [d1 + n1-1, d2 + n2-1,
+    d3 + n3-1] (n3 = 1 in 2D).  W_ij(Q) = sum_G W(|Q+G|) e^{i(Q+G).(tau_i -
+    tau_j)}; the single term with Q + G = 0 takes the mini-BZ average.
     """
     r0 = p.r0 if r0 is None else r0
     eps_env = p.eps_env if eps_env is None else eps_env
     b = reciprocal(p.bravais)
-    g = glist2d(p.bravais, p.g_cut)
+    g = glist(p)
     d1 = np.arange(-(p.n1 - 1), p.n1)
     d2 = np.arange(-(p.n2 - 1), p.n2)
+    d3 = np.arange(-(p.n3 - 1), p.n3)
     dt = p.tau[:, None, :] - p.tau[None, :, :]          # (norb, norb, 3)
-    wbar = keldysh_q0_average(p.area, p.nk, r0, eps_env)
-    out = np.zeros((d1.size, d2.size, p.norb, p.norb), dtype=np.complex128)
+    wbar = w_q0_average(p, r0, eps_env)
+    out = np.zeros((d1.size, d2.size, d3.size, p.norb, p.norb), dtype=np.complex128)
     for a, x in enumerate(d1):
         for c, y in enumerate(d2):
-            qv = (x / p.n1) * b[:, 0] + (y / p.n2) * b[:, 1]
-            qg = qv[:, None] + g                        # (3, ng)
-            qn = np.linalg.norm(qg, axis=0)
-            zero = qn < 1e-12
-            wq = np.where(zero, wbar, keldysh(np.where(zero, 1.0, qn), r0, eps_env))
-            ph = np.exp(1j * np.einsum("xg,ijx->gij", qg, dt))
-            out[a, c] = np.einsum("g,gij->ij", wq, ph)
+            for e, z in enumerate(d3):
+                qv = (x / p.n1) * b[:, 0] + (y / p.n2) * b[:, 1] + (z / p.n3) * b[:, 2]
+                qg = qv[:, None] + g                        # (3, ng)
+                qn = np.linalg.norm(qg, axis=0)
+                zero = qn < 1e-12
+                wq = np.where(zero, wbar, w_model(p, np.where(zero, 1.0, qn), r0, eps_env))
+                ph = np.exp(1j * np.einsum("xg,ijx->gij", qg, dt))
+                out[a, c, e] = np.einsum("g,gij->ij", wq, ph)
     return out
 
 
 def v_exchange(p: Problem):
-    """V_ij = sum_{G != 0} v(|G|) e^{iG.(tau_i - tau_j)} with bare 2D v."""
-    g = glist2d(p.bravais, p.g_cut)
+    """V_ij = sum_{G != 0} v(|G|) e^{iG.(tau_i - tau_j)} with the bare v."""
+    g = glist(p)
     g = np.delete(g, g.shape[1] // 2, axis=1)             # exclude G0 (2811-2816)
     gn = np.linalg.norm(g, axis=0)
     dt = p.tau[:, None, :] - p.tau[None, :, :]
     ph = np.exp(1j * np.einsum("xg,ijx->gij", g, dt))
-    return np.einsum("g,gij->ij", bare2d(gn), ph)
+    return np.einsum("g,gij->ij", v_model(p, gn), ph)
 
 
 # ----------------------------------------------------------------------------
@@ -243,8 +316,8 @@ def v_exchange(p: Problem):
 # ----------------------------------------------------------------------------
 
 def _kidx(p: Problem):
-    i, j = np.meshgrid(np.arange(p.n1), np.arange(p.n2), indexing="ij")
-    return i.ravel(), j.ravel()
+    i, j, l = np.meshgrid(np.arange(p.n1), np.arange(p.n2), np.arange(p.n3), indexing="ij")
+    return i.ravel(), j.ravel(), l.ravel()
 
 
 def direct_term(p: Problem, Cl, Cr, Vl, Vr, wt):
@@ -254,12 +327,12 @@ def direct_term(p: Problem, Cl, Cr, Vl, Vr, wt):
     The four coefficient arrays are separate arguments so that derivatives can
     substitute one factor at a time.  Returns (nc,nv,nk,nc,nv,nk).
     """
-    ki, kj = _kidx(p)
+    ki, kj, kl = _kidx(p)
     nk, nc, nv = p.nk, p.nc, p.nv
-    alpha = -p.direct_scale / (p.area * nk)
+    alpha = -p.direct_scale / (p.cell * nk)
     out = np.empty((nc, nv, nk, nc, nv, nk), dtype=np.complex128)
     for k in range(nk):
-        wk = wt[ki[k] - ki + p.n1 - 1, kj[k] - kj + p.n2 - 1]       # (nk', i, j)
+        wk = wt[ki[k] - ki + p.n1 - 1, kj[k] - kj + p.n2 - 1, kl[k] - kl + p.n3 - 1]  # (nk', i, j)
         X = np.einsum("ci,kdi->kcdi", Cl[k].conj(), Cr)              # (k', c, c', i)
         A = np.einsum("kcdi,kij->kcdj", X, wk)                       # (k', c, c', j)
         Y = np.einsum("vj,kwj->kvwj", Vl[k], Vr.conj())              # (k', v, v', j)
@@ -282,7 +355,7 @@ def assemble(p: Problem, bands=None, wt=None, vx=None):
     N = p.n_bse
     H = direct_term(p, C, C, V, V, wt).reshape(N, N)
     P = exchange_factors(p, C, V)
-    H += (p.exchange_scale * p.f_s / (p.area * p.nk)) * (P @ vx @ P.conj().T)
+    H += (p.exchange_scale * p.f_s / (p.cell * p.nk)) * (P @ vx @ P.conj().T)
     de = (eps_c[:, :, None] - eps_v[:, None, :])                     # (k, c, v)
     H[np.diag_indices(N)] += np.transpose(de, (1, 2, 0)).ravel()
     return H
@@ -386,7 +459,7 @@ def dH_dtheta(p: Problem, name: str):
         # exact to O(h^2) and only enters the (linear) direct term.
         dwt = (w_table(p, **kw_p) - w_table(p, **kw_m)) / (2 * h)
         return direct_term(p, C, C, V, V, dwt).reshape(N, N)
-    kc = kgrid(p.bravais, p.n1, p.n2)
+    kc = p.kpoints()
     q = np.asarray(p.q, dtype=np.float64)[:, None]
     zero = np.all(np.abs(p.r_cart) < 1e-12, axis=1)
     if name == "hopping_scale":
@@ -410,7 +483,7 @@ def dH_dtheta(p: Problem, name: str):
           + direct_term(p, C, C, dV, V, wt) + direct_term(p, C, C, V, dV, wt)).reshape(N, N)
     P = exchange_factors(p, C, V)
     dP = exchange_factors(p, dC, V) + exchange_factors(p, C, dV)
-    bx = p.exchange_scale * p.f_s / (p.area * p.nk)
+    bx = p.exchange_scale * p.f_s / (p.cell * p.nk)
     dX = bx * (dP @ vx @ P.conj().T + P @ vx @ dP.conj().T)
     dde = np.transpose(dec[:, :, None] - dev[:, None, :], (1, 2, 0)).ravel()
     out = dD + dX
@@ -489,7 +562,7 @@ def spectrum_l2_gradient(p: Problem, target, params=("hopping_scale", "scissor",
 
 def _drho(p: Problem, name):
     """Derivative of the dipole vector from the TB coefficient derivatives."""
-    kc = kgrid(p.bravais, p.n1, p.n2)
+    kc = p.kpoints()
     q = np.asarray(p.q, dtype=np.float64)[:, None]
     zero = np.all(np.abs(p.r_cart) < 1e-12, axis=1)
     if name == "hopping_scale":

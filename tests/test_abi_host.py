@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version():
     lib = _abi.load_library()
-    assert lib.bse_abi_version() == 1
+    assert lib.bse_abi_version() == 2
 
 
 def test_create_without_gpu_fails_cleanly():
@@ -54,9 +54,11 @@ def test_struct_layout_matches_c(tmp_path):
 #include <stddef.h>
 #include "{HEADER}"
 int main(void) {{
-  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(bse_tb_model), sizeof(bse_system),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bse_tb_model), sizeof(bse_system),
          offsetof(bse_system, n_occ), offsetof(bse_system, scissor),
-         offsetof(bse_system, r0), offsetof(bse_system, dipole_dir));
+         offsetof(bse_system, r0), offsetof(bse_system, dipole_dir),
+         offsetof(bse_system, dimension), offsetof(bse_system, n3),
+         offsetof(bse_system, shift3), offsetof(bse_system, radius));
   return 0;
 }}''')
     exe = tmp_path / "layout"
@@ -65,7 +67,8 @@ int main(void) {{
                                             check=True).stdout.split()]
     S = _abi.SystemC
     assert vals == [ctypes.sizeof(_abi.TBModelC), ctypes.sizeof(S), S.n_occ.offset,
-                    S.scissor.offset, S.r0.offset, S.dipole_dir.offset]
+                    S.scissor.offset, S.r0.offset, S.dipole_dir.offset, S.dimension.offset,
+                    S.n3.offset, S.shift3.offset, S.radius.offset]
 
 
 def test_hr_dat_roundtrip(tmp_path):

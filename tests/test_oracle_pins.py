@@ -240,3 +240,44 @@ def test_spectrum_gradient_matches_central_differences():
         Lm, _ = O.spectrum_l2(O.with_param(p, name, base[name] - h), target, p.omegas)
         fd = (Lp - Lm) / (2 * h)
         assert abs(g[t] - fd) <= 1e-4 * abs(fd) + 1e-12 * abs(L), (name, g[t], fd)
+
+
+# ---------------------------------------------------------------- 3D systems
+def test_3d_factorised_assembly_equals_loop_bruteforce():
+    """The 3D k grid, G-list and C/(q^2 + radius) potentials through the
+    factorised W_ij(Q) table agree with the loop-literal sums."""
+    s = bse_amd.cubic_3d(2).with_params(radius=0.05)
+    p = prob(s)
+    assert np.abs(O.assemble(p) - B.assemble_bruteforce(p)).max() < 1e-12
+
+
+def test_3d_glist_centred_and_symmetric():
+    s = bse_amd.cubic_3d(2)
+    g = O.glist(prob(s))
+    assert g.shape[1] == 27 and np.allclose(g[:, 13], 0.0) and np.allclose(g[:, ::-1], -g)
+    assert np.allclose(g, s.gvectors())
+
+
+@pytest.mark.parametrize("radius", [0.0, 0.3])
+def test_3d_mini_bz_average_by_quadrature(radius):
+    """Mean of C/(eps (q^2 + r)) over the ball of volume (2 pi)^3/(Omega Nk),
+    by radial quadrature."""
+    vol, nk, eps = 27.0, 64, 11.6
+    vs = (2 * np.pi) ** 3 / (vol * nk)
+    R = (3 * vs / (4 * np.pi)) ** (1 / 3)
+    q = np.linspace(0.0, R, 200001)
+    # q^2/(q^2 + r) with its limit at q = 0 (1 for r = 0, else 0)
+    frac = np.where(q > 0, q ** 2 / np.where(q > 0, q ** 2 + radius, 1.0), 1.0 if radius == 0 else 0.0)
+    f = 4 * np.pi * O.C_EV_A / eps * frac
+    num = np.sum(0.5 * (f[1:] + f[:-1]) * np.diff(q)) / vs
+    assert abs(O.coulomb3d_q0_average(vol, nk, radius, eps) / num - 1) < 1e-8
+
+
+def test_3d_ipa_limit_and_hermitian():
+    s = bse_amd.cubic_3d(3).with_params(direct_scale=0.0, exchange_scale=0.0)
+    p = prob(s)
+    H = O.assemble(p)
+    assert np.abs(H - H.conj().T).max() == 0.0
+    assert np.allclose(np.sort(np.linalg.eigvalsh(H)), np.sort(O.transition_energies(p)), atol=1e-13)
+    H1 = O.assemble(prob(bse_amd.cubic_3d(3)))
+    assert np.abs(H1 - H1.conj().T).max() < 1e-13
