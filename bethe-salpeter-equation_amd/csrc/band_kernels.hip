@@ -1,0 +1,760 @@
+// K5, two-stage form, first stage: dense Hermitian -> Hermitian band of
+// half-bandwidth b = 32 by blocked Householder panels (the dense half of
+// the two-stage reduction that replaces the one-stage hetrd when only a few
+// eigenvectors are wanted; the matrix is the one Armadillo's eig_sym hands to
+// LAPACK zheevd, bse.version13.cpp:3330).
+//
+// Panel p covers columns [p b, (p+1) b); with s = (p+1) b and m = n - s, its
+// part below the band, P = A[s:, p b : s] (m x b), is factored P = Q [R; 0]
+// and the trailing block A22 = A[s:, s:] becomes Q^H A22 Q.  Per panel:
+//   gather      P (FP64, dense) and the diagonal block into band storage;
+//   TSQR        leaf QRs of 64..127-row blocks in LDS, a tree of QRs of
+//               stacked R factors (8 per node), then Q_thin = Q [I; 0] formed
+//               back down the tree (Demmel, Grigori, Hoemmen & Langou 2012);
+//   reconstruct the Householder form I - V T V^H of Q from Q_thin by an LU
+//               factorisation of E - Q_thin D without pivoting, D a diagonal
+//               of unit phases chosen during the elimination so that every
+//               pivot has modulus >= 1 (Ballard, Demmel, Grigori, Jacquelin,
+//               Nguyen & Solomonik 2014): V = L, T = U L1^{-H}, and the band
+//               block R' = D^H R;
+//   X = A22 V   Hermitian block product over the tiled lower storage;
+//   Z = X T - V (T^H (V^H X T)) / 2;
+//   her2k       A22 -= V Z^H + Z V^H (the one-stage trailing-update kernels).
+// Every step is a parallel pass or a small one-workgroup kernel; there is no
+// per-column device-wide synchronisation.  Panels, reflectors and all
+// products are FP64; an FP32-stored matrix is read as FP32 and rounded once
+// per update, as in the one-stage path.
+//
+// Band storage (FP64, used by the second stage): AB[c * ldab + d] = A(c + d, c)
+// for 0 <= d <= 2b, so that the bulge chasing has room for its fill.
+#include "bse_common.hpp"
+
+#define BR_B 32            // band half-width = panel width
+#define TSQR_RB 64         // leaf block rows (the last leaf takes 64..127)
+#define TSQR_G 8           // R factors stacked per tree node
+#define TSQR_MAXROWS 256   // rows of one in-LDS QR: max(2 RB - 1, G b)
+
+// ---------------------------------------------------------------------------
+// Gather of panel p from the tiled lower storage: P (m x b, ld m, FP64) and
+// the lower triangle of the diagonal block A[c0 : c0+b, c0 : c0+b] into the
+// band store.  One thread per row of the panel (b columns each); the threads
+// of the first block also copy the diagonal block.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_band_gather(int n, int c0, int b, const T* __restrict__ A,
+                                                     double2* __restrict__ P, double2* __restrict__ AB,
+                                                     int ldab)
+{
+    const int nt = tile_count(n);
+    const int s = c0 + b, m = n - s;
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r < m) {
+        for (int j = 0; j < b; ++j) P[r + (int64_t)j * m] = cplx<T>::load(A + tix(s + r, c0 + j, nt));
+    }
+    if (blockIdx.x == 0) {
+        // diagonal block, lower triangle (rows and columns below n)
+        for (int e = threadIdx.x; e < b * b; e += 256) {
+            const int i = e % b, j = e / b;
+            const int gr = c0 + i, gc = c0 + j;
+            if (i >= j && gr < n) {
+                double2 x = cplx<T>::load(A + tix(gr, gc, nt));
+                if (i == j) x.y = 0.0;
+                AB[(int64_t)gc * ldab + (i - j)] = x;
+            }
+        }
+    }
+}
+
+// The trailing block that is already inside the band after the last panel:
+// every (r, c) with c0 <= c <= r < n goes to the band store.
+template <typename T>
+__global__ __launch_bounds__(256) void k_band_gather_tail(int n, int c0, const T* __restrict__ A,
+                                                          double2* __restrict__ AB, int ldab)
+{
+    const int nt = tile_count(n);
+    const int w = n - c0;
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < w * w; e += gridDim.x * 256) {
+        const int i = e % w, j = e / w;
+        if (i < j) continue;
+        double2 x = cplx<T>::load(A + tix(c0 + i, c0 + j, nt));
+        if (i == j) x.y = 0.0;
+        AB[(int64_t)(c0 + j) * ldab + (i - j)] = x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Householder QR of a rows x b matrix held in LDS (column-major, ld lda), in
+// place: R on and above the diagonal, the reflector vectors below it (their
+// leading 1 implicit).  H_c = I - tau_c v v^H with H_c^H x = beta e_1, beta
+// real (the LAPACK zlarfg convention); a column that is already zero below its
+// diagonal with a real diagonal gets tau = 0.  256 threads.
+// ---------------------------------------------------------------------------
+__device__ void lds_house_qr(int rows, int b, double2* A, int lda, double2* tau, double* red,
+                             double2* wbuf /* >= 8 * 32 */)
+{
+    const int tid = threadIdx.x;
+    const int kmax = min(rows, b);
+    for (int c = 0; c < kmax; ++c) {
+        double ss = 0.0;
+        for (int i = c + 1 + tid; i < rows; i += 256) ss += cabs2(A[i + c * lda]);
+        ss = block_sum(ss, red);
+        const double2 alpha = A[c + c * lda];
+        double2 tc = cmk(0, 0), scal = cmk(0, 0);
+        double beta = alpha.x;
+        if (!(ss == 0.0 && alpha.y == 0.0)) {
+            const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + ss);
+            beta = (alpha.x >= 0.0) ? -nrm : nrm;
+            tc = cmk((beta - alpha.x) / beta, -alpha.y / beta);
+            scal = cdiv(cmk(1.0, 0.0), cmk(alpha.x - beta, alpha.y));
+        }
+        __syncthreads();   // alpha read by all before the column is rewritten
+        for (int i = c + 1 + tid; i < rows; i += 256) A[i + c * lda] = cmul(A[i + c * lda], scal);
+        if (tid == 0) {
+            A[c + c * lda] = cmk(beta, 0.0);
+            tau[c] = tc;
+        }
+        __syncthreads();
+        // columns j > c:  A(c:, j) -= conj(tc) v (v^H A(c:, j)),  v(c) = 1
+        const int nc = b - c - 1;
+        const int jj = tid & 31, g = tid >> 5;
+        double2 acc = cmk(0, 0);
+        if (jj < nc && (tc.x != 0.0 || tc.y != 0.0)) {
+            const int j = c + 1 + jj;
+            for (int i = c + g; i < rows; i += 8) {
+                const double2 v = (i == c) ? cmk(1.0, 0.0) : A[i + c * lda];
+                cfmac(acc, v, A[i + j * lda]);
+            }
+        }
+        wbuf[g * 32 + jj] = acc;
+        __syncthreads();
+        if (jj < nc && (tc.x != 0.0 || tc.y != 0.0)) {
+            double2 w = wbuf[jj];
+#pragma unroll
+            for (int q = 1; q < 8; ++q) w = cadd(w, wbuf[q * 32 + jj]);
+            const double2 f = cmul(cconj(tc), w);
+            const int j = c + 1 + jj;
+            for (int i = c + g; i < rows; i += 8) {
+                const double2 v = (i == c) ? cmk(1.0, 0.0) : A[i + c * lda];
+                A[i + j * lda] = csub(A[i + j * lda], cmul(v, f));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Y <- (H_0 H_1 ... H_{k-1}) Y for the reflectors stored below the diagonal of
+// V (rows x k, ld ldv, in LDS or global), Y (rows x nc, ld ldy) in LDS.
+__device__ void lds_apply_q(int rows, int k, const double2* V, int ldv, const double2* tau,
+                            double2* Y, int ldy, int nc, double2* wbuf /* >= 8 * 32 */)
+{
+    const int tid = threadIdx.x;
+    const int jj = tid & 31, g = tid >> 5;
+    for (int c = k - 1; c >= 0; --c) {
+        const double2 tc = tau[c];
+        if (tc.x == 0.0 && tc.y == 0.0) continue;   // uniform: tau is shared
+        double2 acc = cmk(0, 0);
+        if (jj < nc) {
+            for (int i = c + g; i < rows; i += 8) {
+                const double2 v = (i == c) ? cmk(1.0, 0.0) : V[i + (int64_t)c * ldv];
+                cfmac(acc, v, Y[i + jj * ldy]);
+            }
+        }
+        wbuf[g * 32 + jj] = acc;
+        __syncthreads();
+        if (jj < nc) {
+            double2 w = wbuf[jj];
+#pragma unroll
+            for (int q = 1; q < 8; ++q) w = cadd(w, wbuf[q * 32 + jj]);
+            const double2 f = cmul(tc, w);
+            for (int i = c + g; i < rows; i += 8) {
+                const double2 v = (i == c) ? cmk(1.0, 0.0) : V[i + (int64_t)c * ldv];
+                Y[i + jj * ldy] = csub(Y[i + jj * ldy], cmul(v, f));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Leaf rows of a TSQR of an m-row panel: nblk = max(1, m / RB) blocks of RB
+// rows, the last one taking the remainder.
+__host__ __device__ inline int tsqr_nblk(int m) { return m < TSQR_RB ? 1 : m / TSQR_RB; }
+__host__ __device__ inline int tsqr_r0(int blk) { return blk * TSQR_RB; }
+__host__ __device__ inline int tsqr_r1(int blk, int nblk, int m) { return blk == nblk - 1 ? m : (blk + 1) * TSQR_RB; }
+
+// Leaf QRs: block blk of P (m x b, ld m) is factored in LDS; its reflector
+// vectors overwrite the block below the diagonal, R (b x b, zero below the
+// diagonal) goes to R + blk b b and tau to tau + blk b.
+__global__ __launch_bounds__(256) void k_tsqr_leaf(int m, int b, double2* __restrict__ P,
+                                                   double2* __restrict__ R, double2* __restrict__ tau)
+{
+    extern __shared__ double2 sA[];
+    __shared__ double red[32];
+    __shared__ double2 wbuf[8 * 32], stau[BR_B];
+    const int nblk = tsqr_nblk(m);
+    const int blk = blockIdx.x;
+    const int r0 = tsqr_r0(blk), rows = tsqr_r1(blk, nblk, m) - r0;
+    for (int e = threadIdx.x; e < rows * b; e += 256) {
+        const int i = e % rows, j = e / rows;
+        sA[i + j * rows] = P[(r0 + i) + (int64_t)j * m];
+    }
+    if (threadIdx.x < BR_B) stau[threadIdx.x] = cmk(0, 0);
+    __syncthreads();
+    lds_house_qr(rows, b, sA, rows, stau, red, wbuf);
+    for (int e = threadIdx.x; e < rows * b; e += 256) {
+        const int i = e % rows, j = e / rows;
+        if (i > j) P[(r0 + i) + (int64_t)j * m] = sA[i + j * rows];
+    }
+    for (int e = threadIdx.x; e < b * b; e += 256) {
+        const int i = e % b, j = e / b;
+        R[(int64_t)blk * b * b + e] = (i <= j && i < rows) ? sA[i + j * rows] : cmk(0, 0);
+    }
+    if (threadIdx.x < b) tau[(int64_t)blk * b + threadIdx.x] = stau[threadIdx.x];
+}
+
+// Tree node: group g stacks R factors g G .. g G + cnt - 1 (each b x b) of the
+// level below, factors the (cnt b) x b stack; its reflectors go to
+// Vn + g G b b (ld cnt b), tau to taun + g b, the new R to Rout + g b b.
+__global__ __launch_bounds__(256) void k_tsqr_node(int nin, int b, const double2* __restrict__ Rin,
+                                                   double2* __restrict__ Vn, double2* __restrict__ taun,
+                                                   double2* __restrict__ Rout)
+{
+    extern __shared__ double2 sA[];
+    __shared__ double red[32];
+    __shared__ double2 wbuf[8 * 32], stau[BR_B];
+    const int g = blockIdx.x;
+    const int cnt = min(TSQR_G, nin - g * TSQR_G);
+    const int rows = cnt * b;
+    for (int e = threadIdx.x; e < rows * b; e += 256) {
+        const int i = e % rows, j = e / rows;
+        const int t = i / b, ii = i % b;
+        sA[i + j * rows] = Rin[(int64_t)(g * TSQR_G + t) * b * b + ii + j * b];
+    }
+    if (threadIdx.x < BR_B) stau[threadIdx.x] = cmk(0, 0);
+    __syncthreads();
+    lds_house_qr(rows, b, sA, rows, stau, red, wbuf);
+    double2* V = Vn + (int64_t)g * TSQR_G * b * b;
+    for (int e = threadIdx.x; e < rows * b; e += 256) V[e] = sA[e];
+    for (int e = threadIdx.x; e < b * b; e += 256) {
+        const int i = e % b, j = e / b;
+        Rout[(int64_t)g * b * b + e] = (i <= j) ? sA[i + j * rows] : cmk(0, 0);
+    }
+    if (threadIdx.x < b) taun[(int64_t)g * b + threadIdx.x] = stau[threadIdx.x];
+}
+
+// Downward pass through a tree level: node g gets its b x k input X (the
+// identity for the root, root != 0) and writes Q_node [X; 0] as cnt pieces of
+// b x k into Xout + (g G + t) b k.
+__global__ __launch_bounds__(256) void k_tsqr_node_q(int nin, int b, int k, const double2* __restrict__ Vn,
+                                                     const double2* __restrict__ taun,
+                                                     const double2* __restrict__ Xin, int root,
+                                                     double2* __restrict__ Xout)
+{
+    extern __shared__ double2 sY[];
+    __shared__ double2 wbuf[8 * 32], stau[BR_B];
+    const int g = blockIdx.x;
+    const int cnt = min(TSQR_G, nin - g * TSQR_G);
+    const int rows = cnt * b;
+    for (int e = threadIdx.x; e < rows * k; e += 256) {
+        const int i = e % rows, j = e / rows;
+        double2 x = cmk(0, 0);
+        if (i < b) x = root ? cmk(i == j ? 1.0 : 0.0, 0.0) : Xin[(int64_t)g * b * k + i + j * b];
+        sY[i + j * rows] = x;
+    }
+    if (threadIdx.x < b) stau[threadIdx.x] = taun[(int64_t)g * b + threadIdx.x];
+    __syncthreads();
+    lds_apply_q(rows, b, Vn + (int64_t)g * TSQR_G * b * b, rows, stau, sY, rows, k, wbuf);
+    for (int e = threadIdx.x; e < rows * k; e += 256) {
+        const int i = e % rows, j = e / rows;
+        const int t = i / b, ii = i % b;
+        Xout[(int64_t)(g * TSQR_G + t) * b * k + ii + j * b] = sY[i + j * rows];
+    }
+}
+
+// Leaves of the downward pass: block blk forms Q_blk [X_blk; 0] (rows x k)
+// into Q (m x k, ld m).  root != 0: a single leaf, X = I.
+__global__ __launch_bounds__(256) void k_tsqr_leaf_q(int m, int b, int k, const double2* __restrict__ P,
+                                                     const double2* __restrict__ tau,
+                                                     const double2* __restrict__ Xin, int root,
+                                                     double2* __restrict__ Q)
+{
+    extern __shared__ double2 sY[];
+    __shared__ double2 wbuf[8 * 32], stau[BR_B];
+    const int nblk = tsqr_nblk(m);
+    const int blk = blockIdx.x;
+    const int r0 = tsqr_r0(blk), rows = tsqr_r1(blk, nblk, m) - r0;
+    const int kv = min(rows, b);
+    for (int e = threadIdx.x; e < rows * k; e += 256) {
+        const int i = e % rows, j = e / rows;
+        double2 x = cmk(0, 0);
+        if (i < b) x = root ? cmk(i == j ? 1.0 : 0.0, 0.0) : Xin[(int64_t)blk * b * k + i + j * b];
+        sY[i + j * rows] = x;
+    }
+    if (threadIdx.x < b) stau[threadIdx.x] = tau[(int64_t)blk * b + threadIdx.x];
+    __syncthreads();
+    lds_apply_q(rows, kv, P + r0, m, stau, sY, rows, k, wbuf);
+    for (int e = threadIdx.x; e < rows * k; e += 256) {
+        const int i = e % rows, j = e / rows;
+        Q[(r0 + i) + (int64_t)j * m] = sY[i + j * rows];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Householder form of Q_thin (m x k, k = min(m, b)).  One workgroup:
+//   top block: LU of E - Q D with d_j = -conj(w_j)/|w_j| from the running
+//   Schur entry w_j, so u_jj = 1 + |w_j| >= 1; U(j, c) = -d_c W(j, c);
+//   L1 (unit lower) -> V rows 0..k-1;  T = U L1^{-H} -> Tp (ld b);
+//   Mlow = -D U^{-1} (for the rows below: L2 = Q2 Mlow);
+//   R' = D^H Rfin (k x b) -> the band store, rows s.. of columns c0..
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_hh_reconstruct(int m, int b, int k, const double2* __restrict__ Q,
+                                                        const double2* __restrict__ Rfin,
+                                                        double2* __restrict__ V, double2* __restrict__ Tp,
+                                                        double2* __restrict__ Mlow, double2* __restrict__ AB,
+                                                        int ldab, int c0)
+{
+    __shared__ double2 W[BR_B][BR_B + 1];     // W[i][c]: running Schur values of the Q part
+    __shared__ double2 L[BR_B][BR_B + 1];
+    __shared__ double2 U[BR_B][BR_B + 1];
+    __shared__ double2 Ui[BR_B][BR_B + 1];    // U^{-1}
+    __shared__ double2 Li[BR_B][BR_B + 1];    // L1^{-1}
+    __shared__ double2 d[BR_B];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, c = e / BR_B;
+        W[i][c] = (i < k && c < k) ? Q[i + (int64_t)c * m] : cmk(0, 0);
+        L[i][c] = cmk(i == c ? 1.0 : 0.0, 0.0);
+        U[i][c] = cmk(0, 0);
+        Ui[i][c] = cmk(0, 0);
+        Li[i][c] = cmk(0, 0);
+    }
+    __syncthreads();
+    for (int j = 0; j < k; ++j) {
+        const double2 wj = W[j][j];
+        const double aw = sqrt(cabs2(wj));
+        const double2 dj = (aw > 0.0) ? cmk(-wj.x / aw, wj.y / aw) : cmk(1.0, 0.0);
+        const double2 ujj = csub(cmk(1.0, 0.0), cmul(dj, wj));
+        if (tid == 0) {
+            d[j] = dj;
+            U[j][j] = ujj;
+        }
+        // multipliers of column j, then the Schur update of the later columns
+        if (tid > j && tid < k) L[tid][j] = cdiv(cmul(cmk(-dj.x, -dj.y), W[tid][j]), ujj);
+        __syncthreads();
+        for (int e = tid; e < BR_B * BR_B; e += 256) {
+            const int i = e % BR_B, c = e / BR_B;
+            if (i > j && i < k && c > j && c < k) W[i][c] = csub(W[i][c], cmul(L[i][j], W[j][c]));
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < BR_B * BR_B; e += 256) {
+        const int j = e % BR_B, c = e / BR_B;
+        if (j < c && c < k) U[j][c] = cmul(cmk(-d[c].x, -d[c].y), W[j][c]);
+    }
+    __syncthreads();
+    // U^{-1} (upper) and L1^{-1} (unit lower), one column per thread
+    if (tid < k) {
+        const int c = tid;
+        for (int i = c; i >= 0; --i) {
+            double2 s = cmk(i == c ? 1.0 : 0.0, 0.0);
+            for (int l = i + 1; l <= c; ++l) s = csub(s, cmul(U[i][l], Ui[l][c]));
+            Ui[i][c] = cdiv(s, U[i][i]);
+        }
+    } else if (tid >= 64 && tid < 64 + k) {
+        const int c = tid - 64;
+        for (int i = c; i < k; ++i) {
+            double2 s = cmk(i == c ? 1.0 : 0.0, 0.0);
+            for (int l = c; l < i; ++l) s = csub(s, cmul(L[i][l], Li[l][c]));
+            Li[i][c] = s;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, c = e / BR_B;
+        if (i < k && c < k) {
+            // V rows 0..k-1 = L1
+            V[i + (int64_t)c * m] = L[i][c];
+            // T = U L1^{-H}:  T(i, c) = sum_l U(i, l) conj(Li(c, l))
+            double2 t = cmk(0, 0);
+            for (int l = 0; l < k; ++l) cfma(t, U[i][l], cconj(Li[c][l]));
+            Tp[i + c * b] = t;
+            // Mlow = -D U^{-1}: row scaling of U^{-1}... (-Q2 D) U^{-1} = Q2 (-D U^{-1})
+            Mlow[i + c * b] = cmul(cmk(-d[i].x, -d[i].y), Ui[i][c]);
+        } else if (i < b && c < b) {
+            Tp[i + c * b] = cmk(0, 0);
+            Mlow[i + c * b] = cmk(0, 0);
+        }
+    }
+    // R' = D^H Rfin (k x b): rows s .. s + k - 1 of columns c0 .. c0 + b - 1
+    for (int e = tid; e < k * b; e += 256) {
+        const int i = e % k, j = e / k;
+        const double2 r = cmul(cconj(d[i]), Rfin[i + j * b]);
+        const int gr = c0 + b + i, gc = c0 + j;
+        AB[(int64_t)gc * ldab + (gr - gc)] = r;
+    }
+}
+
+// V rows k..m-1:  L2 = Q2 Mlow (Mlow k x k, ld b).  One thread per row.
+__global__ __launch_bounds__(256) void k_hh_lower(int m, int b, int k, const double2* __restrict__ Q,
+                                                  const double2* __restrict__ Mlow, double2* __restrict__ V)
+{
+    __shared__ double2 sM[BR_B * BR_B];
+    for (int e = threadIdx.x; e < b * b; e += 256) sM[e] = Mlow[e];
+    __syncthreads();
+    const int r = k + blockIdx.x * 256 + threadIdx.x;
+    if (r >= m) return;
+    double2 q[BR_B];
+#pragma unroll
+    for (int l = 0; l < BR_B; ++l) q[l] = (l < k) ? Q[r + (int64_t)l * m] : cmk(0, 0);
+    for (int c = 0; c < k; ++c) {
+        double2 s = cmk(0, 0);
+#pragma unroll
+        for (int l = 0; l < BR_B; ++l)
+            if (l < k) cfma(s, q[l], sM[l + c * b]);
+        V[r + (int64_t)c * m] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// X = A22 V for the trailing block A22 = A[s:, s:] over its storage tiles
+// (tile rows and columns tcs = s / 64 ..).  Workgroup (i, q) owns tile row
+// I = tcs + i and the q-th of KSPLIT slices of the nT tiles that row i of the
+// full Hermitian block touches: tiles (I, J) for J <= I (stored) and the
+// conjugate transposes of the stored tiles (J, I) for J > I.  Its 64 x k
+// partial goes to Xp + q (nT 64) k.  V is read as zero above row s, so the
+// first tile row and column need no masks; rows of X above s are not used.
+// Diagonal tiles contribute (L + L^H - diag): the stored strict upper half of
+// a diagonal tile is zero (storage invariant).
+// 16 x 16 threads: rows tx + 16 a (a < 4), columns ty + 16 c (c < 2).
+// ---------------------------------------------------------------------------
+#define HEMM_KSPLIT 4
+template <typename T>
+__global__ __launch_bounds__(256) void k_band_hemm(int n, int s, int k, const T* __restrict__ A,
+                                                   const double2* __restrict__ V, int64_t ldv,
+                                                   double2* __restrict__ Xp)
+{
+    __shared__ double2 sA[8][65];     // sA[kk][row]: 8 contraction indices x 64 rows
+    __shared__ double2 sV[8][BR_B];   // sV[kk][col]
+    const int nt = tile_count(n), tcs = s >> 6, nT = nt - tcs;
+    const int i = blockIdx.x, q = blockIdx.y;
+    const int I = tcs + i;
+    const int j0 = (int)((int64_t)nT * q / HEMM_KSPLIT), j1 = (int)((int64_t)nT * (q + 1) / HEMM_KSPLIT);
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double2 acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) { acc[a][0] = cmk(0, 0); acc[a][1] = cmk(0, 0); }
+    for (int jt = j0; jt < j1; ++jt) {
+        const int J = tcs + jt;
+        const bool tr = J > I;                       // use conj(A(J, I))^T
+        const T* tile = A + tile_pos(tr ? J : I, tr ? I : J, nt) * (BSE_TILE * BSE_TILE);
+        const bool diag = (J == I);
+        for (int kc = 0; kc < 64; kc += 8) {
+            // stage 8 contraction columns: A_IJ(row, kc + kk)
+            for (int e = threadIdx.x; e < 8 * 64; e += 256) {
+                int kk, row;
+                double2 x;
+                if (!tr) {
+                    row = e & 63; kk = e >> 6;
+                    x = cplx<T>::load(tile + row + (kc + kk) * 64);
+                    if (diag) {
+                        // (L + L^H - diag): above the diagonal take conj(L(col, row))
+                        if (row < kc + kk) x = cconj(cplx<T>::load(tile + (kc + kk) + row * 64));
+                        else if (row == kc + kk) x.y = 0.0;
+                    }
+                } else {
+                    // A_IJ(row, col) = conj(A_JI(col, row)): read 8 consecutive col entries
+                    kk = e & 7; row = e >> 3;
+                    x = cconj(cplx<T>::load(tile + (kc + kk) + row * 64));
+                }
+                sA[kk][row] = x;
+            }
+            for (int e = threadIdx.x; e < 8 * BR_B; e += 256) {
+                const int kk = e / BR_B, c = e % BR_B;
+                const int64_t gr = (int64_t)J * 64 + kc + kk;   // global row of V
+                sV[kk][c] = (c < k && gr >= s && gr < n) ? V[(gr - s) + (int64_t)c * ldv] : cmk(0, 0);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+                const double2 v0 = sV[kk][ty], v1 = sV[kk][ty + 16];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const double2 x = sA[kk][tx + 16 * a];
+                    cfma(acc[a][0], x, v0);
+                    cfma(acc[a][1], x, v1);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    double2* out = Xp + (int64_t)q * nT * 64 * BR_B + (int64_t)i * 64;
+    const int64_t ldx = (int64_t)nT * 64;
+    (void)ldx;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        out[(tx + 16 * a) + (int64_t)ty * nT * 64] = acc[a][0];
+        out[(tx + 16 * a) + (int64_t)(ty + 16) * nT * 64] = acc[a][1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Z = X T - V M with M = (1/2) T^H (V^H X T).  Rows of X are tile-aligned in
+// Xp (row t of the trailing tile range <-> global row tcs 64 + t); V rows
+// start at s.
+//   k_band_xt:   XT = (sum of the KSPLIT partials) T for 64 rows per block,
+//                into Z (m x b, ld m), and the partial V^H XT of the block
+//   k_band_m:    M from the partials (one workgroup, fixed order)
+//   k_band_z:    Z -= V M
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_band_xt(int n, int s, int k, int b, const double2* __restrict__ Xp,
+                                                 const double2* __restrict__ Tp, const double2* __restrict__ V,
+                                                 double2* __restrict__ Z, double2* __restrict__ Gp)
+{
+    __shared__ double2 sT[BR_B][BR_B + 1];
+    __shared__ double2 sX[64][BR_B + 1];
+    __shared__ double2 sV[64][BR_B + 1];
+    const int m = n - s, tcs = s >> 6, nT = tile_count(n) - tcs;
+    const int r0 = blockIdx.x * 64;                 // local row of Z (global s + r0)
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, j = e / BR_B;
+        sT[i][j] = (i < k && j < k) ? Tp[i + j * b] : cmk(0, 0);
+    }
+    for (int e = threadIdx.x; e < 64 * BR_B; e += 256) {
+        const int rr = e & 63, c = e >> 6;
+        const int r = r0 + rr;
+        double2 x = cmk(0, 0), v = cmk(0, 0);
+        if (r < m && c < k) {
+            const int64_t t = (int64_t)(s + r) - (int64_t)tcs * 64;
+            for (int qq = 0; qq < HEMM_KSPLIT; ++qq)
+                x = cadd(x, Xp[(int64_t)qq * nT * 64 * BR_B + t + (int64_t)c * nT * 64]);
+            v = V[r + (int64_t)c * m];
+        }
+        sX[rr][c] = x;
+        sV[rr][c] = v;
+    }
+    __syncthreads();
+    // XT rows: every thread forms its 8 entries from the X rows, then they
+    // replace X in LDS once all reads are done
+    constexpr int PER = 64 * BR_B / 256;
+    double2 xt[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + 256 * u, rr = e & 63, c = e >> 6;
+        const int r = r0 + rr;
+        double2 acc = cmk(0, 0);
+        for (int l = 0; l < k; ++l) cfma(acc, sX[rr][l], sT[l][c]);
+        if (r < m && c < k) Z[r + (int64_t)c * m] = acc;
+        xt[u] = (r < m && c < k) ? acc : cmk(0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + 256 * u;
+        sX[e & 63][e >> 6] = xt[u];
+    }
+    __syncthreads();
+    // partial G = V^H XT over the block's rows (k x k)
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, j = e / BR_B;
+        double2 acc = cmk(0, 0);
+        for (int rr = 0; rr < 64; ++rr) cfmac(acc, sV[rr][i], sX[rr][j]);
+        Gp[(int64_t)blockIdx.x * BR_B * BR_B + e] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_band_m(int nblk, int k, int b, const double2* __restrict__ Gp,
+                                                const double2* __restrict__ Tp, double2* __restrict__ M)
+{
+    __shared__ double2 sG[BR_B][BR_B + 1];
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        double2 acc = cmk(0, 0);
+        for (int g = 0; g < nblk; ++g) acc = cadd(acc, Gp[(int64_t)g * BR_B * BR_B + e]);
+        sG[e % BR_B][e / BR_B] = acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, j = e / BR_B;
+        double2 acc = cmk(0, 0);
+        if (i < k && j < k)
+            for (int l = 0; l < k; ++l) cfmac(acc, Tp[l + i * b], sG[l][j]);   // (T^H G)(i, j)
+        M[i + j * b] = cscale(acc, 0.5);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_band_z(int m, int k, int b, const double2* __restrict__ V,
+                                                const double2* __restrict__ M, double2* __restrict__ Z)
+{
+    __shared__ double2 sM[BR_B * BR_B];
+    for (int e = threadIdx.x; e < b * b; e += 256) sM[e] = M[e];
+    __syncthreads();
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= m) return;
+    double2 v[BR_B];
+#pragma unroll
+    for (int l = 0; l < BR_B; ++l) v[l] = (l < k) ? V[r + (int64_t)l * m] : cmk(0, 0);
+    for (int c = 0; c < k; ++c) {
+        double2 s = Z[r + (int64_t)c * m];
+#pragma unroll
+        for (int l = 0; l < BR_B; ++l)
+            if (l < k) s = csub(s, cmul(v[l], sM[l + c * b]));
+        Z[r + (int64_t)c * m] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+extern "C" hipError_t bse_band_gather(hipStream_t st, int fp32, int n, int c0, int b, const void* A,
+                                      double2* P, double2* AB, int ldab)
+{
+    const int m = n - c0 - b;
+    const unsigned g = (unsigned)std::max<int64_t>(1, ceil_div64(std::max(m, 1), 256));
+    if (fp32)
+        hipLaunchKernelGGL(k_band_gather<float2>, dim3(g), dim3(256), 0, st, n, c0, b, (const float2*)A, P, AB, ldab);
+    else
+        hipLaunchKernelGGL(k_band_gather<double2>, dim3(g), dim3(256), 0, st, n, c0, b, (const double2*)A, P, AB, ldab);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t bse_band_gather_tail(hipStream_t st, int fp32, int n, int c0, const void* A,
+                                           double2* AB, int ldab)
+{
+    if (c0 >= n) return hipSuccess;
+    const int w = n - c0;
+    const unsigned g = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div64((int64_t)w * w, 256)));
+    if (fp32)
+        hipLaunchKernelGGL(k_band_gather_tail<float2>, dim3(g), dim3(256), 0, st, n, c0, (const float2*)A, AB, ldab);
+    else
+        hipLaunchKernelGGL(k_band_gather_tail<double2>, dim3(g), dim3(256), 0, st, n, c0, (const double2*)A, AB, ldab);
+    return hipGetLastError();
+}
+
+// Workspace of one panel QR of an m-row panel (complex elements): leaf R and
+// tau, tree levels (V, tau, R per node; X pieces), the thin Q.
+extern "C" int64_t bse_tsqr_workspace(int m, int b) {
+    int64_t el = 0;
+    int nin = tsqr_nblk(m);
+    el += (int64_t)nin * b * b + (int64_t)nin * b;       // leaf R, tau
+    el += 2 * (int64_t)nin * b * b;                       // X pieces (two buffers)
+    while (nin > 1) {
+        const int nn = (int)ceil_div64(nin, TSQR_G);
+        el += (int64_t)nn * TSQR_G * b * b + (int64_t)nn * b + (int64_t)nn * b * b;
+        nin = nn;
+    }
+    el += (int64_t)m * b;                                 // Q_thin
+    return el + 64;
+}
+
+// QR of the m x b panel P (ld m) as I - V T V^H: V (m x k) into Vout (ld m),
+// T into Tp (ld b), R' = D^H R into the band store at (c0 + b.., c0..).
+// Returns k = min(m, b) in *kout.
+extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P, double2* Vout,
+                                        double2* Tp, double2* ws, double2* AB, int ldab, int c0, int* kout)
+{
+    if (b > BR_B || m < 1) return hipErrorInvalidValue;
+    // the tree nodes hold up to 8 x 32 x 32 complex doubles (128 KB) of LDS
+    static const hipError_t attr = [] {
+        const int big = 160 * 1024;
+        hipError_t e = hipFuncSetAttribute((const void*)k_tsqr_leaf, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_tsqr_node, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_tsqr_node_q, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_tsqr_leaf_q, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+        return e;
+    }();
+    if (attr != hipSuccess) return attr;
+    const int k = std::min(m, b);
+    *kout = k;
+    const int nleaf = tsqr_nblk(m);
+    // workspace layout (mirrors bse_tsqr_workspace)
+    double2* p = ws;
+    double2* Rleaf = p; p += (int64_t)nleaf * b * b;
+    double2* tauleaf = p; p += (int64_t)nleaf * b;
+    double2* Xa = p; p += (int64_t)nleaf * b * b;
+    double2* Xb = p; p += (int64_t)nleaf * b * b;
+    struct Lvl { int nin, nn; double2 *V, *tau, *R; };
+    Lvl lv[16];
+    int nl = 0, nin = nleaf;
+    const double2* Rin = Rleaf;
+    while (nin > 1) {
+        Lvl L;
+        L.nin = nin;
+        L.nn = (int)ceil_div64(nin, TSQR_G);
+        L.V = p; p += (int64_t)L.nn * TSQR_G * b * b;
+        L.tau = p; p += (int64_t)L.nn * b;
+        L.R = p; p += (int64_t)L.nn * b * b;
+        lv[nl++] = L;
+        nin = L.nn;
+    }
+    double2* Q = p;
+    // leaf QRs
+    const int maxrows_leaf = (nleaf == 1) ? m : 2 * TSQR_RB - 1;
+    if (maxrows_leaf > TSQR_MAXROWS) return hipErrorInvalidValue;
+    const size_t lds_leaf = (size_t)maxrows_leaf * b * sizeof(double2);
+    hipLaunchKernelGGL(k_tsqr_leaf, dim3((unsigned)nleaf), dim3(256), lds_leaf, st, m, b, P, Rleaf, tauleaf);
+    // up the tree
+    for (int l = 0; l < nl; ++l) {
+        const size_t lds = (size_t)TSQR_G * b * b * sizeof(double2);
+        hipLaunchKernelGGL(k_tsqr_node, dim3((unsigned)lv[l].nn), dim3(256), lds, st, lv[l].nin, b,
+                           l == 0 ? Rleaf : lv[l - 1].R, lv[l].V, lv[l].tau, lv[l].R);
+    }
+    const double2* Rfin = nl ? lv[nl - 1].R : Rleaf;
+    // down the tree
+    double2* Xcur = Xa;
+    double2* Xnext = Xb;
+    for (int l = nl - 1; l >= 0; --l) {
+        const size_t lds = (size_t)TSQR_G * b * k * sizeof(double2);
+        hipLaunchKernelGGL(k_tsqr_node_q, dim3((unsigned)lv[l].nn), dim3(256), lds, st, lv[l].nin, b, k,
+                           lv[l].V, lv[l].tau, (const double2*)Xcur, l == nl - 1 ? 1 : 0, Xnext);
+        std::swap(Xcur, Xnext);
+    }
+    hipLaunchKernelGGL(k_tsqr_leaf_q, dim3((unsigned)nleaf), dim3(256), (size_t)maxrows_leaf * k * sizeof(double2),
+                       st, m, b, k, (const double2*)P, (const double2*)tauleaf, (const double2*)Xcur,
+                       nl == 0 ? 1 : 0, Q);
+    // Householder form
+    double2* Mlow = Xnext;   // b x b scratch
+    hipLaunchKernelGGL(k_hh_reconstruct, dim3(1), dim3(256), 0, st, m, b, k, (const double2*)Q, Rfin, Vout, Tp,
+                       Mlow, AB, ldab, c0);
+    if (m > k)
+        hipLaunchKernelGGL(k_hh_lower, dim3((unsigned)ceil_div64(m - k, 256)), dim3(256), 0, st, m, b, k,
+                           (const double2*)Q, (const double2*)Mlow, Vout);
+    return hipGetLastError();
+}
+
+// Workspace of the two-sided update (complex elements) for trailing order m.
+extern "C" int64_t bse_band_update_workspace(int n, int s, int b) {
+    const int nT = tile_count(n) - (s >> 6);
+    const int m = n - s;
+    return (int64_t)HEMM_KSPLIT * nT * 64 * BR_B + (int64_t)ceil_div64(m, 64) * BR_B * BR_B + 2 * BR_B * BR_B
+           + (int64_t)m * b + 64;
+}
+
+// A22 <- Q^H A22 Q for Q = I - V T V^H (V: m x k, ld m, rows from s).
+// her2k: the one-stage trailing-update launcher (kind as tuned).
+extern "C" hipError_t bse_launch_her2k(hipStream_t, int, int, int, int, void*, const double2*,
+                                       const double2*, int64_t, int, int, const double2*);
+extern "C" hipError_t bse_band_update(hipStream_t st, int fp32, int kind, int n, int s, int k, int b, void* A,
+                                      const double2* V, const double2* Tp, double2* ws, const double2* zeros64)
+{
+    const int m = n - s;
+    if (m <= 0 || k <= 0) return hipSuccess;
+    const int tcs = s >> 6, nT = tile_count(n) - tcs;
+    double2* Xp = ws;
+    double2* Gp = Xp + (int64_t)HEMM_KSPLIT * nT * 64 * BR_B;
+    const int nblk = (int)ceil_div64(m, 64);
+    double2* M = Gp + (int64_t)nblk * BR_B * BR_B;
+    double2* Z = M + 2 * BR_B * BR_B;
+    const dim3 gh((unsigned)nT, HEMM_KSPLIT);
+    if (fp32)
+        hipLaunchKernelGGL(k_band_hemm<float2>, gh, dim3(256), 0, st, n, s, k, (const float2*)A, V, (int64_t)m, Xp);
+    else
+        hipLaunchKernelGGL(k_band_hemm<double2>, gh, dim3(256), 0, st, n, s, k, (const double2*)A, V, (int64_t)m, Xp);
+    hipLaunchKernelGGL(k_band_xt, dim3((unsigned)nblk), dim3(256), 0, st, n, s, k, b, (const double2*)Xp, Tp, V, Z, Gp);
+    hipLaunchKernelGGL(k_band_m, dim3(1), dim3(256), 0, st, nblk, k, b, (const double2*)Gp, Tp, M);
+    hipLaunchKernelGGL(k_band_z, dim3((unsigned)ceil_div64(m, 256)), dim3(256), 0, st, m, k, b, V,
+                       (const double2*)M, Z);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return bse_launch_her2k(st, fp32, kind, n, s, A, V, Z, m, s, k, zeros64);
+}

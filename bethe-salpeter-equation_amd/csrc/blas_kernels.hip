@@ -536,6 +536,24 @@ extern "C" hipError_t bse_apply_q_thin(hipStream_t st, int n, int nb, const doub
     return hipGetLastError();
 }
 
+// One panel of any reflector layout: X <- (I - V T^(H) V^H) X for V (m x nbp,
+// ld m, nbp <= 32), T (ld ldt), X (m rows, ld ldx, k <= 16 columns).  ws: at
+// least bse_apply_q_thin_workspace(m) complex elements.
+extern "C" hipError_t bse_apply_q_panel(hipStream_t st, int m, int nbp, const double2* V, const double2* T,
+                                        int ldt, int k, double2* X, int64_t ldx, int adjoint, double2* ws)
+{
+    if (nbp > APQ_NB || k <= 0 || k > PQ_MAXK || m < 1 || nbp < 1) return hipErrorInvalidValue;
+    double2* W = ws;
+    double2* P = ws + 512;
+    const int G = (int)ceil_div64(m, PQ_ROWS);
+    hipLaunchKernelGGL(k_pq_dots, dim3((unsigned)G), dim3(256), 0, st, m, nbp, k, V, (const double2*)X, ldx, P);
+    hipLaunchKernelGGL(k_pq_small, dim3((unsigned)k), dim3(512), 0, st, nbp, k, G, (const double2*)P, T, ldt,
+                       adjoint, W);
+    hipLaunchKernelGGL(k_pq_update, dim3((unsigned)ceil_div64(m, PQU_THREADS)), dim3(PQU_THREADS), 0, st, m, nbp,
+                       k, V, (const double2*)W, X, ldx);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Copies, fills and time stamps as kernels, so that sequences recorded into a
 // graph contain kernel nodes only.

@@ -39,6 +39,8 @@ extern "C" int bse_create(bse_ctx** out, int device) {
     (void)hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming);
     const char* at = getenv("BSE_AUTOTUNE");
     ctx->autotune = !(at && at[0] == '0');
+    const char* ts = getenv("BSE_TWO_STAGE");
+    if (ts) ctx->two_stage = (ts[0] == '0') ? -1 : 1;
     const char* am = getenv("BSE_ASSEMBLY");
     ctx->force_valu_assembly = am && strcmp(am, "valu") == 0;
     *out = ctx;
@@ -603,6 +605,117 @@ static int autotune_her2k(bse_ctx* ctx, int fp32, void* scratch, size_t scratch_
     return BSE_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Two-stage reduction to the real tridiagonal (d, e) in S_D, S_E:
+// dense -> band of half-width kBand (Householder panels; band_kernels.hip),
+// band -> tridiagonal (bulge chasing; chase_kernels.hip).  With rho,
+// y = Q^H rho lands in S_Y (Q1^H by panels, Q2^H inside the chase).  The
+// band matrix is kept in S_BAB0 for inverse iteration on it, the panel
+// reflectors in S_VSTORE / S_TALL (ctx->band_panels).
+// ---------------------------------------------------------------------------
+static const int kBand = 32;
+
+static bool use_two_stage(const bse_ctx* ctx, int n, int n_vectors, int max_vectors) {
+    const int want = (n_vectors < 0) ? n : std::max(n_vectors, max_vectors);
+    if (want > 20 || ctx->two_stage < 0 || n < 2) return false;
+    return ctx->two_stage > 0 || n >= 256;
+}
+
+static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double2* rho) {
+    hipStream_t st = ctx->st;
+    const int b = kBand, ldab = 2 * b + 1;
+    double* d = BUF(double, S_D, n);
+    double* e = BUF(double, S_E, std::max(n - 1, 1));
+    double2* AB = BUF(double2, S_BAB, (size_t)n * ldab);
+    double2* AB0 = BUF(double2, S_BAB0, (size_t)n * ldab);
+    HCK(bse_launch_zero_words(st, 2 * (int64_t)n * ldab, AB));
+    ctx->band_panels.clear();
+    int64_t voff = 0;
+    for (int c0 = 0;; c0 += b) {
+        const int s = c0 + b, m = n - s;
+        if (m < 2) break;
+        ctx->band_panels.push_back(bse_ctx::BandPanel{s, m, std::min(m, b), voff});
+        voff += (int64_t)m * b;
+    }
+    const int npan = (int)ctx->band_panels.size();
+    double2* Vst = BUF(double2, S_VSTORE, std::max<int64_t>(voff, 1));
+    double2* Tall = BUF(double2, S_TALL, (size_t)std::max(npan, 1) * b * b);
+    double2* zeros = BUF(double2, S_BZERO, 64);
+    HCK(bse_launch_zero_words(st, 128, zeros));
+    hipEvent_t* tv = ctx->tev;
+    if (npan > 0) {
+        const int m0 = n - b;
+        double2* P = BUF(double2, S_BPAN, (size_t)m0 * b);
+        double2* tw = BUF(double2, S_BTSQR, bse_tsqr_workspace(m0, b));
+        const int64_t uel = bse_band_update_workspace(n, b, b);
+        double2* uw = BUF(double2, S_BUPD, uel);
+        int rc = autotune_her2k(ctx, fp32, uw, sizeof(double2) * (size_t)uel, Vst, P,
+                                std::min<int64_t>(voff, (int64_t)m0 * b), zeros);
+        if (rc) return rc;
+        HCK(bse_launch_zero_words(st, 128, zeros));
+        HCK(hipEventRecord(tv[0], st));
+        const int kind = ctx->her2k_valu[fp32] >= 0 ? ctx->her2k_valu[fp32] : 1;
+        for (int p = 0; p < npan; ++p) {
+            const bse_ctx::BandPanel& bp = ctx->band_panels[p];
+            double2* Vp = Vst + bp.voff;
+            double2* Tp = Tall + (size_t)p * b * b;
+            int k = 0;
+            HCK(bse_band_gather(st, fp32, n, p * b, b, H, P, AB, ldab));
+            HCK(bse_band_panel_qr(st, bp.m, b, P, Vp, Tp, tw, AB, ldab, p * b, &k));
+            HCK(bse_band_update(st, fp32, kind, n, bp.s, k, b, H, Vp, Tp, uw, zeros));
+        }
+    }
+    if (npan == 0) HCK(hipEventRecord(tv[0], st));
+    HCK(bse_band_gather_tail(st, fp32, n, npan * b, H, AB, ldab));
+    HCK(bse_launch_copy_words(st, 2 * (int64_t)n * ldab, AB, AB0));
+    double2* y = nullptr;
+    if (rho) {
+        y = BUF(double2, S_Y, n);
+        double2* qw = BUF(double2, S_W1, bse_apply_q_thin_workspace(n));
+        HCK(hipMemcpyAsync(y, rho, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+        for (int p = 0; p < npan; ++p) {
+            const bse_ctx::BandPanel& bp = ctx->band_panels[p];
+            HCK(bse_apply_q_panel(st, bp.m, bp.k, Vst + bp.voff, Tall + (size_t)p * b * b, b, 1, y + bp.s, n, 1, qw));
+        }
+    }
+    HCK(hipEventRecord(tv[1], st));
+    const int64_t ncnt = bse_chase_counters(n);
+    int* prog = BUF(int, S_BPROG, ncnt + 4);
+    HCK(bse_launch_zero_words(st, (ncnt + 4) / 2, prog));
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) ncu = 64;
+    HCK(bse_launch_chase(st, n, b, AB, ldab, y, d, e, prog, prog + ncnt, std::max(1, ncu - 16)));
+    int err = 0;
+    HCK(hipMemcpyAsync(&err, prog + ncnt, sizeof(int), hipMemcpyDeviceToHost, st));
+    HCK(hipStreamSynchronize(st));
+    float ms1 = 0.f;
+    HCK(hipEventElapsedTime(&ms1, tv[0], tv[1]));
+    ctx->band_ms = ms1;
+    if (err) return fail(ctx, BSE_ERR_NUMERIC, "band to tridiagonal: a sweep waited too long on its predecessor");
+    return BSE_OK;
+}
+
+// Eigenvectors of the lowest nstore eigenvalues (d_lam, ascending) after a
+// two-stage reduction: inverse iteration on the band matrix, then Z = Q1 X.
+static int twostage_vectors(bse_ctx* ctx, int n, const double* d_lam, int nstore, double tnorm, double2* Z) {
+    hipStream_t st = ctx->st;
+    const int b = kBand, ldab = 2 * b + 1;
+    double2* ws = BUF(double2, S_BII, bse_band_ii_workspace(n, b, nstore));
+    HCK(bse_launch_band_ii(st, n, b, ldab, (const double2*)ctx->bufs[S_BAB0].p, d_lam, nstore, tnorm, ws, Z, n));
+    double2* qw = BUF(double2, S_W1, bse_apply_q_thin_workspace(n));
+    const double2* Vst = (const double2*)ctx->bufs[S_VSTORE].p;
+    const double2* Tall = (const double2*)ctx->bufs[S_TALL].p;
+    for (int p = (int)ctx->band_panels.size() - 1; p >= 0; --p) {
+        const bse_ctx::BandPanel& bp = ctx->band_panels[p];
+        for (int c0 = 0; c0 < nstore; c0 += 16) {
+            const int kc = std::min(16, nstore - c0);
+            HCK(bse_apply_q_panel(st, bp.m, bp.k, Vst + bp.voff, Tall + (size_t)p * b * b, b, kc,
+                                  Z + bp.s + (int64_t)c0 * n, n, 0, qw));
+        }
+    }
+    return BSE_OK;
+}
+
 // guard_tol > 0: eigenvalues within guard_tol above the last requested one
 // are formed as well, up to max_vectors in total (ctx->nvec reports the count).
 static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
@@ -612,19 +725,40 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
     const int n = (int)n64;
     const int nb = kPanel;
     hipStream_t st = ctx->st;
+    const bool two = use_two_stage(ctx, n, n_vectors, max_vectors);
+    ctx->last_two_stage = two;
     // ---------------- K5: tridiagonalisation
-    int64_t vs_el, yr_el, yc_el, part_d;
-    bse_hetrd_workspace(n, nb, &vs_el, &yr_el, &yc_el, &part_d);
+    int64_t vs_el = 1, yr_el = 1, yc_el = 1, part_d = 1;
     std::vector<int64_t> voff;
+    int npan = 0;
+    double2* Vst = nullptr;
+    double2* Tall = nullptr;
+    double2* skw = nullptr;
+    int64_t sk_el = 0;
+    double* d = nullptr;
+    double* e = nullptr;
+    if (two) {
+        stage_begin(ctx, ST_TRD);
+        int rc2 = twostage_tridiag(ctx, n, H, fp32, rho);
+        if (rc2) return rc2;
+        d = (double*)ctx->bufs[S_D].p;
+        e = (double*)ctx->bufs[S_E].p;
+        sk_el = (int64_t)64 * 1024 * 1024 / sizeof(double2);
+        skw = BUF(double2, S_SPLITK, std::min<int64_t>(sk_el, std::max<int64_t>((int64_t)64 * nb * 64, (int64_t)nb * nb * 512)));
+        sk_el = (int64_t)(ctx->bufs[S_SPLITK].cap / sizeof(double2));
+        stage_end(ctx, ST_TRD);
+    } else {
+    bse_hetrd_workspace(n, nb, &vs_el, &yr_el, &yc_el, &part_d);
+    bse_hetrd_workspace(n, nb, &vs_el, &yr_el, &yc_el, &part_d);
     {
         int64_t off = 0;
         for (int j0 = 0; j0 < n - 1; j0 += nb) { voff.push_back(off); off += (int64_t)(n - j0) * nb; }
     }
-    const int npan = (int)voff.size();
-    double* d = BUF(double, S_D, n);
-    double* e = BUF(double, S_E, std::max(n - 1, 1));
+    npan = (int)voff.size();
+    d = BUF(double, S_D, n);
+    e = BUF(double, S_E, std::max(n - 1, 1));
     double2* tauh = BUF(double2, S_TAUH, std::max(n - 1, 1));
-    double2* Vst = BUF(double2, S_VSTORE, vs_el);
+    Vst = BUF(double2, S_VSTORE, vs_el);
     double2* Wp = BUF(double2, S_WP, (size_t)n * nb);
     double2* acol = BUF(double2, S_ACOL, n);
     double2* Yr = BUF(double2, S_YR, yr_el);
@@ -644,7 +778,7 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
         }
         pev = ctx->prof_ev.data();
     }
-    double2* Tall = BUF(double2, S_TALL, (size_t)std::max(npan, 1) * nb * nb);
+    Tall = BUF(double2, S_TALL, (size_t)std::max(npan, 1) * nb * nb);
     double2* Gram = BUF(double2, S_GRAM, (size_t)std::max(npan, 1) * 64 * 64);   // one block per panel
     double2* alpha2 = BUF(double2, S_ALPHA2, 64);
     {
@@ -674,9 +808,10 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
         ctx->hemv_launches = n - 1;
     }
     const int64_t skws = (int64_t)64 * 1024 * 1024 / sizeof(double2) * 8;   // 512 MiB split-K workspace cap
-    const int64_t sk_el = std::min<int64_t>(skws, std::max<int64_t>((int64_t)64 * nb * std::max(n_vectors, 64), (int64_t)nb * nb * 512));
-    double2* skw = BUF(double2, S_SPLITK, sk_el);
+    sk_el = std::min<int64_t>(skws, std::max<int64_t>((int64_t)64 * nb * std::max(n_vectors, 64), (int64_t)nb * nb * 512));
+    skw = BUF(double2, S_SPLITK, sk_el);
     stage_end(ctx, ST_TRD);
+    }
 
     // ---------------- K6: tridiagonal eigenvalues (host bookkeeping + bisection)
     stage_begin(ctx, ST_TEVAL);
@@ -691,7 +826,9 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
     // bookkeeping and the bisection; st waits for it before the first
     // contraction.  Nothing on st touches y, w1, w2 or skw in between.
     double2* y = nullptr;
-    if (rho) {
+    if (rho && two) {
+        y = (double2*)ctx->bufs[S_Y].p;     // Q^H rho came out of the chase
+    } else if (rho) {
         y = BUF(double2, S_Y, n);
         double2* w1 = BUF(double2, S_W1, (size_t)nb * 64);
         double2* w2 = BUF(double2, S_W2, (size_t)nb * 64);
@@ -827,8 +964,11 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
     int2* d_cl = BUF(int2, S_CLUST, std::max<size_t>(clall.size(), 1));
     int* d_memb = BUF(int, S_MEMB, n);
     HCK(hipMemcpyAsync(d_memb, memb.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    if (rho) HCK(hipStreamWaitEvent(st, ctx->join_ev, 0));   // y = Q^H rho is ready
+    if (rho && !two) HCK(hipStreamWaitEvent(st, ctx->join_ev, 0));   // y = Q^H rho is ready
     int m0 = 0;
+    // the two-stage path forms its vectors from the band matrix: the
+    // tridiagonal vectors are needed only for the oscillator contraction
+    if (two && !rho) m0 = n;
     size_t ci = 0;   // clusters are visited in order of their first member
     std::vector<int2> clall_sorted = clall;
     std::sort(clall_sorted.begin(), clall_sorted.end(),
@@ -853,7 +993,7 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
             else
                 HCK(bse_launch_contract(st, bsz, cnt, m0, d_bsv, d_nbv, xb, y, fvec));
         }
-        const int ncommit = std::max(0, std::min(m1, nstore) - m0);
+        const int ncommit = two ? 0 : std::max(0, std::min(m1, nstore) - m0);
         if (ncommit > 0) HCK(bse_launch_commit(st, n, bsz, cnt, ncommit, m0, d_bsv, d_nbv, xb, Z, n, m0));
         HCK(hipStreamSynchronize(st));   // keeps d_cl and the host vectors alive
         m0 = m1;
@@ -862,7 +1002,10 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
 
     // ---------------- K7: back-transform Z <- Q Z
     stage_begin(ctx, ST_BT);
-    if (nstore > 0 && nstore <= 16 && bse_apply_q_thin_workspace(n) <= sk_el) {
+    if (two && nstore > 0) {
+        int rc2 = twostage_vectors(ctx, n, d_lam, nstore, tnorm, Z);
+        if (rc2) return rc2;
+    } else if (nstore > 0 && nstore <= 16 && bse_apply_q_thin_workspace(n) <= sk_el) {
         HCK(bse_apply_q_thin(st, n, nb, Vst, Tall, nstore, Z, n, 0, skw));
     } else if (nstore > 0) {
         double2* w1 = BUF(double2, S_W1, (size_t)nb * nstore);
@@ -1340,8 +1483,10 @@ extern "C" int bse_spectrum(bse_ctx* ctx, const double* omega, int n_omega, doub
     return BSE_OK;
 }
 
-extern "C" int bse_eigh_dense(bse_ctx* ctx, int n, const double* a, int64_t lda, int a_on_device,
-                              int precision, int n_vectors, double* evals, double* z, int64_t ldz) {
+// Dense Hermitian eig of a caller matrix; outputs on the host or the device.
+int eigh_dense_impl(bse_ctx* ctx, int n, const double* a, int64_t lda, int a_on_device, int precision,
+                    int n_vectors, double* evals, int evals_on_device, double* z, int64_t ldz,
+                    int z_on_device) {
     if (!ctx || !a || n <= 0 || lda < n) return BSE_ERR_ARG;
     if (precision != BSE_PREC_FP64 && precision != BSE_PREC_FP32)
         return fail(ctx, BSE_ERR_ARG, "dense eig precision must be FP64 or FP32 storage");
@@ -1367,11 +1512,20 @@ extern "C" int bse_eigh_dense(bse_ctx* ctx, int n, const double* a, int64_t lda,
     ctx->eig_on_bse = false;
     rc = eig_pipeline(ctx, n, Hm, fp32, nullptr, n_vectors);
     if (rc) return rc;
-    if (evals) HCK(hipMemcpy(evals, ctx->bufs[S_EVAL].p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (evals)
+        HCK(hipMemcpyAsync(evals, ctx->bufs[S_EVAL].p, sizeof(double) * n,
+                           evals_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->st));
     if (z && ctx->nvec > 0) {
         if (ldz < n) return fail(ctx, BSE_ERR_ARG, "ldz < n");
-        HCK(hipMemcpy2D(z, sizeof(double2) * ldz, ctx->bufs[S_Z].p, sizeof(double2) * n,
-                        sizeof(double2) * n, ctx->nvec, hipMemcpyDeviceToHost));
+        HCK(hipMemcpy2DAsync(z, sizeof(double2) * ldz, ctx->bufs[S_Z].p, sizeof(double2) * n,
+                             sizeof(double2) * n, ctx->nvec,
+                             z_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->st));
     }
+    HCK(hipStreamSynchronize(ctx->st));
     return BSE_OK;
+}
+
+extern "C" int bse_eigh_dense(bse_ctx* ctx, int n, const double* a, int64_t lda, int a_on_device,
+                              int precision, int n_vectors, double* evals, double* z, int64_t ldz) {
+    return eigh_dense_impl(ctx, n, a, lda, a_on_device, precision, n_vectors, evals, 0, z, ldz, 0);
 }

@@ -58,6 +58,22 @@ hipError_t bse_launch_apply_q(hipStream_t, int, int, const double2*, const doubl
 int64_t bse_apply_q_thin_workspace(int);
 hipError_t bse_apply_q_thin(hipStream_t, int, int, const double2*, const double2*, int, double2*, int64_t,
                             int, double2*);
+hipError_t bse_apply_q_panel(hipStream_t, int, int, const double2*, const double2*, int, int, double2*,
+                             int64_t, int, double2*);
+// two-stage reduction (band_kernels.hip, chase_kernels.hip)
+hipError_t bse_band_gather(hipStream_t, int, int, int, int, const void*, double2*, double2*, int);
+hipError_t bse_band_gather_tail(hipStream_t, int, int, int, const void*, double2*, int);
+int64_t bse_tsqr_workspace(int, int);
+hipError_t bse_band_panel_qr(hipStream_t, int, int, double2*, double2*, double2*, double2*, double2*, int,
+                             int, int*);
+int64_t bse_band_update_workspace(int, int, int);
+hipError_t bse_band_update(hipStream_t, int, int, int, int, int, int, void*, const double2*, const double2*,
+                           double2*, const double2*);
+int64_t bse_chase_counters(int);
+hipError_t bse_launch_chase(hipStream_t, int, int, double2*, int, double2*, double*, double*, int*, int*, int);
+int64_t bse_band_ii_workspace(int, int, int);
+hipError_t bse_launch_band_ii(hipStream_t, int, int, int, const double2*, const double*, int, double,
+                              double2*, double2*, int64_t);
 hipError_t bse_launch_copy_words(hipStream_t, int64_t, const void*, void*);
 hipError_t bse_launch_zero_words(hipStream_t, int64_t, void*);
 hipError_t bse_launch_stamp(hipStream_t, unsigned long long*);
@@ -170,6 +186,21 @@ struct bse_ctx {
     // caller's k order (bse_set_problem_klist): list position of each grid
     // index and its inverse; empty for the plain grid
     std::vector<int> kmap_h, kinv_h;
+    // two-stage reduction: 0 auto (for at most 20 eigenvectors and n >= 256),
+    // 1 whenever the eigenvectors asked for are at most 20, -1 never
+    // (BSE_TWO_STAGE=1 / 0); band panels of the last two-stage reduction
+    // (row start, rows, reflectors, offset into the reflector store)
+    int two_stage = 0;
+    bool last_two_stage = false;
+    struct BandPanel { int s, m, k; int64_t voff; };
+    std::vector<BandPanel> band_panels;
+    double chase_ms = 0.0, band_ms = 0.0;
+    // allocation-free calls: while arena_base is set, buffers are carved from
+    // the device scratch array (bump allocation, arena_off) instead of
+    // hipMalloc; arena_bufs is the slot table of those calls
+    char* arena_base = nullptr;
+    size_t arena_cap = 0, arena_off = 0, arena_hwm = 0;
+    std::vector<DevBuf> arena_bufs;
     // Gershgorin norm of the last tridiagonal (= an upper bound of ||H||)
     double hnorm = 0.0;
     // last refinement: residual norms ||H x_j - theta_j x_j|| (eV) of the
@@ -190,7 +221,8 @@ enum Slot {
     S_Y, S_F, S_Z, S_W1, S_W2, S_SPLITK, S_EVAL, S_EPSOUT, S_STAGE, S_ALPHA2,
     S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL,
     S_GDHR, S_GDEPS, S_GDCOEF, S_GDP, S_GDPV, S_GDRHO, S_GDDE, S_GDW, S_GY, S_GOUT,
-    S_SGFB, S_SGFA, S_SGX, S_SGU, S_SGB, S_SGV, S_SGPART, S_APPLYP, S_MEMB, S_KMAP, S_KINV, S_TBK, S_TBH, S_TBR, S_TBE, S_TBC, S_NSLOTS
+    S_SGFB, S_SGFA, S_SGX, S_SGU, S_SGB, S_SGV, S_SGPART, S_APPLYP, S_MEMB, S_KMAP, S_KINV, S_TBK, S_TBH, S_TBR, S_TBE, S_TBC,
+    S_BAB, S_BAB0, S_BPAN, S_BTSQR, S_BUPD, S_BPROG, S_BII, S_BZERO, S_NSLOTS
 };
 
 int fail(bse_ctx* c, int code, const std::string& msg) {
@@ -209,6 +241,20 @@ void* buf(bse_ctx* ctx, int slot, size_t bytes, int* status) {
     DevBuf& b = ctx->bufs[slot];
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return b.p;
+    if (ctx->arena_base) {
+        // allocation-free mode: carve from the fixed scratch array, never free
+        const size_t off = (ctx->arena_off + 255) & ~(size_t)255;
+        if (off + bytes > ctx->arena_cap) {
+            *status = fail(ctx, BSE_ERR_ARG, "workspace exceeds the fixed scratch array (" +
+                                                 std::to_string(ctx->arena_cap) + " bytes)");
+            return nullptr;
+        }
+        b.p = ctx->arena_base + off;
+        b.cap = bytes;
+        ctx->arena_off = off + bytes;
+        ctx->arena_hwm = std::max(ctx->arena_hwm, ctx->arena_off);
+        return b.p;
+    }
     if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.cap = 0; }
     if (hipMalloc(&b.p, bytes) != hipSuccess) {
         b.p = nullptr;
@@ -378,6 +424,11 @@ int make_geom(const bse_tb_model* m, const bse_system* s, Geom& g, std::string& 
 }
 
 }  // namespace
+
+// Dense eig with host or device outputs (bse_api.hip).
+int eigh_dense_impl(bse_ctx* ctx, int n, const double* a, int64_t lda, int a_on_device, int precision,
+                    int n_vectors, double* evals, int evals_on_device, double* z, int64_t ldz,
+                    int z_on_device);
 
 // Releases the batched-sweep state of a context (batch_api.hip).
 void batch_destroy(bse_ctx* ctx);
