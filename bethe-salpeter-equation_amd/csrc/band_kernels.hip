@@ -651,9 +651,10 @@ extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P
                                         double2* Tp, double2* ws, double2* AB, int ldab, int c0, int* kout)
 {
     if (b > BR_B || m < 1) return hipErrorInvalidValue;
-    // the tree nodes hold up to 8 x 32 x 32 complex doubles (128 KB) of LDS
+    // the largest in-LDS QR holds TSQR_MAXROWS x 32 complex doubles (128 KB);
+    // the limit asked for must leave room for the kernels' static LDS
     static const hipError_t attr = [] {
-        const int big = 160 * 1024;
+        const int big = TSQR_MAXROWS * BR_B * (int)sizeof(double2);
         hipError_t e = hipFuncSetAttribute((const void*)k_tsqr_leaf, hipFuncAttributeMaxDynamicSharedMemorySize, big);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void*)k_tsqr_node, hipFuncAttributeMaxDynamicSharedMemorySize, big);
