@@ -27,6 +27,7 @@ ABI_SYMBOLS = (
     "bse_spectrum", "bse_eigh_dense", "bse_stage_times", "bse_profile", "bse_apply", "bse_grad_energies",
     "bse_batch_prepare", "bse_batch_run", "bse_batch_results", "bse_batch_info",
     "bse_grad_spectrum", "bse_refine_info", "bse_set_problem_klist", "bse_tb_states_at",
+    "bse_eigh_dense_noalloc", "bse_noalloc_info",
 )
 
 

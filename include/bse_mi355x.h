@@ -274,6 +274,25 @@ int bse_batch_results(bse_ctx* ctx, double* evals, double* eps_re, double* eps_i
  *     2 FP64 MFMA on 128 x 128 tiles).                                  */
 int bse_batch_info(const bse_ctx* ctx, int64_t info[6]);
 
+/* ---- allocation-free dense eig --------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory while they run (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): bse_eigh_dense with every input and output
+ * on the device (a: complex n x n, lower triangle read; evals: n doubles; z:
+ * complex n x n_vectors, ld ldz, may be NULL when n_vectors = 0) and every
+ * work buffer cut from one fixed device array of 64 MiB that the library
+ * holds per GPU until the process ends.  n <= 512 (BSE_ERR_ARG beyond);
+ * precision and n_vectors as for bse_eigh_dense.  It calls no hipMalloc or
+ * hipFree.  Calls on one GPU share that GPU's array and leave its contents
+ * behind: calls that overlap on one GPU (two contexts on two host threads)
+ * race.  The call leaves no eigen-results in the context (the getters then
+ * report BSE_ERR_STATE). */
+int bse_eigh_dense_noalloc(bse_ctx* ctx, int n, const double* a, int64_t lda, int precision,
+                           int n_vectors, double* evals, double* z, int64_t ldz);
+/* info[0] bytes of the fixed array, [1] largest n, [2] bytes the last
+ * allocation-free call used. */
+int bse_noalloc_info(const bse_ctx* ctx, int64_t info[3]);
+
 #ifdef __cplusplus
 }
 #endif
