@@ -54,8 +54,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pkg, n_sample: int, n_full: int):
-    """FP64 oracle on the same model at an n_sample x n_sample grid, scaled."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pkg, grids, n_full: int):
+    """FP64 oracle (numpy assembly + LAPACK zheevd, all vectors) on the same
+    model at two reduced k-grids; the eig time is scaled to N = 16 n_full^2
+    with the exponent fitted between the two samples, the build with N^2."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import bse_oracle as O
     threads = None
@@ -65,21 +77,28 @@ def cpu_baseline(pkg, n_sample: int, n_full: int):
         threads = max((i.get("num_threads", 0) for i in info), default=None)
     except Exception:
         pass
-    s = pkg.random_multiorbital(n_sample)
-    p = O.problem_from_system(s)
-    t0 = time.perf_counter()
-    H = O.assemble(p)
-    t1 = time.perf_counter()
-    E, Z = O.eigh(H)
-    t2 = time.perf_counter()
-    n_s, n_f = s.n_bse, 16 * n_full * n_full
-    build_s = (t1 - t0) * (n_f / n_s) ** 2
-    eig_s = (t2 - t1) * (n_f / n_s) ** 3
+    rows = []
+    for g in grids:
+        s = pkg.random_multiorbital(g)
+        p = O.problem_from_system(s)
+        t0 = time.perf_counter()
+        H = O.assemble(p)
+        t1 = time.perf_counter()
+        O.eigh(H)
+        t2 = time.perf_counter()
+        rows.append((s.n_bse, t1 - t0, t2 - t1))
+        log(f"cpu sample {g}x{g} (N={s.n_bse}): build {t1 - t0:.2f} s, eig {t2 - t1:.2f} s")
+    (n1, b1, e1), (n2, b2, e2) = rows[0], rows[-1]
+    expo = float(np.log(e2 / e1) / np.log(n2 / n1)) if n2 != n1 else 3.0
+    n_f = 16 * n_full * n_full
+    build_s = b2 * (n_f / n2) ** 2
+    eig_s = e2 * (n_f / n2) ** expo
     return dict(value=build_s + eig_s, unit="s", cores=threads or os.cpu_count(),
-                kind="port",
-                sample=(f"FP64 oracle (numpy; zheevd via numpy.linalg.eigh) on the same model at "
-                        f"{n_sample}x{n_sample} k (N={n_s}): build {t1 - t0:.2f} s, eig "
-                        f"{t2 - t1:.2f} s; scaled to N={n_f} as build*(N/n)^2 + eig*(N/n)^3"))
+                kind="port", cpu_model=_cpu_model(), fitted_eig_exponent=expo,
+                samples=[dict(N=n, build_s=b, eig_s=e) for n, b, e in rows],
+                sample=(f"FP64 oracle (numpy assembly; zheevd via numpy.linalg.eigh, all vectors) on "
+                        f"the same model at N={n1} and N={n2}; eig scaled to N={n_f} with the fitted "
+                        f"exponent {expo:.2f}, build with N^2"))
 
 
 def copy_calibration(args):
@@ -209,7 +228,8 @@ def main():
     ap.add_argument("--no-refine", dest="refine", action="store_false")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample-grid", type=int, default=20)
+    ap.add_argument("--cpu-sample-grids", default="12,20",
+                    help="two k-grids n (N = 16 n^2) for the CPU oracle timing")
     ap.add_argument("--workload", default="n20736", choices=["n20736", "sweep", "grad", "copy"],
                     help="n20736: the headline single system; sweep: the 64-system batch; "
                          "grad: gradients on the N = 20736 system; copy: plain copy for "
@@ -255,15 +275,30 @@ def main():
     t_mixed = float(np.mean(times))
     stages_mixed = ctx.stage_times()
     E_mixed = ctx.eigenvalues()
+    Z_mixed = ctx.eigenvectors(n_low)
+    refine = None
+    if args.refine:
+        rn, rounds, status = ctx.refine_info(n_low)
+        refine = dict(max_residual_eV=float(rn.max()), rounds=int(rounds), status=int(status))
     omegas = s.omegas()
     eps_mixed = ctx.spectrum(omegas, s.eta)
     # profiled pass for the roofline of the dominant kernel (not part of the timing)
     mixed_step(profile=True)
     prof = ctx.profile(enable=False)
-    log(f"hemv profile: {prof}")
+    log(f"profile: {prof}")
+    asm_ms = ctx.stage_times()["assembly"]
 
     out_fp64 = {}
     if not args.no_fp64:
+        # FP64 with the lowest 16 vectors (the like-for-like comparison) ...
+        t = time.perf_counter()
+        ctx.build("fp64")
+        ctx.eigh(n_low)
+        t_fp64_16 = time.perf_counter() - t
+        stages_fp64_16 = ctx.stage_times()
+        E64_16 = ctx.eigenvalues()
+        log(f"fp64 lowest-16 path: {t_fp64_16:.3f} s, stages {stages_fp64_16}")
+        # ... and with all N vectors (what eig_sym returns in the reference)
         t = time.perf_counter()
         ctx.build("fp64")
         ctx.eigh(-1)
@@ -271,42 +306,107 @@ def main():
         stages_fp64 = ctx.stage_times()
         E64 = ctx.eigenvalues()
         eps64 = ctx.spectrum(omegas, s.eta)
+        log(f"fp64 all-vector path: {t_fp64:.3f} s, stages {stages_fp64}")
+        nchk = 256
+        Z64 = ctx.eigenvectors(nchk)
+        scale = float(np.abs(E64).max())
         err_mev = float(np.abs(E_mixed[:n_low] - E64[:n_low]).max() * 1e3)
         spec_l2 = float(np.linalg.norm(eps_mixed.imag - eps64.imag) / np.linalg.norm(eps64.imag))
-        out_fp64 = dict(fp64_path_s=t_fp64, speedup_vs_fp64=t_fp64 / t_mixed,
-                        lowest16_err_meV_vs_fp64=err_mev, spectrum_rel_l2_vs_fp64=spec_l2,
-                        stages_ms_fp64=stages_fp64)
-        log(f"fp64 path: {t_fp64:.3f} s, stages {stages_fp64}")
+        overlaps = cluster_overlaps(Z_mixed, Z64, E64, 1e-6 * scale, n_low)
+        R = ctx.apply(Z64[:, :64]) - Z64[:, :64] * E64[None, :64]
+        out_fp64 = dict(
+            fp64_all_vectors_s=t_fp64, fp64_lowest16_s=t_fp64_16,
+            speedup_vs_fp64_all_vectors=t_fp64 / t_mixed, speedup_vs_fp64_lowest16=t_fp64_16 / t_mixed,
+            lowest16_err_meV_vs_fp64=err_mev,
+            all_eigenvalues_max_err_meV_vs_fp64=float(np.abs(E_mixed - E64).max() * 1e3),
+            lowest16_two_stage_fp64_vs_one_stage_fp64_eV=float(np.abs(E64_16[:n_low] - E64[:n_low]).max()),
+            eigenvector_overlap_per_cluster=overlaps,
+            spectrum_rel_l2_vs_fp64=spec_l2,
+            fp64_residual_lowest64_rel=float(np.linalg.norm(R, axis=0).max() / scale),
+            fp64_orthonormality_lowest256=float(np.abs(Z64.conj().T @ Z64 - np.eye(nchk)).max()),
+            stages_ms_fp64_all_vectors=stages_fp64, stages_ms_fp64_lowest16=stages_fp64_16)
 
-    ach = prof["hemv_bytes"] / (prof["hemv_ms"] * 1e-3) / 1e9 if prof["hemv_ms"] > 0 else 0.0
     traffic, traffic_src = measured_traffic(args.mixed, N)
-    roof = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
-                traffic=traffic, traffic_source=(traffic_src or {}).get("source") or
-                ("none: rocprofv3 --pmc FETCH_SIZE ends in a host segfault inside the profiler "
-                 "at the first dispatch on this ROCm image, so no counter figure exists"),
-                kernel="kb_hemv<float2, *> (every variant the tuner picks)",
-                launches=prof["hemv_launches"],
-                algorithmic_bytes_per_launch=prof["hemv_bytes"] / max(1, prof["hemv_launches"]),
-                avg_launch_us=prof["hemv_ms"] * 1e3 / max(1, prof["hemv_launches"]))
+    roof = roofline_entry(prof, traffic, traffic_src)
+    # the assembly kernel: algorithmic MFMA flops of the two chained complex
+    # contractions per (k, k') pair of the lower triangle, and the FP32
+    # complex lower triangle it writes
+    nk = s.nk
+    pairs = nk * (nk + 1) // 2
+    fl_pair = 8 * s.nc ** 2 * s.norb ** 2 + 8 * s.nc ** 2 * s.nv ** 2 * s.norb
+    asm_flops = float(pairs * fl_pair)
+    asm_bytes = float(N * (N + 1) // 2 * (16 if args.mixed == "fp64" else 8))
+    asm = dict(kernel=f"k_assemble_mfma<{args.mixed}> (stage time, one launch)", ms=asm_ms,
+               flops=asm_flops, tflops=asm_flops / (asm_ms * 1e-3) / 1e12 if asm_ms > 0 else 0.0,
+               mfma_peak_tflops=2500.0 if args.mixed == "bf16x2" else 157.3,
+               bytes_written=asm_bytes,
+               write_gbs=asm_bytes / (asm_ms * 1e-3) / 1e9 if asm_ms > 0 else 0.0,
+               hbm_peak_gbs=HBM_PEAK_GBS)
+    asm["mfma_frac"] = asm["tflops"] / asm["mfma_peak_tflops"]
+    asm["hbm_frac"] = asm["write_gbs"] / HBM_PEAK_GBS
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(pkg, args.cpu_sample_grid, args.grid)
+        cpu = cpu_baseline(pkg, [int(x) for x in args.cpu_sample_grids.split(",")], args.grid)
+    dtype = {"bf16x2": "fp32-storage (bf16x2 MFMA assembly) + fp64 reflectors/refinement",
+             "fp32": "fp32-storage (fp32 MFMA assembly) + fp64 reflectors/refinement"}[args.mixed]
     res = {
         "metric": "N=20736 BSE build+eig time: mixed vs FP64 GPU at <=0.5 meV; MFMA/HBM % peak",
         "value": t_mixed, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": t_mixed * 1e3, "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": f"{args.mixed}-storage+fp64",
+        "vs_baseline": None, "dtype": dtype,
         "data": "synthetic (seeded PCG64 random TB model)",
         "config": {"workload": f"random_multiorbital_{args.grid}x{args.grid}", "N": N,
                    "norb": s.norb, "nv": s.nv, "nc": s.nc, "nk": s.nk, "r0": s.r0,
-                   "mixed": args.mixed, "lowest_vectors": n_low, "refine": bool(args.refine)},
+                   "mixed": args.mixed, "lowest_vectors": n_low, "refine": bool(args.refine),
+                   "reduction": "two-stage" if stages_mixed.get("band_reduction", 0) > 0 else "one-stage"},
         "stages_ms_mixed": stages_mixed,
+        "refinement": refine,
         **out_fp64,
         "roofline": roof,
+        "assembly_roofline": asm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))
     ctx.close()
+
+
+def cluster_overlaps(Z, Zref, E, tol, ncol):
+    """1 - (smallest singular value of Zref_c^H Z_c) per cluster of eigenvalues
+    closer than tol that lies inside the first ncol columns: 0 when the two
+    bases span the same subspace."""
+    out = []
+    i = 0
+    while i < ncol:
+        j = i + 1
+        while j < len(E) and E[j] - E[j - 1] < tol:
+            j += 1
+        if j > ncol:
+            break
+        sv = np.linalg.svd(Zref[:, i:j].conj().T @ Z[:, i:j], compute_uv=False)
+        out.append(dict(first=i, size=j - i, E_eV=float(E[i]), one_minus_sigma_min=float(1.0 - sv.min())))
+        i = j
+    return out
+
+
+def roofline_entry(prof, traffic, traffic_src):
+    """Roofline object of the dominant kernel from the library's event profile."""
+    if prof.get("kind") == "flops":
+        ach = prof["flops"] / (prof["ms"] * 1e-3) / 1e12 if prof["ms"] > 0 else 0.0
+        return dict(bound="mfma", achieved=ach, peak=prof["peak"], unit="TFLOP/s", frac=ach / prof["peak"],
+                    traffic=None, traffic_source="not measured for this kernel",
+                    kernel=prof["kernel"], launches=prof["launches"],
+                    algorithmic_flops_per_launch=prof["flops"] / max(1, prof["launches"]),
+                    avg_launch_us=prof["ms"] * 1e3 / max(1, prof["launches"]),
+                    second_kernel=dict(prof["second"], tflops=prof["second"]["flops"] /
+                                       max(prof["second"]["ms"] * 1e-3, 1e-12) / 1e12))
+    ach = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0
+    return dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
+                traffic=traffic, traffic_source=(traffic_src or {}).get("source") or
+                ("none: rocprofv3 --pmc FETCH_SIZE ends in a host segfault inside the profiler "
+                 "at the first dispatch on this ROCm image, so no counter figure exists"),
+                kernel=prof["kernel"], launches=prof["launches"],
+                algorithmic_bytes_per_launch=prof["bytes"] / max(1, prof["launches"]),
+                avg_launch_us=prof["ms"] * 1e3 / max(1, prof["launches"]))
 
 
 if __name__ == "__main__":

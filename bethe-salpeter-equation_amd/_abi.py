@@ -27,7 +27,7 @@ ABI_SYMBOLS = (
     "bse_spectrum", "bse_eigh_dense", "bse_stage_times", "bse_profile", "bse_apply", "bse_grad_energies",
     "bse_batch_prepare", "bse_batch_run", "bse_batch_results", "bse_batch_info",
     "bse_grad_spectrum", "bse_refine_info", "bse_set_problem_klist", "bse_tb_states_at",
-    "bse_eigh_dense_noalloc", "bse_noalloc_info",
+    "bse_eigh_dense_noalloc", "bse_noalloc_info", "bse_build_full", "bse_eigh_full", "bse_get_coupling",
 )
 
 
@@ -135,6 +135,9 @@ def load_library(path: pathlib.Path | str | None = None):
         "bse_set_problem_klist": (i32, [vp, C.POINTER(TBModelC), C.POINTER(SystemC), dp, i32]),
         "bse_tb_states_at": (i32, [vp, C.POINTER(TBModelC), dp, i32, i32, i32, i32, C.c_double,
                                    dp, dp]),
+        "bse_build_full": (i32, [vp]),
+        "bse_eigh_full": (i32, [vp, i32]),
+        "bse_get_coupling": (i32, [vp, dp, i64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -149,7 +152,8 @@ def _dptr(a: np.ndarray):
 
 
 STAGES = ("tb", "tables", "assembly", "tridiagonalisation", "tridiagonal_values",
-          "tridiagonal_vectors", "back_transform", "refinement", "spectrum")
+          "tridiagonal_vectors", "back_transform", "refinement", "spectrum",
+          "band_reduction", "bulge_chase")
 
 
 class Context:
@@ -305,6 +309,30 @@ class Context:
     def eigh(self, n_vectors=-1, refine=False):
         self._check(self.lib.bse_eigh(self.h, int(n_vectors), int(bool(refine))), "bse_eigh")
         self.n_vectors = n_vectors
+        self._full = False
+        return self
+
+    def build_full(self):
+        """Assemble A and the coupling block B of the full BSE (FP64)."""
+        self._check(self.lib.bse_build_full(self.h), "bse_build_full")
+        self._full = False
+        return self
+
+    def coupling(self):
+        """The coupling block B (N x N complex symmetric), reference order."""
+        n = self.sizes()["n"]
+        out = np.zeros((n, n), dtype=np.complex128, order="F")
+        self._check(self.lib.bse_get_coupling(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), n),
+                    "bse_get_coupling")
+        return out
+
+    def eigh_full(self, n_vectors=-1):
+        """Structure-preserving eig of [[A, B], [-conj B, -conj A]]: afterwards
+        eigenvalues() are the N positive energies, oscillator() f = rho~^H x,
+        eigenvectors(k) the 2N-row (x; y), spectrum() the full sum."""
+        self._check(self.lib.bse_eigh_full(self.h, int(n_vectors)), "bse_eigh_full")
+        self.n_vectors = n_vectors
+        self._full = True
         return self
 
     def refine_info(self, n):
@@ -323,7 +351,7 @@ class Context:
         return out
 
     def eigenvectors(self, nvec):
-        n = self.sizes()["n"]
+        n = self.sizes()["n"] * (2 if getattr(self, "_full", False) else 1)
         out = np.zeros((n, nvec), dtype=np.complex128, order="F")
         self._check(self.lib.bse_get_eigenvectors(self.h, out.ctypes.data_as(C.POINTER(C.c_double)),
                                                   n, int(nvec), 0), "bse_get_eigenvectors")
@@ -360,14 +388,24 @@ class Context:
         return ev, z[:, :nv]
 
     def profile(self, enable=None):
-        """Hemv kernel profile of the last eig: (ms, algorithmic bytes, launches)."""
-        out = np.zeros(3)
+        """Profile of the last eig's dominant kernel: the one-stage reduction's
+        Hermitian matrix-vector product (algorithmic bytes), or the two-stage
+        band reduction's rank-2k update and block product (algorithmic flops,
+        FP64 arithmetic)."""
+        out = np.zeros(9)
         en = 1 if enable else 0
         if enable is None:
             en = 1 if getattr(self, "_prof", False) else 0
         self._prof = bool(en)
         self._check(self.lib.bse_profile(self.h, en, _dptr(out)), "bse_profile")
-        return dict(hemv_ms=out[0], hemv_bytes=out[1], hemv_launches=int(out[2]))
+        if int(out[4]) == 1:
+            return dict(kind="flops", kernel="band rank-2k update (her2k, FP64 products)", ms=out[0],
+                        flops=out[3], launches=int(out[2]), peak=78.6,
+                        second=dict(kernel="k_band_hemm (X = A22 V, FP64 products)", ms=out[5],
+                                    flops=out[6], launches=int(out[7])))
+        return dict(kind="bytes", kernel="kb_hemv (every variant the tuner picks)", ms=out[0],
+                    bytes=out[1], launches=int(out[2]), hemv_ms=out[0], hemv_bytes=out[1],
+                    hemv_launches=int(out[2]))
 
     def stage_times(self):
         out = np.zeros(len(STAGES))

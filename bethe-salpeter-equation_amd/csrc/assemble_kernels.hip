@@ -21,6 +21,7 @@
 // Four coefficient arrays (Cl, Cr, Vl, Vr) are separate inputs so the same
 // kernel evaluates product-rule derivative terms.
 #include "bse_common.hpp"
+#include <algorithm>
 
 struct AsmArgs {
     int nk, n1, n2, n3, nv, nc, norb, TK;
@@ -47,11 +48,19 @@ __device__ inline void tri_index(int64_t idx, int& ti, int& tj) {
     tj = (int)(idx - r * (r + 1) / 2);
 }
 
-template <typename T>
+// CPL = true: the coupling block B of the full (non-Tamm-Dancoff) BSE instead
+// (add_coupling_part, 2929-3040): the primed conduction and valence states
+// trade places in the direct term,
+//   Bd_rs = -(1/(A Nk)) sum_ij conj(c_{ck,i}) c_{v',k'-q,i} W_ij(k - k')
+//                              c_{v,k-q,j} conj(c_{c'k',j}),
+// and the exchange is sum_i (P V)_{r,i} P_{s,i} (P V P^T, no conjugate);
+// no diagonal.  B is complex symmetric, so its lower triangle is stored.
+template <typename T, bool CPL>
 __global__ __launch_bounds__(256) void k_assemble_tile(AsmArgs a, T* __restrict__ H, int lower_only)
 {
     extern __shared__ double2 smem[];
     const int B = a.nc * a.nv, ns = a.nv + a.nc, no = a.norb, TK = a.TK;
+    const int np = CPL ? a.nv : a.nc;   // primed partner of c in the pair intermediates
     const int TR = TK * B;
     int ti, tj;
     if (lower_only) {
@@ -67,8 +76,8 @@ __global__ __launch_bounds__(256) void k_assemble_tile(AsmArgs a, T* __restrict_
     double2* sCr = sCl + TK * a.nc * no;               // TK x nc x no  (col k')
     double2* sVl = sCr + TK * a.nc * no;               // TK x nv x no
     double2* sVr = sVl + TK * a.nv * no;               // TK x nv x no
-    double2* sA = sVr + TK * a.nv * no;                // TK*TK x nc x nc x no
-    double2* sPV = sA + TK * TK * a.nc * a.nc * no;    // TR x no
+    double2* sA = sVr + TK * a.nv * no;                // TK*TK x nc x np x no
+    double2* sPV = sA + TK * TK * a.nc * np * no;      // TR x no
     double2* sP = sPV + TR * no;                       // TR x no
     const int tid = threadIdx.x;
 
@@ -93,20 +102,20 @@ __global__ __launch_bounds__(256) void k_assemble_tile(AsmArgs a, T* __restrict_
     }
     __syncthreads();
 
-    // Pair intermediates Apair[p][c][c'][j], p = kk*TK + kq.
-    const int nA = TK * TK * a.nc * a.nc * no;
+    // Pair intermediates Apair[p][c][c'][j] (coupling: [p][c][v'][j]), p = kk*TK + kq.
+    const int nA = TK * TK * a.nc * np * no;
     for (int e = tid; e < nA; e += blockDim.x) {
         const int j = e % no;
-        const int cp = (e / no) % a.nc;
-        const int c = (e / (no * a.nc)) % a.nc;
-        const int p = e / (no * a.nc * a.nc);
+        const int cp = (e / no) % np;
+        const int c = (e / (no * np)) % a.nc;
+        const int p = e / (no * np * a.nc);
         const int kk = p / TK, kq = p % TK;
         const int k = k0 + kk, kp = kp0 + kq;
         double2 acc = cmk(0, 0);
         if (k < a.nk && kp < a.nk) {
             const double2* wq = a.wtab + (size_t)ktable_index(k, kp, a.n1, a.n2, a.n3) * no * no;
             const double2* cl = sCl + (kk * a.nc + c) * no;
-            const double2* cr = sCr + (kq * a.nc + cp) * no;
+            const double2* cr = CPL ? sVr + (kq * a.nv + cp) * no : sCr + (kq * a.nc + cp) * no;
             for (int i = 0; i < no; ++i) {
                 const double2 x = cmulc(cl[i], cr[i]);
                 cfma(acc, x, wq[i * no + j]);
@@ -123,9 +132,9 @@ __global__ __launch_bounds__(256) void k_assemble_tile(AsmArgs a, T* __restrict_
         if (grow >= a.N || gcol >= a.N) continue;
         const int kk = rr / B, b = rr % B, kq = cc / B, bp = cc % B;
         const int c = b / a.nv, v = b % a.nv, cp = bp / a.nv, vp = bp % a.nv;
-        const double2* Ap = sA + (((kk * TK + kq) * a.nc + c) * a.nc + cp) * no;
+        const double2* Ap = sA + (((kk * TK + kq) * a.nc + c) * np + (CPL ? vp : cp)) * no;
         const double2* vl = sVl + (kk * a.nv + v) * no;
-        const double2* vr = sVr + (kq * a.nv + vp) * no;
+        const double2* vr = CPL ? sCr + (kq * a.nc + cp) * no : sVr + (kq * a.nv + vp) * no;
         double2 d = cmk(0, 0);
         for (int j = 0; j < no; ++j) {
             const double2 y = cmul(vl[j], cconj(vr[j]));
@@ -136,19 +145,20 @@ __global__ __launch_bounds__(256) void k_assemble_tile(AsmArgs a, T* __restrict_
             double2 x = cmk(0, 0);
             const double2* pv = sPV + rr * no;
             const double2* pc = sP + cc * no;
-            for (int j = 0; j < no; ++j) cfma(x, pv[j], cconj(pc[j]));
+            for (int j = 0; j < no; ++j) cfma(x, pv[j], CPL ? pc[j] : cconj(pc[j]));
             h = cadd(h, cscale(x, a.beta));
         }
-        if (grow == gcol && a.dE != nullptr) h.x += a.dE[grow];
+        if (!CPL && grow == gcol && a.dE != nullptr) h.x += a.dE[grow];
         if (grow >= gcol) cplx<T>::store(H + tix(grow, gcol, a.nt), h);
     }
 }
 
-// LDS bytes for a given tile.
+// LDS bytes for a given tile (the coupling form's pair intermediates are
+// nc x nv instead of nc x nc; the larger of the two is reserved).
 extern "C" size_t bse_assemble_lds_bytes(int TK, int nv, int nc, int norb) {
     const int B = nv * nc;
     size_t n = (size_t)TK * nc * norb * 2 + (size_t)TK * nv * norb * 2 +
-               (size_t)TK * TK * nc * nc * norb + (size_t)2 * TK * B * norb;
+               (size_t)TK * TK * nc * std::max(nc, nv) * norb + (size_t)2 * TK * B * norb;
     return n * sizeof(double2);
 }
 
@@ -160,12 +170,12 @@ extern "C" int bse_assemble_pick_tk(int nv, int nc, int norb) {
     return tk;
 }
 
-template <typename T>
+template <typename T, bool CPL = false>
 static hipError_t launch_assemble(hipStream_t st, const AsmArgs& a, T* H, int lower_only) {
     const int64_t nt = ceil_div64(a.nk, a.TK);
     const int64_t nblocks = lower_only ? nt * (nt + 1) / 2 : nt * nt;
     const size_t lds = bse_assemble_lds_bytes(a.TK, a.nv, a.nc, a.norb);
-    hipLaunchKernelGGL(k_assemble_tile<T>, dim3((unsigned)nblocks), dim3(256), lds, st, a, H,
+    hipLaunchKernelGGL((k_assemble_tile<T, CPL>), dim3((unsigned)nblocks), dim3(256), lds, st, a, H,
                        lower_only);
     return hipGetLastError();
 }
@@ -189,13 +199,31 @@ extern "C" hipError_t bse_launch_assemble(hipStream_t st, int store_fp32, int nk
     return launch_assemble<double2>(st, a, (double2*)H, lower_only);
 }
 
+// Coupling block B (FP64, lower triangle in tiled storage of order N): the
+// direct term with alpha, the exchange P V P^T with beta (PVr = P V, Pc = P).
+extern "C" hipError_t bse_launch_assemble_coupling(hipStream_t st, int nk, int n1, int n2, int n3, int nv,
+                                                   int nc, int norb, int TK, const double2* coef,
+                                                   const double2* wtab, const double2* PVr,
+                                                   const double2* Pc, double alpha, double beta, double2* Bm)
+{
+    AsmArgs a;
+    a.nk = nk; a.n1 = n1; a.n2 = n2; a.n3 = n3; a.nv = nv; a.nc = nc; a.norb = norb; a.TK = TK;
+    a.N = (int64_t)nk * nv * nc;
+    a.nt = tile_count(a.N);
+    a.Cl = coef; a.Cr = coef; a.Vl = coef; a.Vr = coef; a.wtab = wtab; a.PVr = PVr; a.Pc = Pc;
+    a.dE = nullptr; a.alpha = alpha; a.beta = beta;
+    return launch_assemble<double2, true>(st, a, Bm, 1);
+}
+
 // ---------------------------------------------------------------------------
 // Copy-out: internal k-major tiled lower storage -> full Hermitian matrix
 // (column-major, ldd) in reference or internal order.  dst element type
 // matches src.
 // ---------------------------------------------------------------------------
 // kinv (may be null): caller's k-list position -> grid index.
-template <typename T>
+// SYM = true: the stored triangle belongs to a complex symmetric matrix (the
+// coupling block), mirrored without conjugation.
+template <typename T, bool SYM = false>
 __global__ __launch_bounds__(256) void k_expand(int64_t N, int B, int nk, const T* __restrict__ src,
                                                 T* __restrict__ dst, int64_t ldd, int ref_order,
                                                 const int* __restrict__ kinv)
@@ -217,10 +245,20 @@ __global__ __launch_bounds__(256) void k_expand(int64_t N, int B, int nk, const 
     } else {
         T u = src[tix(si, ri, nt)];
         v = u;
-        v.y = -u.y;
+        if (!SYM) v.y = -u.y;
     }
-    if (ri == si) v.y = 0;
+    if (!SYM && ri == si) v.y = 0;
     dst[r + s * ldd] = v;
+}
+
+// The coupling block (FP64, complex symmetric) to a dense column-major matrix.
+extern "C" hipError_t bse_launch_expand_sym(hipStream_t st, int64_t N, int B, int nk, const double2* src,
+                                            double2* dst, int64_t ldd, int ref_order, const int* kinv)
+{
+    dim3 grid((unsigned)ceil_div64(N, 256), (unsigned)N);
+    hipLaunchKernelGGL((k_expand<double2, true>), grid, dim3(256), 0, st, N, B, nk, src, dst, ldd, ref_order,
+                       kinv);
+    return hipGetLastError();
 }
 
 extern "C" hipError_t bse_launch_expand(hipStream_t st, int fp32, int64_t N, int B, int nk,

@@ -114,28 +114,40 @@ __device__ void lds_house_qr(int rows, int b, double2* A, int lda, double2* tau,
             tau[c] = tc;
         }
         __syncthreads();
-        // columns j > c:  A(c:, j) -= conj(tc) v (v^H A(c:, j)),  v(c) = 1
+        // columns j > c:  A(c:, j) -= conj(tc) v (v^H A(c:, j)),  v(c) = 1;
+        // the loops are unrolled over the at most TSQR_MAXROWS / 8 rows of a
+        // thread so that its LDS reads are issued together
+        constexpr int RT = TSQR_MAXROWS / 8;
         const int nc = b - c - 1;
         const int jj = tid & 31, g = tid >> 5;
+        const bool act = jj < nc && (tc.x != 0.0 || tc.y != 0.0);
+        double2 vr[RT];
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const int i = c + g + 8 * t;
+            vr[t] = (act && i < rows) ? ((i == c) ? cmk(1.0, 0.0) : A[i + c * lda]) : cmk(0, 0);
+        }
         double2 acc = cmk(0, 0);
-        if (jj < nc && (tc.x != 0.0 || tc.y != 0.0)) {
+        if (act) {
             const int j = c + 1 + jj;
-            for (int i = c + g; i < rows; i += 8) {
-                const double2 v = (i == c) ? cmk(1.0, 0.0) : A[i + c * lda];
-                cfmac(acc, v, A[i + j * lda]);
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                const int i = c + g + 8 * t;
+                if (i < rows) cfmac(acc, vr[t], A[i + j * lda]);
             }
         }
         wbuf[g * 32 + jj] = acc;
         __syncthreads();
-        if (jj < nc && (tc.x != 0.0 || tc.y != 0.0)) {
+        if (act) {
             double2 w = wbuf[jj];
 #pragma unroll
             for (int q = 1; q < 8; ++q) w = cadd(w, wbuf[q * 32 + jj]);
             const double2 f = cmul(cconj(tc), w);
             const int j = c + 1 + jj;
-            for (int i = c + g; i < rows; i += 8) {
-                const double2 v = (i == c) ? cmk(1.0, 0.0) : A[i + c * lda];
-                A[i + j * lda] = csub(A[i + j * lda], cmul(v, f));
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                const int i = c + g + 8 * t;
+                if (i < rows) A[i + j * lda] = csub(A[i + j * lda], cmul(vr[t], f));
             }
         }
         __syncthreads();
@@ -149,14 +161,24 @@ __device__ void lds_apply_q(int rows, int k, const double2* V, int ldv, const do
 {
     const int tid = threadIdx.x;
     const int jj = tid & 31, g = tid >> 5;
+    // a thread's reflector entries (rows c + g + 8 t) are loaded into
+    // registers at once per reflector: V may live in global memory
+    constexpr int RT = TSQR_MAXROWS / 8;
     for (int c = k - 1; c >= 0; --c) {
         const double2 tc = tau[c];
         if (tc.x == 0.0 && tc.y == 0.0) continue;   // uniform: tau is shared
+        double2 vr[RT];
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const int i = c + g + 8 * t;
+            vr[t] = (i < rows) ? ((i == c) ? cmk(1.0, 0.0) : V[i + (int64_t)c * ldv]) : cmk(0, 0);
+        }
         double2 acc = cmk(0, 0);
         if (jj < nc) {
-            for (int i = c + g; i < rows; i += 8) {
-                const double2 v = (i == c) ? cmk(1.0, 0.0) : V[i + (int64_t)c * ldv];
-                cfmac(acc, v, Y[i + jj * ldy]);
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                const int i = c + g + 8 * t;
+                if (i < rows) cfmac(acc, vr[t], Y[i + jj * ldy]);
             }
         }
         wbuf[g * 32 + jj] = acc;
@@ -166,9 +188,10 @@ __device__ void lds_apply_q(int rows, int k, const double2* V, int ldv, const do
 #pragma unroll
             for (int q = 1; q < 8; ++q) w = cadd(w, wbuf[q * 32 + jj]);
             const double2 f = cmul(tc, w);
-            for (int i = c + g; i < rows; i += 8) {
-                const double2 v = (i == c) ? cmk(1.0, 0.0) : V[i + (int64_t)c * ldv];
-                Y[i + jj * ldy] = csub(Y[i + jj * ldy], cmul(v, f));
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                const int i = c + g + 8 * t;
+                if (i < rows) Y[i + jj * ldy] = csub(Y[i + jj * ldy], cmul(vr[t], f));
             }
         }
         __syncthreads();
@@ -178,6 +201,9 @@ __device__ void lds_apply_q(int rows, int k, const double2* V, int ldv, const do
 // Leaf rows of a TSQR of an m-row panel: nblk = max(1, m / RB) blocks of RB
 // rows, the last one taking the remainder.
 __host__ __device__ inline int tsqr_nblk(int m) { return m < TSQR_RB ? 1 : m / TSQR_RB; }
+// LDS leading dimension of an in-LDS panel of `rows` rows: odd, so that the
+// 32 columns a wavefront reads at one row fall in different banks.
+__host__ __device__ inline int tsqr_ld(int rows) { return rows | 1; }
 __host__ __device__ inline int tsqr_r0(int blk) { return blk * TSQR_RB; }
 __host__ __device__ inline int tsqr_r1(int blk, int nblk, int m) { return blk == nblk - 1 ? m : (blk + 1) * TSQR_RB; }
 
@@ -193,20 +219,21 @@ __global__ __launch_bounds__(256) void k_tsqr_leaf(int m, int b, double2* __rest
     const int nblk = tsqr_nblk(m);
     const int blk = blockIdx.x;
     const int r0 = tsqr_r0(blk), rows = tsqr_r1(blk, nblk, m) - r0;
+    const int ld = tsqr_ld(rows);
     for (int e = threadIdx.x; e < rows * b; e += 256) {
         const int i = e % rows, j = e / rows;
-        sA[i + j * rows] = P[(r0 + i) + (int64_t)j * m];
+        sA[i + j * ld] = P[(r0 + i) + (int64_t)j * m];
     }
     if (threadIdx.x < BR_B) stau[threadIdx.x] = cmk(0, 0);
     __syncthreads();
-    lds_house_qr(rows, b, sA, rows, stau, red, wbuf);
+    lds_house_qr(rows, b, sA, ld, stau, red, wbuf);
     for (int e = threadIdx.x; e < rows * b; e += 256) {
         const int i = e % rows, j = e / rows;
-        if (i > j) P[(r0 + i) + (int64_t)j * m] = sA[i + j * rows];
+        if (i > j) P[(r0 + i) + (int64_t)j * m] = sA[i + j * ld];
     }
     for (int e = threadIdx.x; e < b * b; e += 256) {
         const int i = e % b, j = e / b;
-        R[(int64_t)blk * b * b + e] = (i <= j && i < rows) ? sA[i + j * rows] : cmk(0, 0);
+        R[(int64_t)blk * b * b + e] = (i <= j && i < rows) ? sA[i + j * ld] : cmk(0, 0);
     }
     if (threadIdx.x < b) tau[(int64_t)blk * b + threadIdx.x] = stau[threadIdx.x];
 }
@@ -224,19 +251,23 @@ __global__ __launch_bounds__(256) void k_tsqr_node(int nin, int b, const double2
     const int g = blockIdx.x;
     const int cnt = min(TSQR_G, nin - g * TSQR_G);
     const int rows = cnt * b;
+    const int ld = tsqr_ld(rows);
     for (int e = threadIdx.x; e < rows * b; e += 256) {
         const int i = e % rows, j = e / rows;
         const int t = i / b, ii = i % b;
-        sA[i + j * rows] = Rin[(int64_t)(g * TSQR_G + t) * b * b + ii + j * b];
+        sA[i + j * ld] = Rin[(int64_t)(g * TSQR_G + t) * b * b + ii + j * b];
     }
     if (threadIdx.x < BR_B) stau[threadIdx.x] = cmk(0, 0);
     __syncthreads();
-    lds_house_qr(rows, b, sA, rows, stau, red, wbuf);
+    lds_house_qr(rows, b, sA, ld, stau, red, wbuf);
     double2* V = Vn + (int64_t)g * TSQR_G * b * b;
-    for (int e = threadIdx.x; e < rows * b; e += 256) V[e] = sA[e];
+    for (int e = threadIdx.x; e < rows * b; e += 256) {
+        const int i = e % rows, j = e / rows;
+        V[e] = sA[i + j * ld];
+    }
     for (int e = threadIdx.x; e < b * b; e += 256) {
         const int i = e % b, j = e / b;
-        Rout[(int64_t)g * b * b + e] = (i <= j) ? sA[i + j * rows] : cmk(0, 0);
+        Rout[(int64_t)g * b * b + e] = (i <= j) ? sA[i + j * ld] : cmk(0, 0);
     }
     if (threadIdx.x < b) taun[(int64_t)g * b + threadIdx.x] = stau[threadIdx.x];
 }
@@ -254,19 +285,20 @@ __global__ __launch_bounds__(256) void k_tsqr_node_q(int nin, int b, int k, cons
     const int g = blockIdx.x;
     const int cnt = min(TSQR_G, nin - g * TSQR_G);
     const int rows = cnt * b;
+    const int ld = tsqr_ld(rows);
     for (int e = threadIdx.x; e < rows * k; e += 256) {
         const int i = e % rows, j = e / rows;
         double2 x = cmk(0, 0);
         if (i < b) x = root ? cmk(i == j ? 1.0 : 0.0, 0.0) : Xin[(int64_t)g * b * k + i + j * b];
-        sY[i + j * rows] = x;
+        sY[i + j * ld] = x;
     }
     if (threadIdx.x < b) stau[threadIdx.x] = taun[(int64_t)g * b + threadIdx.x];
     __syncthreads();
-    lds_apply_q(rows, b, Vn + (int64_t)g * TSQR_G * b * b, rows, stau, sY, rows, k, wbuf);
+    lds_apply_q(rows, b, Vn + (int64_t)g * TSQR_G * b * b, rows, stau, sY, ld, k, wbuf);
     for (int e = threadIdx.x; e < rows * k; e += 256) {
         const int i = e % rows, j = e / rows;
         const int t = i / b, ii = i % b;
-        Xout[(int64_t)(g * TSQR_G + t) * b * k + ii + j * b] = sY[i + j * rows];
+        Xout[(int64_t)(g * TSQR_G + t) * b * k + ii + j * b] = sY[i + j * ld];
     }
 }
 
@@ -283,18 +315,19 @@ __global__ __launch_bounds__(256) void k_tsqr_leaf_q(int m, int b, int k, const 
     const int blk = blockIdx.x;
     const int r0 = tsqr_r0(blk), rows = tsqr_r1(blk, nblk, m) - r0;
     const int kv = min(rows, b);
+    const int ld = tsqr_ld(rows);
     for (int e = threadIdx.x; e < rows * k; e += 256) {
         const int i = e % rows, j = e / rows;
         double2 x = cmk(0, 0);
         if (i < b) x = root ? cmk(i == j ? 1.0 : 0.0, 0.0) : Xin[(int64_t)blk * b * k + i + j * b];
-        sY[i + j * rows] = x;
+        sY[i + j * ld] = x;
     }
     if (threadIdx.x < b) stau[threadIdx.x] = tau[(int64_t)blk * b + threadIdx.x];
     __syncthreads();
-    lds_apply_q(rows, kv, P + r0, m, stau, sY, rows, k, wbuf);
+    lds_apply_q(rows, kv, P + r0, m, stau, sY, ld, k, wbuf);
     for (int e = threadIdx.x; e < rows * k; e += 256) {
         const int i = e % rows, j = e / rows;
-        Q[(r0 + i) + (int64_t)j * m] = sY[i + j * rows];
+        Q[(r0 + i) + (int64_t)j * m] = sY[i + j * ld];
     }
 }
 
@@ -566,9 +599,20 @@ __global__ __launch_bounds__(256) void k_band_m(int nblk, int k, int b, const do
 {
     __shared__ double2 sG[BR_B][BR_B + 1];
     for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
-        double2 acc = cmk(0, 0);
-        for (int g = 0; g < nblk; ++g) acc = cadd(acc, Gp[(int64_t)g * BR_B * BR_B + e]);
-        sG[e % BR_B][e / BR_B] = acc;
+        // eight independent partial sums keep eight loads in flight; they
+        // are combined in a fixed order
+        double2 acc[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] = cmk(0, 0);
+        int g = 0;
+        for (; g + 8 <= nblk; g += 8) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[q] = cadd(acc[q], Gp[(int64_t)(g + q) * BR_B * BR_B + e]);
+        }
+        for (; g < nblk; ++g) acc[0] = cadd(acc[0], Gp[(int64_t)g * BR_B * BR_B + e]);
+#pragma unroll
+        for (int q = 1; q < 8; ++q) acc[0] = cadd(acc[0], acc[q]);
+        sG[e % BR_B][e / BR_B] = acc[0];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
@@ -654,7 +698,7 @@ extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P
     // the largest in-LDS QR holds TSQR_MAXROWS x 32 complex doubles (128 KB);
     // the limit asked for must leave room for the kernels' static LDS
     static const hipError_t attr = [] {
-        const int big = TSQR_MAXROWS * BR_B * (int)sizeof(double2);
+        const int big = tsqr_ld(TSQR_MAXROWS) * BR_B * (int)sizeof(double2);
         hipError_t e = hipFuncSetAttribute((const void*)k_tsqr_leaf, hipFuncAttributeMaxDynamicSharedMemorySize, big);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void*)k_tsqr_node, hipFuncAttributeMaxDynamicSharedMemorySize, big);
@@ -692,11 +736,11 @@ extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P
     // leaf QRs
     const int maxrows_leaf = (nleaf == 1) ? m : 2 * TSQR_RB - 1;
     if (maxrows_leaf > TSQR_MAXROWS) return hipErrorInvalidValue;
-    const size_t lds_leaf = (size_t)maxrows_leaf * b * sizeof(double2);
+    const size_t lds_leaf = (size_t)tsqr_ld(maxrows_leaf) * b * sizeof(double2);
     hipLaunchKernelGGL(k_tsqr_leaf, dim3((unsigned)nleaf), dim3(256), lds_leaf, st, m, b, P, Rleaf, tauleaf);
     // up the tree
     for (int l = 0; l < nl; ++l) {
-        const size_t lds = (size_t)TSQR_G * b * b * sizeof(double2);
+        const size_t lds = (size_t)tsqr_ld(TSQR_G * b) * b * sizeof(double2);
         hipLaunchKernelGGL(k_tsqr_node, dim3((unsigned)lv[l].nn), dim3(256), lds, st, lv[l].nin, b,
                            l == 0 ? Rleaf : lv[l - 1].R, lv[l].V, lv[l].tau, lv[l].R);
     }
@@ -705,12 +749,12 @@ extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P
     double2* Xcur = Xa;
     double2* Xnext = Xb;
     for (int l = nl - 1; l >= 0; --l) {
-        const size_t lds = (size_t)TSQR_G * b * k * sizeof(double2);
+        const size_t lds = (size_t)tsqr_ld(TSQR_G * b) * k * sizeof(double2);
         hipLaunchKernelGGL(k_tsqr_node_q, dim3((unsigned)lv[l].nn), dim3(256), lds, st, lv[l].nin, b, k,
                            lv[l].V, lv[l].tau, (const double2*)Xcur, l == nl - 1 ? 1 : 0, Xnext);
         std::swap(Xcur, Xnext);
     }
-    hipLaunchKernelGGL(k_tsqr_leaf_q, dim3((unsigned)nleaf), dim3(256), (size_t)maxrows_leaf * k * sizeof(double2),
+    hipLaunchKernelGGL(k_tsqr_leaf_q, dim3((unsigned)nleaf), dim3(256), (size_t)tsqr_ld(maxrows_leaf) * k * sizeof(double2),
                        st, m, b, k, (const double2*)P, (const double2*)tauleaf, (const double2*)Xcur,
                        nl == 0 ? 1 : 0, Q);
     // Householder form
@@ -735,8 +779,11 @@ extern "C" int64_t bse_band_update_workspace(int n, int s, int b) {
 // her2k: the one-stage trailing-update launcher (kind as tuned).
 extern "C" hipError_t bse_launch_her2k(hipStream_t, int, int, int, int, void*, const double2*,
                                        const double2*, int64_t, int, int, const double2*);
+// ev (may be null): four events recorded before and after the Hermitian block
+// product and before and after the rank-2k update, for per-launch timing.
 extern "C" hipError_t bse_band_update(hipStream_t st, int fp32, int kind, int n, int s, int k, int b, void* A,
-                                      const double2* V, const double2* Tp, double2* ws, const double2* zeros64)
+                                      const double2* V, const double2* Tp, double2* ws, const double2* zeros64,
+                                      hipEvent_t* ev)
 {
     const int m = n - s;
     if (m <= 0 || k <= 0) return hipSuccess;
@@ -747,15 +794,20 @@ extern "C" hipError_t bse_band_update(hipStream_t st, int fp32, int kind, int n,
     double2* M = Gp + (int64_t)nblk * BR_B * BR_B;
     double2* Z = M + 2 * BR_B * BR_B;
     const dim3 gh((unsigned)nT, HEMM_KSPLIT);
+    if (ev) (void)hipEventRecord(ev[0], st);
     if (fp32)
         hipLaunchKernelGGL(k_band_hemm<float2>, gh, dim3(256), 0, st, n, s, k, (const float2*)A, V, (int64_t)m, Xp);
     else
         hipLaunchKernelGGL(k_band_hemm<double2>, gh, dim3(256), 0, st, n, s, k, (const double2*)A, V, (int64_t)m, Xp);
+    if (ev) (void)hipEventRecord(ev[1], st);
     hipLaunchKernelGGL(k_band_xt, dim3((unsigned)nblk), dim3(256), 0, st, n, s, k, b, (const double2*)Xp, Tp, V, Z, Gp);
     hipLaunchKernelGGL(k_band_m, dim3(1), dim3(256), 0, st, nblk, k, b, (const double2*)Gp, Tp, M);
     hipLaunchKernelGGL(k_band_z, dim3((unsigned)ceil_div64(m, 256)), dim3(256), 0, st, m, k, b, V,
                        (const double2*)M, Z);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return bse_launch_her2k(st, fp32, kind, n, s, A, V, Z, m, s, k, zeros64);
+    if (ev) (void)hipEventRecord(ev[2], st);
+    e = bse_launch_her2k(st, fp32, kind, n, s, A, V, Z, m, s, k, zeros64);
+    if (ev) (void)hipEventRecord(ev[3], st);
+    return e;
 }

@@ -34,7 +34,7 @@ extern "C" int bse_create(bse_ctx** out, int device) {
         return BSE_ERR_HIP;
     }
     for (int i = 0; i <= ST_N; ++i) (void)hipEventCreate(&ctx->ev[i]);
-    for (int i = 0; i < 2; ++i) (void)hipEventCreate(&ctx->tev[i]);
+    for (int i = 0; i < 3; ++i) (void)hipEventCreate(&ctx->tev[i]);
     (void)hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
     (void)hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming);
     const char* at = getenv("BSE_AUTOTUNE");
@@ -58,7 +58,7 @@ extern "C" int bse_destroy(bse_ctx* ctx) {
     for (int i = 0; i <= ST_N; ++i)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 3; ++i)
         if (ctx->tev[i]) (void)hipEventDestroy(ctx->tev[i]);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
@@ -81,9 +81,21 @@ extern "C" int bse_profile(bse_ctx* ctx, int enable, double* out) {
     if (!ctx) return BSE_ERR_ARG;
     ctx->prof = enable != 0;
     if (out) {
-        out[0] = ctx->hemv_ms;
-        out[1] = ctx->hemv_bytes;
-        out[2] = (double)ctx->hemv_launches;
+        for (int i = 0; i < 9; ++i) out[i] = 0.0;
+        if (ctx->prof_two_stage) {
+            out[0] = ctx->b2k_ms;
+            out[2] = (double)ctx->band_launches;
+            out[3] = ctx->b2k_flops;
+            out[4] = 1.0;
+            out[5] = ctx->hemm_ms;
+            out[6] = ctx->hemm_flops;
+            out[7] = (double)ctx->band_launches;
+            out[8] = 2.0;
+        } else {
+            out[0] = ctx->hemv_ms;
+            out[1] = ctx->hemv_bytes;
+            out[2] = (double)ctx->hemv_launches;
+        }
     }
     return BSE_OK;
 }
@@ -152,6 +164,7 @@ static int load_problem(bse_ctx* ctx, const bse_tb_model* m, const Geom& g, cons
     HCK(hipStreamSynchronize(ctx->st));
     ctx->have_problem = true;
     ctx->have_tb = ctx->have_tables = ctx->built = ctx->have_eig = ctx->have_f = false;
+    ctx->full_built = ctx->full_eig = ctx->have_coupling = false;
     return BSE_OK;
 }
 
@@ -413,6 +426,7 @@ extern "C" int bse_build(bse_ctx* ctx, int precision) {
     ctx->precision = precision;
     ctx->built = true;
     ctx->have_eig = ctx->have_f = false;
+    ctx->full_built = ctx->full_eig = ctx->have_coupling = false;
     return BSE_OK;
 }
 
@@ -615,10 +629,15 @@ static int autotune_her2k(bse_ctx* ctx, int fp32, void* scratch, size_t scratch_
 // ---------------------------------------------------------------------------
 static const int kBand = 32;
 
-static bool use_two_stage(const bse_ctx* ctx, int n, int n_vectors, int max_vectors) {
+// Automatic choice (two_stage == 0): the two-stage reduction for FP64
+// storage from order 256 up when at most 20 vectors are wanted (measured at
+// N = 20736: 5.0 s against 5.9 s for the one-stage lowest-16 path); FP32
+// storage stays one-stage, whose hemv streams half the bytes (3.1 s against
+// 4.9 s).  BSE_TWO_STAGE=1 / =0 forces it on / off for any size.
+static bool use_two_stage(const bse_ctx* ctx, int n, int n_vectors, int max_vectors, int fp32) {
     const int want = (n_vectors < 0) ? n : std::max(n_vectors, max_vectors);
     if (want > 20 || ctx->two_stage < 0 || n < 2) return false;
-    return ctx->two_stage > 0 || n >= 256;
+    return ctx->two_stage > 0 || (n >= 256 && !fp32);
 }
 
 static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double2* rho) {
@@ -655,6 +674,15 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
         HCK(bse_launch_zero_words(st, 128, zeros));
         HCK(hipEventRecord(tv[0], st));
         const int kind = ctx->her2k_valu[fp32] >= 0 ? ctx->her2k_valu[fp32] : 1;
+        hipEvent_t* pev = nullptr;
+        if (ctx->prof) {
+            while ((int64_t)ctx->prof_ev.size() < 4 * (int64_t)npan) {
+                hipEvent_t ev;
+                HCK(hipEventCreate(&ev));
+                ctx->prof_ev.push_back(ev);
+            }
+            pev = ctx->prof_ev.data();
+        }
         for (int p = 0; p < npan; ++p) {
             const bse_ctx::BandPanel& bp = ctx->band_panels[p];
             double2* Vp = Vst + bp.voff;
@@ -662,7 +690,7 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
             int k = 0;
             HCK(bse_band_gather(st, fp32, n, p * b, b, H, P, AB, ldab));
             HCK(bse_band_panel_qr(st, bp.m, b, P, Vp, Tp, tw, AB, ldab, p * b, &k));
-            HCK(bse_band_update(st, fp32, kind, n, bp.s, k, b, H, Vp, Tp, uw, zeros));
+            HCK(bse_band_update(st, fp32, kind, n, bp.s, k, b, H, Vp, Tp, uw, zeros, pev ? pev + 4 * p : nullptr));
         }
     }
     if (npan == 0) HCK(hipEventRecord(tv[0], st));
@@ -684,13 +712,45 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
     HCK(bse_launch_zero_words(st, (ncnt + 4) / 2, prog));
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) ncu = 64;
-    HCK(bse_launch_chase(st, n, b, AB, ldab, y, d, e, prog, prog + ncnt, std::max(1, ncu - 16)));
+    // one workgroup per CU at most (all resident, so every waiter's
+    // predecessor runs); BSE_CHASE_GRID sets the count for measurements
+    int cgrid = std::max(1, ncu - 16);
+    if (const char* g = getenv("BSE_CHASE_GRID")) cgrid = std::max(1, std::min(atoi(g), ncu));
+    HCK(bse_launch_chase(st, n, b, AB, ldab, y, d, e, prog, prog + ncnt, cgrid));
+    HCK(hipEventRecord(tv[2], st));
     int err = 0;
     HCK(hipMemcpyAsync(&err, prog + ncnt, sizeof(int), hipMemcpyDeviceToHost, st));
     HCK(hipStreamSynchronize(st));
-    float ms1 = 0.f;
+    float ms1 = 0.f, ms2 = 0.f;
     HCK(hipEventElapsedTime(&ms1, tv[0], tv[1]));
+    HCK(hipEventElapsedTime(&ms2, tv[1], tv[2]));
     ctx->band_ms = ms1;
+    ctx->chase_ms = ms2;
+    ctx->stage_ms[ST_BAND] = ms1;
+    ctx->stage_ms[ST_CHASE] = ms2;
+    if (ctx->prof && npan > 0) {
+        // per-launch times of the two panel-update kernels; algorithmic
+        // flops: X = A22 V costs 8 m^2 k, A22 -= V Z^H + Z V^H on the lower
+        // triangle 8 m (m + 1) k (complex multiply-add = 8 real flops)
+        double t2k = 0.0, f2k = 0.0, thm = 0.0, fhm = 0.0;
+        for (int p = 0; p < npan; ++p) {
+            const bse_ctx::BandPanel& bp = ctx->band_panels[p];
+            float a = 0.f, c = 0.f;
+            HCK(hipEventElapsedTime(&a, ctx->prof_ev[4 * p], ctx->prof_ev[4 * p + 1]));
+            HCK(hipEventElapsedTime(&c, ctx->prof_ev[4 * p + 2], ctx->prof_ev[4 * p + 3]));
+            thm += a;
+            t2k += c;
+            const double m = bp.m, k = bp.k;
+            fhm += 8.0 * m * m * k;
+            f2k += 8.0 * m * (m + 1.0) * k;
+        }
+        ctx->hemm_ms = thm;
+        ctx->hemm_flops = fhm;
+        ctx->b2k_ms = t2k;
+        ctx->b2k_flops = f2k;
+        ctx->band_launches = npan;
+        ctx->prof_two_stage = true;
+    }
     if (err) return fail(ctx, BSE_ERR_NUMERIC, "band to tridiagonal: a sweep waited too long on its predecessor");
     return BSE_OK;
 }
@@ -718,15 +778,20 @@ static int twostage_vectors(bse_ctx* ctx, int n, const double* d_lam, int nstore
 
 // guard_tol > 0: eigenvalues within guard_tol above the last requested one
 // are formed as well, up to max_vectors in total (ctx->nvec reports the count).
+// vec_first > 0: the vectors formed are those of the sorted eigenvalues
+// vec_first .. vec_first + n_vectors - 1 (one-stage reduction, no guard).
 static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
                         const double2* rho, int n_vectors, double guard_tol = 0.0,
-                        int max_vectors = 0)
+                        int max_vectors = 0, int vec_first = 0)
 {
     const int n = (int)n64;
     const int nb = kPanel;
     hipStream_t st = ctx->st;
-    const bool two = use_two_stage(ctx, n, n_vectors, max_vectors);
+    if (vec_first < 0 || vec_first > n) return fail(ctx, BSE_ERR_ARG, "first vector index out of range");
+    const bool two = vec_first == 0 && use_two_stage(ctx, n, n_vectors, max_vectors, fp32);
     ctx->last_two_stage = two;
+    ctx->stage_ms[ST_BAND] = ctx->stage_ms[ST_CHASE] = 0.0;
+    if (ctx->prof) ctx->prof_two_stage = false;
     // ---------------- K5: tridiagonalisation
     int64_t vs_el = 1, yr_el = 1, yc_el = 1, part_d = 1;
     std::vector<int64_t> voff;
@@ -920,9 +985,9 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
 
     // ---------------- K6: tridiagonal eigenvectors by batches
     stage_begin(ctx, ST_TVEC);
-    int nstore = (n_vectors < 0) ? n : std::min(n_vectors, n);
+    int nstore = (n_vectors < 0) ? n - vec_first : std::min(n_vectors, n - vec_first);
     if (guard_tol < 0.0) guard_tol = 1e-4 * tnorm;
-    if (nstore > 0 && nstore < n && guard_tol > 0.0) {
+    if (vec_first == 0 && nstore > 0 && nstore < n && guard_tol > 0.0) {
         // guard vectors: eigenvalues just above the last requested one that
         // lie within guard_tol of it (a cluster straddling the cut) are formed
         // too, so that a refinement sees the whole cluster
@@ -993,8 +1058,11 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
             else
                 HCK(bse_launch_contract(st, bsz, cnt, m0, d_bsv, d_nbv, xb, y, fvec));
         }
-        const int ncommit = two ? 0 : std::max(0, std::min(m1, nstore) - m0);
-        if (ncommit > 0) HCK(bse_launch_commit(st, n, bsz, cnt, ncommit, m0, d_bsv, d_nbv, xb, Z, n, m0));
+        // commit the batch's members of [vec_first, vec_first + nstore)
+        const int lo = std::max(m0, vec_first), hi = std::min(m1, vec_first + nstore);
+        if (!two && hi > lo)
+            HCK(bse_launch_commit(st, n, bsz, cnt - (lo - m0), hi - lo, lo, d_bsv, d_nbv, xb + (lo - m0), Z, n,
+                                  lo - vec_first));
         HCK(hipStreamSynchronize(st));   // keeps d_cl and the host vectors alive
         m0 = m1;
     }
@@ -1402,6 +1470,7 @@ extern "C" int bse_eigh(bse_ctx* ctx, int n_vectors, int refine) {
     (void)hipSetDevice(ctx->device);
     const int fp32 = ctx->precision != BSE_PREC_FP64;
     ctx->eig_on_bse = true;
+    ctx->full_eig = false;
     ctx->ref_rnorm.clear();
     ctx->ref_rounds = 0;
     ctx->ref_status = BSE_OK;
@@ -1419,6 +1488,149 @@ extern "C" int bse_eigh(bse_ctx* ctx, int n_vectors, int refine) {
     rc = refine_lowest(ctx, n_vectors, ctx->nvec);
     stage_end(ctx, ST_REF);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Full BSE beyond the Tamm-Dancoff approximation: the coupling block
+// (add_coupling_part, 2929-3040) and a structure-preserving solve of
+// H = [[A, B], [-conj B, -conj A]] in place of eig_gen (3326): Cholesky of
+// M = [[A, B], [conj B, conj A]] = L L^H, the Hermitian W = L^H s3 L through
+// the ordinary eigen-pipeline with z = L^-1 rho~ as the contraction vector,
+// then E > 0 with f = sqrt(E) z^H y and x = sqrt(E) L^-H y (full_bse.hip).
+// ---------------------------------------------------------------------------
+extern "C" int bse_build_full(bse_ctx* ctx) {
+    if (!ctx) return BSE_ERR_ARG;
+    int rc = bse_build(ctx, BSE_PREC_FP64);
+    if (rc) return rc;
+    const bse_system& s = ctx->sys;
+    const int64_t N = ctx->N;
+    hipStream_t st = ctx->st;
+    double2* Bm = BUF(double2, S_BCPL, tiled_elems(N));
+    HCK(hipMemsetAsync(Bm, 0, sizeof(double2) * tiled_elems(N), st));
+    const double alpha = -s.direct_scale / (ctx->cell * ctx->nk);
+    const double beta = s.f_s * s.exchange_scale / (ctx->cell * ctx->nk);
+    const int tk = bse_assemble_pick_tk(s.nv, s.nc, ctx->norb);
+    HCK(bse_launch_assemble_coupling(st, ctx->nk, s.n1, s.n2, ctx->n3, s.nv, s.nc, ctx->norb, tk,
+                                     (const double2*)ctx->bufs[S_COEF].p, (const double2*)ctx->bufs[S_WTAB].p,
+                                     (const double2*)ctx->bufs[S_PV].p, (const double2*)ctx->bufs[S_P].p, alpha,
+                                     beta, Bm));
+    double2* M = BUF(double2, S_MFULL, tiled_elems(2 * N));
+    HCK(bse_launch_compose_full(st, N, (const double2*)ctx->bufs[S_H].p, Bm, M));
+    HCK(hipStreamSynchronize(st));
+    ctx->full_built = true;
+    ctx->have_coupling = true;
+    return BSE_OK;
+}
+
+extern "C" int bse_get_coupling(bse_ctx* ctx, double* dst, int64_t ld) {
+    if (!ctx || !dst) return BSE_ERR_ARG;
+    if (!ctx->have_coupling) return fail(ctx, BSE_ERR_STATE, "coupling block not built (bse_build_full)");
+    const int64_t N = ctx->N;
+    if (ld < N) return fail(ctx, BSE_ERR_ARG, "ld < N");
+    (void)hipSetDevice(ctx->device);
+    double2* tmp = BUF(double2, S_W2, (size_t)ld * N);
+    HCK(bse_launch_expand_sym(ctx->st, N, ctx->B, ctx->nk, (const double2*)ctx->bufs[S_BCPL].p, tmp, ld, 1,
+                              kinv_dev(ctx)));
+    HCK(hipMemcpyAsync(dst, tmp, sizeof(double2) * (size_t)ld * N, hipMemcpyDeviceToHost, ctx->st));
+    HCK(hipStreamSynchronize(ctx->st));
+    return BSE_OK;
+}
+
+extern "C" int bse_eigh_full(bse_ctx* ctx, int n_vectors) {
+    if (!ctx) return BSE_ERR_ARG;
+    if (!ctx->full_built) return fail(ctx, BSE_ERR_STATE, "full BSE matrix not built (bse_build_full)");
+    const int64_t N = ctx->N;
+    if (n_vectors < -1 || n_vectors > N) return fail(ctx, BSE_ERR_ARG, "n_vectors out of range");
+    if (2 * N > (int64_t)1 << 30) return fail(ctx, BSE_ERR_UNSUPPORTED, "full BSE order too large");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->st;
+    const int n2 = (int)(2 * N), nt2 = tile_count(n2);
+    const int nvec = n_vectors < 0 ? (int)N : n_vectors;
+    ctx->full_built = false;                 // M is overwritten by its Cholesky factor
+    ctx->full_eig = false;
+    ctx->have_eig = ctx->have_f = false;
+    double2* M = (double2*)ctx->bufs[S_MFULL].p;
+    double2* Linv = BUF(double2, S_LINV, (size_t)nt2 * 64 * 64);
+    int* ferr = BUF(int, S_FERR, 4);
+    HCK(bse_launch_zero_words(st, 2, ferr));
+    const int64_t pm = std::max<int64_t>(n2 - 64, 1);
+    double2* P = BUF(double2, S_FPAN, (size_t)pm * 64);
+    double2* Ph = BUF(double2, S_FPANH, (size_t)pm * 64);
+    double2* zeros = BUF(double2, S_BZERO, 64);
+    HCK(bse_launch_zero_words(st, 128, zeros));
+    const int kind = ctx->her2k_valu[0] >= 0 ? ctx->her2k_valu[0] : 1;
+    HCK(hipEventRecord(ctx->tev[0], st));
+    for (int J = 0; J < nt2; ++J) {
+        HCK(bse_launch_potrf_tile(st, n2, J, M, Linv, ferr));
+        HCK(bse_launch_trsm_panel(st, n2, J, M, Linv, P, Ph));
+        const int s0 = (J + 1) * 64;
+        if (s0 < n2) {
+            const int m = n2 - s0;
+            // A22 -= P P^H as two rank-64 halves of P Ph^H + Ph P^H, Ph = P / 2
+            HCK(bse_launch_her2k(st, 0, kind, n2, s0, M, P, Ph, m, s0, 32, zeros));
+            HCK(bse_launch_her2k(st, 0, kind, n2, s0, M, P + (int64_t)32 * m, Ph + (int64_t)32 * m, m, s0, 32,
+                                 zeros));
+        }
+    }
+    int herr = 0;
+    HCK(hipMemcpyAsync(&herr, ferr, sizeof(int), hipMemcpyDeviceToHost, st));
+    HCK(hipStreamSynchronize(st));
+    if (herr)
+        return fail(ctx, BSE_ERR_NUMERIC,
+                    "full BSE: [[A, B], [conj B, conj A]] is not positive definite (pivot at row " +
+                        std::to_string(herr) + "): the system is unstable beyond Tamm-Dancoff");
+    double2* W = BUF(double2, S_WFULL, tiled_elems(n2));
+    HCK(bse_launch_lsl(st, N, M, W));
+    const int64_t npad = (int64_t)nt2 * 64;
+    double2* rt = BUF(double2, S_RHOT, npad);
+    double2* z = BUF(double2, S_ZT, npad);
+    HCK(bse_launch_full_rho(st, N, (const double2*)ctx->bufs[S_RHO].p, rt, npad));
+    HCK(bse_launch_lower_solve(st, n2, M, Linv, rt, z));
+    HCK(hipEventRecord(ctx->tev[1], st));
+    HCK(hipEventSynchronize(ctx->tev[1]));
+    float ms = 0.f;
+    HCK(hipEventElapsedTime(&ms, ctx->tev[0], ctx->tev[1]));
+    ctx->eig_on_bse = false;
+    int rc = eig_pipeline(ctx, n2, W, 0, z, nvec, 0.0, 0, (int)N);
+    if (rc) {
+        (void)hipStreamSynchronize(ctx->side);
+        return rc;
+    }
+    ctx->stage_ms[ST_CHOL] = ms;
+    // the spectrum of W is {-E, +E}: keep the positive half
+    std::vector<double> lam(n2);
+    std::vector<double2> fp(n2);
+    HCK(hipMemcpyAsync(lam.data(), ctx->bufs[S_EVAL].p, sizeof(double) * n2, hipMemcpyDeviceToHost, st));
+    HCK(hipMemcpyAsync(fp.data(), ctx->bufs[S_F].p, sizeof(double2) * n2, hipMemcpyDeviceToHost, st));
+    HCK(hipStreamSynchronize(st));
+    double emax = 0.0, pair = 0.0;
+    for (int i = 0; i < n2; ++i) emax = std::max(emax, fabs(lam[i]));
+    for (int64_t i = 0; i < N; ++i) pair = std::max(pair, fabs(lam[N + i] + lam[N - 1 - i]));
+    if (!(lam[N] > 0.0) || !(lam[N - 1] < 0.0) || pair > 1e-8 * emax)
+        return fail(ctx, BSE_ERR_NUMERIC, "full BSE: the spectrum of L^H s3 L is not paired +-E");
+    std::vector<double> E(N), sc(std::max(nvec, 1));
+    std::vector<double2> f(N);
+    for (int64_t i = 0; i < N; ++i) {
+        E[i] = lam[N + i];
+        f[i] = cscale(fp[N + i], sqrt(E[i]));
+    }
+    for (int j = 0; j < nvec; ++j) sc[j] = sqrt(E[j]);
+    HCK(hipMemcpyAsync(ctx->bufs[S_EVAL].p, E.data(), sizeof(double) * N, hipMemcpyHostToDevice, st));
+    HCK(hipMemcpyAsync(ctx->bufs[S_F].p, f.data(), sizeof(double2) * N, hipMemcpyHostToDevice, st));
+    if (nvec > 0) {
+        double* dsc = BUF(double, S_FSCALE, nvec);
+        double2* X = BUF(double2, S_XFULL, (size_t)npad * nvec);
+        HCK(hipMemcpyAsync(dsc, sc.data(), sizeof(double) * nvec, hipMemcpyHostToDevice, st));
+        HCK(bse_launch_upper_solve(st, n2, nvec, M, Linv, (const double2*)ctx->bufs[S_Z].p, n2, X, npad, dsc));
+    }
+    HCK(hipStreamSynchronize(st));
+    ctx->full_ldx = npad;
+    ctx->n_eig = N;
+    ctx->nvec = nvec;
+    ctx->have_eig = ctx->have_f = true;
+    ctx->full_eig = true;
+    ctx->eig_on_bse = true;
+    return BSE_OK;
 }
 
 extern "C" int bse_refine_info(const bse_ctx* ctx, double* rnorm, int n, int* rounds) {
@@ -1440,9 +1652,24 @@ extern "C" int bse_get_eigenvectors(bse_ctx* ctx, double* z, int64_t ld, int nve
     if (!ctx || !z) return BSE_ERR_ARG;
     if (!ctx->have_eig) return fail(ctx, BSE_ERR_STATE, "no eigendecomposition");
     if (nvec < 0 || nvec > ctx->nvec) return fail(ctx, BSE_ERR_ARG, "nvec exceeds the formed eigenvectors");
-    if (ld < ctx->n_eig) return fail(ctx, BSE_ERR_ARG, "ld < N");
+    if (ld < (ctx->full_eig ? 2 : 1) * ctx->n_eig) return fail(ctx, BSE_ERR_ARG, "ld too small");
     if (nvec == 0) return BSE_OK;
     (void)hipSetDevice(ctx->device);
+    if (ctx->full_eig) {
+        // (x; y) halves, each from the internal to the reference row order
+        const int64_t N = ctx->N;
+        const double2* X = (const double2*)ctx->bufs[S_XFULL].p;
+        double2* dst = (double2*)z;
+        if (!dst_on_device) dst = BUF(double2, S_W2, (size_t)ld * nvec);
+        HCK(bse_launch_rows_to_reference(ctx->st, N, ctx->B, ctx->nk, nvec, X, ctx->full_ldx, dst, ld,
+                                         kmap_dev(ctx)));
+        HCK(bse_launch_rows_to_reference(ctx->st, N, ctx->B, ctx->nk, nvec, X + N, ctx->full_ldx, dst + N, ld,
+                                         kmap_dev(ctx)));
+        if (!dst_on_device)
+            HCK(hipMemcpyAsync(z, dst, sizeof(double2) * ld * nvec, hipMemcpyDeviceToHost, ctx->st));
+        HCK(hipStreamSynchronize(ctx->st));
+        return BSE_OK;
+    }
     const int64_t n = ctx->n_eig;
     const double2* Z = (const double2*)ctx->bufs[S_Z].p;
     const int B = ctx->eig_on_bse ? ctx->B : 1;
@@ -1476,10 +1703,35 @@ extern "C" int bse_spectrum(bse_ctx* ctx, const double* omega, int n_omega, doub
     HCK(bse_launch_dielectric(ctx->st, n_omega, d_om, eta, (int)ctx->n_eig,
                               (const double*)ctx->bufs[S_EVAL].p, (const double2*)ctx->bufs[S_F].p,
                               pref, d_om + n_omega, d_om + 2 * n_omega));
+    std::vector<double> re2, im2;
+    if (ctx->full_eig) {
+        // beyond Tamm-Dancoff the anti-resonant branch adds
+        // + pref sum |f|^2 / (w + E + i eta): the same sum at -E, subtracted
+        const int64_t N = ctx->n_eig;
+        double* eneg = BUF(double, S_EVNEG, N + 2 * (int64_t)n_omega);
+        std::vector<double> e(N);
+        HCK(hipMemcpyAsync(e.data(), ctx->bufs[S_EVAL].p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->st));
+        HCK(hipStreamSynchronize(ctx->st));
+        for (double& x : e) x = -x;
+        HCK(hipMemcpyAsync(eneg, e.data(), sizeof(double) * N, hipMemcpyHostToDevice, ctx->st));
+        HCK(bse_launch_dielectric(ctx->st, n_omega, d_om, eta, (int)N, eneg, (const double2*)ctx->bufs[S_F].p,
+                                  pref, eneg + N, eneg + N + n_omega));
+        re2.resize(n_omega);
+        im2.resize(n_omega);
+        HCK(hipMemcpyAsync(re2.data(), eneg + N, sizeof(double) * n_omega, hipMemcpyDeviceToHost, ctx->st));
+        HCK(hipMemcpyAsync(im2.data(), eneg + N + n_omega, sizeof(double) * n_omega, hipMemcpyDeviceToHost,
+                           ctx->st));
+    }
     stage_end(ctx, ST_SPEC);
     HCK(hipMemcpyAsync(eps_re, d_om + n_omega, sizeof(double) * n_omega, hipMemcpyDeviceToHost, ctx->st));
     HCK(hipMemcpyAsync(eps_im, d_om + 2 * n_omega, sizeof(double) * n_omega, hipMemcpyDeviceToHost, ctx->st));
     HCK(hipStreamSynchronize(ctx->st));
+    if (ctx->full_eig) {
+        for (int i = 0; i < n_omega; ++i) {
+            eps_re[i] = eps_re[i] - re2[i] + 1.0;
+            eps_im[i] = eps_im[i] - im2[i];
+        }
+    }
     return BSE_OK;
 }
 

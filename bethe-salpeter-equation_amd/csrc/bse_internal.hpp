@@ -66,9 +66,22 @@ hipError_t bse_band_gather_tail(hipStream_t, int, int, int, const void*, double2
 int64_t bse_tsqr_workspace(int, int);
 hipError_t bse_band_panel_qr(hipStream_t, int, int, double2*, double2*, double2*, double2*, double2*, int,
                              int, int*);
+hipError_t bse_launch_assemble_coupling(hipStream_t, int, int, int, int, int, int, int, int, const double2*,
+                                        const double2*, const double2*, const double2*, double, double,
+                                        double2*);
+hipError_t bse_launch_expand_sym(hipStream_t, int64_t, int, int, const double2*, double2*, int64_t, int,
+                                 const int*);
+hipError_t bse_launch_compose_full(hipStream_t, int64_t, const double2*, const double2*, double2*);
+hipError_t bse_launch_potrf_tile(hipStream_t, int, int, double2*, double2*, int*);
+hipError_t bse_launch_trsm_panel(hipStream_t, int, int, double2*, const double2*, double2*, double2*);
+hipError_t bse_launch_lsl(hipStream_t, int64_t, const double2*, double2*);
+hipError_t bse_launch_lower_solve(hipStream_t, int, const double2*, const double2*, const double2*, double2*);
+hipError_t bse_launch_upper_solve(hipStream_t, int, int, const double2*, const double2*, const double2*,
+                                  int64_t, double2*, int64_t, const double*);
+hipError_t bse_launch_full_rho(hipStream_t, int64_t, const double2*, double2*, int64_t);
 int64_t bse_band_update_workspace(int, int, int);
 hipError_t bse_band_update(hipStream_t, int, int, int, int, int, int, void*, const double2*, const double2*,
-                           double2*, const double2*);
+                           double2*, const double2*, hipEvent_t*);
 int64_t bse_chase_counters(int);
 hipError_t bse_launch_chase(hipStream_t, int, int, double2*, int, double2*, double*, double*, int*, int*, int);
 int64_t bse_band_ii_workspace(int, int, int);
@@ -139,6 +152,9 @@ struct DevBuf {
 };
 
 enum Stage { ST_TB, ST_TABLES, ST_ASM, ST_TRD, ST_TEVAL, ST_TVEC, ST_BT, ST_REF, ST_SPEC, ST_N };
+// stage_ms slots past the event-timed stages: the two halves of a two-stage
+// tridiagonalisation (dense -> band with Q1^H rho; band -> tridiagonal)
+enum { ST_BAND = ST_N, ST_CHASE = ST_N + 1, ST_CHOL = ST_N + 2 };
 
 }  // namespace
 
@@ -168,6 +184,11 @@ struct bse_ctx {
     bool prof = false;
     std::vector<hipEvent_t> prof_ev;
     double hemv_ms = 0.0, hemv_bytes = 0.0;
+    // two-stage profile: the band reduction's rank-2k update and Hermitian
+    // block product (total ms, algorithmic flops, launches)
+    double b2k_ms = 0.0, b2k_flops = 0.0, hemm_ms = 0.0, hemm_flops = 0.0;
+    int64_t band_launches = 0;
+    bool prof_two_stage = false;
     bool force_valu_assembly = false;   // BSE_ASSEMBLY=valu selects the VALU tiles
     int asm_kernel = 0;                 // 0 VALU, 1 FP64 MFMA, 2 FP32 MFMA, 3 BF16x2 MFMA
     int64_t hemv_launches = 0;
@@ -178,7 +199,7 @@ struct bse_ctx {
     // (64 x 64 tiles), 1 FP64 VALU, 2 FP64 MFMA (128 x 128 macro tiles)
     int her2k_valu[2] = {-1, -1};
     bool autotune = true;
-    hipEvent_t tev[2] = {};     // autotuner timing pair
+    hipEvent_t tev[3] = {};     // autotuner timing pair (+1 for the two-stage split)
     // second stream (default priority): Q^H rho runs on it while the host
     // sorts the tridiagonal blocks and the bisection runs on st
     hipStream_t side = nullptr;
@@ -192,6 +213,9 @@ struct bse_ctx {
     // (row start, rows, reflectors, offset into the reflector store)
     int two_stage = 0;
     bool last_two_stage = false;
+    // full (non-Tamm-Dancoff) BSE state: M composed; last eig was of it
+    bool full_built = false, full_eig = false, have_coupling = false;
+    int64_t full_ldx = 0;
     struct BandPanel { int s, m, k; int64_t voff; };
     std::vector<BandPanel> band_panels;
     double chase_ms = 0.0, band_ms = 0.0;
@@ -222,7 +246,11 @@ enum Slot {
     S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL,
     S_GDHR, S_GDEPS, S_GDCOEF, S_GDP, S_GDPV, S_GDRHO, S_GDDE, S_GDW, S_GY, S_GOUT,
     S_SGFB, S_SGFA, S_SGX, S_SGU, S_SGB, S_SGV, S_SGPART, S_APPLYP, S_MEMB, S_KMAP, S_KINV, S_TBK, S_TBH, S_TBR, S_TBE, S_TBC,
-    S_BAB, S_BAB0, S_BPAN, S_BTSQR, S_BUPD, S_BPROG, S_BII, S_BZERO, S_NSLOTS
+    S_BAB, S_BAB0, S_BPAN, S_BTSQR, S_BUPD, S_BPROG, S_BII, S_BZERO,
+    // full BSE: coupling block, M / its Cholesky factor, W = L^H s3 L, tile
+    // inverses, panel, rho~, L^-1 rho~, eigenvectors, error flag, scales
+    S_BCPL, S_MFULL, S_WFULL, S_LINV, S_FPAN, S_FPANH, S_RHOT, S_ZT, S_XFULL, S_FERR, S_FSCALE, S_EVNEG,
+    S_NSLOTS
 };
 
 int fail(bse_ctx* c, int code, const std::string& msg) {

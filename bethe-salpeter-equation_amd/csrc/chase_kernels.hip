@@ -142,15 +142,27 @@ __global__ __launch_bounds__(64 * CH_W) void k_chase(ChaseArgs a)
                     }
                     wave_lds_sync();
                 }
-                // stage the lower triangle of A(D_k, D_k)
-                for (int c = 0; c < L; ++c)
-                    if (lane >= c && lane < L) X[lane][c] = a.AB[(int64_t)(lo + c) * ldab + (lane - c)];
+                // stage the lower triangle of A(D_k, D_k): all loads are
+                // issued before the first one is waited for
+                {
+                    double2 tmp[CH_B];
+#pragma unroll
+                    for (int c = 0; c < CH_B; ++c)
+                        tmp[c] = (c < L && lane >= c && lane < L) ? a.AB[(int64_t)(lo + c) * ldab + (lane - c)]
+                                                                  : cmk(0, 0);
+#pragma unroll
+                    for (int c = 0; c < CH_B; ++c)
+                        if (c < L && lane >= c && lane < L) X[lane][c] = tmp[c];
+                }
                 wave_lds_sync();
                 double2 wv = cmk(0, 0);
                 if (lane < L) {
-                    for (int c = 0; c < L; ++c) {
-                        const double2 h = (lane >= c) ? X[lane][c] : cconj(X[c][lane]);
-                        cfma(wv, h, sv[c]);
+#pragma unroll
+                    for (int c = 0; c < CH_B; ++c) {
+                        if (c < L) {
+                            const double2 h = (lane >= c) ? X[lane][c] : cconj(X[c][lane]);
+                            cfma(wv, h, sv[c]);
+                        }
                     }
                 }
                 const double2 v = (lane < L) ? sv[lane] : cmk(0, 0);
@@ -174,8 +186,16 @@ __global__ __launch_bounds__(64 * CH_W) void k_chase(ChaseArgs a)
                 const int k = (j + 1) >> 1;
                 const int lo = s + 1 + k * b, hi = min(lo + b - 1, n - 1), Lr = hi - lo + 1;
                 const int plo = lo - b;
-                for (int c = 0; c < b; ++c)
-                    if (lane < Lr) X[lane][c] = a.AB[(int64_t)(plo + c) * ldab + (b + lane - c)];
+                {
+                    double2 tmp[CH_B];
+#pragma unroll
+                    for (int c = 0; c < CH_B; ++c)
+                        tmp[c] = (c < b && lane < Lr) ? a.AB[(int64_t)(plo + c) * ldab + (b + lane - c)]
+                                                      : cmk(0, 0);
+#pragma unroll
+                    for (int c = 0; c < CH_B; ++c)
+                        if (c < b && lane < Lr) X[lane][c] = tmp[c];
+                }
                 wave_lds_sync();
                 // (1) X <- X H_{k-1}:  X -= tau (X v) v^H
                 if (lane < Lr) {

@@ -223,14 +223,22 @@ int  bse_eigh_dense(bse_ctx* ctx, int n, const double* a, int64_t lda, int a_on_
 /* Stage timings of the last build/eigh (milliseconds, HIP events):
  * [0] tb, [1] tables, [2] assembly, [3] tridiagonalisation, [4] tridiagonal
  * eigenvalues, [5] tridiagonal vectors, [6] back-transform, [7] refinement,
- * [8] oscillator+spectrum.  n <= 16. */
+ * [8] oscillator+spectrum; after a two-stage reduction (few vectors, large
+ * N) [9] dense -> band with Q1^H rho and [10] band -> tridiagonal, the two
+ * parts of [3] (0 after a one-stage reduction).  n <= 16. */
 int  bse_stage_times(const bse_ctx* ctx, double* ms, int n);
 
-/* Kernel-level profile of the tridiagonalisation's Hermitian matrix-vector
- * kernel (HIP events around every launch on the context stream).  enable
- * switches recording on for later calls; out (may be NULL) receives
- * [0] total kernel time in ms, [1] algorithmic bytes (lower triangle of every
- * trailing block), [2] launches, for the last bse_eigh. */
+/* Kernel-level profile of the tridiagonalisation's dominant kernels (HIP
+ * events around every launch on the context stream).  enable switches
+ * recording on for later calls; out (may be NULL, else 9 doubles) receives,
+ * for the last bse_eigh:
+ *   one-stage reduction: [0] total time in ms of the Hermitian matrix-vector
+ *     kernel, [1] its algorithmic bytes (lower triangle of every trailing
+ *     block), [2] launches, [4] = 0;
+ *   two-stage reduction: [0] ms, [2] launches, [3] algorithmic flops of the
+ *     panels' rank-2k update (8 m (m+1) k per panel), [4] = 1, and [5] ms,
+ *     [6] flops (8 m^2 k), [7] launches of the Hermitian block product,
+ *     [8] = 2. */
 int  bse_profile(bse_ctx* ctx, int enable, double* out);
 
 /* ---- K11: batched sweep of many small systems ------------------------------
@@ -273,6 +281,32 @@ int bse_batch_results(bse_ctx* ctx, double* evals, double* eps_re, double* eps_i
  * [5] trailing-update kernel (0 FP64 MFMA on 64 x 64 tiles, 1 FP64 VALU,
  *     2 FP64 MFMA on 128 x 128 tiles).                                  */
 int bse_batch_info(const bse_ctx* ctx, int64_t info[6]);
+
+/* ---- full BSE beyond Tamm-Dancoff -------------------------------------------
+ * bse_build_full assembles the resonant block A (as bse_build with
+ * BSE_PREC_FP64) and the coupling block B (complex symmetric) and composes
+ * M = [[A, B], [conj B, conj A]] (FP64, order 2N).  Replaces
+ * extract_hbse_and_rcv with tamn_dancoff = 0 (add_coupling_part, 2929-3040).
+ * The anti-resonant half of rho is rho_vc = -conj(rho_cv) (rows N..2N of
+ * rho_q_diagk_cv, 2951). */
+int bse_build_full(bse_ctx* ctx);
+/* B, N x N complex (ld >= N), reference order, both triangles, host. */
+int bse_get_coupling(bse_ctx* ctx, double* dst, int64_t ld);
+/* Eigenpairs of H = [[A, B], [-conj B, -conj A]] by a structure-preserving
+ * solve: Cholesky M = L L^H, the Hermitian W = L^H s3 L (s3 = diag(1, -1)),
+ * its eigenvalues +-E and, for the lowest n_vectors positive ones (-1 =
+ * all N), x = sqrt(E) L^-H y.  Replaces the eig_gen branch of
+ * common_diagonalization (3326) and the augmentation matrix of the non-TDA
+ * spectrum (pull_augmentation_matrix 3504-3522, sum 3607-3620), which the
+ * pseudo-orthonormality x^H s3 x = 1 makes unnecessary.  Afterwards
+ * bse_get_eigenvalues gives the N positive energies (ascending),
+ * bse_oscillator f = rho~^H x (N), bse_get_eigenvectors the 2N-row vectors
+ * (x; y) (ld >= 2N, each half in reference order) and bse_spectrum
+ *   eps(w) = 1 - C/(Omega Nk) sum_E |f|^2 [1/(w - E + i eta) - 1/(w + E + i eta)].
+ * BSE_ERR_NUMERIC when M is not positive definite (the system is unstable
+ * beyond Tamm-Dancoff; the reference's eig_gen then returns complex
+ * energies).  Requires bse_build_full; consumes M. */
+int bse_eigh_full(bse_ctx* ctx, int n_vectors);
 
 /* ---- allocation-free dense eig --------------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device

@@ -92,3 +92,78 @@ def assemble_bruteforce(p: Problem):
                 r = ridx(c, v, k)
                 H[r, r] += (ek[k, cidx[c]] + p.scissor) - eq[k, vidx[v]]
     return H
+
+
+def assemble_coupling_bruteforce(p: Problem):
+    """Coupling block B by the loop structure of add_coupling_part (2929-3040):
+
+    * exchange: factor_v sum_{G != G0} v(G) rho_cv(r, G) rho_cv(s, -G)
+      (the v(G) conj(rho_vc) rho_cv product of 2944-2967 with point-orbital
+      pair densities, rho_vc(s, G) = conj(rho_cv(s, -G)));
+    * direct: factor_w sum_G conj(rho(v' at k', c at k, G)) W(|k'-k+G|)
+      rho(c' at k', v at k, G) -- temporary_matrix3 / temporary_vector3 of
+      2986-3027, the resonant direct loop with the primed pair reversed.
+    """
+    kc = p.kpoints()
+    q = np.asarray(p.q, dtype=np.float64)[:, None]
+    ek, vk = tb_states(p, kc)
+    eq, vq = tb_states(p, kc - q)
+    nk, nc, nv = p.nk, p.nc, p.nv
+    cidx = [p.n_occ + c for c in range(nc)]
+    vidx = [p.n_occ - 1 - v for v in range(nv)]
+    g = glist(p)
+    ng = g.shape[1]
+    g0 = ng // 2
+    tau = p.tau
+    wbar = w_q0_average(p, p.r0, p.eps_env)
+
+    def rho(vl, left, vr, right, kl, kr, gvec):
+        ph = np.exp(1j * (tau @ (gvec + kc[:, kl] - kc[:, kr])))
+        return np.sum(vl[kl][:, left].conj() * ph * vr[kr][:, right])
+
+    def wmodel(k2, k1, G):
+        qv = kc[:, k2] - kc[:, k1] + g[:, G]
+        qn = np.linalg.norm(qv)
+        return wbar if qn < 1e-12 else w_model(p, qn, p.r0, p.eps_env)
+
+    N = p.n_bse
+    B = np.zeros((N, N), dtype=np.complex128)
+    factor_w = -p.direct_scale / (p.cell * nk)
+
+    def ridx(c, v, k):
+        return c * nv * nk + v * nk + k
+
+    for c1 in range(nc):
+        for v1 in range(nv):
+            for k1 in range(nk):
+                r = ridx(c1, v1, k1)
+                for c2 in range(nc):
+                    for v2 in range(nv):
+                        for k2 in range(nk):
+                            acc = 0.0j
+                            for s in range(ng):
+                                left = np.conj(rho(vq, vidx[v2], vk, cidx[c1], k2, k1, g[:, s]))
+                                right = rho(vk, cidx[c2], vq, vidx[v1], k2, k1, g[:, s])
+                                acc += factor_w * left * wmodel(k2, k1, s) * right
+                            B[r, ridx(c2, v2, k2)] = acc
+
+    fx = p.exchange_scale * p.f_s / (p.cell * nk)
+    for c1 in range(nc):
+        for v1 in range(nv):
+            for k1 in range(nk):
+                r = ridx(c1, v1, k1)
+                for c2 in range(nc):
+                    for v2 in range(nv):
+                        for k2 in range(nk):
+                            acc = 0.0j
+                            for s in range(ng):
+                                if s == g0:
+                                    continue
+                                vg = v_model(p, np.linalg.norm(g[:, s]))
+                                a = np.sum(vk[k1][:, cidx[c1]].conj() * np.exp(1j * (tau @ g[:, s]))
+                                           * vq[k1][:, vidx[v1]])
+                                b = np.sum(vk[k2][:, cidx[c2]].conj() * np.exp(-1j * (tau @ g[:, s]))
+                                           * vq[k2][:, vidx[v2]])
+                                acc += fx * vg * a * b
+                            B[r, ridx(c2, v2, k2)] += acc
+    return B

@@ -584,3 +584,161 @@ def _drho(p: Problem, name):
     d = 1j * (np.einsum("kci,i,kvi->cvk", dC.conj(), x, V)
               + np.einsum("kci,i,kvi->cvk", C.conj(), x, dV))
     return d.ravel()
+
+
+# ----------------------------------------------------------------------------
+# Beyond Tamm-Dancoff: coupling block, full BSE, spectrum
+# (add_coupling_part 2929-3040; eig_gen 3326; pull_excitonic_oscillator_force
+# 3494-3500; pull_augmentation_matrix 3504-3522; non-TDA sum 3607-3620)
+# ----------------------------------------------------------------------------
+
+def coupling_direct_term(p: Problem, C, V, wt):
+    """Bd[(c,v,k),(c',v',k')] = -(1/(A Nk)) sum_ij X'_{cv',i} W_ij(k-k') Y'_{vc',j}
+
+    X'_{cv',i} = conj(C[k,c,i]) V[k',v',i];  Y'_{vc',j} = V[k,v,j] conj(C[k',c',j]):
+    the direct term with the primed conduction and valence states exchanged
+    (the temporary_matrix3 / temporary_vector3 contraction of 2986-3027 with
+    point-orbital pair densities).  Returns (nc,nv,nk,nc,nv,nk)."""
+    ki, kj, kl = _kidx(p)
+    nk, nc, nv = p.nk, p.nc, p.nv
+    alpha = -p.direct_scale / (p.cell * nk)
+    out = np.empty((nc, nv, nk, nc, nv, nk), dtype=np.complex128)
+    for k in range(nk):
+        wk = wt[ki[k] - ki + p.n1 - 1, kj[k] - kj + p.n2 - 1, kl[k] - kl + p.n3 - 1]  # (k', i, j)
+        X = np.einsum("ci,kwi->kcwi", C[k].conj(), V)               # (k', c, v', i)
+        A = np.einsum("kcwi,kij->kcwj", X, wk)                       # (k', c, v', j)
+        Y = np.einsum("vj,kdj->kvdj", V[k], C.conj())                # (k', v, c', j)
+        blk = np.einsum("kcwj,kvdj->cvdwk", A, Y)                    # (c, v, c', v', k')
+        out[:, :, k] = alpha * blk
+    return out
+
+
+def assemble_coupling(p: Problem, bands=None, wt=None, vx=None):
+    """Coupling block B = Bd + (f_s/(A Nk)) P V P^T (reference order).
+
+    The exchange half is the reference's factor_v * v(G) * conj(rho_vc) rho_cv
+    sum over G != G0 (2937-2967) with point-orbital pair densities, which for
+    the Hermitian form of the resonant exchange (P V P^H) is P V P^T: both
+    blocks are complex symmetric, B = B^T."""
+    eps_c, C, eps_v, V = band_window(p) if bands is None else bands
+    wt = w_table(p) if wt is None else wt
+    vx = v_exchange(p) if vx is None else vx
+    N = p.n_bse
+    B = coupling_direct_term(p, C, V, wt).reshape(N, N)
+    P = exchange_factors(p, C, V)
+    B += (p.exchange_scale * p.f_s / (p.cell * p.nk)) * (P @ vx @ P.T)
+    return B
+
+
+def full_hamiltonian(p: Problem, bands=None):
+    """H = [[A, B], [-conj(B), -conj(A)]] (2N x 2N): the resonant block, the
+    coupling block and the anti-resonant rows filled as -conj of the transpose
+    (the "Conjugating HBSE" loop of 3034-3037 with B = B^T)."""
+    bands = band_window(p) if bands is None else bands
+    A = assemble(p, bands)
+    B = assemble_coupling(p, bands)
+    H = np.block([[A, B], [-B.conj(), -A.conj()]])
+    return A, B, H
+
+
+def full_dipoles(p: Problem, bands=None, direction=(1.0, 0.0, 0.0)):
+    """rho~ = (rho_cv; rho_vc): the anti-resonant half holds the v <- c pair
+    densities (rho_q_diagk_cv rows N..2N, 2951), rho_vc = -conj(rho_cv) in the
+    point-orbital dipole limit (i tau.qhat is anti-Hermitian)."""
+    rho = dipoles(p, bands, direction)
+    return np.concatenate([rho, -rho.conj()])
+
+
+def eig_full(H):
+    """Eigenpairs of the full BSE matrix through a general eigensolver (the
+    reference's eig_gen, 3326), sorted by real part (3334-3345).  Returns
+    (E, X) with E (2N,) real parts, X unit columns, and the largest imaginary
+    part as a diagnostic."""
+    e, x = np.linalg.eig(H)
+    order = np.argsort(e.real, kind="stable")
+    e, x = e[order], x[:, order]
+    x = x / np.linalg.norm(x, axis=0)[None, :]
+    return e.real, x, float(np.abs(e.imag).max())
+
+
+def spectrum_augmented(E, X, rho_t, omegas, eta, prefactor):
+    """eps(w) = 1 - pref sum_{l,l'} (rho~^H X_l) S^-1_{ll'} (X_l'^H s3 rho~) / (w - E_l + i eta)
+    over all 2N eigenpairs, S = X^H X the overlap ("augmentation") matrix of
+    the non-orthogonal eigenvectors (pull_augmentation_matrix 3504-3522).  This
+    is the resolvent rho~^H (w - H)^-1 s3 rho~ written in the eigenbasis of H,
+    which the reference's sum at 3607-3620 approximates with the N-vector
+    overlap and without the metric s3."""
+    n2 = X.shape[0]
+    s3 = np.concatenate([np.ones(n2 // 2), -np.ones(n2 // 2)])
+    S = X.conj().T @ X
+    left = X.T @ rho_t.conj()                          # rho~^H X_l
+    right = np.linalg.solve(S, X.conj().T @ (s3 * rho_t))
+    w = np.asarray(omegas)[:, None]
+    s = (left * right)[None, :] / (w - E[None, :] + 1j * eta)
+    return 1.0 - prefactor * s.sum(axis=1)
+
+
+def spectrum_resolvent(H, rho_t, omegas, eta, prefactor):
+    """eps(w) = 1 - pref rho~^H (w + i eta - H)^-1 s3 rho~ by dense solves
+    (independent of any eigendecomposition; small N only)."""
+    n2 = H.shape[0]
+    s3 = np.concatenate([np.ones(n2 // 2), -np.ones(n2 // 2)])
+    out = np.empty(len(omegas), dtype=np.complex128)
+    for t, w in enumerate(omegas):
+        y = np.linalg.solve((w + 1j * eta) * np.eye(n2) - H, s3 * rho_t)
+        out[t] = 1.0 - prefactor * (rho_t.conj() @ y)
+    return out
+
+
+def dielectric_full(E, f, omegas, eta, prefactor):
+    """eps(w) = 1 - pref sum_{l>0} |f_l|^2 [1/(w - E_l + i eta) - 1/(w + E_l + i eta)]
+    over the positive branch, f_l = rho~^H X_l with X_l^H s3 X_l = 1 (the
+    negative branch pairs with (conj y; conj x) and conj f_l)."""
+    w = np.asarray(omegas)[:, None]
+    a = (np.abs(f) ** 2)[None, :]
+    s = a / (w - E[None, :] + 1j * eta) - a / (w + E[None, :] + 1j * eta)
+    return 1.0 - prefactor * s.sum(axis=1)
+
+
+def solve_full_structured(A, B, rho_t):
+    """Structure-preserving solve of [[A, B], [-conj B, -conj A]] for a
+    positive definite M = [[A, B], [conj B, conj A]] (the stability condition
+    of the BSE): H = s3 M, M = L L^H (Cholesky), W = L^H s3 L Hermitian with
+    the eigenvalues of H; positive branch E_l, y_l; X_l = sqrt(E_l) L^-H y_l
+    (X^H s3 X = 1); f_l = rho~^H X_l = sqrt(E_l) (L^-1 rho~)^H y_l
+    (Shao, da Jornada, Yang, Deslippe & Lin 2016, cited at 3371)."""
+    N = A.shape[0]
+    M = np.block([[A, B], [B.conj(), A.conj()]])
+    L = np.linalg.cholesky(M)
+    s3 = np.concatenate([np.ones(N), -np.ones(N)])
+    W = L.conj().T @ (s3[:, None] * L)
+    W = 0.5 * (W + W.conj().T)
+    lam, Y = np.linalg.eigh(W)
+    pos = lam > 0
+    E, Yp = lam[pos], Y[:, pos]
+    z = np.linalg.solve(L, rho_t)
+    f = np.sqrt(E) * (Yp.T @ z.conj())
+    X = np.linalg.solve(L.conj().T, Yp) * np.sqrt(E)[None, :]
+    return E, X, f
+
+
+def solve_full(p: Problem, omegas=None):
+    """Full BSE (no Tamm-Dancoff): positive excitation energies E (N,),
+    eigenvectors X (2N x N, X^H s3 X = 1), f = rho~^H X, and the spectrum by the
+    augmentation form over the general eigensolver's 2N eigenpairs."""
+    bands = band_window(p)
+    A, B, H = full_hamiltonian(p, bands)
+    rho_t = full_dipoles(p, bands)
+    E2, X2, im = eig_full(H)
+    N = p.n_bse
+    Ep, Xp = E2[N:], X2[:, N:]
+    s3 = np.concatenate([np.ones(N), -np.ones(N)])
+    nrm = np.real(np.einsum("rl,r,rl->l", Xp.conj(), s3, Xp))
+    if im > 1e-9 * np.abs(E2).max() or np.any(nrm <= 0):
+        raise ValueError("unstable full BSE: [[A, B], [conj B, conj A]] is not positive definite")
+    Xn = Xp / np.sqrt(nrm)[None, :]
+    f = Xn.T @ rho_t.conj()
+    om = p.omegas if omegas is None else omegas
+    pref = C_EV_A / (p.volume * p.nk)
+    eps = spectrum_augmented(E2, X2, rho_t, om, p.eta, pref) if om is not None else None
+    return dict(A=A, B=B, H=H, E=Ep, X=Xn, f=f, rho_t=rho_t, eps=eps, E_all=E2, max_imag=im)
