@@ -235,6 +235,20 @@ class Context:
         del keep
         return eps, coef
 
+    def bands(self, segments, model=None, lattice=None, crystal=True, n_occ=0, nv=0, nc=0, scissor=0.0):
+        """Band energies along a path (Hamiltonian_TB::pull_bands, 823-875):
+        segments from lattice.read_k_path, points by band_path_points, states
+        by bse_tb_states_at (every band ascending when nv = nc = 0, else the
+        window: valence top-down, conduction bottom-up + scissor).  Returns
+        (k 3 x nk Cartesian, energies nk x bands); lattice.write_bands writes
+        the reference's bands file."""
+        from .lattice import band_path_points
+        m = model if model is not None else self.system.model
+        lat = lattice if lattice is not None else m.lattice
+        k = band_path_points(segments, lat, crystal)
+        eps, _ = self.tb_states_at(k, n_occ=n_occ, nv=nv, nc=nc, scissor=scissor, model=model)
+        return k, eps
+
     def sizes(self):
         out = (C.c_int64 * 4)()
         self._check(self.lib.bse_problem_sizes(self.h, out), "bse_problem_sizes")

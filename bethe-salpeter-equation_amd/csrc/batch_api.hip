@@ -197,7 +197,7 @@ static hipError_t record_front(BatchState* b, int s, hipStream_t st) {
     if ((e = bse_launch_w_table(st, sy.n1, sy.n2, g.n3, g.dim, no, g.ng, d.gvec, g.recip, d.tau, sy.r0,
                                 sy.eps_env, sy.radius, kC, g.wbar, 0, d.wtab)))
         return e;
-    if ((e = bse_launch_v_exchange(st, no, g.ng, g.dim, d.gvec, d.tau, kC, sy.radius, d.vx))) return e;
+    if ((e = bse_launch_v_exchange(st, no, g.ng, g.dim, d.gvec, d.tau, kC, sy.radius, sy.q, d.vx))) return e;
     if ((e = bse_launch_transitions(st, g.nk, sy.nv, sy.nc, no, d.eps, d.coef, nullptr, nullptr,
                                     d.tau, g.qhat, d.vx, d.P, d.PV, d.rho, d.dE)))
         return e;

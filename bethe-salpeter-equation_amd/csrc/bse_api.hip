@@ -376,7 +376,7 @@ static int run_tables(bse_ctx* ctx) {
     stage_begin(ctx, ST_TABLES);
     HCK(bse_launch_w_table(ctx->st, s.n1, s.n2, ctx->n3, ctx->dim, no, ctx->ng, gvec, ctx->recip, tau,
                            s.r0, s.eps_env, s.radius, kC, ctx->wbar, 0, wtab));
-    HCK(bse_launch_v_exchange(ctx->st, no, ctx->ng, ctx->dim, gvec, tau, kC, s.radius, vx));
+    HCK(bse_launch_v_exchange(ctx->st, no, ctx->ng, ctx->dim, gvec, tau, kC, s.radius, s.q, vx));
     HCK(bse_launch_transitions(ctx->st, ctx->nk, s.nv, s.nc, no, (const double*)ctx->bufs[S_EPS].p,
                                (const double2*)ctx->bufs[S_COEF].p, nullptr, nullptr, tau, ctx->qhat,
                                vx, P, PV, rho, dE));
@@ -1075,6 +1075,10 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
         if (rc2) return rc2;
     } else if (nstore > 0 && nstore <= 16 && bse_apply_q_thin_workspace(n) <= sk_el) {
         HCK(bse_apply_q_thin(st, n, nb, Vst, Tall, nstore, Z, n, 0, skw));
+    } else if (nstore >= 2048) {
+        // many vectors: one fused launch per panel over 32-column strips
+        // (at least 64 strips, so the launch fills the chip)
+        HCK(bse_apply_q_strips(st, n, nb, npan, Vst, voff.data(), Tall, Z, n, nstore));
     } else if (nstore > 0) {
         double2* w1 = BUF(double2, S_W1, (size_t)nb * nstore);
         double2* w2 = BUF(double2, S_W2, (size_t)nb * nstore);
@@ -1500,6 +1504,12 @@ extern "C" int bse_eigh(bse_ctx* ctx, int n_vectors, int refine) {
 // ---------------------------------------------------------------------------
 extern "C" int bse_build_full(bse_ctx* ctx) {
     if (!ctx) return BSE_ERR_ARG;
+    if (!ctx->have_problem) return fail(ctx, BSE_ERR_STATE, "no problem loaded");
+    {
+        const double* q = ctx->sys.q;
+        if (q[0] * q[0] + q[1] * q[1] + q[2] * q[2] > BSE_OPTICAL_Q * BSE_OPTICAL_Q)
+            return fail(ctx, BSE_ERR_UNSUPPORTED, "the full BSE is formed in the optical limit |q| <= 1e-4 1/A");
+    }
     int rc = bse_build(ctx, BSE_PREC_FP64);
     if (rc) return rc;
     const bse_system& s = ctx->sys;

@@ -222,6 +222,11 @@ __host__ __device__ inline int64_t ktable_size(int n1, int n2, int n3) {
 // a tile reads 32 KB (FP32) or 64 KB (FP64) of consecutive memory, and the
 // upper triangle is not stored (half the memory of a full square).
 // ---------------------------------------------------------------------------
+// Exciton momenta up to this length (1/Angstrom) are the optical limit
+// q -> 0: the exchange then runs over v(|G|); the reference's minval = 1e-5
+// (line 29) lies inside it.
+#define BSE_OPTICAL_Q 1e-4
+
 #define BSE_TILE 64
 __host__ __device__ inline int tile_count(int64_t n) { return (int)ceil_div64(n, BSE_TILE); }
 // Position of tile (ti, tj), ti >= tj, among nt x nt lower tiles.

@@ -20,7 +20,7 @@ hipError_t bse_launch_w_table(hipStream_t, int, int, int, int, int, int, const d
                               const double*, const double*, double, double, double, double, double,
                               int, double2*);
 hipError_t bse_launch_v_exchange(hipStream_t, int, int, int, const double*, const double*, double,
-                                 double, double2*);
+                                 double, const double*, double2*);
 hipError_t bse_launch_transitions(hipStream_t, int, int, int, int, const double*, const double2*,
                                   const double*, const double2*, const double*, const double*,
                                   const double2*, double2*, double2*, double2*, double*);
@@ -79,6 +79,8 @@ hipError_t bse_launch_lower_solve(hipStream_t, int, const double2*, const double
 hipError_t bse_launch_upper_solve(hipStream_t, int, int, const double2*, const double2*, const double2*,
                                   int64_t, double2*, int64_t, const double*);
 hipError_t bse_launch_full_rho(hipStream_t, int64_t, const double2*, double2*, int64_t);
+hipError_t bse_apply_q_strips(hipStream_t, int, int, int, const double2*, const int64_t*, const double2*,
+                              double2*, int64_t, int);
 int64_t bse_band_update_workspace(int, int, int);
 hipError_t bse_band_update(hipStream_t, int, int, int, int, int, int, void*, const double2*, const double2*,
                            double2*, const double2*, hipEvent_t*);

@@ -83,9 +83,12 @@ __global__ __launch_bounds__(256) void k_w_table(TableGeom tg, int norb, int ng,
 }
 
 // V_ij (exchange), sum over G != G0 of v(G) e^{i G.(tau_i - tau_j)}.
+// V_ij = sum_{G != G0} v(|Q + G|) e^{i (Q + G).(tau_i - tau_j)}: Q = 0 in the
+// optical limit, else the exciton momentum (v_coulomb_g of
+// pull_coulomb_potentials, 2567, with G0 excluded as at 2811-2816).
 __global__ void k_v_exchange(int norb, int ng, int dim, const double* __restrict__ gvec,
                              const double* __restrict__ tau, double cc, double radius,
-                             double2* __restrict__ vx)
+                             double qx, double qy, double qz, double2* __restrict__ vx)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= norb * norb) return;
@@ -95,7 +98,7 @@ __global__ void k_v_exchange(int norb, int ng, int dim, const double* __restrict
     double2 acc = cmk(0, 0);
     for (int g = 0; g < ng; ++g) {
         if (g == ng / 2) continue;
-        const double gx = gvec[3 * g], gy = gvec[3 * g + 1], gz = gvec[3 * g + 2];
+        const double gx = qx + gvec[3 * g], gy = qy + gvec[3 * g + 1], gz = qz + gvec[3 * g + 2];
         const double v = bare_v(dim, gx * gx + gy * gy + gz * gz, radius, cc);
         double s, c;
         sincos(gx * tx + gy * ty + gz * tz, &s, &c);
@@ -169,11 +172,16 @@ extern "C" hipError_t bse_launch_w_table(hipStream_t st, int n1, int n2, int n3,
     return hipGetLastError();
 }
 
+// q: the exciton momentum (3 doubles, host); |q| <= BSE_OPTICAL_Q is the
+// optical limit q -> 0.
 extern "C" hipError_t bse_launch_v_exchange(hipStream_t st, int norb, int ng, int dim,
                                             const double* gvec, const double* tau, double cc,
-                                            double radius, double2* vx)
+                                            double radius, const double* q, double2* vx)
 {
-    hipLaunchKernelGGL(k_v_exchange, dim3(1), dim3(256), 0, st, norb, ng, dim, gvec, tau, cc, radius, vx);
+    double qx = q[0], qy = q[1], qz = q[2];
+    if (qx * qx + qy * qy + qz * qz <= BSE_OPTICAL_Q * BSE_OPTICAL_Q) qx = qy = qz = 0.0;
+    hipLaunchKernelGGL(k_v_exchange, dim3(1), dim3(256), 0, st, norb, ng, dim, gvec, tau, cc, radius, qx, qy,
+                       qz, vx);
     return hipGetLastError();
 }
 

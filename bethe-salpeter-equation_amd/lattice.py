@@ -191,3 +191,62 @@ class GList:
     @property
     def g0(self) -> int:
         return self.ng // 2
+
+
+def read_k_path(path, number_segments=None):
+    """Band-path file of Hamiltonian_TB::pull_bands (823-875): per segment the
+    start k, the end k and the number of points, ``k0x k0y k0z k1x k1y k1z n``
+    (whitespace-separated, line breaks free); reading stops after
+    ``number_segments`` segments (all if None) or at the end of the numbers.
+    Returns a list of (k0 (3,), k1 (3,), n)."""
+    with open(path) as f:
+        vals = f.read().split()
+    segs, i = [], 0
+    while i + 7 <= len(vals):
+        if number_segments is not None and len(segs) >= number_segments:
+            break
+        k0 = np.array([float(x) for x in vals[i:i + 3]])
+        k1 = np.array([float(x) for x in vals[i + 3:i + 6]])
+        n = int(float(vals[i + 6]))
+        if n < 2:
+            raise ValueError(f"{path}: a segment needs at least 2 points")
+        segs.append((k0, k1, n))
+        i += 7
+    return segs
+
+
+def band_path_points(segments, lattice: CrystalLattice | None = None, crystal: bool = True) -> np.ndarray:
+    """Points of the band path (pull_bands 846-860): segment (k0, k1, n) gives
+    k_j = k0 (1 - j/(n-1)) + k1 j/(n-1), j = 0 .. n-1, every segment in turn
+    (end points repeated as in the reference).  crystal: the file's k are in
+    reciprocal-lattice coordinates and are converted with the reciprocal
+    vectors of ``lattice`` (the reference's crystal_coordinates = 1 with its
+    primitive_vectors).  Returns 3 x nk Cartesian (1/Angstrom)."""
+    pts = []
+    for k0, k1, n in segments:
+        t = np.arange(n) / (n - 1)
+        pts.append(np.outer(k0, 1.0 - t) + np.outer(k1, t))
+    k = np.concatenate(pts, axis=1)
+    if crystal:
+        if lattice is None:
+            raise ValueError("crystal coordinates need the lattice")
+        k = lattice.reciprocal @ k
+    return k
+
+
+def write_bands(path, energies) -> None:
+    """Bands file of pull_bands (864-868): one line per k point, the band
+    energies (eV) separated by spaces."""
+    e = np.asarray(energies, dtype=np.float64)
+    with open(path, "w") as f:
+        for row in e:
+            f.write(" ".join(f"{x:.12g}" for x in row) + " \n")
+
+
+def read_bands(path) -> np.ndarray:
+    rows = []
+    with open(path) as f:
+        for line in f:
+            if line.strip():
+                rows.append([float(x) for x in line.split()])
+    return np.array(rows)

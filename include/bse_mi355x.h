@@ -80,7 +80,12 @@ typedef struct {
     int32_t n_occ;         /* occupied bands; conduction starts at n_occ       */
     int32_t nv, nc;        /* valence / conduction bands kept                 */
     double scissor;        /* eV added to conduction energies (816)           */
-    double q[3];           /* optical momentum; valence states at k - q       */
+    double q[3];           /* exciton momentum; valence states at k - q.  Up to
+                              |q| = 1e-4 1/A the optical limit q -> 0 (the
+                              reference's minval, 29, lies inside it): the
+                              exchange runs over v(|G|).  Beyond it a finite
+                              momentum (adding_momentum, 2549): v(|q + G|),
+                              G != 0 (2567, 2811-2816)                        */
     double r0, eps_env;    /* Rytova-Keldysh screening                        */
     double g_cut;          /* 2D G-list cutoff (319-385, parallelepiped)      */
     double f_s;            /* exchange spin factor, 2 - spinorial (2801)      */

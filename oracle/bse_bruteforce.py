@@ -22,7 +22,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from bse_oracle import (C_EV_A, Problem, glist, tb_states, w_model, w_q0_average, v_model)
+from bse_oracle import (C_EV_A, Problem, exciton_momentum, glist, tb_states, w_model, w_q0_average, v_model)
 
 
 def assemble_bruteforce(p: Problem):
@@ -71,19 +71,20 @@ def assemble_bruteforce(p: Problem):
                                 acc += factor_w * tmp[s] * rho(vidx[v2], vidx[v1], k2, k1, s, vq)
                             H[r, ridx(c2, v2, k2)] = acc
 
-    # exchange, Hermitian form, G0 excluded
+    # exchange, Hermitian form, G0 excluded, pair densities at Q + G
+    Q = exciton_momentum(p)
     rho_cv = np.zeros((N, ng), dtype=np.complex128)
     for c in range(nc):
         for v in range(nv):
             for k in range(nk):
                 for s in range(ng):
-                    ph = np.exp(1j * (tau @ g[:, s]))
+                    ph = np.exp(1j * (tau @ (Q + g[:, s])))
                     rho_cv[ridx(c, v, k), s] = np.sum(vk[k][:, cidx[c]].conj() * ph * vq[k][:, vidx[v]])
     fx = p.exchange_scale * p.f_s / (p.cell * nk)
     for s in range(ng):
         if s == g0:
             continue
-        vg = v_model(p, np.linalg.norm(g[:, s]))
+        vg = v_model(p, np.linalg.norm(Q + g[:, s]))
         H += fx * vg * np.outer(rho_cv[:, s], rho_cv[:, s].conj())
 
     for c in range(nc):

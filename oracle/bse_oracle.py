@@ -301,10 +301,23 @@ def w_table(p: Problem, r0=None, eps_env=None):
     return out
 
 
+OPTICAL_Q = 1e-4   # exciton momenta up to this length are the optical limit q -> 0
+
+
+def exciton_momentum(p: Problem):
+    """The exchange's Q: 0 in the optical limit |q| <= OPTICAL_Q (the
+    reference's minval = 1e-5 lies inside it), else the exciton momentum q."""
+    q = np.asarray(p.q, dtype=np.float64)
+    return np.zeros(3) if np.linalg.norm(q) <= OPTICAL_Q else q
+
+
 def v_exchange(p: Problem):
-    """V_ij = sum_{G != 0} v(|G|) e^{iG.(tau_i - tau_j)} with the bare v."""
+    """V_ij = sum_{G != 0} v(|Q + G|) e^{i(Q + G).(tau_i - tau_j)} with the bare
+    v: v_coulomb_g = pull(excitonic_momentum + G) (2567) with G0 excluded
+    (2811-2816); Q = exciton_momentum(p)."""
     g = glist(p)
     g = np.delete(g, g.shape[1] // 2, axis=1)             # exclude G0 (2811-2816)
+    g = g + exciton_momentum(p)[:, None]
     gn = np.linalg.norm(g, axis=0)
     dt = p.tau[:, None, :] - p.tau[None, :, :]
     ph = np.exp(1j * np.einsum("xg,ijx->gij", g, dt))
@@ -620,6 +633,8 @@ def assemble_coupling(p: Problem, bands=None, wt=None, vx=None):
     sum over G != G0 (2937-2967) with point-orbital pair densities, which for
     the Hermitian form of the resonant exchange (P V P^H) is P V P^T: both
     blocks are complex symmetric, B = B^T."""
+    if np.any(exciton_momentum(p) != 0):
+        raise ValueError("the coupling block is formed in the optical limit |q| <= OPTICAL_Q")
     eps_c, C, eps_v, V = band_window(p) if bands is None else bands
     wt = w_table(p) if wt is None else wt
     vx = v_exchange(p) if vx is None else vx

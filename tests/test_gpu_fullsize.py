@@ -185,6 +185,25 @@ def test_dense_eigh_cluster_split_by_another_block(ctx):
     assert np.abs(a @ z - z * ev).max() < 1e-12
 
 
+@pytest.mark.parametrize("n", [2100, 4160])
+def test_dense_all_vectors_strip_backtransform(ctx, n):
+    """All eigenvectors of a dense matrix large enough for the strip
+    back-transform (>= 2048 vectors): residual and orthonormality at FP64
+    level, eigenvalues against LAPACK.  2100 leaves a partial strip and a
+    partial last row chunk; 4160 = 65 whole tiles."""
+    rng = np.random.default_rng(n)
+    a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    a = 0.5 * (a + a.conj().T)
+    print(f"dense all vectors n={n}", flush=True)
+    ev, z = ctx.eigh_dense(a, "fp64", -1)
+    ref = np.linalg.eigvalsh(a)
+    scale = np.abs(ref).max()
+    assert np.abs(ev - ref).max() < 1e-12 * scale * n ** 0.5
+    cols = np.r_[0:8, n // 2:n // 2 + 8, n - 8:n]
+    assert np.abs(a @ z[:, cols] - z[:, cols] * ev[None, cols]).max() < 1e-11 * scale * n ** 0.5
+    assert np.abs(z.conj().T @ z - np.eye(n)).max() < 1e-10
+
+
 def test_c4_full_sweep_against_oracle(ctx):
     systems = bse_amd.honeycomb_sweep(64, seed=4)
     om = systems[0].omegas()
