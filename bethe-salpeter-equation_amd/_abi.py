@@ -48,7 +48,8 @@ class SystemC(C.Structure):
                 ("f_s", C.c_double), ("exchange_scale", C.c_double),
                 ("direct_scale", C.c_double), ("dipole_dir", C.c_double * 3),
                 ("dimension", C.c_int32), ("n3", C.c_int32), ("shift3", C.c_double),
-                ("radius", C.c_double)]
+                ("radius", C.c_double), ("screening", C.c_int32), ("rpa_nv", C.c_int32),
+                ("rpa_nc", C.c_int32)]
 
 
 class BatchOptionsC(C.Structure):
@@ -79,6 +80,12 @@ def _system_to_c(s):
     sc.n3 = int(getattr(s.grid, "n3", 1))
     sc.shift3 = float(getattr(s.grid, "shift3", 0.0))
     sc.radius = float(getattr(s, "radius", 0.0))
+    scr = getattr(s, "screening", "model")
+    if scr not in ("model", "rpa"):
+        raise ValueError(f"unknown screening {scr!r}")
+    sc.screening = 1 if scr == "rpa" else 0
+    sc.rpa_nv = int(getattr(s, "rpa_nv", 0))
+    sc.rpa_nc = int(getattr(s, "rpa_nc", 0))
     return mc, sc, [r_cart, h_r, tau]
 
 

@@ -429,6 +429,8 @@ extern "C" int bse_batch_prepare(bse_ctx* ctx, int n_systems, const bse_tb_model
         const int vrc = make_geom(&models[i], &systems[i], b->hs[i].g, msg);
         if (vrc) return fail(ctx, vrc, "system " + std::to_string(i) + ": " + msg);
         const Geom& g = b->hs[i].g;
+        if (systems[i].screening != 0)
+            return fail(ctx, BSE_ERR_UNSUPPORTED, "system " + std::to_string(i) + ": the batch uses model screening");
         if (g.N < 2 || g.N > 4096)
             return fail(ctx, BSE_ERR_UNSUPPORTED, "system " + std::to_string(i) + ": batched N must be in [2, 4096]");
         if (opt->n_lowest > g.N) return fail(ctx, BSE_ERR_ARG, "n_lowest exceeds a system's N");

@@ -132,6 +132,7 @@ static int load_problem(bse_ctx* ctx, const bse_tb_model* m, const Geom& g, cons
     ctx->gvec_h = g.gvec;
     ctx->ng = g.ng;
     ctx->wbar = g.wbar;
+    ctx->vbar = g.vbar;
     ctx->TK = bse_assemble_pick_tk(s->nv, s->nc, m->norb);
     const size_t nh = (size_t)m->nr * m->norb * m->norb * 2;
 
@@ -374,8 +375,27 @@ static int run_tables(bse_ctx* ctx) {
     const double* gvec = (const double*)ctx->bufs[S_GVEC].p;
     const double* tau = (const double*)ctx->bufs[S_TAU].p;
     stage_begin(ctx, ST_TABLES);
-    HCK(bse_launch_w_table(ctx->st, s.n1, s.n2, ctx->n3, ctx->dim, no, ctx->ng, gvec, ctx->recip, tau,
-                           s.r0, s.eps_env, s.radius, kC, ctx->wbar, 0, wtab));
+    if (s.screening == 1) {
+        // static RPA: the screening window's states at the grid points (no
+        // optical shift), Pi(Q) and the screened table (rpa_kernels.hip)
+        const int nvs = s.rpa_nv > 0 ? std::min(s.rpa_nv, s.n_occ) : s.n_occ;
+        const int ncs = s.rpa_nc > 0 ? std::min(s.rpa_nc, no - s.n_occ) : no - s.n_occ;
+        if (nvs < 1 || ncs < 1) return fail(ctx, BSE_ERR_ARG, "RPA screening needs occupied and empty bands");
+        const int nss = nvs + ncs;
+        double* reps = BUF(double, S_RPAEPS, (size_t)ctx->nk * nss);
+        double2* rcoef = BUF(double2, S_RPACOEF, (size_t)ctx->nk * nss * no);
+        double2* pi = BUF(double2, S_RPAPI, bse_rpa_pi_elems(s.n1, s.n2, ctx->n3, no));
+        static const double q0[3] = {0.0, 0.0, 0.0};
+        HCK(bse_launch_tb_states(ctx->st, ctx->nk, no, ctx->nr, (const double*)ctx->bufs[S_RCART].p,
+                                 (const double2*)ctx->bufs[S_HR].p, (const double*)ctx->bufs[S_KPTS].p, q0,
+                                 s.n_occ, nvs, ncs, s.scissor, nullptr, reps, rcoef, nullptr, nullptr));
+        HCK(bse_launch_rpa_table(ctx->st, s.n1, s.n2, ctx->n3, no, ctx->ng, ctx->dim, ctx->recip, nvs, ncs,
+                                 s.f_s / (ctx->cell * ctx->nk), ctx->vbar, s.radius, kC, reps, rcoef, gvec, tau,
+                                 pi, wtab));
+    } else {
+        HCK(bse_launch_w_table(ctx->st, s.n1, s.n2, ctx->n3, ctx->dim, no, ctx->ng, gvec, ctx->recip, tau,
+                               s.r0, s.eps_env, s.radius, kC, ctx->wbar, 0, wtab));
+    }
     HCK(bse_launch_v_exchange(ctx->st, no, ctx->ng, ctx->dim, gvec, tau, kC, s.radius, s.q, vx));
     HCK(bse_launch_transitions(ctx->st, ctx->nk, s.nv, s.nc, no, (const double*)ctx->bufs[S_EPS].p,
                                (const double2*)ctx->bufs[S_COEF].p, nullptr, nullptr, tau, ctx->qhat,
@@ -1350,6 +1370,8 @@ extern "C" int bse_grad_energies(bse_ctx* ctx, int n_exc, const int* which, doub
     if (!ctx || !dE || n_exc <= 0) return BSE_ERR_ARG;
     if (!ctx->have_eig || !ctx->eig_on_bse) return fail(ctx, BSE_ERR_STATE, "run bse_eigh on a built system first");
     if (n_exc > ctx->nvec) return fail(ctx, BSE_ERR_STATE, "bse_eigh formed fewer eigenvectors than n_exc");
+    if (ctx->sys.screening != 0 || ctx->full_eig)
+        return fail(ctx, BSE_ERR_UNSUPPORTED, "gradients hold the model W fixed: TDA with model screening only");
     (void)hipSetDevice(ctx->device);
     const int64_t N = ctx->N;
     const double2* A = (const double2*)ctx->bufs[S_Z].p;
@@ -1386,6 +1408,8 @@ extern "C" int bse_grad_spectrum(bse_ctx* ctx, const double* omega, int n_omega,
         return fail(ctx, BSE_ERR_STATE, "run bse_eigh on a built system first");
     if (ctx->nvec != ctx->n_eig)
         return fail(ctx, BSE_ERR_STATE, "the spectrum gradient needs every eigenvector (bse_eigh(ctx, -1, 0))");
+    if (ctx->sys.screening != 0 || ctx->full_eig)
+        return fail(ctx, BSE_ERR_UNSUPPORTED, "gradients hold the model W fixed: TDA with model screening only");
     (void)hipSetDevice(ctx->device);
     hipStream_t st = ctx->st;
     const int64_t N = ctx->N;

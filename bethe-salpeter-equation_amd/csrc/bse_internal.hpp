@@ -79,6 +79,10 @@ hipError_t bse_launch_lower_solve(hipStream_t, int, const double2*, const double
 hipError_t bse_launch_upper_solve(hipStream_t, int, int, const double2*, const double2*, const double2*,
                                   int64_t, double2*, int64_t, const double*);
 hipError_t bse_launch_full_rho(hipStream_t, int64_t, const double2*, double2*, int64_t);
+int64_t bse_rpa_pi_elems(int, int, int, int);
+hipError_t bse_launch_rpa_table(hipStream_t, int, int, int, int, int, int, const double*, int, int, double, double,
+                                double, double, const double*, const double2*, const double*, const double*,
+                                double2*, double2*);
 hipError_t bse_apply_q_strips(hipStream_t, int, int, int, const double2*, const int64_t*, const double2*,
                               double2*, int64_t, int);
 int64_t bse_band_update_workspace(int, int, int);
@@ -170,7 +174,7 @@ struct bse_ctx {
     bse_system sys{};
     int norb = 0, nr = 0, nk = 0, ng = 0, B = 0, ns = 0, n3 = 1, dim = 2;
     int64_t N = 0;
-    double recip[9] = {0}, area = 0, volume = 0, cell = 0, wbar = 0, qhat[3] = {1, 0, 0};
+    double recip[9] = {0}, area = 0, volume = 0, cell = 0, wbar = 0, vbar = 0, qhat[3] = {1, 0, 0};
     std::vector<double> kpts_h, gvec_h, hr_h, rcart_h;
     // state
     bool have_tb = false, have_tables = false, built = false, have_eig = false, have_f = false;
@@ -252,6 +256,8 @@ enum Slot {
     // full BSE: coupling block, M / its Cholesky factor, W = L^H s3 L, tile
     // inverses, panel, rho~, L^-1 rho~, eigenvectors, error flag, scales
     S_BCPL, S_MFULL, S_WFULL, S_LINV, S_FPAN, S_FPANH, S_RHOT, S_ZT, S_XFULL, S_FERR, S_FSCALE, S_EVNEG,
+    // RPA screening: window states at the grid points, Pi(Q)
+    S_RPAEPS, S_RPACOEF, S_RPAPI,
     S_NSLOTS
 };
 
@@ -341,7 +347,7 @@ struct Geom {
     int64_t N = 0;
     // cell: the measure of the kernel prefactor 1/(cell Nk) -- the in-plane
     // area for 2D potentials, the volume for 3D ones
-    double recip[9] = {0}, volume = 0, area = 0, cell = 0, wbar = 0, qhat[3] = {1, 0, 0};
+    double recip[9] = {0}, volume = 0, area = 0, cell = 0, wbar = 0, vbar = 0, qhat[3] = {1, 0, 0};
     std::vector<double> kpts, gvec;
 };
 
@@ -364,6 +370,11 @@ int make_geom(const bse_tb_model* m, const bse_system* s, Geom& g, std::string& 
     }
     if (!(s->eps_env > 0.0) || !(s->r0 >= 0.0) || !(s->g_cut >= 0.0)) {
         msg = "screening parameters must satisfy eps_env > 0, r0 >= 0, g_cut >= 0";
+        return BSE_ERR_ARG;
+    }
+    if (s->screening != 0 && s->screening != 1) { msg = "screening must be 0 (model) or 1 (RPA)"; return BSE_ERR_ARG; }
+    if (s->screening == 1 && (s->rpa_nv < 0 || s->rpa_nc < 0 || s->n_occ + 1 > m->norb)) {
+        msg = "RPA screening needs rpa_nv, rpa_nc >= 0 and an empty band";
         return BSE_ERR_ARG;
     }
     const size_t nh = (size_t)m->nr * m->norb * m->norb * 2;
@@ -431,6 +442,10 @@ int make_geom(const bse_tb_model* m, const bse_system* s, Geom& g, std::string& 
                 for (int x = 0; x < 3; ++x)
                     g.gvec.push_back(i * g.recip[x] + j * g.recip[3 + x] + l * g.recip[6 + x]);
     g.ng = (int)(g.gvec.size() / 3);
+    if (s->screening == 1 && (g.ng > 48 || m->norb > 16)) {
+        msg = "RPA screening holds at most 48 G vectors and 16 orbitals";
+        return BSE_ERR_UNSUPPORTED;
+    }
     // q = 0 term: mean of W over the mini Brillouin zone around Q = 0, a disk
     // of area (2 pi)^2/(A Nk) (2D) or a ball of volume (2 pi)^3/(Omega Nk) (3D)
     if (dim == 2) {
@@ -438,12 +453,14 @@ int make_geom(const bse_tb_model* m, const bse_system* s, Geom& g, std::string& 
         const double rad = sqrt(da / M_PI);
         g.wbar = (s->r0 > 0.0) ? (M_PI * kC / s->eps_env) * log1p(s->r0 * rad) / (s->r0 * da)
                                : (M_PI * kC / s->eps_env) * rad / da;
+        g.vbar = (M_PI * kC) * rad / da;           // the bare potential's mean (RPA head)
     } else {
         const double vs = 8.0 * M_PI * M_PI * M_PI / (g.volume * g.nk);
         const double rad = cbrt(3.0 * vs / (4.0 * M_PI));
         const double r = s->radius;
         const double integ = (r > 0.0) ? rad - sqrt(r) * atan(rad / sqrt(r)) : rad;
         g.wbar = 4.0 * M_PI * kC * integ / (s->eps_env * vs);
+        g.vbar = 4.0 * M_PI * kC * integ / vs;
     }
     double qh[3] = {s->dipole_dir[0], s->dipole_dir[1], s->dipole_dir[2]};
     const double qn = sqrt(qh[0] * qh[0] + qh[1] * qh[1] + qh[2] * qh[2]);

@@ -26,8 +26,9 @@
 // in LDS (row i <-> lane i, b <= 32).  A workgroup holds CH_W waves that run
 // consecutive sweeps; workgroup g takes sweep chunks g, g + grid, ....  Each
 // sweep publishes its completed task count; the next sweep polls it with
-// agent-scope acquire loads and bounded spins: a wave that waits too long
-// (or sees the error flag) sets the flag and stops, so every wave exits.
+// bounded spins (workgroup-scope ordering inside a chunk, agent scope from
+// one chunk to the next): a wave that waits too long (or sees the error
+// flag) sets the flag and stops, so every wave exits.
 // A waiter only ever waits on a lower sweep, and workgroups are dispatched
 // in index order, so the lowest unfinished sweep always makes progress.
 //
@@ -75,11 +76,22 @@ struct ChaseArgs {
 
 __device__ inline int chase_ntask(int n, int b, int s) { return 2 * ((n - 2 - s) / b + 1) - 1; }
 
-__device__ inline bool chase_wait(const ChaseArgs& a, int s, int need, int lane) {
+// Sweep s waits for sweep s - 1.  Consecutive sweeps of one chunk run as the
+// waves of one workgroup (one CU, one L1 and one L2), so their hand-over
+// needs only workgroup-scope ordering (local = true); the first wave of a
+// chunk waits on the last wave of the previous chunk, in another workgroup
+// and possibly on another XCD, with agent-scope acquire / release (L2
+// write-back and invalidation).  Agent scope is cumulative: what earlier
+// waves of a workgroup wrote (ordered before its last wave's release) is
+// written back by that release too.
+__device__ inline bool chase_wait(const ChaseArgs& a, int s, int need, int lane, bool local) {
     int ok = 1;
     if (lane == 0) {
         unsigned spins = 0;
-        while (__hip_atomic_load(a.prog + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        for (;;) {
+            const int got = local ? __hip_atomic_load(a.prog + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                  : __hip_atomic_load(a.prog + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (got >= need) break;
             if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
             if (++spins > CHASE_SPIN_LIMIT) {
                 __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -90,13 +102,19 @@ __device__ inline bool chase_wait(const ChaseArgs& a, int s, int need, int lane)
         }
     }
     ok = __shfl(ok, 0, 64);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (local) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return ok != 0;
 }
 
-__device__ inline void chase_post(const ChaseArgs& a, int s, int val, int lane) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0) __hip_atomic_store(a.prog + s, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline void chase_post(const ChaseArgs& a, int s, int val, int lane, bool local) {
+    if (local) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __hip_atomic_store(a.prog + s, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane == 0) __hip_atomic_store(a.prog + s, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 __global__ __launch_bounds__(64 * CH_W) void k_chase(ChaseArgs a)
@@ -118,7 +136,7 @@ __global__ __launch_bounds__(64 * CH_W) void k_chase(ChaseArgs a)
         double2 tau = cmk(0, 0);
         bool alive = true;
         for (int j = 0; j < ntask && alive; ++j) {
-            if (s > 0 && !chase_wait(a, s - 1, min(j + 3, nprev), lane)) { alive = false; break; }
+            if (s > 0 && !chase_wait(a, s - 1, min(j + 3, nprev), lane, w > 0)) { alive = false; break; }
             if ((j & 1) == 0) {
                 // ---- D_k: (k = 0: reflector from column s) two-sided update
                 const int k = j >> 1;
@@ -237,7 +255,7 @@ __global__ __launch_bounds__(64 * CH_W) void k_chase(ChaseArgs a)
                 tau = r.tau;
                 wave_lds_sync();
             }
-            chase_post(a, s, j + 1, lane);
+            chase_post(a, s, j + 1, lane, w < CH_W - 1);
         }
         if (!alive) break;
     }

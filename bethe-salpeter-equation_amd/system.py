@@ -41,6 +41,9 @@ class BSESystem:
     meta: dict = field(default_factory=dict)
     dimension: int = 2              # 2: Keldysh W, 2D v; 3: C/(eps (q^2+radius)), C/(q^2+radius)
     radius: float = 0.0             # 3D Coulomb regularisation (1281-1285), 1/Angstrom^2
+    screening: str = "model"        # "model" (r0, eps_env) or "rpa": static RPA from the TB states
+    rpa_nv: int = 0                 # RPA screening window: highest occupied bands (0: all)
+    rpa_nc: int = 0                 # lowest empty bands (0: all)
 
     # ---- derived sizes -------------------------------------------------
     @property

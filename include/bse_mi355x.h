@@ -70,7 +70,8 @@ typedef struct {
 } bse_tb_model;
 
 /* One excitonic problem (q = 0 TDA).  Fields after dipole_dir are new in
- * ABI version 2; zero-initialised they give a 2D system. */
+ * ABI version 2; zero-initialised they give a 2D system with model
+ * screening. */
 typedef struct {
     double bravais[9];     /* column-major 3x3, column i = a_i (93-96)        */
     int32_t n1, n2;        /* k grid, k = ((i+s1)/n1) b1 + ((j+s2)/n2) b2
@@ -101,6 +102,12 @@ typedef struct {
     int32_t n3;            /* k grid along b3 (3D only; 0 or 1 in 2D)        */
     double shift3;         /* grid shift along b3, in grid steps             */
     double radius;         /* 3D regularisation of C/(q^2 + radius), 1/A^2   */
+    int32_t screening;     /* 0: the model W above (r0, eps_env); 1: static
+                              RPA from the TB states (Dielectric_Function::
+                              pull_values, 2214-2303, order 1 at omega = 0;
+                              W = eps^-1 v, 2575-2584)                        */
+    int32_t rpa_nv, rpa_nc;/* RPA screening window: highest occupied / lowest
+                              empty bands (0: all)                           */
 } bse_system;
 
 /* ---- context ------------------------------------------------------------- */

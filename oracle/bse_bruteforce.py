@@ -168,3 +168,57 @@ def assemble_coupling_bruteforce(p: Problem):
                                 acc += fx * vg * a * b
                             B[r, ridx(c2, v2, k2)] += acc
     return B
+
+
+def rpa_w_bruteforce(p: Problem, d):
+    """Static RPA W_GG'(Q) by the loop structure of Dielectric_Function::
+    pull_values (2214-2303, order_approximation = 1, omega = 0, eta -> 0):
+
+      chi0(i, j) = factor_chi sum_r conj(rho_r(i)) rho_r(j) factor_r   (2250-2283)
+      tmp1(i, j) = delta_ij - v_j chi0(i, j);  chi = tmp1^-1 chi0      (2287-2293)
+      eps^-1(i, j) = delta_ij + v_i chi(i, j)                           (2294-2300)
+
+    with the transitions r of both kinds, c at k with v at k - Q and v at k
+    with c at k - Q, each with its own -1/dE (the reference's -2/dE on the
+    first kind is their sum under time reversal), rho_r(G) = sum_a
+    conj(c_left,a) e^{i(Q+G).tau_a} c_right,a over the screening window,
+    factor_chi = f_s/(cell Nk); then W_GG' = eps^-1_GG' v(|Q+G'|) (the Q + G = 0
+    potential is the bare mini-BZ average)."""
+    from bse_oracle import _kidx, bare_q0_average, reciprocal, rpa_window
+    kc = p.kpoints()
+    e, vecs = tb_states(p, kc)
+    ki, kj, kl = _kidx(p)
+    b = reciprocal(p.bravais)
+    qv = (d[0] / p.n1) * b[:, 0] + (d[1] / p.n2) * b[:, 1] + (d[2] / p.n3) * b[:, 2]
+    g = glist(p)
+    ng = g.shape[1]
+    vidx, cidx = rpa_window(p)
+    vg = np.zeros(ng)
+    for s in range(ng):
+        qn = np.linalg.norm(qv + g[:, s])
+        vg[s] = bare_q0_average(p) if qn < 1e-12 else v_model(p, qn)
+    rows, fac = [], []
+    for k in range(p.nk):
+        kq = (((ki[k] - d[0]) % p.n1) * p.n2 + ((kj[k] - d[1]) % p.n2)) * p.n3 + ((kl[k] - d[2]) % p.n3)
+        for c in cidx:
+            for v in vidx:
+                for left, kl_, right, kr_, de in (
+                        (c, k, v, kq, e[k, c] + p.scissor - e[kq, v]),
+                        (v, k, c, kq, e[kq, c] + p.scissor - e[k, v])):
+                    fac.append(-1.0 / de)
+                    rho = np.zeros(ng, dtype=np.complex128)
+                    for s in range(ng):
+                        ph = np.exp(1j * (p.tau @ (qv + g[:, s])))
+                        rho[s] = np.sum(vecs[kl_][:, left].conj() * ph * vecs[kr_][:, right])
+                    rows.append(rho)
+    rho = np.array(rows)
+    fac = np.array(fac)
+    factor_chi = p.f_s / (p.cell * p.nk)
+    chi0 = np.zeros((ng, ng), dtype=np.complex128)
+    for i in range(ng):
+        for j in range(ng):
+            chi0[i, j] = factor_chi * np.sum(rho[:, i].conj() * rho[:, j] * fac)
+    tmp1 = np.eye(ng) - chi0 * vg[None, :]
+    chi = np.linalg.solve(tmp1, chi0)
+    epsinv = np.eye(ng) + vg[:, None] * chi
+    return epsinv * vg[None, :]

@@ -120,6 +120,9 @@ class Problem:
     n3: int = 1
     shift: tuple = (0.0, 0.0, 0.0)
     radius: float = 0.0
+    screening: str = "model"  # "model": W of the dimension's model; "rpa": static RPA from the TB states
+    rpa_nv: int = 0          # screening window (0: every occupied / every empty band)
+    rpa_nc: int = 0
 
     @property
     def nk(self):
@@ -164,7 +167,8 @@ def problem_from_system(s) -> Problem:
                    exchange_scale=s.exchange_scale, direct_scale=s.direct_scale,
                    q=tuple(s.q_opt), eta=s.eta, omegas=s.omegas(),
                    dimension=getattr(s, "dimension", 2), n3=getattr(g, "n3", 1), shift=sh,
-                   radius=getattr(s, "radius", 0.0))
+                   radius=getattr(s, "radius", 0.0), screening=getattr(s, "screening", "model"),
+                   rpa_nv=getattr(s, "rpa_nv", 0), rpa_nc=getattr(s, "rpa_nc", 0))
 
 
 # ----------------------------------------------------------------------------
@@ -301,6 +305,97 @@ def w_table(p: Problem, r0=None, eps_env=None):
     return out
 
 
+def bare_q0_average(p: Problem):
+    """Mini-BZ average of the bare potential around q = 0 (the 2D bare v is
+    the Keldysh form at r0 = 0, eps = 1; in 3D eps = 1)."""
+    if p.dimension == 3:
+        return coulomb3d_q0_average(p.volume, p.nk, p.radius, 1.0)
+    return keldysh_q0_average(p.area, p.nk, 0.0, 1.0)
+
+
+def rpa_window(p: Problem):
+    """Screening bands: the rpa_nv highest occupied and rpa_nc lowest empty
+    (0: all of them)."""
+    nvs = p.n_occ if p.rpa_nv <= 0 else min(p.rpa_nv, p.n_occ)
+    ncs = (p.norb - p.n_occ) if p.rpa_nc <= 0 else min(p.rpa_nc, p.norb - p.n_occ)
+    vidx = p.n_occ - 1 - np.arange(nvs)
+    cidx = p.n_occ + np.arange(ncs)
+    return vidx, cidx
+
+
+def rpa_pi(p: Problem, d, states=None):
+    """Point-orbital factor of the static chi0 at the grid difference d:
+
+      Pi_ij(Q) = (f_s/(cell Nk)) sum_{k,c,v} [ f1 conj(P1_i) P1_j + f2 conj(P2_i) P2_j ],
+      P1_i = conj(c_{c,k,i}) c_{v,k-Q,i},  f1 = -1/(e_c(k) - e_v(k-Q)),
+      P2_i = conj(c_{v,k,i}) c_{c,k-Q,i},  f2 = -1/(e_c(k-Q) - e_v(k)),
+
+    so that chi0_GG'(Q) = sum_ij e^{-i(Q+G).tau_i} Pi_ij e^{i(Q+G').tau_j}.
+    The reference (Dielectric_Function::pull_values 2214-2255) keeps only the
+    first kind of transition, with the static factor 1/(w - dE + i eta) -
+    1/(w + dE - i eta) = -2/dE at w = 0, eta -> 0: the two kinds are equal
+    under time reversal, and summing them separately keeps chi0 reciprocal
+    (chi0_GG'(-Q) = chi0_{-G',-G}(Q)) and the BSE matrix Hermitian without it.
+    The state at k - Q is the grid point k - Q folded into the grid (the TB
+    coefficients are periodic in k)."""
+    kc = p.kpoints()
+    e, v = tb_states(p, kc) if states is None else states
+    ki, kj, kl = _kidx(p)
+    kq = (((ki - d[0]) % p.n1) * p.n2 + ((kj - d[1]) % p.n2)) * p.n3 + ((kl - d[2]) % p.n3)
+    vidx, cidx = rpa_window(p)
+    ec, ev = e[:, cidx] + p.scissor, e[:, vidx]
+    P1 = np.einsum("kic,kiv->kcvi", v[:, :, cidx].conj(), v[kq][:, :, vidx])
+    f1 = -1.0 / (ec[:, :, None] - ev[kq][:, None, :])
+    P2 = np.einsum("kiv,kic->kcvi", v[:, :, vidx].conj(), v[kq][:, :, cidx])
+    f2 = -1.0 / (ec[kq][:, :, None] - ev[:, None, :])
+    pi = (np.einsum("kcv,kcvi,kcvj->ij", f1, P1.conj(), P1)
+          + np.einsum("kcv,kcvi,kcvj->ij", f2, P2.conj(), P2))
+    return (p.f_s / (p.cell * p.nk)) * pi
+
+
+def rpa_w_matrix(p: Problem, d, states=None, scale=1.0):
+    """Static RPA W_GG'(Q) = eps^-1_GG' v(|Q+G'|) = V + V chi V with
+    chi = (1 - chi0 V)^-1 chi0 (the Dyson solve of pull_values 2261-2303,
+    order_approximation = 1), V = diag v(|Q+G|); the Q + G = 0 entry of V is
+    the mini-BZ average of the bare potential (the head and wings of chi0
+    vanish there, the states at k and k - 0 being orthogonal).  scale
+    multiplies chi0 (1: RPA; 0: the bare potential).  Returns (W, Q+G (3, Ng))."""
+    b = reciprocal(p.bravais)
+    qv = (d[0] / p.n1) * b[:, 0] + (d[1] / p.n2) * b[:, 1] + (d[2] / p.n3) * b[:, 2]
+    qg = qv[:, None] + glist(p)
+    qn = np.linalg.norm(qg, axis=0)
+    zero = qn < 1e-12
+    vg = np.where(zero, bare_q0_average(p), v_model(p, np.where(zero, 1.0, qn)))
+    ph = np.exp(1j * (p.tau @ qg))                     # (norb, Ng): e^{i(Q+G).tau_i}
+    chi0 = scale * (ph.conj().T @ rpa_pi(p, d, states) @ ph)
+    chi = np.linalg.solve(np.eye(vg.size) - chi0 * vg[None, :], chi0)
+    W = np.diag(vg) + vg[:, None] * chi * vg[None, :]
+    return W, qg
+
+
+def rpa_w_table(p: Problem, scale=1.0):
+    """W_ij(Q) = sum_GG' e^{i(Q+G).tau_i} W_GG'(Q) e^{-i(Q+G').tau_j} on every
+    grid difference (the layout of w_table); the reference forms
+    v_coulomb_gg(G, G', k-k') from eps^-1 and the potential (2575-2584)."""
+    states = tb_states(p, p.kpoints())
+    d1 = np.arange(-(p.n1 - 1), p.n1)
+    d2 = np.arange(-(p.n2 - 1), p.n2)
+    d3 = np.arange(-(p.n3 - 1), p.n3)
+    out = np.zeros((d1.size, d2.size, d3.size, p.norb, p.norb), dtype=np.complex128)
+    for a, x in enumerate(d1):
+        for c, y in enumerate(d2):
+            for e, z in enumerate(d3):
+                W, qg = rpa_w_matrix(p, (x, y, z), states, scale)
+                ph = np.exp(1j * (p.tau @ qg))
+                out[a, c, e] = ph @ W @ ph.conj().T
+    return out
+
+
+def interaction_table(p: Problem):
+    """The direct term's W_ij(Q): the model W, or the static RPA."""
+    return rpa_w_table(p) if p.screening == "rpa" else w_table(p)
+
+
 OPTICAL_Q = 1e-4   # exciton momenta up to this length are the optical limit q -> 0
 
 
@@ -363,7 +458,7 @@ def exchange_factors(p: Problem, C, V):
 def assemble(p: Problem, bands=None, wt=None, vx=None):
     """Full TDA matrix H = diag(eps_c(k) - eps_v(k-q)) + D + X (reference order)."""
     eps_c, C, eps_v, V = band_window(p) if bands is None else bands
-    wt = w_table(p) if wt is None else wt
+    wt = interaction_table(p) if wt is None else wt
     vx = v_exchange(p) if vx is None else vx
     N = p.n_bse
     H = direct_term(p, C, C, V, V, wt).reshape(N, N)
@@ -460,6 +555,8 @@ def dH_dtheta(p: Problem, name: str):
     parameter of the gradient set (hopping_scale, scissor, r0, eps_env,
     exchange_field).  Uses the product rule on the four coefficient factors."""
     N = p.n_bse
+    if p.screening == "rpa":
+        raise ValueError("the gradient restatement holds W fixed: model screening only")
     if name == "scissor":
         return np.eye(N, dtype=np.complex128)
     bands = band_window(p)
@@ -636,7 +733,7 @@ def assemble_coupling(p: Problem, bands=None, wt=None, vx=None):
     if np.any(exciton_momentum(p) != 0):
         raise ValueError("the coupling block is formed in the optical limit |q| <= OPTICAL_Q")
     eps_c, C, eps_v, V = band_window(p) if bands is None else bands
-    wt = w_table(p) if wt is None else wt
+    wt = interaction_table(p) if wt is None else wt
     vx = v_exchange(p) if vx is None else vx
     N = p.n_bse
     B = coupling_direct_term(p, C, V, wt).reshape(N, N)

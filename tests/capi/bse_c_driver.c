@@ -19,6 +19,7 @@
  *   scales exchange direct   dipole_dir x y z      dimension 2|3
  *   radius x                 precision 0|1|2       n_vectors k (-1: all)
  *   refine 0|1               eta x                 omega w0 w1 n_omega
+ *   screening_model model|rpa                      rpa_window nv nc (0 0: all)
  * <out> gets "N <n>", then "E <energy> <|f|^2>" per exciton, then
  * "eps <omega> <re> <im>" per frequency, then "Z <k>" and the first k
  * eigenvectors (reference row order) as "re im" lines, column by column.
@@ -55,6 +56,7 @@ typedef struct {
     int precision, n_vectors, refine;
     double eta, omega[2];
     int n_omega;
+    int screening, rpa_nv, rpa_nc;
 } params;
 
 static void join_dir(char* dst, size_t cap, const char* base_file, const char* rel) {
@@ -123,6 +125,10 @@ static int read_params(const char* path, params* p) {
         else if (!strcmp(key, "refine")) ok = sscanf(rest, "%d", &p->refine) == 1;
         else if (!strcmp(key, "eta")) ok = sscanf(rest, "%lf", &p->eta) == 1;
         else if (!strcmp(key, "omega")) ok = sscanf(rest, "%lf %lf %d", &p->omega[0], &p->omega[1], &p->n_omega) == 3;
+        else if (!strcmp(key, "screening_model")) {
+            ok = sscanf(rest, "%1023s", val) == 1 && (!strcmp(val, "model") || !strcmp(val, "rpa"));
+            p->screening = ok && !strcmp(val, "rpa");
+        } else if (!strcmp(key, "rpa_window")) ok = sscanf(rest, "%d %d", &p->rpa_nv, &p->rpa_nc) == 2;
         else {
             fprintf(stderr, "%s: unknown key '%s'\n", path, key);
             ok = 0;
@@ -288,6 +294,9 @@ int main(int argc, char** argv) {
     memcpy(sys.dipole_dir, p.dipole_dir, sizeof p.dipole_dir);
     sys.dimension = p.dimension;
     sys.radius = p.radius;
+    sys.screening = p.screening;
+    sys.rpa_nv = p.rpa_nv;
+    sys.rpa_nc = p.rpa_nc;
 
     if (bse_abi_version() != BSE_ABI_VERSION) {
         fprintf(stderr, "library ABI %d, header %d\n", bse_abi_version(), BSE_ABI_VERSION);

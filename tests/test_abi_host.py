@@ -54,11 +54,12 @@ def test_struct_layout_matches_c(tmp_path):
 #include <stddef.h>
 #include "{HEADER}"
 int main(void) {{
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bse_tb_model), sizeof(bse_system),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bse_tb_model), sizeof(bse_system),
          offsetof(bse_system, n_occ), offsetof(bse_system, scissor),
          offsetof(bse_system, r0), offsetof(bse_system, dipole_dir),
          offsetof(bse_system, dimension), offsetof(bse_system, n3),
-         offsetof(bse_system, shift3), offsetof(bse_system, radius));
+         offsetof(bse_system, shift3), offsetof(bse_system, radius),
+         offsetof(bse_system, screening), offsetof(bse_system, rpa_nc));
   return 0;
 }}''')
     exe = tmp_path / "layout"
@@ -68,7 +69,8 @@ int main(void) {{
     S = _abi.SystemC
     assert vals == [ctypes.sizeof(_abi.TBModelC), ctypes.sizeof(S), S.n_occ.offset,
                     S.scissor.offset, S.r0.offset, S.dipole_dir.offset, S.dimension.offset,
-                    S.n3.offset, S.shift3.offset, S.radius.offset]
+                    S.n3.offset, S.shift3.offset, S.radius.offset, S.screening.offset,
+                    S.rpa_nc.offset]
 
 
 def test_hr_dat_roundtrip(tmp_path):

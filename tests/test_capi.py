@@ -59,6 +59,8 @@ def _write_inputs(tmp, s, precision=0, n_vectors=-1, refine=0, omegas=None, eta=
         "dipole_dir 1 0 0",
         f"dimension {int(getattr(s, 'dimension', 2))}",
         f"radius {float(getattr(s, 'radius', 0.0)):.17g}",
+        f"screening_model {getattr(s, 'screening', 'model')}",
+        f"rpa_window {int(getattr(s, 'rpa_nv', 0))} {int(getattr(s, 'rpa_nc', 0))}",
         f"precision {precision}",
         f"n_vectors {n_vectors}",
         f"refine {refine}",
@@ -150,6 +152,18 @@ def test_c_driver_c1_fp32_refined_lowest16(tmp_path):
     scale = np.abs(g["E"]).max()
     assert np.abs(E[:16] - g["E"][:16]).max() < 1e-10 * scale
     assert cluster_overlap_error(Z, g["Z"][:, :16], g["E"], 1e-8 * scale, ncol=16) < 1e-8
+
+
+@pytest.mark.gpu
+def test_c_driver_rpa_screening(tmp_path):
+    s = bse_amd.honeycomb_plain(5).with_params(screening="rpa")
+    params = _write_inputs(tmp_path, s)
+    print("C driver honeycomb 5x5 with RPA screening", flush=True)
+    r = _run(params, tmp_path / "out.txt")
+    assert r.returncode == 0, r.stderr
+    n, E, f2, _, _, _ = _parse(tmp_path / "out.txt")
+    ref = O.solve(O.problem_from_system(s), omegas=None)
+    assert np.abs(E - ref["E"]).max() < 1e-10 * np.abs(ref["E"]).max()
 
 
 @pytest.mark.gpu
