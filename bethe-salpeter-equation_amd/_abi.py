@@ -411,8 +411,8 @@ class Context:
     def profile(self, enable=None):
         """Profile of the last eig's dominant kernel: the one-stage reduction's
         Hermitian matrix-vector product (algorithmic bytes), or the two-stage
-        band reduction's rank-2k update and block product (algorithmic flops,
-        FP64 arithmetic)."""
+        band reduction's rank-2k update and block product (algorithmic flops;
+        FP32 matrix cores for FP32 storage, FP64 arithmetic otherwise)."""
         out = np.zeros(9)
         en = 1 if enable else 0
         if enable is None:
@@ -420,10 +420,12 @@ class Context:
         self._prof = bool(en)
         self._check(self.lib.bse_profile(self.h, en, _dptr(out)), "bse_profile")
         if int(out[4]) == 1:
-            return dict(kind="flops", kernel="band rank-2k update (her2k, FP64 products)", ms=out[0],
-                        flops=out[3], launches=int(out[2]), peak=78.6,
-                        second=dict(kernel="k_band_hemm (X = A22 V, FP64 products)", ms=out[5],
-                                    flops=out[6], launches=int(out[7])))
+            f32 = int(out[1]) == 1
+            tag = "FP32 MFMA" if f32 else "FP64 products"
+            return dict(kind="flops", kernel=f"band rank-2k update ({'k_her2k_mfma32' if f32 else 'her2k'}, {tag})",
+                        ms=out[0], flops=out[3], launches=int(out[2]), peak=157.3 if f32 else 78.6,
+                        second=dict(kernel=f"{'k_band_hemm32' if f32 else 'k_band_hemm'} (X = A22 V, {tag})",
+                                    ms=out[5], flops=out[6], launches=int(out[7])))
         return dict(kind="bytes", kernel="kb_hemv (every variant the tuner picks)", ms=out[0],
                     bytes=out[1], launches=int(out[2]), hemv_ms=out[0], hemv_bytes=out[1],
                     hemv_launches=int(out[2]))

@@ -211,11 +211,13 @@ __host__ __device__ inline int tsqr_r1(int blk, int nblk, int m) { return blk ==
 // vectors overwrite the block below the diagonal, R (b x b, zero below the
 // diagonal) goes to R + blk b b and tau to tau + blk b.
 __global__ __launch_bounds__(256) void k_tsqr_leaf(int m, int b, double2* __restrict__ P,
-                                                   double2* __restrict__ R, double2* __restrict__ tau)
+                                                   double2* __restrict__ R, double2* __restrict__ tau,
+                                                   const int* __restrict__ gate)
 {
     extern __shared__ double2 sA[];
     __shared__ double red[32];
     __shared__ double2 wbuf[8 * 32], stau[BR_B];
+    if (gate && *gate == 0) return;
     const int nblk = tsqr_nblk(m);
     const int blk = blockIdx.x;
     const int r0 = tsqr_r0(blk), rows = tsqr_r1(blk, nblk, m) - r0;
@@ -243,11 +245,12 @@ __global__ __launch_bounds__(256) void k_tsqr_leaf(int m, int b, double2* __rest
 // Vn + g G b b (ld cnt b), tau to taun + g b, the new R to Rout + g b b.
 __global__ __launch_bounds__(256) void k_tsqr_node(int nin, int b, const double2* __restrict__ Rin,
                                                    double2* __restrict__ Vn, double2* __restrict__ taun,
-                                                   double2* __restrict__ Rout)
+                                                   double2* __restrict__ Rout, const int* __restrict__ gate)
 {
     extern __shared__ double2 sA[];
     __shared__ double red[32];
     __shared__ double2 wbuf[8 * 32], stau[BR_B];
+    if (gate && *gate == 0) return;
     const int g = blockIdx.x;
     const int cnt = min(TSQR_G, nin - g * TSQR_G);
     const int rows = cnt * b;
@@ -278,10 +281,11 @@ __global__ __launch_bounds__(256) void k_tsqr_node(int nin, int b, const double2
 __global__ __launch_bounds__(256) void k_tsqr_node_q(int nin, int b, int k, const double2* __restrict__ Vn,
                                                      const double2* __restrict__ taun,
                                                      const double2* __restrict__ Xin, int root,
-                                                     double2* __restrict__ Xout)
+                                                     double2* __restrict__ Xout, const int* __restrict__ gate)
 {
     extern __shared__ double2 sY[];
     __shared__ double2 wbuf[8 * 32], stau[BR_B];
+    if (gate && *gate == 0) return;
     const int g = blockIdx.x;
     const int cnt = min(TSQR_G, nin - g * TSQR_G);
     const int rows = cnt * b;
@@ -307,10 +311,11 @@ __global__ __launch_bounds__(256) void k_tsqr_node_q(int nin, int b, int k, cons
 __global__ __launch_bounds__(256) void k_tsqr_leaf_q(int m, int b, int k, const double2* __restrict__ P,
                                                      const double2* __restrict__ tau,
                                                      const double2* __restrict__ Xin, int root,
-                                                     double2* __restrict__ Q)
+                                                     double2* __restrict__ Q, const int* __restrict__ gate)
 {
     extern __shared__ double2 sY[];
     __shared__ double2 wbuf[8 * 32], stau[BR_B];
+    if (gate && *gate == 0) return;
     const int nblk = tsqr_nblk(m);
     const int blk = blockIdx.x;
     const int r0 = tsqr_r0(blk), rows = tsqr_r1(blk, nblk, m) - r0;
@@ -332,29 +337,25 @@ __global__ __launch_bounds__(256) void k_tsqr_leaf_q(int m, int b, int k, const 
 }
 
 // ---------------------------------------------------------------------------
-// Householder form of Q_thin (m x k, k = min(m, b)).  One workgroup:
-//   top block: LU of E - Q D with d_j = -conj(w_j)/|w_j| from the running
-//   Schur entry w_j, so u_jj = 1 + |w_j| >= 1; U(j, c) = -d_c W(j, c);
+// Householder form of Q_thin (m x k, k = min(m, b)), from its top k x k block
+// W (in LDS, W[i][c]):
+//   LU of E - Q D with d_j = -conj(w_j)/|w_j| from the running Schur entry
+//   w_j, so u_jj = 1 + |w_j| >= 1; U(j, c) = -d_c W(j, c);
 //   L1 (unit lower) -> V rows 0..k-1;  T = U L1^{-H} -> Tp (ld b);
-//   Mlow = -D U^{-1} (for the rows below: L2 = Q2 Mlow);
-//   R' = D^H Rfin (k x b) -> the band store, rows s.. of columns c0..
+//   Mlow = -D U^{-1} (for the rows below: L2 = Q2 Mlow) -> Mlow (ld b), and
+//   left in LDS as Ui (U^{-1}) and d for the caller;
+//   R' = D^H Rfin (k x b, Rfin ld b) -> the band store, rows s.. of columns c0..
+// 256 threads; W is overwritten.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_hh_reconstruct(int m, int b, int k, const double2* __restrict__ Q,
-                                                        const double2* __restrict__ Rfin,
-                                                        double2* __restrict__ V, double2* __restrict__ Tp,
-                                                        double2* __restrict__ Mlow, double2* __restrict__ AB,
-                                                        int ldab, int c0)
+typedef double2 hh_sq[BR_B][BR_B + 1];
+__device__ __attribute__((always_inline)) inline void hh_reconstruct_lds(
+    int m, int b, int k, hh_sq& W, hh_sq& L, hh_sq& U, hh_sq& Ui, hh_sq& Li, double2* d,
+    const double2* __restrict__ Rfin, double2* __restrict__ V, double2* __restrict__ Tp,
+    double2* __restrict__ Mlow, double2* __restrict__ AB, int ldab, int c0)
 {
-    __shared__ double2 W[BR_B][BR_B + 1];     // W[i][c]: running Schur values of the Q part
-    __shared__ double2 L[BR_B][BR_B + 1];
-    __shared__ double2 U[BR_B][BR_B + 1];
-    __shared__ double2 Ui[BR_B][BR_B + 1];    // U^{-1}
-    __shared__ double2 Li[BR_B][BR_B + 1];    // L1^{-1}
-    __shared__ double2 d[BR_B];
     const int tid = threadIdx.x;
     for (int e = tid; e < BR_B * BR_B; e += 256) {
         const int i = e % BR_B, c = e / BR_B;
-        W[i][c] = (i < k && c < k) ? Q[i + (int64_t)c * m] : cmk(0, 0);
         L[i][c] = cmk(i == c ? 1.0 : 0.0, 0.0);
         U[i][c] = cmk(0, 0);
         Ui[i][c] = cmk(0, 0);
@@ -410,11 +411,11 @@ __global__ __launch_bounds__(256) void k_hh_reconstruct(int m, int b, int k, con
             double2 t = cmk(0, 0);
             for (int l = 0; l < k; ++l) cfma(t, U[i][l], cconj(Li[c][l]));
             Tp[i + c * b] = t;
-            // Mlow = -D U^{-1}: row scaling of U^{-1}... (-Q2 D) U^{-1} = Q2 (-D U^{-1})
-            Mlow[i + c * b] = cmul(cmk(-d[i].x, -d[i].y), Ui[i][c]);
+            // Mlow = -D U^{-1}: (-Q2 D) U^{-1} = Q2 (-D U^{-1})
+            if (Mlow) Mlow[i + c * b] = cmul(cmk(-d[i].x, -d[i].y), Ui[i][c]);
         } else if (i < b && c < b) {
             Tp[i + c * b] = cmk(0, 0);
-            Mlow[i + c * b] = cmk(0, 0);
+            if (Mlow) Mlow[i + c * b] = cmk(0, 0);
         }
     }
     // R' = D^H Rfin (k x b): rows s .. s + k - 1 of columns c0 .. c0 + b - 1
@@ -426,25 +427,291 @@ __global__ __launch_bounds__(256) void k_hh_reconstruct(int m, int b, int k, con
     }
 }
 
-// V rows k..m-1:  L2 = Q2 Mlow (Mlow k x k, ld b).  One thread per row.
-__global__ __launch_bounds__(256) void k_hh_lower(int m, int b, int k, const double2* __restrict__ Q,
-                                                  const double2* __restrict__ Mlow, double2* __restrict__ V)
+// gate (may be null): the kernel runs only when *gate != 0 (the CholeskyQR2
+// path below sets it when a panel needs the TSQR fallback).
+__device__ inline bool gate_closed(const int* gate) { return gate && *gate == 0; }
+
+__global__ __launch_bounds__(256) void k_hh_reconstruct(int m, int b, int k, const double2* __restrict__ Q,
+                                                        const double2* __restrict__ Rfin,
+                                                        double2* __restrict__ V, double2* __restrict__ Tp,
+                                                        double2* __restrict__ Mlow, double2* __restrict__ AB,
+                                                        int ldab, int c0, const int* __restrict__ gate)
 {
-    __shared__ double2 sM[BR_B * BR_B];
-    for (int e = threadIdx.x; e < b * b; e += 256) sM[e] = Mlow[e];
-    __syncthreads();
-    const int r = k + blockIdx.x * 256 + threadIdx.x;
-    if (r >= m) return;
-    double2 q[BR_B];
-#pragma unroll
-    for (int l = 0; l < BR_B; ++l) q[l] = (l < k) ? Q[r + (int64_t)l * m] : cmk(0, 0);
-    for (int c = 0; c < k; ++c) {
-        double2 s = cmk(0, 0);
-#pragma unroll
-        for (int l = 0; l < BR_B; ++l)
-            if (l < k) cfma(s, q[l], sM[l + c * b]);
-        V[r + (int64_t)c * m] = s;
+    __shared__ hh_sq W, L, U, Ui, Li;
+    __shared__ double2 d[BR_B];
+    if (gate_closed(gate)) return;
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, c = e / BR_B;
+        W[i][c] = (i < k && c < k) ? Q[i + (int64_t)c * m] : cmk(0, 0);
     }
+    __syncthreads();
+    hh_reconstruct_lds(m, b, k, W, L, U, Ui, Li, d, Rfin, V, Tp, Mlow, AB, ldab, c0);
+}
+
+// Y (rows k0 .. m-1, ld m) = X M (mode 0) or Y -= X M (mode 1), X (ld m) with
+// kx columns, M (kx x ny, ld b) staged in LDS.  Workgroup: 64 rows x 4 groups
+// of 8 output columns (a wave reads 64 consecutive rows of a column).
+__global__ __launch_bounds__(256) void k_rows_small(int m, int k0, int kx, int ny, int b,
+                                                    const double2* __restrict__ X,
+                                                    const double2* __restrict__ M, double2* __restrict__ Y,
+                                                    int mode, const int* __restrict__ gate)
+{
+    __shared__ double2 sM[BR_B][BR_B + 1];
+    if (gate_closed(gate)) return;
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, c = e / BR_B;
+        sM[i][c] = (i < kx && c < ny) ? M[i + c * b] : cmk(0, 0);
+    }
+    __syncthreads();
+    const int r = k0 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int cg = (threadIdx.x >> 6) * 8;
+    if (r >= m || cg >= ny) return;
+    double2 x[BR_B];
+#pragma unroll
+    for (int l = 0; l < BR_B; ++l) x[l] = (l < kx) ? X[r + (int64_t)l * m] : cmk(0, 0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int c = cg + u;
+        if (c >= ny) break;
+        double2 s = mode ? Y[r + (int64_t)c * m] : cmk(0, 0);
+        double2 t = cmk(0, 0);
+#pragma unroll
+        for (int l = 0; l < BR_B; ++l) cfma(t, x[l], sM[l][c]);
+        Y[r + (int64_t)c * m] = mode ? csub(s, t) : t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CholeskyQR2 of the m x 32 panel (the fast path; Fukaya, Nakatsukasa,
+// Yanagisawa & Yamamoto 2014): G1 = P^H P = R1^H R1, Q1 = P R1^{-1},
+// G2 = Q1^H Q1 = R2^H R2, Q = Q1 R2^{-1}, R = R2 R1.  Every pass over the
+// panel is row-parallel; the two Cholesky factorisations and the Householder
+// reconstruction run in one workgroup each.  The panel goes to the TSQR
+// (flag = 1) when a pivot of G1 falls below 1e-12 of its diagonal entry
+// (cond(P) beyond ~1e6, where one more pass no longer restores orthogonality
+// to rounding level), when a column is zero, or when G2 is not within 0.1 of
+// the identity.
+// ---------------------------------------------------------------------------
+#define CQ_G 64           // workgroups of the row-parallel passes
+
+// PASS 0: partial G = P^H P over the workgroup's 64-row chunks (chunks g,
+// g + grid, ...).  PASS 1: Q1 = P R1^{-1} for those rows (to Q) and the
+// partial G = Q1^H Q1.  Partial g (b x b, column-major) -> Gp + g b b.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_cq_gram(int m, const double2* __restrict__ P,
+                                                 const double2* __restrict__ Rinv, double2* __restrict__ Q,
+                                                 double2* __restrict__ Gp, int* __restrict__ flag)
+{
+    __shared__ double2 sP[64][BR_B + 1];
+    __shared__ double2 sR[BR_B][BR_B + 1];
+    const int tid = threadIdx.x;
+    if (PASS == 0) {
+        if (blockIdx.x == 0 && tid == 0) { flag[0] = 0; flag[1] = 0; }
+    } else {
+        if (*flag) return;
+        for (int e = tid; e < BR_B * BR_B; e += 256) sR[e % BR_B][e / BR_B] = Rinv[e];
+    }
+    double2 g[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) g[u] = cmk(0, 0);
+    const int nch = (int)ceil_div64(m, 64);
+    for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        const int r0 = ch * 64, rows = min(64, m - r0);
+        __syncthreads();
+        for (int e = tid; e < 64 * BR_B; e += 256) {
+            const int r = e & 63, c = e >> 6;
+            sP[r][c] = (r < rows) ? P[(r0 + r) + (int64_t)c * m] : cmk(0, 0);
+        }
+        __syncthreads();
+        if (PASS == 1) {
+            const int r = tid & 63, c0 = (tid >> 6) * 8;
+            double2 q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) q[u] = cmk(0, 0);
+            for (int l = 0; l < BR_B; ++l) {
+                const double2 p = sP[r][l];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) cfma(q[u], p, sR[l][c0 + u]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                sP[r][c0 + u] = q[u];
+                if (r < rows) Q[(r0 + r) + (int64_t)(c0 + u) * m] = q[u];
+            }
+            __syncthreads();
+        }
+        const int i = tid & 31, j0 = tid >> 5;
+        for (int r = 0; r < 64; ++r) {
+            const double2 pi = sP[r][i];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cfmac(g[u], pi, sP[r][j0 + 8 * u]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) Gp[(int64_t)blockIdx.x * BR_B * BR_B + tid + 256 * u] = g[u];
+}
+
+// Sum of the ng partial Gram matrices into sG (fixed order), then its
+// Cholesky factor in place (lower triangle, by the first wave: lane = row).
+// pass 1 also checks |G - I| <= 0.1.  Returns false (uniformly) on failure.
+__device__ bool cq_sum_chol(int ng, const double2* __restrict__ Gp, int pass, hh_sq& sG, double* diag0,
+                            int* fl)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) *fl = 0;
+    for (int e = tid; e < BR_B * BR_B; e += 256) {
+        double2 acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = cmk(0, 0);
+        int g = 0;
+        for (; g + 4 <= ng; g += 4) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = cadd(acc[q], Gp[(int64_t)(g + q) * BR_B * BR_B + e]);
+        }
+        for (; g < ng; ++g) acc[0] = cadd(acc[0], Gp[(int64_t)g * BR_B * BR_B + e]);
+        const double2 s = cadd(cadd(acc[0], acc[1]), cadd(acc[2], acc[3]));
+        const int i = e % BR_B, j = e / BR_B;
+        sG[i][j] = s;
+        if (i == j) diag0[i] = s.x;
+    }
+    __syncthreads();
+    if (pass == 1) {
+        for (int e = tid; e < BR_B * BR_B; e += 256) {
+            const int i = e % BR_B, j = e / BR_B;
+            const double2 x = csub(sG[i][j], cmk(i == j ? 1.0 : 0.0, 0.0));
+            if (!(fabs(x.x) <= 0.1 && fabs(x.y) <= 0.1)) *fl = 1;
+        }
+        __syncthreads();
+    }
+    if (tid < 64 && *fl == 0) {
+        const int lane = tid;
+        for (int j = 0; j < BR_B; ++j) {
+            const double dj = sG[j][j].x;
+            if (!(dj > 1e-12 * diag0[j] && diag0[j] > 0.0)) {
+                if (lane == 0) *fl = 1;
+                break;
+            }
+            const double ljj = sqrt(dj);
+            if (lane > j && lane < BR_B) sG[lane][j] = cscale(sG[lane][j], 1.0 / ljj);
+            if (lane == j) sG[j][j] = cmk(ljj, 0.0);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            if (lane > j && lane < BR_B) {
+                const double2 lij = sG[lane][j];
+                for (int c = j + 1; c <= lane; ++c) sG[lane][c] = csub(sG[lane][c], cmul(lij, cconj(sG[c][j])));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    __syncthreads();
+    return *fl == 0;
+}
+
+// Ri = R^{-1} for R = L^H (L lower in sL), one column per thread.
+__device__ void cq_rinv(hh_sq& sL, hh_sq& Ri)
+{
+    const int c = threadIdx.x;
+    if (c < BR_B) {
+        for (int i = BR_B - 1; i > c; --i) Ri[i][c] = cmk(0, 0);
+        for (int i = c; i >= 0; --i) {
+            double2 s = cmk(i == c ? 1.0 : 0.0, 0.0);
+            for (int l = i + 1; l <= c; ++l) s = csub(s, cmul(cconj(sL[l][i]), Ri[l][c]));
+            Ri[i][c] = cscale(s, 1.0 / sL[i][i].x);
+        }
+    }
+    __syncthreads();
+}
+
+// First factorisation: R1 (upper, ld b) and R1^{-1} to global.
+__global__ __launch_bounds__(256) void k_cq_chol1(int ng, const double2* __restrict__ Gp,
+                                                  double2* __restrict__ R1, double2* __restrict__ R1i,
+                                                  int* __restrict__ flag)
+{
+    __shared__ hh_sq sG, Ri;
+    __shared__ double diag0[BR_B];
+    __shared__ int fl;
+    if (!cq_sum_chol(ng, Gp, 0, sG, diag0, &fl)) {
+        if (threadIdx.x == 0) *flag = 1;
+        return;
+    }
+    cq_rinv(sG, Ri);
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, j = e / BR_B;
+        R1[e] = (i <= j) ? cconj(sG[j][i]) : cmk(0, 0);
+        R1i[e] = Ri[i][j];
+    }
+}
+
+// Second factorisation and the Householder form: R = R2 R1 -> Rf; the top
+// block W = Q1(0:32, :) R2^{-1}; hh_reconstruct_lds; Mp = R2^{-1} Mlow for
+// the rows below (V = Q1 Mp there).  Nothing is written when it fails.
+__global__ __launch_bounds__(256) void k_cq_chol2(int m, int b, int ng, const double2* __restrict__ Gp,
+                                                  const double2* __restrict__ Q1, const double2* __restrict__ R1,
+                                                  double2* __restrict__ Rf, double2* __restrict__ V,
+                                                  double2* __restrict__ Tp, double2* __restrict__ Mp,
+                                                  double2* __restrict__ AB, int ldab, int c0,
+                                                  int* __restrict__ flag)
+{
+    __shared__ hh_sq W, L, U, Ui, Li;
+    __shared__ double2 d[BR_B];
+    __shared__ double diag0[BR_B];
+    __shared__ int fl;
+    if (*flag) return;
+    // Cholesky of G2 in W, R2^{-1} in L
+    if (!cq_sum_chol(ng, Gp, 1, W, diag0, &fl)) {
+        if (threadIdx.x == 0) *flag = 1;
+        return;
+    }
+    cq_rinv(W, L);
+    // R = R2 R1 (R2(i, l) = conj(W(l, i))), into Rf; both upper
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, j = e / BR_B;
+        double2 s = cmk(0, 0);
+        for (int l = i; l <= j; ++l) cfma(s, cconj(W[l][i]), R1[l + j * BR_B]);
+        Rf[e] = (i <= j) ? s : cmk(0, 0);
+        // top block of Q = Q1 R2^{-1}
+        double2 t = cmk(0, 0);
+        for (int l = 0; l <= j; ++l) cfma(t, Q1[i + (int64_t)l * m], L[l][j]);
+        U[i][j] = t;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, j = e / BR_B;
+        Li[i][j] = L[i][j];       // keep R2^{-1}
+        W[i][j] = U[i][j];
+    }
+    __syncthreads();
+    // R2^{-1} survives in Ui's storage only after the reconstruction, so it
+    // is kept in registers: 4 entries per thread
+    double2 r2i[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = threadIdx.x + 256 * u;
+        r2i[u] = Li[e % BR_B][e / BR_B];
+    }
+    __syncthreads();
+    hh_reconstruct_lds(m, b, BR_B, W, L, U, Ui, Li, d, Rf, V, Tp, nullptr, AB, ldab, c0);
+    __syncthreads();
+    // Mlow = -D U^{-1} into W, R2^{-1} back into L, then Mp = R2^{-1} Mlow
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, j = e / BR_B;
+        W[i][j] = cmul(cmk(-d[i].x, -d[i].y), Ui[i][j]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = threadIdx.x + 256 * u;
+        L[e % BR_B][e / BR_B] = r2i[u];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, j = e / BR_B;
+        double2 s = cmk(0, 0);
+        for (int l = i; l < BR_B; ++l) cfma(s, L[i][l], W[l][j]);
+        Mp[i + j * b] = s;
+    }
+    if (threadIdx.x == 0) flag[1] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -530,13 +797,134 @@ __global__ __launch_bounds__(256) void k_band_hemm(int n, int s, int k, const T*
 }
 
 // ---------------------------------------------------------------------------
+// The same product for FP32 storage on matrix cores (v_mfma_f32_16x16x4_f32,
+// FP32 operands and accumulation; the KSPLIT partials are summed in FP64 by
+// k_band_xt).  Workgroup (i, q) as in k_band_hemm; wave w owns rows 16 w ..
+// 16 w + 15 of the 64-row block and all 32 output columns.  The realified
+// contraction runs over (k, part) pairs, 4 per MFMA step (lanes 16 h .. 16 h
+// + 15 take k = 2 step + h / 2, part = h % 2):
+//   Re X = sum Re A Re V - Im A Im V,   Im X = sum Im A Re V + Re A Im V,
+// so a lane feeds (Re A | -Im A) to the Re X product and (Im A | Re A) to the
+// Im X product, and (Re V | Im V) to both.  The tile A_IJ (or A_JI^H) is
+// staged in LDS as sA[k][row]; the next tile's loads are issued into
+// registers before the current tile's products.
+// ---------------------------------------------------------------------------
+typedef float hemm_f4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_band_hemm32(int n, int s, int k, const float2* __restrict__ A,
+                                                     const double2* __restrict__ V, int64_t ldv,
+                                                     double2* __restrict__ Xp)
+{
+    __shared__ float2 sA[64][65];
+    __shared__ float2 sV[64][BR_B + 1];
+    const int nt = tile_count(n), tcs = s >> 6, nT = nt - tcs;
+    const int i = blockIdx.x, q = blockIdx.y;
+    const int I = tcs + i;
+    const int j0 = (int)((int64_t)nT * q / HEMM_KSPLIT), j1 = (int)((int64_t)nT * (q + 1) / HEMM_KSPLIT);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int h = lane >> 4, l16 = lane & 15;
+    hemm_f4 aR[2], aI[2];
+    double dR[2][4], dI[2][4];     // FP64 sums of the FP32 partials
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        aR[c] = hemm_f4{0, 0, 0, 0};
+        aI[c] = hemm_f4{0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) { dR[c][r] = 0.0; dI[c][r] = 0.0; }
+    }
+    float2 ra[16];
+    double2 rv[8];
+    // loads of tile jt into registers: element e = tid + 256 u
+    auto fetch = [&](int jt) {
+        const int J = tcs + jt;
+        const bool tr = J > I;
+        const float2* tile = A + tile_pos(tr ? J : I, tr ? I : J, nt) * (BSE_TILE * BSE_TILE);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) ra[u] = tile[tid + 256 * u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = tid + 256 * u, kk = e >> 5, c = e & 31;
+            const int64_t gr = (int64_t)J * 64 + kk;
+            rv[u] = (c < k && gr >= s && gr < n) ? V[(gr - s) + (int64_t)c * ldv] : cmk(0, 0);
+        }
+    };
+    if (j0 < j1) fetch(j0);
+    for (int jt = j0; jt < j1; ++jt) {
+        const int J = tcs + jt;
+        const bool tr = J > I, diag = (J == I);
+        __syncthreads();    // the previous tile's products are done with sA, sV
+        if (!tr) {
+            // element e = row + 64 kk of A_IJ
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = tid + 256 * u, row = e & 63, kk = e >> 6;
+                float2 x = ra[u];
+                if (diag && row == kk) x.y = 0.0f;
+                if (!(diag && row < kk)) sA[kk][row] = x;
+                // on a diagonal tile the stored element (row, kk), row > kk,
+                // also gives A(kk, row) = conj(A(row, kk))
+                if (diag && row > kk) sA[row][kk] = make_float2(x.x, -x.y);
+            }
+        } else {
+            // element e = kk + 64 row of A_JI: A_IJ(row, kk) = conj(A_JI(kk, row))
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = tid + 256 * u, kk = e & 63, row = e >> 6;
+                sA[kk][row] = make_float2(ra[u].x, -ra[u].y);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = tid + 256 * u;
+            sV[e >> 5][e & 31] = make_float2((float)rv[u].x, (float)rv[u].y);
+        }
+        __syncthreads();
+        if (jt + 1 < j1) fetch(jt + 1);
+        const int row = 16 * w + l16;
+        // FP32 accumulation runs over 16 steps (64 real products) at a time;
+        // the partial then goes into the FP64 sums
+        for (int s0 = 0; s0 < 32; s0 += 16) {
+#pragma unroll 8
+            for (int st = s0; st < s0 + 16; ++st) {
+                const int kk = 2 * st + (h >> 1);
+                const bool im = h & 1;
+                const float2 a = sA[kk][row];
+                const float opR = im ? -a.y : a.x;     // Re X operand
+                const float opI = im ? a.x : a.y;      // Im X operand
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const float2 v = sV[kk][16 * c + l16];
+                    const float bv = im ? v.y : v.x;
+                    aR[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(opR, bv, aR[c], 0, 0, 0);
+                    aI[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(opI, bv, aI[c], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) { dR[c][r] += (double)aR[c][r]; dI[c][r] += (double)aI[c][r]; }
+                aR[c] = hemm_f4{0, 0, 0, 0};
+                aI[c] = hemm_f4{0, 0, 0, 0};
+            }
+        }
+    }
+    // lane, reg r: X(16 w + 4 h + r, 16 c + l16)
+    double2* out = Xp + (int64_t)q * nT * 64 * BR_B + (int64_t)i * 64;
+    const int64_t ldx = (int64_t)nT * 64;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            out[(16 * w + 4 * h + r) + (int64_t)(16 * c + l16) * ldx] = cmk(dR[c][r], dI[c][r]);
+}
+
+// ---------------------------------------------------------------------------
 // Z = X T - V M with M = (1/2) T^H (V^H X T).  Rows of X are tile-aligned in
 // Xp (row t of the trailing tile range <-> global row tcs 64 + t); V rows
 // start at s.
 //   k_band_xt:   XT = (sum of the KSPLIT partials) T for 64 rows per block,
 //                into Z (m x b, ld m), and the partial V^H XT of the block
 //   k_band_m:    M from the partials (one workgroup, fixed order)
-//   k_band_z:    Z -= V M
+//   k_rows_small (mode 1): Z -= V M
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_band_xt(int n, int s, int k, int b, const double2* __restrict__ Xp,
                                                  const double2* __restrict__ Tp, const double2* __restrict__ V,
@@ -594,53 +982,31 @@ __global__ __launch_bounds__(256) void k_band_xt(int n, int s, int k, int b, con
     }
 }
 
+// One workgroup per column j < k of G: the nblk partials summed in eight
+// strided groups, the groups in order, then M(:, j) = (1/2) T^H G(:, j).
 __global__ __launch_bounds__(256) void k_band_m(int nblk, int k, int b, const double2* __restrict__ Gp,
                                                 const double2* __restrict__ Tp, double2* __restrict__ M)
 {
-    __shared__ double2 sG[BR_B][BR_B + 1];
-    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
-        // eight independent partial sums keep eight loads in flight; they
-        // are combined in a fixed order
-        double2 acc[8];
+    __shared__ double2 part[8][BR_B];
+    __shared__ double2 sG[BR_B];
+    const int j = blockIdx.x;
+    const int l = threadIdx.x & 31, q = threadIdx.x >> 5;
+    double2 acc = cmk(0, 0);
+    for (int g = q; g < nblk; g += 8) acc = cadd(acc, Gp[(int64_t)g * BR_B * BR_B + l + j * BR_B]);
+    part[q][l] = acc;
+    __syncthreads();
+    if (threadIdx.x < BR_B) {
+        double2 t = part[0][l];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) acc[q] = cmk(0, 0);
-        int g = 0;
-        for (; g + 8 <= nblk; g += 8) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) acc[q] = cadd(acc[q], Gp[(int64_t)(g + q) * BR_B * BR_B + e]);
-        }
-        for (; g < nblk; ++g) acc[0] = cadd(acc[0], Gp[(int64_t)g * BR_B * BR_B + e]);
-#pragma unroll
-        for (int q = 1; q < 8; ++q) acc[0] = cadd(acc[0], acc[q]);
-        sG[e % BR_B][e / BR_B] = acc[0];
+        for (int u = 1; u < 8; ++u) t = cadd(t, part[u][l]);
+        sG[l] = t;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
-        const int i = e % BR_B, j = e / BR_B;
-        double2 acc = cmk(0, 0);
-        if (i < k && j < k)
-            for (int l = 0; l < k; ++l) cfmac(acc, Tp[l + i * b], sG[l][j]);   // (T^H G)(i, j)
-        M[i + j * b] = cscale(acc, 0.5);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_band_z(int m, int k, int b, const double2* __restrict__ V,
-                                                const double2* __restrict__ M, double2* __restrict__ Z)
-{
-    __shared__ double2 sM[BR_B * BR_B];
-    for (int e = threadIdx.x; e < b * b; e += 256) sM[e] = M[e];
-    __syncthreads();
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= m) return;
-    double2 v[BR_B];
-#pragma unroll
-    for (int l = 0; l < BR_B; ++l) v[l] = (l < k) ? V[r + (int64_t)l * m] : cmk(0, 0);
-    for (int c = 0; c < k; ++c) {
-        double2 s = Z[r + (int64_t)c * m];
-#pragma unroll
-        for (int l = 0; l < BR_B; ++l)
-            if (l < k) s = csub(s, cmul(v[l], sM[l + c * b]));
-        Z[r + (int64_t)c * m] = s;
+    const int i = threadIdx.x;
+    if (i < k) {
+        double2 t = cmk(0, 0);
+        for (int ll = 0; ll < k; ++ll) cfmac(t, Tp[ll + i * b], sG[ll]);   // (T^H G)(i, j)
+        M[i + j * b] = cscale(t, 0.5);
     }
 }
 
@@ -673,7 +1039,8 @@ extern "C" hipError_t bse_band_gather_tail(hipStream_t st, int fp32, int n, int 
 }
 
 // Workspace of one panel QR of an m-row panel (complex elements): leaf R and
-// tau, tree levels (V, tau, R per node; X pieces), the thin Q.
+// tau, tree levels (V, tau, R per node; X pieces), the thin Q, then the
+// CholeskyQR2 partial Gram matrices, R1, R1^{-1}, R, Mp and two flags.
 extern "C" int64_t bse_tsqr_workspace(int m, int b) {
     int64_t el = 0;
     int nin = tsqr_nblk(m);
@@ -684,13 +1051,16 @@ extern "C" int64_t bse_tsqr_workspace(int m, int b) {
         el += (int64_t)nn * TSQR_G * b * b + (int64_t)nn * b + (int64_t)nn * b * b;
         nin = nn;
     }
-    el += (int64_t)m * b;                                 // Q_thin
+    el += (int64_t)m * b;                                 // Q_thin (Q1 on the fast path)
+    el += (int64_t)CQ_G * BR_B * BR_B + 4 * BR_B * BR_B + 1;
     return el + 64;
 }
 
 // QR of the m x b panel P (ld m) as I - V T V^H: V (m x k) into Vout (ld m),
 // T into Tp (ld b), R' = D^H R into the band store at (c0 + b.., c0..).
-// Returns k = min(m, b) in *kout.
+// Returns k = min(m, b) in *kout.  Panels of 32 columns and at least 64
+// rows take the CholeskyQR2 path; the TSQR kernels run after it and return
+// at once unless it flagged the panel (so no host round trip decides).
 extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P, double2* Vout,
                                         double2* Tp, double2* ws, double2* AB, int ldab, int c0, int* kout)
 {
@@ -721,7 +1091,6 @@ extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P
     struct Lvl { int nin, nn; double2 *V, *tau, *R; };
     Lvl lv[16];
     int nl = 0, nin = nleaf;
-    const double2* Rin = Rleaf;
     while (nin > 1) {
         Lvl L;
         L.nin = nin;
@@ -732,17 +1101,38 @@ extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P
         lv[nl++] = L;
         nin = L.nn;
     }
-    double2* Q = p;
-    // leaf QRs
+    double2* Q = p; p += (int64_t)m * b;
+    double2* Gp = p; p += (int64_t)CQ_G * BR_B * BR_B;
+    double2* R1 = p; p += BR_B * BR_B;
+    double2* R1i = p; p += BR_B * BR_B;
+    double2* Rf = p; p += BR_B * BR_B;
+    double2* Mp = p; p += BR_B * BR_B;
+    int* flags = reinterpret_cast<int*>(p);     // [0]: TSQR needed, [1]: fast path done
+    const bool fast = (b == BR_B && m >= 2 * BR_B);
+    const int* gate = nullptr;
+    if (fast) {
+        const int g = (int)std::min<int64_t>(CQ_G, ceil_div64(m, 64));
+        hipLaunchKernelGGL(k_cq_gram<0>, dim3((unsigned)g), dim3(256), 0, st, m, (const double2*)P,
+                           (const double2*)nullptr, Q, Gp, flags);
+        hipLaunchKernelGGL(k_cq_chol1, dim3(1), dim3(256), 0, st, g, (const double2*)Gp, R1, R1i, flags);
+        hipLaunchKernelGGL(k_cq_gram<1>, dim3((unsigned)g), dim3(256), 0, st, m, (const double2*)P,
+                           (const double2*)R1i, Q, Gp, flags);
+        hipLaunchKernelGGL(k_cq_chol2, dim3(1), dim3(256), 0, st, m, b, g, (const double2*)Gp,
+                           (const double2*)Q, (const double2*)R1, Rf, Vout, Tp, Mp, AB, ldab, c0, flags);
+        hipLaunchKernelGGL(k_rows_small, dim3((unsigned)ceil_div64(m - k, 64)), dim3(256), 0, st, m, k, BR_B, k, b,
+                           (const double2*)Q, (const double2*)Mp, Vout, 0, (const int*)(flags + 1));
+        gate = flags;
+    }
+    // TSQR: leaf QRs
     const int maxrows_leaf = (nleaf == 1) ? m : 2 * TSQR_RB - 1;
     if (maxrows_leaf > TSQR_MAXROWS) return hipErrorInvalidValue;
     const size_t lds_leaf = (size_t)tsqr_ld(maxrows_leaf) * b * sizeof(double2);
-    hipLaunchKernelGGL(k_tsqr_leaf, dim3((unsigned)nleaf), dim3(256), lds_leaf, st, m, b, P, Rleaf, tauleaf);
+    hipLaunchKernelGGL(k_tsqr_leaf, dim3((unsigned)nleaf), dim3(256), lds_leaf, st, m, b, P, Rleaf, tauleaf, gate);
     // up the tree
     for (int l = 0; l < nl; ++l) {
         const size_t lds = (size_t)tsqr_ld(TSQR_G * b) * b * sizeof(double2);
         hipLaunchKernelGGL(k_tsqr_node, dim3((unsigned)lv[l].nn), dim3(256), lds, st, lv[l].nin, b,
-                           l == 0 ? Rleaf : lv[l - 1].R, lv[l].V, lv[l].tau, lv[l].R);
+                           l == 0 ? Rleaf : lv[l - 1].R, lv[l].V, lv[l].tau, lv[l].R, gate);
     }
     const double2* Rfin = nl ? lv[nl - 1].R : Rleaf;
     // down the tree
@@ -751,19 +1141,19 @@ extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P
     for (int l = nl - 1; l >= 0; --l) {
         const size_t lds = (size_t)tsqr_ld(TSQR_G * b) * k * sizeof(double2);
         hipLaunchKernelGGL(k_tsqr_node_q, dim3((unsigned)lv[l].nn), dim3(256), lds, st, lv[l].nin, b, k,
-                           lv[l].V, lv[l].tau, (const double2*)Xcur, l == nl - 1 ? 1 : 0, Xnext);
+                           lv[l].V, lv[l].tau, (const double2*)Xcur, l == nl - 1 ? 1 : 0, Xnext, gate);
         std::swap(Xcur, Xnext);
     }
     hipLaunchKernelGGL(k_tsqr_leaf_q, dim3((unsigned)nleaf), dim3(256), (size_t)tsqr_ld(maxrows_leaf) * k * sizeof(double2),
                        st, m, b, k, (const double2*)P, (const double2*)tauleaf, (const double2*)Xcur,
-                       nl == 0 ? 1 : 0, Q);
+                       nl == 0 ? 1 : 0, Q, gate);
     // Householder form
     double2* Mlow = Xnext;   // b x b scratch
     hipLaunchKernelGGL(k_hh_reconstruct, dim3(1), dim3(256), 0, st, m, b, k, (const double2*)Q, Rfin, Vout, Tp,
-                       Mlow, AB, ldab, c0);
+                       Mlow, AB, ldab, c0, gate);
     if (m > k)
-        hipLaunchKernelGGL(k_hh_lower, dim3((unsigned)ceil_div64(m - k, 256)), dim3(256), 0, st, m, b, k,
-                           (const double2*)Q, (const double2*)Mlow, Vout);
+        hipLaunchKernelGGL(k_rows_small, dim3((unsigned)ceil_div64(m - k, 64)), dim3(256), 0, st, m, k, k, k, b,
+                           (const double2*)Q, (const double2*)Mlow, Vout, 0, gate);
     return hipGetLastError();
 }
 
@@ -776,7 +1166,8 @@ extern "C" int64_t bse_band_update_workspace(int n, int s, int b) {
 }
 
 // A22 <- Q^H A22 Q for Q = I - V T V^H (V: m x k, ld m, rows from s).
-// her2k: the one-stage trailing-update launcher (kind as tuned).
+// her2k: the one-stage trailing-update launcher (kind as tuned); kind 3
+// (FP32 storage): the block product and the update both on FP32 matrix cores.
 extern "C" hipError_t bse_launch_her2k(hipStream_t, int, int, int, int, void*, const double2*,
                                        const double2*, int64_t, int, int, const double2*);
 // ev (may be null): four events recorded before and after the Hermitian block
@@ -795,15 +1186,17 @@ extern "C" hipError_t bse_band_update(hipStream_t st, int fp32, int kind, int n,
     double2* Z = M + 2 * BR_B * BR_B;
     const dim3 gh((unsigned)nT, HEMM_KSPLIT);
     if (ev) (void)hipEventRecord(ev[0], st);
-    if (fp32)
+    if (fp32 && kind == 3)
+        hipLaunchKernelGGL(k_band_hemm32, gh, dim3(256), 0, st, n, s, k, (const float2*)A, V, (int64_t)m, Xp);
+    else if (fp32)
         hipLaunchKernelGGL(k_band_hemm<float2>, gh, dim3(256), 0, st, n, s, k, (const float2*)A, V, (int64_t)m, Xp);
     else
         hipLaunchKernelGGL(k_band_hemm<double2>, gh, dim3(256), 0, st, n, s, k, (const double2*)A, V, (int64_t)m, Xp);
     if (ev) (void)hipEventRecord(ev[1], st);
     hipLaunchKernelGGL(k_band_xt, dim3((unsigned)nblk), dim3(256), 0, st, n, s, k, b, (const double2*)Xp, Tp, V, Z, Gp);
-    hipLaunchKernelGGL(k_band_m, dim3(1), dim3(256), 0, st, nblk, k, b, (const double2*)Gp, Tp, M);
-    hipLaunchKernelGGL(k_band_z, dim3((unsigned)ceil_div64(m, 256)), dim3(256), 0, st, m, k, b, V,
-                       (const double2*)M, Z);
+    hipLaunchKernelGGL(k_band_m, dim3((unsigned)k), dim3(256), 0, st, nblk, k, b, (const double2*)Gp, Tp, M);
+    hipLaunchKernelGGL(k_rows_small, dim3((unsigned)ceil_div64(m, 64)), dim3(256), 0, st, m, 0, k, k, b, V,
+                       (const double2*)M, Z, 1, (const int*)nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], st);

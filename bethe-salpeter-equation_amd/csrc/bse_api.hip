@@ -74,9 +74,12 @@ extern "C" int bse_stage_times(const bse_ctx* ctx, double* ms, int n) {
     return BSE_OK;
 }
 
-// Kernel-level profile of the last tridiagonalisation: out[0] = total hemv
-// time (ms, HIP events around every launch), out[1] = algorithmic bytes read
-// (lower triangle of each trailing block), out[2] = number of launches.
+// Kernel-level profile of the last tridiagonalisation.  One-stage: out[0] =
+// total hemv time (ms, HIP events around every launch), out[1] = algorithmic
+// bytes read (lower triangle of each trailing block), out[2] = launches.
+// Two-stage (out[4] = 1): out[0], out[3], out[2] = rank-2k update time,
+// algorithmic flops, launches; out[5], out[6], out[7] the same for the
+// Hermitian block product; out[1] = 1 when both ran on FP32 matrix cores.
 extern "C" int bse_profile(bse_ctx* ctx, int enable, double* out) {
     if (!ctx) return BSE_ERR_ARG;
     ctx->prof = enable != 0;
@@ -84,6 +87,7 @@ extern "C" int bse_profile(bse_ctx* ctx, int enable, double* out) {
         for (int i = 0; i < 9; ++i) out[i] = 0.0;
         if (ctx->prof_two_stage) {
             out[0] = ctx->b2k_ms;
+            out[1] = ctx->band_fp32prod ? 1.0 : 0.0;
             out[2] = (double)ctx->band_launches;
             out[3] = ctx->b2k_flops;
             out[4] = 1.0;
@@ -649,15 +653,16 @@ static int autotune_her2k(bse_ctx* ctx, int fp32, void* scratch, size_t scratch_
 // ---------------------------------------------------------------------------
 static const int kBand = 32;
 
-// Automatic choice (two_stage == 0): the two-stage reduction for FP64
-// storage from order 256 up when at most 20 vectors are wanted (measured at
-// N = 20736: 5.0 s against 5.9 s for the one-stage lowest-16 path); FP32
-// storage stays one-stage, whose hemv streams half the bytes (3.1 s against
-// 4.9 s).  BSE_TWO_STAGE=1 / =0 forces it on / off for any size.
+// Automatic choice (two_stage == 0) when at most 20 vectors are wanted: the
+// two-stage reduction for FP64 storage from order 256 up and for FP32
+// storage from order 4096 up (measured at N = 20736: FP64 lowest 16 in
+// 3.0 s against 5.9 s one-stage; FP32 storage 1.8 s against 3.1-3.6 s, the
+// one-stage hemv being bound by streaming the trailing block once per
+// column).  BSE_TWO_STAGE=1 / =0 forces it on / off for any size.
 static bool use_two_stage(const bse_ctx* ctx, int n, int n_vectors, int max_vectors, int fp32) {
     const int want = (n_vectors < 0) ? n : std::max(n_vectors, max_vectors);
     if (want > 20 || ctx->two_stage < 0 || n < 2) return false;
-    return ctx->two_stage > 0 || (n >= 256 && !fp32);
+    return ctx->two_stage > 0 || n >= (fp32 ? 4096 : 256);
 }
 
 static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double2* rho) {
@@ -688,12 +693,20 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
         double2* tw = BUF(double2, S_BTSQR, bse_tsqr_workspace(m0, b));
         const int64_t uel = bse_band_update_workspace(n, b, b);
         double2* uw = BUF(double2, S_BUPD, uel);
-        int rc = autotune_her2k(ctx, fp32, uw, sizeof(double2) * (size_t)uel, Vst, P,
-                                std::min<int64_t>(voff, (int64_t)m0 * b), zeros);
-        if (rc) return rc;
+        // FP32 storage: the block product and the rank-2k update on FP32
+        // matrix cores (kind 3), whose rounding matches the storage's; FP64
+        // storage (or BSE_BAND_FP64PROD=1): FP64 products, kind as tuned
+        const char* fp64prod = getenv("BSE_BAND_FP64PROD");
+        const bool mfma32 = fp32 && !(fp64prod && fp64prod[0] == '1');
+        if (!mfma32) {
+            int rc = autotune_her2k(ctx, fp32, uw, sizeof(double2) * (size_t)uel, Vst, P,
+                                    std::min<int64_t>(voff, (int64_t)m0 * b), zeros);
+            if (rc) return rc;
+        }
         HCK(bse_launch_zero_words(st, 128, zeros));
         HCK(hipEventRecord(tv[0], st));
-        const int kind = ctx->her2k_valu[fp32] >= 0 ? ctx->her2k_valu[fp32] : 1;
+        const int kind = mfma32 ? 3 : (ctx->her2k_valu[fp32] >= 0 ? ctx->her2k_valu[fp32] : 1);
+        ctx->band_fp32prod = mfma32;
         hipEvent_t* pev = nullptr;
         if (ctx->prof) {
             while ((int64_t)ctx->prof_ev.size() < 4 * (int64_t)npan) {

@@ -130,6 +130,27 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// The same sum through DPP lane moves (no LDS round trips): quad
+// permutations, half-row and row mirrors give every lane its row's sum of 16;
+// row_bcast15 and row_bcast31 carry the row sums up to lane 63, which is read
+// back into every lane.  A fixed order, but not the butterfly's.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_mov_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double wave_sum_dpp(double v) {
+    v += dpp_mov_d<0xB1, 0xF>(v);     // quad_perm [1,0,3,2]
+    v += dpp_mov_d<0x4E, 0xF>(v);     // quad_perm [2,3,0,1]
+    v += dpp_mov_d<0x141, 0xF>(v);    // row_half_mirror
+    v += dpp_mov_d<0x140, 0xF>(v);    // row_mirror
+    v += dpp_mov_d<0x142, 0xA>(v);    // row_bcast15 into rows 1 and 3
+    v += dpp_mov_d<0x143, 0xC>(v);    // row_bcast31 into rows 2 and 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
 // Block reduction of a double over blockDim.x threads (multiple of 64, <= 1024).
 // Every thread receives the total.  Deterministic order.
 __device__ inline double block_sum(double v, double* sh /* >= 16 doubles */) {

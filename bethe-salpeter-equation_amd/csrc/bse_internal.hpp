@@ -195,6 +195,7 @@ struct bse_ctx {
     double b2k_ms = 0.0, b2k_flops = 0.0, hemm_ms = 0.0, hemm_flops = 0.0;
     int64_t band_launches = 0;
     bool prof_two_stage = false;
+    bool band_fp32prod = false;     // last band reduction ran on FP32 matrix cores
     bool force_valu_assembly = false;   // BSE_ASSEMBLY=valu selects the VALU tiles
     int asm_kernel = 0;                 // 0 VALU, 1 FP64 MFMA, 2 FP32 MFMA, 3 BF16x2 MFMA
     int64_t hemv_launches = 0;

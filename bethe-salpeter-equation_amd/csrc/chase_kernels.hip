@@ -37,13 +37,18 @@
 #include <cfloat>
 
 #define CH_B 32                  // largest band half-width
-#define CH_W 8                   // sweeps (waves) per workgroup
+#define CH_W 4                   // sweeps (waves) per workgroup
 #define CHASE_SPIN_LIMIT (1u << 26)
 
-__device__ inline double2 wave_sum_c(double2 v) { return cmk(wave_sum(v.x), wave_sum(v.y)); }
+__device__ inline double2 wave_sum_c(double2 v) { return cmk(wave_sum_dpp(v.x), wave_sum_dpp(v.y)); }
 
+// Hand-over of LDS data between the lanes of one wave: an LDS-only fence
+// (s_waitcnt lgkmcnt(0)), so global loads and stores already in flight stay
+// in flight (a full workgroup fence would also wait for every outstanding
+// global access).  The wave's global stores are ordered by chase_post's
+// release fence before any other wave reads them.
 __device__ inline void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
     __builtin_amdgcn_wave_barrier();
 }
 
@@ -84,13 +89,23 @@ __device__ inline int chase_ntask(int n, int b, int s) { return 2 * ((n - 2 - s)
 // write-back and invalidation).  Agent scope is cumulative: what earlier
 // waves of a workgroup wrote (ordered before its last wave's release) is
 // written back by that release too.
+// COH = 0: the band and y go through the caches; a hand-over inside a chunk
+// is ordered at workgroup scope, one across chunks (another workgroup,
+// possibly another XCD) at agent scope, whose release writes back the XCD's
+// L2 and whose acquire invalidates it.
+// COH = 1: every band and y access is device-coherent (sc1 loads and stores,
+// which do not leave stale or dirty lines in a non-coherent L2), so a
+// hand-over needs only the completion of the wave's stores (s_waitcnt
+// vmcnt(0)) before the progress counter is written, and no cache
+// maintenance at all.
+template <int COH>
 __device__ inline bool chase_wait(const ChaseArgs& a, int s, int need, int lane, bool local) {
     int ok = 1;
     if (lane == 0) {
         unsigned spins = 0;
         for (;;) {
-            const int got = local ? __hip_atomic_load(a.prog + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                                  : __hip_atomic_load(a.prog + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int got = (local && !COH) ? __hip_atomic_load(a.prog + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                            : __hip_atomic_load(a.prog + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (got >= need) break;
             if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
             if (++spins > CHASE_SPIN_LIMIT) {
@@ -102,13 +117,18 @@ __device__ inline bool chase_wait(const ChaseArgs& a, int s, int need, int lane,
         }
     }
     ok = __shfl(ok, 0, 64);
-    if (local) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (COH || local) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return ok != 0;
 }
 
+template <int COH>
 __device__ inline void chase_post(const ChaseArgs& a, int s, int val, int lane, bool local) {
-    if (local) {
+    if (COH) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0x0F70);      // vmcnt(0): the wave's sc1 stores are done
+        if (lane == 0) __hip_atomic_store(a.prog + s, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (local) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         if (lane == 0) __hip_atomic_store(a.prog + s, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     } else {
@@ -117,17 +137,50 @@ __device__ inline void chase_post(const ChaseArgs& a, int s, int val, int lane, 
     }
 }
 
+// Band and y accesses through buffer descriptors, COH = 1 with the
+// device-scope cache policy.  A masked-off element gets an offset past the
+// descriptor's range, which the hardware turns into a zero load or a
+// dropped store, so no access needs a branch.
+#define CH_SC1 16
+#define CH_OOB ((int)0xFFFFFFF0u)
+template <int COH> struct ChaseMem {
+    __amdgpu_buffer_rsrc_t rab, ry;
+    __device__ ChaseMem(const ChaseArgs& a) {
+        rab = bse_rsrc(a.AB, (int)((int64_t)a.n * a.ldab * 16));
+        ry = bse_rsrc(a.y ? (void*)a.y : (void*)a.AB, a.y ? a.n * 16 : 0);
+    }
+    static constexpr int AUX = COH ? CH_SC1 : 0;
+    // element index i < 2^27 (the launcher checks the band store's size)
+    __device__ inline double2 ld(int i, bool ok = true) const {
+        return bufld<double2>::ld<AUX>(rab, ok ? i * 16 : CH_OOB, 0);
+    }
+    __device__ inline void st(int i, double2 v, bool ok = true) const {
+        const bse_d2v w = {v.x, v.y};
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(bse_i4v, w), rab, ok ? i * 16 : CH_OOB, 0, AUX);
+    }
+    __device__ inline double2 ldy(int i, bool ok = true) const {
+        return bufld<double2>::ld<AUX>(ry, ok ? i * 16 : CH_OOB, 0);
+    }
+    __device__ inline void sty(int i, double2 v, bool ok = true) const {
+        const bse_d2v w = {v.x, v.y};
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(bse_i4v, w), ry, ok ? i * 16 : CH_OOB, 0, AUX);
+    }
+};
+
+template <int COH>
 __global__ __launch_bounds__(64 * CH_W) void k_chase(ChaseArgs a)
 {
+    const ChaseMem<COH> mem(a);
     __shared__ double2 sBlk[CH_W][CH_B][CH_B + 1];
     __shared__ double2 sV[CH_W][CH_B];
     __shared__ double2 sU[CH_W][CH_B];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & (CH_B - 1);          // in-range LDS row / column index
     const int n = a.n, b = a.b, ldab = a.ldab;
     double2 (*X)[CH_B + 1] = sBlk[w];
     double2* sv = sV[w];
     double2* su = sU[w];
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.d[0] = a.AB[0].x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.d[0] = mem.ld(0).x;
     for (int chunk = blockIdx.x; chunk < a.nchunks; chunk += gridDim.x) {
         const int s = chunk * CH_W + w;
         if (s > n - 2) break;
@@ -136,126 +189,162 @@ __global__ __launch_bounds__(64 * CH_W) void k_chase(ChaseArgs a)
         double2 tau = cmk(0, 0);
         bool alive = true;
         for (int j = 0; j < ntask && alive; ++j) {
-            if (s > 0 && !chase_wait(a, s - 1, min(j + 3, nprev), lane, w > 0)) { alive = false; break; }
+            if (s > 0 && !chase_wait<COH>(a, s - 1, min(j + 3, nprev), lane, w > 0)) { alive = false; break; }
             if ((j & 1) == 0) {
                 // ---- D_k: (k = 0: reflector from column s) two-sided update
                 const int k = j >> 1;
                 const int lo = s + 1 + k * b, hi = min(lo + b - 1, n - 1), L = hi - lo + 1;
+                const bool row = lane < L;
+                // the lower triangle of A(D_k, D_k), row lane in registers;
+                // every load (and for k = 0 the column s and the y segment)
+                // is issued before the first is waited for
+                // element (lo + lane, lo + c) of the band store: dbase + c (ldab - 1) + lane
+                const int dbase = lo * ldab;
+                double2 tmp[CH_B];
+#pragma unroll
+                for (int c = 0; c < CH_B; ++c)
+                    tmp[c] = mem.ld(dbase + c * (ldab - 1) + lane, c < L && lane >= c && row);
+                double2 yy = cmk(0, 0);
                 if (k == 0) {
-                    const double2 x = (lane < L) ? a.AB[(int64_t)s * ldab + 1 + lane] : cmk(0, 0);
-                    const double ss = wave_sum((lane >= 1 && lane < L) ? cabs2(x) : 0.0);
+                    const double2 x = mem.ld(s * ldab + 1 + lane, row);
+                    if (a.y) yy = mem.ldy(lo + lane, row);
+                    const double ss = wave_sum_dpp((lane >= 1 && row) ? cabs2(x) : 0.0);
                     const double2 alpha = cmk(__shfl(x.x, 0, 64), __shfl(x.y, 0, 64));
                     const Refl r = make_refl(alpha, ss);
                     tau = r.tau;
-                    const double2 v = (lane == 0) ? cmk(1.0, 0.0) : cmul(x, r.scal);
-                    if (lane < L) {
-                        sv[lane] = v;
-                        a.AB[(int64_t)s * ldab + 1 + lane] = (lane == 0) ? cmk(r.beta, 0.0) : cmk(0, 0);
-                    }
+                    const double2 v0 = (lane == 0) ? cmk(1.0, 0.0) : cmul(x, r.scal);
+                    if (row) sv[lane] = v0;
+                    mem.st(s * ldab + 1 + lane, (lane == 0) ? cmk(r.beta, 0.0) : cmk(0, 0), row);
                     if (lane == 0) a.e[s] = r.beta;
                     if (a.y) {
-                        const double2 yy = (lane < L) ? a.y[lo + lane] : cmk(0, 0);
-                        const double2 dot = wave_sum_c((lane < L) ? cmul(cconj(v), yy) : cmk(0, 0));
-                        if (lane < L) a.y[lo + lane] = csub(yy, cmul(cmul(cconj(tau), v), dot));
+                        const double2 dot = wave_sum_c(row ? cmul(cconj(v0), yy) : cmk(0, 0));
+                        mem.sty(lo + lane, csub(yy, cmul(cmul(cconj(tau), v0), dot)), row);
                     }
-                    wave_lds_sync();
                 }
-                // stage the lower triangle of A(D_k, D_k): all loads are
-                // issued before the first one is waited for
-                {
-                    double2 tmp[CH_B];
+                if (lane < CH_B) {
 #pragma unroll
-                    for (int c = 0; c < CH_B; ++c)
-                        tmp[c] = (c < L && lane >= c && lane < L) ? a.AB[(int64_t)(lo + c) * ldab + (lane - c)]
-                                                                  : cmk(0, 0);
-#pragma unroll
-                    for (int c = 0; c < CH_B; ++c)
-                        if (c < L && lane >= c && lane < L) X[lane][c] = tmp[c];
+                    for (int c = 0; c < CH_B; ++c) X[lane][c] = tmp[c];
                 }
                 wave_lds_sync();
+                // w = A v over the Hermitian block: A(i, c) = tmp[c] for c <= i,
+                // conj(X[c][i]) above; LDS reads in batches of 8
                 double2 wv = cmk(0, 0);
-                if (lane < L) {
 #pragma unroll
-                    for (int c = 0; c < CH_B; ++c) {
-                        if (c < L) {
-                            const double2 h = (lane >= c) ? X[lane][c] : cconj(X[c][lane]);
-                            cfma(wv, h, sv[c]);
-                        }
+                for (int c0 = 0; c0 < CH_B; c0 += 8) {
+                    double2 xc[8], vc[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) { xc[q] = X[c0 + q][li]; vc[q] = sv[c0 + q]; }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int c = c0 + q;
+                        const double2 h = (lane >= c) ? tmp[c] : cconj(xc[q]);
+                        const double2 vq = (c < L) ? vc[q] : cmk(0, 0);
+                        cfma(wv, h, vq);
                     }
                 }
-                const double2 v = (lane < L) ? sv[lane] : cmk(0, 0);
-                const double gam = wave_sum((lane < L) ? (v.x * wv.x + v.y * wv.y) : 0.0);
+                if (!row) wv = cmk(0, 0);
+                const double2 v = row ? sv[li] : cmk(0, 0);
+                const double gam = wave_sum_dpp(v.x * wv.x + v.y * wv.y);
                 const double t2 = cabs2(tau);
                 const double2 u = csub(cmul(tau, wv), cscale(v, 0.5 * t2 * gam));
-                if (lane < L) su[lane] = u;
+                if (row) su[lane] = u;
                 wave_lds_sync();
-                for (int c = 0; c < L; ++c) {
-                    if (lane >= c && lane < L) {
-                        double2 h = X[lane][c];
-                        h = csub(h, cadd(cmul(u, cconj(sv[c])), cmul(v, cconj(su[c]))));
+                // A <- A - u v^H - v u^H on the lower triangle
+#pragma unroll
+                for (int c0 = 0; c0 < CH_B; c0 += 8) {
+                    double2 vc[8], uc[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) { vc[q] = sv[c0 + q]; uc[q] = su[c0 + q]; }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int c = c0 + q;
+                        double2 h = csub(tmp[c], cadd(cmul(u, cconj(vc[q])), cmul(v, cconj(uc[q]))));
                         if (lane == c) h.y = 0.0;
-                        a.AB[(int64_t)(lo + c) * ldab + (lane - c)] = h;
-                        if (k == 0 && lane == 0) a.d[lo] = h.x;
+                        if (c == 0 && k == 0 && lane == 0) a.d[lo] = h.x;
+                        mem.st(dbase + c * (ldab - 1) + lane, h, c < L && lane >= c && row);
                     }
                 }
                 wave_lds_sync();
             } else {
-                // ---- X_k: right product with H_{k-1}, new reflector, left product
+                // ---- X_k: right product with H_{k-1}, new reflector, left
+                // product; row lane of A(D_k, D_{k-1}) lives in registers
                 const int k = (j + 1) >> 1;
                 const int lo = s + 1 + k * b, hi = min(lo + b - 1, n - 1), Lr = hi - lo + 1;
                 const int plo = lo - b;
-                {
-                    double2 tmp[CH_B];
+                const bool row = lane < Lr;
+                // element (lo + lane, plo + c): xbase + c (ldab - 1) + lane
+                const int xbase = plo * ldab + b;
+                double2 xr[CH_B];
 #pragma unroll
-                    for (int c = 0; c < CH_B; ++c)
-                        tmp[c] = (c < b && lane < Lr) ? a.AB[(int64_t)(plo + c) * ldab + (b + lane - c)]
-                                                      : cmk(0, 0);
-#pragma unroll
-                    for (int c = 0; c < CH_B; ++c)
-                        if (c < b && lane < Lr) X[lane][c] = tmp[c];
-                }
-                wave_lds_sync();
+                for (int c = 0; c < CH_B; ++c) xr[c] = mem.ld(xbase + c * (ldab - 1) + lane, c < b && row);
+                const double2 yy = a.y ? mem.ldy(lo + lane, row) : cmk(0, 0);
                 // (1) X <- X H_{k-1}:  X -= tau (X v) v^H
-                if (lane < Lr) {
-                    double2 t = cmk(0, 0);
-                    for (int c = 0; c < b; ++c) cfma(t, X[lane][c], sv[c]);
-                    const double2 tt = cmul(tau, t);
-                    for (int c = 0; c < b; ++c) X[lane][c] = csub(X[lane][c], cmul(tt, cconj(sv[c])));
+                double2 t = cmk(0, 0);
+#pragma unroll
+                for (int c0 = 0; c0 < CH_B; c0 += 8) {
+                    double2 vc[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) vc[q] = sv[c0 + q];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) cfma(t, xr[c0 + q], (c0 + q < b) ? vc[q] : cmk(0, 0));
                 }
-                wave_lds_sync();
+                const double2 tt = cmul(tau, t);
+#pragma unroll
+                for (int c0 = 0; c0 < CH_B; c0 += 8) {
+                    double2 vc[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) vc[q] = sv[c0 + q];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (c0 + q < b) xr[c0 + q] = csub(xr[c0 + q], cmul(tt, cconj(vc[q])));
+                }
+                wave_lds_sync();     // every lane is done with H_{k-1}'s vector
                 // (2) reflector from column 0
-                const double2 x = (lane < Lr) ? X[lane][0] : cmk(0, 0);
-                const double ss = wave_sum((lane >= 1 && lane < Lr) ? cabs2(x) : 0.0);
+                const double2 x = row ? xr[0] : cmk(0, 0);
+                const double ss = wave_sum_dpp((lane >= 1 && row) ? cabs2(x) : 0.0);
                 const double2 alpha = cmk(__shfl(x.x, 0, 64), __shfl(x.y, 0, 64));
                 const Refl r = make_refl(alpha, ss);
-                const double2 v = (lane == 0) ? cmk(1.0, 0.0) : cmul(x, r.scal);
-                if (lane < Lr) sv[lane] = v;
-                wave_lds_sync();
-                // (3) X(:, 1:) <- H^H X(:, 1:):  s_c = v^H X(:, c)
-                if (lane >= 1 && lane < b) {
-                    double2 sc = cmk(0, 0);
-                    for (int i = 0; i < Lr; ++i) cfmac(sc, sv[i], X[i][lane]);
-                    su[lane] = sc;
+                const double2 v = (lane == 0) ? cmk(1.0, 0.0) : (row ? cmul(x, r.scal) : cmk(0, 0));
+                if (lane < CH_B) {
+                    sv[lane] = v;
+#pragma unroll
+                    for (int c = 0; c < CH_B; ++c) X[lane][c] = xr[c];
                 }
+                wave_lds_sync();
+                // (3) X(:, 1:) <- H^H X(:, 1:):  s_c = v^H X(:, c)  (lane = column c)
+                double2 sc = cmk(0, 0);
+#pragma unroll
+                for (int i0 = 0; i0 < CH_B; i0 += 8) {
+                    double2 xi[8], vi[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) { xi[q] = X[i0 + q][li]; vi[q] = sv[i0 + q]; }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) cfmac(sc, vi[q], xi[q]);
+                }
+                if (lane < CH_B) su[lane] = sc;
                 wave_lds_sync();
                 const double2 ctau = cconj(r.tau);
-                if (lane < Lr) {
-                    const double2 f = cmul(ctau, v);
-                    for (int c = 1; c < b; ++c) X[lane][c] = csub(X[lane][c], cmul(f, su[c]));
-                    X[lane][0] = (lane == 0) ? cmk(r.beta, 0.0) : cmk(0, 0);
+                const double2 f = cmul(ctau, v);
+#pragma unroll
+                for (int c0 = 0; c0 < CH_B; c0 += 8) {
+                    double2 uc[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) uc[q] = su[c0 + q];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int c = c0 + q;
+                        double2 h = (c == 0) ? ((lane == 0) ? cmk(r.beta, 0.0) : cmk(0, 0)) : csub(xr[c], cmul(f, uc[q]));
+                        mem.st(xbase + c * (ldab - 1) + lane, h, c < b && row);
+                    }
                 }
-                wave_lds_sync();
-                for (int c = 0; c < b; ++c)
-                    if (lane < Lr) a.AB[(int64_t)(plo + c) * ldab + (b + lane - c)] = X[lane][c];
                 if (a.y) {
-                    const double2 yy = (lane < Lr) ? a.y[lo + lane] : cmk(0, 0);
-                    const double2 dot = wave_sum_c((lane < Lr) ? cmul(cconj(v), yy) : cmk(0, 0));
-                    if (lane < Lr) a.y[lo + lane] = csub(yy, cmul(cmul(ctau, v), dot));
+                    const double2 dot = wave_sum_c(row ? cmul(cconj(v), yy) : cmk(0, 0));
+                    mem.sty(lo + lane, csub(yy, cmul(cmul(ctau, v), dot)), row);
                 }
                 tau = r.tau;
                 wave_lds_sync();
             }
-            chase_post(a, s, j + 1, lane, w < CH_W - 1);
+            chase_post<COH>(a, s, j + 1, lane, w < CH_W - 1);
         }
         if (!alive) break;
     }
@@ -264,19 +353,23 @@ __global__ __launch_bounds__(64 * CH_W) void k_chase(ChaseArgs a)
 extern "C" int64_t bse_chase_counters(int n) { return n > 1 ? n - 1 : 1; }
 
 // prog (n-1 ints) and err (1 int) must be zero.  grid: workgroups.
+// Device-coherent form (COH = 1) unless BSE_CHASE_COH=0.  The band store
+// must stay below 2 GiB (32-bit buffer offsets; n < 2^21 for b = 32).
 extern "C" hipError_t bse_launch_chase(hipStream_t st, int n, int b, double2* AB, int ldab, double2* y,
                                        double* d, double* e, int* prog, int* err, int grid)
 {
     if (b < 1 || b > CH_B || ldab < 2 * b + 1) return hipErrorInvalidValue;
-    if (n < 2) {
-        hipLaunchKernelGGL(k_chase, dim3(1), dim3(64 * CH_W), 0, st,
-                           ChaseArgs{n, b, ldab, 0, AB, y, d, e, prog, err});
-        return hipGetLastError();
-    }
-    const int nchunks = (int)ceil_div64(n - 1, CH_W);
-    const int g = std::max(1, std::min(grid, nchunks));
-    hipLaunchKernelGGL(k_chase, dim3((unsigned)g), dim3(64 * CH_W), 0, st,
-                       ChaseArgs{n, b, ldab, nchunks, AB, y, d, e, prog, err});
+    const char* ce = getenv("BSE_CHASE_COH");
+    // the band store is addressed with 32-bit byte offsets
+    if ((int64_t)n * ldab * 16 >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+    const bool coh = !(ce && ce[0] == '0');
+    const int nchunks = (n < 2) ? 0 : (int)ceil_div64(n - 1, CH_W);
+    const int g = (n < 2) ? 1 : std::max(1, std::min(grid, nchunks));
+    const ChaseArgs args{n, b, ldab, nchunks, AB, y, d, e, prog, err};
+    if (coh)
+        hipLaunchKernelGGL(k_chase<1>, dim3((unsigned)g), dim3(64 * CH_W), 0, st, args);
+    else
+        hipLaunchKernelGGL(k_chase<0>, dim3((unsigned)g), dim3(64 * CH_W), 0, st, args);
     return hipGetLastError();
 }
 
@@ -308,15 +401,15 @@ __device__ inline double2 band_elem(const double2* AB0, int ldab, int n, int b, 
 }
 
 #define BII_W (2 * CH_B + 1)
-__global__ __launch_bounds__(256) void k_band_ii(int n, int b, int ldab, const double2* __restrict__ AB0,
-                                                 const double* __restrict__ lam, double tnorm,
-                                                 double2* __restrict__ Ub, double2* __restrict__ Lm,
-                                                 int* __restrict__ piv, double2* __restrict__ X, int64_t ldx)
+#define BII_PF 8            // steps of prefetch (rows, multipliers, U rows)
+__global__ __launch_bounds__(64) void k_band_ii(int n, int b, int ldab, const double2* __restrict__ AB0,
+                                                const double* __restrict__ lam, double tnorm,
+                                                double2* __restrict__ Ub, double2* __restrict__ Lm,
+                                                int* __restrict__ piv, double2* __restrict__ X, int64_t ldx)
 {
     __shared__ double2 W[CH_B + 1][BII_W];
     __shared__ double2 lb[CH_B];
-    __shared__ int s_p;
-    const int m = blockIdx.x, tid = threadIdx.x;
+    const int m = blockIdx.x, lane = threadIdx.x;
     const double lm = lam[m];
     const int wb = 2 * b + 1;                    // U row length
     const int nr = b + 1;                        // window rows
@@ -326,108 +419,161 @@ __global__ __launch_bounds__(256) void k_band_ii(int n, int b, int ldab, const d
     const double tiny = DBL_EPSILON * tnorm + DBL_MIN;
     auto rs = [&](int r) { return r % nr; };
     auto cs = [&](int c) { return c % wb; };
+    // band row R, entry c = 0 .. 2b: B(R, R - b + c) - lam delta; lane c
+    // holds entry c, lane 0 also entry 2b (returned in *hi)
+    auto row_load = [&](int R, double2* hi) -> double2 {
+        const double2 v = (R < n) ? band_elem(AB0, ldab, n, b, R, R - b + lane, lm) : cmk(0, 0);
+        *hi = (R < n && lane == 0) ? band_elem(AB0, ldab, n, b, R, R + b, lm) : cmk(0, 0);
+        return v;
+    };
     // initial window: rows 0 .. b, columns 0 .. 2b
-    for (int e = tid; e < nr * wb; e += 256) {
+    for (int e = lane; e < nr * wb; e += 64) {
         const int r = e / wb, c = e % wb;
         W[rs(r)][cs(c)] = band_elem(AB0, ldab, n, b, r, c, lm);
     }
-    __syncthreads();
-    for (int j = 0; j < n; ++j) {
-        const int rmax = min(j + b, n - 1);
-        // prefetch of the row entering after this step: row j+b+1, columns j+1 .. j+2b+1
-        const int rin = j + b + 1;
-        double2 pre = cmk(0, 0);
-        if (tid < wb && rin < n) pre = band_elem(AB0, ldab, n, b, rin, j + 1 + tid, lm);
-        // 1. pivot among rows j .. rmax
-        if (tid < 64) {
+    // rows entering at steps j .. j + BII_PF - 1 (row j + b + 1 at step j)
+    double2 pre[BII_PF], preh[BII_PF];
+#pragma unroll
+    for (int u = 0; u < BII_PF; ++u) pre[u] = row_load(b + 1 + u, &preh[u]);
+    wave_lds_sync();
+    for (int j0 = 0; j0 < n; j0 += BII_PF) {
+#pragma unroll
+        for (int u = 0; u < BII_PF; ++u) {
+            const int j = j0 + u;
+            if (j >= n) break;
+            const int rmax = min(j + b, n - 1);
+            // 1. pivot among rows j .. rmax (ties to the lower row)
             double best = -1.0;
             int bi = j;
-            if (j + tid <= rmax) { best = cabs2(W[rs(j + tid)][cs(j)]); bi = j + tid; }
+            if (lane <= b && j + lane <= rmax) { best = cabs2(W[rs(j + lane)][cs(j)]); bi = j + lane; }
             for (int off = 32; off > 0; off >>= 1) {
                 const double ob = __shfl_xor(best, off, 64);
                 const int oi = __shfl_xor(bi, off, 64);
                 if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
             }
-            if (tid == 0) s_p = bi;
+            const int p = bi;
+            // 2. pivot and multipliers (column j is not touched by the swap)
+            double2 uu = W[rs(p)][cs(j)];
+            if (cabs2(uu) < tiny * tiny) uu = cmk(uu.x >= 0.0 ? tiny : -tiny, 0.0);
+            if (lane < b) {
+                const int r = j + 1 + lane;
+                double2 l = cmk(0, 0);
+                if (r <= rmax) l = cdiv((r == p) ? W[rs(j)][cs(j)] : W[rs(r)][cs(j)], uu);
+                lb[lane] = l;
+                Lmul[(int64_t)j * b + lane] = l;
+            }
+            // 3. swap rows j and p over columns j+1 .. j+2b (lane c: column
+            // j+1+c), 4. U row j, from the swapped pivot row
+            const int col = j + 1 + lane;
+            const bool act = lane < 2 * b;           // lanes beyond the window idle
+            double2 prv = act ? W[rs(j)][cs(col)] : cmk(0, 0);
+            if (p != j && act) {
+                const double2 t = W[rs(p)][cs(col)];
+                W[rs(p)][cs(col)] = prv;
+                prv = t;
+            }
+            if (act) U[(int64_t)j * wb + 1 + lane] = (col < n) ? prv : cmk(0, 0);
+            if (lane == 0) {
+                U[(int64_t)j * wb] = uu;
+                pv[j] = p;
+            }
+            wave_lds_sync();
+            // 5. elimination of rows j+1 .. rmax in column j+1+lane
+            if (act) {
+                for (int t = 0; t < b; ++t) {
+                    const int r = j + 1 + t;
+                    if (r <= rmax) W[rs(r)][cs(col)] = csub(W[rs(r)][cs(col)], cmul(lb[t], prv));
+                }
+            }
+            // 6. the next window: row j+b+1 takes row j's slot (columns j+1 ..
+            // j+2b+1); column j+2b+1 takes column j's slot, zero in rows j+1 .. j+b
+            if (act) W[rs(j + b + 1)][cs(col)] = pre[u];
+            if (lane == 0) W[rs(j + b + 1)][cs(j + 2 * b + 1)] = preh[u];
+            if (lane < b) W[rs(j + 1 + lane)][cs(j + 2 * b + 1)] = cmk(0, 0);
+            pre[u] = row_load(j + b + 1 + BII_PF, &preh[u]);
+            wave_lds_sync();
         }
-        __syncthreads();
-        const int p = s_p;
-        // 2. swap rows j and p over columns j .. j+2b
-        if (p != j && tid < wb) {
-            const int c = cs(j + tid);
-            const double2 t = W[rs(j)][c];
-            W[rs(j)][c] = W[rs(p)][c];
-            W[rs(p)][c] = t;
-        }
-        __syncthreads();
-        // 3. pivot and multipliers
-        double2 u = W[rs(j)][cs(j)];
-        if (cabs2(u) < tiny * tiny) u = cmk(u.x >= 0.0 ? tiny : -tiny, 0.0);
-        if (tid < b) {
-            const int r = j + 1 + tid;
-            const double2 l = (r <= rmax) ? cdiv(W[rs(r)][cs(j)], u) : cmk(0, 0);
-            lb[tid] = l;
-            Lmul[(int64_t)j * b + tid] = l;
-        }
-        __syncthreads();
-        // 4. elimination of rows j+1 .. rmax over columns j+1 .. j+2b
-        for (int e = tid; e < b * 2 * b; e += 256) {
-            const int rr = e / (2 * b), cc = e % (2 * b);
-            const int r = j + 1 + rr, c = j + 1 + cc;
-            if (r <= rmax) W[rs(r)][cs(c)] = csub(W[rs(r)][cs(c)], cmul(lb[rr], W[rs(j)][cs(c)]));
-        }
-        // 5. U row j
-        if (tid < wb) {
-            const int c = j + tid;
-            U[(int64_t)j * wb + tid] = (tid == 0) ? u : (c < n ? W[rs(j)][cs(c)] : cmk(0, 0));
-        }
-        if (tid == 0) pv[j] = p;
-        __syncthreads();
-        // 6. the next window: row j+b+1 replaces row j; column j+2b+1 replaces column j
-        if (tid < wb) W[rs(rin)][cs(j + 1 + tid)] = pre;
-        if (tid < b) W[rs(j + 1 + tid)][cs(j + 2 * b + 1)] = (j + 1 + tid < rin) ? cmk(0, 0) : W[rs(j + 1 + tid)][cs(j + 2 * b + 1)];
-        __syncthreads();
     }
-    // ---- solves (one wave): x <- U^-1 L^-1 P x, three times from a random start
-    if (tid >= 64) return;
-    const int lane = tid;
+    // ---- solves: x <- U^-1 L^-1 P x, three times from a random start
     double2* x = X + (int64_t)m * ldx;
     for (int i = lane; i < n; i += 64) x[i] = cmk(bii_hash(m, 2 * i), bii_hash(m, 2 * i + 1));
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    const int bw = 2 * b;
     for (int it = 0; it < 3; ++it) {
-        // forward: window x[j .. j+b] in lanes 0 .. b
+        // forward: window x[j .. j+b] in lanes 0 .. b; the pivots, the
+        // multipliers and the entering x[j+b+1] of the next BII_PF steps are
+        // loaded one block ahead
         double2 xw = (lane <= b && lane < n) ? x[lane] : cmk(0, 0);
-        for (int j = 0; j < n; ++j) {
-            const int p = pv[j];
-            const int lp = p - j;                 // 0 .. b
-            // swap x[j] and x[p]
-            const double2 xj = cmk(__shfl(xw.x, 0, 64), __shfl(xw.y, 0, 64));
-            const double2 xp = cmk(__shfl(xw.x, lp, 64), __shfl(xw.y, lp, 64));
-            if (lane == lp) xw = xj;
-            if (lane == 0) xw = xp;
-            // x[j+t] -= l_t x[j]  (t = 1 .. b)
-            if (lane >= 1 && lane <= b && j + lane < n) xw = csub(xw, cmul(Lmul[(int64_t)j * b + lane - 1], xp));
-            if (lane == 0) x[j] = xw;
-            // shift the window down by one lane and load x[j+b+1]
-            const double2 nx = cmk(__shfl_down(xw.x, 1, 64), __shfl_down(xw.y, 1, 64));
-            xw = nx;
-            if (lane == b) xw = (j + b + 1 < n) ? x[j + b + 1] : cmk(0, 0);
+        int fp[BII_PF];
+        double2 fl[BII_PF], fx[BII_PF];
+        auto ffetch = [&](int jb) {
+#pragma unroll
+            for (int u = 0; u < BII_PF; ++u) {
+                const int j = jb + u;
+                fp[u] = (j < n) ? pv[j] : 0;
+                fl[u] = (j < n && lane >= 1 && lane <= b && j + lane < n) ? Lmul[(int64_t)j * b + lane - 1] : cmk(0, 0);
+                fx[u] = (lane == b && j + b + 1 < n) ? x[j + b + 1] : cmk(0, 0);
+            }
+        };
+        ffetch(0);
+        for (int j0 = 0; j0 < n; j0 += BII_PF) {
+            int cp[BII_PF];
+            double2 cl[BII_PF], cx[BII_PF];
+#pragma unroll
+            for (int u = 0; u < BII_PF; ++u) { cp[u] = fp[u]; cl[u] = fl[u]; cx[u] = fx[u]; }
+            ffetch(j0 + BII_PF);
+#pragma unroll
+            for (int u = 0; u < BII_PF; ++u) {
+                const int j = j0 + u;
+                if (j >= n) break;
+                const int lp = cp[u] - j;             // 0 .. b
+                const double2 xj = cmk(__shfl(xw.x, 0, 64), __shfl(xw.y, 0, 64));
+                const double2 xp = cmk(__shfl(xw.x, lp, 64), __shfl(xw.y, lp, 64));
+                if (lane == lp) xw = xj;
+                if (lane == 0) xw = xp;
+                // x[j+t] -= l_t x[j]  (t = 1 .. b)
+                if (lane >= 1 && lane <= b && j + lane < n) xw = csub(xw, cmul(cl[u], xp));
+                if (lane == 0) x[j] = xw;
+                // shift the window down by one lane; x[j+b+1] enters at lane b
+                const double2 nx = cmk(__shfl_down(xw.x, 1, 64), __shfl_down(xw.y, 1, 64));
+                xw = nx;
+                if (lane == b) xw = cx[u];
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        // backward: lane t holds x[j+1+t] (t < 2b)
-        const int bw = 2 * b;
-        double2 xr = cmk(0, 0);                  // x[j + 1 + lane] for the current j
+        // backward: lane t holds x[j+1+t] (t < 2b); U rows and x[j] loaded a
+        // block ahead (x[j] is final from the forward pass)
+        double2 xr = cmk(0, 0);
         double nrm = 0.0;
-        for (int j = n - 1; j >= 0; --j) {
-            const double2 uu = (lane < bw && j + 1 + lane < n) ? U[(int64_t)j * wb + 1 + lane] : cmk(0, 0);
-            const double2 acc = wave_sum_c(cmul(uu, xr));
-            const double2 xj = cdiv(csub(x[j], acc), U[(int64_t)j * wb]);
-            nrm += cabs2(xj);
-            if (lane == 0) x[j] = xj;
-            // shift up: new lane t holds old lane t-1; lane 0 gets x[j]
-            const double2 up = cmk(__shfl_up(xr.x, 1, 64), __shfl_up(xr.y, 1, 64));
-            xr = (lane == 0) ? xj : up;
-            if (lane >= bw) xr = cmk(0, 0);
+        double2 bu[BII_PF], bd[BII_PF], bx[BII_PF];
+        auto bfetch = [&](int jb) {
+#pragma unroll
+            for (int u = 0; u < BII_PF; ++u) {
+                const int j = jb - u;
+                bu[u] = (j >= 0 && lane < bw && j + 1 + lane < n) ? U[(int64_t)j * wb + 1 + lane] : cmk(0, 0);
+                bd[u] = (j >= 0) ? U[(int64_t)j * wb] : cmk(1, 0);
+                bx[u] = (j >= 0) ? x[j] : cmk(0, 0);
+            }
+        };
+        bfetch(n - 1);
+        for (int j0 = n - 1; j0 >= 0; j0 -= BII_PF) {
+            double2 cu[BII_PF], cd[BII_PF], cx[BII_PF];
+#pragma unroll
+            for (int u = 0; u < BII_PF; ++u) { cu[u] = bu[u]; cd[u] = bd[u]; cx[u] = bx[u]; }
+            bfetch(j0 - BII_PF);
+#pragma unroll
+            for (int u = 0; u < BII_PF; ++u) {
+                const int j = j0 - u;
+                if (j < 0) break;
+                const double2 acc = wave_sum_c(cmul(cu[u], xr));
+                const double2 xj = cdiv(csub(cx[u], acc), cd[u]);
+                nrm += cabs2(xj);
+                if (lane == 0) x[j] = xj;
+                // shift up: new lane t holds old lane t-1; lane 0 gets x[j]
+                const double2 up = cmk(__shfl_up(xr.x, 1, 64), __shfl_up(xr.y, 1, 64));
+                xr = (lane == 0) ? xj : up;
+                if (lane >= bw) xr = cmk(0, 0);
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         const double inv = 1.0 / sqrt(nrm);
@@ -476,7 +622,7 @@ extern "C" hipError_t bse_launch_band_ii(hipStream_t st, int n, int b, int ldab,
     double2* Ub = ws;
     double2* Lm = Ub + (int64_t)nvec * n * (2 * b + 1);
     int* piv = reinterpret_cast<int*>(Lm + (int64_t)nvec * n * b);
-    hipLaunchKernelGGL(k_band_ii, dim3((unsigned)nvec), dim3(256), 0, st, n, b, ldab, AB0, lam, tnorm, Ub, Lm,
+    hipLaunchKernelGGL(k_band_ii, dim3((unsigned)nvec), dim3(64), 0, st, n, b, ldab, AB0, lam, tnorm, Ub, Lm,
                        piv, X, ldx);
     hipLaunchKernelGGL(k_mgs_columns, dim3(1), dim3(1024), 0, st, n, nvec, X, ldx);
     return hipGetLastError();

@@ -902,14 +902,27 @@ struct her2k_f64 {
     __device__ static inline int row(int lane, int r) { return mfma16_row_f64(lane, r); }
 };
 
-template <typename T>
+// FP32 operands and accumulation (v_mfma_f32_16x16x4_f32): the band
+// reduction's update of an FP32-stored matrix, whose products then carry the
+// same relative rounding as the storage itself.
+typedef float her2k_f4 __attribute__((ext_vector_type(4)));
+struct her2k_f32 {
+    typedef float real;
+    typedef her2k_f4 acc;
+    __device__ static inline acc mfma(real a, real b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int lane, int r) { return mfma16_row_f32(lane, r); }
+};
+
+template <typename T, typename MM = her2k_f64>
 __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int s, T* __restrict__ A,
                                        const double2* __restrict__ V,
                                        const double2* __restrict__ W, int64_t ldv, int j0, int nb,
                                        const double2* __restrict__ alpha2, int64_t idx)
 {
-    typedef her2k_f64::real R;
-    typedef her2k_f64::acc ACC;
+    typedef typename MM::real R;
+    typedef typename MM::acc ACC;
     int ti, tj;
     tri_tile(idx, ti, tj);
     const int tcs = s >> 6;
@@ -996,13 +1009,13 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    accR[a][b] = her2k_f64::mfma(aQ[b], bP[a], accR[a][b]);
-                    accI[a][b] = her2k_f64::mfma(aQ[b], bPi[a], accI[a][b]);
+                    accR[a][b] = MM::mfma(aQ[b], bP[a], accR[a][b]);
+                    accI[a][b] = MM::mfma(aQ[b], bPi[a], accI[a][b]);
                 }
         }
         __syncthreads();
     }
-    // lane holds D'[i][j = lane % 16] with i = her2k_f64::row(lane, reg):
+    // lane holds D'[i][j = lane % 16] with i = MM::row(lane, reg):
     // column i, row j of the subtile
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -1011,7 +1024,7 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int
 #pragma unroll
             for (int reg = 0; reg < 4; ++reg) {
                 const int lr = rowbase + a * 16 + (lane & 15);
-                const int lc = colbase + b * 16 + her2k_f64::row(lane, reg);
+                const int lc = colbase + b * 16 + MM::row(lane, reg);
                 const int gr = r0 + lr, gc = c0 + lc;
                 if (gc >= s && gr < n && gr >= gc) {
                     T* p = tile + lr + lc * 64;
@@ -1172,6 +1185,14 @@ __global__ __launch_bounds__(256) void k_her2k_mfma(int n, int s, T* __restrict_
                                                    int j0, int nb, const double2* __restrict__ alpha2)
 {
     her2k_mfma_body<T>(n, s, A, V, W, ldv, j0, nb, alpha2, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_her2k_mfma32(int n, int s, float2* __restrict__ A,
+                                                     const double2* __restrict__ V,
+                                                     const double2* __restrict__ W, int64_t ldv,
+                                                     int j0, int nb, const double2* __restrict__ alpha2)
+{
+    her2k_mfma_body<float2, her2k_f32>(n, s, A, V, W, ldv, j0, nb, alpha2, blockIdx.x);
 }
 
 __global__ void k_last_diag(int n, const void* A, int fp32, double* d)
@@ -1466,14 +1487,19 @@ extern "C" hipError_t bse_hetrd_batch_panel(hipStream_t st, int fp32, int c, int
 
 // One trailing-update launch on A[s:, s:] of a tiled lower matrix of size n
 // (autotuner timing); V, W start at global row j0 with leading dimension ldv.
+// kind 3 (FP32 storage only): FP32 operands and accumulation on matrix cores.
 extern "C" hipError_t bse_launch_her2k(hipStream_t st, int fp32, int kind, int n, int s, void* A,
                                        const double2* V, const double2* W, int64_t ldv, int j0,
                                        int nb, const double2* alpha2)
 {
     if (s >= n) return hipSuccess;
-    if (s < j0 || ldv < n - j0 || kind < 0 || kind > 2) return hipErrorInvalidValue;
+    if (s < j0 || ldv < n - j0 || kind < 0 || kind > 3 || (kind == 3 && !fp32)) return hipErrorInvalidValue;
     const int64_t nT = tile_count(n) - (s >> 6);
     const dim3 g((unsigned)her2k_blocks(nT, kind));
+    if (kind == 3) {
+        hipLaunchKernelGGL(k_her2k_mfma32, g, dim3(256), 0, st, n, s, (float2*)A, V, W, ldv, j0, nb, alpha2);
+        return hipGetLastError();
+    }
 #define KH(K, TT) hipLaunchKernelGGL(K<TT>, g, dim3(256), 0, st, n, s, (TT*)A, V, W, ldv, j0, nb, alpha2)
     if (fp32) {
         if (kind == 1) KH(k_her2k_lower, float2); else if (kind == 2) KH(k_her2k_mfma2, float2); else KH(k_her2k_mfma, float2);
