@@ -348,6 +348,62 @@ __global__ __launch_bounds__(256) void k_tsqr_leaf_q(int m, int b, int k, const 
 // 256 threads; W is overwritten.
 // ---------------------------------------------------------------------------
 typedef double2 hh_sq[BR_B][BR_B + 1];
+
+__device__ inline void lds_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// X = R^{-1} for an upper triangular R of order k <= 32 in LDS, by one wave:
+// rows from the bottom, lane c forms X(i, c) from the rows below it (all
+// reads of a row issued together).  R(i, l) = A[i][l], or conj(A[l][i])
+// when lower != 0 (R = L^H for a lower L).  Entries outside the triangle
+// (and rows, columns >= k) are zero.
+__device__ inline void lds_upper_inv(hh_sq& A, bool lower, hh_sq& X, int k)
+{
+    const int c = threadIdx.x & 63;
+    const int cc = c & (BR_B - 1);
+    if (c < BR_B) {
+#pragma unroll
+        for (int i = 0; i < BR_B; ++i) X[i][c] = cmk(0, 0);
+    }
+    lds_wave_sync();
+    for (int i = k - 1; i >= 0; --i) {
+        double2 s = cmk(i == c ? 1.0 : 0.0, 0.0);
+#pragma unroll 8
+        for (int l = 0; l < BR_B; ++l) {
+            const double2 r = lower ? cconj(A[l][i]) : A[i][l];
+            const double2 x = X[l][cc];
+            if (l > i && l < k) s = csub(s, cmul(r, x));
+        }
+        const double2 d = lower ? cconj(A[i][i]) : A[i][i];
+        if (c < k && c >= i) X[i][c] = cmul(s, crecip(d));
+        lds_wave_sync();
+    }
+}
+
+// X = L^{-1} for a unit lower triangular L of order k <= 32, by one wave:
+// rows from the top.
+__device__ inline void lds_unit_lower_inv(hh_sq& L, hh_sq& X, int k)
+{
+    const int c = threadIdx.x & 63;
+    const int cc = c & (BR_B - 1);
+    if (c < BR_B) {
+#pragma unroll
+        for (int i = 0; i < BR_B; ++i) X[i][c] = cmk(0, 0);
+    }
+    lds_wave_sync();
+    for (int i = 0; i < k; ++i) {
+        double2 s = cmk(i == c ? 1.0 : 0.0, 0.0);
+#pragma unroll 8
+        for (int l = 0; l < BR_B; ++l) {
+            const double2 x = X[l][cc];
+            if (l < i) s = csub(s, cmul(L[i][l], x));
+        }
+        if (c < k && c <= i) X[i][c] = s;
+        lds_wave_sync();
+    }
+}
 __device__ __attribute__((always_inline)) inline void hh_reconstruct_lds(
     int m, int b, int k, hh_sq& W, hh_sq& L, hh_sq& U, hh_sq& Ui, hh_sq& Li, double2* d,
     const double2* __restrict__ Rfin, double2* __restrict__ V, double2* __restrict__ Tp,
@@ -385,22 +441,9 @@ __device__ __attribute__((always_inline)) inline void hh_reconstruct_lds(
         if (j < c && c < k) U[j][c] = cmul(cmk(-d[c].x, -d[c].y), W[j][c]);
     }
     __syncthreads();
-    // U^{-1} (upper) and L1^{-1} (unit lower), one column per thread
-    if (tid < k) {
-        const int c = tid;
-        for (int i = c; i >= 0; --i) {
-            double2 s = cmk(i == c ? 1.0 : 0.0, 0.0);
-            for (int l = i + 1; l <= c; ++l) s = csub(s, cmul(U[i][l], Ui[l][c]));
-            Ui[i][c] = cdiv(s, U[i][i]);
-        }
-    } else if (tid >= 64 && tid < 64 + k) {
-        const int c = tid - 64;
-        for (int i = c; i < k; ++i) {
-            double2 s = cmk(i == c ? 1.0 : 0.0, 0.0);
-            for (int l = c; l < i; ++l) s = csub(s, cmul(L[i][l], Li[l][c]));
-            Li[i][c] = s;
-        }
-    }
+    // U^{-1} (upper, wave 0) and L1^{-1} (unit lower, wave 1), by rows
+    if (tid < 64) lds_upper_inv(U, false, Ui, k);
+    else if (tid < 128) lds_unit_lower_inv(L, Li, k);
     __syncthreads();
     for (int e = tid; e < BR_B * BR_B; e += 256) {
         const int i = e % BR_B, c = e / BR_B;
@@ -494,22 +537,21 @@ __global__ __launch_bounds__(256) void k_rows_small(int m, int k0, int kx, int n
 // ---------------------------------------------------------------------------
 #define CQ_G 64           // workgroups of the row-parallel passes
 
-// PASS 0: partial G = P^H P over the workgroup's 64-row chunks (chunks g,
-// g + grid, ...).  PASS 1: Q1 = P R1^{-1} for those rows (to Q) and the
-// partial G = Q1^H Q1.  Partial g (b x b, column-major) -> Gp + g b b.
+// Partial Gram matrices G = P^H P over each workgroup's 64-row chunks
+// (chunks g, g + grid, ...), partial g (b x b, column-major) -> Gp + g b b.
+// PASS 0 reads the panel and clears the three flags (flag[0]: the TSQR is
+// needed; flag[1], flag[2]: the first, second factorisation succeeded);
+// PASS 1 reads Q1 = P R1^{-1} and runs only when flag[1] is set.
 template <int PASS>
 __global__ __launch_bounds__(256) void k_cq_gram(int m, const double2* __restrict__ P,
-                                                 const double2* __restrict__ Rinv, double2* __restrict__ Q,
                                                  double2* __restrict__ Gp, int* __restrict__ flag)
 {
     __shared__ double2 sP[64][BR_B + 1];
-    __shared__ double2 sR[BR_B][BR_B + 1];
     const int tid = threadIdx.x;
     if (PASS == 0) {
-        if (blockIdx.x == 0 && tid == 0) { flag[0] = 0; flag[1] = 0; }
+        if (blockIdx.x == 0 && tid == 0) { flag[0] = 0; flag[1] = 0; flag[2] = 0; }
     } else {
-        if (*flag) return;
-        for (int e = tid; e < BR_B * BR_B; e += 256) sR[e % BR_B][e / BR_B] = Rinv[e];
+        if (flag[1] == 0) return;
     }
     double2 g[4];
 #pragma unroll
@@ -523,25 +565,8 @@ __global__ __launch_bounds__(256) void k_cq_gram(int m, const double2* __restric
             sP[r][c] = (r < rows) ? P[(r0 + r) + (int64_t)c * m] : cmk(0, 0);
         }
         __syncthreads();
-        if (PASS == 1) {
-            const int r = tid & 63, c0 = (tid >> 6) * 8;
-            double2 q[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) q[u] = cmk(0, 0);
-            for (int l = 0; l < BR_B; ++l) {
-                const double2 p = sP[r][l];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) cfma(q[u], p, sR[l][c0 + u]);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                sP[r][c0 + u] = q[u];
-                if (r < rows) Q[(r0 + r) + (int64_t)(c0 + u) * m] = q[u];
-            }
-            __syncthreads();
-        }
         const int i = tid & 31, j0 = tid >> 5;
+#pragma unroll 4
         for (int r = 0; r < 64; ++r) {
             const double2 pi = sP[r][i];
 #pragma unroll
@@ -552,28 +577,41 @@ __global__ __launch_bounds__(256) void k_cq_gram(int m, const double2* __restric
     for (int u = 0; u < 4; ++u) Gp[(int64_t)blockIdx.x * BR_B * BR_B + tid + 256 * u] = g[u];
 }
 
-// Sum of the ng partial Gram matrices into sG (fixed order), then its
-// Cholesky factor in place (lower triangle, by the first wave: lane = row).
+// G = sum of the ng partial Gram matrices (fixed order), one element per
+// thread over 4 workgroups, 16 loads in flight per batch.  Runs when flag[f]
+// is set (f < 0: always).
+__global__ __launch_bounds__(256) void k_cq_reduce(int ng, const double2* __restrict__ Gp,
+                                                   double2* __restrict__ G, const int* __restrict__ flag, int f)
+{
+    if (f >= 0 && flag[f] == 0) return;
+    const int e = blockIdx.x * 256 + threadIdx.x;        // < BR_B * BR_B
+    double2 acc = cmk(0, 0);
+    for (int g0 = 0; g0 < ng; g0 += 16) {
+        double2 t[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) t[q] = (g0 + q < ng) ? Gp[(int64_t)(g0 + q) * BR_B * BR_B + e] : cmk(0, 0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc = cadd(acc, t[q]);
+    }
+    G[e] = acc;
+}
+
+// The Gram matrix G (Gp, summed by k_cq_reduce) into sG, then its Cholesky
+// factor in place (lower triangle).
 // pass 1 also checks |G - I| <= 0.1.  Returns false (uniformly) on failure.
 __device__ bool cq_sum_chol(int ng, const double2* __restrict__ Gp, int pass, hh_sq& sG, double* diag0,
                             int* fl)
 {
     const int tid = threadIdx.x;
     if (tid == 0) *fl = 0;
-    for (int e = tid; e < BR_B * BR_B; e += 256) {
-        double2 acc[4];
+    // G summed by k_cq_reduce (ng = 1: Gp is G itself)
+    (void)ng;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] = cmk(0, 0);
-        int g = 0;
-        for (; g + 4 <= ng; g += 4) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = cadd(acc[q], Gp[(int64_t)(g + q) * BR_B * BR_B + e]);
-        }
-        for (; g < ng; ++g) acc[0] = cadd(acc[0], Gp[(int64_t)g * BR_B * BR_B + e]);
-        const double2 s = cadd(cadd(acc[0], acc[1]), cadd(acc[2], acc[3]));
-        const int i = e % BR_B, j = e / BR_B;
-        sG[i][j] = s;
-        if (i == j) diag0[i] = s.x;
+    for (int u = 0; u < 4; ++u) {
+        const int e = tid + 256 * u, i = e % BR_B, j = e / BR_B;
+        const double2 x = Gp[e];
+        sG[i][j] = x;
+        if (i == j) diag0[i] = x.x;
     }
     __syncthreads();
     if (pass == 1) {
@@ -584,43 +622,35 @@ __device__ bool cq_sum_chol(int ng, const double2* __restrict__ Gp, int pass, hh
         }
         __syncthreads();
     }
-    if (tid < 64 && *fl == 0) {
-        const int lane = tid;
-        for (int j = 0; j < BR_B; ++j) {
-            const double dj = sG[j][j].x;
-            if (!(dj > 1e-12 * diag0[j] && diag0[j] > 0.0)) {
-                if (lane == 0) *fl = 1;
-                break;
-            }
-            const double ljj = sqrt(dj);
-            if (lane > j && lane < BR_B) sG[lane][j] = cscale(sG[lane][j], 1.0 / ljj);
-            if (lane == j) sG[j][j] = cmk(ljj, 0.0);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            if (lane > j && lane < BR_B) {
-                const double2 lij = sG[lane][j];
-                for (int c = j + 1; c <= lane; ++c) sG[lane][c] = csub(sG[lane][c], cmul(lij, cconj(sG[c][j])));
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __builtin_amdgcn_wave_barrier();
+    // right-looking Cholesky, the trailing update spread over the workgroup
+    for (int j = 0; j < BR_B; ++j) {
+        if (*fl) break;                                   // uniform
+        const double dj = sG[j][j].x;
+        if (!(dj > 1e-12 * diag0[j] && diag0[j] > 0.0)) {
+            __syncthreads();
+            if (tid == 0) *fl = 1;
+            __syncthreads();
+            break;
         }
+        const double ljj = sqrt(dj);
+        __syncthreads();                                  // every thread has read sG[j][j]
+        if (tid > j && tid < BR_B) sG[tid][j] = cscale(sG[tid][j], 1.0 / ljj);
+        if (tid == j) sG[j][j] = cmk(ljj, 0.0);
+        __syncthreads();
+        for (int e = tid; e < BR_B * BR_B; e += 256) {
+            const int i = e % BR_B, c = e / BR_B;
+            if (c > j && i >= c) sG[i][c] = csub(sG[i][c], cmul(sG[i][j], cconj(sG[c][j])));
+        }
+        __syncthreads();
     }
     __syncthreads();
     return *fl == 0;
 }
 
-// Ri = R^{-1} for R = L^H (L lower in sL), one column per thread.
+// Ri = R^{-1} for R = L^H (L lower in sL).
 __device__ void cq_rinv(hh_sq& sL, hh_sq& Ri)
 {
-    const int c = threadIdx.x;
-    if (c < BR_B) {
-        for (int i = BR_B - 1; i > c; --i) Ri[i][c] = cmk(0, 0);
-        for (int i = c; i >= 0; --i) {
-            double2 s = cmk(i == c ? 1.0 : 0.0, 0.0);
-            for (int l = i + 1; l <= c; ++l) s = csub(s, cmul(cconj(sL[l][i]), Ri[l][c]));
-            Ri[i][c] = cscale(s, 1.0 / sL[i][i].x);
-        }
-    }
+    if (threadIdx.x < 64) lds_upper_inv(sL, true, Ri, BR_B);
     __syncthreads();
 }
 
@@ -633,7 +663,7 @@ __global__ __launch_bounds__(256) void k_cq_chol1(int ng, const double2* __restr
     __shared__ double diag0[BR_B];
     __shared__ int fl;
     if (!cq_sum_chol(ng, Gp, 0, sG, diag0, &fl)) {
-        if (threadIdx.x == 0) *flag = 1;
+        if (threadIdx.x == 0) flag[0] = 1;
         return;
     }
     cq_rinv(sG, Ri);
@@ -642,6 +672,7 @@ __global__ __launch_bounds__(256) void k_cq_chol1(int ng, const double2* __restr
         R1[e] = (i <= j) ? cconj(sG[j][i]) : cmk(0, 0);
         R1i[e] = Ri[i][j];
     }
+    if (threadIdx.x == 0) flag[1] = 1;
 }
 
 // Second factorisation and the Householder form: R = R2 R1 -> Rf; the top
@@ -658,22 +689,34 @@ __global__ __launch_bounds__(256) void k_cq_chol2(int m, int b, int ng, const do
     __shared__ double2 d[BR_B];
     __shared__ double diag0[BR_B];
     __shared__ int fl;
-    if (*flag) return;
+    if (flag[1] == 0) return;
     // Cholesky of G2 in W, R2^{-1} in L
     if (!cq_sum_chol(ng, Gp, 1, W, diag0, &fl)) {
-        if (threadIdx.x == 0) *flag = 1;
+        if (threadIdx.x == 0) flag[0] = 1;
         return;
     }
     cq_rinv(W, L);
-    // R = R2 R1 (R2(i, l) = conj(W(l, i))), into Rf; both upper
+    // R1 and the top 32 x 32 block of Q1 staged in LDS (Ui, Li), one batch
+    // of loads
     for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
         const int i = e % BR_B, j = e / BR_B;
-        double2 s = cmk(0, 0);
-        for (int l = i; l <= j; ++l) cfma(s, cconj(W[l][i]), R1[l + j * BR_B]);
+        Ui[i][j] = R1[e];
+        Li[i][j] = Q1[i + (int64_t)j * m];
+    }
+    __syncthreads();
+    // R = R2 R1 (R2(i, l) = conj(W(l, i))), into Rf; both upper; the top
+    // block of Q = Q1 R2^{-1} into U
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
+        const int i = e % BR_B, j = e / BR_B;
+        double2 s = cmk(0, 0), t = cmk(0, 0);
+#pragma unroll 8
+        for (int l = 0; l < BR_B; ++l) {
+            const double2 r2 = (l >= i && l <= j) ? cconj(W[l][i]) : cmk(0, 0);
+            const double2 r2i = (l <= j) ? L[l][j] : cmk(0, 0);
+            cfma(s, r2, Ui[l][j]);
+            cfma(t, Li[i][l], r2i);
+        }
         Rf[e] = (i <= j) ? s : cmk(0, 0);
-        // top block of Q = Q1 R2^{-1}
-        double2 t = cmk(0, 0);
-        for (int l = 0; l <= j; ++l) cfma(t, Q1[i + (int64_t)l * m], L[l][j]);
         U[i][j] = t;
     }
     __syncthreads();
@@ -711,7 +754,7 @@ __global__ __launch_bounds__(256) void k_cq_chol2(int m, int b, int ng, const do
         for (int l = i; l < BR_B; ++l) cfma(s, L[i][l], W[l][j]);
         Mp[i + j * b] = s;
     }
-    if (threadIdx.x == 0) flag[1] = 1;
+    if (threadIdx.x == 0) flag[2] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -809,10 +852,23 @@ __global__ __launch_bounds__(256) void k_band_hemm(int n, int s, int k, const T*
 // staged in LDS as sA[k][row]; the next tile's loads are issued into
 // registers before the current tile's products.
 // ---------------------------------------------------------------------------
+// FP32 copy of the panel's V (m x k, ld m) for the FP32 block product, row
+// major with 32 entries per row (zero beyond k): the 64 rows a tile needs
+// are one contiguous 16 KB run.
+__global__ __launch_bounds__(256) void k_v_rows_f32(int m, int k, const double2* __restrict__ V,
+                                                    float2* __restrict__ Vr)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;     // row-major element
+    if (e >= (int64_t)m * BR_B) return;
+    const int r = (int)(e / BR_B), c = (int)(e % BR_B);
+    const double2 v = (c < k) ? V[r + (int64_t)c * m] : cmk(0, 0);
+    Vr[e] = make_float2((float)v.x, (float)v.y);
+}
+
 typedef float hemm_f4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_band_hemm32(int n, int s, int k, const float2* __restrict__ A,
-                                                     const double2* __restrict__ V, int64_t ldv,
-                                                     double2* __restrict__ Xp)
+                                                     const float2* __restrict__ V, int64_t ldv,
+                                                     double2* __restrict__ Xp)     // V: FP32, row major (32 per row)
 {
     __shared__ float2 sA[64][65];
     __shared__ float2 sV[64][BR_B + 1];
@@ -831,8 +887,7 @@ __global__ __launch_bounds__(256) void k_band_hemm32(int n, int s, int k, const 
 #pragma unroll
         for (int r = 0; r < 4; ++r) { dR[c][r] = 0.0; dI[c][r] = 0.0; }
     }
-    float2 ra[16];
-    double2 rv[8];
+    float2 ra[16], rv[8];
     // loads of tile jt into registers: element e = tid + 256 u
     auto fetch = [&](int jt) {
         const int J = tcs + jt;
@@ -842,9 +897,9 @@ __global__ __launch_bounds__(256) void k_band_hemm32(int n, int s, int k, const 
         for (int u = 0; u < 16; ++u) ra[u] = tile[tid + 256 * u];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int e = tid + 256 * u, kk = e >> 5, c = e & 31;
+            const int e = tid + 256 * u, kk = e >> 5;
             const int64_t gr = (int64_t)J * 64 + kk;
-            rv[u] = (c < k && gr >= s && gr < n) ? V[(gr - s) + (int64_t)c * ldv] : cmk(0, 0);
+            rv[u] = (gr >= s && gr < n) ? V[(gr - s) * BR_B + (e & 31)] : make_float2(0.f, 0.f);
         }
     };
     if (j0 < j1) fetch(j0);
@@ -875,7 +930,7 @@ __global__ __launch_bounds__(256) void k_band_hemm32(int n, int s, int k, const 
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int e = tid + 256 * u;
-            sV[e >> 5][e & 31] = make_float2((float)rv[u].x, (float)rv[u].y);
+            sV[e >> 5][e & 31] = rv[u];
         }
         __syncthreads();
         if (jt + 1 < j1) fetch(jt + 1);
@@ -1107,20 +1162,25 @@ extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P
     double2* R1i = p; p += BR_B * BR_B;
     double2* Rf = p; p += BR_B * BR_B;
     double2* Mp = p; p += BR_B * BR_B;
-    int* flags = reinterpret_cast<int*>(p);     // [0]: TSQR needed, [1]: fast path done
+    int* flags = reinterpret_cast<int*>(p);     // [0]: TSQR needed, [1], [2]: factorisations done
     const bool fast = (b == BR_B && m >= 2 * BR_B);
     const int* gate = nullptr;
     if (fast) {
-        const int g = (int)std::min<int64_t>(CQ_G, ceil_div64(m, 64));
-        hipLaunchKernelGGL(k_cq_gram<0>, dim3((unsigned)g), dim3(256), 0, st, m, (const double2*)P,
-                           (const double2*)nullptr, Q, Gp, flags);
-        hipLaunchKernelGGL(k_cq_chol1, dim3(1), dim3(256), 0, st, g, (const double2*)Gp, R1, R1i, flags);
-        hipLaunchKernelGGL(k_cq_gram<1>, dim3((unsigned)g), dim3(256), 0, st, m, (const double2*)P,
-                           (const double2*)R1i, Q, Gp, flags);
-        hipLaunchKernelGGL(k_cq_chol2, dim3(1), dim3(256), 0, st, m, b, g, (const double2*)Gp,
+        const int g = (int)std::min<int64_t>(CQ_G - 1, ceil_div64(m, 64));
+        double2* Gs = Gp + (int64_t)(CQ_G - 1) * BR_B * BR_B;    // the summed G (last partial slot is spare)
+        hipLaunchKernelGGL(k_cq_gram<0>, dim3((unsigned)g), dim3(256), 0, st, m, (const double2*)P, Gp, flags);
+        hipLaunchKernelGGL(k_cq_reduce, dim3(BR_B * BR_B / 256), dim3(256), 0, st, g, (const double2*)Gp, Gs,
+                           (const int*)flags, -1);
+        hipLaunchKernelGGL(k_cq_chol1, dim3(1), dim3(256), 0, st, 1, (const double2*)Gs, R1, R1i, flags);
+        hipLaunchKernelGGL(k_rows_small, dim3((unsigned)ceil_div64(m, 64)), dim3(256), 0, st, m, 0, BR_B, BR_B, BR_B,
+                           (const double2*)P, (const double2*)R1i, Q, 0, (const int*)(flags + 1));
+        hipLaunchKernelGGL(k_cq_gram<1>, dim3((unsigned)g), dim3(256), 0, st, m, (const double2*)Q, Gp, flags);
+        hipLaunchKernelGGL(k_cq_reduce, dim3(BR_B * BR_B / 256), dim3(256), 0, st, g, (const double2*)Gp, Gs,
+                           (const int*)flags, 1);
+        hipLaunchKernelGGL(k_cq_chol2, dim3(1), dim3(256), 0, st, m, b, 1, (const double2*)Gs,
                            (const double2*)Q, (const double2*)R1, Rf, Vout, Tp, Mp, AB, ldab, c0, flags);
         hipLaunchKernelGGL(k_rows_small, dim3((unsigned)ceil_div64(m - k, 64)), dim3(256), 0, st, m, k, BR_B, k, b,
-                           (const double2*)Q, (const double2*)Mp, Vout, 0, (const int*)(flags + 1));
+                           (const double2*)Q, (const double2*)Mp, Vout, 0, (const int*)(flags + 2));
         gate = flags;
     }
     // TSQR: leaf QRs
@@ -1162,7 +1222,7 @@ extern "C" int64_t bse_band_update_workspace(int n, int s, int b) {
     const int nT = tile_count(n) - (s >> 6);
     const int m = n - s;
     return (int64_t)HEMM_KSPLIT * nT * 64 * BR_B + (int64_t)ceil_div64(m, 64) * BR_B * BR_B + 2 * BR_B * BR_B
-           + (int64_t)m * b + 64;
+           + (int64_t)m * b + ((int64_t)m * BR_B + 1) / 2 + 64;   // ..., Z, FP32 copy of V
 }
 
 // A22 <- Q^H A22 Q for Q = I - V T V^H (V: m x k, ld m, rows from s).
@@ -1186,8 +1246,13 @@ extern "C" hipError_t bse_band_update(hipStream_t st, int fp32, int kind, int n,
     double2* Z = M + 2 * BR_B * BR_B;
     const dim3 gh((unsigned)nT, HEMM_KSPLIT);
     if (ev) (void)hipEventRecord(ev[0], st);
-    if (fp32 && kind == 3)
-        hipLaunchKernelGGL(k_band_hemm32, gh, dim3(256), 0, st, n, s, k, (const float2*)A, V, (int64_t)m, Xp);
+    if (fp32 && kind == 3) {
+        float2* Vf = reinterpret_cast<float2*>(Z + (int64_t)m * b);     // FP32 copy of V after Z
+        hipLaunchKernelGGL(k_v_rows_f32, dim3((unsigned)ceil_div64((int64_t)m * BR_B, 256)), dim3(256), 0, st,
+                           m, k, V, Vf);
+        hipLaunchKernelGGL(k_band_hemm32, gh, dim3(256), 0, st, n, s, k, (const float2*)A, (const float2*)Vf,
+                           (int64_t)m, Xp);
+    }
     else if (fp32)
         hipLaunchKernelGGL(k_band_hemm<float2>, gh, dim3(256), 0, st, n, s, k, (const float2*)A, V, (int64_t)m, Xp);
     else

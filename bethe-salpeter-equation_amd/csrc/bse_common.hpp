@@ -151,6 +151,24 @@ __device__ inline double wave_sum_dpp(double v) {
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
     return __hiloint2double(hi, lo);
 }
+// Wave maximum of a non-negative double by the same DPP steps; lanes that a
+// row-masked step leaves out take max(v, 0), which changes nothing.
+__device__ inline double wave_max_dpp(double v) {
+    v = fmax(v, dpp_mov_d<0xB1, 0xF>(v));
+    v = fmax(v, dpp_mov_d<0x4E, 0xF>(v));
+    v = fmax(v, dpp_mov_d<0x141, 0xF>(v));
+    v = fmax(v, dpp_mov_d<0x140, 0xF>(v));
+    v = fmax(v, dpp_mov_d<0x142, 0xA>(v));
+    v = fmax(v, dpp_mov_d<0x143, 0xC>(v));
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+// 1 / a for complex a as conj(a) / |a|^2: one real division.
+__host__ __device__ inline double2 crecip(double2 a) {
+    const double r = 1.0 / (a.x * a.x + a.y * a.y);
+    return make_double2(a.x * r, -a.y * r);
+}
 // Block reduction of a double over blockDim.x threads (multiple of 64, <= 1024).
 // Every thread receives the total.  Deterministic order.
 __device__ inline double block_sum(double v, double* sh /* >= 16 doubles */) {

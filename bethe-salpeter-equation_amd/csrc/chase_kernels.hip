@@ -390,25 +390,39 @@ __device__ inline double bii_hash(uint64_t a, uint64_t b) {
     return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-// B(r, c) - lam delta_rc for 0 <= r, c < n (zero outside the band)
-__device__ inline double2 band_elem(const double2* AB0, int ldab, int n, int b, int r, int c, double lam) {
-    if (r >= n || c >= n || r < 0 || c < 0) return cmk(0, 0);
-    const int dd = r - c;
-    if (dd > b || dd < -b) return cmk(0, 0);
-    double2 x = (dd >= 0) ? AB0[(int64_t)c * ldab + dd] : cconj(AB0[(int64_t)r * ldab - dd]);
-    if (dd == 0) x = cmk(x.x - lam, 0.0);
+#define BII_W (2 * CH_B + 1)
+#define BII_BLK 64          // steps whose inputs are staged in LDS at once
+
+// B(R, C) - lam delta_RC without branches (0 outside the band and the
+// matrix): a masked-off element reads past the descriptor's range.
+__device__ inline double2 band_elem_nb(__amdgpu_buffer_rsrc_t rab, int ldab, int n, int b, int R, int C,
+                                       double lam) {
+    const bool lower = C <= R;
+    const bool ok = R >= 0 && R < n && C >= 0 && C < n && (lower ? R - C : C - R) <= b;
+    const int idx = lower ? C * ldab + (R - C) : R * ldab + (C - R);
+    double2 x = bufld<double2>::ld<0>(rab, ok ? idx * 16 : CH_OOB, 0);
+    if (!lower) x.y = -x.y;
+    if (R == C) x = cmk(x.x - lam, 0.0);
     return x;
 }
 
-#define BII_W (2 * CH_B + 1)
-#define BII_PF 8            // steps of prefetch (rows, multipliers, U rows)
+// One wave per eigenvalue.  LU: lane c updates window column j+1+c; the
+// band rows entering over the next BII_BLK steps are staged in LDS in one
+// batch of loads, so a step touches only LDS and registers (its U row,
+// multipliers and pivot index are stored, never waited for).  Solves: the
+// multipliers (forward) and U rows (backward) of BII_BLK steps are staged
+// the same way.  The band store is addressed with 32-bit byte offsets
+// (n (2b + 1) 16 < 2^31).
 __global__ __launch_bounds__(64) void k_band_ii(int n, int b, int ldab, const double2* __restrict__ AB0,
                                                 const double* __restrict__ lam, double tnorm,
                                                 double2* __restrict__ Ub, double2* __restrict__ Lm,
                                                 int* __restrict__ piv, double2* __restrict__ X, int64_t ldx)
 {
     __shared__ double2 W[CH_B + 1][BII_W];
+    __shared__ double2 buf[BII_BLK * BII_W];
     __shared__ double2 lb[CH_B];
+    __shared__ double2 sx[BII_BLK];
+    __shared__ int spv[BII_BLK];
     const int m = blockIdx.x, lane = threadIdx.x;
     const double lm = lam[m];
     const int wb = 2 * b + 1;                    // U row length
@@ -417,156 +431,182 @@ __global__ __launch_bounds__(64) void k_band_ii(int n, int b, int ldab, const do
     double2* Lmul = Lm + (int64_t)m * n * b;
     int* pv = piv + (int64_t)m * n;
     const double tiny = DBL_EPSILON * tnorm + DBL_MIN;
-    auto rs = [&](int r) { return r % nr; };
-    auto cs = [&](int c) { return c % wb; };
-    // band row R, entry c = 0 .. 2b: B(R, R - b + c) - lam delta; lane c
-    // holds entry c, lane 0 also entry 2b (returned in *hi)
-    auto row_load = [&](int R, double2* hi) -> double2 {
-        const double2 v = (R < n) ? band_elem(AB0, ldab, n, b, R, R - b + lane, lm) : cmk(0, 0);
-        *hi = (R < n && lane == 0) ? band_elem(AB0, ldab, n, b, R, R + b, lm) : cmk(0, 0);
-        return v;
-    };
+    const __amdgpu_buffer_rsrc_t rab = bse_rsrc(AB0, (int)((int64_t)n * ldab * 16));
     // initial window: rows 0 .. b, columns 0 .. 2b
     for (int e = lane; e < nr * wb; e += 64) {
         const int r = e / wb, c = e % wb;
-        W[rs(r)][cs(c)] = band_elem(AB0, ldab, n, b, r, c, lm);
+        W[r % nr][c % wb] = band_elem_nb(rab, ldab, n, b, r, c, lm);
     }
-    // rows entering at steps j .. j + BII_PF - 1 (row j + b + 1 at step j)
-    double2 pre[BII_PF], preh[BII_PF];
+    // circular slots of row j (mod b + 1) and column j (mod 2b + 1), carried
+    // from step to step; a slot offset by d < modulus wraps at most once
+    int rj = 0, cj = 0;
+    auto wrap = [](int v, int mod) { return v >= mod ? v - mod : v; };
+    const bool act = lane < 2 * b;           // lane c: column j+1+c (lanes beyond the window idle)
+    for (int j0 = 0; j0 < n; j0 += BII_BLK) {
+        // band rows R = j0 + b + 1 + r entering at steps j0 + r: entry c
+        // (0 .. 2b) is B(R, R - b + c); lane c loads entry c, lane 0 also 2b
+        wave_lds_sync();
+        for (int r0 = 0; r0 < BII_BLK; r0 += 16) {
+            double2 t[16];
 #pragma unroll
-    for (int u = 0; u < BII_PF; ++u) pre[u] = row_load(b + 1 + u, &preh[u]);
-    wave_lds_sync();
-    for (int j0 = 0; j0 < n; j0 += BII_PF) {
-#pragma unroll
-        for (int u = 0; u < BII_PF; ++u) {
-            const int j = j0 + u;
-            if (j >= n) break;
-            const int rmax = min(j + b, n - 1);
-            // 1. pivot among rows j .. rmax (ties to the lower row)
-            double best = -1.0;
-            int bi = j;
-            if (lane <= b && j + lane <= rmax) { best = cabs2(W[rs(j + lane)][cs(j)]); bi = j + lane; }
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ob = __shfl_xor(best, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            for (int r = 0; r < 16; ++r) {
+                const int R = j0 + b + 1 + r0 + r;
+                t[r] = band_elem_nb(rab, ldab, n, b, R, R - b + lane, lm);
             }
-            const int p = bi;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) buf[(r0 + r) * BII_W + lane] = t[r];
+        }
+        {
+            if (lane == 0) {
+#pragma unroll 8
+                for (int r = 0; r < BII_BLK; ++r) {
+                    const int R = j0 + b + 1 + r;
+                    buf[r * BII_W + 2 * b] = band_elem_nb(rab, ldab, n, b, R, R + b, lm);
+                }
+            }
+        }
+        wave_lds_sync();
+        const int jend = min(n, j0 + BII_BLK);
+        for (int j = j0; j < jend; ++j) {
+            const int rmax = min(j + b, n - 1);
+            const int ccol = wrap(cj + 1 + lane, wb);   // slot of column j+1+lane (lanes < 2b)
+            // 1. pivot among rows j .. rmax: the largest |.|^2 (wave maximum),
+            // ties to the lower row (first set bit of the ballot)
+            const bool cand = lane <= b && j + lane <= rmax;
+            const double2 wj = W[wrap(rj + (lane <= b ? lane : 0), nr)][cj];
+            const double a2 = cand ? cabs2(wj) : 0.0;
+            const double amax = wave_max_dpp(a2);
+            const int p = j + (int)__builtin_ctzll(__ballot(cand && a2 == amax));
+            const int rp = wrap(rj + (p - j), nr);
             // 2. pivot and multipliers (column j is not touched by the swap)
-            double2 uu = W[rs(p)][cs(j)];
+            double2 uu = W[rp][cj];
             if (cabs2(uu) < tiny * tiny) uu = cmk(uu.x >= 0.0 ? tiny : -tiny, 0.0);
+            const double2 uinv = crecip(uu);
             if (lane < b) {
                 const int r = j + 1 + lane;
                 double2 l = cmk(0, 0);
-                if (r <= rmax) l = cdiv((r == p) ? W[rs(j)][cs(j)] : W[rs(r)][cs(j)], uu);
+                if (r <= rmax) l = cmul((r == p) ? W[rj][cj] : W[wrap(rj + 1 + lane, nr)][cj], uinv);
                 lb[lane] = l;
                 Lmul[(int64_t)j * b + lane] = l;
             }
             // 3. swap rows j and p over columns j+1 .. j+2b (lane c: column
             // j+1+c), 4. U row j, from the swapped pivot row
             const int col = j + 1 + lane;
-            const bool act = lane < 2 * b;           // lanes beyond the window idle
-            double2 prv = act ? W[rs(j)][cs(col)] : cmk(0, 0);
+            double2 prv = act ? W[rj][ccol] : cmk(0, 0);
             if (p != j && act) {
-                const double2 t = W[rs(p)][cs(col)];
-                W[rs(p)][cs(col)] = prv;
+                const double2 t = W[rp][ccol];
+                W[rp][ccol] = prv;
                 prv = t;
             }
             if (act) U[(int64_t)j * wb + 1 + lane] = (col < n) ? prv : cmk(0, 0);
             if (lane == 0) {
-                U[(int64_t)j * wb] = uu;
+                U[(int64_t)j * wb] = uinv;           // the backward solve multiplies by it
                 pv[j] = p;
             }
             wave_lds_sync();
-            // 5. elimination of rows j+1 .. rmax in column j+1+lane
+            // 5. elimination of rows j+1 .. rmax in column j+1+lane: all
+            // reads of the column issued, then the updates written
+            // (rows past the matrix end have zero multipliers, so only the
+            // uniform t < b decides; the reads of a batch precede its writes)
             if (act) {
-                for (int t = 0; t < b; ++t) {
-                    const int r = j + 1 + t;
-                    if (r <= rmax) W[rs(r)][cs(col)] = csub(W[rs(r)][cs(col)], cmul(lb[t], prv));
+#pragma unroll
+                for (int t0 = 0; t0 < CH_B; t0 += 8) {
+                    double2 wv8[8], lv8[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        wv8[q] = W[wrap(rj + 1 + t0 + q, nr)][ccol];
+                        lv8[q] = lb[t0 + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (t0 + q < b) W[wrap(rj + 1 + t0 + q, nr)][ccol] = csub(wv8[q], cmul(lv8[q], prv));
                 }
             }
             // 6. the next window: row j+b+1 takes row j's slot (columns j+1 ..
             // j+2b+1); column j+2b+1 takes column j's slot, zero in rows j+1 .. j+b
-            if (act) W[rs(j + b + 1)][cs(col)] = pre[u];
-            if (lane == 0) W[rs(j + b + 1)][cs(j + 2 * b + 1)] = preh[u];
-            if (lane < b) W[rs(j + 1 + lane)][cs(j + 2 * b + 1)] = cmk(0, 0);
-            pre[u] = row_load(j + b + 1 + BII_PF, &preh[u]);
+            const int r = j - j0;
+            if (act) W[rj][ccol] = buf[r * BII_W + lane];
+            if (lane == 0) W[rj][cj] = buf[r * BII_W + 2 * b];
+            if (lane < b) W[wrap(rj + 1 + lane, nr)][cj] = cmk(0, 0);
+            rj = wrap(rj + 1, nr);
+            cj = wrap(cj + 1, wb);
             wave_lds_sync();
         }
     }
     // ---- solves: x <- U^-1 L^-1 P x, three times from a random start
     double2* x = X + (int64_t)m * ldx;
     for (int i = lane; i < n; i += 64) x[i] = cmk(bii_hash(m, 2 * i), bii_hash(m, 2 * i + 1));
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   // the LU's stores and x are complete
     const int bw = 2 * b;
     for (int it = 0; it < 3; ++it) {
-        // forward: window x[j .. j+b] in lanes 0 .. b; the pivots, the
-        // multipliers and the entering x[j+b+1] of the next BII_PF steps are
-        // loaded one block ahead
+        // forward: window x[j .. j+b] in lanes 0 .. b; a block's pivots,
+        // multipliers and entering x[j+b+1] staged in LDS
         double2 xw = (lane <= b && lane < n) ? x[lane] : cmk(0, 0);
-        int fp[BII_PF];
-        double2 fl[BII_PF], fx[BII_PF];
-        auto ffetch = [&](int jb) {
+        for (int j0 = 0; j0 < n; j0 += BII_BLK) {
+            const int jend = min(n, j0 + BII_BLK);
+            wave_lds_sync();
+            for (int q0 = 0; q0 < BII_BLK * CH_B / 64; q0 += 16) {
+                double2 t[16];
 #pragma unroll
-            for (int u = 0; u < BII_PF; ++u) {
-                const int j = jb + u;
-                fp[u] = (j < n) ? pv[j] : 0;
-                fl[u] = (j < n && lane >= 1 && lane <= b && j + lane < n) ? Lmul[(int64_t)j * b + lane - 1] : cmk(0, 0);
-                fx[u] = (lane == b && j + b + 1 < n) ? x[j + b + 1] : cmk(0, 0);
+                for (int q = 0; q < 16; ++q) {
+                    const int e = lane + 64 * (q0 + q), u = e / CH_B, tt = e % CH_B;
+                    const int j = j0 + u;
+                    t[q] = (j < n && tt < b) ? Lmul[(int64_t)j * b + tt] : cmk(0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < 16; ++q) buf[lane + 64 * (q0 + q)] = t[q];
             }
-        };
-        ffetch(0);
-        for (int j0 = 0; j0 < n; j0 += BII_PF) {
-            int cp[BII_PF];
-            double2 cl[BII_PF], cx[BII_PF];
-#pragma unroll
-            for (int u = 0; u < BII_PF; ++u) { cp[u] = fp[u]; cl[u] = fl[u]; cx[u] = fx[u]; }
-            ffetch(j0 + BII_PF);
-#pragma unroll
-            for (int u = 0; u < BII_PF; ++u) {
-                const int j = j0 + u;
-                if (j >= n) break;
-                const int lp = cp[u] - j;             // 0 .. b
+            {
+                const int jp = j0 + lane;
+                spv[lane] = (jp < n) ? pv[jp] : jp;
+                sx[lane] = (j0 + b + 1 + lane < n) ? x[j0 + b + 1 + lane] : cmk(0, 0);
+            }
+            wave_lds_sync();
+            for (int j = j0; j < jend; ++j) {
+                const int u = j - j0;
+                const int lp = spv[u] - j;             // 0 .. b
                 const double2 xj = cmk(__shfl(xw.x, 0, 64), __shfl(xw.y, 0, 64));
                 const double2 xp = cmk(__shfl(xw.x, lp, 64), __shfl(xw.y, lp, 64));
                 if (lane == lp) xw = xj;
                 if (lane == 0) xw = xp;
                 // x[j+t] -= l_t x[j]  (t = 1 .. b)
-                if (lane >= 1 && lane <= b && j + lane < n) xw = csub(xw, cmul(cl[u], xp));
+                const double2 l = buf[u * CH_B + ((lane - 1) & (CH_B - 1))];
+                if (lane >= 1 && lane <= b && j + lane < n) xw = csub(xw, cmul(l, xp));
                 if (lane == 0) x[j] = xw;
                 // shift the window down by one lane; x[j+b+1] enters at lane b
                 const double2 nx = cmk(__shfl_down(xw.x, 1, 64), __shfl_down(xw.y, 1, 64));
                 xw = nx;
-                if (lane == b) xw = cx[u];
+                if (lane == b) xw = sx[u];
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        // backward: lane t holds x[j+1+t] (t < 2b); U rows and x[j] loaded a
-        // block ahead (x[j] is final from the forward pass)
+        // backward: lane t holds x[j+1+t] (t < 2b); a block's U rows and x[j]
+        // (final from the forward pass) staged in LDS, rows in descending order
         double2 xr = cmk(0, 0);
         double nrm = 0.0;
-        double2 bu[BII_PF], bd[BII_PF], bx[BII_PF];
-        auto bfetch = [&](int jb) {
+        for (int j0 = n - 1; j0 >= 0; j0 -= BII_BLK) {
+            const int jlo = max(0, j0 - BII_BLK + 1);
+            wave_lds_sync();
+            for (int u0 = 0; u0 < BII_BLK; u0 += 16) {
+                double2 t[16];
 #pragma unroll
-            for (int u = 0; u < BII_PF; ++u) {
-                const int j = jb - u;
-                bu[u] = (j >= 0 && lane < bw && j + 1 + lane < n) ? U[(int64_t)j * wb + 1 + lane] : cmk(0, 0);
-                bd[u] = (j >= 0) ? U[(int64_t)j * wb] : cmk(1, 0);
-                bx[u] = (j >= 0) ? x[j] : cmk(0, 0);
+                for (int u = 0; u < 16; ++u) {
+                    const int j = j0 - u0 - u;
+                    t[u] = (j >= jlo && lane < bw && j + 1 + lane < n) ? U[(int64_t)j * wb + 1 + lane] : cmk(0, 0);
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u) buf[(u0 + u) * 64 + lane] = t[u];
             }
-        };
-        bfetch(n - 1);
-        for (int j0 = n - 1; j0 >= 0; j0 -= BII_PF) {
-            double2 cu[BII_PF], cd[BII_PF], cx[BII_PF];
-#pragma unroll
-            for (int u = 0; u < BII_PF; ++u) { cu[u] = bu[u]; cd[u] = bd[u]; cx[u] = bx[u]; }
-            bfetch(j0 - BII_PF);
-#pragma unroll
-            for (int u = 0; u < BII_PF; ++u) {
-                const int j = j0 - u;
-                if (j < 0) break;
-                const double2 acc = wave_sum_c(cmul(cu[u], xr));
-                const double2 xj = cdiv(csub(cx[u], acc), cd[u]);
+            {
+                const int jd = j0 - lane;
+                buf[BII_BLK * 64 + lane] = (jd >= jlo) ? U[(int64_t)jd * wb] : cmk(1, 0);   // 1 / U(j, j)
+                sx[lane] = (jd >= jlo) ? x[jd] : cmk(0, 0);
+            }
+            wave_lds_sync();
+            for (int j = j0; j >= jlo; --j) {
+                const int u = j0 - j;
+                const double2 acc = wave_sum_c(cmul(buf[u * 64 + lane], xr));
+                const double2 xj = cmul(csub(sx[u], acc), buf[BII_BLK * 64 + u]);
                 nrm += cabs2(xj);
                 if (lane == 0) x[j] = xj;
                 // shift up: new lane t holds old lane t-1; lane 0 gets x[j]

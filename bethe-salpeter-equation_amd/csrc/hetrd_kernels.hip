@@ -927,8 +927,11 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int
     tri_tile(idx, ti, tj);
     const int tcs = s >> 6;
     T* tile = A + tile_pos(tcs + ti, tcs + tj, tile_count(n)) * (BSE_TILE * BSE_TILE);
-    // chunk of 16 complex K columns: real and imaginary parts, [kcol][row]
-    __shared__ R sPr[16][64], sPi[16][64], sQr[16][64], sQi[16][64];
+    // chunk of 16 complex K columns: real and imaginary parts, [kcol][row];
+    // rows padded so that the four 16-lane groups of an operand read (four
+    // consecutive kcol) start 16 banks apart
+    constexpr int LPAD = 64 + 64 / (int)sizeof(R);
+    __shared__ R sPr[16][LPAD], sPi[16][LPAD], sQr[16][LPAD], sQi[16][LPAD];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int r0 = (tcs + ti) * 64, c0 = (tcs + tj) * 64;
     const int rowbase = (wv >> 1) * 32, colbase = (wv & 1) * 32;
@@ -971,6 +974,19 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int
         }
     };
     fetch(0);
+    // the tile entries this lane updates, loaded before the products so that
+    // their latency hides under them
+    T old[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int reg = 0; reg < 4; ++reg) {
+                const int lr = rowbase + a * 16 + (lane & 15);
+                const int lc = colbase + b * 16 + MM::row(lane, reg);
+                old[a][b][reg] = tile[lr + lc * 64];
+            }
     for (int l0 = 0; l0 < K2; l0 += 16) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -1028,7 +1044,7 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int
                 const int gr = r0 + lr, gc = c0 + lc;
                 if (gc >= s && gr < n && gr >= gc) {
                     T* p = tile + lr + lc * 64;
-                    double2 x = cplx<T>::load(p);
+                    double2 x = cplx<T>::load(&old[a][b][reg]);
                     x.x -= (double)accR[a][b][reg];
                     x.y -= (double)accI[a][b][reg];
                     if (gr == gc) x.y = 0.0;
