@@ -28,6 +28,7 @@ ABI_SYMBOLS = (
     "bse_batch_prepare", "bse_batch_run", "bse_batch_results", "bse_batch_info",
     "bse_grad_spectrum", "bse_refine_info", "bse_set_problem_klist", "bse_tb_states_at",
     "bse_eigh_dense_noalloc", "bse_noalloc_info", "bse_build_full", "bse_eigh_full", "bse_get_coupling",
+    "bse_wannier_integrals",
 )
 
 
@@ -49,7 +50,13 @@ class SystemC(C.Structure):
                 ("direct_scale", C.c_double), ("dipole_dir", C.c_double * 3),
                 ("dimension", C.c_int32), ("n3", C.c_int32), ("shift3", C.c_double),
                 ("radius", C.c_double), ("screening", C.c_int32), ("rpa_nv", C.c_int32),
-                ("rpa_nc", C.c_int32)]
+                ("rpa_nc", C.c_int32), ("dipole_model", C.c_int32), ("dipole_nr", C.c_int32),
+                ("dipole_r", C.POINTER(C.c_double)), ("dipole_d", C.POINTER(C.c_double))]
+
+
+class WannierGridC(C.Structure):
+    _fields_ = [("nw", C.c_int32), ("nspin", C.c_int32), ("n_sc", C.c_int32 * 3), ("p", C.c_int32 * 3),
+                ("origin", C.c_double * 3), ("axis", C.c_double * 9), ("w", C.POINTER(C.c_double))]
 
 
 class BatchOptionsC(C.Structure):
@@ -86,7 +93,17 @@ def _system_to_c(s):
     sc.screening = 1 if scr == "rpa" else 0
     sc.rpa_nv = int(getattr(s, "rpa_nv", 0))
     sc.rpa_nc = int(getattr(s, "rpa_nc", 0))
-    return mc, sc, [r_cart, h_r, tau]
+    keep = [r_cart, h_r, tau]
+    dr = getattr(s, "dipole_r", None)
+    if dr is not None:
+        dr = np.ascontiguousarray(dr, dtype=np.float64).reshape(-1, 3)
+        dd = np.ascontiguousarray(s.dipole_d, dtype=np.float64)
+        if dd.shape != (dr.shape[0], m.norb, m.norb, 3):
+            raise ValueError(f"dipole_d must be (nR, norb, norb, 3), got {dd.shape}")
+        sc.dipole_model, sc.dipole_nr = 1, dr.shape[0]
+        sc.dipole_r, sc.dipole_d = _dptr(dr), _dptr(dd)
+        keep += [dr, dd]
+    return mc, sc, keep
 
 
 def _model_to_c(m):
@@ -145,6 +162,8 @@ def load_library(path: pathlib.Path | str | None = None):
         "bse_build_full": (i32, [vp]),
         "bse_eigh_full": (i32, [vp, i32]),
         "bse_get_coupling": (i32, [vp, dp, i64]),
+        "bse_wannier_integrals": (i32, [vp, C.POINTER(WannierGridC), i32, C.POINTER(C.c_int32), i32, dp,
+                                        dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -326,6 +345,33 @@ class Context:
         self._check(self.lib.bse_grad_spectrum(self.h, _dptr(om), om.size, float(eta), _dptr(tg), w,
                                                C.byref(L), _dptr(dL)), "bse_grad_spectrum")
         return L.value, dL
+
+    def wannier_integrals(self, grid, r_cells, kvec=None):
+        """Real-space Wannier integrals on the GPU (bse_wannier_integrals):
+        A (nspin, nw, nw, nR, nK) complex, D (nspin, nw, nw, nR, 3), S
+        (nspin, nw, nw, nR) for the lattice translations r_cells (nR x 3,
+        unit cells) and Cartesian vectors kvec (nK x 3, 1/A)."""
+        w = np.ascontiguousarray(grid.values, dtype=np.float64)
+        gc = WannierGridC()
+        gc.nw, gc.nspin = grid.nw, grid.nspin
+        gc.n_sc[:] = [int(x) for x in grid.n_sc]
+        gc.p[:] = [int(x) for x in grid.p]
+        gc.origin[:] = [float(x) for x in grid.origin]
+        gc.axis[:] = list(np.asarray(grid.axes, dtype=np.float64).T.ravel())
+        gc.w = _dptr(w)
+        rc = np.ascontiguousarray(np.asarray(r_cells, dtype=np.int32).reshape(-1, 3))
+        kv = np.ascontiguousarray(np.zeros((0, 3)) if kvec is None else np.asarray(kvec, dtype=np.float64)
+                                  ).reshape(-1, 3)
+        nr, nk, ns, nw = rc.shape[0], kv.shape[0], grid.nspin, grid.nw
+        A = np.zeros(max(ns * nw * nw * nr * nk, 1), dtype=np.complex128)
+        D = np.zeros(ns * nw * nw * nr * 3)
+        S = np.zeros(ns * nw * nw * nr)
+        self._check(self.lib.bse_wannier_integrals(
+            self.h, C.byref(gc), nr, rc.ctypes.data_as(C.POINTER(C.c_int32)), nk,
+            _dptr(kv) if nk else None, A.ctypes.data_as(C.POINTER(C.c_double)) if nk else None,
+            _dptr(D), _dptr(S)), "bse_wannier_integrals")
+        A = A[:ns * nw * nw * nr * nk].reshape(ns, nw, nw, nr, nk)
+        return A, D.reshape(ns, nw, nw, nr, 3), S.reshape(ns, nw, nw, nr)
 
     def eigh(self, n_vectors=-1, refine=False):
         self._check(self.lib.bse_eigh(self.h, int(n_vectors), int(bool(refine))), "bse_eigh")

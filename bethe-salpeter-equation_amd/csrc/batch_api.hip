@@ -431,6 +431,8 @@ extern "C" int bse_batch_prepare(bse_ctx* ctx, int n_systems, const bse_tb_model
         const Geom& g = b->hs[i].g;
         if (systems[i].screening != 0)
             return fail(ctx, BSE_ERR_UNSUPPORTED, "system " + std::to_string(i) + ": the batch uses model screening");
+        if (systems[i].dipole_model != 0)
+            return fail(ctx, BSE_ERR_UNSUPPORTED, "system " + std::to_string(i) + ": the batch uses point-orbital dipoles");
         if (g.N < 2 || g.N > 4096)
             return fail(ctx, BSE_ERR_UNSUPPORTED, "system " + std::to_string(i) + ": batched N must be in [2, 4096]");
         if (opt->n_lowest > g.N) return fail(ctx, BSE_ERR_ARG, "n_lowest exceeds a system's N");

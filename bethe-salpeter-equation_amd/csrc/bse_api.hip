@@ -154,6 +154,19 @@ static int load_problem(bse_ctx* ctx, const bse_tb_model* m, const Geom& g, cons
     HCK(hipMemcpyAsync(d_k, ctx->kpts_h.data(), 3 * sizeof(double) * ctx->nk, hipMemcpyHostToDevice, ctx->st));
     HCK(hipMemcpyAsync(d_g, ctx->gvec_h.data(), 3 * sizeof(double) * ctx->ng, hipMemcpyHostToDevice, ctx->st));
     HCK(hipMemcpyAsync(d_q, s->q, 3 * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    ctx->dip_nr = 0;
+    if (s->dipole_model == 1) {
+        // the Wannier position table is copied in (the caller's arrays need not outlive the call)
+        const int nr = s->dipole_nr;
+        double* d_dr = BUF(double, S_DIPR, 3 * (size_t)nr);
+        double* d_dd = BUF(double, S_DIPD, (size_t)nr * m->norb * m->norb * 3);
+        HCK(hipMemcpyAsync(d_dr, s->dipole_r, 3 * sizeof(double) * nr, hipMemcpyHostToDevice, ctx->st));
+        HCK(hipMemcpyAsync(d_dd, s->dipole_d, sizeof(double) * nr * m->norb * m->norb * 3,
+                           hipMemcpyHostToDevice, ctx->st));
+        ctx->dip_nr = nr;
+        ctx->sys.dipole_r = nullptr;       // the host pointers are not kept
+        ctx->sys.dipole_d = nullptr;
+    }
     ctx->kmap_h.clear();
     ctx->kinv_h.clear();
     if (kpos) {
@@ -401,9 +414,18 @@ static int run_tables(bse_ctx* ctx) {
                                s.r0, s.eps_env, s.radius, kC, ctx->wbar, 0, wtab));
     }
     HCK(bse_launch_v_exchange(ctx->st, no, ctx->ng, ctx->dim, gvec, tau, kC, s.radius, s.q, vx));
+    const double2* dq = nullptr;
+    if (ctx->dip_nr > 0) {
+        // Wannier position matrix on the grid: Dq_ij(k) = sum_R e^{-ik.R} qhat.D_ij(R)
+        double2* dqb = BUF(double2, S_DIPQ, (size_t)ctx->nk * no * no);
+        HCK(bse_launch_dipole_table(ctx->st, ctx->nk, no, ctx->dip_nr, (const double*)ctx->bufs[S_KPTS].p,
+                                    (const double*)ctx->bufs[S_DIPR].p, (const double*)ctx->bufs[S_DIPD].p,
+                                    ctx->qhat, dqb));
+        dq = dqb;
+    }
     HCK(bse_launch_transitions(ctx->st, ctx->nk, s.nv, s.nc, no, (const double*)ctx->bufs[S_EPS].p,
                                (const double2*)ctx->bufs[S_COEF].p, nullptr, nullptr, tau, ctx->qhat,
-                               vx, P, PV, rho, dE));
+                               vx, P, PV, rho, dE, dq));
     stage_end(ctx, ST_TABLES);
     ctx->have_tables = true;
     return BSE_OK;
@@ -1325,7 +1347,8 @@ static int grad_tb_apply(bse_ctx* ctx, int which, int k, const double2* A, doubl
     HCK(bse_launch_transitions(ctx->st, ctx->nk, s.nv, s.nc, no, (const double*)ctx->bufs[S_EPS].p,
                                (const double2*)ctx->bufs[S_COEF].p, deps, dcoef,
                                (const double*)ctx->bufs[S_TAU].p, ctx->qhat,
-                               (const double2*)ctx->bufs[S_VX].p, dP, dPV, drho, ddE));
+                               (const double2*)ctx->bufs[S_VX].p, dP, dPV, drho, ddE,
+                               ctx->dip_nr > 0 ? (const double2*)ctx->bufs[S_DIPQ].p : nullptr));
     const double2* coef = (const double2*)ctx->bufs[S_COEF].p;
     const double2* wt = (const double2*)ctx->bufs[S_WTAB].p;
     const double alpha = -s.direct_scale / (ctx->cell * ctx->nk);
@@ -1421,6 +1444,8 @@ extern "C" int bse_grad_spectrum(bse_ctx* ctx, const double* omega, int n_omega,
         return fail(ctx, BSE_ERR_STATE, "run bse_eigh on a built system first");
     if (ctx->nvec != ctx->n_eig)
         return fail(ctx, BSE_ERR_STATE, "the spectrum gradient needs every eigenvector (bse_eigh(ctx, -1, 0))");
+    if (ctx->dip_nr > 0)
+        return fail(ctx, BSE_ERR_UNSUPPORTED, "spectrum gradients use point-orbital dipoles");
     if (ctx->sys.screening != 0 || ctx->full_eig)
         return fail(ctx, BSE_ERR_UNSUPPORTED, "gradients hold the model W fixed: TDA with model screening only");
     (void)hipSetDevice(ctx->device);

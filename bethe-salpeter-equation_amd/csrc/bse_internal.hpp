@@ -23,7 +23,13 @@ hipError_t bse_launch_v_exchange(hipStream_t, int, int, int, const double*, cons
                                  double, const double*, double2*);
 hipError_t bse_launch_transitions(hipStream_t, int, int, int, int, const double*, const double2*,
                                   const double*, const double2*, const double*, const double*,
-                                  const double2*, double2*, double2*, double2*, double*);
+                                  const double2*, double2*, double2*, double2*, double*,
+                                  const double2* dq = nullptr);
+hipError_t bse_launch_dipole_table(hipStream_t, int, int, int, const double*, const double*, const double*,
+                                   const double*, double2*);
+hipError_t bse_launch_wannier_integrals(hipStream_t, const int*, const int*, const double*, const double*, double,
+                                        int, int, int, const int*, int, const double*, const double*, double2*,
+                                        double*, double*);
 size_t bse_assemble_lds_bytes(int, int, int, int);
 int bse_assemble_pick_tk(int, int, int);
 hipError_t bse_launch_assemble(hipStream_t, int, int, int, int, int, int, int, int, int, int64_t,
@@ -176,6 +182,7 @@ struct bse_ctx {
     int64_t N = 0;
     double recip[9] = {0}, area = 0, volume = 0, cell = 0, wbar = 0, vbar = 0, qhat[3] = {1, 0, 0};
     std::vector<double> kpts_h, gvec_h, hr_h, rcart_h;
+    int dip_nr = 0;                 // lattice vectors of the Wannier position table (0: point orbitals)
     // state
     bool have_tb = false, have_tables = false, built = false, have_eig = false, have_f = false;
     int precision = BSE_PREC_FP64;
@@ -259,6 +266,9 @@ enum Slot {
     S_BCPL, S_MFULL, S_WFULL, S_LINV, S_FPAN, S_FPANH, S_RHOT, S_ZT, S_XFULL, S_FERR, S_FSCALE, S_EVNEG,
     // RPA screening: window states at the grid points, Pi(Q)
     S_RPAEPS, S_RPACOEF, S_RPAPI,
+    // Wannier position table (R, D_ij(R), Dq_ij(k)) and the real-space
+    // integrals' grid, translations, vectors and outputs
+    S_DIPR, S_DIPD, S_DIPQ, S_WGRID, S_WR, S_WK, S_WA, S_WD, S_WS,
     S_NSLOTS
 };
 
@@ -374,6 +384,11 @@ int make_geom(const bse_tb_model* m, const bse_system* s, Geom& g, std::string& 
         return BSE_ERR_ARG;
     }
     if (s->screening != 0 && s->screening != 1) { msg = "screening must be 0 (model) or 1 (RPA)"; return BSE_ERR_ARG; }
+    if (s->dipole_model != 0 && s->dipole_model != 1) { msg = "dipole_model must be 0 or 1"; return BSE_ERR_ARG; }
+    if (s->dipole_model == 1 && (s->dipole_nr < 1 || !s->dipole_r || !s->dipole_d)) {
+        msg = "the Wannier dipole model needs dipole_nr >= 1 lattice vectors and the position table";
+        return BSE_ERR_ARG;
+    }
     if (s->screening == 1 && (s->rpa_nv < 0 || s->rpa_nc < 0 || s->n_occ + 1 > m->norb)) {
         msg = "RPA screening needs rpa_nv, rpa_nc >= 0 and an empty band";
         return BSE_ERR_ARG;

@@ -120,7 +120,8 @@ __global__ __launch_bounds__(256) void k_transitions(
     const double2* __restrict__ coef, const double* __restrict__ deps,
     const double2* __restrict__ dcoef, const double* __restrict__ tau, double qhx,
     double qhy, double qhz, const double2* __restrict__ vx, double2* __restrict__ P,
-    double2* __restrict__ PV, double2* __restrict__ rho, double* __restrict__ dE)
+    double2* __restrict__ PV, double2* __restrict__ rho, double* __restrict__ dE,
+    const double2* __restrict__ dq)
 {
     const int B = nc * nv, ns = nv + nc;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -144,6 +145,24 @@ __global__ __launch_bounds__(256) void k_transitions(
         P[r * norb + i] = pi;
         const double x = tau[3 * i] * qhx + tau[3 * i + 1] * qhy + tau[3 * i + 2] * qhz;
         rh = cadd(rh, cscale(pi, x));
+    }
+    if (dq) {
+        // Wannier position matrix: rh = sum_ij conj(c_c,i) Dq_ij(k) c_v,j
+        // (with derivatives: the product rule over the two coefficient sets)
+        const double2* D = dq + (size_t)k * norb * norb;
+        const double2* dcc = dcoef ? dcoef + ((size_t)k * ns + nv + c) * norb : nullptr;
+        const double2* dcv = dcoef ? dcoef + ((size_t)k * ns + v) * norb : nullptr;
+        rh = cmk(0, 0);
+        for (int i = 0; i < norb; ++i) {
+            double2 t = cmk(0, 0), td = cmk(0, 0);
+            for (int j = 0; j < norb; ++j) {
+                const double2 d = D[i * norb + j];
+                cfma(t, d, dcoef ? dcv[j] : cv[j]);
+                if (dcoef) cfma(td, d, cv[j]);
+            }
+            rh = cadd(rh, cmulc(cc[i], t));
+            if (dcoef) rh = cadd(rh, cmulc(dcc[i], td));
+        }
     }
     rho[r] = cmk(-rh.y, rh.x);   // i * rh
     for (int j = 0; j < norb; ++j) {
@@ -190,11 +209,11 @@ extern "C" hipError_t bse_launch_transitions(hipStream_t st, int nk, int nv, int
                                              const double* deps, const double2* dcoef,
                                              const double* tau, const double* qhat,
                                              const double2* vx, double2* P, double2* PV,
-                                             double2* rho, double* dE)
+                                             double2* rho, double* dE, const double2* dq)
 {
     const int64_t n = (int64_t)nk * nv * nc;
     hipLaunchKernelGGL(k_transitions, dim3((unsigned)ceil_div64(n, 256)), dim3(256), 0, st, nk,
                        nv, nc, norb, eps, coef, deps, dcoef, tau, qhat[0], qhat[1], qhat[2], vx,
-                       P, PV, rho, dE);
+                       P, PV, rho, dE, dq);
     return hipGetLastError();
 }

@@ -44,6 +44,11 @@ class BSESystem:
     screening: str = "model"        # "model" (r0, eps_env) or "rpa": static RPA from the TB states
     rpa_nv: int = 0                 # RPA screening window: highest occupied bands (0: all)
     rpa_nc: int = 0                 # lowest empty bands (0: all)
+    # optical dipole from a Wannier position table (wannier.position_table):
+    # Cartesian lattice vectors (nR, 3) and D_ij(R) = int w_i(r - R) r w_j(r)
+    # (nR, norb, norb, 3); None: point orbitals (tau_i)
+    dipole_r: np.ndarray = None
+    dipole_d: np.ndarray = None
 
     # ---- derived sizes -------------------------------------------------
     @property

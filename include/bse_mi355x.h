@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define BSE_ABI_VERSION 2
+#define BSE_ABI_VERSION 3
 
 /* status codes */
 #define BSE_OK               0
@@ -70,7 +70,8 @@ typedef struct {
 } bse_tb_model;
 
 /* One excitonic problem (q = 0 TDA).  Fields after dipole_dir are new in
- * ABI version 2; zero-initialised they give a 2D system with model
+ * ABI version 2 (dipole_model .. dipole_d: version 3); zero-initialised they
+ * give a 2D system with model
  * screening. */
 typedef struct {
     double bravais[9];     /* column-major 3x3, column i = a_i (93-96)        */
@@ -108,7 +109,35 @@ typedef struct {
                               W = eps^-1 v, 2575-2584)                        */
     int32_t rpa_nv, rpa_nc;/* RPA screening window: highest occupied / lowest
                               empty bands (0: all)                           */
+    int32_t dipole_model;  /* 0: point orbitals, M_ii = i tau_i.qhat (1459-1501
+                              in the small-q limit); 1: the Wannier position
+                              matrix below, the real-space route's small-q
+                              dipole (1535-1570, 1609-1650):
+                              rho = i sum_ij conj(c_ck,i) Dq_ij(k) c_v,k-q,j,
+                              Dq_ij(k) = sum_R e^{-ik.R} qhat.D_ij(R)         */
+    int32_t dipole_nr;     /* lattice vectors of the position table          */
+    const double* dipole_r;/* dipole_nr x 3 Cartesian R, Angstrom            */
+    const double* dipole_d;/* real [R][i][j][3]: D_ij(R) = int w_i(r - R) r
+                              w_j(r) dr (bse_wannier_integrals), norb x norb
+                              per R in the model's orbital order              */
 } bse_system;
+
+/* Real-space Wannier functions on the grid of a supercell (Real_space_wannier,
+ * bse.version13.cpp:1081-1212): n_sc[a] unit cells along a_a, p[a] points per
+ * cell; point (cell c, sub-point s) lies at origin + sum_a (c_a + s_a/p_a) a_a.
+ * Values: w[(spin*nw + f)*Npts + cell*P + point], cell = (c0 n1 + c1) n2 + c2,
+ * point = (s0 p1 + s1) p2 + s2, P = p0 p1 p2 (the reference's
+ * real_space_wannier_functions_list; the xsf reader in wannier.py fills it). */
+typedef struct {
+    int32_t nw;            /* Wannier functions per spin block                */
+    int32_t nspin;         /* 1, or 2 (collinear: up block, then down block)  */
+    int32_t n_sc[3];       /* unit cells of the supercell along a1, a2, a3    */
+    int32_t p[3];          /* grid points per unit cell, at most 256 per
+                              supercell axis (n_sc[a] p[a] <= 256)           */
+    double origin[3];      /* Cartesian position of grid point 0, Angstrom    */
+    double axis[9];        /* column-major, column a = unit-cell vector a_a   */
+    const double* w;       /* host values, layout above                       */
+} bse_wannier_grid;
 
 /* ---- context ------------------------------------------------------------- */
 int  bse_create(bse_ctx** ctx, int device);
@@ -159,6 +188,20 @@ int  bse_tb_states(bse_ctx* ctx, double* eps, double* coef);
  * Used for band structures (pull_bands, 823-875) as well. */
 int  bse_tb_states_at(bse_ctx* ctx, const bse_tb_model* model, const double* kpts, int nk,
                       int n_occ, int nv, int nc, double scissor, double* eps, double* coef);
+
+/* Real-space Wannier integrals on the GPU (Dipole_Elements::
+ * function_building_real_space_wannier_dipole_ij 1503-1533 and _small_q
+ * 1535-1570; overlaps as in the orthogonality test 1165-1180), for every
+ * same-spin pair (i, j) and lattice translation R = r_cells[r] (in unit
+ * cells; the translated function's grid values come from cell c - R and are
+ * zero outside the supercell, 1441-1456):
+ *   A[s][i][j][r][q] = f sum_x w_si(x - R) e^{i K_q.x} w_sj(x)  (complex)
+ *   D[s][i][j][r][a] = f sum_x w_si(x - R) x_a w_sj(x)
+ *   S[s][i][j][r]    = f sum_x w_si(x - R) w_sj(x)
+ * f = Omega_supercell / Npts; kvec: n_k x 3 Cartesian 1/A (n_k may be 0,
+ * then A may be null).  Host arrays in and out.                             */
+int  bse_wannier_integrals(bse_ctx* ctx, const bse_wannier_grid* grid, int n_r, const int32_t* r_cells,
+                           int n_k, const double* kvec, double* A, double* D, double* S);
 
 /* K3+K4: assemble the TDA matrix on the device in the given precision.
  * Replaces Excitonic_Hamiltonian::extract_hbse_and_rcv (3041-3047), i.e.

@@ -123,6 +123,8 @@ class Problem:
     screening: str = "model"  # "model": W of the dimension's model; "rpa": static RPA from the TB states
     rpa_nv: int = 0          # screening window (0: every occupied / every empty band)
     rpa_nc: int = 0
+    dipole_r: np.ndarray = None   # Wannier position table: Cartesian R (nR, 3) ...
+    dipole_d: np.ndarray = None   # ... and D_ij(R) = int w_i(r - R) r w_j(r) (nR, norb, norb, 3)
 
     @property
     def nk(self):
@@ -168,7 +170,8 @@ def problem_from_system(s) -> Problem:
                    q=tuple(s.q_opt), eta=s.eta, omegas=s.omegas(),
                    dimension=getattr(s, "dimension", 2), n3=getattr(g, "n3", 1), shift=sh,
                    radius=getattr(s, "radius", 0.0), screening=getattr(s, "screening", "model"),
-                   rpa_nv=getattr(s, "rpa_nv", 0), rpa_nc=getattr(s, "rpa_nc", 0))
+                   rpa_nv=getattr(s, "rpa_nv", 0), rpa_nc=getattr(s, "rpa_nc", 0),
+                   dipole_r=getattr(s, "dipole_r", None), dipole_d=getattr(s, "dipole_d", None))
 
 
 # ----------------------------------------------------------------------------
@@ -478,11 +481,21 @@ def dipoles(p: Problem, bands=None, direction=(1.0, 0.0, 0.0)):
     """rho_r(G0) = i sum_i conj(c_{ck,i}) (tau_i . qhat) c_{v,k-q,i}.
 
     Point-orbital small-q limit of the reference's dipole route (1535-1570,
-    1614-1633): M_ii = i tau_i . qhat, independent of G.
+    1614-1633): M_ii = i tau_i . qhat, independent of G.  With a Wannier
+    position table (p.dipole_r, p.dipole_d) the small-q M of the real-space
+    route: rho = i sum_ij conj(c_{ck,i}) Dq_ij(k) c_{v,k-q,j}, Dq_ij(k) =
+    sum_R e^{-ik.R} qhat.D_ij(R) (the Bloch sum of <w_iR| r |w_j0>).
     """
     eps_c, C, eps_v, V = band_window(p) if bands is None else bands
     qh = np.asarray(direction, dtype=np.float64)
     qh = qh / np.linalg.norm(qh)
+    if p.dipole_d is not None:
+        Rc = np.asarray(p.dipole_r, dtype=float).reshape(-1, 3)
+        Dr = np.asarray(p.dipole_d, dtype=float) @ qh                     # (nR, norb, norb)
+        ph = np.exp(-1j * p.kpoints().T @ Rc.T)                            # (nk, nR)
+        Dq = np.einsum("kr,rij->kij", ph, Dr)
+        rho = 1j * np.einsum("kci,kij,kvj->cvk", C.conj(), Dq, V)
+        return rho.ravel()
     x = p.tau @ qh
     rho = 1j * np.einsum("kci,i,kvi->cvk", C.conj(), x, V)
     return rho.ravel()

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version():
     lib = _abi.load_library()
-    assert lib.bse_abi_version() == 2
+    assert lib.bse_abi_version() == 3
 
 
 def test_create_without_gpu_fails_cleanly():
