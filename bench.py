@@ -7,20 +7,28 @@ model inputs already resident on the device: per-k TB eig, W/V tables,
 assembly of the N x N TDA matrix, and its eigendecomposition.
 
   mixed path (the timed metric): compensated BF16 (hi+lo) MFMA assembly with
-  FP32 accumulation and storage, Householder reduction with FP64 reflectors,
-  all eigenvalues, the lowest 16 eigenvectors back-transformed in FP64 and
-  refined by FP64 Rayleigh-Ritz against the matrix recomputed on the fly,
-  oscillator strengths of all excitons;
-  FP64 path (reported beside it): FP64 assembly and FP64 reduction with all
-  N eigenvectors formed.
+  FP32 accumulation and storage; two-stage Hermitian reduction (dense to band
+  of width 32 by CholeskyQR2 Householder panels whose block product and
+  rank-2k trailing update run on FP32 matrix cores with FP64 partial sums,
+  then band to tridiagonal by bulge chasing); all eigenvalues by bisection;
+  the lowest 16 eigenvectors by band inverse iteration, back-transformed in
+  FP64 and refined by FP64 Rayleigh-Ritz against the matrix recomputed on the
+  fly until the residuals pass; oscillator strengths of all excitons;
+  FP64 paths (reported beside it): FP64 assembly and FP64 reduction, once
+  with the lowest 16 eigenvectors and once with all N eigenvectors formed.
 
-Prints one JSON line.  The roofline object is for the dominant kernel, the
-tridiagonalisation's Hermitian matrix-vector product, whose launches are
-timed with HIP events on the library's stream; its algorithmic bytes are the
-lower triangle of every trailing block (s_e * m (m+1)/2 per launch for a
-trailing block of order m, s_e = 8 B for FP32 complex storage).
-The CPU baseline times the FP64 oracle (numpy/LAPACK zheevd) on a reduced
-k-grid of the same model and scales it to N = 20736 (eig ~ N^3, build ~ N^2).
+Prints one JSON line.  The roofline object is for the dominant kernel of the
+mixed path as the library's HIP-event profile on its own stream finds it:
+the FP32-MFMA rank-2k trailing update of the band reduction.  Its
+algorithmic flops per launch are 8 m (m + 1) b for a trailing block of order
+m and panel width b = 32: the lower triangle of A - V Z^H - Z V^H is two
+products of m (m + 1) / 2 b complex multiply-adds each, at 8 real flops per
+complex multiply-add.  The second kernel named there is the block product
+X = A V of the panels (8 m^2 b flops).  A separate entry gives the assembly kernel's
+MFMA and HBM-write rates.
+The CPU baseline times the FP64 oracle (numpy assembly, LAPACK zheevd) on
+two reduced k-grids of the same model and scales it to N = 20736 with the
+fitted eig exponent (build ~ N^2).
 """
 from __future__ import annotations
 

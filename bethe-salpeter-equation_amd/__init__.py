@@ -11,6 +11,6 @@ The directory name is not a Python identifier; load it with
 :func:`load_package` from a file path, e.g. ``importlib`` with
 ``submodule_search_locations`` (see ``__graft_entry__.py``).
 """
-from . import lattice, tight_binding, system, wannier  # noqa: F401
+from . import lattice, tight_binding, system, wannier, formats  # noqa: F401
 from .system import (BSESystem, honeycomb_plain, honeycomb_magnetic,  # noqa: F401
                      random_multiorbital, honeycomb_sweep, cubic_3d, random_cubic_3d, GRAD_PARAMS)

@@ -472,6 +472,32 @@ def assemble(p: Problem, bands=None, wt=None, vx=None):
     return H
 
 
+def assemble_reference_quirks(p: Problem, bands=None, wt=None):
+    """The matrix the reference's build stores for a spin-less system, with
+    two of its defects kept (the reference-quirks mode; the product path
+    implements the intended equations only):
+
+    * the direct row is written with '=' over the whole row after the
+      exchange was added there (2818-2825 vs 2876-2878): the exchange is
+      lost, whatever exchange_scale says;
+    * the direct row goes through Armadillo's conjugating .t() (2878): the
+      stored block is conj(K^d) (same eigenvalues, conjugated eigenvectors).
+
+    The collinear-spin form of the first defect (only the spin-flip exchange
+    blocks survive) belongs to the reference's spin-block layout, which the
+    spinor formulation here does not have; spinful problems are refused.
+    """
+    if p.f_s != 2.0:
+        raise ValueError("reference-quirks mode covers spin-less problems (f_s = 2) only")
+    eps_c, C, eps_v, V = band_window(p) if bands is None else bands
+    wt = interaction_table(p) if wt is None else wt
+    N = p.n_bse
+    H = np.conj(direct_term(p, C, C, V, V, wt).reshape(N, N))
+    de = (eps_c[:, :, None] - eps_v[:, None, :])
+    H[np.diag_indices(N)] += np.transpose(de, (1, 2, 0)).ravel()
+    return H
+
+
 def transition_energies(p: Problem, bands=None):
     eps_c, C, eps_v, V = band_window(p) if bands is None else bands
     return np.transpose(eps_c[:, :, None] - eps_v[:, None, :], (1, 2, 0)).ravel()
