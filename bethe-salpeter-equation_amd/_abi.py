@@ -468,7 +468,10 @@ class Context:
         if int(out[4]) == 1:
             f32 = int(out[1]) == 1
             tag = "FP32 MFMA" if f32 else "FP64 products"
-            return dict(kind="flops", kernel=f"band rank-2k update ({'k_her2k_mfma32' if f32 else 'her2k'}, {tag})",
+            look = int(out[8]) == 3
+            kname = ("k_her2k_part<float2, 3, 2>" if look else "k_her2k_mfma32") if f32 else "her2k"
+            part = ", tiles beyond the next panel's tile column" if look else ""
+            return dict(kind="flops", kernel=f"band rank-2k update ({kname}, {tag}{part})",
                         ms=out[0], flops=out[3], launches=int(out[2]), peak=157.3 if f32 else 78.6,
                         second=dict(kernel=f"{'k_band_hemm32' if f32 else 'k_band_hemm'} (X = A22 V, {tag})",
                                     ms=out[5], flops=out[6], launches=int(out[7])))

@@ -1230,20 +1230,22 @@ extern "C" int64_t bse_band_update_workspace(int n, int s, int b) {
 // (FP32 storage): the block product and the update both on FP32 matrix cores.
 extern "C" hipError_t bse_launch_her2k(hipStream_t, int, int, int, int, void*, const double2*,
                                        const double2*, int64_t, int, int, const double2*);
-// ev (may be null): four events recorded before and after the Hermitian block
-// product and before and after the rank-2k update, for per-launch timing.
-extern "C" hipError_t bse_band_update(hipStream_t st, int fp32, int kind, int n, int s, int k, int b, void* A,
-                                      const double2* V, const double2* Tp, double2* ws, const double2* zeros64,
-                                      hipEvent_t* ev)
+// The two-sided update's first half: Z = X T - 1/2 V (T^H V^H X T) with
+// X = A22 V.  ev (may be null): three events recorded before and after the
+// Hermitian block product and after Z.  Returns Z's address in *zout.
+extern "C" hipError_t bse_band_product(hipStream_t st, int fp32, int kind, int n, int s, int k, int b,
+                                       const void* A, const double2* V, const double2* Tp, double2* ws,
+                                       hipEvent_t* ev, double2** zout)
 {
     const int m = n - s;
-    if (m <= 0 || k <= 0) return hipSuccess;
     const int tcs = s >> 6, nT = tile_count(n) - tcs;
     double2* Xp = ws;
     double2* Gp = Xp + (int64_t)HEMM_KSPLIT * nT * 64 * BR_B;
     const int nblk = (int)ceil_div64(m, 64);
     double2* M = Gp + (int64_t)nblk * BR_B * BR_B;
     double2* Z = M + 2 * BR_B * BR_B;
+    *zout = Z;
+    if (m <= 0 || k <= 0) return hipSuccess;
     const dim3 gh((unsigned)nT, HEMM_KSPLIT);
     if (ev) (void)hipEventRecord(ev[0], st);
     if (fp32 && kind == 3) {
@@ -1262,9 +1264,21 @@ extern "C" hipError_t bse_band_update(hipStream_t st, int fp32, int kind, int n,
     hipLaunchKernelGGL(k_band_m, dim3((unsigned)k), dim3(256), 0, st, nblk, k, b, (const double2*)Gp, Tp, M);
     hipLaunchKernelGGL(k_rows_small, dim3((unsigned)ceil_div64(m, 64)), dim3(256), 0, st, m, 0, k, k, b, V,
                        (const double2*)M, Z, 1, (const int*)nullptr);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[2], st);
+    return hipGetLastError();
+}
+
+// ev (may be null): four events recorded before and after the Hermitian block
+// product and before and after the rank-2k update, for per-launch timing.
+extern "C" hipError_t bse_band_update(hipStream_t st, int fp32, int kind, int n, int s, int k, int b, void* A,
+                                      const double2* V, const double2* Tp, double2* ws, const double2* zeros64,
+                                      hipEvent_t* ev)
+{
+    const int m = n - s;
+    if (m <= 0 || k <= 0) return hipSuccess;
+    double2* Z = nullptr;
+    hipError_t e = bse_band_product(st, fp32, kind, n, s, k, b, A, V, Tp, ws, ev, &Z);
+    if (e != hipSuccess) return e;
     e = bse_launch_her2k(st, fp32, kind, n, s, A, V, Z, m, s, k, zeros64);
     if (ev) (void)hipEventRecord(ev[3], st);
     return e;

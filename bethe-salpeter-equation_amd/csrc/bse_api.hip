@@ -37,6 +37,7 @@ extern "C" int bse_create(bse_ctx** out, int device) {
     for (int i = 0; i < 3; ++i) (void)hipEventCreate(&ctx->tev[i]);
     (void)hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
     (void)hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming);
+    for (int i = 0; i < 2; ++i) (void)hipEventCreateWithFlags(&ctx->la_ev[i], hipEventDisableTiming);
     const char* at = getenv("BSE_AUTOTUNE");
     ctx->autotune = !(at && at[0] == '0');
     const char* ts = getenv("BSE_TWO_STAGE");
@@ -62,6 +63,8 @@ extern "C" int bse_destroy(bse_ctx* ctx) {
         if (ctx->tev[i]) (void)hipEventDestroy(ctx->tev[i]);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    for (int i = 0; i < 2; ++i)
+        if (ctx->la_ev[i]) (void)hipEventDestroy(ctx->la_ev[i]);
     (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->st);
     delete ctx;
@@ -94,7 +97,7 @@ extern "C" int bse_profile(bse_ctx* ctx, int enable, double* out) {
             out[5] = ctx->hemm_ms;
             out[6] = ctx->hemm_flops;
             out[7] = (double)ctx->band_launches;
-            out[8] = 2.0;
+            out[8] = ctx->band_lookahead ? 3.0 : 2.0;
         } else {
             out[0] = ctx->hemv_ms;
             out[1] = ctx->hemv_bytes;
@@ -709,6 +712,7 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
     double2* zeros = BUF(double2, S_BZERO, 64);
     HCK(bse_launch_zero_words(st, 128, zeros));
     hipEvent_t* tv = ctx->tev;
+    double2* y = nullptr;
     if (npan > 0) {
         const int m0 = n - b;
         double2* P = BUF(double2, S_BPAN, (size_t)m0 * b);
@@ -738,28 +742,60 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
             }
             pev = ctx->prof_ev.data();
         }
+        // Look-ahead (trailing-update kinds on 64 x 64 tiles): once Z of panel
+        // p is known, the side stream updates the trailing block's first tile
+        // column (the next panel's columns), factors panel p + 1 and applies
+        // its reflectors to y = Q^H rho, while st updates every other tile;
+        // st then waits for the factorisation before the next block product.
+        // BSE_BAND_LOOKAHEAD=0 keeps everything on st.
+        const char* lae = getenv("BSE_BAND_LOOKAHEAD");
+        const bool look = (kind == 0 || kind == 1 || kind == 3) && !(lae && lae[0] == '0') && npan > 1;
+        ctx->band_lookahead = look;
+        hipStream_t ss = ctx->side;
+        double2* qw = rho ? BUF(double2, S_W1, bse_apply_q_thin_workspace(n)) : nullptr;
+        y = rho ? BUF(double2, S_Y, n) : nullptr;
+        // factor panel q (and apply its reflectors to y) on stream sq
+        auto factor = [&](hipStream_t sq, int q) -> int {
+            const bse_ctx::BandPanel& bq = ctx->band_panels[q];
+            int kq = 0;
+            HCK(bse_band_gather(sq, fp32, n, q * b, b, H, P, AB, ldab));
+            HCK(bse_band_panel_qr(sq, bq.m, b, P, Vst + bq.voff, Tall + (size_t)q * b * b, tw, AB, ldab, q * b, &kq));
+            if (kq != bq.k) return fail(ctx, BSE_ERR_STATE, "band panel sizes disagree");
+            if (rho)
+                HCK(bse_apply_q_panel(sq, bq.m, bq.k, Vst + bq.voff, Tall + (size_t)q * b * b, b, 1, y + bq.s, n, 1, qw));
+            return BSE_OK;
+        };
+        if (rho) HCK(hipMemcpyAsync(y, rho, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+        if (int rc = factor(st, 0)) return rc;
         for (int p = 0; p < npan; ++p) {
             const bse_ctx::BandPanel& bp = ctx->band_panels[p];
             double2* Vp = Vst + bp.voff;
             double2* Tp = Tall + (size_t)p * b * b;
-            int k = 0;
-            HCK(bse_band_gather(st, fp32, n, p * b, b, H, P, AB, ldab));
-            HCK(bse_band_panel_qr(st, bp.m, b, P, Vp, Tp, tw, AB, ldab, p * b, &k));
-            HCK(bse_band_update(st, fp32, kind, n, bp.s, k, b, H, Vp, Tp, uw, zeros, pev ? pev + 4 * p : nullptr));
+            hipEvent_t* pe = pev ? pev + 4 * p : nullptr;
+            if (!look || p + 1 == npan) {
+                HCK(bse_band_update(st, fp32, kind, n, bp.s, bp.k, b, H, Vp, Tp, uw, zeros, pe));
+                if (p + 1 < npan)
+                    if (int rc = factor(st, p + 1)) return rc;
+                continue;
+            }
+            double2* Z = nullptr;
+            HCK(bse_band_product(st, fp32, kind, n, bp.s, bp.k, b, H, Vp, Tp, uw, pe, &Z));
+            HCK(hipEventRecord(ctx->la_ev[0], st));
+            HCK(hipStreamWaitEvent(ss, ctx->la_ev[0], 0));
+            HCK(bse_launch_her2k_part(ss, fp32, kind, 1, n, bp.s, H, Vp, Z, bp.m, bp.s, bp.k, zeros));
+            if (int rc = factor(ss, p + 1)) return rc;
+            HCK(hipEventRecord(ctx->la_ev[1], ss));
+            HCK(bse_launch_her2k_part(st, fp32, kind, 2, n, bp.s, H, Vp, Z, bp.m, bp.s, bp.k, zeros));
+            if (pe) HCK(hipEventRecord(pe[3], st));
+            HCK(hipStreamWaitEvent(st, ctx->la_ev[1], 0));
         }
     }
     if (npan == 0) HCK(hipEventRecord(tv[0], st));
     HCK(bse_band_gather_tail(st, fp32, n, npan * b, H, AB, ldab));
     HCK(bse_launch_copy_words(st, 2 * (int64_t)n * ldab, AB, AB0));
-    double2* y = nullptr;
-    if (rho) {
+    if (rho && npan == 0) {
         y = BUF(double2, S_Y, n);
-        double2* qw = BUF(double2, S_W1, bse_apply_q_thin_workspace(n));
         HCK(hipMemcpyAsync(y, rho, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
-        for (int p = 0; p < npan; ++p) {
-            const bse_ctx::BandPanel& bp = ctx->band_panels[p];
-            HCK(bse_apply_q_panel(st, bp.m, bp.k, Vst + bp.voff, Tall + (size_t)p * b * b, b, 1, y + bp.s, n, 1, qw));
-        }
     }
     HCK(hipEventRecord(tv[1], st));
     const int64_t ncnt = bse_chase_counters(n);
@@ -797,7 +833,9 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
             t2k += c;
             const double m = bp.m, k = bp.k;
             fhm += 8.0 * m * m * k;
-            f2k += 8.0 * m * (m + 1.0) * k;
+            // with the look-ahead the timed launch skips the first tile column
+            const double mr = (ctx->band_lookahead && p + 1 < npan) ? (double)std::max(0, n - 64 * ((bp.s >> 6) + 1)) : m;
+            f2k += 8.0 * mr * (mr + 1.0) * k;
         }
         ctx->hemm_ms = thm;
         ctx->hemm_flops = fhm;

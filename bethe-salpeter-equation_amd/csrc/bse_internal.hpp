@@ -94,6 +94,10 @@ hipError_t bse_apply_q_strips(hipStream_t, int, int, int, const double2*, const 
 int64_t bse_band_update_workspace(int, int, int);
 hipError_t bse_band_update(hipStream_t, int, int, int, int, int, int, void*, const double2*, const double2*,
                            double2*, const double2*, hipEvent_t*);
+hipError_t bse_band_product(hipStream_t, int, int, int, int, int, int, const void*, const double2*,
+                            const double2*, double2*, hipEvent_t*, double2**);
+hipError_t bse_launch_her2k_part(hipStream_t, int, int, int, int, int, void*, const double2*, const double2*,
+                                 int64_t, int, int, const double2*);
 int64_t bse_chase_counters(int);
 hipError_t bse_launch_chase(hipStream_t, int, int, double2*, int, double2*, double*, double*, int*, int*, int);
 int64_t bse_band_ii_workspace(int, int, int);
@@ -203,6 +207,7 @@ struct bse_ctx {
     int64_t band_launches = 0;
     bool prof_two_stage = false;
     bool band_fp32prod = false;     // last band reduction ran on FP32 matrix cores
+    bool band_lookahead = false;    // ... and updated each next panel's tile column first
     bool force_valu_assembly = false;   // BSE_ASSEMBLY=valu selects the VALU tiles
     int asm_kernel = 0;                 // 0 VALU, 1 FP64 MFMA, 2 FP32 MFMA, 3 BF16x2 MFMA
     int64_t hemv_launches = 0;
@@ -218,6 +223,9 @@ struct bse_ctx {
     // sorts the tridiagonal blocks and the bisection runs on st
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // band reduction look-ahead: product done (st -> side), next panel
+    // factored (side -> st)
+    hipEvent_t la_ev[2] = {};
     // caller's k order (bse_set_problem_klist): list position of each grid
     // index and its inverse; empty for the plain grid
     std::vector<int> kmap_h, kinv_h;

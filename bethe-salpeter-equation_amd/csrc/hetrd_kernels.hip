@@ -1525,3 +1525,55 @@ extern "C" hipError_t bse_launch_her2k(hipStream_t st, int fp32, int kind, int n
 #undef KH
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// The trailing update in two parts, for the band reduction's look-ahead:
+// PART 1 updates the first tile column of the trailing block A[s:, s:] (it
+// holds the next panel's columns, so the next panel can be factored while
+// PART 2 updates every other lower tile).  Lower 64 x 64 tiles only (kinds
+// 0, 1 and 3); block x of PART 1 is tile (x, 0), of PART 2 tile (a+1, c+1)
+// for the x-th lower tile (a, c) of the remaining order.
+// ---------------------------------------------------------------------------
+__device__ inline int64_t her2k_part_idx(int part, int64_t x) {
+    if (part == 1) return x * (x + 1) / 2;
+    int a, c;
+    tri_tile(x, a, c);
+    return (int64_t)(a + 1) * (a + 2) / 2 + c + 1;
+}
+
+template <typename T, int KIND, int PART>
+__global__ __launch_bounds__(256) void k_her2k_part(int n, int s, T* __restrict__ A, const double2* __restrict__ V,
+                                                   const double2* __restrict__ W, int64_t ldv, int j0, int nb,
+                                                   const double2* __restrict__ alpha2)
+{
+    const int64_t idx = her2k_part_idx(PART, blockIdx.x);
+    if constexpr (KIND == 3) her2k_mfma_body<float2, her2k_f32>(n, s, (float2*)A, V, W, ldv, j0, nb, alpha2, idx);
+    else if constexpr (KIND == 1) her2k_valu_body<T>(n, s, A, V, W, ldv, j0, nb, alpha2, idx);
+    else her2k_mfma_body<T>(n, s, A, V, W, ldv, j0, nb, alpha2, idx);
+}
+
+extern "C" hipError_t bse_launch_her2k_part(hipStream_t st, int fp32, int kind, int part, int n, int s, void* A,
+                                            const double2* V, const double2* W, int64_t ldv, int j0, int nb,
+                                            const double2* alpha2)
+{
+    if (s >= n) return hipSuccess;
+    if (s < j0 || ldv < n - j0 || (kind != 0 && kind != 1 && kind != 3) || (kind == 3 && !fp32) ||
+        (part != 1 && part != 2))
+        return hipErrorInvalidValue;
+    const int64_t nT = tile_count(n) - (s >> 6);
+    const int64_t nblk = part == 1 ? nT : (nT - 1) * nT / 2;
+    if (nblk <= 0) return hipSuccess;
+    const dim3 g((unsigned)nblk);
+#define KP(TT, K, P) hipLaunchKernelGGL((k_her2k_part<TT, K, P>), g, dim3(256), 0, st, n, s, (TT*)A, V, W, ldv, j0, nb, alpha2)
+    if (kind == 3) {
+        if (part == 1) KP(float2, 3, 1); else KP(float2, 3, 2);
+    } else if (fp32) {
+        if (kind == 1) { if (part == 1) KP(float2, 1, 1); else KP(float2, 1, 2); }
+        else { if (part == 1) KP(float2, 0, 1); else KP(float2, 0, 2); }
+    } else {
+        if (kind == 1) { if (part == 1) KP(double2, 1, 1); else KP(double2, 1, 2); }
+        else { if (part == 1) KP(double2, 0, 1); else KP(double2, 0, 2); }
+    }
+#undef KP
+    return hipGetLastError();
+}

@@ -290,10 +290,13 @@ int  bse_stage_times(const bse_ctx* ctx, double* ms, int n);
  *   one-stage reduction: [0] total time in ms of the Hermitian matrix-vector
  *     kernel, [1] its algorithmic bytes (lower triangle of every trailing
  *     block), [2] launches, [4] = 0;
- *   two-stage reduction: [0] ms, [2] launches, [3] algorithmic flops of the
- *     panels' rank-2k update (8 m (m+1) k per panel), [4] = 1, and [5] ms,
- *     [6] flops (8 m^2 k), [7] launches of the Hermitian block product,
- *     [8] = 2. */
+ *   two-stage reduction: [0] ms, [1] = 1 when the products ran on FP32
+ *     matrix cores, [2] launches, [3] algorithmic flops of the panels'
+ *     rank-2k update (8 m (m+1) k per panel), [4] = 1, and [5] ms, [6] flops
+ *     (8 m^2 k), [7] launches of the Hermitian block product, [8] = 2; [8] = 3
+ *     when the look-ahead updated each next panel's tile column on the second
+ *     stream, and [0], [3] then cover the other tiles (8 m_r (m_r+1) k, m_r
+ *     the order beyond that tile column). */
 int  bse_profile(bse_ctx* ctx, int enable, double* out);
 
 /* ---- K11: batched sweep of many small systems ------------------------------

@@ -141,3 +141,41 @@ def test_two_stage_matches_one_stage_mixed(ctx1, ctx2):
     scale = np.abs(E1).max()
     assert np.abs(E1[:16] - E2[:16]).max() < 1e-10 * scale
     assert np.abs(E1 - E2).max() < 1e-6 * scale
+
+
+def test_two_stage_structured_fp32(ctx2):
+    """Zero panels in FP32 storage: the CholeskyQR2 path flags them and the
+    TSQR fallback runs inside the look-ahead's side stream."""
+    n = 300
+    i = np.arange(n)
+    lap = np.zeros((n, n), dtype=np.complex128)
+    lap[i, i] = 2.0
+    lap[i[:-1], i[:-1] + 1] = -1.0
+    lap[i[:-1] + 1, i[:-1]] = -1.0
+    ev, z = ctx2.eigh_dense(lap, "fp32", 4)
+    ref = 2.0 - 2.0 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1))
+    assert np.abs(ev - ref).max() < 1e-5 * 4
+    assert np.abs(z.conj().T @ z - np.eye(4)).max() < 1e-10
+    dg = np.diag(np.linspace(-3, 5, n)).astype(np.complex128)
+    ev, _ = ctx2.eigh_dense(dg, "fp32", 3)
+    assert np.abs(ev - np.linspace(-3, 5, n)).max() < 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_two_stage_lookahead_is_exact(ctx2, prec):
+    """The look-ahead splits each trailing update into the next panel's tile
+    column (second stream) and the other tiles: the same arithmetic, so the
+    results equal those of the single-stream order bit for bit."""
+    a = _herm(1100, 5)
+    old = os.environ.get("BSE_BAND_LOOKAHEAD")
+    os.environ["BSE_BAND_LOOKAHEAD"] = "0"
+    try:
+        ev0, z0 = ctx2.eigh_dense(a, prec, 8)
+    finally:
+        if old is None:
+            del os.environ["BSE_BAND_LOOKAHEAD"]
+        else:
+            os.environ["BSE_BAND_LOOKAHEAD"] = old
+    ev1, z1 = ctx2.eigh_dense(a, prec, 8)
+    assert np.array_equal(ev0, ev1)
+    assert np.array_equal(z0, z1)
