@@ -404,20 +404,16 @@ __device__ inline void lds_unit_lower_inv(hh_sq& L, hh_sq& X, int k)
         lds_wave_sync();
     }
 }
+// The LU runs in place in W: the multipliers (L1 below the unit diagonal)
+// replace W's strict lower part, U (diagonal u_jj, upper -d_c W(j, c)) its
+// upper part, so W, Ui and Li are the only LDS matrices (the kernels that
+// call this fit beside a running trailing update).
 __device__ __attribute__((always_inline)) inline void hh_reconstruct_lds(
-    int m, int b, int k, hh_sq& W, hh_sq& L, hh_sq& U, hh_sq& Ui, hh_sq& Li, double2* d,
+    int m, int b, int k, hh_sq& W, hh_sq& Ui, hh_sq& Li, double2* d, double2* ud,
     const double2* __restrict__ Rfin, double2* __restrict__ V, double2* __restrict__ Tp,
     double2* __restrict__ Mlow, double2* __restrict__ AB, int ldab, int c0)
 {
     const int tid = threadIdx.x;
-    for (int e = tid; e < BR_B * BR_B; e += 256) {
-        const int i = e % BR_B, c = e / BR_B;
-        L[i][c] = cmk(i == c ? 1.0 : 0.0, 0.0);
-        U[i][c] = cmk(0, 0);
-        Ui[i][c] = cmk(0, 0);
-        Li[i][c] = cmk(0, 0);
-    }
-    __syncthreads();
     for (int j = 0; j < k; ++j) {
         const double2 wj = W[j][j];
         const double aw = sqrt(cabs2(wj));
@@ -425,34 +421,37 @@ __device__ __attribute__((always_inline)) inline void hh_reconstruct_lds(
         const double2 ujj = csub(cmk(1.0, 0.0), cmul(dj, wj));
         if (tid == 0) {
             d[j] = dj;
-            U[j][j] = ujj;
+            ud[j] = ujj;
         }
-        // multipliers of column j, then the Schur update of the later columns
-        if (tid > j && tid < k) L[tid][j] = cdiv(cmul(cmk(-dj.x, -dj.y), W[tid][j]), ujj);
+        // multipliers of column j (below the diagonal), then the Schur
+        // update of the later columns
+        if (tid > j && tid < k) W[tid][j] = cdiv(cmul(cmk(-dj.x, -dj.y), W[tid][j]), ujj);
         __syncthreads();
         for (int e = tid; e < BR_B * BR_B; e += 256) {
             const int i = e % BR_B, c = e / BR_B;
-            if (i > j && i < k && c > j && c < k) W[i][c] = csub(W[i][c], cmul(L[i][j], W[j][c]));
+            if (i > j && i < k && c > j && c < k) W[i][c] = csub(W[i][c], cmul(W[i][j], W[j][c]));
         }
         __syncthreads();
     }
+    // U on and above the diagonal
     for (int e = tid; e < BR_B * BR_B; e += 256) {
         const int j = e % BR_B, c = e / BR_B;
-        if (j < c && c < k) U[j][c] = cmul(cmk(-d[c].x, -d[c].y), W[j][c]);
+        if (j < c && c < k) W[j][c] = cmul(cmk(-d[c].x, -d[c].y), W[j][c]);
+        else if (j == c && c < k) W[j][j] = ud[j];
     }
     __syncthreads();
     // U^{-1} (upper, wave 0) and L1^{-1} (unit lower, wave 1), by rows
-    if (tid < 64) lds_upper_inv(U, false, Ui, k);
-    else if (tid < 128) lds_unit_lower_inv(L, Li, k);
+    if (tid < 64) lds_upper_inv(W, false, Ui, k);
+    else if (tid < 128) lds_unit_lower_inv(W, Li, k);
     __syncthreads();
     for (int e = tid; e < BR_B * BR_B; e += 256) {
         const int i = e % BR_B, c = e / BR_B;
         if (i < k && c < k) {
             // V rows 0..k-1 = L1
-            V[i + (int64_t)c * m] = L[i][c];
-            // T = U L1^{-H}:  T(i, c) = sum_l U(i, l) conj(Li(c, l))
+            V[i + (int64_t)c * m] = (i == c) ? cmk(1.0, 0.0) : (i > c ? W[i][c] : cmk(0, 0));
+            // T = U L1^{-H}:  T(i, c) = sum_{l >= i} U(i, l) conj(Li(c, l))
             double2 t = cmk(0, 0);
-            for (int l = 0; l < k; ++l) cfma(t, U[i][l], cconj(Li[c][l]));
+            for (int l = i; l < k; ++l) cfma(t, W[i][l], cconj(Li[c][l]));
             Tp[i + c * b] = t;
             // Mlow = -D U^{-1}: (-Q2 D) U^{-1} = Q2 (-D U^{-1})
             if (Mlow) Mlow[i + c * b] = cmul(cmk(-d[i].x, -d[i].y), Ui[i][c]);
@@ -480,15 +479,15 @@ __global__ __launch_bounds__(256) void k_hh_reconstruct(int m, int b, int k, con
                                                         double2* __restrict__ Mlow, double2* __restrict__ AB,
                                                         int ldab, int c0, const int* __restrict__ gate)
 {
-    __shared__ hh_sq W, L, U, Ui, Li;
-    __shared__ double2 d[BR_B];
+    __shared__ hh_sq W, Ui, Li;
+    __shared__ double2 d[BR_B], ud[BR_B];
     if (gate_closed(gate)) return;
     for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
         const int i = e % BR_B, c = e / BR_B;
         W[i][c] = (i < k && c < k) ? Q[i + (int64_t)c * m] : cmk(0, 0);
     }
     __syncthreads();
-    hh_reconstruct_lds(m, b, k, W, L, U, Ui, Li, d, Rfin, V, Tp, Mlow, AB, ldab, c0);
+    hh_reconstruct_lds(m, b, k, W, Ui, Li, d, ud, Rfin, V, Tp, Mlow, AB, ldab, c0);
 }
 
 // Y (rows k0 .. m-1, ld m) = X M (mode 0) or Y -= X M (mode 1), X (ld m) with
@@ -657,13 +656,16 @@ __device__ void cq_rinv(hh_sq& sL, hh_sq& Ri)
 // First factorisation: R1 (upper, ld b) and R1^{-1} to global.
 __global__ __launch_bounds__(256) void k_cq_chol1(int ng, const double2* __restrict__ Gp,
                                                   double2* __restrict__ R1, double2* __restrict__ R1i,
-                                                  int* __restrict__ flag)
+                                                  int* __restrict__ flag, int* __restrict__ fail)
 {
     __shared__ hh_sq sG, Ri;
     __shared__ double diag0[BR_B];
     __shared__ int fl;
     if (!cq_sum_chol(ng, Gp, 0, sG, diag0, &fl)) {
-        if (threadIdx.x == 0) flag[0] = 1;
+        if (threadIdx.x == 0) {
+            flag[0] = 1;
+            *fail = 1;
+        }
         return;
     }
     cq_rinv(sG, Ri);
@@ -683,61 +685,53 @@ __global__ __launch_bounds__(256) void k_cq_chol2(int m, int b, int ng, const do
                                                   double2* __restrict__ Rf, double2* __restrict__ V,
                                                   double2* __restrict__ Tp, double2* __restrict__ Mp,
                                                   double2* __restrict__ AB, int ldab, int c0,
-                                                  int* __restrict__ flag)
+                                                  int* __restrict__ flag, int* __restrict__ fail)
 {
-    __shared__ hh_sq W, L, U, Ui, Li;
-    __shared__ double2 d[BR_B];
+    __shared__ hh_sq W, Ui, Li;
+    __shared__ double2 d[BR_B], ud[BR_B];
     __shared__ double diag0[BR_B];
     __shared__ int fl;
     if (flag[1] == 0) return;
-    // Cholesky of G2 in W, R2^{-1} in L
+    // Cholesky of G2 in W, R2^{-1} in Li
     if (!cq_sum_chol(ng, Gp, 1, W, diag0, &fl)) {
-        if (threadIdx.x == 0) flag[0] = 1;
+        if (threadIdx.x == 0) {
+            flag[0] = 1;
+            *fail = 1;
+        }
         return;
     }
-    cq_rinv(W, L);
-    // R1 and the top 32 x 32 block of Q1 staged in LDS (Ui, Li), one batch
-    // of loads
-    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
-        const int i = e % BR_B, j = e / BR_B;
-        Ui[i][j] = R1[e];
-        Li[i][j] = Q1[i + (int64_t)j * m];
-    }
+    cq_rinv(W, Li);
+    for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) Ui[e % BR_B][e / BR_B] = R1[e];
     __syncthreads();
-    // R = R2 R1 (R2(i, l) = conj(W(l, i))), into Rf; both upper; the top
-    // block of Q = Q1 R2^{-1} into U
+    // R = R2 R1 (both upper; R2(i, l) = conj(W(l, i))) into Rf
     for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
         const int i = e % BR_B, j = e / BR_B;
-        double2 s = cmk(0, 0), t = cmk(0, 0);
-#pragma unroll 8
-        for (int l = 0; l < BR_B; ++l) {
-            const double2 r2 = (l >= i && l <= j) ? cconj(W[l][i]) : cmk(0, 0);
-            const double2 r2i = (l <= j) ? L[l][j] : cmk(0, 0);
-            cfma(s, r2, Ui[l][j]);
-            cfma(t, Li[i][l], r2i);
-        }
+        double2 s = cmk(0, 0);
+        for (int l = i; l <= j; ++l) cfma(s, cconj(W[l][i]), Ui[l][j]);
         Rf[e] = (i <= j) ? s : cmk(0, 0);
-        U[i][j] = t;
     }
     __syncthreads();
+    // the top 32 x 32 block of Q1 into Ui, then the top block of
+    // Q = Q1 R2^{-1} into W; R2^{-1} is kept in registers (4 entries per
+    // thread) while the reconstruction uses Li
     for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
         const int i = e % BR_B, j = e / BR_B;
-        Li[i][j] = L[i][j];       // keep R2^{-1}
-        W[i][j] = U[i][j];
+        Ui[i][j] = Q1[i + (int64_t)j * m];
     }
     __syncthreads();
-    // R2^{-1} survives in Ui's storage only after the reconstruction, so it
-    // is kept in registers: 4 entries per thread
     double2 r2i[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        const int e = threadIdx.x + 256 * u;
-        r2i[u] = Li[e % BR_B][e / BR_B];
+        const int e = threadIdx.x + 256 * u, i = e % BR_B, j = e / BR_B;
+        double2 t = cmk(0, 0);
+        for (int l = 0; l <= j; ++l) cfma(t, Ui[i][l], Li[l][j]);
+        W[i][j] = t;
+        r2i[u] = Li[i][j];
     }
     __syncthreads();
-    hh_reconstruct_lds(m, b, BR_B, W, L, U, Ui, Li, d, Rf, V, Tp, nullptr, AB, ldab, c0);
+    hh_reconstruct_lds(m, b, BR_B, W, Ui, Li, d, ud, Rf, V, Tp, nullptr, AB, ldab, c0);
     __syncthreads();
-    // Mlow = -D U^{-1} into W, R2^{-1} back into L, then Mp = R2^{-1} Mlow
+    // Mlow = -D U^{-1} into W, R2^{-1} back into Li, then Mp = R2^{-1} Mlow
     for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
         const int i = e % BR_B, j = e / BR_B;
         W[i][j] = cmul(cmk(-d[i].x, -d[i].y), Ui[i][j]);
@@ -745,13 +739,13 @@ __global__ __launch_bounds__(256) void k_cq_chol2(int m, int b, int ng, const do
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const int e = threadIdx.x + 256 * u;
-        L[e % BR_B][e / BR_B] = r2i[u];
+        Li[e % BR_B][e / BR_B] = r2i[u];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < BR_B * BR_B; e += 256) {
         const int i = e % BR_B, j = e / BR_B;
         double2 s = cmk(0, 0);
-        for (int l = i; l < BR_B; ++l) cfma(s, L[i][l], W[l][j]);
+        for (int l = i; l < BR_B; ++l) cfma(s, Li[i][l], W[l][j]);
         Mp[i + j * b] = s;
     }
     if (threadIdx.x == 0) flag[2] = 1;
@@ -1114,10 +1108,15 @@ extern "C" int64_t bse_tsqr_workspace(int m, int b) {
 // QR of the m x b panel P (ld m) as I - V T V^H: V (m x k) into Vout (ld m),
 // T into Tp (ld b), R' = D^H R into the band store at (c0 + b.., c0..).
 // Returns k = min(m, b) in *kout.  Panels of 32 columns and at least 64
-// rows take the CholeskyQR2 path; the TSQR kernels run after it and return
-// at once unless it flagged the panel (so no host round trip decides).
+// rows take the CholeskyQR2 path.  With tsqr != 0 the TSQR kernels run
+// after it and return at once unless it flagged the panel (no host round
+// trip decides); with tsqr == 0 they are not launched, and a panel whose
+// Gram matrix is not numerically positive definite sets *fail (never
+// cleared here) and leaves V, T and R unwritten: the caller then repeats the
+// whole reduction with tsqr = 1.  Other panels always take the TSQR.
 extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P, double2* Vout,
-                                        double2* Tp, double2* ws, double2* AB, int ldab, int c0, int* kout)
+                                        double2* Tp, double2* ws, double2* AB, int ldab, int c0, int* kout,
+                                        int* fail, int tsqr)
 {
     if (b > BR_B || m < 1) return hipErrorInvalidValue;
     // the largest in-LDS QR holds TSQR_MAXROWS x 32 complex doubles (128 KB);
@@ -1171,17 +1170,18 @@ extern "C" hipError_t bse_band_panel_qr(hipStream_t st, int m, int b, double2* P
         hipLaunchKernelGGL(k_cq_gram<0>, dim3((unsigned)g), dim3(256), 0, st, m, (const double2*)P, Gp, flags);
         hipLaunchKernelGGL(k_cq_reduce, dim3(BR_B * BR_B / 256), dim3(256), 0, st, g, (const double2*)Gp, Gs,
                            (const int*)flags, -1);
-        hipLaunchKernelGGL(k_cq_chol1, dim3(1), dim3(256), 0, st, 1, (const double2*)Gs, R1, R1i, flags);
+        hipLaunchKernelGGL(k_cq_chol1, dim3(1), dim3(256), 0, st, 1, (const double2*)Gs, R1, R1i, flags, fail);
         hipLaunchKernelGGL(k_rows_small, dim3((unsigned)ceil_div64(m, 64)), dim3(256), 0, st, m, 0, BR_B, BR_B, BR_B,
                            (const double2*)P, (const double2*)R1i, Q, 0, (const int*)(flags + 1));
         hipLaunchKernelGGL(k_cq_gram<1>, dim3((unsigned)g), dim3(256), 0, st, m, (const double2*)Q, Gp, flags);
         hipLaunchKernelGGL(k_cq_reduce, dim3(BR_B * BR_B / 256), dim3(256), 0, st, g, (const double2*)Gp, Gs,
                            (const int*)flags, 1);
         hipLaunchKernelGGL(k_cq_chol2, dim3(1), dim3(256), 0, st, m, b, 1, (const double2*)Gs,
-                           (const double2*)Q, (const double2*)R1, Rf, Vout, Tp, Mp, AB, ldab, c0, flags);
+                           (const double2*)Q, (const double2*)R1, Rf, Vout, Tp, Mp, AB, ldab, c0, flags, fail);
         hipLaunchKernelGGL(k_rows_small, dim3((unsigned)ceil_div64(m - k, 64)), dim3(256), 0, st, m, k, BR_B, k, b,
                            (const double2*)Q, (const double2*)Mp, Vout, 0, (const int*)(flags + 2));
         gate = flags;
+        if (!tsqr) return hipGetLastError();
     }
     // TSQR: leaf QRs
     const int maxrows_leaf = (nleaf == 1) ? m : 2 * TSQR_RB - 1;

@@ -754,40 +754,78 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
         hipStream_t ss = ctx->side;
         double2* qw = rho ? BUF(double2, S_W1, bse_apply_q_thin_workspace(n)) : nullptr;
         y = rho ? BUF(double2, S_Y, n) : nullptr;
-        // factor panel q (and apply its reflectors to y) on stream sq
+        // First attempt: CholeskyQR2 panels only.  A panel whose Gram matrix is
+        // not numerically positive definite sets *fail; the matrix, saved
+        // before, is then restored and the reduction repeated with the TSQR
+        // kernels behind every CholeskyQR2 panel (flag-gated, so only the
+        // flagged panels run them).  One host read of *fail decides.
+        int* qr_fail = BUF(int, S_BFAIL, 4);
+        const size_t hbytes = (size_t)tiled_elems(n) * (fp32 ? sizeof(float2) : sizeof(double2));
+        void* H0 = BUF(char, S_BH0, hbytes);
+        HCK(hipMemcpyAsync(H0, H, hbytes, hipMemcpyDeviceToDevice, st));
+        int tsqr = 0;
+        // factor panel q on stream sq; apply its reflectors to y = Q^H rho
         auto factor = [&](hipStream_t sq, int q) -> int {
             const bse_ctx::BandPanel& bq = ctx->band_panels[q];
             int kq = 0;
             HCK(bse_band_gather(sq, fp32, n, q * b, b, H, P, AB, ldab));
-            HCK(bse_band_panel_qr(sq, bq.m, b, P, Vst + bq.voff, Tall + (size_t)q * b * b, tw, AB, ldab, q * b, &kq));
+            HCK(bse_band_panel_qr(sq, bq.m, b, P, Vst + bq.voff, Tall + (size_t)q * b * b, tw, AB, ldab, q * b, &kq,
+                                  qr_fail, tsqr));
             if (kq != bq.k) return fail(ctx, BSE_ERR_STATE, "band panel sizes disagree");
+            return BSE_OK;
+        };
+        auto apply_y = [&](hipStream_t sq, int q) -> int {
+            const bse_ctx::BandPanel& bq = ctx->band_panels[q];
             if (rho)
                 HCK(bse_apply_q_panel(sq, bq.m, bq.k, Vst + bq.voff, Tall + (size_t)q * b * b, b, 1, y + bq.s, n, 1, qw));
             return BSE_OK;
         };
-        if (rho) HCK(hipMemcpyAsync(y, rho, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
-        if (int rc = factor(st, 0)) return rc;
-        for (int p = 0; p < npan; ++p) {
-            const bse_ctx::BandPanel& bp = ctx->band_panels[p];
-            double2* Vp = Vst + bp.voff;
-            double2* Tp = Tall + (size_t)p * b * b;
-            hipEvent_t* pe = pev ? pev + 4 * p : nullptr;
-            if (!look || p + 1 == npan) {
-                HCK(bse_band_update(st, fp32, kind, n, bp.s, bp.k, b, H, Vp, Tp, uw, zeros, pe));
-                if (p + 1 < npan)
-                    if (int rc = factor(st, p + 1)) return rc;
-                continue;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            tsqr = attempt;
+            if (attempt == 1) {
+                HCK(hipMemcpyAsync(H, H0, hbytes, hipMemcpyDeviceToDevice, st));
+                HCK(bse_launch_zero_words(st, 2 * (int64_t)n * ldab, AB));
             }
-            double2* Z = nullptr;
-            HCK(bse_band_product(st, fp32, kind, n, bp.s, bp.k, b, H, Vp, Tp, uw, pe, &Z));
-            HCK(hipEventRecord(ctx->la_ev[0], st));
-            HCK(hipStreamWaitEvent(ss, ctx->la_ev[0], 0));
-            HCK(bse_launch_her2k_part(ss, fp32, kind, 1, n, bp.s, H, Vp, Z, bp.m, bp.s, bp.k, zeros));
-            if (int rc = factor(ss, p + 1)) return rc;
-            HCK(hipEventRecord(ctx->la_ev[1], ss));
-            HCK(bse_launch_her2k_part(st, fp32, kind, 2, n, bp.s, H, Vp, Z, bp.m, bp.s, bp.k, zeros));
-            if (pe) HCK(hipEventRecord(pe[3], st));
-            HCK(hipStreamWaitEvent(st, ctx->la_ev[1], 0));
+            HCK(bse_launch_zero_words(st, 2, qr_fail));
+            if (rho) HCK(hipMemcpyAsync(y, rho, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+            if (int rc = factor(st, 0)) return rc;
+            if (int rc = apply_y(st, 0)) return rc;
+            for (int p = 0; p < npan; ++p) {
+                const bse_ctx::BandPanel& bp = ctx->band_panels[p];
+                double2* Vp = Vst + bp.voff;
+                double2* Tp = Tall + (size_t)p * b * b;
+                hipEvent_t* pe = pev ? pev + 4 * p : nullptr;
+                if (!look || p + 1 == npan) {
+                    HCK(bse_band_update(st, fp32, kind, n, bp.s, bp.k, b, H, Vp, Tp, uw, zeros, pe));
+                    if (p + 1 < npan) {
+                        if (int rc = factor(st, p + 1)) return rc;
+                        if (int rc = apply_y(st, p + 1)) return rc;
+                    }
+                    continue;
+                }
+                double2* Z = nullptr;
+                HCK(bse_band_product(st, fp32, kind, n, bp.s, bp.k, b, H, Vp, Tp, uw, pe, &Z));
+                HCK(hipEventRecord(ctx->la_ev[0], st));
+                HCK(hipStreamWaitEvent(ss, ctx->la_ev[0], 0));
+                HCK(bse_launch_her2k_part(ss, fp32, kind, 1, n, bp.s, H, Vp, Z, bp.m, bp.s, bp.k, zeros));
+                if (int rc = factor(ss, p + 1)) return rc;
+                HCK(hipEventRecord(ctx->la_ev[1], ss));
+                // y's update runs on behind the factorisation, beside st's next
+                // block product (it reads only this panel's V and T)
+                if (int rc = apply_y(ss, p + 1)) return rc;
+                HCK(bse_launch_her2k_part(st, fp32, kind, 2, n, bp.s, H, Vp, Z, bp.m, bp.s, bp.k, zeros));
+                if (pe) HCK(hipEventRecord(pe[3], st));
+                HCK(hipStreamWaitEvent(st, ctx->la_ev[1], 0));
+            }
+            // join the side stream's last update of y
+            if (look) {
+                HCK(hipEventRecord(ctx->la_ev[0], ss));
+                HCK(hipStreamWaitEvent(st, ctx->la_ev[0], 0));
+            }
+            int failed = 0;
+            HCK(hipMemcpyAsync(&failed, qr_fail, sizeof(int), hipMemcpyDeviceToHost, st));
+            HCK(hipStreamSynchronize(st));
+            if (!failed) break;
         }
     }
     if (npan == 0) HCK(hipEventRecord(tv[0], st));

@@ -71,7 +71,7 @@ hipError_t bse_band_gather(hipStream_t, int, int, int, int, const void*, double2
 hipError_t bse_band_gather_tail(hipStream_t, int, int, int, const void*, double2*, int);
 int64_t bse_tsqr_workspace(int, int);
 hipError_t bse_band_panel_qr(hipStream_t, int, int, double2*, double2*, double2*, double2*, double2*, int,
-                             int, int*);
+                             int, int*, int*, int);
 hipError_t bse_launch_assemble_coupling(hipStream_t, int, int, int, int, int, int, int, int, const double2*,
                                         const double2*, const double2*, const double2*, double, double,
                                         double2*);
@@ -268,7 +268,7 @@ enum Slot {
     S_RS, S_RHS, S_RX, S_RG, S_RHG, S_RC, S_RSMALL,
     S_GDHR, S_GDEPS, S_GDCOEF, S_GDP, S_GDPV, S_GDRHO, S_GDDE, S_GDW, S_GY, S_GOUT,
     S_SGFB, S_SGFA, S_SGX, S_SGU, S_SGB, S_SGV, S_SGPART, S_APPLYP, S_MEMB, S_KMAP, S_KINV, S_TBK, S_TBH, S_TBR, S_TBE, S_TBC,
-    S_BAB, S_BAB0, S_BPAN, S_BTSQR, S_BUPD, S_BPROG, S_BII, S_BZERO,
+    S_BAB, S_BAB0, S_BPAN, S_BTSQR, S_BUPD, S_BPROG, S_BII, S_BZERO, S_BH0, S_BFAIL,
     // full BSE: coupling block, M / its Cholesky factor, W = L^H s3 L, tile
     // inverses, panel, rho~, L^-1 rho~, eigenvectors, error flag, scales
     S_BCPL, S_MFULL, S_WFULL, S_LINV, S_FPAN, S_FPANH, S_RHOT, S_ZT, S_XFULL, S_FERR, S_FSCALE, S_EVNEG,
