@@ -888,8 +888,8 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
 
 // Eigenvectors of the lowest nstore eigenvalues (d_lam, ascending) after a
 // two-stage reduction: inverse iteration on the band matrix, then Z = Q1 X.
-static int twostage_vectors(bse_ctx* ctx, int n, const double* d_lam, int nstore, double tnorm, double2* Z) {
-    hipStream_t st = ctx->st;
+static int twostage_vectors(bse_ctx* ctx, hipStream_t st, int n, const double* d_lam, int nstore, double tnorm,
+                            double2* Z) {
     const int b = kBand, ldab = 2 * b + 1;
     double2* ws = BUF(double2, S_BII, bse_band_ii_workspace(n, b, nstore));
     HCK(bse_launch_band_ii(st, n, b, ldab, (const double2*)ctx->bufs[S_BAB0].p, d_lam, nstore, tnorm, ws, Z, n));
@@ -1161,6 +1161,17 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
     int* d_memb = BUF(int, S_MEMB, n);
     HCK(hipMemcpyAsync(d_memb, memb.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     if (rho && !two) HCK(hipStreamWaitEvent(st, ctx->join_ev, 0));   // y = Q^H rho is ready
+    // two-stage: the band inverse iteration and Z = Q1 X need only the
+    // eigenvalues, so they run on the side stream beside the tridiagonal
+    // vectors of the oscillator contraction; st joins before the refinement
+    const bool vec_side = two && nstore > 0;
+    if (vec_side) {
+        HCK(hipEventRecord(ctx->fork_ev, st));
+        HCK(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+        int rc2 = twostage_vectors(ctx, ctx->side, n, d_lam, nstore, tnorm, Z);
+        if (rc2) return rc2;
+        HCK(hipEventRecord(ctx->join_ev, ctx->side));
+    }
     int m0 = 0;
     // the two-stage path forms its vectors from the band matrix: the
     // tridiagonal vectors are needed only for the oscillator contraction
@@ -1201,9 +1212,8 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
 
     // ---------------- K7: back-transform Z <- Q Z
     stage_begin(ctx, ST_BT);
-    if (two && nstore > 0) {
-        int rc2 = twostage_vectors(ctx, n, d_lam, nstore, tnorm, Z);
-        if (rc2) return rc2;
+    if (vec_side) {
+        HCK(hipStreamWaitEvent(st, ctx->join_ev, 0));
     } else if (nstore > 0 && nstore <= 16 && bse_apply_q_thin_workspace(n) <= sk_el) {
         HCK(bse_apply_q_thin(st, n, nb, Vst, Tall, nstore, Z, n, 0, skw));
     } else if (nstore >= 2048) {
