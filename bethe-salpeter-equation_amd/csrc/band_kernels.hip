@@ -28,6 +28,7 @@
 // Band storage (FP64, used by the second stage): AB[c * ldab + d] = A(c + d, c)
 // for 0 <= d <= 2b, so that the bulge chasing has room for its fill.
 #include "bse_common.hpp"
+#include <type_traits>
 
 #define BR_B 32            // band half-width = panel width
 #define TSQR_RB 64         // leaf block rows (the last leaf takes 64..127)
@@ -763,7 +764,7 @@ __global__ __launch_bounds__(256) void k_cq_chol2(int m, int b, int ng, const do
 // a diagonal tile is zero (storage invariant).
 // 16 x 16 threads: rows tx + 16 a (a < 4), columns ty + 16 c (c < 2).
 // ---------------------------------------------------------------------------
-#define HEMM_KSPLIT 4
+#define HEMM_KSPLIT 8
 template <typename T>
 __global__ __launch_bounds__(256) void k_band_hemm(int n, int s, int k, const T* __restrict__ A,
                                                    const double2* __restrict__ V, int64_t ldv,
@@ -837,14 +838,14 @@ __global__ __launch_bounds__(256) void k_band_hemm(int n, int s, int k, const T*
 // The same product for FP32 storage on matrix cores (v_mfma_f32_16x16x4_f32,
 // FP32 operands and accumulation; the KSPLIT partials are summed in FP64 by
 // k_band_xt).  Workgroup (i, q) as in k_band_hemm; wave w owns rows 16 w ..
-// 16 w + 15 of the 64-row block and all 32 output columns.  The realified
-// contraction runs over (k, part) pairs, 4 per MFMA step (lanes 16 h .. 16 h
-// + 15 take k = 2 step + h / 2, part = h % 2):
-//   Re X = sum Re A Re V - Im A Im V,   Im X = sum Im A Re V + Re A Im V,
-// so a lane feeds (Re A | -Im A) to the Re X product and (Im A | Re A) to the
-// Im X product, and (Re V | Im V) to both.  The tile A_IJ (or A_JI^H) is
-// staged in LDS as sA[k][row]; the next tile's loads are issued into
-// registers before the current tile's products.
+// 16 w + 15 of the 64-row block and all 32 output columns.  Lanes 16 h ..
+// 16 h + 15 take complex k = 4 step + h, and the four real products
+//   Re X = sum Re A Re V - Im A Im V,   Im X = sum Re A Im V + Im A Re V
+// have accumulators of their own, so every operand read serves four MFMAs
+// and no lane selects or negates.  The tile A_IJ (or A_JI^H) and the V block
+// are staged in LDS; the next tile's loads are issued into registers before
+// the current tile's products, and within the products each step's operands
+// are read one step ahead.
 // ---------------------------------------------------------------------------
 // FP32 copy of the panel's V (m x k, ld m) for the FP32 block product, row
 // major with 32 entries per row (zero beyond k): the 64 rows a tile needs
@@ -864,20 +865,30 @@ __global__ __launch_bounds__(256) void k_band_hemm32(int n, int s, int k, const 
                                                      const float2* __restrict__ V, int64_t ldv,
                                                      double2* __restrict__ Xp)     // V: FP32, row major (32 per row)
 {
-    __shared__ float2 sA[64][65];
-    __shared__ float2 sV[64][BR_B + 1];
+    // A tile (64 x 64) and V block (64 x 32) in LDS with XOR swizzles so
+    // that the four 16-lane groups of an operand read hit disjoint banks:
+    // A_IJ(r, kk) at sA[kk][r ^ 16 (kk & 1)] for a stored tile, and at
+    // sA[r][kk ^ 2 (r & 15)] for a transposed one (J > I, read from A_JI);
+    // V(kk, c) at sV[kk][c ^ 16 (kk & 1)].
+    __shared__ float2 sA[64 * 64];
+    __shared__ float2 sV[64 * BR_B];
     const int nt = tile_count(n), tcs = s >> 6, nT = nt - tcs;
     const int i = blockIdx.x, q = blockIdx.y;
     const int I = tcs + i;
     const int j0 = (int)((int64_t)nT * q / HEMM_KSPLIT), j1 = (int)((int64_t)nT * (q + 1) / HEMM_KSPLIT);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int h = lane >> 4, l16 = lane & 15;
-    hemm_f4 aR[2], aI[2];
-    double dR[2][4], dI[2][4];     // FP64 sums of the FP32 partials
+    const int row = 16 * w + l16;
+    // accumulators per column block c: Re A Re V, Im A Im V, Re A Im V,
+    // Im A Re V (each MFMA takes four distinct complex k, one per 16-lane
+    // group); per tile their FP32 partials (64 real products) go into FP64
+    // sums X = (RR - II) + i (RI + IR)
+    hemm_f4 acc[2][4];
+    double dR[2][4], dI[2][4];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        aR[c] = hemm_f4{0, 0, 0, 0};
-        aI[c] = hemm_f4{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[c][t] = hemm_f4{0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < 4; ++r) { dR[c][r] = 0.0; dI[c][r] = 0.0; }
     }
@@ -897,63 +908,75 @@ __global__ __launch_bounds__(256) void k_band_hemm32(int n, int s, int k, const 
         }
     };
     if (j0 < j1) fetch(j0);
+    const int swn = 16 * (h & 1);                 // kk & 1 = h & 1 for every kk this lane reads
+    const int aoff = h * 64 + (row ^ swn);        // stored tile: + 256 st
+    const int swt = 2 * (row & 15);               // transposed tile
+    const int voff0 = h * BR_B + (l16 ^ swn), voff1 = h * BR_B + ((16 + l16) ^ swn);   // + 128 st
     for (int jt = j0; jt < j1; ++jt) {
         const int J = tcs + jt;
         const bool tr = J > I, diag = (J == I);
         __syncthreads();    // the previous tile's products are done with sA, sV
         if (!tr) {
-            // element e = row + 64 kk of A_IJ
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
-                const int e = tid + 256 * u, row = e & 63, kk = e >> 6;
+                const int e = tid + 256 * u, r = e & 63, kk = e >> 6;
                 float2 x = ra[u];
-                if (diag && row == kk) x.y = 0.0f;
-                if (!(diag && row < kk)) sA[kk][row] = x;
-                // on a diagonal tile the stored element (row, kk), row > kk,
-                // also gives A(kk, row) = conj(A(row, kk))
-                if (diag && row > kk) sA[row][kk] = make_float2(x.x, -x.y);
+                if (diag && r == kk) x.y = 0.0f;
+                if (!(diag && r < kk)) sA[kk * 64 + (r ^ (16 * (kk & 1)))] = x;
+                // on a diagonal tile the stored element (r, kk), r > kk, also
+                // gives A(kk, r) = conj(A(r, kk))
+                if (diag && r > kk) sA[r * 64 + (kk ^ (16 * (r & 1)))] = make_float2(x.x, -x.y);
             }
         } else {
-            // element e = kk + 64 row of A_JI: A_IJ(row, kk) = conj(A_JI(kk, row))
+            // element e = kk + 64 r of A_JI: A_IJ(r, kk) = conj(A_JI(kk, r))
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
-                const int e = tid + 256 * u, kk = e & 63, row = e >> 6;
-                sA[kk][row] = make_float2(ra[u].x, -ra[u].y);
+                const int e = tid + 256 * u, kk = e & 63, r = e >> 6;
+                sA[r * 64 + (kk ^ (2 * (r & 15)))] = make_float2(ra[u].x, -ra[u].y);
             }
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int e = tid + 256 * u;
-            sV[e >> 5][e & 31] = rv[u];
+            const int e = tid + 256 * u, kk = e >> 5, cc = e & 31;
+            sV[kk * BR_B + (cc ^ (16 * (kk & 1)))] = rv[u];
         }
         __syncthreads();
         if (jt + 1 < j1) fetch(jt + 1);
-        const int row = 16 * w + l16;
-        // FP32 accumulation runs over 16 steps (64 real products) at a time;
-        // the partial then goes into the FP64 sums
-        for (int s0 = 0; s0 < 32; s0 += 16) {
-#pragma unroll 8
-            for (int st = s0; st < s0 + 16; ++st) {
-                const int kk = 2 * st + (h >> 1);
-                const bool im = h & 1;
-                const float2 a = sA[kk][row];
-                const float opR = im ? -a.y : a.x;     // Re X operand
-                const float opI = im ? a.x : a.y;      // Im X operand
+        // the products, one loop per layout (no branch inside), operands
+        // read one step ahead
+        auto mloop = [&](auto trc) {
+            constexpr bool TRT = decltype(trc)::value;
+            float2 a[2], v[2][2];
+            auto ld = [&](int st, int slot) {
+                a[slot] = TRT ? sA[row * 64 + ((4 * st + h) ^ swt)] : sA[aoff + 256 * st];
+                v[slot][0] = sV[voff0 + 128 * st];
+                v[slot][1] = sV[voff1 + 128 * st];
+            };
+            ld(0, 0);
+#pragma unroll
+            for (int st = 0; st < 16; ++st) {
+                const int sl = st & 1;
+                if (st + 1 < 16) ld(st + 1, sl ^ 1);
 #pragma unroll
                 for (int c = 0; c < 2; ++c) {
-                    const float2 v = sV[kk][16 * c + l16];
-                    const float bv = im ? v.y : v.x;
-                    aR[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(opR, bv, aR[c], 0, 0, 0);
-                    aI[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(opI, bv, aI[c], 0, 0, 0);
+                    acc[c][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sl].x, v[sl][c].x, acc[c][0], 0, 0, 0);
+                    acc[c][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sl].y, v[sl][c].y, acc[c][1], 0, 0, 0);
+                    acc[c][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sl].x, v[sl][c].y, acc[c][2], 0, 0, 0);
+                    acc[c][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sl].y, v[sl][c].x, acc[c][3], 0, 0, 0);
                 }
             }
+        };
+        if (tr) mloop(std::true_type{});
+        else mloop(std::false_type{});
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < 2; ++c) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) { dR[c][r] += (double)aR[c][r]; dI[c][r] += (double)aI[c][r]; }
-                aR[c] = hemm_f4{0, 0, 0, 0};
-                aI[c] = hemm_f4{0, 0, 0, 0};
+            for (int r = 0; r < 4; ++r) {
+                dR[c][r] += (double)acc[c][0][r] - (double)acc[c][1][r];
+                dI[c][r] += (double)acc[c][2][r] + (double)acc[c][3][r];
             }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[c][t] = hemm_f4{0, 0, 0, 0};
         }
     }
     // lane, reg r: X(16 w + 4 h + r, 16 c + l16)
