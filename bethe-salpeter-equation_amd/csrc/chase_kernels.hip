@@ -378,8 +378,9 @@ extern "C" hipError_t bse_launch_chase(hipStream_t st, int n, int b, double2* AB
 // taken before the chase) for eigenvalue lam[m] (one workgroup each):
 // LU with partial pivoting of B - lam I in a sliding LDS window of rows
 // j .. j+b and columns j .. j+2b (circular slots), U rows (2b + 1 entries)
-// and the b multipliers of each step to global memory, then three solves
-// from a deterministic pseudo-random start, each normalised.  A pivot below
+// and the b multipliers of each step to global memory, then two solves
+// from a deterministic pseudo-random start (the first a backward solve
+// only), each normalised.  A pivot below
 // eps ||T|| is replaced by eps ||T|| (the shift is an eigenvalue).
 // ---------------------------------------------------------------------------
 __device__ inline double bii_hash(uint64_t a, uint64_t b) {
@@ -391,7 +392,8 @@ __device__ inline double bii_hash(uint64_t a, uint64_t b) {
 }
 
 #define BII_W (2 * CH_B + 1)
-#define BII_BLK 64          // steps whose inputs are staged in LDS at once
+#define BII_BLK 64          // steps whose inputs are staged in LDS at once (one per lane of the wave)
+static_assert(BII_BLK == 64, "the staging loads one entering row per lane");
 
 // B(R, C) - lam delta_RC without branches (0 outside the band and the
 // matrix): a masked-off element reads past the descriptor's range.
@@ -457,13 +459,9 @@ __global__ __launch_bounds__(64) void k_band_ii(int n, int b, int ldab, const do
             for (int r = 0; r < 16; ++r) buf[(r0 + r) * BII_W + lane] = t[r];
         }
         {
-            if (lane == 0) {
-#pragma unroll 8
-                for (int r = 0; r < BII_BLK; ++r) {
-                    const int R = j0 + b + 1 + r;
-                    buf[r * BII_W + 2 * b] = band_elem_nb(rab, ldab, n, b, R, R + b, lm);
-                }
-            }
+            // entry 2b of every entering row: one row per lane, all loads at once
+            const int R = j0 + b + 1 + lane;
+            buf[lane * BII_W + 2 * b] = band_elem_nb(rab, ldab, n, b, R, R + b, lm);
         }
         wave_lds_sync();
         const int jend = min(n, j0 + BII_BLK);
@@ -533,16 +531,18 @@ __global__ __launch_bounds__(64) void k_band_ii(int n, int b, int ldab, const do
             wave_lds_sync();
         }
     }
-    // ---- solves: x <- U^-1 L^-1 P x, three times from a random start
+    // ---- solves: x <- U^-1 L^-1 P x, twice.  The first starts from a
+    // pseudo-random vector taken as L^-1 P x0 itself (x0 is as random as
+    // it is), so it is a backward solve alone; the second is a full solve.
     double2* x = X + (int64_t)m * ldx;
     for (int i = lane; i < n; i += 64) x[i] = cmk(bii_hash(m, 2 * i), bii_hash(m, 2 * i + 1));
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   // the LU's stores and x are complete
     const int bw = 2 * b;
-    for (int it = 0; it < 3; ++it) {
+    for (int it = 0; it < 2; ++it) {
         // forward: window x[j .. j+b] in lanes 0 .. b; a block's pivots,
         // multipliers and entering x[j+b+1] staged in LDS
         double2 xw = (lane <= b && lane < n) ? x[lane] : cmk(0, 0);
-        for (int j0 = 0; j0 < n; j0 += BII_BLK) {
+        for (int j0 = 0; j0 < (it > 0 ? n : 0); j0 += BII_BLK) {
             const int jend = min(n, j0 + BII_BLK);
             wave_lds_sync();
             for (int q0 = 0; q0 < BII_BLK * CH_B / 64; q0 += 16) {
