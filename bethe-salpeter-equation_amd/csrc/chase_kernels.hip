@@ -98,17 +98,31 @@ __device__ inline int chase_ntask(int n, int b, int s) { return 2 * ((n - 2 - s)
 // hand-over needs only the completion of the wave's stores (s_waitcnt
 // vmcnt(0)) before the progress counter is written, and no cache
 // maintenance at all.
+// COH = 1 inside a chunk: the waves of one workgroup hand over through
+// counters in LDS (lprog[w], wave w's completed tasks plus base = the
+// workgroup's chunk iteration << 20, so the values only grow and a slot is
+// never reset); the band data still goes through device-coherent memory,
+// and the producer's stores have completed (vmcnt(0)) before its counter
+// is written.  Global counters are polled with the error flag read only
+// every 32 spins, so a poll costs one memory round trip.
 template <int COH>
-__device__ inline bool chase_wait(const ChaseArgs& a, int s, int need, int lane, bool local) {
+__device__ inline bool chase_wait(const ChaseArgs& a, int s, int need, int lane, bool local, int* lprog, int w,
+                                  int base) {
     int ok = 1;
     if (lane == 0) {
         unsigned spins = 0;
         for (;;) {
-            const int got = (local && !COH) ? __hip_atomic_load(a.prog + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                                            : __hip_atomic_load(a.prog + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int got;
+            if (COH && local) got = __hip_atomic_load(lprog + (w - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) - base;
+            else if (local) got = __hip_atomic_load(a.prog + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            else got = __hip_atomic_load(a.prog + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (got >= need) break;
-            if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
-            if (++spins > CHASE_SPIN_LIMIT) {
+            ++spins;
+            if ((spins & 31) == 0 && __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                ok = 0;
+                break;
+            }
+            if (spins > CHASE_SPIN_LIMIT) {
                 __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
@@ -123,11 +137,15 @@ __device__ inline bool chase_wait(const ChaseArgs& a, int s, int need, int lane,
 }
 
 template <int COH>
-__device__ inline void chase_post(const ChaseArgs& a, int s, int val, int lane, bool local) {
+__device__ inline void chase_post(const ChaseArgs& a, int s, int val, int lane, bool local, int* lprog, int w,
+                                  int base) {
     if (COH) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_s_waitcnt(0x0F70);      // vmcnt(0): the wave's sc1 stores are done
-        if (lane == 0) __hip_atomic_store(a.prog + s, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) {
+            if (local) __hip_atomic_store(lprog + w, base + val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            else __hip_atomic_store(a.prog + s, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     } else if (local) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         if (lane == 0) __hip_atomic_store(a.prog + s, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -180,16 +198,21 @@ __global__ __launch_bounds__(64 * CH_W) void k_chase(ChaseArgs a)
     double2 (*X)[CH_B + 1] = sBlk[w];
     double2* sv = sV[w];
     double2* su = sU[w];
+    __shared__ int lprog[CH_W];
+    if (threadIdx.x < CH_W) lprog[threadIdx.x] = 0;
+    __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) a.d[0] = mem.ld(0).x;
-    for (int chunk = blockIdx.x; chunk < a.nchunks; chunk += gridDim.x) {
+    int iter = 0;
+    for (int chunk = blockIdx.x; chunk < a.nchunks; chunk += gridDim.x, ++iter) {
         const int s = chunk * CH_W + w;
+        const int base = (iter + 1) << 20;     // above any count of an earlier iteration
         if (s > n - 2) break;
         const int ntask = chase_ntask(n, b, s);
         const int nprev = (s > 0) ? chase_ntask(n, b, s - 1) : 0;
         double2 tau = cmk(0, 0);
         bool alive = true;
         for (int j = 0; j < ntask && alive; ++j) {
-            if (s > 0 && !chase_wait<COH>(a, s - 1, min(j + 3, nprev), lane, w > 0)) { alive = false; break; }
+            if (s > 0 && !chase_wait<COH>(a, s - 1, min(j + 3, nprev), lane, w > 0, lprog, w, base)) { alive = false; break; }
             if ((j & 1) == 0) {
                 // ---- D_k: (k = 0: reflector from column s) two-sided update
                 const int k = j >> 1;
@@ -344,7 +367,7 @@ __global__ __launch_bounds__(64 * CH_W) void k_chase(ChaseArgs a)
                 tau = r.tau;
                 wave_lds_sync();
             }
-            chase_post<COH>(a, s, j + 1, lane, w < CH_W - 1);
+            chase_post<COH>(a, s, j + 1, lane, w < CH_W - 1, lprog, w, base);
         }
         if (!alive) break;
     }
