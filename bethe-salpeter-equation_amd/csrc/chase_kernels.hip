@@ -438,11 +438,16 @@ __device__ inline double2 band_elem_nb(__amdgpu_buffer_rsrc_t rab, int ldab, int
 // multipliers (forward) and U rows (backward) of BII_BLK steps are staged
 // the same way.  The band store is addressed with 32-bit byte offsets
 // (n (2b + 1) 16 < 2^31).
-__global__ __launch_bounds__(64) void k_band_ii(int n, int b, int ldab, const double2* __restrict__ AB0,
+// BB > 0: the half-bandwidth as a compile-time constant (the band
+// reduction's b = 32), so the per-element bounds of the elimination vanish;
+// BB = 0: b at run time.
+template <int BB>
+__global__ __launch_bounds__(64) void k_band_ii(int n, int b_rt, int ldab, const double2* __restrict__ AB0,
                                                 const double* __restrict__ lam, double tnorm,
                                                 double2* __restrict__ Ub, double2* __restrict__ Lm,
                                                 int* __restrict__ piv, double2* __restrict__ X, int64_t ldx)
 {
+    const int b = BB > 0 ? BB : b_rt;
     __shared__ double2 W[CH_B + 1][BII_W];
     __shared__ double2 buf[BII_BLK * BII_W];
     __shared__ double2 lb[CH_B];
@@ -685,8 +690,12 @@ extern "C" hipError_t bse_launch_band_ii(hipStream_t st, int n, int b, int ldab,
     double2* Ub = ws;
     double2* Lm = Ub + (int64_t)nvec * n * (2 * b + 1);
     int* piv = reinterpret_cast<int*>(Lm + (int64_t)nvec * n * b);
-    hipLaunchKernelGGL(k_band_ii, dim3((unsigned)nvec), dim3(64), 0, st, n, b, ldab, AB0, lam, tnorm, Ub, Lm,
-                       piv, X, ldx);
+    if (b == CH_B)
+        hipLaunchKernelGGL(k_band_ii<CH_B>, dim3((unsigned)nvec), dim3(64), 0, st, n, b, ldab, AB0, lam, tnorm, Ub,
+                           Lm, piv, X, ldx);
+    else
+        hipLaunchKernelGGL(k_band_ii<0>, dim3((unsigned)nvec), dim3(64), 0, st, n, b, ldab, AB0, lam, tnorm, Ub,
+                           Lm, piv, X, ldx);
     hipLaunchKernelGGL(k_mgs_columns, dim3(1), dim3(1024), 0, st, n, nvec, X, ldx);
     return hipGetLastError();
 }
