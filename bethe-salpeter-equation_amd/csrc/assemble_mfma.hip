@@ -248,10 +248,12 @@ extern "C" int bse_assemble_mfma_supported(int nv, int nc, int norb);
 // summation order), and k_apply_sum adds, in a fixed order, the partials,
 // beta PV Z and the diagonal dE X.  Requires TR <= 64 (bse_assemble_mfma_tk).
 // ---------------------------------------------------------------------------
-#define AM_K 16
+#define AM_K 32      // vectors per launch: KG groups of 16, one MFMA column block each
+template <int KG>
 __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __restrict__ X, int64_t ldx, int k,
                                                     int ncg, double2* __restrict__ Yp)
 {
+    constexpr int KX = 16 * KG;               // vectors staged per tile
     extern __shared__ double2 smem[];
     const int B = a.nc * a.nv, ns = a.nv + a.nc, no = a.norb, TK = a.TK;
     const int TR = TK * B;
@@ -263,8 +265,8 @@ __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __r
     double* sT = (double*)(sCc + TK * ns * 8); // per-wave 16x17 transposition
     double* sHr = sT + 4 * 272;                // [64][65]
     double* sHi = sHr + 64 * 65;
-    double* sXr = sHi + 64 * 65;               // [64][16]
-    double* sXi = sXr + 64 * 16;
+    double* sXr = sHi + 64 * 65;               // [64][KX]
+    double* sXi = sXr + 64 * KX;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int e = tid; e < TK * ns * 8; e += 256) {
         const int i = e & 7, sl = (e >> 3) % ns, kk = e / (8 * ns);
@@ -275,7 +277,9 @@ __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __r
     for (int e = tid; e < 64 * 65; e += 256) { sHr[e] = 0.0; sHi[e] = 0.0; }
     double* tw = sT + wv * 272;
     const int li = lane & 15, lq = lane >> 4;
-    mf_d4 yr = mf_d4{0, 0, 0, 0}, yi = mf_d4{0, 0, 0, 0};
+    mf_d4 yr[KG], yi[KG];
+#pragma unroll
+    for (int q = 0; q < KG; ++q) { yr[q] = mf_d4{0, 0, 0, 0}; yi[q] = mf_d4{0, 0, 0, 0}; }
     for (int tj = g; tj < nt; tj += ncg) {
         const int kp0 = tj * TK;
         __syncthreads();   // previous tile fully consumed
@@ -284,8 +288,8 @@ __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __r
             const int kp = kp0 + kk;
             sCc[e] = (kp < a.nk && i < no) ? a.coef[((size_t)kp * ns + sl) * no + i] : cmk(0, 0);
         }
-        for (int e = tid; e < 64 * AM_K; e += 256) {
-            const int rr = e >> 4, j = e & 15;
+        for (int e = tid; e < 64 * KX; e += 256) {
+            const int rr = e / KX, j = e % KX;
             const int64_t col = (int64_t)kp0 * B + rr;
             double2 x = cmk(0, 0);
             if (rr < TR && col < a.N && j < k) x = X[col + (int64_t)j * ldx];
@@ -361,52 +365,55 @@ __global__ __launch_bounds__(256) void k_apply_mfma(MfArgs a, const double2* __r
             }
         }
         __syncthreads();
-        // Y rows 16 wv .. 16 wv + 15 += H(rows, 0:64) X(0:64, 0:16)
+        // Y rows 16 wv .. 16 wv + 15 += H(rows, 0:64) X(0:64, 0:KX), one H
+        // tile formed for all KX vectors
         const int hr0 = (16 * wv + li) * 65;
 #pragma unroll 4
         for (int kc = 0; kc < 64; kc += 4) {
             const double hr = sHr[hr0 + kc + lq], hi = sHi[hr0 + kc + lq];
-            const double xr = sXr[(kc + lq) * 16 + li], xi = sXi[(kc + lq) * 16 + li];
-            yr = __builtin_amdgcn_mfma_f64_16x16x4f64(hr, xr, yr, 0, 0, 0);
-            yr = __builtin_amdgcn_mfma_f64_16x16x4f64(-hi, xi, yr, 0, 0, 0);
-            yi = __builtin_amdgcn_mfma_f64_16x16x4f64(hr, xi, yi, 0, 0, 0);
-            yi = __builtin_amdgcn_mfma_f64_16x16x4f64(hi, xr, yi, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < KG; ++q) {
+                const double xr = sXr[(kc + lq) * KX + 16 * q + li], xi = sXi[(kc + lq) * KX + 16 * q + li];
+                yr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(hr, xr, yr[q], 0, 0, 0);
+                yr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-hi, xi, yr[q], 0, 0, 0);
+                yi[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(hr, xi, yi[q], 0, 0, 0);
+                yi[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(hi, xr, yi[q], 0, 0, 0);
+            }
         }
     }
-    // lane holds Y[row 16 wv + mfma16_row_f64(lane, reg)][vector li]
+    // lane holds Y[row 16 wv + mfma16_row_f64(lane, reg)][vector 16 q + li]
 #pragma unroll
-    for (int reg = 0; reg < 4; ++reg) {
-        const int rl = 16 * wv + mfma16_row_f64(lane, reg);
-        const int64_t grow = (int64_t)k0 * B + rl;
-        if (rl < TR && grow < a.N) Yp[((int64_t)g * a.N + grow) * AM_K + li] = cmk(yr[reg], yi[reg]);
-    }
+    for (int q = 0; q < KG; ++q)
+#pragma unroll
+        for (int reg = 0; reg < 4; ++reg) {
+            const int rl = 16 * wv + mfma16_row_f64(lane, reg);
+            const int64_t grow = (int64_t)k0 * B + rl;
+            if (rl < TR && grow < a.N) Yp[((int64_t)g * a.N + grow) * AM_K + 16 * q + li] = cmk(yr[q][reg], yi[q][reg]);
+        }
 }
 
-// Z partials: block g sums conj(P(r, i)) X(r, j) over its rows (two row
-// phases per (i, j)); 128 entries (i < 8, j < 16) per block.
+// Z partials: block g sums conj(P(r, i)) X(r, j) over its rows; 256
+// entries (i < 8, j < 32), one thread each.
 #define APX_BLOCKS 256
 __global__ __launch_bounds__(256) void k_apply_px(int64_t N, int no, int k, const double2* __restrict__ P,
                                                   const double2* __restrict__ X, int64_t ldx,
                                                   double2* __restrict__ Zp)
 {
-    __shared__ double2 s[256];
-    const int t = threadIdx.x, i = t & 7, j = (t >> 3) & 15, h = t >> 7;
+    const int t = threadIdx.x, i = t & 7, j = t >> 3;
     const int64_t per = ceil_div64(N, (int64_t)gridDim.x);
     const int64_t r0 = (int64_t)blockIdx.x * per, r1 = min(N, r0 + per);
     double2 acc = cmk(0, 0);
     if (i < no && j < k)
-        for (int64_t r = r0 + h; r < r1; r += 2) cfmac(acc, P[r * no + i], X[r + (int64_t)j * ldx]);
-    s[t] = acc;
-    __syncthreads();
-    if (h == 0) Zp[(int64_t)blockIdx.x * 128 + t] = cadd(s[t], s[t + 128]);
+        for (int64_t r = r0; r < r1; ++r) cfmac(acc, P[r * no + i], X[r + (int64_t)j * ldx]);
+    Zp[(int64_t)blockIdx.x * 256 + t] = acc;
 }
 
 // Z(i, j) = sum over the G partials in order (one thread per entry).
-__global__ __launch_bounds__(128) void k_apply_pxsum(int G, const double2* __restrict__ Zp,
+__global__ __launch_bounds__(256) void k_apply_pxsum(int G, const double2* __restrict__ Zp,
                                                      double2* __restrict__ Z)
 {
     double2 z = cmk(0, 0);
-    for (int g = 0; g < G; ++g) z = cadd(z, Zp[(int64_t)g * 128 + threadIdx.x]);
+    for (int g = 0; g < G; ++g) z = cadd(z, Zp[(int64_t)g * 256 + threadIdx.x]);
     Z[threadIdx.x] = z;
 }
 
@@ -433,11 +440,11 @@ __global__ __launch_bounds__(256) void k_apply_sum(int64_t N, int k, int ncg, in
     Y[r + (int64_t)j * ldy] = s;
 }
 
-static size_t apply_mfma_lds_bytes(int TK, int nv, int nc) {
+static size_t apply_mfma_lds_bytes(int TK, int nv, int nc, int kg) {
     const int ns = nv + nc, B = nv * nc;
     (void)B;
     return ((size_t)2 * TK * ns * 8) * sizeof(double2) +
-           ((size_t)4 * 272 + 2 * 64 * 65 + 2 * 64 * 16) * sizeof(double);
+           ((size_t)4 * 272 + 2 * 64 * 65 + 2 * 64 * 16 * kg) * sizeof(double);
 }
 
 // Column groups per row block of the matrix-core product (about four
@@ -447,7 +454,7 @@ extern "C" int bse_apply_mfma_groups(int nk, int nv, int nc) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(nt, ceil_div64(1024, nt)));
 }
 extern "C" int64_t bse_apply_mfma_workspace(int nk, int nv, int nc) {
-    return (int64_t)bse_apply_mfma_groups(nk, nv, nc) * nk * nv * nc * AM_K + (APX_BLOCKS + 1) * 128;
+    return (int64_t)bse_apply_mfma_groups(nk, nv, nc) * nk * nv * nc * AM_K + (APX_BLOCKS + 1) * 256;
 }
 
 extern "C" hipError_t bse_launch_apply_mfma(hipStream_t st, int nk, int n1, int n2, int n3, int nv, int nc,
@@ -466,13 +473,27 @@ extern "C" hipError_t bse_launch_apply_mfma(hipStream_t st, int nk, int n1, int 
     a.coef = coef; a.wtab = wtab; a.PV = PV; a.P = P; a.dE = dE; a.alpha = alpha; a.beta = beta;
     const int nt = (int)ceil_div64(nk, a.TK);
     const int ncg = bse_apply_mfma_groups(nk, nv, nc);
-    const size_t lds = apply_mfma_lds_bytes(a.TK, nv, nc);
+    const int kg = k > 16 ? 2 : 1;
+    const size_t lds = apply_mfma_lds_bytes(a.TK, nv, nc, kg);
     double2* Zp = ws + (int64_t)ncg * a.N * AM_K;   // exchange: Z = P^H X
-    double2* Z = Zp + (int64_t)APX_BLOCKS * 128;
+    double2* Z = Zp + (int64_t)APX_BLOCKS * 256;
     hipLaunchKernelGGL(k_apply_px, dim3(APX_BLOCKS), dim3(256), 0, st, a.N, norb, k, P, X, ldx, Zp);
-    hipLaunchKernelGGL(k_apply_pxsum, dim3(1), dim3(128), 0, st, APX_BLOCKS, (const double2*)Zp, Z);
-    hipLaunchKernelGGL(k_apply_mfma, dim3((unsigned)nt, (unsigned)ncg), dim3(256), lds, st, a, X, ldx, k, ncg,
-                       ws);
+    hipLaunchKernelGGL(k_apply_pxsum, dim3(1), dim3(256), 0, st, APX_BLOCKS, (const double2*)Zp, Z);
+    // the LDS beyond 64 KB is asked for explicitly (once per process)
+    static const hipError_t attr = [] {
+        const int big = 160 * 1024 - 1024;
+        hipError_t e = hipFuncSetAttribute((const void*)k_apply_mfma<1>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_apply_mfma<2>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+        return e;
+    }();
+    if (attr != hipSuccess) return attr;
+    if (kg == 2)
+        hipLaunchKernelGGL(k_apply_mfma<2>, dim3((unsigned)nt, (unsigned)ncg), dim3(256), lds, st, a, X, ldx, k,
+                           ncg, ws);
+    else
+        hipLaunchKernelGGL(k_apply_mfma<1>, dim3((unsigned)nt, (unsigned)ncg), dim3(256), lds, st, a, X, ldx, k,
+                           ncg, ws);
     hipLaunchKernelGGL(k_apply_sum, dim3((unsigned)ceil_div64(a.N * AM_K, 256)), dim3(256), 0, st, a.N, k,
                        ncg, norb, (const double2*)ws, PV, (const double2*)Z, dE, beta, X, ldx, Y, ldy);
     return hipGetLastError();

@@ -1256,10 +1256,10 @@ static int apply_H(bse_ctx* ctx, int k, const double2* X, int64_t ldx, double2* 
     const double beta = s.f_s * s.exchange_scale / (ctx->cell * ctx->nk);
     if (!ctx->force_valu_assembly && bse_assemble_mfma_supported(s.nv, s.nc, ctx->norb)) {
         // matrix cores: tiles of H formed in LDS and multiplied at once, at
-        // most 16 columns per launch
+        // most 32 columns per launch (one H tile serves them all)
         double2* ws = BUF(double2, S_APPLYP, bse_apply_mfma_workspace(ctx->nk, s.nv, s.nc));
-        for (int c0 = 0; c0 < k; c0 += 16) {
-            const int kc = std::min(16, k - c0);
+        for (int c0 = 0; c0 < k; c0 += 32) {
+            const int kc = std::min(32, k - c0);
             HCK(bse_launch_apply_mfma(ctx->st, ctx->nk, s.n1, s.n2, ctx->n3, s.nv, s.nc, ctx->norb, coef,
                                       (const double2*)ctx->bufs[S_WTAB].p, (const double2*)ctx->bufs[S_PV].p,
                                       (const double2*)ctx->bufs[S_P].p, (const double*)ctx->bufs[S_DE].p,
