@@ -1216,6 +1216,13 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
         HCK(hipStreamWaitEvent(st, ctx->join_ev, 0));
     } else if (nstore > 0 && nstore <= 16 && bse_apply_q_thin_workspace(n) <= sk_el) {
         HCK(bse_apply_q_thin(st, n, nb, Vst, Tall, nstore, Z, n, 0, skw));
+    } else if (nstore >= 2048 && nb == 32 && !getenv("BSE_BT_STRIPS")) {
+        // many vectors: four panels per pass on FP64 matrix cores
+        int64_t* voff_d = BUF(int64_t, S_BTVOFF, npan);
+        HCK(hipMemcpyAsync(voff_d, voff.data(), sizeof(int64_t) * npan, hipMemcpyHostToDevice, st));
+        const int64_t wel = bse_apply_q_grouped_workspace(n, npan, nstore);
+        double2* bws = BUF(double2, S_BTWS, wel);
+        HCK(bse_apply_q_grouped(st, n, nb, npan, Vst, voff_d, Tall, Z, n, nstore, bws, wel));
     } else if (nstore >= 2048) {
         // many vectors: one fused launch per panel over 32-column strips
         // (at least 64 strips, so the launch fills the chip)

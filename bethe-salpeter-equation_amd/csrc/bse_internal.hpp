@@ -91,6 +91,9 @@ hipError_t bse_launch_rpa_table(hipStream_t, int, int, int, int, int, int, const
                                 double2*, double2*);
 hipError_t bse_apply_q_strips(hipStream_t, int, int, int, const double2*, const int64_t*, const double2*,
                               double2*, int64_t, int);
+int64_t bse_apply_q_grouped_workspace(int, int, int);
+hipError_t bse_apply_q_grouped(hipStream_t, int, int, int, const double2*, const int64_t*, const double2*, double2*,
+                               int64_t, int, double2*, int64_t);
 int64_t bse_band_update_workspace(int, int, int);
 hipError_t bse_band_update(hipStream_t, int, int, int, int, int, int, void*, const double2*, const double2*,
                            double2*, const double2*, hipEvent_t*);
@@ -269,6 +272,8 @@ enum Slot {
     S_GDHR, S_GDEPS, S_GDCOEF, S_GDP, S_GDPV, S_GDRHO, S_GDDE, S_GDW, S_GY, S_GOUT,
     S_SGFB, S_SGFA, S_SGX, S_SGU, S_SGB, S_SGV, S_SGPART, S_APPLYP, S_MEMB, S_KMAP, S_KINV, S_TBK, S_TBH, S_TBR, S_TBE, S_TBC,
     S_BAB, S_BAB0, S_BPAN, S_BTSQR, S_BUPD, S_BPROG, S_BII, S_BZERO, S_BH0, S_BFAIL,
+    // grouped back-transform: panel offsets, workspace
+    S_BTVOFF, S_BTWS,
     // full BSE: coupling block, M / its Cholesky factor, W = L^H s3 L, tile
     // inverses, panel, rho~, L^-1 rho~, eigenvectors, error flag, scales
     S_BCPL, S_MFULL, S_WFULL, S_LINV, S_FPAN, S_FPANH, S_RHOT, S_ZT, S_XFULL, S_FERR, S_FSCALE, S_EVNEG,

@@ -185,12 +185,15 @@ def test_dense_eigh_cluster_split_by_another_block(ctx):
     assert np.abs(a @ z - z * ev).max() < 1e-12
 
 
-@pytest.mark.parametrize("n", [2100, 4160])
-def test_dense_all_vectors_strip_backtransform(ctx, n):
-    """All eigenvectors of a dense matrix large enough for the strip
-    back-transform (>= 2048 vectors): residual and orthonormality at FP64
-    level, eigenvalues against LAPACK.  2100 leaves a partial strip and a
-    partial last row chunk; 4160 = 65 whole tiles."""
+@pytest.mark.parametrize("n", [2100, 2150, 2200, 4160])
+def test_dense_all_vectors_grouped_backtransform(ctx, n):
+    """All eigenvectors of a dense matrix large enough for the grouped
+    back-transform (>= 2048 vectors, four panels of 32 reflectors per pass):
+    residual and orthonormality at FP64 level, eigenvalues against LAPACK.
+    Panels (n - 1) / 32 rounded up: 2100 -> 66 (last group two panels, the
+    last panel 19 reflectors, partial column and row tiles), 2150 -> 68 (whole
+    groups, last panel 5 reflectors), 2200 -> 69 (last group one panel),
+    4160 -> 130 (65 whole 64-column tiles)."""
     rng = np.random.default_rng(n)
     a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
     a = 0.5 * (a + a.conj().T)
@@ -202,6 +205,23 @@ def test_dense_all_vectors_strip_backtransform(ctx, n):
     cols = np.r_[0:8, n // 2:n // 2 + 8, n - 8:n]
     assert np.abs(a @ z[:, cols] - z[:, cols] * ev[None, cols]).max() < 1e-11 * scale * n ** 0.5
     assert np.abs(z.conj().T @ z - np.eye(n)).max() < 1e-10
+
+
+def test_grouped_backtransform_matches_strips(ctx, monkeypatch):
+    """The grouped back-transform (one compact-WY block of four panels) and
+    the per-panel strip kernel apply the same Q to the same tridiagonal
+    eigenvectors: the eigenvector matrices agree to FP64 rounding."""
+    n = 2150
+    rng = np.random.default_rng(7)
+    a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    a = 0.5 * (a + a.conj().T)
+    print(f"grouped vs strip back-transform n={n}", flush=True)
+    ev_g, z_g = ctx.eigh_dense(a, "fp64", -1)
+    monkeypatch.setenv("BSE_BT_STRIPS", "1")
+    ev_s, z_s = ctx.eigh_dense(a, "fp64", -1)
+    scale = np.abs(ev_g).max()
+    assert np.abs(ev_g - ev_s).max() < 1e-13 * scale
+    assert np.abs(z_g - z_s).max() < 1e-11
 
 
 def test_c4_full_sweep_against_oracle(ctx):
