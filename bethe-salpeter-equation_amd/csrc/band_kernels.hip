@@ -1245,7 +1245,23 @@ extern "C" int64_t bse_band_update_workspace(int n, int s, int b) {
     const int nT = tile_count(n) - (s >> 6);
     const int m = n - s;
     return (int64_t)HEMM_KSPLIT * nT * 64 * BR_B + (int64_t)ceil_div64(m, 64) * BR_B * BR_B + 2 * BR_B * BR_B
-           + (int64_t)m * b + ((int64_t)m * BR_B + 1) / 2 + 64;   // ..., Z, FP32 copy of V
+           + (int64_t)m * b + ((int64_t)m * BR_B + 1) / 2     // ..., Z, FP32 copy of V
+           + (int64_t)m * BR_B + 64;                            // FP32 planes of V and Z
+}
+
+// The rank-2k update's operands in FP32 planes: pl[q k m + l m + r] for
+// plane q = Re V, Im V, Re Z, Im Z (V, Z: m x k, ld m), rounded as the update
+// stages them in LDS.
+__global__ __launch_bounds__(256) void k_her2k_pack(int m, int k, const double2* __restrict__ V,
+                                                    const double2* __restrict__ Z, float* __restrict__ pl)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x, mk = (int64_t)m * k;
+    if (e >= mk) return;
+    const double2 v = V[e], z = Z[e];
+    pl[e] = (float)v.x;
+    pl[mk + e] = (float)v.y;
+    pl[2 * mk + e] = (float)z.x;
+    pl[3 * mk + e] = (float)z.y;
 }
 
 // A22 <- Q^H A22 Q for Q = I - V T V^H (V: m x k, ld m, rows from s).
@@ -1255,10 +1271,11 @@ extern "C" hipError_t bse_launch_her2k(hipStream_t, int, int, int, int, void*, c
                                        const double2*, int64_t, int, int, const double2*);
 // The two-sided update's first half: Z = X T - 1/2 V (T^H V^H X T) with
 // X = A22 V.  ev (may be null): three events recorded before and after the
-// Hermitian block product and after Z.  Returns Z's address in *zout.
+// Hermitian block product and after Z.  Returns Z's address in *zout; with
+// pout non-null also V and Z packed in FP32 planes (k_her2k_pack) at *pout.
 extern "C" hipError_t bse_band_product(hipStream_t st, int fp32, int kind, int n, int s, int k, int b,
                                        const void* A, const double2* V, const double2* Tp, double2* ws,
-                                       hipEvent_t* ev, double2** zout)
+                                       hipEvent_t* ev, double2** zout, float** pout)
 {
     const int m = n - s;
     const int tcs = s >> 6, nT = tile_count(n) - tcs;
@@ -1287,6 +1304,12 @@ extern "C" hipError_t bse_band_product(hipStream_t st, int fp32, int kind, int n
     hipLaunchKernelGGL(k_band_m, dim3((unsigned)k), dim3(256), 0, st, nblk, k, b, (const double2*)Gp, Tp, M);
     hipLaunchKernelGGL(k_rows_small, dim3((unsigned)ceil_div64(m, 64)), dim3(256), 0, st, m, 0, k, k, b, V,
                        (const double2*)M, Z, 1, (const int*)nullptr);
+    if (pout) {
+        float* pl = reinterpret_cast<float*>(Z + (int64_t)m * b + ((int64_t)m * BR_B + 1) / 2);
+        hipLaunchKernelGGL(k_her2k_pack, dim3((unsigned)ceil_div64((int64_t)m * k, 256)), dim3(256), 0, st, m, k,
+                           V, (const double2*)Z, pl);
+        *pout = pl;
+    }
     if (ev) (void)hipEventRecord(ev[2], st);
     return hipGetLastError();
 }
@@ -1300,7 +1323,7 @@ extern "C" hipError_t bse_band_update(hipStream_t st, int fp32, int kind, int n,
     const int m = n - s;
     if (m <= 0 || k <= 0) return hipSuccess;
     double2* Z = nullptr;
-    hipError_t e = bse_band_product(st, fp32, kind, n, s, k, b, A, V, Tp, ws, ev, &Z);
+    hipError_t e = bse_band_product(st, fp32, kind, n, s, k, b, A, V, Tp, ws, ev, &Z, nullptr);
     if (e != hipSuccess) return e;
     e = bse_launch_her2k(st, fp32, kind, n, s, A, V, Z, m, s, k, zeros64);
     if (ev) (void)hipEventRecord(ev[3], st);

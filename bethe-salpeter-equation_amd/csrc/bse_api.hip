@@ -751,6 +751,10 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
         const char* lae = getenv("BSE_BAND_LOOKAHEAD");
         const bool look = (kind == 0 || kind == 1 || kind == 3) && !(lae && lae[0] == '0') && npan > 1;
         ctx->band_lookahead = look;
+        // FP32 matrix-core update: its operands V and Z packed once per panel
+        // in FP32 planes (BSE_HER2K_PACK=0 reads the FP64 panels per tile)
+        const char* pke = getenv("BSE_HER2K_PACK");
+        const bool pack = look && kind == 3 && !(pke && pke[0] == '0');
         hipStream_t ss = ctx->side;
         double2* qw = rho ? BUF(double2, S_W1, bse_apply_q_thin_workspace(n)) : nullptr;
         y = rho ? BUF(double2, S_Y, n) : nullptr;
@@ -804,16 +808,21 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
                     continue;
                 }
                 double2* Z = nullptr;
-                HCK(bse_band_product(st, fp32, kind, n, bp.s, bp.k, b, H, Vp, Tp, uw, pe, &Z));
+                float* planes = nullptr;
+                HCK(bse_band_product(st, fp32, kind, n, bp.s, bp.k, b, H, Vp, Tp, uw, pe, &Z,
+                                     pack ? &planes : nullptr));
+                // FP32 storage on matrix cores: both parts read V and Z from FP32 planes
+                const int pk = pack ? 4 : kind;
+                const double2* pv = pack ? reinterpret_cast<const double2*>(planes) : Vp;
                 HCK(hipEventRecord(ctx->la_ev[0], st));
                 HCK(hipStreamWaitEvent(ss, ctx->la_ev[0], 0));
-                HCK(bse_launch_her2k_part(ss, fp32, kind, 1, n, bp.s, H, Vp, Z, bp.m, bp.s, bp.k, zeros));
+                HCK(bse_launch_her2k_part(ss, fp32, pk, 1, n, bp.s, H, pv, Z, bp.m, bp.s, bp.k, zeros));
                 if (int rc = factor(ss, p + 1)) return rc;
                 HCK(hipEventRecord(ctx->la_ev[1], ss));
                 // y's update runs on behind the factorisation, beside st's next
                 // block product (it reads only this panel's V and T)
                 if (int rc = apply_y(ss, p + 1)) return rc;
-                HCK(bse_launch_her2k_part(st, fp32, kind, 2, n, bp.s, H, Vp, Z, bp.m, bp.s, bp.k, zeros));
+                HCK(bse_launch_her2k_part(st, fp32, pk, 2, n, bp.s, H, pv, Z, bp.m, bp.s, bp.k, zeros));
                 if (pe) HCK(hipEventRecord(pe[3], st));
                 HCK(hipStreamWaitEvent(st, ctx->la_ev[1], 0));
             }

@@ -98,7 +98,7 @@ int64_t bse_band_update_workspace(int, int, int);
 hipError_t bse_band_update(hipStream_t, int, int, int, int, int, int, void*, const double2*, const double2*,
                            double2*, const double2*, hipEvent_t*);
 hipError_t bse_band_product(hipStream_t, int, int, int, int, int, int, const void*, const double2*,
-                            const double2*, double2*, hipEvent_t*, double2**);
+                            const double2*, double2*, hipEvent_t*, double2**, float**);
 hipError_t bse_launch_her2k_part(hipStream_t, int, int, int, int, int, void*, const double2*, const double2*,
                                  int64_t, int, int, const double2*);
 int64_t bse_chase_counters(int);

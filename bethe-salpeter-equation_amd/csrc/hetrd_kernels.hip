@@ -915,11 +915,17 @@ struct her2k_f32 {
     __device__ static inline int row(int lane, int r) { return mfma16_row_f32(lane, r); }
 };
 
-template <typename T, typename MM = her2k_f64>
+// PK: the operands come packed in FP32 planes pl (real V, imaginary V, real
+// W, imaginary W; plane q at pl + q nb ldv, element (row r, column l) at
+// l ldv + r - j0) and alpha2 is taken as zero -- the band reduction's
+// update, whose panels are then read at 4 bytes per real part instead of 8,
+// and V once instead of twice; the values staged in LDS are the same.
+template <typename T, typename MM = her2k_f64, bool PK = false>
 __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int s, T* __restrict__ A,
                                        const double2* __restrict__ V,
                                        const double2* __restrict__ W, int64_t ldv, int j0, int nb,
-                                       const double2* __restrict__ alpha2, int64_t idx)
+                                       const double2* __restrict__ alpha2, int64_t idx,
+                                       const float* __restrict__ pl = nullptr)
 {
     typedef typename MM::real R;
     typedef typename MM::acc ACC;
@@ -955,7 +961,31 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int
     const bool rok = gr >= s && gr < n, cok = gc >= s && gc < n;
     const int64_t ro = (int64_t)(gr - j0), co = (int64_t)(gc - j0);
     double2 pvr[4], pwr[4], pc[4];           // row V, row W, column operand
+    float fpr[4], fpi[4], fqr[4], fqi[4];    // PK: row P, column Q (real, imaginary)
+    const int64_t pstride = (int64_t)nb * ldv;
     auto fetch = [&](int l0) {
+        if constexpr (PK) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kk = l0 + (threadIdx.x >> 6) + 4 * u;
+                fpr[u] = fpi[u] = fqr[u] = fqi[u] = 0.0f;
+                if (kk < K2) {
+                    const bool first = kk < nb;
+                    const int64_t lo = (int64_t)(first ? kk : kk - nb) * ldv;
+                    const float* P = pl + (first ? 0 : 2 * pstride);     // P = [V, W]
+                    const float* Q = pl + (first ? 2 * pstride : 0);     // Q = [W, V]
+                    if (rok) {
+                        fpr[u] = P[ro + lo];
+                        fpi[u] = P[pstride + ro + lo];
+                    }
+                    if (cok) {
+                        fqr[u] = Q[co + lo];
+                        fqi[u] = Q[pstride + co + lo];
+                    }
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int kk = l0 + (threadIdx.x >> 6) + 4 * u;
@@ -991,6 +1021,13 @@ __device__ __attribute__((always_inline)) inline void her2k_mfma_body(int n, int
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int kc = (threadIdx.x >> 6) + 4 * u, kk = l0 + kc;
+            if constexpr (PK) {
+                sPr[kc][rr] = (R)fpr[u];
+                sPi[kc][rr] = (R)fpi[u];
+                sQr[kc][rr] = (R)fqr[u];
+                sQi[kc][rr] = (R)fqi[u];
+                continue;
+            }
             double2 pv = pvr[u];
             if (kk < K2 && kk >= nb) {
                 const double ar = 2.0 * alpha2[kk - nb].x;
@@ -1531,8 +1568,9 @@ extern "C" hipError_t bse_launch_her2k(hipStream_t st, int fp32, int kind, int n
 // PART 1 updates the first tile column of the trailing block A[s:, s:] (it
 // holds the next panel's columns, so the next panel can be factored while
 // PART 2 updates every other lower tile).  Lower 64 x 64 tiles only (kinds
-// 0, 1 and 3); block x of PART 1 is tile (x, 0), of PART 2 tile (a+1, c+1)
-// for the x-th lower tile (a, c) of the remaining order.
+// 0, 1 and 3, and 4: kind 3 with the operands packed in FP32 planes, passed
+// in place of V, W unused, alpha2 zero); block x of PART 1 is tile (x, 0), of
+// PART 2 tile (a+1, c+1) for the x-th lower tile (a, c) of the remaining order.
 // ---------------------------------------------------------------------------
 __device__ inline int64_t her2k_part_idx(int part, int64_t x) {
     if (part == 1) return x * (x + 1) / 2;
@@ -1547,7 +1585,10 @@ __global__ __launch_bounds__(256) void k_her2k_part(int n, int s, T* __restrict_
                                                    const double2* __restrict__ alpha2)
 {
     const int64_t idx = her2k_part_idx(PART, blockIdx.x);
-    if constexpr (KIND == 3) her2k_mfma_body<float2, her2k_f32>(n, s, (float2*)A, V, W, ldv, j0, nb, alpha2, idx);
+    if constexpr (KIND == 4)
+        her2k_mfma_body<float2, her2k_f32, true>(n, s, (float2*)A, V, W, ldv, j0, nb, alpha2, idx,
+                                                 reinterpret_cast<const float*>(V));
+    else if constexpr (KIND == 3) her2k_mfma_body<float2, her2k_f32>(n, s, (float2*)A, V, W, ldv, j0, nb, alpha2, idx);
     else if constexpr (KIND == 1) her2k_valu_body<T>(n, s, A, V, W, ldv, j0, nb, alpha2, idx);
     else her2k_mfma_body<T>(n, s, A, V, W, ldv, j0, nb, alpha2, idx);
 }
@@ -1557,7 +1598,7 @@ extern "C" hipError_t bse_launch_her2k_part(hipStream_t st, int fp32, int kind, 
                                             const double2* alpha2)
 {
     if (s >= n) return hipSuccess;
-    if (s < j0 || ldv < n - j0 || (kind != 0 && kind != 1 && kind != 3) || (kind == 3 && !fp32) ||
+    if (s < j0 || ldv < n - j0 || (kind != 0 && kind != 1 && kind != 3 && kind != 4) || (kind >= 3 && !fp32) ||
         (part != 1 && part != 2))
         return hipErrorInvalidValue;
     const int64_t nT = tile_count(n) - (s >> 6);
@@ -1565,7 +1606,9 @@ extern "C" hipError_t bse_launch_her2k_part(hipStream_t st, int fp32, int kind, 
     if (nblk <= 0) return hipSuccess;
     const dim3 g((unsigned)nblk);
 #define KP(TT, K, P) hipLaunchKernelGGL((k_her2k_part<TT, K, P>), g, dim3(256), 0, st, n, s, (TT*)A, V, W, ldv, j0, nb, alpha2)
-    if (kind == 3) {
+    if (kind == 4) {
+        if (part == 1) KP(float2, 4, 1); else KP(float2, 4, 2);
+    } else if (kind == 3) {
         if (part == 1) KP(float2, 3, 1); else KP(float2, 3, 2);
     } else if (fp32) {
         if (kind == 1) { if (part == 1) KP(float2, 1, 1); else KP(float2, 1, 2); }
