@@ -298,25 +298,39 @@ def main():
     log(f"profile: {prof}")
     asm_ms = ctx.stage_times()["assembly"]
 
+    # measured instruction ceilings (no memory traffic) beside the published peaks
+    rates = {}
+    for form in ("mfma_f32_16x16x4", "mfma_f64_16x16x4", "fma_f64"):
+        rates[form] = _abi.probe_rate(form)
+    log(f"instruction rates (TFLOP/s): {rates}")
     out_fp64 = {}
     if not args.no_fp64:
         # FP64 with the lowest 16 vectors (the like-for-like comparison) ...
-        t = time.perf_counter()
-        ctx.build("fp64")
-        ctx.eigh(n_low)
-        t_fp64_16 = time.perf_counter() - t
+        # Each FP64 path runs twice and the second (warm) pass is reported:
+        # the first call of a path tunes its kernels for this device.
+        cold = {}
+        for rep in range(2):
+            t = time.perf_counter()
+            ctx.build("fp64")
+            ctx.eigh(n_low)
+            t_fp64_16 = time.perf_counter() - t
+            cold.setdefault("fp64_lowest16_s", t_fp64_16)
         stages_fp64_16 = ctx.stage_times()
         E64_16 = ctx.eigenvalues()
-        log(f"fp64 lowest-16 path: {t_fp64_16:.3f} s, stages {stages_fp64_16}")
+        log(f"fp64 lowest-16 path: {t_fp64_16:.3f} s (first call {cold['fp64_lowest16_s']:.3f} s), "
+            f"stages {stages_fp64_16}")
         # ... and with all N vectors (what eig_sym returns in the reference)
-        t = time.perf_counter()
-        ctx.build("fp64")
-        ctx.eigh(-1)
-        t_fp64 = time.perf_counter() - t
+        for rep in range(2):
+            t = time.perf_counter()
+            ctx.build("fp64")
+            ctx.eigh(-1)
+            t_fp64 = time.perf_counter() - t
+            cold.setdefault("fp64_all_vectors_s", t_fp64)
         stages_fp64 = ctx.stage_times()
         E64 = ctx.eigenvalues()
         eps64 = ctx.spectrum(omegas, s.eta)
-        log(f"fp64 all-vector path: {t_fp64:.3f} s, stages {stages_fp64}")
+        log(f"fp64 all-vector path: {t_fp64:.3f} s (first call {cold['fp64_all_vectors_s']:.3f} s), "
+            f"stages {stages_fp64}")
         nchk = 256
         Z64 = ctx.eigenvectors(nchk)
         scale = float(np.abs(E64).max())
@@ -326,6 +340,8 @@ def main():
         R = ctx.apply(Z64[:, :64]) - Z64[:, :64] * E64[None, :64]
         out_fp64 = dict(
             fp64_all_vectors_s=t_fp64, fp64_lowest16_s=t_fp64_16,
+            fp64_first_call_s=dict(cold, note="first call of each FP64 path on the context, "
+                                   "including its one-time kernel tuning"),
             speedup_vs_fp64_all_vectors=t_fp64 / t_mixed, speedup_vs_fp64_lowest16=t_fp64_16 / t_mixed,
             lowest16_err_meV_vs_fp64=err_mev,
             all_eigenvalues_max_err_meV_vs_fp64=float(np.abs(E_mixed - E64).max() * 1e3),
@@ -335,9 +351,25 @@ def main():
             fp64_residual_lowest64_rel=float(np.linalg.norm(R, axis=0).max() / scale),
             fp64_orthonormality_lowest256=float(np.abs(Z64.conj().T @ Z64 - np.eye(nchk)).max()),
             stages_ms_fp64_all_vectors=stages_fp64, stages_ms_fp64_lowest16=stages_fp64_16)
+        # the all-vector back-transform Z <- Q Z: 8 N^2 nvec algorithmic flops
+        # (V^H Z and Z - V (T W) over the reflectors of every column), FP64
+        # matrix cores, four panels of 32 reflectors per pass
+        bt_ms = stages_fp64.get("back_transform", 0.0)
+        bt_fl = 8.0 * float(N) ** 3
+        bt_tf = bt_fl / (bt_ms * 1e-3) / 1e12 if bt_ms > 0 else 0.0
+        out_fp64["fp64_backtransform"] = dict(
+            kernels="k_btg_w1 + k_btg_upd (+ k_btg_w2, k_btg_gram, k_btg_tbig)", ms=bt_ms,
+            algorithmic_flops=bt_fl, tflops=bt_tf, peak_tflops=78.6, frac=bt_tf / 78.6,
+            ceiling_tflops_measured=rates.get("mfma_f64_16x16x4"),
+            frac_of_ceiling=bt_tf / rates["mfma_f64_16x16x4"] if rates.get("mfma_f64_16x16x4") else None)
 
     traffic, traffic_src = measured_traffic(args.mixed, N)
     roof = roofline_entry(prof, traffic, traffic_src)
+    if roof.get("bound") == "mfma" and args.mixed != "fp64":
+        # FP32 storage: the panel kernels run v_mfma_f32_16x16x4
+        ceil = rates["mfma_f32_16x16x4"]
+        roof["ceiling_measured"] = dict(instruction="v_mfma_f32_16x16x4", tflops=ceil,
+                                        frac_of_ceiling=roof["achieved"] / ceil if ceil > 0 else None)
     # the assembly kernel: algorithmic MFMA flops of the two chained complex
     # contractions per (k, k') pair of the lower triangle, and the FP32
     # complex lower triangle it writes

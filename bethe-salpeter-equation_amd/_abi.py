@@ -28,7 +28,7 @@ ABI_SYMBOLS = (
     "bse_batch_prepare", "bse_batch_run", "bse_batch_results", "bse_batch_info",
     "bse_grad_spectrum", "bse_refine_info", "bse_set_problem_klist", "bse_tb_states_at",
     "bse_eigh_dense_noalloc", "bse_noalloc_info", "bse_build_full", "bse_eigh_full", "bse_get_coupling",
-    "bse_wannier_integrals",
+    "bse_wannier_integrals", "bse_probe_rate",
 )
 
 
@@ -164,6 +164,7 @@ def load_library(path: pathlib.Path | str | None = None):
         "bse_get_coupling": (i32, [vp, dp, i64]),
         "bse_wannier_integrals": (i32, [vp, C.POINTER(WannierGridC), i32, C.POINTER(C.c_int32), i32, dp,
                                         dp, dp, dp]),
+        "bse_probe_rate": (i32, [i32, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -175,6 +176,20 @@ def load_library(path: pathlib.Path | str | None = None):
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+PROBE_FORMS = {"mfma_f32_16x16x4": 0, "mfma_f64_16x16x4": 1, "fma_f64": 2}
+
+
+def probe_rate(form: str) -> float:
+    """TFLOP/s one instruction form reaches on the current GPU with no memory
+    traffic (bse_probe_rate): the measured ceiling beside the published peak."""
+    lib = load_library()
+    out = C.c_double(0.0)
+    rc = lib.bse_probe_rate(PROBE_FORMS[form], C.byref(out))
+    if rc != 0:
+        raise BSEError(f"bse_probe_rate failed ({rc})")
+    return float(out.value)
 
 
 STAGES = ("tb", "tables", "assembly", "tridiagonalisation", "tridiagonal_values",
@@ -468,9 +483,13 @@ class Context:
         if int(out[4]) == 1:
             f32 = int(out[1]) == 1
             tag = "FP32 MFMA" if f32 else "FP64 products"
-            look = int(out[8]) == 3
-            kname = ("k_her2k_part<float2, 3, 2>" if look else "k_her2k_mfma32") if f32 else "her2k"
+            look = int(out[8]) >= 3
+            packed = int(out[8]) == 4
+            kname = ((f"k_her2k_part<float2, {4 if packed else 3}, 2>" if look else "k_her2k_mfma32")
+                     if f32 else "her2k")
             part = ", tiles beyond the next panel's tile column" if look else ""
+            if packed:
+                part += ", operands packed in FP32 planes"
             return dict(kind="flops", kernel=f"band rank-2k update ({kname}, {tag}{part})",
                         ms=out[0], flops=out[3], launches=int(out[2]), peak=157.3 if f32 else 78.6,
                         second=dict(kernel=f"{'k_band_hemm32' if f32 else 'k_band_hemm'} (X = A22 V, {tag})",

@@ -97,7 +97,7 @@ extern "C" int bse_profile(bse_ctx* ctx, int enable, double* out) {
             out[5] = ctx->hemm_ms;
             out[6] = ctx->hemm_flops;
             out[7] = (double)ctx->band_launches;
-            out[8] = ctx->band_lookahead ? 3.0 : 2.0;
+            out[8] = ctx->band_lookahead ? (ctx->band_pack ? 4.0 : 3.0) : 2.0;
         } else {
             out[0] = ctx->hemv_ms;
             out[1] = ctx->hemv_bytes;
@@ -755,6 +755,7 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
         // in FP32 planes (BSE_HER2K_PACK=0 reads the FP64 panels per tile)
         const char* pke = getenv("BSE_HER2K_PACK");
         const bool pack = look && kind == 3 && !(pke && pke[0] == '0');
+        ctx->band_pack = pack;
         hipStream_t ss = ctx->side;
         double2* qw = rho ? BUF(double2, S_W1, bse_apply_q_thin_workspace(n)) : nullptr;
         y = rho ? BUF(double2, S_Y, n) : nullptr;

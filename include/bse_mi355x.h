@@ -296,7 +296,8 @@ int  bse_stage_times(const bse_ctx* ctx, double* ms, int n);
  *     (8 m^2 k), [7] launches of the Hermitian block product, [8] = 2; [8] = 3
  *     when the look-ahead updated each next panel's tile column on the second
  *     stream, and [0], [3] then cover the other tiles (8 m_r (m_r+1) k, m_r
- *     the order beyond that tile column). */
+ *     the order beyond that tile column); [8] = 4 as 3, with the update's
+ *     operands packed once per panel in FP32 planes. */
 int  bse_profile(bse_ctx* ctx, int enable, double* out);
 
 /* ---- K11: batched sweep of many small systems ------------------------------
@@ -384,6 +385,15 @@ int bse_eigh_dense_noalloc(bse_ctx* ctx, int n, const double* a, int64_t lda, in
 /* info[0] bytes of the fixed array, [1] largest n, [2] bytes the last
  * allocation-free call used. */
 int bse_noalloc_info(const bse_ctx* ctx, int64_t info[3]);
+
+/* ---- instruction-rate probe ----------------------------------------------
+ * No counterpart in the reference; for the roofline report.  The rate in
+ * TFLOP/s that one instruction form reaches on the current device with no
+ * memory traffic (best of three launches of a few ms, on the null stream):
+ * which = 0 v_mfma_f32_16x16x4 (the band reduction's products), 1
+ * v_mfma_f64_16x16x4 (FP64 back-transform and matrix-free apply), 2 FP64
+ * vector FMA.  Allocates and frees a small device buffer. */
+int bse_probe_rate(int which, double* tflops);
 
 #ifdef __cplusplus
 }

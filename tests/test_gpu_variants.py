@@ -10,6 +10,8 @@ vectors) and in FP32 storage (16 vectors), against LAPACK eigenvalues.
 Tolerances: FP64 1e-12 x max|lambda| x sqrt(n), residual 1e-11, orthogonality
 1e-10; FP32 storage 1e-5 x max|lambda|.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -63,3 +65,17 @@ def test_trailing_update_kind(problem, monkeypatch, kind, prec):
     with _abi.Context(0) as ctx:
         ev, z = ctx.eigh_dense(a, prec, -1 if prec == "fp64" else 16)
     _check(a, ref, prec, ev, z)
+
+
+def test_instruction_rate_probe():
+    """The instruction-rate probe (a pure arithmetic loop) reports a positive
+    rate below the published peak of each form (FP32 MFMA 157.3, FP64 MFMA
+    and FP64 FMA 78.6 TFLOP/s); an unknown form is refused."""
+    peaks = {"mfma_f32_16x16x4": 157.3, "mfma_f64_16x16x4": 78.6, "fma_f64": 78.6}
+    for form, peak in peaks.items():
+        r = _abi.probe_rate(form)
+        print(f"{form}: {r:.1f} TFLOP/s", flush=True)
+        assert 0.05 * peak < r < 1.05 * peak
+    lib = _abi.load_library()
+    out = ctypes.c_double(0.0)
+    assert lib.bse_probe_rate(7, ctypes.byref(out)) != 0
