@@ -898,14 +898,24 @@ static int twostage_tridiag(bse_ctx* ctx, int n, void* H, int fp32, const double
 
 // Eigenvectors of the lowest nstore eigenvalues (d_lam, ascending) after a
 // two-stage reduction: inverse iteration on the band matrix, then Z = Q1 X.
+// prep (may be null): recorded once bse_apply_q_grouped_prepare has formed
+// the T of every four band panels in gws (panel offsets at voff_d); Z = Q1 X
+// then runs four panels per pass, else one panel and 16 columns per pass.
 static int twostage_vectors(bse_ctx* ctx, hipStream_t st, int n, const double* d_lam, int nstore, double tnorm,
-                            double2* Z) {
+                            double2* Z, hipEvent_t prep = nullptr, const int64_t* voff_d = nullptr,
+                            double2* gws = nullptr, int64_t gws_el = 0) {
     const int b = kBand, ldab = 2 * b + 1;
     double2* ws = BUF(double2, S_BII, bse_band_ii_workspace(n, b, nstore));
     HCK(bse_launch_band_ii(st, n, b, ldab, (const double2*)ctx->bufs[S_BAB0].p, d_lam, nstore, tnorm, ws, Z, n));
-    double2* qw = BUF(double2, S_W1, bse_apply_q_thin_workspace(n));
     const double2* Vst = (const double2*)ctx->bufs[S_VSTORE].p;
     const double2* Tall = (const double2*)ctx->bufs[S_TALL].p;
+    if (prep) {
+        HCK(hipStreamWaitEvent(st, prep, 0));
+        HCK(bse_apply_q_grouped_run(st, n, (int)ctx->band_panels.size(), b, 0, Vst, voff_d, Z, n, nstore, gws,
+                                    gws_el));
+        return BSE_OK;
+    }
+    double2* qw = BUF(double2, S_W1, bse_apply_q_thin_workspace(n));
     for (int p = (int)ctx->band_panels.size() - 1; p >= 0; --p) {
         const bse_ctx::BandPanel& bp = ctx->band_panels[p];
         for (int c0 = 0; c0 < nstore; c0 += 16) {
@@ -1175,10 +1185,34 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
     // eigenvalues, so they run on the side stream beside the tridiagonal
     // vectors of the oscillator contraction; st joins before the refinement
     const bool vec_side = two && nstore > 0;
+    std::vector<int64_t> bvoff;
     if (vec_side) {
         HCK(hipEventRecord(ctx->fork_ev, st));
+        // Z = Q1 X one panel and 16 columns per pass (k_pq_*: about 4000
+        // small launches that pipeline behind each other).  BSE_BT16_GROUPED=1:
+        // four band panels per pass on FP64 matrix cores, the T of every group
+        // formed on st and awaited by the side stream after the band inverse
+        // iteration (measured 8 ms slower at N = 20736 with 20 vectors: a
+        // V^H Z launch then has about 32 workgroups).
+        const int npb = (int)ctx->band_panels.size();
+        hipEvent_t prep = nullptr;
+        int64_t* bvoff_d = nullptr;
+        double2* gws = nullptr;
+        int64_t gws_el = 0;
+        if (npb > 0 && getenv("BSE_BT16_GROUPED")) {
+            bvoff.resize(npb);
+            for (int p = 0; p < npb; ++p) bvoff[p] = ctx->band_panels[p].voff;
+            bvoff_d = BUF(int64_t, S_BTVOFF, npb);
+            gws_el = bse_apply_q_grouped_workspace(n, npb, nstore);
+            gws = BUF(double2, S_BTWS, gws_el);
+            HCK(hipMemcpyAsync(bvoff_d, bvoff.data(), sizeof(int64_t) * npb, hipMemcpyHostToDevice, st));
+            HCK(bse_apply_q_grouped_prepare(st, n, npb, kBand, 0, (const double2*)ctx->bufs[S_VSTORE].p, bvoff_d,
+                                            (const double2*)ctx->bufs[S_TALL].p, gws));
+            HCK(hipEventRecord(ctx->la_ev[0], st));
+            prep = ctx->la_ev[0];
+        }
         HCK(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
-        int rc2 = twostage_vectors(ctx, ctx->side, n, d_lam, nstore, tnorm, Z);
+        int rc2 = twostage_vectors(ctx, ctx->side, n, d_lam, nstore, tnorm, Z, prep, bvoff_d, gws, gws_el);
         if (rc2) return rc2;
         HCK(hipEventRecord(ctx->join_ev, ctx->side));
     }

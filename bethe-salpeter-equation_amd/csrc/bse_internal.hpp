@@ -92,6 +92,10 @@ hipError_t bse_launch_rpa_table(hipStream_t, int, int, int, int, int, int, const
 hipError_t bse_apply_q_strips(hipStream_t, int, int, int, const double2*, const int64_t*, const double2*,
                               double2*, int64_t, int);
 int64_t bse_apply_q_grouped_workspace(int, int, int);
+hipError_t bse_apply_q_grouped_prepare(hipStream_t, int, int, int, int, const double2*, const int64_t*,
+                                       const double2*, double2*);
+hipError_t bse_apply_q_grouped_run(hipStream_t, int, int, int, int, const double2*, const int64_t*, double2*,
+                                   int64_t, int, double2*, int64_t);
 hipError_t bse_apply_q_grouped(hipStream_t, int, int, int, const double2*, const int64_t*, const double2*, double2*,
                                int64_t, int, double2*, int64_t);
 int64_t bse_band_update_workspace(int, int, int);

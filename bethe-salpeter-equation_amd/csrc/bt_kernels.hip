@@ -247,16 +247,21 @@ __device__ __forceinline__ void btg_zero(btg_d4 (&cr)[MT][NT], btg_d4 (&ci)[MT][
         }
 }
 
+// Panel p starts at row rb + 32 p and holds min(32, n - co - (rb + 32 p))
+// reflectors: rb = 0, co = 1 for the one-stage reduction (reflector j acts
+// below row j), rb = 32, co = 0 for the band reduction's panels (panel p
+// factors rows 32 (p + 1) ..).
 struct BtgGroup {
-    int p0, gq, j0, m;
-    __device__ BtgGroup(int n, int npan, int g) {
+    int p0, gq, j0, m, co;
+    __device__ BtgGroup(int n, int npan, int g, int rb, int co_) {
         p0 = g * BTG_P;
         gq = min(BTG_P, npan - p0);
-        j0 = p0 * BT_NB;
+        j0 = rb + p0 * BT_NB;
         m = n - j0;
+        co = co_;
     }
     // reflectors of panel q of the group
-    __device__ int nbp(int n, int q) const { return min(BT_NB, n - 1 - j0 - BT_NB * q); }
+    __device__ int nbp(int n, int q) const { return min(BT_NB, n - co - j0 - BT_NB * q); }
 };
 
 // Reduction over group rows r in [rlo, rhi) of conj(Va[r][a]) Vb[r][b] or
@@ -301,13 +306,13 @@ __device__ __forceinline__ void btg_rows(btg_d4 (&cr)[2][2], btg_d4 (&ci)[2][2],
 
 // Gram blocks V_i^H V_q (32 x 32, column-major) of every group: grid
 // (6 pairs, groups); the four waves split the rows, summed in wave order.
-__global__ __launch_bounds__(256) void k_btg_gram(int n, int npan, const double2* __restrict__ Vst,
+__global__ __launch_bounds__(256) void k_btg_gram(int n, int npan, int rb, int co, const double2* __restrict__ Vst,
                                                   const int64_t* __restrict__ voff, double2* __restrict__ Gram)
 {
     __shared__ double2 sR[4][BT_NB * BT_NB];
     const int pi[6] = {0, 0, 1, 0, 1, 2}, pq[6] = {1, 2, 2, 3, 3, 3};
     const int pair = blockIdx.x, g = blockIdx.y;
-    const BtgGroup G(n, npan, g);
+    const BtgGroup G(n, npan, g, rb, co);
     const int i = pi[pair], q = pq[pair];
     if (q >= G.gq) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lq = lane >> 4;
@@ -349,14 +354,14 @@ __global__ __launch_bounds__(256) void k_btg_gram(int n, int npan, const double2
 
 // T of every group (128 x 128, column-major, zero below the diagonal and for
 // missing reflectors): one workgroup per group.
-__global__ __launch_bounds__(256) void k_btg_tbig(int n, int npan, const double2* __restrict__ Tall,
+__global__ __launch_bounds__(256) void k_btg_tbig(int n, int npan, int rb, int co, const double2* __restrict__ Tall,
                                                   const double2* __restrict__ Gram, double2* __restrict__ Tbig)
 {
     __shared__ double2 sS[3 * BT_NB][BT_NB + 1];
     __shared__ double2 sY[3 * BT_NB][BT_NB + 1];
     __shared__ double2 sT[BT_NB][BT_NB + 1];
     const int g = blockIdx.x, tid = threadIdx.x;
-    const BtgGroup G(n, npan, g);
+    const BtgGroup G(n, npan, g, rb, co);
     double2* T = Tbig + (int64_t)g * BTG_K * BTG_K;
     for (int e = tid; e < BTG_K * BTG_K; e += 256) {
         const int k = e % BTG_K, kp = e / BTG_K, q = k / BT_NB;
@@ -401,11 +406,11 @@ __global__ __launch_bounds__(256) void k_btg_tbig(int n, int npan, const double2
 
 // W1p[s][k][c] = sum over group rows r of chunk s of conj(V[r][k]) Z[j0 + r][c]
 // (ld ncols).  Grid (column tiles of 32, chunks); wave w: panel w of the group.
-__global__ __launch_bounds__(256) void k_btg_w1(int n, int npan, int g, const double2* __restrict__ Vst,
+__global__ __launch_bounds__(256) void k_btg_w1(int n, int npan, int rb, int co, int g, const double2* __restrict__ Vst,
                                                 const int64_t* __restrict__ voff, const double2* __restrict__ Z,
                                                 int64_t ldz, int ncols, int rchunk, double2* __restrict__ W1p)
 {
-    const BtgGroup G(n, npan, g);
+    const BtgGroup G(n, npan, g, rb, co);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lq = lane >> 4;
     const int c0 = blockIdx.x * 32, s = blockIdx.y;
     const int q = w;
@@ -482,11 +487,11 @@ __global__ __launch_bounds__(256) void k_btg_w2(int nsplit, const double2* __res
 // (output rows = columns c, output columns = rows r, so 16 lanes of one
 // register hold 16 consecutive rows of Z).  Grid (column tiles of 64, row
 // tiles of 64); wave w: columns 32 (w & 1) .., rows 32 (w >> 1) ..
-__global__ __launch_bounds__(256) void k_btg_upd(int n, int npan, int g, const double2* __restrict__ Vst,
+__global__ __launch_bounds__(256) void k_btg_upd(int n, int npan, int rb, int co, int g, const double2* __restrict__ Vst,
                                                  const int64_t* __restrict__ voff, const double2* __restrict__ W2,
                                                  double2* __restrict__ Z, int64_t ldz, int ncols)
 {
-    const BtgGroup G(n, npan, g);
+    const BtgGroup G(n, npan, g, rb, co);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lq = lane >> 4;
     const int c0 = blockIdx.x * 64 + 32 * (w & 1), r0 = blockIdx.y * 64 + 32 * (w >> 1);
     btg_d4 cr[2][2], ci[2][2];
@@ -551,15 +556,62 @@ __global__ __launch_bounds__(256) void k_btg_upd(int n, int npan, int g, const d
 }
 
 static int btg_nsplit(int ncols) {
-    // about 4096 workgroups for the V^H Z partials of a full-height group
-    return (int)std::max<int64_t>(1, ceil_div64(4096, ceil_div64(ncols, 32)));
+    // about 4096 workgroups for the V^H Z partials of a full-height group,
+    // at most 32 partials for the reduction to sum
+    return (int)std::min<int64_t>(32, std::max<int64_t>(1, ceil_div64(4096, ceil_div64(ncols, 32))));
 }
 
-// Workspace (complex elements) of bse_apply_q_grouped.
+// Workspace (complex elements) of the grouped back-transform: the Gram blocks
+// and T of every group, then W2 and the W1 partials for ncols columns.
 extern "C" int64_t bse_apply_q_grouped_workspace(int n, int npan, int ncols)
 {
     const int64_t ng = ceil_div64(npan, BTG_P);
     return ng * 6 * BT_NB * BT_NB + ng * BTG_K * BTG_K + (int64_t)(btg_nsplit(ncols) + 1) * BTG_K * ncols;
+}
+
+// T of every group of four panels into the workspace (independent of Z).
+// Panels as for BtgGroup (rb, co); panel p at Vst + voff_d[p] with ld
+// n - rb - 32 p, T_p at Tall + p 32 32.
+extern "C" hipError_t bse_apply_q_grouped_prepare(hipStream_t st, int n, int npan, int rb, int co,
+                                                  const double2* Vst, const int64_t* voff_d, const double2* Tall,
+                                                  double2* ws)
+{
+    if (npan <= 0) return hipSuccess;
+    const int ng = (int)ceil_div64(npan, BTG_P);
+    double2* gram = ws;
+    double2* tbig = gram + (int64_t)ng * 6 * BT_NB * BT_NB;
+    hipLaunchKernelGGL(k_btg_gram, dim3(6, ng), dim3(256), 0, st, n, npan, rb, co, Vst, voff_d, gram);
+    hipLaunchKernelGGL(k_btg_tbig, dim3(ng), dim3(256), 0, st, n, npan, rb, co, Tall, gram, tbig);
+    return hipGetLastError();
+}
+
+// Z (n x ncols, ld ldz) <- Q Z, last group first, with the T blocks that
+// bse_apply_q_grouped_prepare left in ws.
+extern "C" hipError_t bse_apply_q_grouped_run(hipStream_t st, int n, int npan, int rb, int co, const double2* Vst,
+                                              const int64_t* voff_d, double2* Z, int64_t ldz, int ncols,
+                                              double2* ws, int64_t ws_el)
+{
+    if (ncols <= 0 || npan <= 0) return hipSuccess;
+    if (ws_el < bse_apply_q_grouped_workspace(n, npan, ncols)) return hipErrorInvalidValue;
+    const int ng = (int)ceil_div64(npan, BTG_P);
+    double2* tbig = ws + (int64_t)ng * 6 * BT_NB * BT_NB;
+    double2* w2 = tbig + (int64_t)ng * BTG_K * BTG_K;
+    double2* w1p = w2 + (int64_t)BTG_K * ncols;
+    const int ns = btg_nsplit(ncols);
+    const unsigned c32 = (unsigned)ceil_div64(ncols, 32), c64 = (unsigned)ceil_div64(ncols, 64);
+    for (int g = ng - 1; g >= 0; --g) {
+        const int m = n - rb - g * BTG_K;
+        // chunks of at least 64 rows, a multiple of 4 BTG_KB
+        const int rchunk = (int)std::max<int64_t>(64, ceil_div64(ceil_div64(m, ns), 4 * BTG_KB) * 4 * BTG_KB);
+        const int nsg = (int)ceil_div64(m, rchunk);
+        hipLaunchKernelGGL(k_btg_w1, dim3(c32, nsg), dim3(256), 0, st, n, npan, rb, co, g, Vst, voff_d, Z, ldz,
+                           ncols, rchunk, w1p);
+        hipLaunchKernelGGL(k_btg_w2, dim3(c32), dim3(256), 0, st, nsg, tbig + (int64_t)g * BTG_K * BTG_K, w1p, ncols,
+                           w2);
+        hipLaunchKernelGGL(k_btg_upd, dim3(c64, (unsigned)ceil_div64(m, 64)), dim3(256), 0, st, n, npan, rb, co, g,
+                           Vst, voff_d, w2, Z, ldz, ncols);
+    }
+    return hipGetLastError();
 }
 
 // Z (n x ncols, ld ldz) <- Q Z for the one-stage panels (nb = 32; panel p at
@@ -571,26 +623,7 @@ extern "C" hipError_t bse_apply_q_grouped(hipStream_t st, int n, int nb, int npa
 {
     if (nb != BT_NB || ncols <= 0 || npan <= 0) return nb != BT_NB ? hipErrorInvalidValue : hipSuccess;
     if (ws_el < bse_apply_q_grouped_workspace(n, npan, ncols)) return hipErrorInvalidValue;
-    const int ng = (int)ceil_div64(npan, BTG_P);
-    double2* gram = ws;
-    double2* tbig = gram + (int64_t)ng * 6 * BT_NB * BT_NB;
-    double2* w2 = tbig + (int64_t)ng * BTG_K * BTG_K;
-    double2* w1p = w2 + (int64_t)BTG_K * ncols;
-    hipLaunchKernelGGL(k_btg_gram, dim3(6, ng), dim3(256), 0, st, n, npan, Vst, voff_d, gram);
-    hipLaunchKernelGGL(k_btg_tbig, dim3(ng), dim3(256), 0, st, n, npan, Tall, gram, tbig);
-    const int ns = btg_nsplit(ncols);
-    const unsigned c32 = (unsigned)ceil_div64(ncols, 32), c64 = (unsigned)ceil_div64(ncols, 64);
-    for (int g = ng - 1; g >= 0; --g) {
-        const int m = n - g * BTG_K;
-        // chunks of at least 64 rows, a multiple of 4 BTG_KB
-        const int rchunk = (int)std::max<int64_t>(64, ceil_div64(ceil_div64(m, ns), 4 * BTG_KB) * 4 * BTG_KB);
-        const int nsg = (int)ceil_div64(m, rchunk);
-        hipLaunchKernelGGL(k_btg_w1, dim3(c32, nsg), dim3(256), 0, st, n, npan, g, Vst, voff_d, Z, ldz, ncols,
-                           rchunk, w1p);
-        hipLaunchKernelGGL(k_btg_w2, dim3(c32), dim3(256), 0, st, nsg, tbig + (int64_t)g * BTG_K * BTG_K, w1p, ncols,
-                           w2);
-        hipLaunchKernelGGL(k_btg_upd, dim3(c64, (unsigned)ceil_div64(m, 64)), dim3(256), 0, st, n, npan, g, Vst,
-                           voff_d, w2, Z, ldz, ncols);
-    }
-    return hipGetLastError();
+    hipError_t e = bse_apply_q_grouped_prepare(st, n, npan, 0, 1, Vst, voff_d, Tall, ws);
+    if (e != hipSuccess) return e;
+    return bse_apply_q_grouped_run(st, n, npan, 0, 1, Vst, voff_d, Z, ldz, ncols, ws, ws_el);
 }

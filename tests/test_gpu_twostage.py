@@ -179,3 +179,25 @@ def test_two_stage_lookahead_is_exact(ctx2, prec):
     ev1, z1 = ctx2.eigh_dense(a, prec, 8)
     assert np.array_equal(ev0, ev1)
     assert np.array_equal(z0, z1)
+
+
+@pytest.mark.parametrize("n", [65, 333, 1000, 2049])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_two_stage_grouped_q1_matches_panel_order(ctx2, monkeypatch, n, prec):
+    """Z = Q1 X four band panels per pass (BSE_BT16_GROUPED=1: FP64 matrix
+    cores, the T of each group of four formed on the main stream) against the
+    default one panel and 16 columns per pass: the same eigenvalues bit for bit
+    (Q1 does not touch them) and the same vectors to FP64 rounding.  Band
+    panels: 1 (n = 65), 10, 30 and 63 (n = 2049: the last group three panels,
+    the last panel one reflector)."""
+    a = _herm(n, 11)
+    k = min(16, n)
+    print(f"grouped Q1 vs panels n={n} {prec}", flush=True)
+    monkeypatch.setenv("BSE_BT16_GROUPED", "1")
+    ev_g, z_g = ctx2.eigh_dense(a, prec, k)
+    monkeypatch.delenv("BSE_BT16_GROUPED")
+    ev_p, z_p = ctx2.eigh_dense(a, prec, k)
+    assert np.array_equal(ev_g, ev_p)
+    assert np.abs(z_g - z_p).max() < 1e-12
+    r = a @ z_g - z_g * ev_g[None, :k]
+    assert np.abs(r).max() < (1e-10 if prec == "fp64" else 1e-4) * np.abs(ev_g).max()
