@@ -764,7 +764,9 @@ __global__ __launch_bounds__(256) void k_cq_chol2(int m, int b, int ng, const do
 // a diagonal tile is zero (storage invariant).
 // 16 x 16 threads: rows tx + 16 a (a < 4), columns ty + 16 c (c < 2).
 // ---------------------------------------------------------------------------
-#define HEMM_KSPLIT 8
+#ifndef HEMM_KSPLIT
+#define HEMM_KSPLIT 16
+#endif
 template <typename T>
 __global__ __launch_bounds__(256) void k_band_hemm(int n, int s, int k, const T* __restrict__ A,
                                                    const double2* __restrict__ V, int64_t ldv,
