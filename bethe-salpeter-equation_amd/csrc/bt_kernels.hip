@@ -207,7 +207,12 @@ extern "C" hipError_t bse_apply_q_strips(hipStream_t st, int n, int nb, int npan
 typedef double btg_d4 __attribute__((ext_vector_type(4)));
 #define BTG_P 4                   // panels per group
 #define BTG_K (BTG_P * BT_NB)     // reflectors per group
+#ifndef BTG_KB
 #define BTG_KB 4                  // K steps per prefetched block
+#endif
+#ifndef BTG_WG
+#define BTG_WG 4096               // target workgroups of a full-height V^H Z launch
+#endif
 
 __device__ __forceinline__ btg_d4 btg_mfma(double a, double b, btg_d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -556,9 +561,9 @@ __global__ __launch_bounds__(256) void k_btg_upd(int n, int npan, int rb, int co
 }
 
 static int btg_nsplit(int ncols) {
-    // about 4096 workgroups for the V^H Z partials of a full-height group,
+    // about BTG_WG workgroups for the V^H Z partials of a full-height group,
     // at most 32 partials for the reduction to sum
-    return (int)std::min<int64_t>(32, std::max<int64_t>(1, ceil_div64(4096, ceil_div64(ncols, 32))));
+    return (int)std::min<int64_t>(32, std::max<int64_t>(1, ceil_div64(BTG_WG, ceil_div64(ncols, 32))));
 }
 
 // Workspace (complex elements) of the grouped back-transform: the Gram blocks
