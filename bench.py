@@ -198,19 +198,29 @@ def sweep_bench(args, pkg):
         log(f"sweep step {k}: {tot:.2f} ms")
     ms = float(np.mean(tots))
     per = np.concatenate(pers)
-    ev, _ = ctx.batch_results()
+    ev, eps = ctx.batch_results()
     cpu = None
+    check = None
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import bse_oracle as O
-        sample = sorted(systems, key=lambda x: x.n_bse)[::8]       # 8 systems across the size range
+        order = sorted(range(len(systems)), key=lambda i: systems[i].n_bse)[::8]   # 8 systems across the sizes
+        sample = [systems[i] for i in order]
         t0 = time.perf_counter()
-        for sy in sample:
-            O.solve(O.problem_from_system(sy))
+        sols = [O.solve(O.problem_from_system(sy)) for sy in sample]
         dt = time.perf_counter() - t0
         cpu = dict(value=len(sample) / dt, unit="systems/s", cores=os.cpu_count(), kind="port",
                    sample=f"FP64 oracle (numpy, zheevd) on {len(sample)} of the 64 systems "
                           f"(every 8th by size, N={[x.n_bse for x in sample]}): {dt:.1f} s")
+        # the same systems' GPU results against the oracle solutions just timed
+        e_err, s_err = 0.0, 0.0
+        for i, sol in zip(order, sols):
+            scale = float(np.abs(sol["E"]).max())
+            e_err = max(e_err, float(np.abs(ev[i] - sol["E"][:16]).max() / scale))
+            ref = sol["eps"].imag
+            s_err = max(s_err, float(np.linalg.norm(eps[i].imag - ref) / np.linalg.norm(ref)))
+        check = dict(systems=len(sample), lowest16_max_rel_err=e_err, spectrum_max_rel_l2=s_err,
+                     note="GPU batch results of the CPU-timed sample against the FP64 oracle")
     res = {
         "metric": "batched sweep throughput (64 uneven honeycomb systems, one graph launch)",
         "value": 64.0 / (ms * 1e-3), "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
@@ -222,6 +232,7 @@ def sweep_bench(args, pkg):
         "per_system_ms": {"p50": float(np.percentile(per, 50)), "p99": float(np.percentile(per, 99))},
         "batch_info": ctx.batch_info(),
         "lowest_exciton_eV": {"min": float(ev[:, 0].min()), "max": float(ev[:, 0].max())},
+        "accuracy_vs_oracle": check,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))
