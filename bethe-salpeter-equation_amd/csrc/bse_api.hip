@@ -907,6 +907,9 @@ static int twostage_vectors(bse_ctx* ctx, hipStream_t st, int n, const double* d
     const int b = kBand, ldab = 2 * b + 1;
     double2* ws = BUF(double2, S_BII, bse_band_ii_workspace(n, b, nstore));
     HCK(bse_launch_band_ii(st, n, b, ldab, (const double2*)ctx->bufs[S_BAB0].p, d_lam, nstore, tnorm, ws, Z, n));
+    // the band vectors are formed: columns beyond the first 16 may be
+    // back-transformed by another stream (twostage_vectors_rest)
+    HCK(hipEventRecord(ctx->la_ev[1], st));
     const double2* Vst = (const double2*)ctx->bufs[S_VSTORE].p;
     const double2* Tall = (const double2*)ctx->bufs[S_TALL].p;
     if (prep) {
@@ -916,9 +919,30 @@ static int twostage_vectors(bse_ctx* ctx, hipStream_t st, int n, const double* d
         return BSE_OK;
     }
     double2* qw = BUF(double2, S_W1, bse_apply_q_thin_workspace(n));
+    // the first 16 columns here; the rest by twostage_vectors_rest on the
+    // stream that joins this one (it would otherwise only wait)
+    const int kc = std::min(16, nstore);
     for (int p = (int)ctx->band_panels.size() - 1; p >= 0; --p) {
         const bse_ctx::BandPanel& bp = ctx->band_panels[p];
-        for (int c0 = 0; c0 < nstore; c0 += 16) {
+        HCK(bse_apply_q_panel(st, bp.m, bp.k, Vst + bp.voff, Tall + (size_t)p * b * b, b, kc, Z + bp.s, n, 0, qw));
+    }
+    return BSE_OK;
+}
+
+// Z = Q1 X for the columns 16 .. nstore - 1 of the band vectors, on stream st
+// once twostage_vectors' band inverse iteration is done (its own workspace,
+// so the two streams' column groups run side by side; each group's result
+// is the same as in one stream).
+static int twostage_vectors_rest(bse_ctx* ctx, hipStream_t st, int n, int nstore, double2* Z) {
+    if (nstore <= 16) return BSE_OK;
+    const int b = kBand;
+    HCK(hipStreamWaitEvent(st, ctx->la_ev[1], 0));
+    double2* qw = BUF(double2, S_W2, bse_apply_q_thin_workspace(n));
+    const double2* Vst = (const double2*)ctx->bufs[S_VSTORE].p;
+    const double2* Tall = (const double2*)ctx->bufs[S_TALL].p;
+    for (int p = (int)ctx->band_panels.size() - 1; p >= 0; --p) {
+        const bse_ctx::BandPanel& bp = ctx->band_panels[p];
+        for (int c0 = 16; c0 < nstore; c0 += 16) {
             const int kc = std::min(16, nstore - c0);
             HCK(bse_apply_q_panel(st, bp.m, bp.k, Vst + bp.voff, Tall + (size_t)p * b * b, b, kc,
                                   Z + bp.s + (int64_t)c0 * n, n, 0, qw));
@@ -1212,6 +1236,7 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
             prep = ctx->la_ev[0];
         }
         HCK(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+        ctx->vec_grouped = prep != nullptr;
         int rc2 = twostage_vectors(ctx, ctx->side, n, d_lam, nstore, tnorm, Z, prep, bvoff_d, gws, gws_el);
         if (rc2) return rc2;
         HCK(hipEventRecord(ctx->join_ev, ctx->side));
@@ -1257,6 +1282,10 @@ static int eig_pipeline(bse_ctx* ctx, int64_t n64, void* H, int fp32,
     // ---------------- K7: back-transform Z <- Q Z
     stage_begin(ctx, ST_BT);
     if (vec_side) {
+        if (!ctx->vec_grouped) {
+            int rc3 = twostage_vectors_rest(ctx, st, n, nstore, Z);
+            if (rc3) return rc3;
+        }
         HCK(hipStreamWaitEvent(st, ctx->join_ev, 0));
     } else if (nstore > 0 && nstore <= 16 && bse_apply_q_thin_workspace(n) <= sk_el) {
         HCK(bse_apply_q_thin(st, n, nb, Vst, Tall, nstore, Z, n, 0, skw));

@@ -216,6 +216,7 @@ struct bse_ctx {
     bool band_fp32prod = false;     // last band reduction ran on FP32 matrix cores
     bool band_lookahead = false;    // ... and updated each next panel's tile column first
     bool band_pack = false;         // ... with the update's operands packed in FP32 planes
+    bool vec_grouped = false;       // the lowest vectors' Q1 back-transform ran four panels per pass
     bool force_valu_assembly = false;   // BSE_ASSEMBLY=valu selects the VALU tiles
     int asm_kernel = 0;                 // 0 VALU, 1 FP64 MFMA, 2 FP32 MFMA, 3 BF16x2 MFMA
     int64_t hemv_launches = 0;

@@ -186,12 +186,13 @@ def test_two_stage_lookahead_is_exact(ctx2, prec):
 def test_two_stage_grouped_q1_matches_panel_order(ctx2, monkeypatch, n, prec):
     """Z = Q1 X four band panels per pass (BSE_BT16_GROUPED=1: FP64 matrix
     cores, the T of each group of four formed on the main stream) against the
-    default one panel and 16 columns per pass: the same eigenvalues bit for bit
+    default one panel and 16 columns per pass (columns 16.. on the main
+    stream beside the second stream's first 16): the same eigenvalues bit for bit
     (Q1 does not touch them) and the same vectors to FP64 rounding.  Band
     panels: 1 (n = 65), 10, 30 and 63 (n = 2049: the last group three panels,
     the last panel one reflector)."""
     a = _herm(n, 11)
-    k = min(16, n)
+    k = min(20, n)       # 20: the default form splits the columns 16 + 4 over two streams
     print(f"grouped Q1 vs panels n={n} {prec}", flush=True)
     monkeypatch.setenv("BSE_BT16_GROUPED", "1")
     ev_g, z_g = ctx2.eigh_dense(a, prec, k)
