@@ -136,6 +136,18 @@ def measured_traffic(mixed: str, n: int):
     return t["hbm_bytes_per_launch"], t
 
 
+def band_product_traffic(n: int, mixed: str):
+    """HBM bytes per launch of the band reduction's block product from the
+    committed rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE, copy-calibrated)."""
+    p = ROOT / "profiles" / "band_product_traffic_n20736.json"
+    if not p.exists():
+        return None
+    t = json.loads(p.read_text())
+    if t.get("N") != n or t.get("storage") != ("fp32" if mixed != "fp64" else "fp64"):
+        return None
+    return t
+
+
 def grad_bench(args, pkg):
     """Differentiation on the N = 20736 system: Hellmann-Feynman gradients of
     the lowest 8 exciton energies and of the spectrum misfit against a seeded
@@ -376,6 +388,13 @@ def main():
 
     traffic, traffic_src = measured_traffic(args.mixed, N)
     roof = roofline_entry(prof, traffic, traffic_src)
+    tr = band_product_traffic(N, args.mixed)
+    if tr and str(roof.get("kernel", "")).startswith(tr["kernel"]):
+        roof["traffic"] = tr["hbm_bytes_per_launch"]
+        roof["traffic_source"] = tr["source"]
+        roof["traffic_vs_algorithmic"] = tr["hbm_bytes_per_launch"] / tr["algorithmic_bytes_per_launch"]
+        roof["traffic_gbs_at_live_launch_time"] = (tr["hbm_bytes_per_launch"] / (roof["avg_launch_us"] * 1e-6) / 1e9
+                                                   if roof.get("avg_launch_us") else None)
     if roof.get("bound") == "mfma" and args.mixed != "fp64":
         # FP32 storage: the panel kernels run v_mfma_f32_16x16x4
         ceil = rates["mfma_f32_16x16x4"]
